@@ -1,0 +1,164 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X FDTD step engine on one GPU.
+
+Default workload: a 512^3-cell graded CPML box (8-cell CPML on all faces, 64
+seeded FR-4 blocks, soft Gaussian Ez source, 48 voltage probes), 5000 steps.
+One "step" is one full FDTD time step (H and E updates, CPML, source,
+probes) over the whole grid; value = cells x steps / wall time (Gcell/s),
+with all inputs resident in HBM before the timed region.
+
+Prints one JSON line with a roofline figure for the dominant kernel (HIP
+events on the engine's stream) and a CPU baseline: the C oracle (a port of
+the reference engine, x-slab threads) timed on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def physical_cores() -> int:
+    avail = len(os.sched_getaffinity(0))
+    sib = set()
+    cpu = 0
+    while True:
+        p = f"/sys/devices/system/cpu/cpu{cpu}/topology/thread_siblings_list"
+        if not os.path.exists(p):
+            break
+        with open(p) as f:
+            sib.add(f.read().strip())
+        cpu += 1
+    smt = max(cpu // max(len(sib), 1), 1) if sib else 1
+    return max(avail // smt, 1)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, seconds: float, log) -> dict:
+    """The oracle (C port of the reference engine) on the same compiled
+    arrays, x-slab threads = physical cores (at most 16, the box's share)."""
+    from oracle.bridge import OracleEngine
+    threads = min(16, physical_cores())
+    nx = wl.grid.nodes()[0]
+    threads = max(1, min(threads, nx // 5))  # default_threads_for (engine/mod.rs:404-409)
+    o = OracleEngine(wl.grid, wl.dt, wl.lanes, threads=threads)
+    wl.build_ops(o)
+    o.run(1)
+    t0 = time.perf_counter()
+    o.run(1)
+    one = max(time.perf_counter() - t0, 1e-6)
+    n = max(1, min(int(seconds / one), 100000))
+    log(f"cpu baseline: {threads} threads, {one*1e3:.1f} ms/step, timing {n} steps")
+    t0 = time.perf_counter()
+    o.run(n)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": wl.cells * n / dt / 1e9, "unit": "Gcell/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} steps of the same workload after 2 warm-up steps "
+                      f"({dt:.1f} s; host CPU: {cpu_model()})"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="cpml_box", choices=["cpml_box", "vacuum_box"])
+    ap.add_argument("--n", type=int, default=None, help="cells per axis")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-steps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this engine has no multi-GPU path "
+              "(every measured figure is per GPU); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    import ems_amd as E
+    from ems_amd import workloads as W
+
+    t0 = time.perf_counter()
+    if args.workload == "cpml_box":
+        wl = W.cpml_box(n=args.n or 512)
+    else:
+        wl = W.vacuum_box(n=args.n or 50)
+    steps = args.steps if args.steps is not None else wl.steps
+    log(f"workload {wl.name}: {wl.grid.nodes()} nodes, built in {time.perf_counter()-t0:.1f} s")
+
+    eng = E.Engine(wl.grid, wl.dt, wl.lanes)
+    probes = wl.build_ops(eng)
+    eng.run(args.warmup)
+    eng.synchronize()
+    log(f"setup + warm-up done ({args.warmup} steps)")
+
+    t0 = time.perf_counter()
+    eng.run(steps)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    value = wl.cells * steps / wall / 1e9
+    log(f"{steps} steps in {wall:.3f} s: {value:.2f} Gcell/s")
+    vals = probes[0].values()
+    import math
+    assert all(math.isfinite(v) for v in vals), "non-finite probe"
+
+    # Dominant kernel: mean launch duration from HIP events on the engine's
+    # stream, algorithmic bytes from the lanes the engine holds.
+    th, te, ts = eng.time_kernels(args.kernel_steps)
+    hb, eb, cb = eng.traffic_model()
+    if th >= te:
+        name, dur, byt = "h_update", th, hb
+    else:
+        name, dur, byt = "e_update", te, eb
+    achieved = byt / (dur * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": byt, "launch_ms": round(dur, 4),
+            "step_ms_events": round(ts, 4)}
+    log(f"kernels: H {th:.3f} ms, E {te:.3f} ms, step {ts:.3f} ms; {name} {achieved:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        eng.close()
+        cpu = cpu_baseline(wl, args.cpu_seconds, log)
+
+    n = wl.grid.cells()
+    line = {
+        "metric": "Gcell-updates/s", "value": round(value, 3), "unit": "Gcell/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded graded mesh, FR-4 blocks, Gaussian source)"
+        if args.workload == "cpml_box" else "synthetic (vacuum PEC box, Gaussian source)",
+        "config": {"workload": wl.name, "cells": list(n), "nodes": list(wl.grid.nodes()),
+                   "cpml_cells": 8 if args.workload == "cpml_box" else 0,
+                   "probes": len(probes), "parallelism": "single GPU"},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,13 @@
+"""ems_amd -- MI355X-native FDTD step engine with the API of the ems engine.
+
+The device engine (HIP, gfx950) sits behind the C ABI of include/ems_gpu.h;
+the host side (coefficients, CPML layers, waveforms, run loop) is C++
+(include/ems_host.h). This package is the Python mirror of the reference
+engine API over both.
+"""
+from ._native import EmsError, LIB_PATH  # noqa: F401  (fails loudly if unbuilt)
+from .engine import (  # noqa: F401
+    C0, EPS0, MU0, Z0, Axis, CellMaterials, Cpml, CpmlBoundary, EdgeSource, EndCriteria, Engine,
+    EngineBuilder, Face, FaceCondition, Grid, Material, Probe, ProbeQuantity, RunLimits,
+    RunOutcome, Settle, SourceKind, StopReason, TimeStep, Waveform, coefficients,
+    courant_limit, cpml_coefficients, drive_script, pool_lane)

@@ -1,0 +1,147 @@
+"""ctypes binding of the in-tree native library ``libems_gpu.so``.
+
+The library holds the HIP engine (include/ems_gpu.h) and the C++ host side
+(include/ems_host.h). There is no fallback: if the library is missing the
+import fails with the command that builds it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libems_gpu.so")
+
+EMS_OK = 0
+EMS_E_INVALID = -1
+EMS_E_UNSUPPORTED = -2
+EMS_E_IO = -3
+EMS_E_DEVICE = -4
+
+
+class EmsError(RuntimeError):
+    """An ``ems::Error`` from the native side (src/error.rs variants)."""
+
+    KINDS = {EMS_E_INVALID: "Invalid", EMS_E_UNSUPPORTED: "Unsupported",
+             EMS_E_IO: "Io", EMS_E_DEVICE: "Device"}
+
+    def __init__(self, code: int, msg: str):
+        self.code = code
+        self.kind = self.KINDS.get(code, "Unknown")
+        super().__init__(f"{self.kind}: {msg}")
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+            " (hipcc --offload-arch=gfx950); there is no CPU fallback")
+    return C.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+_d = C.c_double
+_i = C.c_int
+_u64 = C.c_uint64
+_p = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
+_u16p = C.POINTER(C.c_uint16)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nodes", C.c_int32 * 3), ("dt", C.c_double), ("curl_order", C.c_int32)]
+
+
+class CpmlSlab(C.Structure):
+    _fields_ = [("e_side", C.c_int32), ("comp", C.c_int32), ("normal", C.c_int32),
+                ("lo", C.c_int32 * 3), ("hi", C.c_int32 * 3),
+                ("b", _fp), ("a", _fp), ("k", _fp)]
+
+
+class CpmlParams(C.Structure):
+    _fields_ = [("cells", C.c_int32), ("grading_order", C.c_double),
+                ("sigma_factor", C.c_double), ("kappa_max", C.c_double),
+                ("alpha_max", C.c_double), ("convolution", C.c_int32)]
+
+
+class RunLimitsC(C.Structure):
+    _fields_ = [("max_timesteps", C.c_uint64), ("has_criteria", C.c_int32),
+                ("energy_ratio", C.c_double), ("check_interval", C.c_uint64),
+                ("has_excitation_end", C.c_int32), ("excitation_end", C.c_double),
+                ("has_settle", C.c_int32), ("settle_frequency", C.c_double),
+                ("settle_tolerance", C.c_double), ("settle_window", C.c_double),
+                ("settle_max_extension", C.c_double)]
+
+
+class RunOutcomeC(C.Structure):
+    _fields_ = [("reason", C.c_int32), ("ratio_db", C.c_double), ("steps", C.c_uint64),
+                ("timestep", C.c_uint64), ("time", C.c_double), ("energy", C.c_double),
+                ("max_energy", C.c_double), ("elapsed_s", C.c_double),
+                ("has_criterion_steps", C.c_int32), ("criterion_steps", C.c_uint64)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+# include/ems_gpu.h
+ems_gpu_create = _sig("ems_gpu_create", _i, C.POINTER(GridDesc), C.POINTER(_p))
+ems_gpu_destroy = _sig("ems_gpu_destroy", None, _p)
+ems_gpu_last_error = _sig("ems_gpu_last_error", C.c_char_p, _p)
+ems_gpu_set_lane = _sig("ems_gpu_set_lane", _i, _p, _i, _fp, _u64, _u32p)
+ems_gpu_add_cpml = _sig("ems_gpu_add_cpml", _i, _p, C.POINTER(CpmlSlab), _i)
+ems_gpu_add_source = _sig("ems_gpu_add_source", _i, _p, _i, _i, _ip, _ip, _fp, _fp, _u64,
+                          C.POINTER(_i))
+ems_gpu_add_probe = _sig("ems_gpu_add_probe", _i, _p, _i, _i, _ip, _ip, _dp, _u64,
+                         C.POINTER(_i))
+ems_gpu_run = _sig("ems_gpu_run", _i, _p, _u64)
+ems_gpu_step_with_energy = _sig("ems_gpu_step_with_energy", _i, _p, _dp)
+ems_gpu_energy = _sig("ems_gpu_energy", _i, _p, _dp)
+ems_gpu_timestep = _sig("ems_gpu_timestep", _u64, _p)
+ems_gpu_read_component = _sig("ems_gpu_read_component", _i, _p, _i, _i, _fp)
+ems_gpu_write_component = _sig("ems_gpu_write_component", _i, _p, _i, _i, _fp)
+ems_gpu_probe_values = _sig("ems_gpu_probe_values", _i, _p, _i, _dp, _u64, C.POINTER(_u64))
+ems_gpu_probe_times = _sig("ems_gpu_probe_times", _i, _p, _i, _dp, _u64, C.POINTER(_u64))
+ems_gpu_dt = _sig("ems_gpu_dt", _d, _p)
+ems_gpu_synchronize = _sig("ems_gpu_synchronize", _i, _p)
+ems_gpu_time_kernels = _sig("ems_gpu_time_kernels", _i, _p, _u64, _dp)
+ems_gpu_traffic_model = _sig("ems_gpu_traffic_model", _i, _p, _dp)
+
+# include/ems_host.h
+ems_courant_limit = _sig("ems_courant_limit", _d, _dp, _i, _dp, _i, _dp, _i)
+ems_coefficients_from_cells = _sig("ems_coefficients_from_cells", _i, _dp, _i, _dp, _i, _dp, _i,
+                                   _u16p, _dp, _i, _u8p, _ip, _d, _i, _i, _fp)
+ems_pool_lane = _sig("ems_pool_lane", _i, _fp, _i, _i, _i, _fp, _u32p, C.POINTER(_u64))
+ems_cpml_coefficients = _sig("ems_cpml_coefficients", None, C.POINTER(CpmlParams), _d, _d, _d,
+                             _d, _dp)
+ems_cpml_layer_index = _sig("ems_cpml_layer_index", _d, _dp, _i, _dp, _i, _dp, _i, _fp, _d, _i, _i)
+ems_cpml_layer = _sig("ems_cpml_layer", _i, _dp, _i, _dp, _i, _dp, _i, _i, C.POINTER(CpmlParams),
+                      _d, _d, C.POINTER(CpmlSlab), _fp)
+ems_gaussian = _sig("ems_gaussian", _d, _d, _d, _d)
+ems_gaussian_duration = _sig("ems_gaussian_duration", _d, _d)
+ems_gaussian_table = _sig("ems_gaussian_table", None, _d, _d, _d, _d, _u64, _fp)
+ems_run_until = _sig("ems_run_until", _i, _p, C.POINTER(RunLimitsC), _ip, _i,
+                     C.POINTER(RunOutcomeC))
+ems_drive_script = _sig("ems_drive_script", _i, _dp, _u64, _dp, _u64, C.POINTER(RunLimitsC),
+                        C.POINTER(RunOutcomeC))
+
+
+def ptr(a, ctype):
+    """Pointer to a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != EMS_OK:
+        msg = ems_gpu_last_error(handle).decode() if handle else ""
+        raise EmsError(rc, msg)

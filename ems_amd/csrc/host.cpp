@@ -1,0 +1,609 @@
+// host.cpp -- the host side above the device boundary (include/ems_host.h):
+// coefficient builder, row pooling, CPML layer builder, Gaussian waveform
+// and the end-criterion run loop, restated from the reference in C++ with
+// the same f64 expressions (compiled without FP contraction).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ems_gpu.h"
+#include "../../include/ems_host.h"
+
+namespace {
+
+// src/constants.rs
+constexpr double C0 = 299792458.0;
+constexpr double MU0 = 1.25663706127e-6;
+const double EPS0 = 1.0 / (MU0 * C0 * C0);
+const double Z0 = MU0 * C0;
+// CoeffRounding::OpenEms vacuum constants (src/constants.rs:29, :37)
+constexpr double OPENEMS_EPS0 = 8.85418781762e-12;
+constexpr double OPENEMS_MU0 = 1.256637062e-6;
+
+struct Grid {
+  const double *l[3];
+  int n[3];
+  // Grid::delta (grid.rs:562-566)
+  double delta(int a, int i) const { return l[a][i + 1] - l[a][i]; }
+  // Grid::dual_delta (grid.rs:592-597)
+  double dual_delta(int a, int i) const {
+    double lo = i > 0 ? delta(a, i - 1) : 0.0;
+    double hi = i + 1 < n[a] ? delta(a, i) : 0.0;
+    return 0.5 * (lo + hi);
+  }
+  size_t index(int i, int j, int k) const { return ((size_t)i * n[1] + j) * n[2] + k; }
+};
+
+Grid make_grid(const double *x, int nx, const double *y, int ny, const double *z, int nz) {
+  Grid g;
+  g.l[0] = x, g.l[1] = y, g.l[2] = z;
+  g.n[0] = nx, g.n[1] = ny, g.n[2] = nz;
+  return g;
+}
+
+// Axis::others (src/axis.rs): (next, next of next).
+inline int nxt(int a) { return (a + 1) % 3; }
+inline int prv(int a) { return (a + 2) % 3; }
+
+struct Mat {
+  double eps_r[3], mu_r[3], kappa[3], sigma_m[3];
+};
+
+template <class F>
+void parallel_for(int n, int threads, F f) {
+  if (threads <= 1 || n < 2) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  threads = std::min(threads, n);
+  std::vector<std::thread> ts;
+  for (int t = 0; t < threads; t++)
+    ts.emplace_back([=, &f]() {
+      for (int i = t; i < n; i += threads) f(i);
+    });
+  for (auto &t : ts) t.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+double ems_courant_limit(const double *x, int nx, const double *y, int ny, const double *z,
+                         int nz) {
+  const double *L[3] = {x, y, z};
+  const int n[3] = {nx, ny, nz};
+  double s = -0.0;  // Iterator::sum folds from -0.0
+  for (int a = 0; a < 3; a++) {
+    double m = INFINITY;  // Grid::min_delta (grid.rs:600-606)
+    for (int i = 0; i + 1 < n[a]; i++) m = std::fmin(m, L[a][i + 1] - L[a][i]);
+    s += 1.0 / (m * m);
+  }
+  return 1.0 / (C0 * std::sqrt(s));
+}
+
+int ems_coefficients_from_cells(const double *x, int nx, const double *y, int ny,
+                                const double *z, int nz, const uint16_t *cell_mat,
+                                const double *mat_table, int n_mat, const uint8_t *pec,
+                                const int face_pec[6], double dt, int rounding, int threads,
+                                float *lanes) {
+  if (nx < 2 || ny < 2 || nz < 2 || !mat_table || n_mat < 1 || !lanes || !face_pec)
+    return EMS_E_INVALID;
+  if (!(dt > 0.0) || !std::isfinite(dt)) return EMS_E_INVALID;
+  const Grid g = make_grid(x, nx, y, ny, z, nz);
+  const size_t N = (size_t)nx * ny * nz;
+  const int cn[3] = {nx - 1, ny - 1, nz - 1};
+  std::vector<Mat> mats(n_mat);
+  for (int m = 0; m < n_mat; m++) {
+    const double *t = mat_table + 12 * m;
+    for (int a = 0; a < 3; a++) {
+      mats[m].eps_r[a] = t[a], mats[m].mu_r[a] = t[3 + a];
+      mats[m].kappa[a] = t[6 + a], mats[m].sigma_m[a] = t[9 + a];
+    }
+  }
+  if (cell_mat) {
+    const size_t nc = (size_t)cn[0] * cn[1] * cn[2];
+    for (size_t c = 0; c < nc; c++)
+      if (cell_mat[c] >= n_mat) return EMS_E_INVALID;
+  }
+  auto cell = [&](const int c[3]) -> const Mat & {
+    if (!cell_mat) return mats[0];
+    return mats[cell_mat[((size_t)c[0] * cn[1] + c[1]) * cn[2] + c[2]]];
+  };
+  // CoeffRounding::store / eps0 / mu0 (coeffs.rs:735-760)
+  auto store = [&](double v) { return rounding ? (double)(float)v : v; };
+  const double eps0 = rounding ? OPENEMS_EPS0 : EPS0;
+  const double mu0 = rounding ? OPENEMS_MU0 : MU0;
+  std::memset(lanes, 0, 12 * N * sizeof(float));
+  bool pec_face[3][2];
+  for (int a = 0; a < 3; a++) pec_face[a][0] = face_pec[2 * a], pec_face[a][1] = face_pec[2 * a + 1];
+
+  parallel_for(nx, threads, [&](int i) {
+    for (int a = 0; a < 3; a++) {
+      const int b = nxt(a), c = prv(a);
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          const int p[3] = {i, j, k};
+          const size_t idx = g.index(i, j, k);
+          if (p[a] + 1 < g.n[a]) {
+            const bool on_face = (p[b] == 0 && pec_face[b][0]) ||
+                                 (p[b] == g.n[b] - 1 && pec_face[b][1]) ||
+                                 (p[c] == 0 && pec_face[c][0]) ||
+                                 (p[c] == g.n[c] - 1 && pec_face[c][1]);
+            const bool pm = pec && pec[(size_t)a * N + idx];
+            if (!(on_face || pm)) {
+              // primary_edge_cg (coeffs.rs:264-295)
+              const double l = g.delta(a, p[a]);
+              double cap = 0.0, cond = 0.0;
+              for (int cb : {p[b] - 1, p[b]}) {
+                if (cb < 0 || cb >= g.n[b] - 1) continue;
+                for (int cc : {p[c] - 1, p[c]}) {
+                  if (cc < 0 || cc >= g.n[c] - 1) continue;
+                  int ce[3];
+                  ce[a] = p[a], ce[b] = cb, ce[c] = cc;
+                  const double area = 0.25 * g.delta(b, cb) * g.delta(c, cc);
+                  const Mat &m = cell(ce);
+                  cap += EPS0 * m.eps_r[a] * area;
+                  cond += m.kappa[a] * area;
+                }
+              }
+              cap = cap / l, cond = cond / l;
+              // EdgeMaterials::from_cells (edge_material.rs:396-401)
+              const double kk = g.delta(a, p[a]) / (g.dual_delta(b, p[b]) * g.dual_delta(c, p[c]));
+              const double eps_e = cap * kk / EPS0;
+              const double kap_e = cond * kk;
+              // e_edge_geometry + e_coefficients (coeffs.rs:766-774, 817-820, 930-933)
+              const double geo = g.dual_delta(b, p[b]) * g.dual_delta(c, p[c]) / g.delta(a, p[a]);
+              const double C = store(eps0 * eps_e * geo);
+              const double G = store(kap_e * geo);
+              const double den = C + 0.5 * G * dt;
+              lanes[(size_t)a * N + idx] = (float)((C - 0.5 * G * dt) / den);
+              lanes[(size_t)(3 + a) * N + idx] = (float)(dt / den);
+            }
+          }
+          if (p[b] + 1 < g.n[b] && p[c] + 1 < g.n[c]) {
+            // dual_edge_lr (coeffs.rs:303-325)
+            const double area = g.delta(b, p[b]) * g.delta(c, p[c]);
+            double inv_l = 0.0, r_over_l2 = 0.0;
+            for (int ca : {p[a] - 1, p[a]}) {
+              if (ca < 0 || ca >= g.n[a] - 1) continue;
+              int ce[3] = {p[0], p[1], p[2]};
+              ce[a] = ca;
+              const double half = 0.5 * g.delta(a, ca);
+              const Mat &m = cell(ce);
+              const double lc = MU0 * m.mu_r[a] * area / half;
+              const double rc = m.sigma_m[a] * area / half;
+              inv_l += 1.0 / lc;
+              r_over_l2 += rc / (lc * lc);
+            }
+            const double L = 1.0 / inv_l;
+            const double R = L * L * r_over_l2;
+            // EdgeMaterials::from_cells (edge_material.rs:404-410)
+            const double kk = g.dual_delta(a, p[a]) / (g.delta(b, p[b]) * g.delta(c, p[c]));
+            const double mu_e = L * kk / MU0;
+            const double sig_e = R * kk;
+            // coeffs.rs:939-955
+            const double geo = g.delta(b, p[b]) * g.delta(c, p[c]) / g.dual_delta(a, p[a]);
+            const double Ls = store(mu0 * mu_e * geo);
+            const double Rs = store(sig_e * geo);
+            const double den = Ls + 0.5 * Rs * dt;
+            lanes[(size_t)(6 + a) * N + idx] = (float)((Ls - 0.5 * Rs * dt) / den);
+            lanes[(size_t)(9 + a) * N + idx] = (float)(dt / den);
+          }
+        }
+    }
+  });
+  return EMS_OK;
+}
+
+int ems_pool_lane(const float *dense, int nx, int ny, int nz, float *rows_out,
+                  uint32_t *index_out, uint64_t *nrows_out) {
+  if (!dense || nx < 1 || ny < 1 || nz < 1) return EMS_E_INVALID;
+  const size_t count = (size_t)nx * ny;
+  const size_t limit = std::max<size_t>(count / 4, 1);  // POOL_RATIO (coeffs.rs:418)
+  // hash (coeffs.rs:425-431): FNV-1a over the bit patterns
+  auto hash = [&](const float *r) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (int k = 0; k < nz; k++) {
+      uint32_t b;
+      std::memcpy(&b, r + k, 4);
+      h = (h ^ (uint64_t)b) * 0x00000100000001b3ull;
+    }
+    return h;
+  };
+  std::unordered_map<uint64_t, std::vector<uint32_t>> seen;
+  std::vector<float> pool;
+  for (size_t r = 0; r < count; r++) {
+    const float *row = dense + r * nz;
+    auto &cands = seen[hash(row)];
+    int64_t hit = -1;
+    for (uint32_t c : cands)
+      if (std::memcmp(pool.data() + (size_t)c * nz, row, nz * sizeof(float)) == 0) {
+        hit = c;
+        break;
+      }
+    if (hit < 0) {
+      const size_t c = pool.size() / nz;
+      if (c >= limit) return 0;
+      pool.insert(pool.end(), row, row + nz);
+      cands.push_back((uint32_t)c);
+      hit = (int64_t)c;
+    }
+    if (index_out) index_out[r] = (uint32_t)hit;
+  }
+  if (rows_out) std::memcpy(rows_out, pool.data(), pool.size() * sizeof(float));
+  if (nrows_out) *nrows_out = pool.size() / nz;
+  return 1;
+}
+
+void ems_cpml_coefficients(const ems_cpml_params *p, double rho, double delta, double dt,
+                           double n, double out[7]) {
+  const double m = p->grading_order;
+  const double g = std::pow(rho, m);
+  // Cpml::sigma_max (fdtd.rs:266-271) with eta = n*Z0
+  const double eta = n * Z0;
+  const double sigma = p->sigma_factor * 0.8 * (m + 1.0) / (eta * delta) * g;
+  const double kappa = 1.0 + (p->kappa_max - 1.0) * g;
+  const double alpha = p->alpha_max * (1.0 - rho);
+  const double b = std::exp(-(sigma / kappa + alpha) * dt / EPS0);
+  const double den = sigma * kappa + kappa * kappa * alpha;
+  const double x = (sigma / kappa + alpha) * dt / EPS0;
+  double a, shift;
+  if (!(sigma > 0.0 && den > 0.0)) {
+    a = 0.0, shift = 0.0;
+  } else if (p->convolution == 1 && x <= 8.0) {  // LINEAR_UP_TO (cpml.rs:337)
+    const double cdt = sigma * dt / (EPS0 * kappa * kappa);
+    const double gg =
+        x < 1e-3 ? 0.5 - x / 3.0 + x * x / 8.0 : (-std::expm1(-x) - x * b) / (x * x);
+    const double a1 = -cdt * gg;
+    const double a0 = cdt * std::expm1(-x) / x - a1;
+    a = a0 + a1 / b, shift = a1 / b;
+  } else {
+    a = sigma * (b - 1.0) / den, shift = 0.0;
+  }
+  out[0] = rho, out[1] = sigma, out[2] = kappa, out[3] = alpha, out[4] = b, out[5] = a;
+  out[6] = 1.0 / kappa - 1.0 - shift;
+}
+
+double ems_cpml_layer_index(const double *x, int nx, const double *y, int ny, const double *z,
+                            int nz, const float *lanes, double dt, int face, int cells) {
+  const Grid g = make_grid(x, nx, y, ny, z, nz);
+  const size_t N = (size_t)nx * ny * nz;
+  const int p = face / 2;
+  const bool is_max = face % 2 == 1;
+  const int np = g.n[p];
+  const int inner = is_max ? np - 1 - cells : cells;
+  const int l0 = is_max ? inner : 0, l1 = is_max ? np : inner + 1;      // lines
+  const int c0 = is_max ? inner : 0, c1 = is_max ? np - 1 : inner;      // cells
+  double we = 0.0, se = 0.0, wh = 0.0, sh = 0.0;
+  for (int comp : {nxt(p), prv(p)}) {
+    const int b = nxt(comp), c = prv(comp);
+    const float *ca = lanes + (size_t)comp * N, *cb = lanes + (size_t)(3 + comp) * N;
+    const float *da = lanes + (size_t)(6 + comp) * N, *db = lanes + (size_t)(9 + comp) * N;
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          const int node[3] = {i, j, k};
+          const int along = node[p];
+          if (!((along >= l0 && along < l1) || (along >= c0 && along < c1))) continue;
+          const size_t idx = g.index(i, j, k);
+          if (along >= l0 && along < l1 && node[comp] + 1 < g.n[comp] && cb[idx] != 0.0f) {
+            const double area = g.dual_delta(b, node[b]) * g.dual_delta(c, node[c]);
+            const double l = g.delta(comp, node[comp]);
+            const double cap = dt * (1.0 + (double)ca[idx]) / (2.0 * (double)cb[idx]);
+            const double vol = area * l;
+            we += vol;
+            se += vol * cap * l / (EPS0 * area);
+          }
+          if (along >= c0 && along < c1 && node[b] + 1 < g.n[b] && node[c] + 1 < g.n[c] &&
+              db[idx] != 0.0f) {
+            const double area = g.delta(b, node[b]) * g.delta(c, node[c]);
+            const double l = g.dual_delta(comp, node[comp]);
+            const double ind = dt * (1.0 + (double)da[idx]) / (2.0 * (double)db[idx]);
+            const double vol = area * l;
+            wh += vol;
+            sh += vol * ind * l / (MU0 * area);
+          }
+        }
+  }
+  const double eps = we > 0.0 ? se / we : 1.0;
+  const double mu = wh > 0.0 ? sh / wh : 1.0;
+  const double n_r = std::sqrt(eps * mu);
+  if (std::fabs(n_r - 1.0) < 1e-6 || !std::isfinite(n_r) || n_r <= 0.0) return 1.0;
+  return n_r;
+}
+
+int ems_cpml_layer(const double *x, int nx, const double *y, int ny, const double *z, int nz,
+                   int face, const ems_cpml_params *prm, double dt, double n_face,
+                   ems_cpml_slab out[4], float *coeff) {
+  if (face < 0 || face > 5 || !prm || !out || !coeff) return EMS_E_INVALID;
+  const Grid g = make_grid(x, nx, y, ny, z, nz);
+  const int p = face / 2;
+  const bool is_max = face % 2 == 1;
+  const int np = g.n[p];
+  const int c = prm->cells;
+  if (c < 1 || c > np - 1) return EMS_E_INVALID;
+  const int inner = is_max ? np - 1 - c : c;
+  auto rho = [&](double v) { return std::clamp(std::fabs(v - (double)inner) / (double)c, 0.0, 1.0); };
+  std::vector<int> e_idx, h_idx;
+  if (is_max) {
+    for (int k = inner + 1; k < np; k++) e_idx.push_back(k);
+    for (int k = inner; k < np - 1; k++) h_idx.push_back(k);
+  } else {
+    for (int k = 0; k < inner; k++) e_idx.push_back(k), h_idx.push_back(k);
+  }
+  auto node_delta = [&](int k) {
+    const bool lo = k > 0, hi = k + 1 < np;
+    if (lo && hi) return 0.5 * (g.delta(p, k - 1) + g.delta(p, k));
+    return lo ? g.delta(p, k - 1) : g.delta(p, k);
+  };
+  std::vector<double> ec(7 * e_idx.size()), hc(7 * h_idx.size());
+  for (size_t q = 0; q < e_idx.size(); q++)
+    ems_cpml_coefficients(prm, rho((double)e_idx[q]), node_delta(e_idx[q]), dt, n_face, &ec[7 * q]);
+  for (size_t q = 0; q < h_idx.size(); q++)
+    ems_cpml_coefficients(prm, rho((double)h_idx[q] + 0.5), g.delta(p, h_idx[q]), dt, n_face,
+                          &hc[7 * q]);
+  const int a1 = nxt(p), a2 = prv(p);
+  auto fill = [&](int q, int e_side, int comp, const std::vector<int> &idx,
+                  const std::vector<double> &co) {
+    ems_cpml_slab &s = out[q];
+    s.e_side = e_side, s.comp = comp, s.normal = p;
+    for (int d = 0; d < 3; d++) s.lo[d] = 0, s.hi[d] = g.n[d];
+    if (e_side) {
+      s.hi[comp] = g.n[comp] - 1;  // no padding
+    } else {
+      const int o = comp == a1 ? a2 : a1;
+      s.hi[o] = g.n[o] - 1;
+    }
+    s.lo[p] = idx.empty() ? 0 : idx.front();
+    s.hi[p] = idx.empty() ? 0 : idx.back() + 1;
+    float *B = coeff + (size_t)q * 3 * np, *A = B + np, *K = A + np;
+    for (size_t d = 0; d < idx.size(); d++) {
+      B[d] = (float)co[7 * d + 4];
+      A[d] = (float)co[7 * d + 5];
+      K[d] = (float)co[7 * d + 6];
+    }
+    s.b = B, s.a = A, s.k = K;
+  };
+  fill(0, 1, a1, e_idx, ec);
+  fill(1, 1, a2, e_idx, ec);
+  fill(2, 0, a1, h_idx, hc);
+  fill(3, 0, a2, h_idx, hc);
+  return EMS_OK;
+}
+
+// waveform.rs:240-248 (gaussian_tau, gaussian_delay)
+static double gaussian_tau(double fc) { return std::sqrt(M_LN10) / (M_PI * fc); }
+static double gaussian_delay(double fc) { return gaussian_tau(fc) * std::sqrt(6.0 * M_LN10); }
+
+double ems_gaussian(double f0, double fc, double t) {
+  const double delay = gaussian_delay(fc);
+  if (!(t >= 0.0 && t <= 2.0 * delay)) return 0.0;
+  const double x = (t - delay) / gaussian_tau(fc);
+  return std::cos(2.0 * M_PI * f0 * (t - delay)) * std::exp(-x * x);
+}
+
+double ems_gaussian_duration(double fc) { return 2.0 * gaussian_delay(fc); }
+
+void ems_gaussian_table(double f0, double fc, double dt, double offset, uint64_t count,
+                        float *amp) {
+  for (uint64_t n = 0; n < count; n++)
+    amp[n] = (float)ems_gaussian(f0, fc, ((double)n + offset) * dt);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- run loop --
+namespace {
+
+struct Stepper {
+  virtual ~Stepper() = default;
+  virtual uint64_t timestep() = 0;
+  virtual double dt() = 0;
+  virtual int step_with_energy(double *w) = 0;
+  virtual int run(uint64_t n) = 0;
+  // per probe: sample times and values (run.rs:302-306)
+  virtual int signals(std::function<void(const std::vector<double> &, const std::vector<double> &)>) {
+    return 0;
+  }
+};
+
+// Settling (run.rs:309-381)
+struct Settling {
+  double freq, tol, window, max_ext;
+  struct Sum {
+    size_t done = 0;
+    double re = 0, im = 0, abs = 0;
+  };
+  std::vector<Sum> sums;
+  struct Checked {
+    uint64_t steps;
+    double w;
+    std::vector<std::pair<double, double>> x;
+  };
+  std::vector<Checked> history;
+
+  int update(Stepper &s, uint64_t steps, double w) {
+    const double om = 2.0 * M_PI * freq;
+    size_t q = 0;
+    int rc = s.signals([&](const std::vector<double> &t, const std::vector<double> &v) {
+      if (sums.size() == q) sums.push_back(Sum{});
+      Sum &p = sums[q];
+      const size_t n = std::min(t.size(), v.size());
+      for (size_t k = p.done; k < n; k++) {
+        const double sn = std::sin(om * t[k]), cs = std::cos(om * t[k]);
+        p.re += v[k] * cs;
+        p.im -= v[k] * sn;
+        p.abs += std::fabs(v[k]);
+      }
+      p.done = n;
+      q++;
+    });
+    Checked c{steps, w, {}};
+    for (auto &p : sums) c.x.emplace_back(p.re, p.im);
+    history.push_back(std::move(c));
+    return rc;
+  }
+
+  bool settled() const {
+    if (history.empty()) return true;
+    const Checked &last = history.back();
+    const uint64_t span = (uint64_t)std::ceil(window * (double)last.steps);
+    const uint64_t from = last.steps > span ? last.steps - span : 0;
+    size_t n = 0;
+    while (n < history.size() && history[n].steps <= from) n++;
+    if (n == 0) return false;
+    const Checked &then = history[n - 1];
+    if (last.w <= 0.0) return true;
+    const double fall = std::log(then.w / last.w);
+    if (std::isnan(fall) || fall <= 0.0) return false;
+    for (size_t q = 0; q < sums.size(); q++) {
+      const double re = q < then.x.size() ? then.x[q].first : 0.0;
+      const double im = q < then.x.size() ? then.x[q].second : 0.0;
+      const double moved = std::hypot(last.x[q].first - re, last.x[q].second - im);
+      if (!(2.0 * moved <= fall * tol * sums[q].abs)) return false;
+    }
+    return true;
+  }
+};
+
+constexpr double NEGATIVE_ENERGY = 1e-12;  // run.rs:419
+
+// run::drive (run.rs:424-540), default schedule (energy as a by-product of
+// the step after each check; invariant measure, nothing held).
+int drive(Stepper &s, const ems_run_limits &lim, ems_run_outcome &out) {
+  if (lim.has_criteria && !(lim.energy_ratio >= 0.0)) return EMS_E_INVALID;
+  if (lim.has_excitation_end && std::isnan(lim.excitation_end)) return EMS_E_INVALID;
+  const uint64_t interval = lim.check_interval ? lim.check_interval : 50;
+  const uint64_t max = lim.max_timesteps;
+  const auto start = std::chrono::steady_clock::now();
+  uint64_t steps = 0;
+  double w = 0.0, w_max = 0.0, ratio_db = 0.0;
+  bool met = false;
+  uint64_t met_steps = 0;
+  Settling settle{lim.settle_frequency, lim.settle_tolerance, lim.settle_window,
+                  lim.settle_max_extension, {}, {}};
+  int reason;
+  for (;;) {
+    if (steps >= max) {
+      reason = 1;
+      break;
+    }
+    const uint64_t n = s.timestep();
+    int rc = s.step_with_energy(&w);
+    if (rc) return rc;
+    steps += 1;
+    w_max = std::fmax(w_max, w);
+    const double ratio = w_max > 0.0 ? w / w_max : 1.0;
+    ratio_db = ratio > 0.0 ? 10.0 * std::log10(ratio) : -INFINITY;
+    const double t = (double)n * s.dt();
+    if (!std::isfinite(w) || w < -NEGATIVE_ENERGY * w_max) {
+      reason = 3;
+      break;
+    }
+    if (lim.has_settle && lim.has_criteria)
+      if ((rc = settle.update(s, steps, w))) return rc;
+    if (lim.has_criteria && (!lim.has_excitation_end || t > lim.excitation_end) && w_max > 0.0 &&
+        ratio < lim.energy_ratio) {
+      if (!met) met = true, met_steps = steps;
+      const bool done = !lim.has_settle ||
+                        (double)steps >= (double)met_steps * (1.0 + lim.settle_max_extension) ||
+                        settle.settled();
+      if (done) {
+        reason = 0;
+        break;
+      }
+    }
+    const uint64_t chunk = std::min(interval - 1, max - steps);
+    if ((rc = s.run(chunk))) return rc;
+    steps += chunk;
+  }
+  out.reason = reason;
+  out.ratio_db = ratio_db;
+  out.steps = steps;
+  out.timestep = s.timestep();
+  out.time = (double)s.timestep() * s.dt();
+  out.energy = w;
+  out.max_energy = w_max;
+  out.elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+  out.has_criterion_steps = met;
+  out.criterion_steps = met_steps;
+  return EMS_OK;
+}
+
+struct GpuStepper : Stepper {
+  ems_gpu *h;
+  double dtv;
+  std::vector<int> probes;
+  std::vector<std::vector<double>> vals;
+  GpuStepper(ems_gpu *h_, double dt_, const int *p, int np) : h(h_), dtv(dt_), probes(p, p + np) {}
+  uint64_t timestep() override { return ems_gpu_timestep(h); }
+  double dt() override { return dtv; }
+  int step_with_energy(double *w) override { return ems_gpu_step_with_energy(h, w); }
+  int run(uint64_t n) override { return ems_gpu_run(h, n); }
+  int signals(std::function<void(const std::vector<double> &, const std::vector<double> &)> f)
+      override {
+    for (int id : probes) {
+      uint64_t count = 0;
+      int rc = ems_gpu_probe_values(h, id, nullptr, 0, &count);
+      if (rc) return rc;
+      std::vector<double> v(count), t(count);
+      if (count && (rc = ems_gpu_probe_values(h, id, v.data(), count, &count))) return rc;
+      if (count && (rc = ems_gpu_probe_times(h, id, t.data(), count, &count))) return rc;
+      f(t, v);
+    }
+    return 0;
+  }
+};
+
+struct ScriptStepper : Stepper {
+  const double *w;
+  uint64_t nw, n = 0;
+  const double *sig;
+  uint64_t ns;
+  uint64_t timestep() override { return n; }
+  double dt() override { return 1.0; }
+  int step_with_energy(double *out) override {
+    *out = w[std::min<uint64_t>(n, nw - 1)];
+    n += 1;
+    return 0;
+  }
+  int run(uint64_t k) override {
+    n += k;
+    return 0;
+  }
+  int signals(std::function<void(const std::vector<double> &, const std::vector<double> &)> f)
+      override {
+    if (!sig) return 0;
+    std::vector<double> t(n), v(n);
+    for (uint64_t k = 0; k < n; k++) t[k] = (double)k, v[k] = k < ns ? sig[k] : 0.0;
+    f(t, v);
+    return 0;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ems_run_until(ems_gpu *h, const ems_run_limits *limits, const int *probes, int n_probes,
+                  ems_run_outcome *out) {
+  if (!h || !limits || !out) return EMS_E_INVALID;
+  GpuStepper s(h, ems_gpu_dt(h), probes, n_probes);
+  return drive(s, *limits, *out);
+}
+
+int ems_drive_script(const double *w, uint64_t nw, const double *signal, uint64_t ns,
+                     const ems_run_limits *limits, ems_run_outcome *out) {
+  if (!w || nw == 0 || !limits || !out) return EMS_E_INVALID;
+  ScriptStepper s;
+  s.w = w, s.nw = nw, s.sig = signal, s.ns = ns;
+  return drive(s, *limits, *out);
+}
+
+}  // extern "C"

@@ -1,0 +1,798 @@
+"""Host-side mirror of the reference engine API (src/engine/mod.rs), driving
+the MI355X step engine of include/ems_gpu.h.
+
+Names, argument meanings and error behaviour follow the reference:
+``Engine.builder(grid).materials(m).faces(f).timestep(t).build()``,
+``engine.add_operator(op)``, ``engine.run(n)``, ``engine.step_with_energy()``,
+``engine.run_until(limits)``. Compile-side work (coefficients, CPML layers,
+waveform tables) runs in the C++ host library (include/ems_host.h); stepping
+runs only on the device — there is no CPU path here.
+
+Every array handed to the device is also kept in ``engine.setup`` (the
+"compiled operators"), in registration order, so the same arrays can be
+replayed elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from enum import IntEnum
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as nat
+from ._native import EmsError
+
+# src/constants.rs
+C0 = 299_792_458.0
+MU0 = 1.256_637_061_27e-6
+EPS0 = 1.0 / (MU0 * C0 * C0)
+Z0 = MU0 * C0
+
+
+class Axis(IntEnum):
+    """src/axis.rs"""
+    X = 0
+    Y = 1
+    Z = 2
+
+    def next(self) -> "Axis":
+        return Axis((self + 1) % 3)
+
+    def prev(self) -> "Axis":
+        return Axis((self + 2) % 3)
+
+    def others(self) -> Tuple["Axis", "Axis"]:
+        return (self.next(), self.next().next())
+
+
+class Face(IntEnum):
+    """src/fdtd.rs Face: x_min, x_max, y_min, y_max, z_min, z_max."""
+    XMin = 0
+    XMax = 1
+    YMin = 2
+    YMax = 3
+    ZMin = 4
+    ZMax = 5
+
+    @property
+    def axis(self) -> Axis:
+        return Axis(self // 2)
+
+    @property
+    def is_max(self) -> bool:
+        return self % 2 == 1
+
+    @staticmethod
+    def all() -> List["Face"]:
+        return list(Face)
+
+
+class FaceCondition(IntEnum):
+    """src/engine/coeffs.rs:36-48"""
+    Pec = 0
+    Pmc = 1
+
+
+# ---------------------------------------------------------------- mesh --
+
+class Grid:
+    """Rectilinear Cartesian mesh (src/engine/grid.rs)."""
+
+    def __init__(self, lines: Sequence[Sequence[float]]):
+        ls = [np.ascontiguousarray(np.asarray(l, dtype=np.float64)) for l in lines]
+        if len(ls) != 3:
+            raise EmsError(nat.EMS_E_INVALID, "a grid needs three axes")
+        for a, l in enumerate(ls):
+            if l.size < 2 or not np.all(np.isfinite(l)) or not np.all(np.diff(l) > 0):
+                raise EmsError(nat.EMS_E_INVALID,
+                               f"mesh lines of axis {a} must be finite, increasing, at least two")
+        self.lines = ls
+
+    @staticmethod
+    def uniform(cells, cell_size) -> "Grid":
+        """Grid::uniform (grid.rs:514-518): line i at i*size."""
+        return Grid([[i * float(cell_size[a]) for i in range(int(cells[a]) + 1)] for a in range(3)])
+
+    def nodes(self) -> Tuple[int, int, int]:
+        return tuple(int(l.size) for l in self.lines)
+
+    def num_nodes(self) -> int:
+        n = self.nodes()
+        return n[0] * n[1] * n[2]
+
+    def cells(self) -> Tuple[int, int, int]:
+        n = self.nodes()
+        return (n[0] - 1, n[1] - 1, n[2] - 1)
+
+    def num_cells(self) -> int:
+        c = self.cells()
+        return c[0] * c[1] * c[2]
+
+    def index(self, node) -> int:
+        n = self.nodes()
+        return (node[0] * n[1] + node[1]) * n[2] + node[2]
+
+    def delta(self, axis: int, i: int) -> float:
+        l = self.lines[axis]
+        return float(l[i + 1] - l[i])
+
+    def _args(self):
+        x, y, z = self.lines
+        return (nat.ptr(x, C.c_double), x.size, nat.ptr(y, C.c_double), y.size,
+                nat.ptr(z, C.c_double), z.size)
+
+
+@dataclass
+class Material:
+    """src/engine/material.rs:22-31 (per-axis values)."""
+    eps_r: Tuple[float, float, float] = (1.0, 1.0, 1.0)
+    mu_r: Tuple[float, float, float] = (1.0, 1.0, 1.0)
+    kappa: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    sigma_m: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+
+    @staticmethod
+    def vacuum() -> "Material":
+        return Material()
+
+    @staticmethod
+    def dielectric(eps_r: float) -> "Material":
+        return Material(eps_r=(eps_r,) * 3)
+
+    def with_kappa(self, k: float) -> "Material":
+        return Material(self.eps_r, self.mu_r, (k,) * 3, self.sigma_m)
+
+    def with_mu_r(self, m: float) -> "Material":
+        return Material(self.eps_r, (m,) * 3, self.kappa, self.sigma_m)
+
+    def with_sigma_m(self, s: float) -> "Material":
+        return Material(self.eps_r, self.mu_r, self.kappa, (s,) * 3)
+
+    def row(self) -> List[float]:
+        return list(self.eps_r) + list(self.mu_r) + list(self.kappa) + list(self.sigma_m)
+
+
+class CellMaterials:
+    """Per-cell materials (src/engine/material.rs:111-270): a material table,
+    a u16 id per cell and a PEC mask per E edge."""
+
+    def __init__(self, grid: Grid, background: Material = None):
+        self.grid = grid
+        self.table: List[Material] = [background or Material.vacuum()]
+        self.ids = np.zeros(grid.cells(), dtype=np.uint16)
+        self.pec: Optional[np.ndarray] = None
+
+    @staticmethod
+    def vacuum(grid: Grid) -> "CellMaterials":
+        return CellMaterials(grid)
+
+    def _id(self, m: Material) -> int:
+        for q, t in enumerate(self.table):
+            if t == m:
+                return q
+        if len(self.table) >= 65535:
+            raise EmsError(nat.EMS_E_INVALID, "too many materials")
+        self.table.append(m)
+        return len(self.table) - 1
+
+    def fill_box(self, ranges, m: Material) -> None:
+        """CellMaterials::fill_box: cells in the three index ranges."""
+        q = self._id(m)
+        self.ids[ranges[0][0]:ranges[0][1], ranges[1][0]:ranges[1][1], ranges[2][0]:ranges[2][1]] = q
+
+    def set(self, cell, m: Material) -> None:
+        self.ids[cell[0], cell[1], cell[2]] = self._id(m)
+
+    def _pec(self) -> np.ndarray:
+        if self.pec is None:
+            n = self.grid.nodes()
+            self.pec = np.zeros((3,) + n, dtype=np.uint8)
+        return self.pec
+
+    def set_pec_edge(self, axis: int, node) -> None:
+        self._pec()[axis, node[0], node[1], node[2]] = 1
+
+    def fill_pec_box(self, ranges) -> None:
+        """CellMaterials::fill_pec_box (material.rs:262-281): every edge of
+        the node box whose end stays inside it."""
+        pm = self._pec()
+        for a in range(3):
+            r = [list(x) for x in ranges]
+            r[a][1] = r[a][1] - 1
+            if r[a][1] > r[a][0]:
+                pm[a, r[0][0]:r[0][1], r[1][0]:r[1][1], r[2][0]:r[2][1]] = 1
+
+
+class TimeStep:
+    """src/engine/coeffs.rs:125-239 (Courant and Fixed)."""
+
+    def __init__(self, kind: str, value: float):
+        self.kind, self.value = kind, float(value)
+
+    @staticmethod
+    def courant(s: float = 0.99) -> "TimeStep":
+        return TimeStep("courant", s)
+
+    @staticmethod
+    def fixed(dt: float) -> "TimeStep":
+        return TimeStep("fixed", dt)
+
+    def resolve(self, grid: Grid) -> float:
+        mx = courant_limit(grid)
+        v = self.value
+        if self.kind == "courant":
+            if math.isfinite(v) and 0.0 < v <= 1.0:
+                return v * mx
+            raise EmsError(nat.EMS_E_INVALID, f"Courant factor {v} must be in (0, 1]")
+        if math.isfinite(v) and 0.0 < v <= mx:
+            return v
+        raise EmsError(nat.EMS_E_INVALID,
+                       f"time step {v:e} s must be positive and at most the stability limit {mx:e} s")
+
+
+def courant_limit(grid: Grid) -> float:
+    return float(nat.ems_courant_limit(*grid._args()))
+
+
+def coefficients(grid: Grid, mats: Optional[CellMaterials], faces: Sequence[int], dt: float,
+                 rounding: int = 0, threads: int = 0) -> np.ndarray:
+    """Coefficients (12 dense lanes, shape (12, nx, ny, nz)) from cell
+    materials: EdgeMaterials::from_cells + Coefficients::from_edges_rounded."""
+    n = grid.nodes()
+    lanes = np.empty((12,) + n, dtype=np.float32)
+    if mats is None:
+        mats = CellMaterials.vacuum(grid)
+    table = np.ascontiguousarray(np.array([m.row() for m in mats.table], dtype=np.float64))
+    ids = np.ascontiguousarray(mats.ids)
+    pec = None if mats.pec is None else np.ascontiguousarray(mats.pec)
+    fp = (C.c_int32 * 6)(*[1 if f == FaceCondition.Pec else 0 for f in faces])
+    if threads <= 0:
+        import os
+        threads = min(32, os.cpu_count() or 1)
+    rc = nat.ems_coefficients_from_cells(*grid._args(), nat.ptr(ids, C.c_uint16),
+                                         nat.ptr(table, C.c_double), len(mats.table),
+                                         nat.ptr(pec, C.c_uint8), fp, dt, rounding, threads,
+                                         nat.ptr(lanes, C.c_float))
+    nat.check(rc)
+    return lanes
+
+
+def pool_lane(dense: np.ndarray):
+    """Lane::pool: (rows, index) if pooled, else None."""
+    nx, ny, nz = dense.shape
+    d = np.ascontiguousarray(dense)
+    rows = np.empty((max(nx * ny // 4, 1), nz), dtype=np.float32)
+    index = np.empty(nx * ny, dtype=np.uint32)
+    nrows = C.c_uint64(0)
+    r = nat.ems_pool_lane(nat.ptr(d, C.c_float), nx, ny, nz, nat.ptr(rows, C.c_float),
+                          nat.ptr(index, C.c_uint32), C.byref(nrows))
+    if r < 0:
+        nat.check(r)
+    if r == 0:
+        return None
+    return np.ascontiguousarray(rows[: nrows.value]), index
+
+
+# ------------------------------------------------------------ waveform --
+
+class Waveform:
+    """src/waveform.rs: the ems Gaussian (cos * exp on [0, 2T])."""
+
+    def __init__(self, f0: float, fc: float):
+        self.f0, self.fc = float(f0), float(fc)
+
+    @staticmethod
+    def gaussian(f0: float, fc: float) -> "Waveform":
+        return Waveform(f0, fc)
+
+    def value(self, t: float, dt: float = 0.0) -> float:
+        return float(nat.ems_gaussian(self.f0, self.fc, t))
+
+    def duration(self) -> float:
+        return float(nat.ems_gaussian_duration(self.fc))
+
+    def table(self, dt: float, offset: float, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=np.float32)
+        nat.ems_gaussian_table(self.f0, self.fc, dt, offset, count, nat.ptr(out, C.c_float))
+        return out
+
+
+# ----------------------------------------------------------- operators --
+
+class SourceKind(IntEnum):
+    """src/engine/excite.rs:10-28"""
+    SoftE = 0
+    HardE = 1
+    SoftH = 2
+    HardH = 3
+
+
+# Sample time offset of each kind, in steps (excite.rs:156-172).
+_SOURCE_OFFSET = {SourceKind.SoftE: 0.5, SourceKind.HardE: 1.0,
+                  SourceKind.SoftH: 0.0, SourceKind.HardH: 0.5}
+
+
+class EdgeSource:
+    """EdgeSource (src/engine/excite.rs:55-116). The amplitude is a
+    :class:`Waveform` (tabulated over its duration) or a callable with an
+    explicit ``horizon`` in steps after which it is taken as zero."""
+
+    def __init__(self, amplitude, horizon: Optional[int] = None):
+        self.amplitude = amplitude
+        self.horizon = horizon
+        self._kind = SourceKind.SoftE
+        self.edges: List[Tuple[int, Tuple[int, int, int], float]] = []
+
+    def kind(self, k: SourceKind) -> "EdgeSource":
+        self._kind = SourceKind(k)
+        return self
+
+    def edge(self, axis: int, node, weight: float) -> "EdgeSource":
+        self.edges.append((int(axis), tuple(int(v) for v in node), float(weight)))
+        return self
+
+    def _table(self, dt: float) -> np.ndarray:
+        off = _SOURCE_OFFSET[self._kind]
+        if isinstance(self.amplitude, Waveform):
+            count = int(math.ceil(self.amplitude.duration() / dt)) + 2
+            return self.amplitude.table(dt, off, count)
+        if self.horizon is None:
+            raise EmsError(nat.EMS_E_INVALID, "a callable amplitude needs a horizon (steps)")
+        return np.array([np.float32(self.amplitude((n + off) * dt)) for n in range(self.horizon)],
+                        dtype=np.float32)
+
+
+class ProbeQuantity(IntEnum):
+    Voltage = 0
+    Current = 1
+
+
+class Probe:
+    """Probe (src/engine/probe.rs:240-513)."""
+
+    def __init__(self, quantity: ProbeQuantity):
+        self.quantity = ProbeQuantity(quantity)
+        self.entries: List[Tuple[int, Tuple[int, int, int], float]] = []
+        self._interval = 1
+        self._engine = None
+        self._id = -1
+
+    @staticmethod
+    def voltage(axis: int, node) -> "Probe":
+        return Probe(ProbeQuantity.Voltage).entry(axis, node, 1.0)
+
+    @staticmethod
+    def current(axis: int, node) -> "Probe":
+        return Probe(ProbeQuantity.Current).entry(axis, node, 1.0)
+
+    def entry(self, axis: int, node, weight: float) -> "Probe":
+        self.entries.append((int(axis), tuple(int(v) for v in node), float(weight)))
+        return self
+
+    def interval(self, n: int) -> "Probe":
+        self._interval = max(int(n), 1)
+        return self
+
+    def values(self) -> np.ndarray:
+        return self._engine._probe_values(self._id)
+
+    def times(self) -> np.ndarray:
+        return self._engine._probe_times(self._id)
+
+
+@dataclass
+class Cpml:
+    """Cpml settings (src/fdtd.rs:177-296)."""
+    cells: int = 8
+    grading_order: float = 3.0
+    sigma_factor: float = 1.0
+    kappa_max: float = 1.0
+    alpha_max: float = 0.0
+    convolution: int = 1  # 0 Constant, 1 Linear (default)
+
+    @staticmethod
+    def with_cells(n: int) -> "Cpml":
+        return Cpml(cells=n)
+
+    def c_params(self) -> nat.CpmlParams:
+        return nat.CpmlParams(self.cells, self.grading_order, self.sigma_factor, self.kappa_max,
+                              self.alpha_max, self.convolution)
+
+    def validate(self) -> None:
+        if self.cells < 1:
+            raise EmsError(nat.EMS_E_INVALID, "CPML: needs at least one cell")
+        if not (math.isfinite(self.grading_order) and self.grading_order >= 0):
+            raise EmsError(nat.EMS_E_INVALID, "CPML: grading order must be finite and >= 0")
+        if not (math.isfinite(self.kappa_max) and self.kappa_max >= 1):
+            raise EmsError(nat.EMS_E_INVALID, "CPML: kappa_max must be finite and >= 1")
+        if not (math.isfinite(self.alpha_max) and self.alpha_max >= 0):
+            raise EmsError(nat.EMS_E_INVALID, "CPML: alpha_max must be finite and >= 0")
+        if not (math.isfinite(self.sigma_factor) and self.sigma_factor >= 0):
+            raise EmsError(nat.EMS_E_INVALID, "CPML: sigma factor must be finite and >= 0")
+
+
+def cpml_coefficients(p: Cpml, rho: float, delta: float, dt: float, n: float = 1.0) -> dict:
+    out = (C.c_double * 7)()
+    prm = p.c_params()
+    nat.ems_cpml_coefficients(C.byref(prm), rho, delta, dt, n, out)
+    keys = ("rho", "sigma", "kappa", "alpha", "b", "a", "direct")
+    return dict(zip(keys, list(out)))
+
+
+class CpmlBoundary:
+    """CpmlBoundary (src/engine/boundary/cpml.rs:752-1047): one layer per
+    (face, Cpml), sigma_max tuned to each layer's mean refractive index
+    (layer_index) measured from the engine's coefficients."""
+
+    def __init__(self, slabs: List[dict], faces: List[Tuple[Face, Cpml, float]]):
+        self.slabs = slabs
+        self.faces = faces
+
+    @staticmethod
+    def new(engine: "Engine", layers) -> "CpmlBoundary":
+        return CpmlBoundary._build(engine.grid, engine.dt, layers, engine._dense_lanes)
+
+    @staticmethod
+    def with_grid(grid: Grid, dt: float, layers) -> "CpmlBoundary":
+        return CpmlBoundary._build(grid, dt, layers, None)
+
+    @staticmethod
+    def _build(grid: Grid, dt: float, layers, lanes) -> "CpmlBoundary":
+        if not (math.isfinite(dt) and dt > 0):
+            raise EmsError(nat.EMS_E_INVALID, f"CPML: invalid time step {dt}")
+        n = grid.nodes()
+        cells = [0] * 6
+        slabs, faces = [], []
+        for face, params in layers:
+            face = Face(face)
+            params.validate()
+            if cells[face] != 0:
+                raise EmsError(nat.EMS_E_INVALID, f"CPML: face {face.name} given twice")
+            total = n[face.axis] - 1
+            if params.cells > total:
+                raise EmsError(nat.EMS_E_INVALID,
+                               f"CPML on {face.name}: {params.cells} cells but the mesh has only {total}")
+            cells[face] = params.cells
+            if lanes is None:
+                nf = 1.0
+            else:
+                lf = np.ascontiguousarray(lanes)
+                nf = float(nat.ems_cpml_layer_index(*grid._args(), nat.ptr(lf, C.c_float), dt,
+                                                    int(face), params.cells))
+            np_ = n[face.axis]
+            out = (nat.CpmlSlab * 4)()
+            coeff = np.zeros(4 * 3 * np_, dtype=np.float32)
+            prm = params.c_params()
+            rc = nat.ems_cpml_layer(*grid._args(), int(face), C.byref(prm), dt, nf, out,
+                                    nat.ptr(coeff, C.c_float))
+            nat.check(rc)
+            for q in range(4):
+                s = out[q]
+                depth = s.hi[s.normal] - s.lo[s.normal]
+                base = q * 3 * np_
+                slabs.append(dict(e_side=s.e_side, comp=s.comp, normal=s.normal,
+                                  lo=tuple(s.lo), hi=tuple(s.hi),
+                                  b=coeff[base:base + depth].copy(),
+                                  a=coeff[base + np_:base + np_ + depth].copy(),
+                                  k=coeff[base + 2 * np_:base + 2 * np_ + depth].copy()))
+            faces.append((face, params, nf))
+        for p in range(3):
+            if cells[2 * p] + cells[2 * p + 1] > n[p] - 1:
+                raise EmsError(nat.EMS_E_INVALID, f"CPML: opposite layers along axis {p} overlap")
+        return CpmlBoundary(slabs, faces)
+
+    def refractive_index(self, face: Face) -> Optional[float]:
+        for f, _, nf in self.faces:
+            if f == face:
+                return nf
+        return None
+
+
+# ------------------------------------------------------------ run loop --
+
+class StopReason(IntEnum):
+    """src/engine/run.rs:141-162"""
+    EnergyDecay = 0
+    MaxTimesteps = 1
+    Aborted = 2
+    Diverged = 3
+
+
+@dataclass
+class Settle:
+    """src/fdtd.rs Settle (defaults tolerance 1e-4, window 1/8, max_extension 1/2)."""
+    frequency: float
+    tolerance: float = 1e-4
+    window: float = 0.125
+    max_extension: float = 0.5
+
+
+@dataclass
+class EndCriteria:
+    """src/fdtd.rs:428-605"""
+    energy_ratio: float
+    check_interval: int = 50
+    settle: Optional[Settle] = None
+
+    @staticmethod
+    def energy_decay_db(db: float) -> "EndCriteria":
+        return EndCriteria(10.0 ** (db / 10.0))
+
+
+@dataclass
+class RunLimits:
+    """src/engine/run.rs:22-139"""
+    max_timesteps: int
+    end_criteria: Optional[EndCriteria] = None
+    excitation_end: Optional[float] = None
+
+    def c(self) -> nat.RunLimitsC:
+        ec = self.end_criteria
+        st = ec.settle if ec else None
+        return nat.RunLimitsC(
+            self.max_timesteps, 1 if ec else 0, ec.energy_ratio if ec else 0.0,
+            ec.check_interval if ec else 50, 0 if self.excitation_end is None else 1,
+            self.excitation_end if self.excitation_end is not None else 0.0,
+            1 if st else 0, st.frequency if st else 0.0, st.tolerance if st else 0.0,
+            st.window if st else 0.0, st.max_extension if st else 0.0)
+
+
+@dataclass
+class RunOutcome:
+    """src/engine/run.rs:192-214"""
+    reason: StopReason
+    ratio_db: float
+    steps: int
+    timestep: int
+    time: float
+    energy: float
+    max_energy: float
+    elapsed: float
+    criterion_steps: Optional[int]
+
+    @staticmethod
+    def from_c(o: nat.RunOutcomeC) -> "RunOutcome":
+        return RunOutcome(StopReason(o.reason), o.ratio_db, o.steps, o.timestep, o.time, o.energy,
+                          o.max_energy, o.elapsed_s,
+                          o.criterion_steps if o.has_criterion_steps else None)
+
+
+def drive_script(w: Sequence[float], limits: RunLimits, signal: Optional[Sequence[float]] = None):
+    """run::drive over a scripted stepper (run.rs:548-582), for testing."""
+    wa = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+    sa = None if signal is None else np.ascontiguousarray(np.asarray(signal, dtype=np.float64))
+    out = nat.RunOutcomeC()
+    lim = limits.c()
+    rc = nat.ems_drive_script(nat.ptr(wa, C.c_double), wa.size, nat.ptr(sa, C.c_double),
+                              0 if sa is None else sa.size, C.byref(lim), C.byref(out))
+    nat.check(rc)
+    return RunOutcome.from_c(out)
+
+
+# -------------------------------------------------------------- engine --
+
+class EngineBuilder:
+    """EngineBuilder (src/engine/mod.rs:225-382)."""
+
+    def __init__(self, grid: Grid):
+        self.grid = grid
+        self._mats: Optional[CellMaterials] = None
+        self._faces = [FaceCondition.Pec] * 6
+        self._ts = TimeStep.courant(0.99)
+        self._curl = 0
+        self._rounding = 0
+
+    def materials(self, m: CellMaterials) -> "EngineBuilder":
+        self._mats = m
+        return self
+
+    def faces(self, f: Sequence[int]) -> "EngineBuilder":
+        self._faces = [FaceCondition(x) for x in f]
+        return self
+
+    def timestep(self, t: TimeStep) -> "EngineBuilder":
+        self._ts = t
+        return self
+
+    def curl_order(self, oe: bool) -> "EngineBuilder":
+        self._curl = 1 if oe else 0
+        return self
+
+    def coeff_rounding(self, openems: bool) -> "EngineBuilder":
+        self._rounding = 1 if openems else 0
+        return self
+
+    def build(self) -> "Engine":
+        dt = self._ts.resolve(self.grid)
+        lanes = coefficients(self.grid, self._mats, self._faces, dt, self._rounding)
+        return Engine(self.grid, dt, lanes, curl_order=self._curl)
+
+
+class Engine:
+    """A device FDTD engine (Engine, src/engine/mod.rs:460-992)."""
+
+    @staticmethod
+    def builder(grid: Grid) -> EngineBuilder:
+        return EngineBuilder(grid)
+
+    def __init__(self, grid: Grid, dt: float, lanes: np.ndarray, curl_order: int = 0,
+                 pool: bool = True):
+        self.grid = grid
+        self.dt = float(dt)
+        self.curl_order = curl_order
+        n = grid.nodes()
+        desc = nat.GridDesc((C.c_int32 * 3)(*n), self.dt, curl_order)
+        h = C.c_void_p()
+        rc = nat.ems_gpu_create(C.byref(desc), C.byref(h))
+        if rc != 0:
+            raise EmsError(rc, "ems_gpu_create failed (no usable GPU?)")
+        self._h = h
+        self._dense_lanes = lanes
+        self.setup: List[tuple] = []
+        self.ops: list = []
+        for l in range(12):
+            lane = np.ascontiguousarray(lanes[l])
+            pooled = pool_lane(lane) if pool else None
+            if pooled is None:
+                rows = lane.reshape(-1, n[2])
+                self._check(nat.ems_gpu_set_lane(h, l, nat.ptr(rows, C.c_float), rows.shape[0], None))
+                self.setup.append(("lane", l, rows, None))
+            else:
+                rows, index = pooled
+                self._check(nat.ems_gpu_set_lane(h, l, nat.ptr(rows, C.c_float), rows.shape[0],
+                                                 nat.ptr(index, C.c_uint32)))
+                self.setup.append(("lane", l, rows, index))
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise EmsError(rc, nat.ems_gpu_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            nat.ems_gpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def release_dense_lanes(self) -> None:
+        """Drops the host copy of the dense coefficients (after setup)."""
+        self._dense_lanes = None
+
+    # -- properties
+    @property
+    def timestep_count(self) -> int:
+        return int(nat.ems_gpu_timestep(self._h))
+
+    def timestep(self) -> int:
+        return int(nat.ems_gpu_timestep(self._h))
+
+    def time(self) -> float:
+        return self.timestep() * self.dt
+
+    def coefficients(self) -> np.ndarray:
+        return self._dense_lanes
+
+    # -- operators
+    def add_operator(self, op):
+        """Engine::add_operator (mod.rs:549-572); returns the operator."""
+        h = self._h
+        if isinstance(op, CpmlBoundary):
+            arr = (nat.CpmlSlab * len(op.slabs))()
+            keep = []
+            for q, s in enumerate(op.slabs):
+                b, a, k = (np.ascontiguousarray(s[x]) for x in ("b", "a", "k"))
+                keep += [b, a, k]
+                arr[q] = nat.CpmlSlab(s["e_side"], s["comp"], s["normal"],
+                                      (C.c_int32 * 3)(*s["lo"]), (C.c_int32 * 3)(*s["hi"]),
+                                      nat.ptr(b, C.c_float), nat.ptr(a, C.c_float),
+                                      nat.ptr(k, C.c_float))
+            self._check(nat.ems_gpu_add_cpml(h, arr, len(op.slabs)))
+            self.setup.append(("cpml", op.slabs))
+        elif isinstance(op, EdgeSource):
+            amp = np.ascontiguousarray(op._table(self.dt))
+            axis, nodes, w = self._edges(op.edges, np.float32)
+            oid = C.c_int()
+            self._check(nat.ems_gpu_add_source(h, int(op._kind), len(op.edges),
+                                               nat.ptr(axis, C.c_int32), nat.ptr(nodes, C.c_int32),
+                                               nat.ptr(w, C.c_float), nat.ptr(amp, C.c_float),
+                                               amp.size, C.byref(oid)))
+            self.setup.append(("source", int(op._kind), axis, nodes, w, amp))
+            op._id = oid.value
+        elif isinstance(op, Probe):
+            axis, nodes, w = self._edges(op.entries, np.float64)
+            oid = C.c_int()
+            self._check(nat.ems_gpu_add_probe(h, int(op.quantity), len(op.entries),
+                                              nat.ptr(axis, C.c_int32), nat.ptr(nodes, C.c_int32),
+                                              nat.ptr(w, C.c_double), op._interval, C.byref(oid)))
+            self.setup.append(("probe", int(op.quantity), axis, nodes, w, op._interval))
+            op._engine = self
+            op._id = oid.value
+        else:
+            raise EmsError(nat.EMS_E_UNSUPPORTED, f"operator {type(op).__name__}")
+        self.ops.append(op)
+        return op
+
+    @staticmethod
+    def _edges(edges, wtype):
+        axis = np.ascontiguousarray(np.array([e[0] for e in edges], dtype=np.int32))
+        nodes = np.ascontiguousarray(np.array([e[1] for e in edges], dtype=np.int32).reshape(-1))
+        w = np.ascontiguousarray(np.array([e[2] for e in edges], dtype=wtype))
+        return axis, nodes, w
+
+    # -- stepping
+    def run(self, n: int) -> None:
+        self._check(nat.ems_gpu_run(self._h, int(n)))
+
+    def step(self) -> None:
+        self.run(1)
+
+    def step_with_energy(self) -> float:
+        w = C.c_double()
+        self._check(nat.ems_gpu_step_with_energy(self._h, C.byref(w)))
+        return w.value
+
+    def energy(self) -> float:
+        w = C.c_double()
+        self._check(nat.ems_gpu_energy(self._h, C.byref(w)))
+        return w.value
+
+    def run_until(self, limits: RunLimits, watch: Sequence[Probe] = ()) -> RunOutcome:
+        out = nat.RunOutcomeC()
+        lim = limits.c()
+        ids = np.ascontiguousarray(np.array([p._id for p in watch], dtype=np.int32))
+        self._check(nat.ems_run_until(self._h, C.byref(lim), nat.ptr(ids, C.c_int32), ids.size,
+                                      C.byref(out)))
+        return RunOutcome.from_c(out)
+
+    def synchronize(self) -> None:
+        self._check(nat.ems_gpu_synchronize(self._h))
+
+    # -- fields
+    def _read(self, field: int, axis: int) -> np.ndarray:
+        out = np.empty(self.grid.nodes(), dtype=np.float32)
+        self._check(nat.ems_gpu_read_component(self._h, field, int(axis), nat.ptr(out, C.c_float)))
+        return out
+
+    def u_component(self, axis: int) -> np.ndarray:
+        return self._read(0, axis)
+
+    def i_component(self, axis: int) -> np.ndarray:
+        return self._read(1, axis)
+
+    def set_component(self, field: int, axis: int, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float32).reshape(self.grid.nodes()))
+        self._check(nat.ems_gpu_write_component(self._h, field, int(axis), nat.ptr(v, C.c_float)))
+        self.setup.append(("write", field, int(axis), v.copy()))
+
+    def _probe_values(self, pid: int) -> np.ndarray:
+        cnt = C.c_uint64()
+        self._check(nat.ems_gpu_probe_values(self._h, pid, None, 0, C.byref(cnt)))
+        out = np.empty(cnt.value, dtype=np.float64)
+        self._check(nat.ems_gpu_probe_values(self._h, pid, nat.ptr(out, C.c_double), out.size,
+                                             C.byref(cnt)))
+        return out
+
+    def _probe_times(self, pid: int) -> np.ndarray:
+        cnt = C.c_uint64()
+        self._check(nat.ems_gpu_probe_times(self._h, pid, None, 0, C.byref(cnt)))
+        out = np.empty(cnt.value, dtype=np.float64)
+        self._check(nat.ems_gpu_probe_times(self._h, pid, nat.ptr(out, C.c_double), out.size,
+                                            C.byref(cnt)))
+        return out
+
+    # -- measurement
+    def time_kernels(self, steps: int) -> Tuple[float, float, float]:
+        out = (C.c_double * 3)()
+        self._check(nat.ems_gpu_time_kernels(self._h, int(steps), out))
+        return tuple(out)
+
+    def traffic_model(self) -> Tuple[float, float, float]:
+        out = (C.c_double * 3)()
+        self._check(nat.ems_gpu_traffic_model(self._h, out))
+        return tuple(out)

@@ -1,0 +1,138 @@
+/*
+ * ems_gpu.h -- the drop-in boundary: an MI355X (gfx950) FDTD step engine
+ * behind a C ABI.
+ *
+ * It replaces the reference engine's step scheduler and kernels, i.e.
+ * everything under `Engine::advance` -> `pool::drive`
+ * (src/engine/mod.rs:833-992, src/engine/pool.rs:878-940). The host keeps
+ * mesh compilation (`Compiled::build_driving`, src/compile/mod.rs:926-936)
+ * and hands over the compiled per-edge arrays once; the engine owns all
+ * device memory and one HIP stream.
+ *
+ * Conventions (mirroring src/engine/mod.rs:1-62):
+ *   - nodes n = [nx, ny, nz]; every field and coefficient array has one entry
+ *     per node at (i*ny + j)*nz + k (z contiguous);
+ *   - u[a] are the E-edge voltages, i[a] the H dual-edge currents; after n
+ *     completed steps u holds t = n*dt and i holds (n - 1/2)*dt;
+ *   - coefficient lanes: ca 0-2, cb 3-5, da 6-8, db 9-11 (coeffs.rs:385-391);
+ *   - stepping is infallible in the reference; here every call returns 0 on
+ *     success and a negative EMS_E_* code otherwise, with the message from
+ *     ems_gpu_last_error. Device faults are sticky.
+ *   - not thread-safe per handle; distinct handles may be used concurrently.
+ *
+ * Arithmetic: f32 fields and coefficients, f64 probes and energy, compiled
+ * without FP contraction: fields, probes and energies are bit-identical to
+ * the reference CPU engine given the same arrays.
+ */
+#ifndef EMS_GPU_H
+#define EMS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMS_OK 0
+#define EMS_E_INVALID (-1)     /* ems::Error::Invalid (src/error.rs) */
+#define EMS_E_UNSUPPORTED (-2) /* ems::Error::Unsupported */
+#define EMS_E_IO (-3)          /* ems::Error::Io */
+#define EMS_E_DEVICE (-4)      /* HIP error (no reference counterpart) */
+
+typedef struct ems_gpu ems_gpu;
+
+/* Grid of the engine: EngineBuilder::build_from (src/engine/mod.rs:325-382)
+ * after TimeStep::resolve. curl_order: 0 Paired (default), 1 OpenEms
+ * (CurlOrder, src/engine/kernel.rs:81-107). */
+typedef struct {
+    int32_t nodes[3];
+    double dt;
+    int32_t curl_order;
+} ems_grid_desc;
+
+/* Engine::builder(...).build() -> one device engine. */
+int ems_gpu_create(const ems_grid_desc *desc, ems_gpu **out);
+void ems_gpu_destroy(ems_gpu *h);
+const char *ems_gpu_last_error(const ems_gpu *h);
+
+/* Coefficient lane `lane` (0..11): `nrows` z-rows of nz floats, and the
+ * row of each (i, j) pair (index i*ny + j) among them, or NULL for a dense
+ * lane (nrows = nx*ny). Mirrors Lane::{Dense, Pooled} and CoeffRows::row
+ * (src/engine/coeffs.rs:393-599, src/engine/rows.rs:45-50). Lanes default to
+ * all zero. */
+int ems_gpu_set_lane(ems_gpu *h, int lane, const float *rows, uint64_t nrows,
+                     const uint32_t *row_index);
+
+/* One CPML slab (src/engine/boundary/cpml.rs:394-437): e_side 1 corrects u
+ * from the i currents (after the E kernel), 0 corrects i from u (after the
+ * H kernel). `comp` is the corrected component, `normal` the layer normal;
+ * the node box lo..hi and b, a, k (= 1/kappa - 1 - shift) per depth index
+ * node[normal] - lo[normal]. Slabs are applied in the order given, as the
+ * reference applies its layers in registration order (cpml.rs:815-850). */
+typedef struct {
+    int32_t e_side, comp, normal;
+    int32_t lo[3], hi[3];
+    const float *b, *a, *k;
+} ems_cpml_slab;
+int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *slabs, int n);
+
+/* EdgeSource (src/engine/excite.rs:10-173): kind 0 SoftE, 1 HardE,
+ * 2 SoftH, 3 HardH; edges (axis, node[3], weight f32). The host tabulates
+ * the amplitude `a(t) as f32` at each step's sample time (SoftE (n+1/2)dt,
+ * HardE (n+1)dt, SoftH n*dt, HardH (n+1/2)dt); steps at or past n_amp have
+ * amplitude 0. *id receives the operator id. */
+int ems_gpu_add_source(ems_gpu *h, int kind, int n_edges, const int32_t *axis,
+                       const int32_t *nodes, const float *w, const float *amp, uint64_t n_amp,
+                       int *id);
+
+/* Probe (src/engine/probe.rs:240-513): quantity 0 voltage (u after the E
+ * update, t = (n+1)dt), 1 current (i after the H update, t = (n+1/2)dt);
+ * value = sum w*f64(x) in entry order, every `interval` steps. */
+int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *axis,
+                      const int32_t *nodes, const double *w, uint64_t interval, int *id);
+
+/* Engine::run (src/engine/mod.rs:825-828). */
+int ems_gpu_run(ems_gpu *h, uint64_t n);
+/* Engine::step_with_energy (src/engine/mod.rs:808-823): one step, *w = the
+ * energy invariant W^n of the state before it. */
+int ems_gpu_step_with_energy(ems_gpu *h, double *w);
+/* Engine::energy (src/engine/mod.rs:722-726): read-only W^n. */
+int ems_gpu_energy(ems_gpu *h, double *w);
+uint64_t ems_gpu_timestep(const ems_gpu *h);
+
+/* Engine::u_component / i_component / fields_mut (src/engine/mod.rs:659-690):
+ * field 0 = u, 1 = i; nx*ny*nz floats. */
+int ems_gpu_read_component(ems_gpu *h, int field, int axis, float *out);
+int ems_gpu_write_component(ems_gpu *h, int field, int axis, const float *in);
+
+/* Probe::values (src/engine/probe.rs:354-357): copies up to cap samples;
+ * *count receives the number recorded so far. */
+int ems_gpu_probe_values(ems_gpu *h, int id, double *out, uint64_t cap, uint64_t *count);
+
+/* Probe::times (src/engine/probe.rs:339-342, SampleTimes::get :93-104):
+ * sample k at ((step_k as f64) + offset)*dt, offset 1 for voltages and 1/2
+ * for currents. */
+int ems_gpu_probe_times(ems_gpu *h, int id, double *out, uint64_t cap, uint64_t *count);
+
+/* Engine::dt (src/engine/mod.rs). */
+double ems_gpu_dt(const ems_gpu *h);
+
+/* Waits for all queued device work. */
+int ems_gpu_synchronize(ems_gpu *h);
+
+/* Measurement hook (no reference counterpart): runs `steps` steps with HIP
+ * events around the main stencil kernels on the engine's stream and
+ * returns their mean durations in ms: out[0] H update, out[1] E update,
+ * out[2] whole step. */
+int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]);
+
+/* Algorithmic HBM bytes of one launch of each stencil kernel, from the lanes
+ * and slabs the engine holds (no reference counterpart): out[0] H update,
+ * out[1] E update, out[2] all CPML slab launches of one step. */
+int ems_gpu_traffic_model(ems_gpu *h, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

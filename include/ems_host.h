@@ -1,0 +1,124 @@
+/*
+ * ems_host.h -- the host side above the boundary, in C++: the parts of the
+ * reference's compile/setup and run loop that produce what include/ems_gpu.h
+ * consumes, and drive it.
+ *
+ * The reference keeps these on its (Rust) host: a Rust caller would use its
+ * own `Coefficients::from_edges_rounded`, `CpmlBoundary::new` and
+ * `run::drive` and bind only ems_gpu.h. They are restated here so the
+ * engine can be driven without the Rust toolchain, with the same names and
+ * argument meanings.
+ *
+ * Arrays use the engine layout (i*ny + j)*nz + k; cells use
+ * (i*(ny-1) + j)*(nz-1) + k. Returns 0 or a negative EMS_E_* code.
+ */
+#ifndef EMS_HOST_H
+#define EMS_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ems_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* courant_limit (src/engine/coeffs.rs:249-255). */
+double ems_courant_limit(const double *x, int nx, const double *y, int ny, const double *z,
+                         int nz);
+
+/* CellMaterials -> EdgeMaterials::from_cells (src/engine/edge_material.rs:376-417)
+ * -> Coefficients::from_edges_rounded (src/engine/coeffs.rs:879-966).
+ * cell_mat: material id per cell (NULL = all material 0); mat_table: per
+ * material 12 doubles eps_r[3], mu_r[3], kappa[3], sigma_m[3] (Material,
+ * src/engine/material.rs:22-31); pec: PEC mask per E edge, 3*N bytes (axis
+ * major) or NULL; face_pec[6]: FaceCondition::Pec of x_min..z_max;
+ * rounding 0 Exact, 1 OpenEms (CoeffRounding, coeffs.rs:706-761).
+ * lanes: out, 12*N floats (ca 0-2, cb 3-5, da 6-8, db 9-11).
+ * threads: worker threads for the x-planes (results do not depend on it). */
+int ems_coefficients_from_cells(const double *x, int nx, const double *y, int ny,
+                                const double *z, int nz, const uint16_t *cell_mat,
+                                const double *mat_table, int n_mat, const uint8_t *pec,
+                                const int face_pec[6], double dt, int rounding, int threads,
+                                float *lanes);
+
+/* Lane::pool (src/engine/coeffs.rs:560-598): pools the nx*ny z-rows of a
+ * dense lane if at most max(count/4, 1) are distinct. Returns 1 and fills
+ * rows_out (<= count/4 rows, first-seen order) / index_out (nx*ny) when
+ * pooled, 0 when the lane stays dense. */
+int ems_pool_lane(const float *dense, int nx, int ny, int nz, float *rows_out,
+                  uint32_t *index_out, uint64_t *nrows_out);
+
+/* Cpml settings (src/fdtd.rs:177-296). */
+typedef struct {
+    int32_t cells;
+    double grading_order, sigma_factor, kappa_max, alpha_max;
+    int32_t convolution; /* 0 Constant, 1 Linear (default) */
+} ems_cpml_params;
+
+/* CpmlCoefficients::in_medium (src/engine/boundary/cpml.rs:350-388):
+ * out = rho, sigma, kappa, alpha, b, a, direct. */
+void ems_cpml_coefficients(const ems_cpml_params *p, double rho, double delta, double dt,
+                           double n, double out[7]);
+
+/* layer_index (cpml.rs:1054-1120): mean refractive index of the `cells`
+ * layer on face (0 x_min .. 5 z_max), from dense lanes (12*N floats). */
+double ems_cpml_layer_index(const double *x, int nx, const double *y, int ny, const double *z,
+                            int nz, const float *lanes, double dt, int face, int cells);
+
+/* Layer::new (cpml.rs:1122-1207): the four slabs of one face layer, in the
+ * order E(a1), E(a2), H(a1), H(a2) with (a1, a2) the face's transverse
+ * axes. b/a/k of slab q are written to coeff[q*3*np + {0,1,2}*np ..] with
+ * np = nodes along the normal; the slab pointers point into coeff. */
+int ems_cpml_layer(const double *x, int nx, const double *y, int ny, const double *z, int nz,
+                   int face, const ems_cpml_params *p, double dt, double n_face,
+                   ems_cpml_slab out[4], float *coeff);
+
+/* Waveform::gaussian(f0, fc).value(t) (src/waveform.rs:442-453). */
+double ems_gaussian(double f0, double fc, double t);
+/* Its duration 2T (waveform.rs:506-513). */
+double ems_gaussian_duration(double fc);
+/* amp[n] = (gaussian((n + offset)*dt) as f32), n < count. */
+void ems_gaussian_table(double f0, double fc, double dt, double offset, uint64_t count,
+                        float *amp);
+
+/* RunLimits + EndCriteria (src/engine/run.rs:22-139, src/fdtd.rs:428-605). */
+typedef struct {
+    uint64_t max_timesteps;
+    int32_t has_criteria;
+    double energy_ratio;
+    uint64_t check_interval;
+    int32_t has_excitation_end;
+    double excitation_end;
+    /* Settle (fdtd.rs:440-490); has_settle 0 = none */
+    int32_t has_settle;
+    double settle_frequency, settle_tolerance, settle_window, settle_max_extension;
+} ems_run_limits;
+
+/* RunOutcome / StopReason (run.rs:141-214): reason 0 EnergyDecay,
+ * 1 MaxTimesteps, 2 Aborted, 3 Diverged. */
+typedef struct {
+    int32_t reason;
+    double ratio_db;
+    uint64_t steps, timestep;
+    double time, energy, max_energy, elapsed_s;
+    int32_t has_criterion_steps;
+    uint64_t criterion_steps;
+} ems_run_outcome;
+
+/* Engine::run_until (run.rs:216-273, drive 424-540) on a device engine.
+ * Probe ids whose signals the settle test watches: probes[n_probes]. */
+int ems_run_until(ems_gpu *h, const ems_run_limits *limits, const int *probes, int n_probes,
+                  ems_run_outcome *out);
+
+/* The same loop over a scripted stepper (run.rs:548-582): energy
+ * w[min(n, nw-1)] at timestep n, dt = 1, and one probe sampled every step
+ * with value signal[k] (k < ns; none if ns = 0). For testing the loop. */
+int ems_drive_script(const double *w, uint64_t nw, const double *signal, uint64_t ns,
+                     const ems_run_limits *limits, ems_run_outcome *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,194 @@
+"""ctypes bridge to the CPU oracle (liboracle.so in this folder): TEST INFRASTRUCTURE.
+
+The oracle is the checker: tests build an :class:`OracleEngine` from the same
+grid, dt, coefficient lanes and operators as the device engine and compare.
+This module is test infrastructure; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from ems_amd import engine as E
+from ems_amd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+
+def _load():
+    if not os.path.exists(ORACLE_LIB):
+        subprocess.run(["make", "-C", ORACLE_DIR], check=True, capture_output=True)
+    return C.CDLL(ORACLE_LIB)
+
+
+lib = _load()
+_d, _i, _u64, _p = C.c_double, C.c_int, C.c_uint64, C.c_void_p
+_fp, _dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+_ip, _u32p, _u8p = C.POINTER(C.c_int), C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype, f.argtypes = res, list(args)
+    return f
+
+
+orc_create = _sig("orc_create", _p, _i, _i, _i, _d, _i, _i)
+orc_destroy = _sig("orc_destroy", None, _p)
+orc_set_lane = _sig("orc_set_lane", _i, _p, _i, _fp, _u64, _u32p)
+orc_add_cpml = _sig("orc_add_cpml", _i, _p, _i, _i, _i, _ip, _ip, _fp, _fp, _fp)
+orc_add_source = _sig("orc_add_source", _i, _p, _i, _i, _ip, _ip, _fp, _fp, _u64)
+orc_add_probe = _sig("orc_add_probe", _i, _p, _i, _i, _ip, _ip, _dp, _u64)
+orc_run = _sig("orc_run", None, _p, _u64)
+orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
+orc_energy = _sig("orc_energy", _d, _p)
+orc_timestep = _sig("orc_timestep", _u64, _p)
+orc_read = _sig("orc_read", _i, _p, _i, _i, _fp)
+orc_write = _sig("orc_write", _i, _p, _i, _i, _fp)
+orc_probe_count = _sig("orc_probe_count", _u64, _p, _i)
+orc_probe_values = _sig("orc_probe_values", _u64, _p, _i, _dp, _u64)
+orc_courant_limit = _sig("orc_courant_limit", _d, _dp, _i, _dp, _i, _dp, _i)
+orc_coefficients = _sig("orc_coefficients", _i, _dp, _i, _dp, _i, _dp, _i, _dp, _dp, _dp, _dp, _u8p,
+                        _ip, _d, _fp)
+orc_cpml_coefficients = _sig("orc_cpml_coefficients", None, _d, _d, _d, _d, _i, _d, _d, _d, _d, _dp)
+orc_drive_script = _sig("orc_drive_script", _i, _dp, _u64, _u64, _i, _d, _u64, _i, _d,
+                        C.POINTER(_u64))
+
+
+def P(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+class OracleEngine:
+    """The CPU restatement, fed the same arrays as :class:`ems_amd.Engine`."""
+
+    def __init__(self, grid: E.Grid, dt: float, lanes: np.ndarray, curl_order: int = 0,
+                 threads: int = 1, pool: bool = True):
+        self.grid, self.dt = grid, float(dt)
+        self._dense_lanes = lanes
+        n = grid.nodes()
+        self._h = orc_create(n[0], n[1], n[2], self.dt, curl_order, threads)
+        assert self._h, "orc_create failed"
+        for l in range(12):
+            lane = np.ascontiguousarray(lanes[l])
+            pooled = E.pool_lane(lane) if pool else None
+            if pooled is None:
+                rows = lane.reshape(-1, n[2])
+                assert orc_set_lane(self._h, l, P(rows, C.c_float), rows.shape[0], None) == 0
+            else:
+                rows, index = pooled
+                assert orc_set_lane(self._h, l, P(rows, C.c_float), rows.shape[0],
+                                    P(index, C.c_uint32)) == 0
+        self.ops = []
+
+    def close(self):
+        if self._h:
+            orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_operator(self, op):
+        h = self._h
+        if isinstance(op, E.CpmlBoundary):
+            for s in op.slabs:
+                lo = (C.c_int * 3)(*s["lo"])
+                hi = (C.c_int * 3)(*s["hi"])
+                b, a, k = (np.ascontiguousarray(s[x]) for x in ("b", "a", "k"))
+                assert orc_add_cpml(h, s["e_side"], s["comp"], s["normal"], lo, hi,
+                                    P(b, C.c_float), P(a, C.c_float), P(k, C.c_float)) >= 0
+        elif isinstance(op, E.EdgeSource):
+            amp = np.ascontiguousarray(op._table(self.dt))
+            axis, nodes, w = E.Engine._edges(op.edges, np.float32)
+            axis = axis.astype(np.intc)
+            nodes = nodes.astype(np.intc)
+            op._oid = orc_add_source(h, int(op._kind), len(op.edges), P(axis, C.c_int),
+                                     P(nodes, C.c_int), P(w, C.c_float), P(amp, C.c_float),
+                                     amp.size)
+            assert op._oid >= 0
+        elif isinstance(op, E.Probe):
+            axis, nodes, w = E.Engine._edges(op.entries, np.float64)
+            axis = axis.astype(np.intc)
+            nodes = nodes.astype(np.intc)
+            op._oid = orc_add_probe(h, int(op.quantity), len(op.entries), P(axis, C.c_int),
+                                    P(nodes, C.c_int), P(w, C.c_double), op._interval)
+            assert op._oid >= 0
+        else:
+            raise TypeError(type(op))
+        self.ops.append(op)
+        return op
+
+    def run(self, n):
+        orc_run(self._h, int(n))
+
+    def step_with_energy(self):
+        return orc_step_with_energy(self._h)
+
+    def energy(self):
+        return orc_energy(self._h)
+
+    def timestep(self):
+        return orc_timestep(self._h)
+
+    def _read(self, field, axis):
+        out = np.empty(self.grid.nodes(), dtype=np.float32)
+        assert orc_read(self._h, field, int(axis), P(out, C.c_float)) == 0
+        return out
+
+    def u_component(self, axis):
+        return self._read(0, axis)
+
+    def i_component(self, axis):
+        return self._read(1, axis)
+
+    def set_component(self, field, axis, values):
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float32).reshape(self.grid.nodes()))
+        assert orc_write(self._h, field, int(axis), P(v, C.c_float)) == 0
+
+    def probe_values(self, op):
+        n = orc_probe_count(self._h, op._oid)
+        out = np.empty(n, dtype=np.float64)
+        orc_probe_values(self._h, op._oid, P(out, C.c_double), n)
+        return out
+
+
+def oracle_coefficients(grid: E.Grid, eps_r, kappa, mu_r, sigma_m, pec, pec_face, dt):
+    """Coefficients::from_edges restated in the oracle (per-edge inputs)."""
+    n = grid.nodes()
+    N = n[0] * n[1] * n[2]
+    out = np.empty((12,) + n, dtype=np.float32)
+    arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(3 * N))
+            for a in (eps_r, kappa, mu_r, sigma_m)]
+    pm = None if pec is None else np.ascontiguousarray(np.asarray(pec, dtype=np.uint8).reshape(3 * N))
+    pf = (C.c_int * 6)(*pec_face)
+    x, y, z = grid.lines
+    rc = orc_coefficients(P(x, C.c_double), x.size, P(y, C.c_double), y.size, P(z, C.c_double),
+                          z.size, *[P(a, C.c_double) for a in arrs], P(pm, C.c_uint8), pf, dt,
+                          P(out, C.c_float))
+    assert rc == 0
+    return out
+
+
+def oracle_cpml_coefficients(p: E.Cpml, rho, delta, dt, n=1.0):
+    out = (C.c_double * 7)()
+    orc_cpml_coefficients(p.grading_order, p.sigma_factor, p.kappa_max, p.alpha_max,
+                          p.convolution, rho, delta, dt, n, out)
+    return dict(zip(("rho", "sigma", "kappa", "alpha", "b", "a", "direct"), list(out)))
+
+
+def oracle_drive_script(w, max_steps, energy_ratio=None, interval=50, excitation_end=None):
+    wa = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+    steps = C.c_uint64()
+    r = orc_drive_script(P(wa, C.c_double), wa.size, max_steps, energy_ratio is not None,
+                         energy_ratio or 0.0, interval, excitation_end is not None,
+                         excitation_end or 0.0, C.byref(steps))
+    return r, steps.value

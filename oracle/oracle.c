@@ -1,0 +1,806 @@
+/*
+ * oracle.c -- CPU restatement of the ems engine step (TEST INFRASTRUCTURE).
+ *
+ * Parity oracle and CPU baseline only; see oracle.h for what may use it and
+ * how each part is pinned. Compiled with -O2 -ffp-contract=off and no
+ * fast-math, so every f32/f64 expression rounds exactly as the reference's
+ * Rust code (no FMA contraction, SSE arithmetic on x86-64).
+ *
+ * Reference files cited below are in the ems repository (src/...).
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* constants (src/constants.rs)                                        */
+/* ------------------------------------------------------------------ */
+static const double C0 = 299792458.0;
+static const double MU0 = 1.25663706127e-6;
+#define EPS0 (1.0 / (MU0 * C0 * C0))
+#define Z0 (MU0 * C0)
+
+/* kernel.rs:30-33: subnormal results become 0, everything else unchanged. */
+static inline float flush(float x) { return fabsf(x) < FLT_MIN ? 0.0f : x; }
+
+/* kernel.rs:38-42 (openEMS order, every op flushed). */
+static inline float oe_h(float da, float h, float db, float a, float b, float c, float d) {
+    float curl = flush(flush(flush(a - b) + d) - c);
+    return flush(flush(da * h) - flush(db * curl));
+}
+/* kernel.rs:46-50 */
+static inline float oe_e(float ca, float u, float cb, float a, float b, float c, float d) {
+    float curl = flush(flush(flush(a - b) - c) + d);
+    return flush(flush(ca * u) + flush(cb * curl));
+}
+/* kernel.rs:76 (Paired order) */
+static inline float pair_h(float da, float h, float db, float a, float b, float c, float d) {
+    return flush(da * h - db * ((a - b) - (c - d)));
+}
+/* kernel.rs:270 */
+static inline float pair_e(float ca, float u, float cb, float a, float b, float c, float d) {
+    return flush(ca * u + cb * ((a - b) - (c - d)));
+}
+/* kernel.rs:114-117: (1 + a)*p/b, 0 where b = 0. */
+static inline double weight(float a, double p, float b) {
+    double w = (1.0 + (double)a) * p / (double)b;
+    return b != 0.0f ? w : 0.0;
+}
+
+/* ------------------------------------------------------------------ */
+/* engine state                                                        */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    float *rows;
+    uint32_t *index; /* NULL: dense */
+    uint64_t nrows;
+} lane_t;
+
+typedef struct {
+    int e_side, comp, other, normal;
+    float sign;
+    int lo[3], hi[3];
+    float *b, *a, *k, *psi;
+} cpml_t;
+
+enum { OP_SOURCE = 1, OP_PROBE = 2 };
+
+typedef struct {
+    int type;
+    /* source */
+    int kind;
+    int n;
+    int *axis;
+    size_t *idx;
+    float *wf;
+    float *amp;
+    uint64_t namp;
+    /* probe */
+    int quantity;
+    double *wd;
+    uint64_t interval;
+    double *vals;
+    uint64_t nvals, cap;
+} op_t;
+
+struct orc_engine {
+    int n[3];
+    size_t N, plane;
+    double dt;
+    int oe;
+    float *u[3], *i[3];
+    float *zero;
+    lane_t lanes[12];
+    cpml_t *cpml;
+    int ncpml;
+    op_t *ops;
+    int nops;
+    uint64_t step;
+    double *plane_energy;
+    /* threads */
+    int threads;
+    int *slab_start;
+    pthread_t *workers;
+    pthread_barrier_t bar;
+    int cmd; /* 1 H, 2 E, 3 energy (read only), 9 exit */
+    int energy_flag;
+    char err[256];
+};
+
+static inline int axis_next(int a) { return (a + 1) % 3; }
+static inline int axis_prev(int a) { return (a + 2) % 3; }
+
+/* CoeffRows::row (rows.rs:45-50): row (gi, j) of lane l. */
+static inline const float *row(const orc_engine *e, int l, int gi, int j) {
+    const lane_t *L = &e->lanes[l];
+    size_t r = (size_t)gi * e->n[1] + j;
+    size_t id = L->index ? L->index[r] : r;
+    return L->rows + id * e->n[2];
+}
+
+/* ------------------------------------------------------------------ */
+/* H update of one x-plane, with the energy share (kernel.rs:119-173,   */
+/* 364-423, 443-511).                                                   */
+/* ------------------------------------------------------------------ */
+
+/* One H row; with energy the 4 interleaved f64 accumulators of
+ * h_row_energy (kernel.rs:145-172): entry k into acc[k & 3], result
+ * (a0 + a1) + (a2 + a3). */
+static double hrow(const orc_engine *e, int energy, int commit, float *h, const float *da,
+                   const float *db, const float *a, const float *b, const float *c,
+                   const float *d, int n) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < n; k++) {
+        float old = h[k];
+        float nw = e->oe ? oe_h(da[k], old, db[k], a[k], b[k], c[k], d[k])
+                         : pair_h(da[k], old, db[k], a[k], b[k], c[k], d[k]);
+        if (energy) acc[k & 3] += weight(da[k], (double)old * (double)nw, db[k]);
+        if (commit) h[k] = nw;
+    }
+    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+/* h_plane (kernel.rs:447-511) + the E part of h_slab's energy
+ * (kernel.rs:404-420). Returns the plane value we + wh (energy) or 0. */
+static double h_plane(orc_engine *e, int gi, int energy, int commit) {
+    const int nx = e->n[0], ny = e->n[1], nz = e->n[2];
+    const size_t p = e->plane, off = (size_t)gi * p;
+    float *ix = e->i[0] + off, *iy = e->i[1] + off, *iz = e->i[2] + off;
+    const float *ux = e->u[0] + off, *uy = e->u[1] + off, *uz = e->u[2] + off;
+    const int m = nz - 1;
+    double wh = 0.0;
+    for (int j = 0; j < ny - 1; j++) {
+        size_t r = (size_t)j * nz;
+        wh += hrow(e, energy, commit, ix + r, row(e, 6, gi, j), row(e, 9, gi, j), uz + r + nz,
+                   uz + r, uy + r + 1, uy + r, m);
+    }
+    if (gi + 1 < nx) {
+        const float *uyn = uy + p, *uzn = uz + p;
+        for (int j = 0; j < ny; j++) {
+            size_t r = (size_t)j * nz;
+            wh += hrow(e, energy, commit, iy + r, row(e, 7, gi, j), row(e, 10, gi, j), ux + r + 1,
+                       ux + r, uzn + r, uz + r, m);
+        }
+        for (int j = 0; j < ny - 1; j++) {
+            size_t r = (size_t)j * nz;
+            wh += hrow(e, energy, commit, iz + r, row(e, 8, gi, j), row(e, 11, gi, j), uyn + r,
+                       uy + r, ux + r + nz, ux + r, nz);
+        }
+    }
+    if (!energy) return 0.0;
+    /* e_energy_row (kernel.rs:218-244): per component, 4 accumulators
+     * chained over the whole plane by in-plane index & 3. */
+    double we = 0.0;
+    const float *uc[3] = {ux, uy, uz};
+    for (int a = 0; a < 3; a++) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < ny; j++) {
+            const float *ca = row(e, a, gi, j), *cb = row(e, 3 + a, gi, j);
+            size_t r = (size_t)j * nz;
+            for (int k = 0; k < nz; k++) {
+                double x = (double)uc[a][r + k];
+                acc[(r + k) & 3] += weight(ca[k], x * x, cb[k]);
+            }
+        }
+        we += (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    }
+    return we + wh;
+}
+
+/* update_e_plane / e_rows (kernel.rs:552-697). */
+static void e_plane(orc_engine *e, int gi) {
+    const int nx = e->n[0], ny = e->n[1], nz = e->n[2];
+    const size_t p = e->plane, off = (size_t)gi * p;
+    float *ux = e->u[0] + off, *uy = e->u[1] + off, *uz = e->u[2] + off;
+    const float *ix = e->i[0] + off, *iy = e->i[1] + off, *iz = e->i[2] + off;
+    const float *iyp = gi == 0 ? e->zero : iy - p;
+    const float *izp = gi == 0 ? e->zero : iz - p;
+    const float *zero = e->zero;
+    const int oe = e->oe;
+#define EUPD(ca, u, cb, a, b, c, d) (oe ? oe_e(ca, u, cb, a, b, c, d) : pair_e(ca, u, cb, a, b, c, d))
+    for (int j = 0; j < ny; j++) {
+        size_t r = (size_t)j * nz;
+        if (gi + 1 < nx) {
+            const float *ca = row(e, 0, gi, j), *cb = row(e, 3, gi, j);
+            const float *izm = j == 0 ? zero : iz + r - nz;
+            for (int k = 0; k < nz; k++) {
+                float dm = k == 0 ? 0.0f : iy[r + k - 1];
+                ux[r + k] = EUPD(ca[k], ux[r + k], cb[k], iz[r + k], izm[k], iy[r + k], dm);
+            }
+        }
+        if (j + 1 < ny) {
+            const float *ca = row(e, 1, gi, j), *cb = row(e, 4, gi, j);
+            for (int k = 0; k < nz; k++) {
+                float bm = k == 0 ? 0.0f : ix[r + k - 1];
+                uy[r + k] = EUPD(ca[k], uy[r + k], cb[k], ix[r + k], bm, iz[r + k], izp[r + k]);
+            }
+        }
+        {
+            const float *ca = row(e, 2, gi, j), *cb = row(e, 5, gi, j);
+            const float *ixm = j == 0 ? zero : ix + r - nz;
+            for (int k = 0; k < nz - 1; k++)
+                uz[r + k] = EUPD(ca[k], uz[r + k], cb[k], iy[r + k], iyp[r + k], ix[r + k], ixm[k]);
+        }
+    }
+#undef EUPD
+}
+
+/* ------------------------------------------------------------------ */
+/* CPML slab correction (cpml.rs:469-543, 693-736): for the x-planes of  */
+/* [x0, x1).                                                             */
+/* ------------------------------------------------------------------ */
+static void cpml_apply(orc_engine *e, cpml_t *s, int x0, int x1) {
+    const int ny = e->n[1], nz = e->n[2];
+    const size_t p = e->plane;
+    const int es = s->e_side;
+    float **tgt = es ? e->u : e->i;
+    float **src = es ? e->i : e->u;
+    const float scale = es ? 1.0f : -1.0f;
+    const int lane = es ? 3 + s->comp : 9 + s->comp;
+    const int len = s->hi[2] - s->lo[2], rows = s->hi[1] - s->lo[1];
+    int a0 = s->lo[0] > x0 ? s->lo[0] : x0;
+    int a1 = s->hi[0] < x1 ? s->hi[0] : x1;
+    for (int i = a0; i < a1; i++) {
+        const float *cur = src[s->other] + (size_t)i * p;
+        const float *oth = cur;
+        if (s->normal == 0) oth = es ? (i == 0 ? e->zero : cur - p) : cur + p;
+        float *t = tgt[s->comp] + (size_t)i * p;
+        for (int j = s->lo[1]; j < s->hi[1]; j++) {
+            const float *cf = row(e, lane, i, j);
+            size_t q = ((size_t)(i - s->lo[0]) * rows + (j - s->lo[1])) * len;
+            for (int k = s->lo[2]; k < s->hi[2]; k++, q++) {
+                size_t o = (size_t)j * nz + k;
+                float h, l;
+                int z;
+                switch (s->normal) {
+                case 0:
+                    h = es ? cur[o] : oth[o];
+                    l = es ? oth[o] : cur[o];
+                    z = i - s->lo[0];
+                    break;
+                case 1:
+                    h = es ? cur[o] : cur[o + nz];
+                    l = es ? (j == 0 ? 0.0f : cur[o - nz]) : cur[o];
+                    z = j - s->lo[1];
+                    break;
+                default:
+                    h = es ? cur[o] : cur[o + 1];
+                    l = es ? (k == 0 ? 0.0f : cur[o - 1]) : cur[o];
+                    z = k - s->lo[2];
+                    break;
+                }
+                float sv = s->sign * (h - l);
+                float ps = flush(s->b[z] * s->psi[q] + s->a[z] * sv);
+                s->psi[q] = ps;
+                t[o] = flush(t[o] + scale * cf[k] * (s->k[z] * sv + ps));
+            }
+        }
+    }
+    (void)ny;
+}
+
+/* ------------------------------------------------------------------ */
+/* slab-parallel phases (pool.rs:314-403)                               */
+/* ------------------------------------------------------------------ */
+static void phase(orc_engine *e, int cmd, int s) {
+    int p0 = e->slab_start[s], p1 = e->slab_start[s + 1];
+    if (cmd == 1) {
+        for (int gi = p0; gi < p1; gi++) {
+            double w = h_plane(e, gi, e->energy_flag, 1);
+            if (e->energy_flag) e->plane_energy[gi] = w;
+        }
+        for (int c = 0; c < e->ncpml; c++)
+            if (!e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], p0, p1);
+    } else if (cmd == 2) {
+        for (int gi = p0; gi < p1; gi++) e_plane(e, gi);
+        for (int c = 0; c < e->ncpml; c++)
+            if (e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], p0, p1);
+    } else if (cmd == 3) {
+        for (int gi = p0; gi < p1; gi++) e->plane_energy[gi] = h_plane(e, gi, 1, 0);
+    }
+}
+
+typedef struct {
+    orc_engine *e;
+    int s;
+} warg_t;
+
+static void *worker(void *arg) {
+    warg_t *w = (warg_t *)arg;
+    orc_engine *e = w->e;
+    int s = w->s;
+    free(w);
+    for (;;) {
+        pthread_barrier_wait(&e->bar);
+        int cmd = e->cmd;
+        if (cmd == 9) break;
+        phase(e, cmd, s);
+        pthread_barrier_wait(&e->bar);
+    }
+    return NULL;
+}
+
+static void parallel(orc_engine *e, int cmd) {
+    if (e->threads <= 1) {
+        phase(e, cmd, 0);
+        return;
+    }
+    e->cmd = cmd;
+    pthread_barrier_wait(&e->bar);
+    phase(e, cmd, 0);
+    pthread_barrier_wait(&e->bar);
+}
+
+/* ------------------------------------------------------------------ */
+/* serial hooks (engine/mod.rs:833-992 gap schedule; excite.rs,         */
+/* probe.rs)                                                            */
+/* ------------------------------------------------------------------ */
+static float op_amp(const op_t *o, uint64_t n) { return n < o->namp ? o->amp[n] : 0.0f; }
+
+/* EdgeSource::apply (excite.rs:104-115) */
+static void source_apply(orc_engine *e, op_t *o, uint64_t n) {
+    float a = op_amp(o, n);
+    float **f = (o->kind == 2 || o->kind == 3) ? e->i : e->u;
+    int hard = o->kind == 1 || o->kind == 3;
+    for (int q = 0; q < o->n; q++) {
+        float *x = f[o->axis[q]] + o->idx[q];
+        if (hard)
+            *x = o->wf[q] * a;
+        else
+            *x += o->wf[q] * a;
+    }
+}
+
+/* Probe::sample (probe.rs:364-414): f64 sum in entry order. Rust's f64
+ * Sum folds from -0.0. */
+static void probe_sample(orc_engine *e, op_t *o) {
+    float **f = o->quantity == 1 ? e->i : e->u;
+    double v = -0.0;
+    for (int q = 0; q < o->n; q++) v += o->wd[q] * (double)f[o->axis[q]][o->idx[q]];
+    if (o->nvals == o->cap) {
+        o->cap = o->cap ? 2 * o->cap : 1024;
+        o->vals = (double *)realloc(o->vals, o->cap * sizeof(double));
+    }
+    o->vals[o->nvals++] = v;
+}
+
+static int is_hsrc(const op_t *o) { return o->type == OP_SOURCE && (o->kind == 2 || o->kind == 3); }
+static int is_esrc(const op_t *o) { return o->type == OP_SOURCE && (o->kind == 0 || o->kind == 1); }
+static int is_iprobe(const op_t *o) { return o->type == OP_PROBE && o->quantity == 1; }
+static int is_vprobe(const op_t *o) { return o->type == OP_PROBE && o->quantity == 0; }
+
+/* hooks_at of each operator (excite.rs:129-140, probe.rs:495-500). */
+static int needs(const op_t *o, uint64_t n) {
+    if (o->type == OP_SOURCE) {
+        if (o->kind == 1 || o->kind == 3) return 1;
+        return op_amp(o, n) != 0.0f;
+    }
+    return n % o->interval == 0;
+}
+
+/* One step: H phase, mid gap, E phase, post-E gap, with the gap rules of
+ * Engine::advance (mod.rs:873-915) for these operator kinds. */
+static void one_step(orc_engine *e, int energy) {
+    uint64_t n = e->step;
+    int need_post_e = 0, mid = 0, mid_movable = 1;
+    for (int q = 0; q < e->nops; q++) {
+        op_t *o = &e->ops[q];
+        if (!needs(o, n)) continue;
+        if (is_esrc(o) || is_vprobe(o)) need_post_e = 1;
+        if (is_hsrc(o)) mid = 1, mid_movable = 0;
+        if (is_iprobe(o)) mid = 1;
+    }
+    int gap_b = mid && !mid_movable;
+    int moved = mid && mid_movable;
+    int gap_a = need_post_e || moved;
+
+    e->energy_flag = energy;
+    parallel(e, 1);
+    e->energy_flag = 0;
+    if (gap_b) {
+        for (int q = 0; q < e->nops; q++) {
+            op_t *o = &e->ops[q];
+            if (is_hsrc(o)) source_apply(e, o, n);
+            if (is_iprobe(o) && n % o->interval == 0) probe_sample(e, o);
+        }
+    }
+    parallel(e, 2);
+    if (gap_a) {
+        if (moved)
+            for (int q = 0; q < e->nops; q++) {
+                op_t *o = &e->ops[q];
+                if (is_iprobe(o) && n % o->interval == 0) probe_sample(e, o);
+            }
+        for (int q = 0; q < e->nops; q++) {
+            op_t *o = &e->ops[q];
+            if (is_esrc(o)) source_apply(e, o, n);
+            if (is_vprobe(o) && n % o->interval == 0) probe_sample(e, o);
+        }
+    }
+    e->step++;
+}
+
+/* plane sums in plane order (pool.rs:673-692): Rust's f64 Sum folds from
+ * -0.0; the invariant is scaled by dt/4. */
+static double plane_sum(orc_engine *e) {
+    double s = -0.0;
+    for (int gi = 0; gi < e->n[0]; gi++) s += e->plane_energy[gi];
+    return 0.25 * e->dt * s;
+}
+
+/* ------------------------------------------------------------------ */
+/* public API                                                           */
+/* ------------------------------------------------------------------ */
+orc_engine *orc_create(int nx, int ny, int nz, double dt, int curl_order, int threads) {
+    if (nx < 2 || ny < 2 || nz < 2 || !(dt > 0.0)) return NULL;
+    orc_engine *e = (orc_engine *)calloc(1, sizeof(orc_engine));
+    e->n[0] = nx, e->n[1] = ny, e->n[2] = nz;
+    e->plane = (size_t)ny * nz;
+    e->N = (size_t)nx * e->plane;
+    e->dt = dt;
+    e->oe = curl_order == 1;
+    for (int a = 0; a < 3; a++) {
+        e->u[a] = (float *)calloc(e->N, sizeof(float));
+        e->i[a] = (float *)calloc(e->N, sizeof(float));
+    }
+    e->zero = (float *)calloc(e->plane, sizeof(float));
+    e->plane_energy = (double *)calloc(nx, sizeof(double));
+    /* Layout::new (fields.rs:38-59) */
+    if (threads < 1) threads = 1;
+    if (threads > nx) threads = nx;
+    e->threads = threads;
+    e->slab_start = (int *)calloc(threads + 1, sizeof(int));
+    int q = nx / threads, r = nx % threads, p = 0;
+    for (int s = 0; s < threads; s++) {
+        e->slab_start[s] = p;
+        p += q + (s < r ? 1 : 0);
+    }
+    e->slab_start[threads] = nx;
+    if (threads > 1) {
+        pthread_barrier_init(&e->bar, NULL, threads);
+        e->workers = (pthread_t *)calloc(threads, sizeof(pthread_t));
+        for (int s = 1; s < threads; s++) {
+            warg_t *w = (warg_t *)malloc(sizeof(warg_t));
+            w->e = e, w->s = s;
+            pthread_create(&e->workers[s], NULL, worker, w);
+        }
+    }
+    return e;
+}
+
+void orc_destroy(orc_engine *e) {
+    if (!e) return;
+    if (e->threads > 1) {
+        e->cmd = 9;
+        pthread_barrier_wait(&e->bar);
+        for (int s = 1; s < e->threads; s++) pthread_join(e->workers[s], NULL);
+        pthread_barrier_destroy(&e->bar);
+        free(e->workers);
+    }
+    for (int a = 0; a < 3; a++) free(e->u[a]), free(e->i[a]);
+    for (int l = 0; l < 12; l++) free(e->lanes[l].rows), free(e->lanes[l].index);
+    for (int c = 0; c < e->ncpml; c++) {
+        cpml_t *s = &e->cpml[c];
+        free(s->b), free(s->a), free(s->k), free(s->psi);
+    }
+    free(e->cpml);
+    for (int q = 0; q < e->nops; q++) {
+        op_t *o = &e->ops[q];
+        free(o->axis), free(o->idx), free(o->wf), free(o->amp), free(o->wd), free(o->vals);
+    }
+    free(e->ops);
+    free(e->zero), free(e->plane_energy), free(e->slab_start);
+    free(e);
+}
+
+const char *orc_last_error(orc_engine *e) { return e ? e->err : "null engine"; }
+
+int orc_set_lane(orc_engine *e, int lane, const float *rows, uint64_t nrows,
+                 const uint32_t *index) {
+    if (lane < 0 || lane >= 12) return -1;
+    size_t nr = (size_t)e->n[0] * e->n[1];
+    if (!index && nrows != nr) {
+        snprintf(e->err, sizeof e->err, "dense lane needs %zu rows", nr);
+        return -1;
+    }
+    lane_t *L = &e->lanes[lane];
+    free(L->rows), free(L->index);
+    L->rows = (float *)malloc(nrows * e->n[2] * sizeof(float));
+    memcpy(L->rows, rows, nrows * e->n[2] * sizeof(float));
+    L->nrows = nrows;
+    L->index = NULL;
+    if (index) {
+        L->index = (uint32_t *)malloc(nr * sizeof(uint32_t));
+        memcpy(L->index, index, nr * sizeof(uint32_t));
+        for (size_t q = 0; q < nr; q++)
+            if (index[q] >= nrows) {
+                snprintf(e->err, sizeof e->err, "row index out of range");
+                return -1;
+            }
+    }
+    return 0;
+}
+
+int orc_add_cpml(orc_engine *e, int e_side, int comp, int normal, const int lo[3],
+                 const int hi[3], const float *b, const float *a, const float *k) {
+    if (comp == normal || comp < 0 || comp > 2 || normal < 0 || normal > 2) return -1;
+    for (int d = 0; d < 3; d++)
+        if (lo[d] < 0 || hi[d] > e->n[d] || lo[d] > hi[d]) return -1;
+    e->cpml = (cpml_t *)realloc(e->cpml, (e->ncpml + 1) * sizeof(cpml_t));
+    cpml_t *s = &e->cpml[e->ncpml++];
+    memset(s, 0, sizeof *s);
+    s->e_side = e_side;
+    s->comp = comp;
+    s->normal = normal;
+    /* Slab::new (cpml.rs:415-437) */
+    if (normal == axis_next(comp)) {
+        s->sign = 1.0f;
+        s->other = axis_prev(comp);
+    } else {
+        s->sign = -1.0f;
+        s->other = axis_next(comp);
+    }
+    memcpy(s->lo, lo, sizeof s->lo);
+    memcpy(s->hi, hi, sizeof s->hi);
+    int depth = hi[normal] - lo[normal];
+    s->b = (float *)malloc((depth + 1) * sizeof(float));
+    s->a = (float *)malloc((depth + 1) * sizeof(float));
+    s->k = (float *)malloc((depth + 1) * sizeof(float));
+    memcpy(s->b, b, depth * sizeof(float));
+    memcpy(s->a, a, depth * sizeof(float));
+    memcpy(s->k, k, depth * sizeof(float));
+    size_t len = (size_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    s->psi = (float *)calloc(len ? len : 1, sizeof(float));
+    return e->ncpml - 1;
+}
+
+static int add_op(orc_engine *e) {
+    e->ops = (op_t *)realloc(e->ops, (e->nops + 1) * sizeof(op_t));
+    memset(&e->ops[e->nops], 0, sizeof(op_t));
+    return e->nops++;
+}
+
+static int node_ok(const orc_engine *e, const int *nd) {
+    return nd[0] >= 0 && nd[0] < e->n[0] && nd[1] >= 0 && nd[1] < e->n[1] && nd[2] >= 0 &&
+           nd[2] < e->n[2];
+}
+
+int orc_add_source(orc_engine *e, int kind, int n_edges, const int *axis, const int *nodes,
+                   const float *w, const float *amp, uint64_t n_amp) {
+    if (kind < 0 || kind > 3) return -1;
+    for (int q = 0; q < n_edges; q++)
+        if (!node_ok(e, nodes + 3 * q) || axis[q] < 0 || axis[q] > 2) return -1;
+    int id = add_op(e);
+    op_t *o = &e->ops[id];
+    o->type = OP_SOURCE;
+    o->kind = kind;
+    o->n = n_edges;
+    o->axis = (int *)malloc(n_edges * sizeof(int) + 1);
+    o->idx = (size_t *)malloc(n_edges * sizeof(size_t) + 1);
+    o->wf = (float *)malloc(n_edges * sizeof(float) + 1);
+    for (int q = 0; q < n_edges; q++) {
+        o->axis[q] = axis[q];
+        o->idx[q] = ((size_t)nodes[3 * q] * e->n[1] + nodes[3 * q + 1]) * e->n[2] + nodes[3 * q + 2];
+        o->wf[q] = w[q];
+    }
+    o->amp = (float *)malloc(n_amp * sizeof(float) + 1);
+    memcpy(o->amp, amp, n_amp * sizeof(float));
+    o->namp = n_amp;
+    return id;
+}
+
+int orc_add_probe(orc_engine *e, int quantity, int n_entries, const int *axis, const int *nodes,
+                  const double *w, uint64_t interval) {
+    for (int q = 0; q < n_entries; q++)
+        if (!node_ok(e, nodes + 3 * q) || axis[q] < 0 || axis[q] > 2) return -1;
+    int id = add_op(e);
+    op_t *o = &e->ops[id];
+    o->type = OP_PROBE;
+    o->quantity = quantity;
+    o->n = n_entries;
+    o->interval = interval ? interval : 1;
+    o->axis = (int *)malloc(n_entries * sizeof(int) + 1);
+    o->idx = (size_t *)malloc(n_entries * sizeof(size_t) + 1);
+    o->wd = (double *)malloc(n_entries * sizeof(double) + 1);
+    for (int q = 0; q < n_entries; q++) {
+        o->axis[q] = axis[q];
+        o->idx[q] = ((size_t)nodes[3 * q] * e->n[1] + nodes[3 * q + 1]) * e->n[2] + nodes[3 * q + 2];
+        o->wd[q] = w[q];
+    }
+    return id;
+}
+
+void orc_run(orc_engine *e, uint64_t n) {
+    for (uint64_t t = 0; t < n; t++) one_step(e, 0);
+}
+
+/* Engine::step_with_energy (mod.rs:819-823): operator energies (all zero
+ * for these kinds; Sum folds from -0.0) plus the in-step invariant. */
+double orc_step_with_energy(orc_engine *e) {
+    double w_ops = -0.0;
+    for (int q = 0; q < e->nops; q++) w_ops += 0.0;
+    one_step(e, 1);
+    return plane_sum(e) + w_ops;
+}
+
+/* Engine::energy (mod.rs:722-726): read-only evaluation of the state. */
+double orc_energy(orc_engine *e) {
+    parallel(e, 3);
+    double w_ops = -0.0;
+    for (int q = 0; q < e->nops; q++) w_ops += 0.0;
+    return plane_sum(e) + w_ops;
+}
+
+uint64_t orc_timestep(orc_engine *e) { return e->step; }
+
+int orc_read(orc_engine *e, int field, int axis, float *out) {
+    if (axis < 0 || axis > 2) return -1;
+    memcpy(out, (field ? e->i : e->u)[axis], e->N * sizeof(float));
+    return 0;
+}
+
+int orc_write(orc_engine *e, int field, int axis, const float *in) {
+    if (axis < 0 || axis > 2) return -1;
+    memcpy((field ? e->i : e->u)[axis], in, e->N * sizeof(float));
+    return 0;
+}
+
+uint64_t orc_probe_count(orc_engine *e, int id) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_PROBE) return 0;
+    return e->ops[id].nvals;
+}
+
+uint64_t orc_probe_values(orc_engine *e, int id, double *out, uint64_t cap) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_PROBE) return 0;
+    op_t *o = &e->ops[id];
+    uint64_t n = o->nvals < cap ? o->nvals : cap;
+    memcpy(out, o->vals, n * sizeof(double));
+    return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* setup restatements                                                   */
+/* ------------------------------------------------------------------ */
+
+/* Grid::delta / dual_delta (grid.rs:562-597) */
+static double delta(const double *l, int i) { return l[i + 1] - l[i]; }
+static double dual_delta(const double *l, int n, int i) {
+    double lo = i > 0 ? delta(l, i - 1) : 0.0;
+    double hi = i + 1 < n ? delta(l, i) : 0.0;
+    return 0.5 * (lo + hi);
+}
+static double min_delta(const double *l, int n) {
+    double m = INFINITY;
+    for (int i = 0; i + 1 < n; i++) {
+        double d = l[i + 1] - l[i];
+        m = d < m ? d : m; /* f64::min */
+    }
+    return m;
+}
+
+double orc_courant_limit(const double *x, int nx, const double *y, int ny, const double *z,
+                         int nz) {
+    double d[3] = {min_delta(x, nx), min_delta(y, ny), min_delta(z, nz)};
+    /* Iterator::sum over the three terms: folds from -0.0 */
+    double s = -0.0;
+    for (int a = 0; a < 3; a++) s += 1.0 / (d[a] * d[a]);
+    return 1.0 / (C0 * sqrt(s));
+}
+
+int orc_coefficients(const double *x, int nx, const double *y, int ny, const double *z,
+                     int nz, const double *eps_r, const double *kappa, const double *mu_r,
+                     const double *sigma_m, const uint8_t *pec, const int pec_face[6],
+                     double dt, float *lanes) {
+    const double *L[3] = {x, y, z};
+    const int n[3] = {nx, ny, nz};
+    const size_t N = (size_t)nx * ny * nz;
+    memset(lanes, 0, 12 * N * sizeof(float));
+    for (int a = 0; a < 3; a++) {
+        int b = (a + 1) % 3, c = (a + 2) % 3;
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++)
+                for (int k = 0; k < nz; k++) {
+                    int p[3] = {i, j, k};
+                    size_t idx = ((size_t)i * ny + j) * nz + k;
+                    size_t ai = (size_t)a * N + idx;
+                    if (p[a] + 1 < n[a]) {
+                        int on_face = (p[b] == 0 && pec_face[2 * b]) ||
+                                      (p[b] == n[b] - 1 && pec_face[2 * b + 1]) ||
+                                      (p[c] == 0 && pec_face[2 * c]) ||
+                                      (p[c] == n[c] - 1 && pec_face[2 * c + 1]);
+                        int pm = pec ? pec[ai] : 0;
+                        if (!(on_face || pm)) {
+                            /* e_edge_geometry (coeffs.rs:766-774) */
+                            double geo = dual_delta(L[b], n[b], p[b]) * dual_delta(L[c], n[c], p[c]) /
+                                         delta(L[a], p[a]);
+                            double cap = EPS0 * eps_r[ai] * geo;
+                            double g = kappa[ai] * geo;
+                            /* e_coefficients (coeffs.rs:817-820) */
+                            double den = cap + 0.5 * g * dt;
+                            lanes[(size_t)a * N + idx] = (float)((cap - 0.5 * g * dt) / den);
+                            lanes[(size_t)(3 + a) * N + idx] = (float)(dt / den);
+                        }
+                    }
+                    if (p[b] + 1 < n[b] && p[c] + 1 < n[c]) {
+                        double geo = delta(L[b], p[b]) * delta(L[c], p[c]) / dual_delta(L[a], n[a], p[a]);
+                        double l = MU0 * mu_r[ai] * geo;
+                        double r = sigma_m[ai] * geo;
+                        double den = l + 0.5 * r * dt;
+                        lanes[(size_t)(6 + a) * N + idx] = (float)((l - 0.5 * r * dt) / den);
+                        lanes[(size_t)(9 + a) * N + idx] = (float)(dt / den);
+                    }
+                }
+    }
+    return 0;
+}
+
+void orc_cpml_coefficients(double m, double sigma_factor, double kappa_max, double alpha_max,
+                           int convolution, double rho, double delta_, double dt, double n,
+                           double out[7]) {
+    double g = pow(rho, m);
+    /* Cpml::sigma_max (fdtd.rs:266-271) with eta = n*Z0 */
+    double eta = n * Z0;
+    double sigma = sigma_factor * 0.8 * (m + 1.0) / (eta * delta_) * g;
+    double kappa = 1.0 + (kappa_max - 1.0) * g;
+    double alpha = alpha_max * (1.0 - rho);
+    double b = exp(-(sigma / kappa + alpha) * dt / EPS0);
+    double den = sigma * kappa + kappa * kappa * alpha;
+    double x = (sigma / kappa + alpha) * dt / EPS0;
+    double a, shift;
+    if (!(sigma > 0.0 && den > 0.0)) {
+        a = 0.0, shift = 0.0;
+    } else if (convolution == 1 && x <= 8.0) {
+        double cdt = sigma * dt / (EPS0 * kappa * kappa);
+        double gg = x < 1e-3 ? 0.5 - x / 3.0 + x * x / 8.0 : (-expm1(-x) - x * b) / (x * x);
+        double a1 = -cdt * gg;
+        double a0 = cdt * expm1(-x) / x - a1;
+        a = a0 + a1 / b, shift = a1 / b;
+    } else {
+        a = sigma * (b - 1.0) / den, shift = 0.0;
+    }
+    out[0] = rho, out[1] = sigma, out[2] = kappa, out[3] = alpha, out[4] = b, out[5] = a;
+    out[6] = 1.0 / kappa - 1.0 - shift;
+}
+
+/* run::drive (run.rs:424-540) without settle, over a scripted stepper
+ * (run.rs:548-582: dt = 1, energy w[min(n, len-1)] at timestep n). */
+int orc_drive_script(const double *w, uint64_t nw, uint64_t max, int has_criteria,
+                     double energy_ratio, uint64_t interval, int has_excitation_end,
+                     double excitation_end, uint64_t *steps_out) {
+    uint64_t steps = 0, n = 0;
+    double wv = 0.0, w_max = 0.0;
+    int reason;
+    for (;;) {
+        if (steps >= max) {
+            reason = 1;
+            break;
+        }
+        uint64_t tn = n;
+        wv = w[tn < nw ? tn : nw - 1];
+        n += 1;
+        steps += 1;
+        w_max = fmax(w_max, wv);
+        double ratio = w_max > 0.0 ? wv / w_max : 1.0;
+        double t = (double)tn * 1.0;
+        if (!isfinite(wv) || wv < -1e-12 * w_max) {
+            reason = 3;
+            break;
+        }
+        if (has_criteria && (!has_excitation_end || t > excitation_end) && w_max > 0.0 &&
+            ratio < energy_ratio) {
+            reason = 0;
+            break;
+        }
+        uint64_t chunk = interval - 1 < max - steps ? interval - 1 : max - steps;
+        n += chunk;
+        steps += chunk;
+    }
+    *steps_out = steps;
+    return reason;
+}

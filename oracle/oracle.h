@@ -1,0 +1,111 @@
+/*
+ * oracle.h -- CPU restatement of the ems FDTD engine step (TEST INFRASTRUCTURE).
+ *
+ * This library is the parity oracle and the CPU baseline. It is NOT part of
+ * the product: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it, always as the checker or the timed CPU
+ * reference, never as the thing measured or shipped.
+ *
+ * Every function restates the reference algorithm with the same f32/f64
+ * split and the same expression order (compiled with -ffp-contract=off, no
+ * fast-math), citing the reference file:line it follows. The engine state
+ * is built from the same arrays the GPU boundary (include/ems_gpu.h)
+ * receives, so both consume identical compiled operators.
+ *
+ * Pinning (see the test files named in each comment):
+ *   - kernel (A1/A2): closed forms -- PEC cavity modes, Yee dispersion,
+ *     lossy decay rate, invariant drift (tests/test_oracle_physics.py);
+ *   - energy order (A3): determinism over thread counts + the in-step value
+ *     equals the read-only value bit for bit (tests/test_oracle_engine.py);
+ *   - coefficients / Courant (A5): the reference's unit-test numbers
+ *     (tests/test_oracle_setup.py);
+ *   - CPML coefficients (A6): exact grading/convolution ratios of the
+ *     reference unit tests; CPML apply (A7): reflection levels;
+ *   - run loop (A12): the scripted energy sequences of run.rs tests.
+ */
+#ifndef EMS_ORACLE_H
+#define EMS_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_engine orc_engine;
+
+/* Engine over nx*ny*nz nodes, layout (i*ny+j)*nz+k.  curl_order: 0 Paired,
+ * 1 OpenEms (kernel.rs:81-107).  threads >= 1: x-slab worker threads
+ * (fields.rs:38-59). */
+orc_engine *orc_create(int nx, int ny, int nz, double dt, int curl_order, int threads);
+void orc_destroy(orc_engine *e);
+const char *orc_last_error(orc_engine *e);
+
+/* Coefficient lane (ca 0-2, cb 3-5, da 6-8, db 9-11): `nrows` rows of nz
+ * values and a row index per (i,j) (NULL = dense, nrows = nx*ny). */
+int orc_set_lane(orc_engine *e, int lane, const float *rows, uint64_t nrows,
+                 const uint32_t *index);
+
+/* One CPML slab (cpml.rs:394-420): side 1 = E (corrects u from i), 0 = H.
+ * Slabs are applied in the order added, E slabs after the E kernel, H
+ * slabs after the H kernel (cpml.rs:815-850). b/a/k per depth index
+ * node[normal]-lo[normal]. */
+int orc_add_cpml(orc_engine *e, int e_side, int comp, int normal, const int lo[3],
+                 const int hi[3], const float *b, const float *a, const float *k);
+
+/* Edge source (excite.rs:10-115): kind 0 SoftE, 1 HardE, 2 SoftH, 3 HardH.
+ * amp[n] = (a(t_n) as f32) for the step n's sample time; beyond n_amp the
+ * amplitude is 0. Returns operator id. */
+int orc_add_source(orc_engine *e, int kind, int n_edges, const int *axis,
+                   const int *nodes, const float *w, const float *amp, uint64_t n_amp);
+
+/* Probe (probe.rs:364-414): quantity 0 voltage (u, after E), 1 current (i,
+ * after H). Returns operator id. */
+int orc_add_probe(orc_engine *e, int quantity, int n_entries, const int *axis,
+                  const int *nodes, const double *w, uint64_t interval);
+
+void orc_run(orc_engine *e, uint64_t n);
+double orc_step_with_energy(orc_engine *e);
+double orc_energy(orc_engine *e);
+uint64_t orc_timestep(orc_engine *e);
+
+/* field 0 = u (E voltages), 1 = i (H currents). */
+int orc_read(orc_engine *e, int field, int axis, float *out);
+int orc_write(orc_engine *e, int field, int axis, const float *in);
+uint64_t orc_probe_count(orc_engine *e, int id);
+uint64_t orc_probe_values(orc_engine *e, int id, double *out, uint64_t cap);
+
+/* ---- setup restatements, for cross-checking the product's host side ---- */
+
+/* courant_limit (coeffs.rs:249-255) for per-axis line arrays. */
+double orc_courant_limit(const double *x, int nx, const double *y, int ny, const double *z,
+                         int nz);
+
+/* Coefficients::from_edges_rounded (coeffs.rs:879-966) with Exact rounding,
+ * Cartesian grid: per-edge eps_r/kappa (E, 3 arrays each) and mu_r/sigma_m
+ * (H), pec mask per E axis (may be NULL), pec_face[6] (x_min..z_max).
+ * Writes 12 dense lanes of N floats into `lanes` (12*N). */
+int orc_coefficients(const double *x, int nx, const double *y, int ny, const double *z,
+                     int nz, const double *eps_r, const double *kappa, const double *mu_r,
+                     const double *sigma_m, const uint8_t *pec, const int pec_face[6],
+                     double dt, float *lanes);
+
+/* CpmlCoefficients::in_medium (cpml.rs:350-388): out = [rho,sigma,kappa,
+ * alpha,b,a,direct]. convolution 0 Constant, 1 Linear. */
+void orc_cpml_coefficients(double m, double sigma_factor, double kappa_max, double alpha_max,
+                           int convolution, double rho, double delta, double dt, double n,
+                           double out[7]);
+
+/* run::drive (run.rs:424-540) over a scripted stepper whose energy at
+ * timestep n is w[min(n, nw-1)] (dt = 1): the run-loop restatement used to
+ * pin the end-criterion logic. Returns the stop reason (0 EnergyDecay,
+ * 1 MaxTimesteps, 2 Aborted, 3 Diverged) and the steps in *steps. */
+int orc_drive_script(const double *w, uint64_t nw, uint64_t max_steps, int has_criteria,
+                     double energy_ratio, uint64_t interval, int has_excitation_end,
+                     double excitation_end, uint64_t *steps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

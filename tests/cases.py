@@ -1,0 +1,156 @@
+"""Models of the reference's own engine tests (tests/engine.rs), built for any
+engine (device or oracle) from the same grid, coefficients and operators."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+import ems_amd as E
+from ems_amd import engine as EE
+
+DIMS = (0.10, 0.045, 0.07)
+
+
+def pulse(f0: float, bw: float):
+    """tests/engine.rs:15-23: Gaussian-modulated pulse and its end 2*t0."""
+    tau = 1.0 / (math.pi * bw)
+    t0 = 4.0 * tau
+
+    def f(t):
+        x = (t - t0) / tau
+        return math.exp(-x * x) * math.cos(2.0 * math.pi * f0 * (t - t0))
+
+    return f, 2.0 * t0
+
+
+def graded(length: float, cells: int, alpha: float):
+    """tests/engine.rs:72-81: sinusoidally graded lines."""
+    return [length * (q / cells - alpha * math.sin(2.0 * math.pi * q / cells) / (2.0 * math.pi))
+            for q in range(cells + 1)]
+
+
+def make(kind: str, grid, mats=None, faces=None, ts=None, threads=1, curl_order=0, pool=True):
+    faces = faces or [E.FaceCondition.Pec] * 6
+    ts = ts or E.TimeStep.courant(0.99)
+    dt = ts.resolve(grid)
+    lanes = E.coefficients(grid, mats, faces, dt)
+    if kind == "gpu":
+        return E.Engine(grid, dt, lanes, curl_order=curl_order, pool=pool)
+    from oracle.bridge import OracleEngine
+    return OracleEngine(grid, dt, lanes, curl_order=curl_order, threads=threads, pool=pool)
+
+
+def nearest(grid, axis, x):
+    l = grid.lines[axis]
+    return int(np.argmin(np.abs(l - x)))
+
+
+def cavity(kind, grid, mats, steps, scale, threads=4, horizon=None):
+    """run_cavity (tests/engine.rs:92-128)."""
+    src = [nearest(grid, 0, 0.3 * DIMS[0]), nearest(grid, 1, 0.3 * DIMS[1]),
+           nearest(grid, 2, 0.4 * DIMS[2])]
+    prb = [nearest(grid, 0, 0.7 * DIMS[0]), nearest(grid, 1, 0.5 * DIMS[1]),
+           nearest(grid, 2, 0.6 * DIMS[2])]
+    e = make(kind, grid, mats, threads=threads)
+    f, t_end = pulse(3.2e9 * scale, 2.5e9 * scale)
+    e.add_operator(E.EdgeSource(f, horizon=horizon or steps).edge(E.Axis.Y, src, 1.0))
+    p = e.add_operator(E.Probe.voltage(E.Axis.Y, prb))
+    e.run(steps)
+    return e, p, t_end
+
+
+def small_cavity(kind, mats_fn=None, threads=2, horizon=20000):
+    """small_cavity (tests/engine.rs:196-211)."""
+    g = E.Grid.uniform([10, 8, 6], [0.01, 0.009, 0.011])
+    m = mats_fn(g) if mats_fn else None
+    e = make(kind, g, m, threads=threads)
+    f, t_end = pulse(3e9, 1e9)
+    e.add_operator(E.EdgeSource(f, horizon=horizon).edge(E.Axis.Z, [3, 2, 2], 1.0)
+                   .edge(E.Axis.X, [6, 5, 3], -0.5))
+    return e, t_end
+
+
+def det_model(kind, threads=1, faces=None, steps=300):
+    """run_det (tests/engine.rs:497-531): graded, lossy dielectric, magnetic
+    loss, PEC box, PMC faces, soft source, current probe every 3 steps."""
+    g = E.Grid([graded(0.1, 13, 0.2), graded(0.08, 9, 0.1), graded(0.06, 7, 0.0)])
+    m = E.CellMaterials.vacuum(g)
+    m.fill_box([(2, 6), (1, 5), (2, 6)], E.Material.dielectric(3.0).with_kappa(0.02))
+    m.fill_box([(7, 12), (0, 9), (0, 3)], E.Material.vacuum().with_mu_r(2.0).with_sigma_m(50.0))
+    m.fill_pec_box([(9, 10), (2, 7), (1, 6)])
+    if faces is None:
+        faces = [E.FaceCondition.Pec] * 6
+        faces[E.Face.XMax] = E.FaceCondition.Pmc
+        faces[E.Face.ZMin] = E.FaceCondition.Pmc
+    e = make(kind, g, m, faces=faces, ts=E.TimeStep.courant(0.95), threads=threads)
+    f, _ = pulse(5e9, 4e9)
+    e.add_operator(E.EdgeSource(f, horizon=steps + 10).edge(E.Axis.Z, [4, 4, 3], 1.0))
+    p = e.add_operator(E.Probe.current(E.Axis.X, [5, 3, 3]).interval(3))
+    return e, p
+
+
+def spectrum(times, values, t_start, f):
+    """tests/engine.rs:26-39: |DFT| of the Hann-windowed tail."""
+    first = int(np.argmax(times >= t_start))
+    t, v = times[first:], values[first:]
+    n = float(t.size)
+    q = np.arange(t.size)
+    w = 0.5 - 0.5 * np.cos(2.0 * math.pi * q / (n - 1.0))
+    ph = -2.0 * math.pi * f * t
+    return math.hypot(float(np.sum(w * v * np.cos(ph))), float(np.sum(w * v * np.sin(ph))))
+
+
+def peak(times, values, t_start, lo, hi):
+    """tests/engine.rs:43-65: coarse scan then golden-section search."""
+    s = lambda f: spectrum(times, values, t_start, f)
+    n = 200
+    step = (hi - lo) / n
+    vals = [s(lo + q * step) for q in range(n + 1)]
+    best = int(np.argmax(vals))
+    a, b = lo + (best - 1.0) * step, lo + (best + 1.0) * step
+    g = (math.sqrt(5.0) - 1.0) / 2.0
+    for _ in range(60):
+        c = b - g * (b - a)
+        d = a + g * (b - a)
+        if s(c) > s(d):
+            b = d
+        else:
+            a = c
+    f = 0.5 * (a + b)
+    return f, s(f)
+
+
+def cavity_freq(m, n, p, dims, eps_r):
+    return E.C0 / (2.0 * math.sqrt(eps_r)) * math.sqrt((m / dims[0]) ** 2 + (n / dims[1]) ** 2
+                                                       + (p / dims[2]) ** 2)
+
+
+def check_modes(times, values, t_start, eps_r, tol):
+    """check_modes (tests/engine.rs:130-154). Returns the relative errors."""
+    errs = []
+    for (m, n, p) in [(1.0, 0.0, 1.0), (2.0, 0.0, 1.0)]:
+        fa = cavity_freq(m, n, p, DIMS, eps_r)
+        f, mag = peak(times, values, t_start, 0.95 * fa, 1.05 * fa)
+        err = (f - fa) / fa
+        assert abs(err) < tol, f"mode ({m},{n},{p}) off by {err*100:.3f}%"
+        assert 0.96 * fa < f < 1.04 * fa
+        off = spectrum(times, values, t_start, 0.9 * fa)
+        assert mag > 20.0 * off, f"no clear peak for ({m},{n},{p})"
+        errs.append(err)
+    return errs
+
+
+def staggered_energy(e, dt):
+    """Engine::energy_staggered (mod.rs:786-806), in numpy (diagnostic)."""
+    lanes = e._dense_lanes
+    w = 0.0
+    for field, base in ((0, 0), (1, 6)):
+        for a in range(3):
+            x = (e.u_component(a) if field == 0 else e.i_component(a)).astype(np.float64)
+            ca = lanes[base + a].astype(np.float64)
+            cb = lanes[base + 3 + a].astype(np.float64)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                c = np.where(cb == 0.0, 0.0, dt * (1.0 + ca) / (2.0 * cb))
+            w += float(np.sum(0.5 * c * x * x))
+    return w
