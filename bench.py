@@ -87,6 +87,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=20)
+    ap.add_argument("--unfused", action="store_true", help="unfused schedule (H, CPML, E launches)")
+    ap.add_argument("--tile-rows", type=int, default=0, help="fused tile height (3, 7, 15)")
+    ap.add_argument("--x-chunks", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -110,9 +114,25 @@ def main() -> int:
 
     eng = E.Engine(wl.grid, wl.dt, wl.lanes)
     probes = wl.build_ops(eng)
-    eng.run(args.warmup)
+    stream_gbs = eng.copy_bandwidth(1 << 30, 10)
+    log(f"device copy bandwidth: {stream_gbs:.0f} GB/s (read + write)")
+    tuned = None
+    warm = args.warmup
+    if args.unfused:
+        eng.set_schedule(False)
+    elif args.tile_rows or args.no_autotune:
+        eng.set_schedule(True, args.tile_rows, args.x_chunks)
+    else:
+        # The autotuner's trial steps are part of the (untimed) warm-up.
+        t0 = eng.timestep()
+        best, trials = eng.autotune()
+        tuned = {"mode": best[0], "tile_rows": best[1], "x_chunks": best[2],
+                 "trials_ms": {f"m{m}_t{t}_c{c}": round(ms, 4) for m, t, c, ms in trials}}
+        log(f"autotune: mode {best[0]}, tile_rows {best[1]}, x_chunks {best[2]} ({best[3]:.3f} ms)")
+        warm = max(0, args.warmup - (eng.timestep() - t0))
+    eng.run(warm)
     eng.synchronize()
-    log(f"setup + warm-up done ({args.warmup} steps)")
+    log(f"setup + warm-up done ({eng.timestep()} steps)")
 
     t0 = time.perf_counter()
     eng.run(steps)
@@ -128,7 +148,9 @@ def main() -> int:
     # stream, algorithmic bytes from the lanes the engine holds.
     th, te, ts = eng.time_kernels(args.kernel_steps)
     hb, eb, cb = eng.traffic_model()
-    if th >= te:
+    if te == 0.0:
+        name, dur, byt = "step_fused", th, hb
+    elif th >= te:
         name, dur, byt = "h_update", th, hb
     else:
         name, dur, byt = "e_update", te, eb
@@ -136,8 +158,9 @@ def main() -> int:
     roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": byt, "launch_ms": round(dur, 4),
-            "step_ms_events": round(ts, 4)}
-    log(f"kernels: H {th:.3f} ms, E {te:.3f} ms, step {ts:.3f} ms; {name} {achieved:.0f} GB/s")
+            "step_ms_events": round(ts, 4), "device_copy_GBs": round(stream_gbs, 1),
+            "frac_of_copy": round(achieved / stream_gbs, 4)}
+    log(f"kernels: {name} {th:.3f} ms (E {te:.3f} ms), step {ts:.3f} ms; {achieved:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -153,7 +176,8 @@ def main() -> int:
         if args.workload == "cpml_box" else "synthetic (vacuum PEC box, Gaussian source)",
         "config": {"workload": wl.name, "cells": list(n), "nodes": list(wl.grid.nodes()),
                    "cpml_cells": 8 if args.workload == "cpml_box" else 0,
-                   "probes": len(probes), "parallelism": "single GPU"},
+                   "probes": len(probes), "parallelism": "single GPU",
+                   "schedule": "unfused" if args.unfused else "fused", "autotune": tuned},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

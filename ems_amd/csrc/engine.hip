@@ -238,6 +238,301 @@ __global__ void __launch_bounds__(256) cpml_slab(Fields f, Lane cfl, SlabArg s, 
   (void)nx;
 }
 
+// ---- fused step (H, CPML-H, E, CPML-E in one pass) ----
+//
+// Ping-pong buffers: reads step n from A, writes step n+1 to B, so tiles can
+// recompute their neighbours' H values without racing. A block owns a tile
+// of TJ rows (j) by tk nodes (k) and marches a chunk of x-planes; its
+// threads cover the tile plus one halo row (j0-1) and one halo column
+// (k0-1), whose H values (after the CPML correction) the E update of the
+// tile needs. H values of the plane go through LDS; the previous plane's
+// y and z currents stay in registers. Every value is the reference's
+// expression on the same inputs, so results equal the unfused schedule bit
+// for bit (the reference's fused schedule rests on the same fact,
+// pool.rs:475-487).
+
+// One CPML slab as the fused kernel reads it (cpml.rs:394-420). H-side psi
+// is double-buffered with the fields; E-side psi is updated in place (only
+// the owner of a node computes its E update).
+struct FSlab {
+  int e_side, comp, normal;
+  float sign;
+  int lo0, hi0, lo1, hi1, lo2, hi2;
+  int ly, lz;
+  const float *b, *a, *k;
+  float *psi0, *psi1;
+};
+
+// CPML corrections of one side at node (i, j, k), in registration order,
+// over the slabs whose bits are set in `mask` (the slabs that can reach the
+// block's tile). d<c><n> is the difference h - l of the derivative along
+// normal n used by component c (cpml.rs:495-521); cf<c> the cb (E) or db (H)
+// of component c. (Scalars and selects, not arrays, so nothing goes to
+// scratch memory.)
+template <bool ESIDE>
+__device__ __forceinline__ void cpml_node(const FSlab *__restrict__ slabs, uint32_t mask, int par,
+                                          bool write_psi, int i, int j, int k, float d01,
+                                          float d02, float d10, float d12, float d20, float d21,
+                                          float cf0, float cf1, float cf2, float &t0, float &t1,
+                                          float &t2) {
+  while (mask) {
+    const int q = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const FSlab &s = slabs[q];
+    if (i < s.lo0 || i >= s.hi0) continue;
+    if (j < s.lo1 || j >= s.hi1 || k < s.lo2 || k >= s.hi2) continue;
+    const int z = s.normal == 0 ? i - s.lo0 : (s.normal == 1 ? j - s.lo1 : k - s.lo2);
+    const size_t qq = ((size_t)(i - s.lo0) * s.ly + (j - s.lo1)) * s.lz + (k - s.lo2);
+    const int c = s.comp, n = s.normal;
+    float d = d01;
+    d = (c == 0 && n == 2) ? d02 : d;
+    d = (c == 1 && n == 0) ? d10 : d;
+    d = (c == 1 && n == 2) ? d12 : d;
+    d = (c == 2 && n == 0) ? d20 : d;
+    d = (c == 2 && n == 1) ? d21 : d;
+    const float cf = c == 0 ? cf0 : (c == 1 ? cf1 : cf2);
+    float t = c == 0 ? t0 : (c == 1 ? t1 : t2);
+    const float sv = s.sign * d;
+    const float *pin = (ESIDE || par == 0) ? s.psi0 : s.psi1;
+    float *pout = (ESIDE || par == 1) ? s.psi0 : s.psi1;
+    const float ps = flush(s.b[z] * pin[qq] + s.a[z] * sv);
+    if (write_psi) pout[qq] = ps;
+    const float scale = ESIDE ? 1.0f : -1.0f;
+    t = flush(t + scale * cf * (s.k[z] * sv + ps));
+    if (c == 0) t0 = t;
+    else if (c == 1) t1 = t;
+    else t2 = t;
+  }
+}
+
+// Loads/stores at a wave-uniform base plus a 32-bit per-lane byte offset,
+// so the address is a scalar register pair plus one VGPR (saddr form)
+// instead of a 64-bit vector add per access.
+__device__ __forceinline__ float ldf(const float *base, unsigned boff) {
+  return *(const float *)((const char *)base + boff);
+}
+__device__ __forceinline__ void stf(float *base, unsigned boff, float v) {
+  *(float *)((char *)base + boff) = v;
+}
+
+// Coefficient of lane L at row r (i*ny + j, wave-uniform) at byte offset
+// k4 of the row; `ob` is the row's first node ((i*ny + j)*nz).
+__device__ __forceinline__ float coef_row(const Lane &L, size_t r, size_t ob, int nz,
+                                          unsigned k4) {
+  if (L.rows == nullptr) return 0.0f;
+  const float *row = L.index == nullptr ? L.rows + ob : L.rows + (size_t)L.index[r] * nz;
+  return ldf(row, k4);
+}
+
+// What one node of one x-plane reads: its three voltages and the
+// neighbours the curls need, its three currents, and its 12 coefficients.
+struct PlaneIn {
+  float uxc, ux_j1, ux_k1, uy_k1, uz_j1, uyn, uzn;
+  float ix, iy, iz;
+  float c[12];
+};
+
+// Field buffer F holds u_x, u_y, u_z, i_x, i_y, i_z, N nodes each; the
+// coefficient pool holds every lane's rows, and croff[16*r + l] is the
+// offset of lane l's row for node row r (i*ny + j).
+__device__ __forceinline__ void load_plane(PlaneIn &P, const float *F, size_t N,
+                                           const float *cpool, const uint32_t *croff, size_t ob,
+                                           size_t r, size_t plane, int nz, unsigned k4, bool iv,
+                                           bool jv, bool kv) {
+  // Branch-free, every address valid (neighbour offsets fall back to the
+  // node itself where the neighbour does not exist; the value is then
+  // replaced by 0): all loads sit in one basic block with the zero
+  // extension of the lane offset, and use the saddr form.
+  asm volatile("" : "+v"(k4));
+  const unsigned kn = k4 + (kv ? 4u : 0u);
+  const size_t jn = jv ? (size_t)nz : 0, in = iv ? plane : 0;
+  const float *ux = F + ob, *uy = F + N + ob, *uz = F + 2 * N + ob;
+  const float uxc = ldf(ux, k4), uxj = ldf(ux + jn, k4), uzj = ldf(uz + jn, k4);
+  const float uxk = ldf(ux, kn), uyk = ldf(uy, kn);
+  const float uyi = ldf(uy + in, k4), uzi = ldf(uz + in, k4);
+  P.ix = ldf(F + 3 * N + ob, k4);
+  P.iy = ldf(F + 4 * N + ob, k4);
+  P.iz = ldf(F + 5 * N + ob, k4);
+  // One base for all lanes and a 32-bit byte offset per lane (the pool is
+  // kept under 4 GiB), so the coefficients need no 64-bit address math.
+  const uint32_t *ro = croff + 16 * r;
+#pragma unroll
+  for (int l = 0; l < 12; l++) P.c[l] = ldf(cpool, ro[l] * 4u + k4);
+  P.uxc = uxc;
+  P.ux_j1 = jv ? uxj : 0.0f;
+  P.uz_j1 = jv ? uzj : 0.0f;
+  P.ux_k1 = kv ? uxk : 0.0f;
+  P.uy_k1 = kv ? uyk : 0.0f;
+  P.uyn = iv ? uyi : 0.0f;
+  P.uzn = iv ? uzi : 0.0f;
+}
+
+// H update of a node from its plane inputs (kernel.rs:443-511) and its CPML
+// correction; padding components keep their old value.
+template <bool OE>
+__device__ __forceinline__ void h_from(const PlaneIn &P, float uyc, float uzc, bool iv, bool jv,
+                                       bool kv, const FSlab *slabs, uint32_t mask, int par,
+                                       bool write_psi, int i, int j, int k, float h[3]) {
+  h[0] = P.ix, h[1] = P.iy, h[2] = P.iz;
+  if (jv && kv) h[0] = h_expr<OE>(P.c[6], h[0], P.c[9], P.uz_j1, uzc, P.uy_k1, uyc);
+  if (iv && kv) h[1] = h_expr<OE>(P.c[7], h[1], P.c[10], P.ux_k1, P.uxc, P.uzn, uzc);
+  if (iv && jv) h[2] = h_expr<OE>(P.c[8], h[2], P.c[11], P.uyn, uyc, P.ux_j1, P.uxc);
+  if (mask)
+    // T_p = u_other[node + e_p] - u_other[node] for each (component, normal).
+    cpml_node<false>(slabs, mask, par, write_psi, i, j, k, P.uz_j1 - uzc, P.uy_k1 - uyc,
+                     P.uzn - uzc, P.ux_k1 - P.uxc, P.uyn - uyc, P.ux_j1 - P.uxc, P.c[9],
+                     P.c[10], P.c[11], h[0], h[1], h[2]);
+}
+
+// ---- fused step kernel ----
+template <bool OE, int TJ, bool PF>
+__global__ void __launch_bounds__(64 * (TJ + 1))
+    step_fused(const float *__restrict__ A, float *__restrict__ B, size_t N,
+               const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
+               const FSlab *__restrict__ slabs, int nslab, int par, int nx, int ny, int nz, int tk,
+               int chunk, int ktiles, int jtiles, int ntiles) {
+  // XCD-aware order: workgroups go round-robin to the 8 XCDs, so give each
+  // XCD a contiguous range of tiles (k fastest, then j, then x chunk) and
+  // the halo rows its neighbours read stay in its own L2.
+  const int b = blockIdx.x;
+  const int q8 = ntiles / 8, r8 = ntiles % 8, x8 = b % 8;
+  const int t = x8 * q8 + min(x8, r8) + b / 8;
+  const int kt = t % ktiles;
+  const int t2 = t / ktiles;
+  const int jt = t2 % jtiles;
+  const int c = t2 / jtiles;
+  const int tx = threadIdx.x;
+  // One wave per tile row: the row (and so every row address) is
+  // wave-uniform and lives in scalar registers.
+  const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int k0 = kt * tk, j0 = jt * TJ;
+  const int k = k0 - 1 + tx, j = j0 - 1 + ty;
+  const bool row_ok = j >= 0 && j < ny;
+  const bool inb = row_ok && tx <= tk && k >= 0 && k < nz;
+  const bool own = inb && tx > 0 && ty > 0;
+  // Addresses use the row and lane clamped into the grid, so every load is
+  // valid; values of nodes outside it are not used.
+  const int jc = min(max(j, 0), ny - 1);
+  const unsigned ku = (unsigned)min(max(k, 0), nz - 1);
+  const unsigned k4_ = ku * 4u;
+  const int i0 = c * chunk, i1 = min(nx, i0 + chunk);
+  const size_t plane = (size_t)ny * nz;
+  const bool jv = jc < ny - 1, kv = (int)ku < nz - 1;
+  // CPML slabs whose (j, k) box meets this tile (halo included).
+  uint32_t mh = 0, me = 0;
+  for (int q = 0; q < nslab; q++) {
+    const FSlab &s = slabs[q];
+    if (s.lo1 < j0 + TJ && s.hi1 > j0 - 1 && s.lo2 < k0 + tk && s.hi2 > k0 - 1) {
+      if (s.e_side) me |= 1u << q;
+      else mh |= 1u << q;
+    }
+  }
+  __shared__ float sh[2][3][TJ + 1][64];
+
+  float uyc, uzc, hy_prev = 0.0f, hz_prev = 0.0f;
+  PlaneIn P;
+  {
+    const size_t ob = ((size_t)i0 * ny + jc) * nz;
+    const unsigned k4 = k4_;
+    uyc = ldf(A + N + ob, k4);
+    uzc = ldf(A + 2 * N + ob, k4);
+    if (i0 > 0) {
+      // H of plane i0-1 at this node (read-only: its owner writes it).
+      PlaneIn Q;
+      load_plane(Q, A, N, cpool, croff, ob - plane, (size_t)(i0 - 1) * ny + jc, plane, nz, k4,
+                 true, jv, kv);
+      const float uyp = ldf(A + N + (ob - plane), k4), uzp = ldf(A + 2 * N + (ob - plane), k4);
+      if (inb) {
+        float h[3];
+        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh, par, false, i0 - 1, j, k, h);
+        hy_prev = h[1];
+        hz_prev = h[2];
+      }
+    }
+    load_plane(P, A, N, cpool, croff, ob, (size_t)i0 * ny + jc, plane, nz, k4, i0 + 1 < nx, jv,
+               kv);
+  }
+  for (int i = i0; i < i1; i++) {
+    const int buf = i & 1;
+    const size_t r = (size_t)i * ny + jc;
+    const size_t ob = r * nz;
+    const bool iv = i + 1 < nx;
+    // An opaque copy of the lane offset in each iteration, so the compiler
+    // sees "scalar base + zero-extended 32-bit offset" at every access and
+    // uses the saddr form (a hoisted 64-bit copy defeats that).
+    unsigned k4 = k4_;
+    asm volatile("" : "+v"(k4));
+    PlaneIn Nx;
+    if (PF && i + 1 < i1)
+      load_plane(Nx, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv);
+    float h[3] = {0.0f, 0.0f, 0.0f};
+    if (inb) h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mh, par, own, i, j, k, h);
+    sh[buf][0][ty][tx] = h[0];
+    sh[buf][1][ty][tx] = h[1];
+    sh[buf][2][ty][tx] = h[2];
+    __syncthreads();
+    if (own) {
+      const float hx_km = k > 0 ? sh[buf][0][ty][tx - 1] : 0.0f;
+      const float hy_km = k > 0 ? sh[buf][1][ty][tx - 1] : 0.0f;
+      const float hx_jm = j > 0 ? sh[buf][0][ty - 1][tx] : 0.0f;
+      const float hz_jm = j > 0 ? sh[buf][2][ty - 1][tx] : 0.0f;
+      float e0 = P.uxc, e1 = uyc, e2 = uzc;
+      // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
+      if (iv) e0 = e_expr<OE>(P.c[0], e0, P.c[3], h[2], hz_jm, h[1], hy_km);
+      // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
+      if (jv) e1 = e_expr<OE>(P.c[1], e1, P.c[4], h[0], hx_km, h[2], hz_prev);
+      // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
+      if (kv) e2 = e_expr<OE>(P.c[2], e2, P.c[5], h[1], hy_prev, h[0], hx_jm);
+      if (me)
+        // S_p = i_other[node] - i_other[node - e_p] (cpml.rs:495-521).
+        cpml_node<true>(slabs, me, par, true, i, j, k, h[2] - hz_jm, h[1] - hy_km,
+                        h[2] - hz_prev, h[0] - hx_km, h[1] - hy_prev, h[0] - hx_jm, P.c[3],
+                        P.c[4], P.c[5], e0, e1, e2);
+      stf(B + ob, k4, e0);
+      stf(B + N + ob, k4, e1);
+      stf(B + 2 * N + ob, k4, e2);
+      stf(B + 3 * N + ob, k4, h[0]);
+      stf(B + 4 * N + ob, k4, h[1]);
+      stf(B + 5 * N + ob, k4, h[2]);
+    }
+    hy_prev = h[1];
+    hz_prev = h[2];
+    uyc = P.uyn;
+    uzc = P.uzn;
+    if (i + 1 < i1) {
+      if (PF)
+        P = Nx;
+      else
+        load_plane(P, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv);
+    }
+  }
+}
+
+// croff[16*r + l]: offset in the pool of lane l's row for node row r.
+__global__ void build_row_offsets(uint32_t *croff, const uint32_t *const *index,
+                                  const uint32_t *base, size_t rows, int nz) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  for (int l = 0; l < 16; l++) {
+    uint32_t v = 0;
+    if (l < 12) {
+      const uint32_t *ix = index[l];
+      // Lanes without rows point at the pool's zero row (base[l] == 0).
+      if (base[l] != 0) v = base[l] + (ix ? ix[r] : (uint32_t)r) * (uint32_t)nz;
+    }
+    croff[16 * r + l] = v;
+  }
+}
+
+// Plain device copy (read n4 float4, write n4 float4): the stream-bandwidth
+// reference for the roofline and the rocprof byte counters.
+__global__ void __launch_bounds__(256) copy_f4(const float4 *__restrict__ a, float4 *__restrict__ b,
+                                               size_t n4) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4;
+       q += (size_t)gridDim.x * blockDim.x)
+    b[q] = a[q];
+}
+
 // Serial operators (excite.rs, probe.rs) and the gap rules of
 // Engine::advance (mod.rs:873-915).
 enum { OP_SOURCE = 1, OP_PROBE = 2 };
@@ -463,8 +758,9 @@ __global__ void __launch_bounds__(256) energy_combine(int nx, int ny, const doub
 // ------------------------------------------------------------------ host --
 
 struct HostSlab {
-  SlabArg arg;
-  float *d_b = nullptr, *d_a = nullptr, *d_k = nullptr, *d_psi = nullptr;
+  SlabArg arg;  // arg.psi is set per launch (parity)
+  float *d_b = nullptr, *d_a = nullptr, *d_k = nullptr;
+  float *d_psi[2] = {nullptr, nullptr};  // H side: double-buffered; E side: [0] only
   size_t len = 0;
 };
 
@@ -477,7 +773,6 @@ struct HostOp {
   double *d_wd = nullptr;
   float *d_amp = nullptr;
   uint64_t namp = 0;
-  std::vector<float> amp_host;
   double *d_vals = nullptr;
   uint64_t cap = 0;
   uint64_t count = 0;  // samples recorded (host mirror of *d_count)
@@ -493,20 +788,31 @@ struct ems_gpu {
   double dt = 0;
   int oe = 0;
   hipStream_t stream = nullptr;
-  Fields f{};
+  Fields fb[2]{};  // ping-pong field buffers (views into fbuf)
+  float *fbuf[2] = {nullptr, nullptr};  // u_x..u_z, i_x..i_z, N nodes each
+  // coefficient pool: a zero row, then every lane's rows; croff[16*r + l]
+  float *cpool = nullptr;
+  uint32_t *croff = nullptr;
+  bool lanes_dirty = true;
+  int par = 0;     // buffer holding the current state
   Lane lanes[12]{};
+  uint64_t lane_rows[12] = {0};
   std::vector<void *> lane_mem;
   std::vector<HostSlab> slabs;
+  FSlab *d_fslabs = nullptr;
   std::vector<HostOp> ops;
   DevOp *d_ops = nullptr;
-  bool ops_dirty = true;
+  bool ops_dirty = true, slabs_dirty = true;
   uint64_t step = 0;
   uint64_t *d_step = nullptr;
+  // fused schedule and its tiling
+  bool fused = true, prefetch = true;
+  int tj = 7, nchunk = 0;
   // energy scratch
   double *d_rowsum = nullptr, *d_chains = nullptr, *d_planes = nullptr, *d_w = nullptr;
-  // graphs of whole steps
-  hipGraphExec_t g1 = nullptr, gK = nullptr;
-  int K = 32;
+  // graphs of whole steps, per starting buffer
+  hipGraphExec_t g1[2] = {nullptr, nullptr}, gK[2] = {nullptr, nullptr};
+  int K = 32;  // even: a K-step graph ends on the buffer it started from
   std::string err;
   bool dead = false;
 };
@@ -546,42 +852,10 @@ Lanes lanes_arg(const ems_gpu *h) {
   return L;
 }
 
-int launch_h(ems_gpu *h) {
-  int nch, chunk = x_chunk(h, &nch);
-  dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
-  if (h->oe)
-    hipLaunchKernelGGL(h_update<true>, grid, dim3(256), 0, h->stream, h->f, lanes_arg(h), h->n[0],
-                       h->n[1], h->n[2], chunk);
-  else
-    hipLaunchKernelGGL(h_update<false>, grid, dim3(256), 0, h->stream, h->f, lanes_arg(h),
-                       h->n[0], h->n[1], h->n[2], chunk);
-  CK(h, hipGetLastError());
-  return 0;
-}
-
-int launch_e(ems_gpu *h) {
-  int nch, chunk = x_chunk(h, &nch);
-  dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
-  if (h->oe)
-    hipLaunchKernelGGL(e_update<true>, grid, dim3(256), 0, h->stream, h->f, lanes_arg(h), h->n[0],
-                       h->n[1], h->n[2], chunk);
-  else
-    hipLaunchKernelGGL(e_update<false>, grid, dim3(256), 0, h->stream, h->f, lanes_arg(h),
-                       h->n[0], h->n[1], h->n[2], chunk);
-  CK(h, hipGetLastError());
-  return 0;
-}
-
-int launch_cpml(ems_gpu *h, int e_side) {
-  for (auto &s : h->slabs) {
-    if (s.arg.e_side != e_side || s.len == 0) continue;
-    const Lane cf = h->lanes[e_side ? 3 + s.arg.comp : 9 + s.arg.comp];
-    unsigned blocks = (unsigned)((s.len + 255) / 256);
-    hipLaunchKernelGGL(cpml_slab, dim3(blocks), dim3(256), 0, h->stream, h->f, cf, s.arg, h->n[0],
-                       h->n[1], h->n[2]);
-    CK(h, hipGetLastError());
-  }
-  return 0;
+bool has_hsrc(const ems_gpu *h) {
+  for (auto &o : h->ops)
+    if (o.type == OP_SOURCE && (o.kind == 2 || o.kind == 3)) return true;
+  return false;
 }
 
 bool has_mid_ops(const ems_gpu *h) {
@@ -592,34 +866,185 @@ bool has_mid_ops(const ems_gpu *h) {
   return false;
 }
 
-int enqueue_step(ems_gpu *h) {
-  int rc;
-  if ((rc = launch_h(h))) return rc;
-  if ((rc = launch_cpml(h, 0))) return rc;
-  if (has_mid_ops(h)) {
-    hipLaunchKernelGGL(gap_mid, dim3(1), dim3(64), 0, h->stream, h->f, h->d_ops,
-                       (int)h->ops.size(), (const uint64_t *)h->d_step);
-    CK(h, hipGetLastError());
-  }
-  if ((rc = launch_e(h))) return rc;
-  if ((rc = launch_cpml(h, 1))) return rc;
-  hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(64), 0, h->stream, h->f, h->d_ops,
-                     (int)h->ops.size(), h->d_step);
+// Whether steps run fused: not while an H source needs the serial gap
+// between the H and E updates (GAP_B, mod.rs:899-913; pool.rs:870-875).
+// The fused kernel keeps the slabs that reach a tile in a 32-bit mask.
+bool use_fused(const ems_gpu *h) {
+  size_t ns = 0;
+  for (auto &s : h->slabs) ns += s.len != 0;
+  return h->fused && !has_hsrc(h) && ns <= 32;
+}
+
+int launch_h(ems_gpu *h, const Fields &f) {
+  int nch, chunk = x_chunk(h, &nch);
+  dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
+  if (h->oe)
+    hipLaunchKernelGGL(h_update<true>, grid, dim3(256), 0, h->stream, f, lanes_arg(h), h->n[0],
+                       h->n[1], h->n[2], chunk);
+  else
+    hipLaunchKernelGGL(h_update<false>, grid, dim3(256), 0, h->stream, f, lanes_arg(h), h->n[0],
+                       h->n[1], h->n[2], chunk);
   CK(h, hipGetLastError());
   return 0;
 }
 
-void drop_graphs(ems_gpu *h) {
-  if (h->g1) hipGraphExecDestroy(h->g1);
-  if (h->gK) hipGraphExecDestroy(h->gK);
-  h->g1 = h->gK = nullptr;
+int launch_e(ems_gpu *h, const Fields &f) {
+  int nch, chunk = x_chunk(h, &nch);
+  dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
+  if (h->oe)
+    hipLaunchKernelGGL(e_update<true>, grid, dim3(256), 0, h->stream, f, lanes_arg(h), h->n[0],
+                       h->n[1], h->n[2], chunk);
+  else
+    hipLaunchKernelGGL(e_update<false>, grid, dim3(256), 0, h->stream, f, lanes_arg(h), h->n[0],
+                       h->n[1], h->n[2], chunk);
+  CK(h, hipGetLastError());
+  return 0;
 }
 
-int capture(ems_gpu *h, int steps, hipGraphExec_t *out) {
+int launch_cpml(ems_gpu *h, const Fields &f, int e_side, int par) {
+  for (auto &s : h->slabs) {
+    if (s.arg.e_side != e_side || s.len == 0) continue;
+    const Lane cf = h->lanes[e_side ? 3 + s.arg.comp : 9 + s.arg.comp];
+    SlabArg a = s.arg;
+    a.psi = e_side ? s.d_psi[0] : s.d_psi[par];
+    unsigned blocks = (unsigned)((s.len + 255) / 256);
+    hipLaunchKernelGGL(cpml_slab, dim3(blocks), dim3(256), 0, h->stream, f, cf, a, h->n[0],
+                       h->n[1], h->n[2]);
+    CK(h, hipGetLastError());
+  }
+  return 0;
+}
+
+struct FusedGeom {
+  int tk, ktiles, jtiles, chunk, nchunks, ntiles;
+};
+
+FusedGeom fused_geom(const ems_gpu *h) {
+  FusedGeom g;
+  const int nz = h->n[2], ny = h->n[1], nx = h->n[0];
+  // Owned k per tile <= 63 (one wave with a halo lane), spread evenly.
+  g.ktiles = (nz + 62) / 63;
+  g.tk = (nz + g.ktiles - 1) / g.ktiles;
+  g.jtiles = (ny + h->tj - 1) / h->tj;
+  int want = h->nchunk;
+  if (want <= 0) {
+    // ~2048 blocks (eight per CU) unless that makes chunks very short.
+    const int per = g.ktiles * g.jtiles;
+    want = (2048 + per - 1) / per;
+    want = std::max(1, std::min(want, std::max(1, nx / 16)));
+  }
+  want = std::min(want, nx);
+  g.chunk = (nx + want - 1) / want;
+  g.nchunks = (nx + g.chunk - 1) / g.chunk;
+  g.ntiles = g.ktiles * g.jtiles * g.nchunks;
+  return g;
+}
+
+int upload_fslabs(ems_gpu *h) {
+  if (!h->slabs_dirty) return 0;
+  if (h->d_fslabs) CK(h, hipFree(h->d_fslabs));
+  h->d_fslabs = nullptr;
+  std::vector<FSlab> v;
+  for (auto &s : h->slabs) {
+    if (s.len == 0) continue;
+    FSlab f;
+    f.e_side = s.arg.e_side, f.comp = s.arg.comp, f.normal = s.arg.normal, f.sign = s.arg.sign;
+    f.lo0 = s.arg.lo[0], f.hi0 = s.arg.hi[0], f.lo1 = s.arg.lo[1], f.hi1 = s.arg.hi[1];
+    f.lo2 = s.arg.lo[2], f.hi2 = s.arg.hi[2];
+    f.ly = s.arg.hi[1] - s.arg.lo[1], f.lz = s.arg.hi[2] - s.arg.lo[2];
+    f.b = s.d_b, f.a = s.d_a, f.k = s.d_k;
+    f.psi0 = s.d_psi[0];
+    f.psi1 = s.arg.e_side ? s.d_psi[0] : s.d_psi[1];
+    v.push_back(f);
+  }
+  if (!v.empty()) {
+    CK(h, hipMalloc(&h->d_fslabs, v.size() * sizeof(FSlab)));
+    CK(h, hipMemcpy(h->d_fslabs, v.data(), v.size() * sizeof(FSlab), hipMemcpyHostToDevice));
+  }
+  h->slabs_dirty = false;
+  return 0;
+}
+
+int n_fslabs(const ems_gpu *h) {
+  int n = 0;
+  for (auto &s : h->slabs) n += s.len != 0;
+  return n;
+}
+
+template <int TJ, bool PF>
+void launch_fused_tj(ems_gpu *h, int par, const FusedGeom &g) {
+  const float *A = h->fbuf[par];
+  float *B = h->fbuf[1 - par];
+  dim3 block(64, TJ + 1);
+  const int ns = n_fslabs(h);
+  if (h->oe)
+    hipLaunchKernelGGL((step_fused<true, TJ, PF>), dim3(g.ntiles), block, 0, h->stream, A, B, h->N,
+                       (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs, ns, par,
+                       h->n[0], h->n[1], h->n[2], g.tk, g.chunk, g.ktiles, g.jtiles, g.ntiles);
+  else
+    hipLaunchKernelGGL((step_fused<false, TJ, PF>), dim3(g.ntiles), block, 0, h->stream, A, B,
+                       h->N, (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs, ns,
+                       par, h->n[0], h->n[1], h->n[2], g.tk, g.chunk, g.ktiles, g.jtiles,
+                       g.ntiles);
+}
+
+int launch_fused(ems_gpu *h, int par) {
+  const FusedGeom g = fused_geom(h);
+  const int key = h->tj * 2 + (h->prefetch ? 1 : 0);
+  switch (key) {
+    case 6: launch_fused_tj<3, false>(h, par, g); break;
+    case 7: launch_fused_tj<3, true>(h, par, g); break;
+    case 14: launch_fused_tj<7, false>(h, par, g); break;
+    case 15: launch_fused_tj<7, true>(h, par, g); break;
+    case 30: launch_fused_tj<15, false>(h, par, g); break;
+    case 31: launch_fused_tj<15, true>(h, par, g); break;
+    default: return fail(h, EMS_E_INVALID, "unsupported tile height");
+  }
+  CK(h, hipGetLastError());
+  return 0;
+}
+
+int launch_gap_post_e(ems_gpu *h, const Fields &f) {
+  hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(64), 0, h->stream, f, h->d_ops, (int)h->ops.size(),
+                     h->d_step);
+  CK(h, hipGetLastError());
+  return 0;
+}
+
+// One step starting from buffer `par`; returns the buffer of the result.
+int enqueue_step(ems_gpu *h, int &par) {
+  int rc;
+  if (use_fused(h)) {
+    if ((rc = launch_fused(h, par))) return rc;
+    par ^= 1;
+    return launch_gap_post_e(h, h->fb[par]);
+  }
+  const Fields &f = h->fb[par];
+  if ((rc = launch_h(h, f))) return rc;
+  if ((rc = launch_cpml(h, f, 0, par))) return rc;
+  if (has_mid_ops(h)) {
+    hipLaunchKernelGGL(gap_mid, dim3(1), dim3(64), 0, h->stream, f, h->d_ops, (int)h->ops.size(),
+                       (const uint64_t *)h->d_step);
+    CK(h, hipGetLastError());
+  }
+  if ((rc = launch_e(h, f))) return rc;
+  if ((rc = launch_cpml(h, f, 1, par))) return rc;
+  return launch_gap_post_e(h, f);
+}
+
+void drop_graphs(ems_gpu *h) {
+  for (int p = 0; p < 2; p++) {
+    if (h->g1[p]) (void)hipGraphExecDestroy(h->g1[p]);
+    if (h->gK[p]) (void)hipGraphExecDestroy(h->gK[p]);
+    h->g1[p] = h->gK[p] = nullptr;
+  }
+}
+
+int capture(ems_gpu *h, int steps, int par, hipGraphExec_t *out) {
   hipGraph_t g;
   CK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
-  for (int s = 0; s < steps && !rc; s++) rc = enqueue_step(h);
+  for (int s = 0; s < steps && !rc; s++) rc = enqueue_step(h, par);
   hipError_t e = hipStreamEndCapture(h->stream, &g);
   if (rc) return rc;
   CK(h, e);
@@ -628,9 +1053,71 @@ int capture(ems_gpu *h, int steps, hipGraphExec_t *out) {
   return 0;
 }
 
-// Uploads the operator table, and grows probe buffers to hold `steps` more
-// steps of samples.
+// Uploads the operator and slab tables, and grows probe buffers to hold
+// `steps` more steps of samples.
+// Moves every lane's rows into one pool (a zero row first) and builds the
+// per-row offset table the fused kernel reads; the lanes then point into
+// the pool, so each coefficient is stored once.
+int build_pool(ems_gpu *h) {
+  const int nz = h->n[2];
+  const size_t rows = (size_t)h->n[0] * h->n[1];
+  uint64_t base[12] = {0};
+  uint64_t total = (uint64_t)nz;  // the zero row
+  for (int l = 0; l < 12; l++)
+    if (h->lanes[l].rows) {
+      base[l] = total;
+      total += h->lane_rows[l] * (uint64_t)nz;
+    }
+  // The fused kernel addresses the pool with 32-bit byte offsets.
+  if (total * sizeof(float) >= (1ull << 32))
+    return fail(h, EMS_E_UNSUPPORTED, "coefficient pool over 4 GiB");
+  float *pool = nullptr;
+  CK(h, hipMalloc(&pool, total * sizeof(float)));
+  CK(h, hipMemset(pool, 0, (size_t)nz * sizeof(float)));
+  for (int l = 0; l < 12; l++)
+    if (h->lanes[l].rows)
+      CK(h, hipMemcpy(pool + base[l], h->lanes[l].rows, h->lane_rows[l] * nz * sizeof(float),
+                      hipMemcpyDeviceToDevice));
+  if (!h->croff) CK(h, hipMalloc(&h->croff, rows * 16 * sizeof(uint32_t)));
+  const uint32_t *ix[12];
+  uint32_t b32[12];
+  for (int l = 0; l < 12; l++) ix[l] = h->lanes[l].index, b32[l] = (uint32_t)base[l];
+  void *d_ix = nullptr, *d_b = nullptr;
+  CK(h, hipMalloc(&d_ix, sizeof ix));
+  CK(h, hipMalloc(&d_b, sizeof b32));
+  CK(h, hipMemcpy(d_ix, ix, sizeof ix, hipMemcpyHostToDevice));
+  CK(h, hipMemcpy(d_b, b32, sizeof b32, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(build_row_offsets, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0,
+                     h->stream, h->croff, (const uint32_t *const *)d_ix, (const uint32_t *)d_b,
+                     rows, nz);
+  CK(h, hipGetLastError());
+  CK(h, hipStreamSynchronize(h->stream));
+  CK(h, hipFree(d_ix));
+  CK(h, hipFree(d_b));
+  // Point the lanes into the pool and free their own copies.
+  for (int l = 0; l < 12; l++)
+    if (h->lanes[l].rows) {
+      void *old = (void *)h->lanes[l].rows;
+      h->lanes[l].rows = pool + base[l];
+      for (auto it = h->lane_mem.begin(); it != h->lane_mem.end(); ++it)
+        if (*it == old) {
+          CK(h, hipFree(old));
+          h->lane_mem.erase(it);
+          break;
+        }
+    }
+  if (h->cpool) CK(h, hipFree(h->cpool));
+  h->cpool = pool;
+  h->lanes_dirty = false;
+  return 0;
+}
+
 int prepare(ems_gpu *h, uint64_t steps) {
+  if (h->lanes_dirty) {
+    drop_graphs(h);
+    int rc = build_pool(h);
+    if (rc) return rc;
+  }
   bool grown = false;
   for (auto &o : h->ops) {
     if (o.type != OP_PROBE) continue;
@@ -654,6 +1141,11 @@ int prepare(ems_gpu *h, uint64_t steps) {
     }
   }
   if (grown) h->ops_dirty = true;
+  if (h->slabs_dirty) {
+    drop_graphs(h);
+    int rc = upload_fslabs(h);
+    if (rc) return rc;
+  }
   if (h->ops_dirty) {
     drop_graphs(h);
     std::vector<DevOp> dv(h->ops.size());
@@ -689,6 +1181,7 @@ void advance_counts(ems_gpu *h, uint64_t steps) {
 int energy_now(ems_gpu *h, double *w) {
   const int nx = h->n[0], ny = h->n[1], nz = h->n[2];
   const size_t rows = (size_t)nx * (3 * ny - 2);
+  const Fields &f = h->fb[h->par];
   if (!h->d_rowsum) {
     CK(h, hipMalloc(&h->d_rowsum, rows * sizeof(double)));
     CK(h, hipMalloc(&h->d_chains, (size_t)nx * 12 * sizeof(double)));
@@ -698,14 +1191,14 @@ int energy_now(ems_gpu *h, double *w) {
   const size_t threads = rows * 4;
   dim3 g1((unsigned)((threads + 255) / 256));
   if (h->oe)
-    hipLaunchKernelGGL(energy_h_rows<true>, g1, dim3(256), 0, h->stream, h->f, lanes_arg(h), nx, ny,
+    hipLaunchKernelGGL(energy_h_rows<true>, g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny,
                        nz, h->d_rowsum);
   else
-    hipLaunchKernelGGL(energy_h_rows<false>, g1, dim3(256), 0, h->stream, h->f, lanes_arg(h), nx,
-                       ny, nz, h->d_rowsum);
+    hipLaunchKernelGGL(energy_h_rows<false>, g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny,
+                       nz, h->d_rowsum);
   CK(h, hipGetLastError());
   dim3 g2((unsigned)(((size_t)nx * 12 + 63) / 64));
-  hipLaunchKernelGGL(energy_e_chains, g2, dim3(64), 0, h->stream, h->f, lanes_arg(h), nx, ny, nz,
+  hipLaunchKernelGGL(energy_e_chains, g2, dim3(64), 0, h->stream, f, lanes_arg(h), nx, ny, nz,
                      h->d_chains);
   CK(h, hipGetLastError());
   hipLaunchKernelGGL(energy_combine, dim3(1), dim3(256), 0, h->stream, nx, ny, h->d_rowsum,
@@ -722,17 +1215,22 @@ int run_steps(ems_gpu *h, uint64_t n) {
   if (n == 0) return 0;
   int rc = prepare(h, n);
   if (rc) return rc;
-  if (!h->g1) {
-    if ((rc = capture(h, 1, &h->g1))) return rc;
-    if ((rc = capture(h, h->K, &h->gK))) return rc;
+  const int p0 = h->par;
+  if (!h->g1[p0]) {
+    for (int p = 0; p < 2; p++) {
+      if ((rc = capture(h, 1, p, &h->g1[p]))) return rc;
+      if ((rc = capture(h, h->K, p, &h->gK[p]))) return rc;
+    }
   }
+  const int flip = use_fused(h) ? 1 : 0;
   uint64_t left = n;
   while (left >= (uint64_t)h->K) {
-    CK(h, hipGraphLaunch(h->gK, h->stream));
-    left -= h->K;
+    CK(h, hipGraphLaunch(h->gK[h->par], h->stream));
+    left -= h->K;  // K is even: the buffer does not change
   }
   while (left > 0) {
-    CK(h, hipGraphLaunch(h->g1, h->stream));
+    CK(h, hipGraphLaunch(h->g1[h->par], h->stream));
+    h->par ^= flip;
     left--;
   }
   advance_counts(h, n);
@@ -750,6 +1248,12 @@ bool node_ok(const ems_gpu *h, const int32_t *nd) {
     if (nd[d] < 0 || nd[d] >= h->n[d]) return false;
   return true;
 }
+
+#define ALIVE(h)                                                                          \
+  do {                                                                                    \
+    if (!(h)) return EMS_E_INVALID;                                                       \
+    if ((h)->dead) return fail((h), EMS_E_DEVICE, "engine has a device fault: " + (h)->err); \
+  } while (0)
 
 }  // namespace
 
@@ -772,20 +1276,21 @@ int ems_gpu_create(const ems_grid_desc *d, ems_gpu **out) {
   h->plane = (size_t)d->nodes[1] * d->nodes[2];
   h->dt = d->dt;
   h->oe = d->curl_order;
-  int rc = 0;
   auto init = [&]() -> int {
     CK(h, hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    for (int a = 0; a < 3; a++) {
-      CK(h, hipMalloc(&h->f.u[a], N * sizeof(float)));
-      CK(h, hipMalloc(&h->f.i[a], N * sizeof(float)));
-      CK(h, hipMemset(h->f.u[a], 0, N * sizeof(float)));
-      CK(h, hipMemset(h->f.i[a], 0, N * sizeof(float)));
+    for (int b = 0; b < 2; b++) {
+      CK(h, hipMalloc(&h->fbuf[b], 6 * N * sizeof(float)));
+      CK(h, hipMemset(h->fbuf[b], 0, 6 * N * sizeof(float)));
+      for (int a = 0; a < 3; a++) {
+        h->fb[b].u[a] = h->fbuf[b] + a * N;
+        h->fb[b].i[a] = h->fbuf[b] + (3 + a) * N;
+      }
     }
     CK(h, hipMalloc(&h->d_step, sizeof(uint64_t)));
     CK(h, hipMemset(h->d_step, 0, sizeof(uint64_t)));
     return 0;
   };
-  rc = init();
+  int rc = init();
   if (rc) {
     std::fprintf(stderr, "ems_gpu_create: %s\n", h->err.c_str());
     ems_gpu_destroy(h);
@@ -797,25 +1302,31 @@ int ems_gpu_create(const ems_grid_desc *d, ems_gpu **out) {
 
 void ems_gpu_destroy(ems_gpu *h) {
   if (!h) return;
-  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
   drop_graphs(h);
-  for (int a = 0; a < 3; a++) {
-    if (h->f.u[a]) hipFree(h->f.u[a]);
-    if (h->f.i[a]) hipFree(h->f.i[a]);
-  }
-  for (void *p : h->lane_mem) hipFree(p);
+  for (int b = 0; b < 2; b++)
+    if (h->fbuf[b]) (void)hipFree(h->fbuf[b]);
+  for (void *p : h->lane_mem) (void)hipFree(p);
+  if (h->cpool) (void)hipFree(h->cpool);
+  if (h->croff) (void)hipFree(h->croff);
   for (auto &s : h->slabs) {
-    hipFree(s.d_b), hipFree(s.d_a), hipFree(s.d_k), hipFree(s.d_psi);
+    (void)hipFree(s.d_b), (void)hipFree(s.d_a), (void)hipFree(s.d_k);
+    (void)hipFree(s.d_psi[0]);
+    if (s.d_psi[1]) (void)hipFree(s.d_psi[1]);
   }
+  if (h->d_fslabs) (void)hipFree(h->d_fslabs);
   for (auto &o : h->ops) {
-    hipFree(o.d_axis), hipFree(o.d_idx), hipFree(o.d_wf), hipFree(o.d_wd), hipFree(o.d_amp);
-    hipFree(o.d_vals), hipFree(o.d_count);
+    (void)hipFree(o.d_axis), (void)hipFree(o.d_idx), (void)hipFree(o.d_wf);
+    (void)hipFree(o.d_wd), (void)hipFree(o.d_amp), (void)hipFree(o.d_vals);
+    (void)hipFree(o.d_count);
   }
-  if (h->d_ops) hipFree(h->d_ops);
-  if (h->d_step) hipFree(h->d_step);
-  if (h->d_rowsum) hipFree(h->d_rowsum), hipFree(h->d_chains), hipFree(h->d_planes),
-      hipFree(h->d_w);
-  if (h->stream) hipStreamDestroy(h->stream);
+  if (h->d_ops) (void)hipFree(h->d_ops);
+  if (h->d_step) (void)hipFree(h->d_step);
+  if (h->d_rowsum) {
+    (void)hipFree(h->d_rowsum), (void)hipFree(h->d_chains);
+    (void)hipFree(h->d_planes), (void)hipFree(h->d_w);
+  }
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
 
@@ -823,12 +1334,10 @@ const char *ems_gpu_last_error(const ems_gpu *h) { return h ? h->err.c_str() : "
 
 int ems_gpu_set_lane(ems_gpu *h, int lane, const float *rows, uint64_t nrows,
                      const uint32_t *row_index) {
-  if (!h) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
+  ALIVE(h);
   if (lane < 0 || lane >= 12 || !rows) return fail(h, EMS_E_INVALID, "lane out of range");
   const size_t nr = (size_t)h->n[0] * h->n[1];
-  if (!row_index && nrows != nr)
-    return fail(h, EMS_E_INVALID, "dense lane needs nx*ny rows");
+  if (!row_index && nrows != nr) return fail(h, EMS_E_INVALID, "dense lane needs nx*ny rows");
   if (row_index)
     for (size_t q = 0; q < nr; q++)
       if (row_index[q] >= nrows) return fail(h, EMS_E_INVALID, "row index out of range");
@@ -842,13 +1351,15 @@ int ems_gpu_set_lane(ems_gpu *h, int lane, const float *rows, uint64_t nrows,
   }
   h->lanes[lane].rows = (const float *)dr;
   h->lanes[lane].index = (const uint32_t *)di;
+  h->lane_rows[lane] = nrows;
+  h->lanes_dirty = true;
   drop_graphs(h);
   return EMS_OK;
 }
 
 int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *s, int n) {
-  if (!h || (n > 0 && !s)) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
+  ALIVE(h);
+  if (n > 0 && !s) return EMS_E_INVALID;
   for (int q = 0; q < n; q++) {
     const ems_cpml_slab &in = s[q];
     if (in.comp < 0 || in.comp > 2 || in.normal < 0 || in.normal > 2 || in.comp == in.normal)
@@ -856,7 +1367,8 @@ int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *s, int n) {
     for (int d = 0; d < 3; d++)
       if (in.lo[d] < 0 || in.hi[d] > h->n[d] || in.lo[d] > in.hi[d])
         return fail(h, EMS_E_INVALID, "CPML slab: box out of range");
-    // The derivative reads one node past the box on the H side.
+    // The H-side derivative reads one node past the box along the normal;
+    // an E-side box covers only existing E edges of its component.
     if (!in.e_side && in.hi[in.normal] >= h->n[in.normal] && in.lo[in.normal] < in.hi[in.normal])
       return fail(h, EMS_E_INVALID, "CPML slab: H box touches the last line");
     HostSlab hs;
@@ -879,11 +1391,15 @@ int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *s, int n) {
     if ((rc = upload(h, (void **)&hs.d_a, in.a, depth * sizeof(float)))) return rc;
     if ((rc = upload(h, (void **)&hs.d_k, in.k, depth * sizeof(float)))) return rc;
     hs.len = (size_t)(in.hi[0] - in.lo[0]) * (in.hi[1] - in.lo[1]) * (in.hi[2] - in.lo[2]);
-    CK(h, hipMalloc(&hs.d_psi, (hs.len ? hs.len : 1) * sizeof(float)));
-    CK(h, hipMemset(hs.d_psi, 0, (hs.len ? hs.len : 1) * sizeof(float)));
-    a.b = hs.d_b, a.a = hs.d_a, a.k = hs.d_k, a.psi = hs.d_psi;
+    const size_t bytes = (hs.len ? hs.len : 1) * sizeof(float);
+    for (int b = 0; b < (a.e_side ? 1 : 2); b++) {
+      CK(h, hipMalloc(&hs.d_psi[b], bytes));
+      CK(h, hipMemset(hs.d_psi[b], 0, bytes));
+    }
+    a.b = hs.d_b, a.a = hs.d_a, a.k = hs.d_k, a.psi = nullptr;
     h->slabs.push_back(hs);
   }
+  h->slabs_dirty = true;
   drop_graphs(h);
   return EMS_OK;
 }
@@ -891,8 +1407,7 @@ int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *s, int n) {
 int ems_gpu_add_source(ems_gpu *h, int kind, int n_edges, const int32_t *axis,
                        const int32_t *nodes, const float *w, const float *amp, uint64_t n_amp,
                        int *id) {
-  if (!h) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
+  ALIVE(h);
   if (kind < 0 || kind > 3) return fail(h, EMS_E_INVALID, "source kind");
   if (n_edges < 0 || (n_edges && (!axis || !nodes || !w)) || (n_amp && !amp))
     return fail(h, EMS_E_INVALID, "source arrays");
@@ -920,8 +1435,7 @@ int ems_gpu_add_source(ems_gpu *h, int kind, int n_edges, const int32_t *axis,
 
 int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *axis,
                       const int32_t *nodes, const double *w, uint64_t interval, int *id) {
-  if (!h) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
+  ALIVE(h);
   if (quantity < 0 || quantity > 1) return fail(h, EMS_E_INVALID, "probe quantity");
   if (n_entries < 0 || (n_entries && (!axis || !nodes || !w)))
     return fail(h, EMS_E_INVALID, "probe arrays");
@@ -949,19 +1463,38 @@ int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *ax
   return EMS_OK;
 }
 
+int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks) {
+  ALIVE(h);
+  // fused 2: the fused kernel without the next-plane prefetch.
+  h->prefetch = fused != 2;
+  if (tile_rows != 0 && tile_rows != 3 && tile_rows != 7 && tile_rows != 15)
+    return fail(h, EMS_E_INVALID, "tile_rows must be 3, 7 or 15");
+  if (x_chunks < 0) return fail(h, EMS_E_INVALID, "x_chunks must be >= 0");
+  h->fused = fused != 0;
+  if (tile_rows) h->tj = tile_rows;
+  h->nchunk = x_chunks;
+  drop_graphs(h);
+  return EMS_OK;
+}
+
 int ems_gpu_run(ems_gpu *h, uint64_t n) {
-  if (!h) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
+  ALIVE(h);
   return run_steps(h, n);
 }
 
+// Operator energies (Operator::energy, operator.rs:106-108): 0 for every
+// kind held here; Sum over the registered operators folds from -0.0
+// (mod.rs:772-774), so it is -0.0 with none and 0.0 otherwise.
+static double ops_energy(const ems_gpu *h) {
+  double w = -0.0;
+  if (!h->ops.empty() || !h->slabs.empty()) w += 0.0;
+  return w;
+}
+
 int ems_gpu_step_with_energy(ems_gpu *h, double *w) {
-  if (!h || !w) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
-  // Operator energies (Operator::energy, operator.rs): 0 for every kind
-  // held here; Sum folds from -0.0 (mod.rs:772-774).
-  double w_ops = -0.0;
-  for (size_t q = 0; q < h->ops.size() + (h->slabs.empty() ? 0 : 1); q++) w_ops += 0.0;
+  ALIVE(h);
+  if (!w) return EMS_E_INVALID;
+  const double w_ops = ops_energy(h);
   double wf = 0;
   int rc = energy_now(h, &wf);
   if (rc) return rc;
@@ -971,53 +1504,53 @@ int ems_gpu_step_with_energy(ems_gpu *h, double *w) {
 }
 
 int ems_gpu_energy(ems_gpu *h, double *w) {
-  if (!h || !w) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
-  double w_ops = -0.0;
-  for (size_t q = 0; q < h->ops.size() + (h->slabs.empty() ? 0 : 1); q++) w_ops += 0.0;
+  ALIVE(h);
+  if (!w) return EMS_E_INVALID;
   double wf = 0;
   int rc = energy_now(h, &wf);
   if (rc) return rc;
-  *w = wf + w_ops;
+  *w = wf + ops_energy(h);
   return EMS_OK;
 }
 
 uint64_t ems_gpu_timestep(const ems_gpu *h) { return h ? h->step : 0; }
 
 int ems_gpu_read_component(ems_gpu *h, int field, int axis, float *out) {
-  if (!h || !out || axis < 0 || axis > 2 || field < 0 || field > 1) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
-  float *src = field ? h->f.i[axis] : h->f.u[axis];
+  ALIVE(h);
+  if (!out || axis < 0 || axis > 2 || field < 0 || field > 1) return EMS_E_INVALID;
+  const Fields &f = h->fb[h->par];
+  float *src = field ? f.i[axis] : f.u[axis];
   CK(h, hipMemcpyAsync(out, src, h->N * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   CK(h, hipStreamSynchronize(h->stream));
   return EMS_OK;
 }
 
 int ems_gpu_write_component(ems_gpu *h, int field, int axis, const float *in) {
-  if (!h || !in || axis < 0 || axis > 2 || field < 0 || field > 1) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
-  float *dst = field ? h->f.i[axis] : h->f.u[axis];
+  ALIVE(h);
+  if (!in || axis < 0 || axis > 2 || field < 0 || field > 1) return EMS_E_INVALID;
+  const Fields &f = h->fb[h->par];
+  float *dst = field ? f.i[axis] : f.u[axis];
   CK(h, hipMemcpyAsync(dst, in, h->N * sizeof(float), hipMemcpyHostToDevice, h->stream));
   CK(h, hipStreamSynchronize(h->stream));
   return EMS_OK;
 }
 
 int ems_gpu_probe_values(ems_gpu *h, int id, double *out, uint64_t cap, uint64_t *count) {
-  if (!h || id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_PROBE)
-    return h ? fail(h, EMS_E_INVALID, "not a probe id") : EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
+  ALIVE(h);
+  if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_PROBE)
+    return fail(h, EMS_E_INVALID, "not a probe id");
   HostOp &o = h->ops[id];
   CK(h, hipStreamSynchronize(h->stream));
   uint64_t n = o.count < cap ? o.count : cap;
-  if (n && out)
-    CK(h, hipMemcpy(out, o.d_vals, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (n && out) CK(h, hipMemcpy(out, o.d_vals, n * sizeof(double), hipMemcpyDeviceToHost));
   if (count) *count = o.count;
   return EMS_OK;
 }
 
 int ems_gpu_probe_times(ems_gpu *h, int id, double *out, uint64_t cap, uint64_t *count) {
-  if (!h || id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_PROBE)
-    return h ? fail(h, EMS_E_INVALID, "not a probe id") : EMS_E_INVALID;
+  ALIVE(h);
+  if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_PROBE)
+    return fail(h, EMS_E_INVALID, "not a probe id");
   const HostOp &o = h->ops[id];
   const double offset = o.quantity == 1 ? 0.5 : 1.0;
   const uint64_t n = o.count < cap ? o.count : cap;
@@ -1030,65 +1563,113 @@ int ems_gpu_probe_times(ems_gpu *h, int id, double *out, uint64_t cap, uint64_t 
 double ems_gpu_dt(const ems_gpu *h) { return h ? h->dt : 0.0; }
 
 int ems_gpu_synchronize(ems_gpu *h) {
-  if (!h) return EMS_E_INVALID;
+  ALIVE(h);
   CK(h, hipStreamSynchronize(h->stream));
   return EMS_OK;
 }
 
 int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
-  if (!h || !out || steps == 0) return EMS_E_INVALID;
-  if (h->dead) return fail(h, EMS_E_DEVICE, "engine has a device fault: " + h->err);
+  ALIVE(h);
+  if (!out || steps == 0) return EMS_E_INVALID;
   int rc = prepare(h, steps);
   if (rc) return rc;
   hipEvent_t ev[5];
   for (auto &e : ev) CK(h, hipEventCreate(&e));
-  double th = 0, te = 0, ts = 0;
+  double t0 = 0, t1 = 0, ts = 0;
   for (uint64_t s = 0; s < steps; s++) {
-    CK(h, hipEventRecord(ev[0], h->stream));
-    if ((rc = launch_h(h))) return rc;
-    CK(h, hipEventRecord(ev[1], h->stream));
-    if ((rc = launch_cpml(h, 0))) return rc;
-    if (has_mid_ops(h)) {
-      hipLaunchKernelGGL(gap_mid, dim3(1), dim3(64), 0, h->stream, h->f, h->d_ops,
-                         (int)h->ops.size(), (const uint64_t *)h->d_step);
-      CK(h, hipGetLastError());
+    float a = 0, b = 0, c = 0;
+    if (use_fused(h)) {
+      CK(h, hipEventRecord(ev[0], h->stream));
+      if ((rc = launch_fused(h, h->par))) return rc;
+      CK(h, hipEventRecord(ev[1], h->stream));
+      h->par ^= 1;
+      if ((rc = launch_gap_post_e(h, h->fb[h->par]))) return rc;
+      CK(h, hipEventRecord(ev[4], h->stream));
+      CK(h, hipEventSynchronize(ev[4]));
+      CK(h, hipEventElapsedTime(&a, ev[0], ev[1]));
+      CK(h, hipEventElapsedTime(&c, ev[0], ev[4]));
+    } else {
+      const Fields &f = h->fb[h->par];
+      CK(h, hipEventRecord(ev[0], h->stream));
+      if ((rc = launch_h(h, f))) return rc;
+      CK(h, hipEventRecord(ev[1], h->stream));
+      if ((rc = launch_cpml(h, f, 0, h->par))) return rc;
+      if (has_mid_ops(h)) {
+        hipLaunchKernelGGL(gap_mid, dim3(1), dim3(64), 0, h->stream, f, h->d_ops,
+                           (int)h->ops.size(), (const uint64_t *)h->d_step);
+        CK(h, hipGetLastError());
+      }
+      CK(h, hipEventRecord(ev[2], h->stream));
+      if ((rc = launch_e(h, f))) return rc;
+      CK(h, hipEventRecord(ev[3], h->stream));
+      if ((rc = launch_cpml(h, f, 1, h->par))) return rc;
+      if ((rc = launch_gap_post_e(h, f))) return rc;
+      CK(h, hipEventRecord(ev[4], h->stream));
+      CK(h, hipEventSynchronize(ev[4]));
+      CK(h, hipEventElapsedTime(&a, ev[0], ev[1]));
+      CK(h, hipEventElapsedTime(&b, ev[2], ev[3]));
+      CK(h, hipEventElapsedTime(&c, ev[0], ev[4]));
     }
-    CK(h, hipEventRecord(ev[2], h->stream));
-    if ((rc = launch_e(h))) return rc;
-    CK(h, hipEventRecord(ev[3], h->stream));
-    if ((rc = launch_cpml(h, 1))) return rc;
-    hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(64), 0, h->stream, h->f, h->d_ops,
-                       (int)h->ops.size(), h->d_step);
-    CK(h, hipGetLastError());
-    CK(h, hipEventRecord(ev[4], h->stream));
-    CK(h, hipEventSynchronize(ev[4]));
-    float a, b, c;
-    CK(h, hipEventElapsedTime(&a, ev[0], ev[1]));
-    CK(h, hipEventElapsedTime(&b, ev[2], ev[3]));
-    CK(h, hipEventElapsedTime(&c, ev[0], ev[4]));
-    th += a, te += b, ts += c;
+    t0 += a, t1 += b, ts += c;
   }
-  for (auto &e : ev) hipEventDestroy(e);
+  for (auto &e : ev) (void)hipEventDestroy(e);
   advance_counts(h, steps);
-  out[0] = th / steps, out[1] = te / steps, out[2] = ts / steps;
+  out[0] = t0 / steps, out[1] = t1 / steps, out[2] = ts / steps;
   return EMS_OK;
 }
 
 int ems_gpu_traffic_model(ems_gpu *h, double out[3]) {
-  if (!h || !out) return EMS_E_INVALID;
-  // Algorithmic bytes of each launch: H reads u (3) and i (3) and writes
-  // i (3); E reads i (3) and u (3) and writes u (3); each dense coefficient
-  // lane adds 4 B per node (pooled rows stay cache-resident).
-  double hb = 36.0, eb = 36.0;
+  ALIVE(h);
+  if (!out) return EMS_E_INVALID;
+  // Algorithmic bytes of one launch. Unfused: H reads u (3) and i (3) and
+  // writes i (3); E reads i (3) and u (3) and writes u (3). Fused: reads u
+  // and i (6) and writes u and i (6) once per node, plus psi (read + write)
+  // of every CPML node. Each dense coefficient lane adds 4 B per node
+  // (pooled rows stay cache-resident).
+  double dense_e = 0, dense_h = 0;
   for (int l = 0; l < 12; l++)
-    if (h->lanes[l].rows && !h->lanes[l].index) (l < 6 ? eb : hb) += 4.0;
-  out[0] = hb * (double)h->N;
-  out[1] = eb * (double)h->N;
-  // CPML slabs per step: target r+w, psi r+w, the two source values and
-  // the coefficient, 4 B each (cache hits for the neighbour not counted).
+    if (h->lanes[l].rows && !h->lanes[l].index) (l < 6 ? dense_e : dense_h) += 4.0;
   double psi = 0;
-  for (auto &s : h->slabs) psi += 24.0 * (double)s.len;
-  out[2] = psi;
+  for (auto &s : h->slabs) psi += 8.0 * (double)s.len;
+  if (use_fused(h)) {
+    out[0] = (48.0 + dense_e + dense_h) * (double)h->N + psi;
+    out[1] = 0.0;
+  } else {
+    out[0] = (36.0 + dense_h) * (double)h->N;
+    out[1] = (36.0 + dense_e) * (double)h->N;
+  }
+  // CPML slab launches of an unfused step: target r+w, psi r+w, two source
+  // values and the coefficient, 4 B each.
+  double cp = 0;
+  for (auto &s : h->slabs) cp += 28.0 * (double)s.len;
+  out[2] = use_fused(h) ? 0.0 : cp;
+  return EMS_OK;
+}
+
+int ems_gpu_copy_bandwidth(ems_gpu *h, uint64_t bytes, int reps, double *gbs) {
+  ALIVE(h);
+  if (!gbs || reps < 1 || bytes < 4096) return EMS_E_INVALID;
+  const size_t n4 = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  CK(h, hipMalloc(&a, n4 * 16));
+  CK(h, hipMalloc(&b, n4 * 16));
+  CK(h, hipMemsetAsync(a, 0, n4 * 16, h->stream));
+  hipEvent_t e0, e1;
+  CK(h, hipEventCreate(&e0));
+  CK(h, hipEventCreate(&e1));
+  const unsigned blocks = (unsigned)std::min<size_t>((n4 + 255) / 256, 1u << 20);
+  hipLaunchKernelGGL(copy_f4, dim3(blocks), dim3(256), 0, h->stream, a, b, n4);
+  CK(h, hipEventRecord(e0, h->stream));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(copy_f4, dim3(blocks), dim3(256), 0, h->stream, a, b, n4);
+  CK(h, hipEventRecord(e1, h->stream));
+  CK(h, hipEventSynchronize(e1));
+  float ms = 0;
+  CK(h, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+  CK(h, hipFree(a));
+  CK(h, hipFree(b));
+  *gbs = 2.0 * (double)(n4 * 16) * reps / (ms * 1e-3) / 1e9;
   return EMS_OK;
 }
 

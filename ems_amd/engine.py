@@ -792,6 +792,38 @@ class Engine:
         self._check(nat.ems_gpu_time_kernels(self._h, int(steps), out))
         return tuple(out)
 
+    def set_schedule(self, fused=True, tile_rows: int = 0, x_chunks: int = 0) -> None:
+        """Fused (one pass per step, double-buffered fields; 2 = without the
+        next-plane prefetch) or unfused (H, CPML-H, E, CPML-E launches)
+        steps; the fused tile height (3, 7 or 15 rows) and the number of x
+        chunks (0 = automatic)."""
+        mode = int(fused) if not isinstance(fused, bool) else (1 if fused else 0)
+        self._check(nat.ems_gpu_set_schedule(self._h, mode, int(tile_rows), int(x_chunks)))
+
+    def autotune(self, candidates=None, steps_each: int = 4):
+        """Times a few real steps of each fused tiling on this engine (they
+        advance the simulation like any other steps) and keeps the fastest.
+        Candidates are (mode, tile_rows, x_chunks), mode 1 with and 2
+        without the next-plane prefetch. Returns the choice with its mean
+        kernel time in ms, and all timings."""
+        if candidates is None:
+            candidates = [(m, t, c) for m in (1, 2) for t in (3, 7, 15) for c in (0, 2, 4, 8, 16)]
+        results = []
+        for m, t, c in candidates:
+            self.set_schedule(m, t, c)
+            self.time_kernels(1)  # first launch of a tiling: not timed
+            ms = self.time_kernels(steps_each)[0]
+            results.append((m, t, c, ms))
+        best = min(results, key=lambda r: r[3])
+        self.set_schedule(best[0], best[1], best[2])
+        return best, results
+
+    def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
+        """Device-to-device copy bandwidth (GB/s of read + write)."""
+        out = C.c_double()
+        self._check(nat.ems_gpu_copy_bandwidth(self._h, int(nbytes), int(reps), C.byref(out)))
+        return out.value
+
     def traffic_model(self) -> Tuple[float, float, float]:
         out = (C.c_double * 3)()
         self._check(nat.ems_gpu_traffic_model(self._h, out))

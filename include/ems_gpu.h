@@ -118,19 +118,34 @@ int ems_gpu_probe_times(ems_gpu *h, int id, double *out, uint64_t cap, uint64_t 
 /* Engine::dt (src/engine/mod.rs). */
 double ems_gpu_dt(const ems_gpu *h);
 
+/* Step schedule (no reference counterpart; the reference picks fused steps
+ * by itself, pool.rs:131-164): fused 1 runs H, CPML-H, E and CPML-E of a
+ * step in one pass over HBM with double-buffered fields (bit-identical to
+ * the unfused schedule; 2 is the same without the next-plane prefetch);
+ * tile_rows 3, 7 or 15 (0 keeps the current);
+ * x_chunks 0 = automatic. Steps that need the serial gap between the H and
+ * E updates (H sources) always run unfused. */
+int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks);
+
 /* Waits for all queued device work. */
 int ems_gpu_synchronize(ems_gpu *h);
 
 /* Measurement hook (no reference counterpart): runs `steps` steps with HIP
  * events around the main stencil kernels on the engine's stream and
- * returns their mean durations in ms: out[0] H update, out[1] E update,
- * out[2] whole step. */
+ * returns their mean durations in ms: out[0] the fused step kernel (or the
+ * H update when unfused), out[1] the E update (0 when fused), out[2] the
+ * whole step. */
 int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]);
 
 /* Algorithmic HBM bytes of one launch of each stencil kernel, from the lanes
- * and slabs the engine holds (no reference counterpart): out[0] H update,
- * out[1] E update, out[2] all CPML slab launches of one step. */
+ * and slabs the engine holds (no reference counterpart): out[0] the fused
+ * step kernel (or the H update when unfused), out[1] the E update (0 when
+ * fused), out[2] all CPML slab launches of one unfused step. */
 int ems_gpu_traffic_model(ems_gpu *h, double out[3]);
+
+/* Measurement hook (no reference counterpart): device-to-device copy of
+ * `bytes` (read + write), `reps` times; *gbs = bytes moved / time. */
+int ems_gpu_copy_bandwidth(ems_gpu *h, uint64_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

@@ -30,13 +30,19 @@ def graded(length: float, cells: int, alpha: float):
             for q in range(cells + 1)]
 
 
+# Schedule of the device engines the tests build: (fused, tile_rows, x_chunks).
+SCHEDULE = {"fused": 1, "tile_rows": 0, "x_chunks": 0}
+
+
 def make(kind: str, grid, mats=None, faces=None, ts=None, threads=1, curl_order=0, pool=True):
     faces = faces or [E.FaceCondition.Pec] * 6
     ts = ts or E.TimeStep.courant(0.99)
     dt = ts.resolve(grid)
     lanes = E.coefficients(grid, mats, faces, dt)
     if kind == "gpu":
-        return E.Engine(grid, dt, lanes, curl_order=curl_order, pool=pool)
+        e = E.Engine(grid, dt, lanes, curl_order=curl_order, pool=pool)
+        e.set_schedule(SCHEDULE["fused"], SCHEDULE["tile_rows"], SCHEDULE["x_chunks"])
+        return e
     from oracle.bridge import OracleEngine
     return OracleEngine(grid, dt, lanes, curl_order=curl_order, threads=threads, pool=pool)
 

@@ -9,7 +9,22 @@ import pytest
 import ems_amd as E
 from cases import det_model, graded, make, pulse, small_cavity
 
+import cases
+
 pytestmark = pytest.mark.gpu
+
+# Every parity test runs on the unfused schedule and on the fused one with
+# each tile height and with many x chunks (chunk seams inside the box).
+SCHEDULES = [(0, 0, 0), (1, 3, 0), (1, 7, 0), (1, 15, 0), (1, 7, 5), (2, 7, 3), (2, 15, 0)]
+
+
+@pytest.fixture(autouse=True, params=SCHEDULES, ids=lambda s: "mode%d_tj%d_xc%d" % s)
+def schedule(request):
+    f, t, c = request.param
+    old = dict(cases.SCHEDULE)
+    cases.SCHEDULE.update(fused=f, tile_rows=t, x_chunks=c)
+    yield
+    cases.SCHEDULE.update(old)
 
 
 def bits(a):
