@@ -127,6 +127,8 @@ ems_cpml_coefficients = _sig("ems_cpml_coefficients", None, C.POINTER(CpmlParams
 ems_cpml_layer_index = _sig("ems_cpml_layer_index", _d, _dp, _i, _dp, _i, _dp, _i, _fp, _d, _i, _i)
 ems_cpml_layer = _sig("ems_cpml_layer", _i, _dp, _i, _dp, _i, _dp, _i, _i, C.POINTER(CpmlParams),
                       _d, _d, C.POINTER(CpmlSlab), _fp)
+ems_cpml_profile = _sig("ems_cpml_profile", _i, _dp, _i, _dp, _i, _dp, _i, _i,
+                        C.POINTER(CpmlParams), _d, _d, _i, _dp, C.POINTER(_i))
 ems_gaussian = _sig("ems_gaussian", _d, _d, _d, _d)
 ems_gaussian_duration = _sig("ems_gaussian_duration", _d, _d)
 ems_gaussian_table = _sig("ems_gaussian_table", None, _d, _d, _d, _d, _u64, _fp)

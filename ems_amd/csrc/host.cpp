@@ -318,19 +318,22 @@ double ems_cpml_layer_index(const double *x, int nx, const double *y, int ny, co
   return n_r;
 }
 
-int ems_cpml_layer(const double *x, int nx, const double *y, int ny, const double *z, int nz,
-                   int face, const ems_cpml_params *prm, double dt, double n_face,
-                   ems_cpml_slab out[4], float *coeff) {
-  if (face < 0 || face > 5 || !prm || !out || !coeff) return EMS_E_INVALID;
-  const Grid g = make_grid(x, nx, y, ny, z, nz);
+// Layer::new (cpml.rs:1122-1207): depth positions and grading of the E
+// side (the layer's lines, outer face included, interface excluded) and the
+// H side (the midpoints of its cells), in increasing index order.
+static int layer_profiles(const Grid &g, int face, const ems_cpml_params *prm, double dt,
+                          double n_face, std::vector<int> &e_idx, std::vector<int> &h_idx,
+                          std::vector<double> &ec, std::vector<double> &hc) {
   const int p = face / 2;
   const bool is_max = face % 2 == 1;
   const int np = g.n[p];
   const int c = prm->cells;
   if (c < 1 || c > np - 1) return EMS_E_INVALID;
   const int inner = is_max ? np - 1 - c : c;
-  auto rho = [&](double v) { return std::clamp(std::fabs(v - (double)inner) / (double)c, 0.0, 1.0); };
-  std::vector<int> e_idx, h_idx;
+  auto rho = [&](double v) {
+    return std::clamp(std::fabs(v - (double)inner) / (double)c, 0.0, 1.0);
+  };
+  e_idx.clear(), h_idx.clear();
   if (is_max) {
     for (int k = inner + 1; k < np; k++) e_idx.push_back(k);
     for (int k = inner; k < np - 1; k++) h_idx.push_back(k);
@@ -342,12 +345,42 @@ int ems_cpml_layer(const double *x, int nx, const double *y, int ny, const doubl
     if (lo && hi) return 0.5 * (g.delta(p, k - 1) + g.delta(p, k));
     return lo ? g.delta(p, k - 1) : g.delta(p, k);
   };
-  std::vector<double> ec(7 * e_idx.size()), hc(7 * h_idx.size());
+  ec.assign(7 * e_idx.size(), 0.0);
+  hc.assign(7 * h_idx.size(), 0.0);
   for (size_t q = 0; q < e_idx.size(); q++)
     ems_cpml_coefficients(prm, rho((double)e_idx[q]), node_delta(e_idx[q]), dt, n_face, &ec[7 * q]);
   for (size_t q = 0; q < h_idx.size(); q++)
     ems_cpml_coefficients(prm, rho((double)h_idx[q] + 0.5), g.delta(p, h_idx[q]), dt, n_face,
                           &hc[7 * q]);
+  return EMS_OK;
+}
+
+int ems_cpml_profile(const double *x, int nx, const double *y, int ny, const double *z, int nz,
+                     int face, const ems_cpml_params *prm, double dt, double n_face, int e_side,
+                     double *out, int *count) {
+  if (face < 0 || face > 5 || !prm) return EMS_E_INVALID;
+  const Grid g = make_grid(x, nx, y, ny, z, nz);
+  std::vector<int> e_idx, h_idx;
+  std::vector<double> ec, hc;
+  int rc = layer_profiles(g, face, prm, dt, n_face, e_idx, h_idx, ec, hc);
+  if (rc) return rc;
+  const std::vector<double> &v = e_side ? ec : hc;
+  if (out) std::memcpy(out, v.data(), v.size() * sizeof(double));
+  if (count) *count = (int)(v.size() / 7);
+  return EMS_OK;
+}
+
+int ems_cpml_layer(const double *x, int nx, const double *y, int ny, const double *z, int nz,
+                   int face, const ems_cpml_params *prm, double dt, double n_face,
+                   ems_cpml_slab out[4], float *coeff) {
+  if (face < 0 || face > 5 || !prm || !out || !coeff) return EMS_E_INVALID;
+  const Grid g = make_grid(x, nx, y, ny, z, nz);
+  const int p = face / 2;
+  const int np = g.n[p];
+  std::vector<int> e_idx, h_idx;
+  std::vector<double> ec, hc;
+  int rc = layer_profiles(g, face, prm, dt, n_face, e_idx, h_idx, ec, hc);
+  if (rc) return rc;
   const int a1 = nxt(p), a2 = prv(p);
   auto fill = [&](int q, int e_side, int comp, const std::vector<int> &idx,
                   const std::vector<double> &co) {

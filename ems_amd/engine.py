@@ -426,9 +426,11 @@ class CpmlBoundary:
     (face, Cpml), sigma_max tuned to each layer's mean refractive index
     (layer_index) measured from the engine's coefficients."""
 
-    def __init__(self, slabs: List[dict], faces: List[Tuple[Face, Cpml, float]]):
+    def __init__(self, slabs: List[dict], faces: List[Tuple[Face, Cpml, float]], grid=None,
+                 dt=None):
         self.slabs = slabs
         self.faces = faces
+        self._grid, self._dt = grid, dt
 
     @staticmethod
     def new(engine: "Engine", layers) -> "CpmlBoundary":
@@ -481,13 +483,35 @@ class CpmlBoundary:
         for p in range(3):
             if cells[2 * p] + cells[2 * p + 1] > n[p] - 1:
                 raise EmsError(nat.EMS_E_INVALID, f"CPML: opposite layers along axis {p} overlap")
-        return CpmlBoundary(slabs, faces)
+        return CpmlBoundary(slabs, faces, grid, dt)
 
     def refractive_index(self, face: Face) -> Optional[float]:
         for f, _, nf in self.faces:
             if f == face:
                 return nf
         return None
+
+    def _profile(self, face: Face, e_side: int):
+        for f, params, nf in self.faces:
+            if f == face:
+                g = self._grid
+                np_ = g.nodes()[Face(face).axis]
+                out = np.zeros(7 * np_, dtype=np.float64)
+                cnt = C.c_int()
+                prm = params.c_params()
+                nat.check(nat.ems_cpml_profile(*g._args(), int(face), C.byref(prm), self._dt, nf,
+                                               e_side, nat.ptr(out, C.c_double), C.byref(cnt)))
+                keys = ("rho", "sigma", "kappa", "alpha", "b", "a", "direct")
+                return [dict(zip(keys, out[7 * q:7 * q + 7])) for q in range(cnt.value)]
+        return None
+
+    def e_profile(self, face: Face):
+        """Grading at the E positions of `face` (cpml.rs:1005-1011)."""
+        return self._profile(face, 1)
+
+    def h_profile(self, face: Face):
+        """Grading at the H positions of `face` (cpml.rs:1013-1018)."""
+        return self._profile(face, 0)
 
 
 # ------------------------------------------------------------ run loop --

@@ -67,6 +67,14 @@ void ems_cpml_coefficients(const ems_cpml_params *p, double rho, double delta, d
 double ems_cpml_layer_index(const double *x, int nx, const double *y, int ny, const double *z,
                             int nz, const float *lanes, double dt, int face, int cells);
 
+/* CpmlBoundary::e_profile / h_profile (cpml.rs:1005-1018): the grading of
+ * the layer on `face` at its E positions (e_side 1: the layer's lines, outer
+ * face included, interface excluded) or H positions (cell midpoints), in
+ * increasing index order; 7 doubles each as ems_cpml_coefficients. */
+int ems_cpml_profile(const double *x, int nx, const double *y, int ny, const double *z, int nz,
+                     int face, const ems_cpml_params *p, double dt, double n_face, int e_side,
+                     double *out, int *count);
+
 /* Layer::new (cpml.rs:1122-1207): the four slabs of one face layer, in the
  * order E(a1), E(a2), H(a1), H(a2) with (a1, a2) the face's transverse
  * axes. b/a/k of slab q are written to coeff[q*3*np + {0,1,2}*np ..] with

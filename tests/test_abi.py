@@ -1,0 +1,32 @@
+"""The C-ABI library loads and exports every function include/*.h declares
+(no device calls)."""
+import ctypes as C
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ems_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(os.path.join(ROOT, "ems_amd", "libems_gpu.so"))
+    names = declared("ems_gpu.h") + declared("ems_host.h")
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_invalid_arguments_are_rejected_without_a_device():
+    import ems_amd._native as nat
+    assert nat.ems_gpu_create(None, None) == nat.EMS_E_INVALID
+    d = nat.GridDesc((C.c_int32 * 3)(1, 4, 4), 1e-12, 0)
+    h = C.c_void_p()
+    assert nat.ems_gpu_create(C.byref(d), C.byref(h)) == nat.EMS_E_INVALID
+    d = nat.GridDesc((C.c_int32 * 3)(4, 4, 4), -1.0, 0)
+    assert nat.ems_gpu_create(C.byref(d), C.byref(h)) == nat.EMS_E_INVALID
+    assert nat.ems_gpu_run(None, 1) == nat.EMS_E_INVALID
