@@ -88,6 +88,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-steps", type=int, default=20)
     ap.add_argument("--unfused", action="store_true", help="unfused schedule (H, CPML, E launches)")
+    ap.add_argument("--mode", type=int, default=1,
+                    help="fused kernel with --tile-rows: 1 tiled, 2 tiled without prefetch, 3 full rows")
     ap.add_argument("--tile-rows", type=int, default=0, help="fused tile height (3, 7, 15)")
     ap.add_argument("--x-chunks", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
@@ -121,7 +123,7 @@ def main() -> int:
     if args.unfused:
         eng.set_schedule(False)
     elif args.tile_rows or args.no_autotune:
-        eng.set_schedule(True, args.tile_rows, args.x_chunks)
+        eng.set_schedule(args.mode, args.tile_rows, args.x_chunks)
     else:
         # The autotuner's trial steps are part of the (untimed) warm-up.
         t0 = eng.timestep()
@@ -177,7 +179,9 @@ def main() -> int:
         "config": {"workload": wl.name, "cells": list(n), "nodes": list(wl.grid.nodes()),
                    "cpml_cells": 8 if args.workload == "cpml_box" else 0,
                    "probes": len(probes), "parallelism": "single GPU",
-                   "schedule": "unfused" if args.unfused else "fused", "autotune": tuned},
+                   "schedule": "unfused" if args.unfused else "fused",
+                   "mode": None if args.unfused else (tuned["mode"] if tuned else args.mode),
+                   "autotune": tuned},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

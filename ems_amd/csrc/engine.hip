@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -279,7 +280,7 @@ __device__ __forceinline__ void cpml_node(const FSlab *__restrict__ slabs, uint3
     const int q = __builtin_ctz(mask);
     mask &= mask - 1;
     const FSlab &s = slabs[q];
-    if (i < s.lo0 || i >= s.hi0) continue;
+    // (The plane is inside the slab's x range: the caller's mask says so.)
     if (j < s.lo1 || j >= s.hi1 || k < s.lo2 || k >= s.hi2) continue;
     const int z = s.normal == 0 ? i - s.lo0 : (s.normal == 1 ? j - s.lo1 : k - s.lo2);
     const size_t qq = ((size_t)(i - s.lo0) * s.ly + (j - s.lo1)) * s.lz + (k - s.lo2);
@@ -341,23 +342,26 @@ __device__ __forceinline__ void load_plane(PlaneIn &P, const float *F, size_t N,
                                            bool jv, bool kv) {
   // Branch-free, every address valid (neighbour offsets fall back to the
   // node itself where the neighbour does not exist; the value is then
-  // replaced by 0): all loads sit in one basic block with the zero
-  // extension of the lane offset, and use the saddr form.
+  // replaced by 0). One wave-uniform base per component array; the
+  // neighbours are vector offsets from it, so the scalar unit computes six
+  // bases per plane and nothing per neighbour.
   asm volatile("" : "+v"(k4));
   const unsigned kn = k4 + (kv ? 4u : 0u);
-  const size_t jn = jv ? (size_t)nz : 0, in = iv ? plane : 0;
+  const unsigned kj = k4 + (jv ? 4u * (unsigned)nz : 0u);
+  const unsigned ki = k4 + (iv ? 4u * (unsigned)plane : 0u);
   const float *ux = F + ob, *uy = F + N + ob, *uz = F + 2 * N + ob;
-  const float uxc = ldf(ux, k4), uxj = ldf(ux + jn, k4), uzj = ldf(uz + jn, k4);
+  const float uxc = ldf(ux, k4), uxj = ldf(ux, kj), uzj = ldf(uz, kj);
   const float uxk = ldf(ux, kn), uyk = ldf(uy, kn);
-  const float uyi = ldf(uy + in, k4), uzi = ldf(uz + in, k4);
+  const float uyi = ldf(uy, ki), uzi = ldf(uz, ki);
   P.ix = ldf(F + 3 * N + ob, k4);
   P.iy = ldf(F + 4 * N + ob, k4);
   P.iz = ldf(F + 5 * N + ob, k4);
   // One base for all lanes and a 32-bit byte offset per lane (the pool is
-  // kept under 4 GiB), so the coefficients need no 64-bit address math.
+  // kept under 4 GiB; croff holds byte offsets), so the coefficients need
+  // no 64-bit address math.
   const uint32_t *ro = croff + 16 * r;
 #pragma unroll
-  for (int l = 0; l < 12; l++) P.c[l] = ldf(cpool, ro[l] * 4u + k4);
+  for (int l = 0; l < 12; l++) P.c[l] = ldf(cpool, ro[l] + k4);
   P.uxc = uxc;
   P.ux_j1 = jv ? uxj : 0.0f;
   P.uz_j1 = jv ? uzj : 0.0f;
@@ -389,14 +393,29 @@ template <bool OE, int TJ, bool PF>
 __global__ void __launch_bounds__(64 * (TJ + 1))
     step_fused(const float *__restrict__ A, float *__restrict__ B, size_t N,
                const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
-               const FSlab *__restrict__ slabs, int nslab, int par, int nx, int ny, int nz, int tk,
-               int chunk, int ktiles, int jtiles, int ntiles) {
+               const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
+               int par, int nx, int ny, int nz, int tk, int chunk, int ktiles, int jtiles,
+               int ntiles, int xcd_group) {
   // XCD-aware order: workgroups go round-robin to the 8 XCDs, so give each
   // XCD a contiguous range of tiles (k fastest, then j, then x chunk) and
   // the halo rows its neighbours read stay in its own L2.
+  // Tiles come in rows of ktiles k-neighbours; rows are numbered
+  // R = c*jtiles + jt (j-neighbours adjacent). Groups of `xcd_group`
+  // consecutive rows go round-robin to the 8 XCDs (workgroups are handed to
+  // the XCDs round-robin), so k-neighbours and most j-neighbours share an
+  // L2 while every XCD gets a share of every x chunk. The grid is padded
+  // to whole groups of 8; padding blocks exit at once.
   const int b = blockIdx.x;
-  const int q8 = ntiles / 8, r8 = ntiles % 8, x8 = b % 8;
-  const int t = x8 * q8 + min(x8, r8) + b / 8;
+  int t;
+  if (xcd_group > 0) {
+    const int x8 = b % 8, sl = b / 8;
+    const int lr = sl / ktiles, ktl = sl % ktiles;
+    const int R = ((lr / xcd_group) * 8 + x8) * xcd_group + lr % xcd_group;
+    if (R >= ntiles / ktiles) return;
+    t = R * ktiles + ktl;
+  } else {
+    t = b;
+  }
   const int kt = t % ktiles;
   const int t2 = t / ktiles;
   const int jt = t2 % jtiles;
@@ -444,7 +463,8 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
       const float uyp = ldf(A + N + (ob - plane), k4), uzp = ldf(A + 2 * N + (ob - plane), k4);
       if (inb) {
         float h[3];
-        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh, par, false, i0 - 1, j, k, h);
+        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh & pmask[2 * (i0 - 1)], par, false, i0 - 1,
+                   j, k, h);
         hy_prev = h[1];
         hz_prev = h[2];
       }
@@ -465,8 +485,10 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
     PlaneIn Nx;
     if (PF && i + 1 < i1)
       load_plane(Nx, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv);
+    // Slabs that reach this tile and this plane (bit masks, scalar).
+    const uint32_t mhp = mh & pmask[2 * i], mep = me & pmask[2 * i + 1];
     float h[3] = {0.0f, 0.0f, 0.0f};
-    if (inb) h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mh, par, own, i, j, k, h);
+    if (inb) h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mhp, par, own, i, j, k, h);
     sh[buf][0][ty][tx] = h[0];
     sh[buf][1][ty][tx] = h[1];
     sh[buf][2][ty][tx] = h[2];
@@ -483,9 +505,9 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
       if (jv) e1 = e_expr<OE>(P.c[1], e1, P.c[4], h[0], hx_km, h[2], hz_prev);
       // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
       if (kv) e2 = e_expr<OE>(P.c[2], e2, P.c[5], h[1], hy_prev, h[0], hx_jm);
-      if (me)
+      if (mep)
         // S_p = i_other[node] - i_other[node - e_p] (cpml.rs:495-521).
-        cpml_node<true>(slabs, me, par, true, i, j, k, h[2] - hz_jm, h[1] - hy_km,
+        cpml_node<true>(slabs, mep, par, true, i, j, k, h[2] - hz_jm, h[1] - hy_km,
                         h[2] - hz_prev, h[0] - hx_km, h[1] - hy_prev, h[0] - hx_jm, P.c[3],
                         P.c[4], P.c[5], e0, e1, e2);
       stf(B + ob, k4, e0);
@@ -508,7 +530,151 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
   }
 }
 
-// croff[16*r + l]: offset in the pool of lane l's row for node row r.
+// ---- fused step, full-row variant ----
+//
+// One wave per j-row covering all k in NSEG segments of 64 lanes (k = 64m +
+// lane), so a row is read and written contiguously by one wave and no cache
+// line is shared between tiles along k. A block holds TJ rows plus the halo
+// row j0-1 and marches a chunk of x-planes. Per plane: H of every segment of
+// every row (halo row included, CPML-corrected) into LDS; barrier; E of the
+// owned rows from LDS (k-1 and j-1 neighbours) and the previous plane's
+// y/z currents kept in registers; barrier. Same expressions and inputs as
+// the other schedules, so the same bits.
+template <bool OE, int TJ, int NSEG>
+__global__ void __launch_bounds__(64 * (TJ + 1))
+    step_rows(const float *__restrict__ A, float *__restrict__ B, size_t N,
+              const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
+              const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
+              int par, int nx, int ny, int nz, int chunk, int jtiles, int ntiles, int xcd_group) {
+  // Tiles are (x chunk, j tile), j fastest; groups of xcd_group consecutive
+  // tiles go round-robin to the 8 XCDs (halo rows shared within a group).
+  const int b = blockIdx.x;
+  int t;
+  if (xcd_group > 0) {
+    const int x8 = b % 8, sl = b / 8;
+    t = ((sl / xcd_group) * 8 + x8) * xcd_group + sl % xcd_group;
+    if (t >= ntiles) return;
+  } else {
+    t = b;
+  }
+  const int jt = t % jtiles, c = t / jtiles;
+  const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int j0 = jt * TJ, j = j0 - 1 + w;
+  const bool row_ok = j >= 0 && j < ny;
+  const bool own_row = row_ok && w > 0;
+  const int jc = min(max(j, 0), ny - 1);
+  const bool jv = jc < ny - 1;
+  const int i0 = c * chunk, i1 = min(nx, i0 + chunk);
+  const size_t plane = (size_t)ny * nz;
+  constexpr int W = NSEG * 64;
+  __shared__ float sh[3][TJ + 1][W];
+  // CPML slabs whose j range meets the tile (k is checked per lane).
+  uint32_t mh = 0, me = 0;
+  for (int q = 0; q < nslab; q++) {
+    const FSlab &s = slabs[q];
+    if (s.lo1 < j0 + TJ && s.hi1 > j0 - 1) {
+      if (s.e_side) me |= 1u << q;
+      else mh |= 1u << q;
+    }
+  }
+  float hy_prev[NSEG], hz_prev[NSEG];
+#pragma unroll
+  for (int m = 0; m < NSEG; m++) hy_prev[m] = hz_prev[m] = 0.0f;
+  if (i0 > 0) {
+    // H of plane i0-1 (read-only: its owner writes it).
+    const int ip = i0 - 1;
+    const size_t ob = ((size_t)ip * ny + jc) * nz, r = (size_t)ip * ny + jc;
+    const uint32_t mhp = mh & pmask[2 * ip];
+#pragma unroll
+    for (int m = 0; m < NSEG; m++) {
+      const int k = 64 * m + lane;
+      const bool inb = row_ok && k < nz;
+      const unsigned ku = (unsigned)min(k, nz - 1);
+      const bool kv = (int)ku < nz - 1;
+      PlaneIn Q;
+      load_plane(Q, A, N, cpool, croff, ob, r, plane, nz, ku * 4u, true, jv, kv);
+      const float uyc = ldf(A + N + ob, ku * 4u), uzc = ldf(A + 2 * N + ob, ku * 4u);
+      if (inb) {
+        float h[3];
+        h_from<OE>(Q, uyc, uzc, true, jv, kv, slabs, mhp, par, false, ip, j, k, h);
+        hy_prev[m] = h[1];
+        hz_prev[m] = h[2];
+      }
+    }
+  }
+  for (int i = i0; i < i1; i++) {
+    const size_t r = (size_t)i * ny + jc;
+    const size_t ob = r * nz;
+    const bool iv = i + 1 < nx;
+    const uint32_t mhp = mh & pmask[2 * i], mep = me & pmask[2 * i + 1];
+    // H of every segment of this row.
+#pragma unroll
+    for (int m = 0; m < NSEG; m++) {
+      const int k = 64 * m + lane;
+      const bool inb = row_ok && k < nz;
+      const unsigned ku = (unsigned)min(k, nz - 1);
+      const bool kv = (int)ku < nz - 1;
+      PlaneIn P;
+      load_plane(P, A, N, cpool, croff, ob, r, plane, nz, ku * 4u, iv, jv, kv);
+      const float uyc = ldf(A + N + ob, ku * 4u), uzc = ldf(A + 2 * N + ob, ku * 4u);
+      float h[3] = {0.0f, 0.0f, 0.0f};
+      if (inb) h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mhp, par, own_row, i, j, k, h);
+      sh[0][w][64 * m + lane] = h[0];
+      sh[1][w][64 * m + lane] = h[1];
+      sh[2][w][64 * m + lane] = h[2];
+      if (own_row && k < nz) {
+        unsigned k4 = ku * 4u;
+        asm volatile("" : "+v"(k4));
+        stf(B + 3 * N + ob, k4, h[0]);
+        stf(B + 4 * N + ob, k4, h[1]);
+        stf(B + 5 * N + ob, k4, h[2]);
+      }
+    }
+    __syncthreads();
+    if (own_row) {
+#pragma unroll
+      for (int m = 0; m < NSEG; m++) {
+        const int k = 64 * m + lane;
+        const unsigned ku = (unsigned)min(k, nz - 1);
+        unsigned k4 = ku * 4u;
+        asm volatile("" : "+v"(k4));
+        const float hx = sh[0][w][64 * m + lane], hy = sh[1][w][64 * m + lane];
+        const float hz = sh[2][w][64 * m + lane];
+        if (k < nz) {
+          const bool kv = k < nz - 1;
+          const float hx_km = k > 0 ? sh[0][w][64 * m + lane - 1] : 0.0f;
+          const float hy_km = k > 0 ? sh[1][w][64 * m + lane - 1] : 0.0f;
+          const float hx_jm = j > 0 ? sh[0][w - 1][64 * m + lane] : 0.0f;
+          const float hz_jm = j > 0 ? sh[2][w - 1][64 * m + lane] : 0.0f;
+          const uint32_t *ro = croff + 16 * r;
+          float e0 = ldf(A + ob, k4), e1 = ldf(A + N + ob, k4), e2 = ldf(A + 2 * N + ob, k4);
+          const float ca0 = ldf(cpool, ro[0] + k4), ca1 = ldf(cpool, ro[1] + k4);
+          const float ca2 = ldf(cpool, ro[2] + k4), cb0 = ldf(cpool, ro[3] + k4);
+          const float cb1 = ldf(cpool, ro[4] + k4), cb2 = ldf(cpool, ro[5] + k4);
+          // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
+          if (iv) e0 = e_expr<OE>(ca0, e0, cb0, hz, hz_jm, hy, hy_km);
+          // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
+          if (jv) e1 = e_expr<OE>(ca1, e1, cb1, hx, hx_km, hz, hz_prev[m]);
+          // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
+          if (kv) e2 = e_expr<OE>(ca2, e2, cb2, hy, hy_prev[m], hx, hx_jm);
+          if (mep)
+            cpml_node<true>(slabs, mep, par, true, i, j, k, hz - hz_jm, hy - hy_km,
+                            hz - hz_prev[m], hx - hx_km, hy - hy_prev[m], hx - hx_jm, cb0, cb1,
+                            cb2, e0, e1, e2);
+          stf(B + ob, k4, e0);
+          stf(B + N + ob, k4, e1);
+          stf(B + 2 * N + ob, k4, e2);
+        }
+        hy_prev[m] = hy;
+        hz_prev[m] = hz;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// croff[16*r + l]: byte offset in the pool of lane l's row for node row r.
 __global__ void build_row_offsets(uint32_t *croff, const uint32_t *const *index,
                                   const uint32_t *base, size_t rows, int nz) {
   const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -518,7 +684,7 @@ __global__ void build_row_offsets(uint32_t *croff, const uint32_t *const *index,
     if (l < 12) {
       const uint32_t *ix = index[l];
       // Lanes without rows point at the pool's zero row (base[l] == 0).
-      if (base[l] != 0) v = base[l] + (ix ? ix[r] : (uint32_t)r) * (uint32_t)nz;
+      if (base[l] != 0) v = 4u * (base[l] + (ix ? ix[r] : (uint32_t)r) * (uint32_t)nz);
     }
     croff[16 * r + l] = v;
   }
@@ -613,35 +779,56 @@ __device__ Gaps gaps_at(const DevOp *ops, int nops, uint64_t n) {
   return g;
 }
 
-__global__ void gap_mid(Fields f, const DevOp *ops, int nops, const uint64_t *step) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const uint64_t n = *step;
-  Gaps g = gaps_at(ops, nops, n);
-  if (!g.gap_b) return;
-  for (int q = 0; q < nops; q++) {
+// The hooks of one gap in registration order. Probes only read the fields,
+// so a run of consecutive probes samples in parallel (a thread each);
+// sources write and run one after another, with a barrier on either side.
+// `which` selects the ops of the gap: 0 the mid gap (H sources, current
+// probes), 1 the post-E gap (E sources, voltage probes).
+__device__ void run_gap(Fields f, const DevOp *ops, int nops, uint64_t n, int which) {
+  int q = 0;
+  while (q < nops) {
     const DevOp &o = ops[q];
-    if (is_hsrc(o)) source_apply(f, o, n);
-    if (is_iprobe(o) && n % o.interval == 0) probe_sample(f, o);
+    const bool src = which == 0 ? is_hsrc(o) : is_esrc(o);
+    if (src) {
+      if (threadIdx.x == 0) source_apply(f, o, n);
+      __syncthreads();
+      q++;
+      continue;
+    }
+    int q1 = q;
+    while (q1 < nops && !(which == 0 ? is_hsrc(ops[q1]) : is_esrc(ops[q1]))) q1++;
+    for (int p = q + (int)threadIdx.x; p < q1; p += (int)blockDim.x) {
+      const DevOp &pr = ops[p];
+      const bool mine = which == 0 ? is_iprobe(pr) : is_vprobe(pr);
+      if (mine && n % pr.interval == 0) probe_sample(f, pr);
+    }
+    __syncthreads();
+    q = q1;
   }
 }
 
-__global__ void gap_post_e(Fields f, const DevOp *ops, int nops, uint64_t *step) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(256) gap_mid(Fields f, const DevOp *ops, int nops,
+                                               const uint64_t *step) {
   const uint64_t n = *step;
-  Gaps g = gaps_at(ops, nops, n);
+  const Gaps g = gaps_at(ops, nops, n);
+  if (g.gap_b) run_gap(f, ops, nops, n, 0);
+}
+
+__global__ void __launch_bounds__(256) gap_post_e(Fields f, const DevOp *ops, int nops,
+                                                  uint64_t *step) {
+  const uint64_t n = *step;
+  const Gaps g = gaps_at(ops, nops, n);
   if (g.gap_a) {
-    if (g.moved)
-      for (int q = 0; q < nops; q++) {
-        const DevOp &o = ops[q];
-        if (is_iprobe(o) && n % o.interval == 0) probe_sample(f, o);
-      }
-    for (int q = 0; q < nops; q++) {
-      const DevOp &o = ops[q];
-      if (is_esrc(o)) source_apply(f, o, n);
-      if (is_vprobe(o) && n % o.interval == 0) probe_sample(f, o);
+    if (g.moved) {
+      // Moved post_h hooks (current probes) first (mod.rs:936-942).
+      for (int p = threadIdx.x; p < nops; p += blockDim.x)
+        if (is_iprobe(ops[p]) && n % ops[p].interval == 0) probe_sample(f, ops[p]);
+      __syncthreads();
     }
+    run_gap(f, ops, nops, n, 1);
   }
-  *step = n + 1;
+  __syncthreads();
+  if (threadIdx.x == 0) *step = n + 1;
 }
 
 // ---- energy invariant (kernel.rs:119-244, 364-423; pool.rs:673-692) ----
@@ -800,13 +987,15 @@ struct ems_gpu {
   std::vector<void *> lane_mem;
   std::vector<HostSlab> slabs;
   FSlab *d_fslabs = nullptr;
+  uint32_t *d_pmask = nullptr;  // per x-plane: slabs (H, E) whose x range holds it
   std::vector<HostOp> ops;
   DevOp *d_ops = nullptr;
   bool ops_dirty = true, slabs_dirty = true;
   uint64_t step = 0;
   uint64_t *d_step = nullptr;
   // fused schedule and its tiling
-  bool fused = true, prefetch = true;
+  bool fused = true, prefetch = true, rows = false;
+  int xcd_group = getenv("EMS_XCD_GROUP") ? atoi(getenv("EMS_XCD_GROUP")) : 2;
   int tj = 7, nchunk = 0;
   // energy scratch
   double *d_rowsum = nullptr, *d_chains = nullptr, *d_planes = nullptr, *d_w = nullptr;
@@ -916,7 +1105,7 @@ int launch_cpml(ems_gpu *h, const Fields &f, int e_side, int par) {
 }
 
 struct FusedGeom {
-  int tk, ktiles, jtiles, chunk, nchunks, ntiles;
+  int tk, ktiles, jtiles, chunk, nchunks, ntiles, nblocks;
 };
 
 FusedGeom fused_geom(const ems_gpu *h) {
@@ -937,6 +1126,9 @@ FusedGeom fused_geom(const ems_gpu *h) {
   g.chunk = (nx + want - 1) / want;
   g.nchunks = (nx + g.chunk - 1) / g.chunk;
   g.ntiles = g.ktiles * g.jtiles * g.nchunks;
+  const int rows = g.jtiles * g.nchunks, G = h->xcd_group;
+  const int prow = G > 0 ? (rows + 8 * G - 1) / (8 * G) * (8 * G) : rows;
+  g.nblocks = prow * g.ktiles;
   return g;
 }
 
@@ -961,6 +1153,12 @@ int upload_fslabs(ems_gpu *h) {
     CK(h, hipMalloc(&h->d_fslabs, v.size() * sizeof(FSlab)));
     CK(h, hipMemcpy(h->d_fslabs, v.data(), v.size() * sizeof(FSlab), hipMemcpyHostToDevice));
   }
+  std::vector<uint32_t> pm(2 * (size_t)h->n[0], 0u);
+  for (size_t q = 0; q < v.size() && q < 32; q++)
+    for (int i = v[q].lo0; i < v[q].hi0; i++) pm[2 * i + (v[q].e_side ? 1 : 0)] |= 1u << q;
+  if (h->d_pmask) CK(h, hipFree(h->d_pmask));
+  CK(h, hipMalloc(&h->d_pmask, pm.size() * sizeof(uint32_t)));
+  CK(h, hipMemcpy(h->d_pmask, pm.data(), pm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   h->slabs_dirty = false;
   return 0;
 }
@@ -978,17 +1176,72 @@ void launch_fused_tj(ems_gpu *h, int par, const FusedGeom &g) {
   dim3 block(64, TJ + 1);
   const int ns = n_fslabs(h);
   if (h->oe)
-    hipLaunchKernelGGL((step_fused<true, TJ, PF>), dim3(g.ntiles), block, 0, h->stream, A, B, h->N,
-                       (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs, ns, par,
-                       h->n[0], h->n[1], h->n[2], g.tk, g.chunk, g.ktiles, g.jtiles, g.ntiles);
+    hipLaunchKernelGGL((step_fused<true, TJ, PF>), dim3(g.nblocks), block, 0, h->stream, A, B, h->N,
+                       (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
+                       (const uint32_t *)h->d_pmask, ns, par, h->n[0], h->n[1], h->n[2], g.tk,
+                       g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group);
   else
-    hipLaunchKernelGGL((step_fused<false, TJ, PF>), dim3(g.ntiles), block, 0, h->stream, A, B,
-                       h->N, (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs, ns,
-                       par, h->n[0], h->n[1], h->n[2], g.tk, g.chunk, g.ktiles, g.jtiles,
-                       g.ntiles);
+    hipLaunchKernelGGL((step_fused<false, TJ, PF>), dim3(g.nblocks), block, 0, h->stream, A, B,
+                       h->N, (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
+                       (const uint32_t *)h->d_pmask, ns, par, h->n[0], h->n[1], h->n[2], g.tk,
+                       g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group);
+}
+
+// Full-row kernel: one wave per row, rows of up to 9 segments of 64.
+template <bool OE, int TJ, int NSEG>
+void launch_rows_n(ems_gpu *h, int par, int chunk, int jtiles, int ntiles, int nblocks) {
+  const float *A = h->fbuf[par];
+  float *B = h->fbuf[1 - par];
+  hipLaunchKernelGGL((step_rows<OE, TJ, NSEG>), dim3(nblocks), dim3(64, TJ + 1), 0, h->stream, A,
+                     B, h->N, (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
+                     (const uint32_t *)h->d_pmask, n_fslabs(h), par, h->n[0], h->n[1], h->n[2],
+                     chunk, jtiles, ntiles, h->xcd_group);
+}
+
+template <bool OE, int TJ>
+int launch_rows_tj(ems_gpu *h, int par, int chunk, int jtiles, int ntiles, int nblocks) {
+  switch ((h->n[2] + 63) / 64) {
+    case 1: launch_rows_n<OE, TJ, 1>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    case 2: launch_rows_n<OE, TJ, 2>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    case 3: launch_rows_n<OE, TJ, 3>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    case 4: launch_rows_n<OE, TJ, 4>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    case 5: launch_rows_n<OE, TJ, 5>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    case 6: launch_rows_n<OE, TJ, 6>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    case 7: launch_rows_n<OE, TJ, 7>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    case 8: launch_rows_n<OE, TJ, 8>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    case 9: launch_rows_n<OE, TJ, 9>(h, par, chunk, jtiles, ntiles, nblocks); break;
+    default: return fail(h, EMS_E_UNSUPPORTED, "row kernel: nz above 576");
+  }
+  return 0;
+}
+
+// Rows mode needs nz <= 576 (nine segments) and the tile height 3 or 7.
+bool rows_ok(const ems_gpu *h) { return h->n[2] <= 576 && (h->tj == 3 || h->tj == 7); }
+
+int launch_rows(ems_gpu *h, int par) {
+  const int nx = h->n[0], ny = h->n[1];
+  const int jtiles = (ny + h->tj - 1) / h->tj;
+  int want = h->nchunk;
+  if (want <= 0) want = std::max(1, std::min(nx / 16, (1024 + jtiles - 1) / jtiles));
+  want = std::min(want, nx);
+  const int chunk = (nx + want - 1) / want;
+  const int nchunks = (nx + chunk - 1) / chunk;
+  const int ntiles = jtiles * nchunks, G = h->xcd_group;
+  const int nblocks = G > 0 ? (ntiles + 8 * G - 1) / (8 * G) * (8 * G) : ntiles;
+  int rc;
+  if (h->tj == 3)
+    rc = h->oe ? launch_rows_tj<true, 3>(h, par, chunk, jtiles, ntiles, nblocks)
+               : launch_rows_tj<false, 3>(h, par, chunk, jtiles, ntiles, nblocks);
+  else
+    rc = h->oe ? launch_rows_tj<true, 7>(h, par, chunk, jtiles, ntiles, nblocks)
+               : launch_rows_tj<false, 7>(h, par, chunk, jtiles, ntiles, nblocks);
+  if (rc) return rc;
+  CK(h, hipGetLastError());
+  return 0;
 }
 
 int launch_fused(ems_gpu *h, int par) {
+  if (h->rows && rows_ok(h)) return launch_rows(h, par);
   const FusedGeom g = fused_geom(h);
   const int key = h->tj * 2 + (h->prefetch ? 1 : 0);
   switch (key) {
@@ -1005,8 +1258,8 @@ int launch_fused(ems_gpu *h, int par) {
 }
 
 int launch_gap_post_e(ems_gpu *h, const Fields &f) {
-  hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(64), 0, h->stream, f, h->d_ops, (int)h->ops.size(),
-                     h->d_step);
+  hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(256), 0, h->stream, f, h->d_ops,
+                     (int)h->ops.size(), h->d_step);
   CK(h, hipGetLastError());
   return 0;
 }
@@ -1023,8 +1276,8 @@ int enqueue_step(ems_gpu *h, int &par) {
   if ((rc = launch_h(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 0, par))) return rc;
   if (has_mid_ops(h)) {
-    hipLaunchKernelGGL(gap_mid, dim3(1), dim3(64), 0, h->stream, f, h->d_ops, (int)h->ops.size(),
-                       (const uint64_t *)h->d_step);
+    hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops,
+                       (int)h->ops.size(), (const uint64_t *)h->d_step);
     CK(h, hipGetLastError());
   }
   if ((rc = launch_e(h, f))) return rc;
@@ -1315,6 +1568,7 @@ void ems_gpu_destroy(ems_gpu *h) {
     if (s.d_psi[1]) (void)hipFree(s.d_psi[1]);
   }
   if (h->d_fslabs) (void)hipFree(h->d_fslabs);
+  if (h->d_pmask) (void)hipFree(h->d_pmask);
   for (auto &o : h->ops) {
     (void)hipFree(o.d_axis), (void)hipFree(o.d_idx), (void)hipFree(o.d_wf);
     (void)hipFree(o.d_wd), (void)hipFree(o.d_amp), (void)hipFree(o.d_vals);
@@ -1465,8 +1719,10 @@ int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *ax
 
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks) {
   ALIVE(h);
-  // fused 2: the fused kernel without the next-plane prefetch.
+  // fused 2: the fused kernel without the next-plane prefetch; 3: the
+  // full-row fused kernel (nz <= 576, tile rows 3 or 7).
   h->prefetch = fused != 2;
+  h->rows = fused == 3;
   if (tile_rows != 0 && tile_rows != 3 && tile_rows != 7 && tile_rows != 15)
     return fail(h, EMS_E_INVALID, "tile_rows must be 3, 7 or 15");
   if (x_chunks < 0) return fail(h, EMS_E_INVALID, "x_chunks must be >= 0");
@@ -1595,7 +1851,7 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       CK(h, hipEventRecord(ev[1], h->stream));
       if ((rc = launch_cpml(h, f, 0, h->par))) return rc;
       if (has_mid_ops(h)) {
-        hipLaunchKernelGGL(gap_mid, dim3(1), dim3(64), 0, h->stream, f, h->d_ops,
+        hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops,
                            (int)h->ops.size(), (const uint64_t *)h->d_step);
         CK(h, hipGetLastError());
       }

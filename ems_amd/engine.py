@@ -831,7 +831,9 @@ class Engine:
         without the next-plane prefetch. Returns the choice with its mean
         kernel time in ms, and all timings."""
         if candidates is None:
-            candidates = [(m, t, c) for m in (1, 2) for t in (3, 7, 15) for c in (0, 2, 4, 8, 16)]
+            candidates = [(m, t, c) for m in (1, 2) for t in (3, 7, 15) for c in (0, 8, 16, 32)]
+            if self.grid.nodes()[2] <= 576:
+                candidates += [(3, t, c) for t in (3, 7) for c in (0, 4, 8, 16, 32)]
         results = []
         for m, t, c in candidates:
             self.set_schedule(m, t, c)

@@ -121,7 +121,8 @@ double ems_gpu_dt(const ems_gpu *h);
 /* Step schedule (no reference counterpart; the reference picks fused steps
  * by itself, pool.rs:131-164): fused 1 runs H, CPML-H, E and CPML-E of a
  * step in one pass over HBM with double-buffered fields (bit-identical to
- * the unfused schedule; 2 is the same without the next-plane prefetch);
+ * the unfused schedule; 2 is the same without the next-plane prefetch; 3
+ * the full-row variant: one wave per row, nz <= 576, tile_rows 3 or 7);
  * tile_rows 3, 7 or 15 (0 keeps the current);
  * x_chunks 0 = automatic. Steps that need the serial gap between the H and
  * E updates (H sources) always run unfused. */

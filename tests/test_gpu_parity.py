@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 
 # Every parity test runs on the unfused schedule and on the fused one with
 # each tile height and with many x chunks (chunk seams inside the box).
-SCHEDULES = [(0, 0, 0), (1, 3, 0), (1, 7, 0), (1, 15, 0), (1, 7, 5), (2, 7, 3), (2, 15, 0)]
+SCHEDULES = [(0, 0, 0), (1, 3, 0), (1, 7, 0), (1, 15, 0), (1, 7, 5), (2, 7, 3), (2, 15, 0),
+             (3, 3, 0), (3, 7, 0), (3, 7, 4)]
 
 
 @pytest.fixture(autouse=True, params=SCHEDULES, ids=lambda s: "mode%d_tj%d_xc%d" % s)
