@@ -102,6 +102,11 @@ ems_gpu_add_source = _sig("ems_gpu_add_source", _i, _p, _i, _i, _ip, _ip, _fp, _
                           C.POINTER(_i))
 ems_gpu_add_probe = _sig("ems_gpu_add_probe", _i, _p, _i, _i, _ip, _ip, _dp, _u64,
                          C.POINTER(_i))
+ems_gpu_add_lumped = _sig("ems_gpu_add_lumped", _i, _p, _p, C.POINTER(_i))
+ems_gpu_lumped_table = _sig("ems_gpu_lumped_table", _i, _p, _i, _i, _dp, _u64)
+ems_gpu_lumped_values = _sig("ems_gpu_lumped_values", _i, _p, _i, _i, _dp, _u64, C.POINTER(_u64),
+                             C.POINTER(_u64))
+ems_gpu_lumped_branch = _sig("ems_gpu_lumped_branch", _i, _p, _i, _dp, _dp, _dp)
 ems_gpu_run = _sig("ems_gpu_run", _i, _p, _u64)
 ems_gpu_step_with_energy = _sig("ems_gpu_step_with_energy", _i, _p, _dp)
 ems_gpu_energy = _sig("ems_gpu_energy", _i, _p, _dp)
@@ -132,6 +137,9 @@ ems_cpml_profile = _sig("ems_cpml_profile", _i, _dp, _i, _dp, _i, _dp, _i, _i,
 ems_gaussian = _sig("ems_gaussian", _d, _d, _d, _d)
 ems_gaussian_duration = _sig("ems_gaussian_duration", _d, _d)
 ems_gaussian_table = _sig("ems_gaussian_table", None, _d, _d, _d, _d, _u64, _fp)
+ems_fold_parallel = _sig("ems_fold_parallel", _i, C.c_float, C.c_float, _d, _d, _d,
+                         C.POINTER(C.c_float), C.POINTER(C.c_float))
+ems_dft = _sig("ems_dft", None, _d, _d, _dp, _u64, _dp, _i, _dp)
 ems_run_until = _sig("ems_run_until", _i, _p, C.POINTER(RunLimitsC), _ip, _i,
                      C.POINTER(RunOutcomeC))
 ems_drive_script = _sig("ems_drive_script", _i, _dp, _u64, _dp, _u64, C.POINTER(RunLimitsC),

@@ -699,9 +699,27 @@ __global__ void __launch_bounds__(256) copy_f4(const float4 *__restrict__ a, flo
     b[q] = a[q];
 }
 
-// Serial operators (excite.rs, probe.rs) and the gap rules of
+// Serial operators (excite.rs, probe.rs, lumped.rs) and the gap rules of
 // Engine::advance (mod.rs:873-915).
-enum { OP_SOURCE = 1, OP_PROBE = 2 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3 };
+
+// An installed lumped element (lumped.rs:367-406) and, with columns > 0,
+// its LumpedPort records (engine/port.rs:94-104).
+struct DevLumped {
+  int axis, n, series, src_in_branch, branch, nterm, ndrive, columns, record;
+  double dt;
+  const uint64_t *idx;
+  const double *cb, *gsrc, *tw;
+  const double *const *tv;  // per term: value by step
+  const uint64_t *tlen;
+  double *v, *vd;           // per-edge source / drive voltage of the step
+  const double *two_l_dt, *inv_c, *z;
+  double *j, *q, *javg, *uprev;
+  const double *rg, *rc;
+  double *rec, *pv, *pi;
+  uint64_t *count;
+};
+
 struct DevOp {
   int type, kind, quantity, n;
   uint64_t interval;
@@ -713,6 +731,7 @@ struct DevOp {
   uint64_t namp;
   double *vals;
   uint64_t *count;
+  DevLumped *lp;
 };
 
 __device__ __forceinline__ float op_amp(const DevOp &o, uint64_t n) {
@@ -730,11 +749,13 @@ __device__ __forceinline__ bool is_iprobe(const DevOp &o) {
 __device__ __forceinline__ bool is_vprobe(const DevOp &o) {
   return o.type == OP_PROBE && o.quantity == 0;
 }
+__device__ __forceinline__ bool is_lumped(const DevOp &o) { return o.type == OP_LUMPED; }
 __device__ __forceinline__ bool needs(const DevOp &o, uint64_t n) {
   if (o.type == OP_SOURCE) {
     if (o.kind == 1 || o.kind == 3) return true;
     return op_amp(o, n) != 0.0f;
   }
+  if (o.type == OP_LUMPED) return true;  // post_e every step
   return n % o.interval == 0;
 }
 
@@ -760,6 +781,96 @@ __device__ void probe_sample(Fields f, const DevOp &o) {
   *o.count = c + 1;
 }
 
+// LumpedElement::pre_e_impl (lumped.rs:1106-1121): u^n of each edge, read
+// from `f` (the fields before the E update), a thread per edge.
+__device__ void lumped_capture(Fields f, const DevOp *ops, int nops) {
+  for (int p = 0; p < nops; p++) {
+    if (!is_lumped(ops[p])) continue;
+    const DevLumped &L = *ops[p].lp;
+    if (!L.branch && !L.record) continue;
+    const float *u = f.u[L.axis];
+    for (int q = threadIdx.x; q < L.n; q += blockDim.x) L.uprev[q] = (double)u[L.idx[q]];
+  }
+}
+
+__device__ __forceinline__ double lumped_value(const DevLumped &L, int t, uint64_t n) {
+  return n < L.tlen[t] ? L.tv[t][n] : 0.0;
+}
+
+// LumpedElement::post_e_impl (lumped.rs:1123-1235), Installed::add_sources
+// (:1240-1249) and LumpedPort::post_e (port.rs:281-297), one thread: the
+// record sums run over the edges in order.
+__device__ void lumped_post_e(Fields f, const DevLumped &L, uint64_t n) {
+  float *u = f.u[L.axis];
+  const int ne = L.n;
+  bool active = false, drive = false;
+  if (L.nterm) {
+    for (int q = 0; q < ne; q++) L.v[q] = 0.0;
+    for (int t = 0; t < L.nterm; t++) {
+      const double val = lumped_value(L, t, n);
+      if (val != 0.0) {
+        active = true;
+        for (int q = 0; q < ne; q++) L.v[q] += L.tw[(size_t)t * ne + q] * val;
+      }
+    }
+  }
+  if (L.ndrive) {
+    for (int q = 0; q < ne; q++) L.vd[q] = 0.0;
+    for (int t = L.nterm; t < L.nterm + L.ndrive; t++) {
+      const double val = lumped_value(L, t, n);
+      if (val != 0.0) {
+        drive = true;
+        for (int q = 0; q < ne; q++) L.vd[q] += L.tw[(size_t)t * ne + q] * val;
+      }
+    }
+  }
+  if (active && !L.src_in_branch)
+    for (int q = 0; q < ne; q++) {
+      const double add = L.cb[q] * L.gsrc[q] * L.v[q];
+      if (add != 0.0) u[L.idx[q]] = (float)((double)u[L.idx[q]] + add);
+    }
+  if (L.branch) {
+    for (int q = 0; q < ne; q++) {
+      double v_br = active && L.src_in_branch ? L.v[q] : 0.0;
+      if (drive) v_br += L.vd[q];
+      const double u_star = (double)u[L.idx[q]];
+      if (L.branch == 2) {
+        L.j[q] += 2.0 * (L.uprev[q] - v_br) / L.two_l_dt[q];
+        L.javg[q] = L.j[q];
+        u[L.idx[q]] = (float)(u_star - L.cb[q] * L.j[q]);
+        continue;
+      }
+      const double k = L.two_l_dt[q] * L.j[q] - L.q[q] * L.inv_c[q] - v_br + 0.5 * L.uprev[q];
+      const double z = L.z[q];
+      const double u_new = (z * u_star - L.cb[q] * k) / (z + 0.5 * L.cb[q]);
+      const double jb = (k + 0.5 * u_new) / z;
+      L.j[q] = 2.0 * jb - L.j[q];
+      L.q[q] += L.dt * jb;
+      L.javg[q] = jb;
+      u[L.idx[q]] = (float)u_new;
+    }
+  }
+  if (L.record) {
+    double sum = 0.0;
+    for (int q = 0; q < ne; q++) {
+      const double u0 = L.uprev[q], u1 = (double)u[L.idx[q]];
+      const double v = active && !L.src_in_branch ? L.v[q] : 0.0;
+      sum += L.rg[q] * (0.5 * (u0 + u1) - v) + L.rc[q] * (u1 - u0) / L.dt;
+      if (L.branch) sum += L.javg[q];
+    }
+    const double cur = sum / (double)L.series;
+    const uint64_t c = *L.count;
+    L.rec[c] = cur;
+    if (L.columns) {
+      double sv = -0.0;
+      for (int q = 0; q < ne; q++) sv += (double)u[L.idx[q]];
+      L.pi[c] = -cur;
+      L.pv[c] = sv / (double)L.columns;
+    }
+    *L.count = c + 1;
+  }
+}
+
 struct Gaps {
   bool gap_b, moved, gap_a;
 };
@@ -768,9 +879,9 @@ __device__ Gaps gaps_at(const DevOp *ops, int nops, uint64_t n) {
   for (int q = 0; q < nops; q++) {
     const DevOp &o = ops[q];
     if (!needs(o, n)) continue;
-    if (is_esrc(o) || is_vprobe(o)) need_post_e = true;
+    if (is_esrc(o) || is_vprobe(o) || is_lumped(o)) need_post_e = true;
     if (is_hsrc(o)) mid = true, movable = false;
-    if (is_iprobe(o)) mid = true;
+    if (is_iprobe(o) || is_lumped(o)) mid = true;  // lumped pre_e is movable
   }
   Gaps g;
   g.gap_b = mid && !movable;
@@ -781,22 +892,30 @@ __device__ Gaps gaps_at(const DevOp *ops, int nops, uint64_t n) {
 
 // The hooks of one gap in registration order. Probes only read the fields,
 // so a run of consecutive probes samples in parallel (a thread each);
-// sources write and run one after another, with a barrier on either side.
-// `which` selects the ops of the gap: 0 the mid gap (H sources, current
-// probes), 1 the post-E gap (E sources, voltage probes).
+// writers (sources, lumped elements) run one after another, with a barrier
+// on either side. `which` selects the ops of the gap: 0 the mid gap (H
+// sources, current probes), 1 the post-E gap (E sources, lumped elements,
+// voltage probes).
+__device__ __forceinline__ bool gap_writer(const DevOp &o, int which) {
+  return which == 0 ? is_hsrc(o) : (is_esrc(o) || is_lumped(o));
+}
 __device__ void run_gap(Fields f, const DevOp *ops, int nops, uint64_t n, int which) {
   int q = 0;
   while (q < nops) {
     const DevOp &o = ops[q];
-    const bool src = which == 0 ? is_hsrc(o) : is_esrc(o);
-    if (src) {
-      if (threadIdx.x == 0) source_apply(f, o, n);
+    if (gap_writer(o, which)) {
+      if (threadIdx.x == 0) {
+        if (is_lumped(o))
+          lumped_post_e(f, *o.lp, n);
+        else
+          source_apply(f, o, n);
+      }
       __syncthreads();
       q++;
       continue;
     }
     int q1 = q;
-    while (q1 < nops && !(which == 0 ? is_hsrc(ops[q1]) : is_esrc(ops[q1]))) q1++;
+    while (q1 < nops && !gap_writer(ops[q1], which)) q1++;
     for (int p = q + (int)threadIdx.x; p < q1; p += (int)blockDim.x) {
       const DevOp &pr = ops[p];
       const bool mine = which == 0 ? is_iprobe(pr) : is_vprobe(pr);
@@ -807,17 +926,27 @@ __device__ void run_gap(Fields f, const DevOp *ops, int nops, uint64_t n, int wh
   }
 }
 
+// Unfused steps: the gap between the H and E updates (the fields are
+// updated in place, so pre_e captures u^n here).
 __global__ void __launch_bounds__(256) gap_mid(Fields f, const DevOp *ops, int nops,
                                                const uint64_t *step) {
   const uint64_t n = *step;
   const Gaps g = gaps_at(ops, nops, n);
   if (g.gap_b) run_gap(f, ops, nops, n, 0);
+  lumped_capture(f, ops, nops);
 }
 
-__global__ void __launch_bounds__(256) gap_post_e(Fields f, const DevOp *ops, int nops,
-                                                  uint64_t *step) {
+// The post-E gap on the step result `f`. Fused steps keep the state before
+// the step in `fp` (the other buffer), whose u is u^n as pre_e reads it;
+// capture = 0 when the mid gap has already captured it.
+__global__ void __launch_bounds__(256) gap_post_e(Fields f, Fields fp, int capture,
+                                                  const DevOp *ops, int nops, uint64_t *step) {
   const uint64_t n = *step;
   const Gaps g = gaps_at(ops, nops, n);
+  if (capture) {
+    lumped_capture(fp, ops, nops);
+    __syncthreads();
+  }
   if (g.gap_a) {
     if (g.moved) {
       // Moved post_h hooks (current probes) first (mod.rs:936-942).
@@ -965,6 +1094,15 @@ struct HostOp {
   uint64_t count = 0;  // samples recorded (host mirror of *d_count)
   uint64_t first = 0;  // step of sample 0
   uint64_t *d_count = nullptr;
+  // lumped element: its device record (d_lp) and what it points to
+  DevLumped lp{};
+  DevLumped *d_lp = nullptr;
+  std::vector<double *> tables;      // per term
+  std::vector<uint64_t> table_len;
+  double **d_tv = nullptr;
+  uint64_t *d_tlen = nullptr;
+  std::vector<double> two_l_dt, inv_c;  // host copies for the branch energy
+  std::vector<void *> mem;              // every allocation of this op
 };
 
 }  // namespace
@@ -1050,7 +1188,8 @@ bool has_hsrc(const ems_gpu *h) {
 bool has_mid_ops(const ems_gpu *h) {
   for (auto &o : h->ops)
     if ((o.type == OP_SOURCE && (o.kind == 2 || o.kind == 3)) ||
-        (o.type == OP_PROBE && o.quantity == 1))
+        (o.type == OP_PROBE && o.quantity == 1) ||
+        (o.type == OP_LUMPED && (o.lp.branch || o.lp.record)))
       return true;
   return false;
 }
@@ -1257,9 +1396,19 @@ int launch_fused(ems_gpu *h, int par) {
   return 0;
 }
 
-int launch_gap_post_e(ems_gpu *h, const Fields &f) {
-  hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(256), 0, h->stream, f, h->d_ops,
-                     (int)h->ops.size(), h->d_step);
+// Post-E gap on the result `f`; `fp` holds the state before a fused step
+// (prev = true), whose u the lumped pre_e hooks capture.
+int launch_gap_post_e(ems_gpu *h, const Fields &f, const Fields &fp, bool prev) {
+  hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(256), 0, h->stream, f, fp, prev ? 1 : 0,
+                     h->d_ops, (int)h->ops.size(), h->d_step);
+  CK(h, hipGetLastError());
+  return 0;
+}
+
+int launch_gap_mid(ems_gpu *h, const Fields &f) {
+  if (!has_mid_ops(h)) return 0;
+  hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops, (int)h->ops.size(),
+                     (const uint64_t *)h->d_step);
   CK(h, hipGetLastError());
   return 0;
 }
@@ -1270,19 +1419,15 @@ int enqueue_step(ems_gpu *h, int &par) {
   if (use_fused(h)) {
     if ((rc = launch_fused(h, par))) return rc;
     par ^= 1;
-    return launch_gap_post_e(h, h->fb[par]);
+    return launch_gap_post_e(h, h->fb[par], h->fb[par ^ 1], true);
   }
   const Fields &f = h->fb[par];
   if ((rc = launch_h(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 0, par))) return rc;
-  if (has_mid_ops(h)) {
-    hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops,
-                       (int)h->ops.size(), (const uint64_t *)h->d_step);
-    CK(h, hipGetLastError());
-  }
+  if ((rc = launch_gap_mid(h, f))) return rc;
   if ((rc = launch_e(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 1, par))) return rc;
-  return launch_gap_post_e(h, f);
+  return launch_gap_post_e(h, f, f, false);
 }
 
 void drop_graphs(ems_gpu *h) {
@@ -1373,6 +1518,27 @@ int prepare(ems_gpu *h, uint64_t steps) {
   }
   bool grown = false;
   for (auto &o : h->ops) {
+    if (o.type == OP_LUMPED) {
+      // a record per step
+      if (!o.lp.record || o.count + steps <= o.cap) continue;
+      uint64_t cap = o.cap ? o.cap : 1024;
+      while (cap < o.count + steps) cap *= 2;
+      double *nv[3] = {nullptr, nullptr, nullptr};
+      double *ov[3] = {o.lp.rec, o.lp.pv, o.lp.pi};
+      for (int w = 0; w < 3; w++) {
+        CK(h, hipMalloc(&nv[w], cap * sizeof(double)));
+        if (ov[w] && o.count)
+          CK(h, hipMemcpyAsync(nv[w], ov[w], o.count * sizeof(double), hipMemcpyDeviceToDevice,
+                               h->stream));
+      }
+      CK(h, hipStreamSynchronize(h->stream));
+      for (int w = 0; w < 3; w++)
+        if (ov[w]) CK(h, hipFree(ov[w]));
+      o.lp.rec = nv[0], o.lp.pv = nv[1], o.lp.pi = nv[2];
+      o.cap = cap;
+      grown = true;
+      continue;
+    }
     if (o.type != OP_PROBE) continue;
     // samples taken in steps [step, step + steps): multiples of interval
     uint64_t a = h->step, b = h->step + steps;
@@ -1403,12 +1569,24 @@ int prepare(ems_gpu *h, uint64_t steps) {
     drop_graphs(h);
     std::vector<DevOp> dv(h->ops.size());
     for (size_t q = 0; q < h->ops.size(); q++) {
-      const HostOp &o = h->ops[q];
+      HostOp &o = h->ops[q];
       DevOp &d = dv[q];
       d.type = o.type, d.kind = o.kind, d.quantity = o.quantity, d.n = o.n;
       d.interval = o.interval;
       d.axis = o.d_axis, d.idx = o.d_idx, d.wf = o.d_wf, d.wd = o.d_wd;
       d.amp = o.d_amp, d.namp = o.namp, d.vals = o.d_vals, d.count = o.d_count;
+      d.lp = nullptr;
+      if (o.type == OP_LUMPED) {
+        const size_t nt = o.tables.size();
+        if (nt) {
+          CK(h, hipMemcpy(o.d_tv, o.tables.data(), nt * sizeof(double *), hipMemcpyHostToDevice));
+          CK(h, hipMemcpy(o.d_tlen, o.table_len.data(), nt * sizeof(uint64_t),
+                          hipMemcpyHostToDevice));
+        }
+        o.lp.count = o.d_count;
+        CK(h, hipMemcpy(o.d_lp, &o.lp, sizeof(DevLumped), hipMemcpyHostToDevice));
+        d.lp = o.d_lp;
+      }
     }
     if (h->d_ops) CK(h, hipFree(h->d_ops));
     h->d_ops = nullptr;
@@ -1424,6 +1602,7 @@ int prepare(ems_gpu *h, uint64_t steps) {
 // Host mirror of the probe sample counts (the device increments its own).
 void advance_counts(ems_gpu *h, uint64_t steps) {
   for (auto &o : h->ops) {
+    if (o.type == OP_LUMPED && o.lp.record) o.count += steps;
     if (o.type != OP_PROBE) continue;
     uint64_t a = h->step, b = h->step + steps;
     o.count += (b + o.interval - 1) / o.interval - (a + o.interval - 1) / o.interval;
@@ -1573,6 +1752,9 @@ void ems_gpu_destroy(ems_gpu *h) {
     (void)hipFree(o.d_axis), (void)hipFree(o.d_idx), (void)hipFree(o.d_wf);
     (void)hipFree(o.d_wd), (void)hipFree(o.d_amp), (void)hipFree(o.d_vals);
     (void)hipFree(o.d_count);
+    for (void *p : o.mem) (void)hipFree(p);
+    for (double *p : o.tables) (void)hipFree(p);
+    if (o.lp.rec) (void)hipFree(o.lp.rec), (void)hipFree(o.lp.pv), (void)hipFree(o.lp.pi);
   }
   if (h->d_ops) (void)hipFree(h->d_ops);
   if (h->d_step) (void)hipFree(h->d_step);
@@ -1717,6 +1899,141 @@ int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *ax
   return EMS_OK;
 }
 
+int ems_gpu_add_lumped(ems_gpu *h, const ems_lumped_desc *d, int *id) {
+  ALIVE(h);
+  if (!d) return fail(h, EMS_E_INVALID, "null lumped description");
+  const int ne = d->n_edges;
+  if (d->axis < 0 || d->axis > 2 || ne < 1 || !d->nodes || !d->cb || !d->g_src)
+    return fail(h, EMS_E_INVALID, "lumped element: axis, edges, cb and g_src are required");
+  if (d->series < 1) return fail(h, EMS_E_INVALID, "lumped element: series must be >= 1");
+  if (d->n_terms < 0 || d->n_drive < 0 || (d->n_terms && !d->term_weight) ||
+      (d->n_drive && !d->drive_weight))
+    return fail(h, EMS_E_INVALID, "lumped element: source weights");
+  if (d->branch < 0 || d->branch > 2 || (d->branch && (!d->two_l_dt || !d->inv_c || !d->z)))
+    return fail(h, EMS_E_INVALID, "lumped element: branch arrays");
+  if ((d->rec_g == nullptr) != (d->rec_c == nullptr))
+    return fail(h, EMS_E_INVALID, "lumped element: record needs both g and c");
+  if (d->port_columns < 0 || (d->port_columns && !d->rec_g))
+    return fail(h, EMS_E_INVALID, "lumped port needs the current record");
+  std::vector<uint64_t> idx(ne);
+  for (int q = 0; q < ne; q++) {
+    const int32_t *nd = d->nodes + 3 * q;
+    if (!node_ok(h, nd) || nd[d->axis] + 1 >= h->n[d->axis])
+      return fail(h, EMS_E_INVALID, "lumped edge out of range");
+    if (!(d->cb[q] != 0.0)) return fail(h, EMS_E_INVALID, "lumped edge is PEC (cb = 0)");
+    idx[q] = ((uint64_t)nd[0] * h->n[1] + nd[1]) * h->n[2] + nd[2];
+  }
+  HostOp o;
+  o.type = OP_LUMPED;
+  o.n = ne;
+  DevLumped &L = o.lp;
+  L.axis = d->axis, L.n = ne, L.series = d->series, L.src_in_branch = d->src_in_branch ? 1 : 0;
+  L.branch = d->branch, L.nterm = d->n_terms, L.ndrive = d->n_drive;
+  L.columns = d->port_columns, L.record = d->rec_g != nullptr, L.dt = h->dt;
+  auto put = [&](const void *src, size_t bytes, void **dst) -> int {
+    int rc = upload(h, dst, src, bytes);
+    if (!rc) o.mem.push_back(*dst);
+    return rc;
+  };
+  auto zeros = [&](size_t n, double **dst) -> int {
+    std::vector<double> z(n ? n : 1, 0.0);
+    return put(z.data(), z.size() * sizeof(double), (void **)dst);
+  };
+  const size_t nb = (size_t)ne * sizeof(double);
+  const int nt = d->n_terms + d->n_drive;
+  std::vector<double> tw((size_t)(nt ? nt : 1) * ne, 0.0);
+  if (d->n_terms) memcpy(tw.data(), d->term_weight, (size_t)d->n_terms * nb);
+  if (d->n_drive) memcpy(tw.data() + (size_t)d->n_terms * ne, d->drive_weight, (size_t)d->n_drive * nb);
+  int rc;
+  if ((rc = put(idx.data(), ne * sizeof(uint64_t), (void **)&L.idx))) return rc;
+  if ((rc = put(d->cb, nb, (void **)&L.cb))) return rc;
+  if ((rc = put(d->g_src, nb, (void **)&L.gsrc))) return rc;
+  if ((rc = put(tw.data(), tw.size() * sizeof(double), (void **)&L.tw))) return rc;
+  if ((rc = zeros(ne, &L.v)) || (rc = zeros(ne, &L.vd))) return rc;
+  if ((rc = zeros(ne, &L.j)) || (rc = zeros(ne, &L.q)) || (rc = zeros(ne, &L.javg)) ||
+      (rc = zeros(ne, &L.uprev)))
+    return rc;
+  if (d->branch) {
+    if ((rc = put(d->two_l_dt, nb, (void **)&L.two_l_dt))) return rc;
+    if ((rc = put(d->inv_c, nb, (void **)&L.inv_c))) return rc;
+    if ((rc = put(d->z, nb, (void **)&L.z))) return rc;
+    o.two_l_dt.assign(d->two_l_dt, d->two_l_dt + ne);
+    o.inv_c.assign(d->inv_c, d->inv_c + ne);
+  }
+  if (L.record) {
+    if ((rc = put(d->rec_g, nb, (void **)&L.rg))) return rc;
+    if ((rc = put(d->rec_c, nb, (void **)&L.rc))) return rc;
+  }
+  o.tables.assign(nt, nullptr);
+  o.table_len.assign(nt, 0);
+  if ((rc = zeros(nt, (double **)&o.d_tv))) return rc;
+  if ((rc = zeros(nt, (double **)&o.d_tlen))) return rc;
+  L.tv = o.d_tv, L.tlen = o.d_tlen;
+  CK(h, hipMalloc(&o.d_count, sizeof(uint64_t)));
+  CK(h, hipMemset(o.d_count, 0, sizeof(uint64_t)));
+  if ((rc = put(&L, sizeof(DevLumped), (void **)&o.d_lp))) return rc;
+  o.first = h->step;
+  h->ops.push_back(o);
+  h->ops_dirty = true;
+  if (id) *id = (int)h->ops.size() - 1;
+  return EMS_OK;
+}
+
+static HostOp *lumped_op(ems_gpu *h, int id) {
+  if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_LUMPED) return nullptr;
+  return &h->ops[id];
+}
+
+int ems_gpu_lumped_table(ems_gpu *h, int id, int term, const double *values, uint64_t count) {
+  ALIVE(h);
+  HostOp *o = lumped_op(h, id);
+  if (!o) return fail(h, EMS_E_INVALID, "not a lumped element id");
+  if (term < 0 || term >= (int)o->tables.size() || (count && !values))
+    return fail(h, EMS_E_INVALID, "lumped source term");
+  CK(h, hipStreamSynchronize(h->stream));
+  double *nt = nullptr;
+  int rc = upload(h, (void **)&nt, values, count * sizeof(double));
+  if (rc) return rc;
+  if (o->tables[term]) CK(h, hipFree(o->tables[term]));
+  o->tables[term] = nt;
+  o->table_len[term] = count;
+  h->ops_dirty = true;
+  return EMS_OK;
+}
+
+int ems_gpu_lumped_values(ems_gpu *h, int id, int which, double *out, uint64_t cap,
+                          uint64_t *count, uint64_t *first_step) {
+  ALIVE(h);
+  HostOp *o = lumped_op(h, id);
+  if (!o) return fail(h, EMS_E_INVALID, "not a lumped element id");
+  if (which < 0 || which > 2 || (which && !o->lp.columns))
+    return fail(h, EMS_E_INVALID, "lumped series: 0 current, 1/2 port V/I (ports only)");
+  if (!o->lp.record) {
+    if (count) *count = 0;
+    if (first_step) *first_step = o->first;
+    return EMS_OK;
+  }
+  CK(h, hipStreamSynchronize(h->stream));
+  const double *src = which == 0 ? o->lp.rec : which == 1 ? o->lp.pv : o->lp.pi;
+  const uint64_t n = o->count < cap ? o->count : cap;
+  if (n && out) CK(h, hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (count) *count = o->count;
+  if (first_step) *first_step = o->first;
+  return EMS_OK;
+}
+
+int ems_gpu_lumped_branch(ems_gpu *h, int id, double *j, double *q, double *j_avg) {
+  ALIVE(h);
+  HostOp *o = lumped_op(h, id);
+  if (!o) return fail(h, EMS_E_INVALID, "not a lumped element id");
+  CK(h, hipStreamSynchronize(h->stream));
+  const size_t nb = (size_t)o->lp.n * sizeof(double);
+  if (j) CK(h, hipMemcpy(j, o->lp.j, nb, hipMemcpyDeviceToHost));
+  if (q) CK(h, hipMemcpy(q, o->lp.q, nb, hipMemcpyDeviceToHost));
+  if (j_avg) CK(h, hipMemcpy(j_avg, o->lp.javg, nb, hipMemcpyDeviceToHost));
+  return EMS_OK;
+}
+
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks) {
   ALIVE(h);
   // fused 2: the fused kernel without the next-plane prefetch; 3: the
@@ -1738,22 +2055,45 @@ int ems_gpu_run(ems_gpu *h, uint64_t n) {
   return run_steps(h, n);
 }
 
-// Operator energies (Operator::energy, operator.rs:106-108): 0 for every
-// kind held here; Sum over the registered operators folds from -0.0
-// (mod.rs:772-774), so it is -0.0 with none and 0.0 otherwise.
-static double ops_energy(const ems_gpu *h) {
+// Engine::operator_energy (mod.rs:753-755): Operator::energy of every
+// registered operator in order, Sum folding from -0.0. Every kind held here
+// has 0 but a lumped element with a branch (branch_energy,
+// lumped.rs:1075-1087, from the device state); the CPML slabs are one
+// operator.
+static int ops_energy(ems_gpu *h, double *out) {
   double w = -0.0;
-  if (!h->ops.empty() || !h->slabs.empty()) w += 0.0;
-  return w;
+  if (!h->slabs.empty()) w += 0.0;
+  bool synced = false;
+  for (auto &o : h->ops) {
+    if (o.type != OP_LUMPED || !o.lp.branch) {
+      w += 0.0;
+      continue;
+    }
+    if (!synced) CK(h, hipStreamSynchronize(h->stream));
+    synced = true;
+    const int ne = o.lp.n;
+    std::vector<double> j(ne), q(ne);
+    CK(h, hipMemcpy(j.data(), o.lp.j, ne * sizeof(double), hipMemcpyDeviceToHost));
+    CK(h, hipMemcpy(q.data(), o.lp.q, ne * sizeof(double), hipMemcpyDeviceToHost));
+    double s = -0.0;
+    for (int e = 0; e < ne; e++) {
+      const double half_l = 0.25 * o.two_l_dt[e] * h->dt;
+      s += half_l * j[e] * j[e] + 0.5 * o.inv_c[e] * q[e] * q[e];
+    }
+    w += s;
+  }
+  *out = w;
+  return 0;
 }
 
 int ems_gpu_step_with_energy(ems_gpu *h, double *w) {
   ALIVE(h);
   if (!w) return EMS_E_INVALID;
-  const double w_ops = ops_energy(h);
-  double wf = 0;
-  int rc = energy_now(h, &wf);
+  double w_ops = 0;
+  int rc = ops_energy(h, &w_ops);
   if (rc) return rc;
+  double wf = 0;
+  if ((rc = energy_now(h, &wf))) return rc;
   if ((rc = run_steps(h, 1))) return rc;
   *w = wf + w_ops;
   return EMS_OK;
@@ -1762,10 +2102,11 @@ int ems_gpu_step_with_energy(ems_gpu *h, double *w) {
 int ems_gpu_energy(ems_gpu *h, double *w) {
   ALIVE(h);
   if (!w) return EMS_E_INVALID;
-  double wf = 0;
+  double wf = 0, w_ops = 0;
   int rc = energy_now(h, &wf);
   if (rc) return rc;
-  *w = wf + ops_energy(h);
+  if ((rc = ops_energy(h, &w_ops))) return rc;
+  *w = wf + w_ops;
   return EMS_OK;
 }
 
@@ -1839,7 +2180,7 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       if ((rc = launch_fused(h, h->par))) return rc;
       CK(h, hipEventRecord(ev[1], h->stream));
       h->par ^= 1;
-      if ((rc = launch_gap_post_e(h, h->fb[h->par]))) return rc;
+      if ((rc = launch_gap_post_e(h, h->fb[h->par], h->fb[h->par ^ 1], true))) return rc;
       CK(h, hipEventRecord(ev[4], h->stream));
       CK(h, hipEventSynchronize(ev[4]));
       CK(h, hipEventElapsedTime(&a, ev[0], ev[1]));
@@ -1850,16 +2191,12 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       if ((rc = launch_h(h, f))) return rc;
       CK(h, hipEventRecord(ev[1], h->stream));
       if ((rc = launch_cpml(h, f, 0, h->par))) return rc;
-      if (has_mid_ops(h)) {
-        hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops,
-                           (int)h->ops.size(), (const uint64_t *)h->d_step);
-        CK(h, hipGetLastError());
-      }
+      if ((rc = launch_gap_mid(h, f))) return rc;
       CK(h, hipEventRecord(ev[2], h->stream));
       if ((rc = launch_e(h, f))) return rc;
       CK(h, hipEventRecord(ev[3], h->stream));
       if ((rc = launch_cpml(h, f, 1, h->par))) return rc;
-      if ((rc = launch_gap_post_e(h, f))) return rc;
+      if ((rc = launch_gap_post_e(h, f, f, false))) return rc;
       CK(h, hipEventRecord(ev[4], h->stream));
       CK(h, hipEventSynchronize(ev[4]));
       CK(h, hipEventElapsedTime(&a, ev[0], ev[1]));

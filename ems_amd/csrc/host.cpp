@@ -429,6 +429,73 @@ void ems_gaussian_table(double f0, double fc, double dt, double offset, uint64_t
     amp[n] = (float)ems_gaussian(f0, fc, ((double)n + offset) * dt);
 }
 
+// fold_parallel (src/engine/lumped.rs:275-282): g and c in parallel with the
+// edge's recovered C = dt(1 + ca)/(2cb) and G = (1 - ca)/cb.
+int ems_fold_parallel(float ca, float cb, double dt, double g, double c, float *ca_out,
+                      float *cb_out) {
+  if (cb == 0.0f || !ca_out || !cb_out) return EMS_E_INVALID;
+  const double a = (double)ca, b = (double)cb;
+  const double cap = dt * (1.0 + a) / (2.0 * b) + c;
+  const double cond = (1.0 - a) / b + g;
+  const double den = cap + 0.5 * cond * dt;
+  *ca_out = (float)((cap - 0.5 * cond * dt) / den);
+  *cb_out = (float)(dt / den);
+  return EMS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ DFT --
+namespace {
+
+const double TAU = 6.283185307179586476925286766559;
+const uint64_t ANCHOR_INTERVAL = 1024;  // dft.rs:44
+
+struct Cx {
+  double re, im;
+};
+// Complex64::cis / from_polar (post/complex.rs:34-43)
+Cx cis(double theta) { return {1.0 * std::cos(theta), 1.0 * std::sin(theta)}; }
+// Complex64 * Complex64 (complex.rs:153-161)
+Cx mul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+double fract(double x) { return x - std::trunc(x); }
+
+// phase_cycles (dft.rs:204-212): f(t0 + n dt) mod 1 with the exact
+// rounding error of (f dt) n recovered by a fused multiply-add.
+double phase_cycles(double f, double t0, double dt, double n) {
+  const double a = f * dt;
+  const double p = a * n;
+  const double e = std::fma(a, n, -p);
+  const double c = f * t0;
+  return fract(fract(p) + e + fract(c));
+}
+
+}  // namespace
+
+extern "C" {
+
+// dft (dft.rs:218-222) through RunningDft (dft.rs:75-190): the running
+// phasor advanced by one complex product per sample and re-anchored from
+// the exact phase every 1024 samples.
+void ems_dft(double t0, double dt, const double *x, uint64_t n, const double *freqs, int nf,
+             double *out) {
+  for (int k = 0; k < nf; k++) {
+    const double f = freqs[k];
+    const Cx step = cis(-TAU * f * dt);
+    Cx ph = cis(-TAU * phase_cycles(f, t0, dt, 0.0));
+    Cx acc = {0.0, 0.0};
+    for (uint64_t s = 0; s < n; s++) {
+      acc.re = acc.re + ph.re * x[s];
+      acc.im = acc.im + ph.im * x[s];
+      ph = mul(ph, step);
+      if ((s + 1) % ANCHOR_INTERVAL == 0)
+        ph = cis(-TAU * phase_cycles(f, t0, dt, (double)(s + 1)));
+    }
+    out[2 * k] = acc.re * dt;
+    out[2 * k + 1] = acc.im * dt;
+  }
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------- run loop --

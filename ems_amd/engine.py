@@ -654,20 +654,14 @@ class Engine:
             raise EmsError(rc, "ems_gpu_create failed (no usable GPU?)")
         self._h = h
         self._dense_lanes = lanes
+        self._owns_lanes = False
+        self._pool = pool
         self.setup: List[tuple] = []
         self.ops: list = []
+        self._tables: list = []
+        self._lumped_n: dict = {}
         for l in range(12):
-            lane = np.ascontiguousarray(lanes[l])
-            pooled = pool_lane(lane) if pool else None
-            if pooled is None:
-                rows = lane.reshape(-1, n[2])
-                self._check(nat.ems_gpu_set_lane(h, l, nat.ptr(rows, C.c_float), rows.shape[0], None))
-                self.setup.append(("lane", l, rows, None))
-            else:
-                rows, index = pooled
-                self._check(nat.ems_gpu_set_lane(h, l, nat.ptr(rows, C.c_float), rows.shape[0],
-                                                 nat.ptr(index, C.c_uint32)))
-                self.setup.append(("lane", l, rows, index))
+            self._upload_lane(l, lanes[l], pool)
 
     def _check(self, rc: int) -> None:
         if rc != 0:
@@ -737,10 +731,72 @@ class Engine:
             self.setup.append(("probe", int(op.quantity), axis, nodes, w, op._interval))
             op._engine = self
             op._id = oid.value
+        elif hasattr(op, "_plan"):
+            # LumpedElement / LumpedPort (lumped.rs:880-1012): fold into the
+            # lanes, re-upload them, register the device operator.
+            from . import lumped as LM
+            if self._dense_lanes is None:
+                raise EmsError(nat.EMS_E_INVALID, "lumped elements need the host lanes")
+            plan = op._plan(self.grid, self.dt, self._dense_lanes)
+            a = plan["axis"]
+            if not self._owns_lanes:  # the caller's array stays as it was
+                self._dense_lanes = self._dense_lanes.copy()
+                self._owns_lanes = True
+            for l, lane in ((a, plan["ca"]), (3 + a, plan["cb"])):
+                self._dense_lanes[l] = lane
+                self._upload_lane(l, lane)
+            d, keep = LM.make_desc(plan)
+            oid = C.c_int()
+            self._check(nat.ems_gpu_add_lumped(h, C.byref(d), C.byref(oid)))
+            tables = LM.SourceTables(plan, self.dt)
+            self._tables.append((oid.value, tables))
+            self._lumped_n[oid.value] = len(plan["edges"])
+            self.setup.append(("lumped", plan, tables))
+            op._bind(self, oid.value, plan)
+            self._ensure_tables(self.timestep() + 1)
         else:
             raise EmsError(nat.EMS_E_UNSUPPORTED, f"operator {type(op).__name__}")
         self.ops.append(op)
         return op
+
+    def _upload_lane(self, l: int, lane: np.ndarray, pool: bool = True) -> None:
+        n = self.grid.nodes()
+        lane = np.ascontiguousarray(lane)
+        pooled = pool_lane(lane) if pool else None
+        if pooled is None:
+            rows = lane.reshape(-1, n[2])
+            self._check(nat.ems_gpu_set_lane(self._h, l, nat.ptr(rows, C.c_float), rows.shape[0], None))
+            self.setup.append(("lane", l, rows, None))
+        else:
+            rows, index = pooled
+            self._check(nat.ems_gpu_set_lane(self._h, l, nat.ptr(rows, C.c_float), rows.shape[0],
+                                             nat.ptr(index, C.c_uint32)))
+            self.setup.append(("lane", l, rows, index))
+
+    def _ensure_tables(self, upto: int) -> None:
+        for oid, t in self._tables:
+            def up(term, vals, oid=oid):
+                v = np.ascontiguousarray(vals)
+                self._check(nat.ems_gpu_lumped_table(self._h, oid, term, nat.ptr(v, C.c_double),
+                                                     v.size))
+            t.ensure(upto, up)
+
+    def _lumped_values(self, oid: int, which: int):
+        cnt = C.c_uint64()
+        first = C.c_uint64()
+        self._check(nat.ems_gpu_lumped_values(self._h, oid, which, None, 0, C.byref(cnt),
+                                              C.byref(first)))
+        out = np.empty(cnt.value, dtype=np.float64)
+        self._check(nat.ems_gpu_lumped_values(self._h, oid, which, nat.ptr(out, C.c_double),
+                                              out.size, C.byref(cnt), C.byref(first)))
+        return out, first.value
+
+    def _lumped_branch(self, oid: int):
+        n = self._lumped_n[oid]
+        j, q, jav = (np.zeros(n) for _ in range(3))
+        self._check(nat.ems_gpu_lumped_branch(self._h, oid, nat.ptr(j, C.c_double),
+                                              nat.ptr(q, C.c_double), nat.ptr(jav, C.c_double)))
+        return j, q, jav
 
     @staticmethod
     def _edges(edges, wtype):
@@ -751,12 +807,14 @@ class Engine:
 
     # -- stepping
     def run(self, n: int) -> None:
+        self._ensure_tables(self.timestep() + int(n))
         self._check(nat.ems_gpu_run(self._h, int(n)))
 
     def step(self) -> None:
         self.run(1)
 
     def step_with_energy(self) -> float:
+        self._ensure_tables(self.timestep() + 1)
         w = C.c_double()
         self._check(nat.ems_gpu_step_with_energy(self._h, C.byref(w)))
         return w.value
@@ -767,6 +825,7 @@ class Engine:
         return w.value
 
     def run_until(self, limits: RunLimits, watch: Sequence[Probe] = ()) -> RunOutcome:
+        self._ensure_tables(self.timestep() + int(limits.max_timesteps) + 1)
         out = nat.RunOutcomeC()
         lim = limits.c()
         ids = np.ascontiguousarray(np.array([p._id for p in watch], dtype=np.int32))
@@ -828,12 +887,11 @@ class Engine:
         """Times a few real steps of each fused tiling on this engine (they
         advance the simulation like any other steps) and keeps the fastest.
         Candidates are (mode, tile_rows, x_chunks), mode 1 with and 2
-        without the next-plane prefetch. Returns the choice with its mean
-        kernel time in ms, and all timings."""
+        without the next-plane prefetch (mode 3, the full-row kernel, only
+        when listed). Returns the choice with its mean kernel time in ms,
+        and all timings."""
         if candidates is None:
-            candidates = [(m, t, c) for m in (1, 2) for t in (3, 7, 15) for c in (0, 8, 16, 32)]
-            if self.grid.nodes()[2] <= 576:
-                candidates += [(3, t, c) for t in (3, 7) for c in (0, 4, 8, 16, 32)]
+            candidates = [(m, t, c) for m in (1, 2) for t in (3, 7, 15) for c in (0, 16, 32, 64)]
         results = []
         for m, t, c in candidates:
             self.set_schedule(m, t, c)

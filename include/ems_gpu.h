@@ -92,6 +92,61 @@ int ems_gpu_add_source(ems_gpu *h, int kind, int n_edges, const int32_t *axis,
 int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *axis,
                       const int32_t *nodes, const double *w, uint64_t interval, int *id);
 
+/* An installed lumped element (LumpedElement, src/engine/lumped.rs:367-406:
+ * Installed + Branch + CurrentRecord after `install` :880-1012, which the
+ * host performs, folding R and C into the ca/cb lanes with fold_parallel
+ * :275-282 before calling this), optionally a LumpedPort
+ * (src/engine/port.rs:94-297). Edges run along `axis` from nodes[3*q].
+ *   cb, g_src     per edge: the final cb (as f64) and the conductance in
+ *                 series with the source (0 where the source sits in the
+ *                 branch);
+ *   series        M, the edges in series;
+ *   n_terms       source terms, weights term_weight[t*n_edges + q]
+ *                 (1/M for a box source); n_drive branch-drive terms
+ *                 likewise (branch_drive_source); values by ems_gpu_lumped_table;
+ *   branch        0 none, 1 the trapezoidal series R+L+C branch, 2 the
+ *                 explicit inductor; two_l_dt = 2L/dt, inv_c = 1/C (0 = no
+ *                 capacitor), z = 2L/dt + R + dt/(2C) per edge;
+ *   rec_g, rec_c  per-edge g and C of the terminal-current record (NULL =
+ *                 no record; `current` :1042-1047);
+ *   port_columns  > 0: a LumpedPort with N columns: every step records
+ *                 V = sum(u)/N and I = -record (port.rs:281-297).
+ * pre_e (u^n of each edge) and post_e (:1123-1249) run on the device every
+ * step from the next one on, in registration order with the other
+ * post_e operators. The openEMS lumped order and cylindrical edge copies
+ * are not supported. */
+typedef struct {
+    int32_t axis, n_edges;
+    const int32_t *nodes;
+    const double *cb, *g_src;
+    int32_t series, src_in_branch;
+    int32_t n_terms;
+    const double *term_weight;
+    int32_t n_drive;
+    const double *drive_weight;
+    int32_t branch;
+    const double *two_l_dt, *inv_c, *z;
+    const double *rec_g, *rec_c;
+    int32_t port_columns;
+} ems_lumped_desc;
+int ems_gpu_add_lumped(ems_gpu *h, const ems_lumped_desc *d, int *id);
+
+/* Source::value (lumped.rs:241-247) of term `term` (drive terms numbered
+ * after the source terms) at h_time = (n + 1/2)dt, tabulated for steps
+ * n < count (absolute step index); 0 beyond. Replaces the previous table. */
+int ems_gpu_lumped_table(ems_gpu *h, int id, int term, const double *values, uint64_t count);
+
+/* Recorded series of a lumped element: which 0 the terminal current
+ * (LumpedElement::current), 1 the port voltage, 2 the port current
+ * (LumpedPort::voltage / current). *count = samples recorded; the first
+ * was taken at the step *first_step (NULL allowed). */
+int ems_gpu_lumped_values(ems_gpu *h, int id, int which, double *out, uint64_t cap,
+                          uint64_t *count, uint64_t *first_step);
+
+/* Branch state j^n, q^n and the last j-bar per edge (branch_energy,
+ * branch_current, lumped.rs:1033-1087). */
+int ems_gpu_lumped_branch(ems_gpu *h, int id, double *j, double *q, double *j_avg);
+
 /* Engine::run (src/engine/mod.rs:825-828). */
 int ems_gpu_run(ems_gpu *h, uint64_t n);
 /* Engine::step_with_energy (src/engine/mod.rs:808-823): one step, *w = the

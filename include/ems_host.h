@@ -91,6 +91,18 @@ double ems_gaussian_duration(double fc);
 void ems_gaussian_table(double f0, double fc, double dt, double offset, uint64_t count,
                         float *amp);
 
+/* fold_parallel (src/engine/lumped.rs:275-282): adds conductance g [S] and
+ * capacitance c [F] in parallel to an E edge with coefficients (ca, cb).
+ * EMS_E_INVALID for cb = 0 (a PEC edge). */
+int ems_fold_parallel(float ca, float cb, double dt, double g, double c, float *ca_out,
+                      float *cb_out);
+
+/* dft (src/post/dft.rs:218-222, RunningDft :75-190): X(f) = sum x(t_n)
+ * e^{-j2 pi f t_n} dt with t_n = t0 + n dt; out[2k], out[2k+1] = Re, Im of
+ * X(freqs[k]). */
+void ems_dft(double t0, double dt, const double *x, uint64_t n, const double *freqs, int nf,
+             double *out);
+
 /* RunLimits + EndCriteria (src/engine/run.rs:22-139, src/fdtd.rs:428-605). */
 typedef struct {
     uint64_t max_timesteps;

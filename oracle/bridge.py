@@ -44,6 +44,10 @@ orc_set_lane = _sig("orc_set_lane", _i, _p, _i, _fp, _u64, _u32p)
 orc_add_cpml = _sig("orc_add_cpml", _i, _p, _i, _i, _i, _ip, _ip, _fp, _fp, _fp)
 orc_add_source = _sig("orc_add_source", _i, _p, _i, _i, _ip, _ip, _fp, _fp, _u64)
 orc_add_probe = _sig("orc_add_probe", _i, _p, _i, _i, _ip, _ip, _dp, _u64)
+orc_add_lumped = _sig("orc_add_lumped", _i, _p, _p)
+orc_lumped_table = _sig("orc_lumped_table", _i, _p, _i, _i, _dp, _u64)
+orc_lumped_values = _sig("orc_lumped_values", _u64, _p, _i, _i, _dp, _u64)
+orc_lumped_branch = _sig("orc_lumped_branch", _i, _p, _i, _dp, _dp, _dp)
 orc_run = _sig("orc_run", None, _p, _u64)
 orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
 orc_energy = _sig("orc_energy", _d, _p)
@@ -85,6 +89,40 @@ class OracleEngine:
                 assert orc_set_lane(self._h, l, P(rows, C.c_float), rows.shape[0],
                                     P(index, C.c_uint32)) == 0
         self.ops = []
+        self._tables = []
+        self._lumped = {}
+
+    def _set_lane(self, l, lane):
+        n = self.grid.nodes()
+        lane = np.ascontiguousarray(lane)
+        pooled = E.pool_lane(lane)
+        if pooled is None:
+            rows = lane.reshape(-1, n[2])
+            assert orc_set_lane(self._h, l, P(rows, C.c_float), rows.shape[0], None) == 0
+        else:
+            rows, index = pooled
+            assert orc_set_lane(self._h, l, P(rows, C.c_float), rows.shape[0],
+                                P(index, C.c_uint32)) == 0
+
+    def _ensure_tables(self, upto):
+        for oid, t in self._tables:
+            def up(term, vals, oid=oid):
+                v = np.ascontiguousarray(vals)
+                assert orc_lumped_table(self._h, oid, term, P(v, C.c_double), v.size) == 0
+            t.ensure(upto, up)
+
+    def _lumped_values(self, oid, which):
+        n = orc_lumped_values(self._h, oid, which, None, 0)
+        out = np.empty(n, dtype=np.float64)
+        orc_lumped_values(self._h, oid, which, P(out, C.c_double), n)
+        return out, self._lumped[oid][1]
+
+    def _lumped_branch(self, oid):
+        n = self._lumped[oid][0]
+        j, q, jav = (np.zeros(n) for _ in range(3))
+        assert orc_lumped_branch(self._h, oid, P(j, C.c_double), P(q, C.c_double),
+                                 P(jav, C.c_double)) == 0
+        return j, q, jav
 
     def close(self):
         if self._h:
@@ -122,15 +160,31 @@ class OracleEngine:
             op._oid = orc_add_probe(h, int(op.quantity), len(op.entries), P(axis, C.c_int),
                                     P(nodes, C.c_int), P(w, C.c_double), op._interval)
             assert op._oid >= 0
+        elif hasattr(op, "_plan"):
+            from ems_amd import lumped as LM
+            plan = op._plan(self.grid, self.dt, self._dense_lanes)
+            a = plan["axis"]
+            self._dense_lanes = self._dense_lanes.copy()
+            for l, lane in ((a, plan["ca"]), (3 + a, plan["cb"])):
+                self._dense_lanes[l] = lane
+                self._set_lane(l, lane)
+            d, keep = LM.make_desc(plan)
+            oid = orc_add_lumped(h, C.addressof(d))
+            assert oid >= 0
+            self._lumped[oid] = (len(plan["edges"]), self.timestep())
+            self._tables.append((oid, LM.SourceTables(plan, self.dt)))
+            op._bind(self, oid, plan)
         else:
             raise TypeError(type(op))
         self.ops.append(op)
         return op
 
     def run(self, n):
+        self._ensure_tables(self.timestep() + int(n))
         orc_run(self._h, int(n))
 
     def step_with_energy(self):
+        self._ensure_tables(self.timestep() + 1)
         return orc_step_with_energy(self._h)
 
     def energy(self):

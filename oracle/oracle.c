@@ -69,7 +69,25 @@ typedef struct {
     float *b, *a, *k, *psi;
 } cpml_t;
 
-enum { OP_SOURCE = 1, OP_PROBE = 2 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3 };
+
+/* An installed lumped element (lumped.rs:367-406, Installed + Branch +
+ * CurrentRecord), optionally a LumpedPort (engine/port.rs:94-104). */
+typedef struct {
+    int axis, n, series, src_in_branch, branch, nterm, ndrive, columns;
+    size_t *idx;
+    double *cb, *gsrc;
+    double *tw;               /* nterm+ndrive weight rows of n */
+    double **tv;              /* per term: value table by step */
+    uint64_t *tlen;
+    double *v, *vd;           /* scratch: source / drive voltage per edge */
+    double *two_l_dt, *inv_c, *z, *j, *q, *javg;
+    double *uprev;            /* u^n saved by pre_e */
+    int record;
+    double *rg, *rc;
+    double *rec, *pv, *pi;    /* record values, port V and I */
+    uint64_t nrec, cap;
+} lumped_t;
 
 typedef struct {
     int type;
@@ -87,6 +105,8 @@ typedef struct {
     uint64_t interval;
     double *vals;
     uint64_t nvals, cap;
+    /* lumped element */
+    lumped_t *lp;
 } op_t;
 
 struct orc_engine {
@@ -370,6 +390,115 @@ static void probe_sample(orc_engine *e, op_t *o) {
     o->vals[o->nvals++] = v;
 }
 
+/* LumpedElement::pre_e_impl (lumped.rs:1106-1121): u^n of every edge, as
+ * f64, for the branch and the current record. pre_e is movable
+ * (lumped.rs:1257-1259): it may run anywhere between the previous post_e
+ * gap and the E update, where u does not change. */
+static void lumped_pre_e(orc_engine *e, op_t *o) {
+    lumped_t *L = o->lp;
+    if (!L->branch && !L->record) return;
+    const float *u = e->u[L->axis];
+    for (int q = 0; q < L->n; q++) L->uprev[q] = (double)u[L->idx[q]];
+}
+
+static double lumped_value(const lumped_t *L, int t, uint64_t n) {
+    return n < L->tlen[t] ? L->tv[t][n] : 0.0;
+}
+
+/* LumpedElement::post_e_impl (lumped.rs:1123-1235) without the openEMS
+ * order and cylindrical copies, Installed::add_sources (:1240-1249), and
+ * LumpedPort::post_e (engine/port.rs:281-297). The source values are
+ * Source::value at h_time = (n + 1/2)dt, tabulated by step. */
+static void lumped_post_e(orc_engine *e, op_t *o, uint64_t n) {
+    lumped_t *L = o->lp;
+    float *u = e->u[L->axis];
+    const int ne = L->n;
+    int active = 0, drive = 0;
+    if (L->nterm) {
+        for (int q = 0; q < ne; q++) L->v[q] = 0.0;
+        for (int t = 0; t < L->nterm; t++) {
+            double val = lumped_value(L, t, n);
+            if (val != 0.0) {
+                active = 1;
+                for (int q = 0; q < ne; q++) L->v[q] += L->tw[(size_t)t * ne + q] * val;
+            }
+        }
+    }
+    if (L->ndrive) {
+        for (int q = 0; q < ne; q++) L->vd[q] = 0.0;
+        for (int t = L->nterm; t < L->nterm + L->ndrive; t++) {
+            double val = lumped_value(L, t, n);
+            if (val != 0.0) {
+                drive = 1;
+                for (int q = 0; q < ne; q++) L->vd[q] += L->tw[(size_t)t * ne + q] * val;
+            }
+        }
+    }
+    if (active && !L->src_in_branch)
+        for (int q = 0; q < ne; q++) {
+            double add = L->cb[q] * L->gsrc[q] * L->v[q];
+            if (add != 0.0) u[L->idx[q]] = (float)((double)u[L->idx[q]] + add);
+        }
+    if (L->branch) {
+        for (int q = 0; q < ne; q++) {
+            double v_br = active && L->src_in_branch ? L->v[q] : 0.0;
+            if (drive) v_br += L->vd[q];
+            double u_star = (double)u[L->idx[q]];
+            if (L->branch == 2) {
+                /* explicit inductor: dt/L_e = 2/two_l_dt */
+                L->j[q] += 2.0 * (L->uprev[q] - v_br) / L->two_l_dt[q];
+                L->javg[q] = L->j[q];
+                u[L->idx[q]] = (float)(u_star - L->cb[q] * L->j[q]);
+                continue;
+            }
+            double k = L->two_l_dt[q] * L->j[q] - L->q[q] * L->inv_c[q] - v_br + 0.5 * L->uprev[q];
+            double z = L->z[q];
+            double u_new = (z * u_star - L->cb[q] * k) / (z + 0.5 * L->cb[q]);
+            double jb = (k + 0.5 * u_new) / z;
+            L->j[q] = 2.0 * jb - L->j[q];
+            L->q[q] += e->dt * jb;
+            L->javg[q] = jb;
+            u[L->idx[q]] = (float)u_new;
+        }
+    }
+    if (L->record) {
+        if (L->nrec == L->cap) {
+            L->cap = L->cap ? 2 * L->cap : 1024;
+            L->rec = (double *)realloc(L->rec, L->cap * sizeof(double));
+            L->pv = (double *)realloc(L->pv, L->cap * sizeof(double));
+            L->pi = (double *)realloc(L->pi, L->cap * sizeof(double));
+        }
+        double sum = 0.0;
+        for (int q = 0; q < ne; q++) {
+            double u0 = L->uprev[q], u1 = (double)u[L->idx[q]];
+            double v = active && !L->src_in_branch ? L->v[q] : 0.0;
+            sum += L->rg[q] * (0.5 * (u0 + u1) - v) + L->rc[q] * (u1 - u0) / e->dt;
+            if (L->branch) sum += L->javg[q];
+        }
+        double cur = sum / (double)L->series;
+        L->rec[L->nrec] = cur;
+        if (L->columns) {
+            double sv = -0.0; /* Iterator::sum */
+            for (int q = 0; q < ne; q++) sv += (double)u[L->idx[q]];
+            L->pi[L->nrec] = -cur;
+            L->pv[L->nrec] = sv / (double)L->columns;
+        }
+        L->nrec++;
+    }
+}
+
+/* LumpedElement::branch_energy (lumped.rs:1075-1087). */
+static double lumped_energy(const orc_engine *e, const op_t *o) {
+    const lumped_t *L = o->lp;
+    if (!L->branch) return 0.0;
+    double s = -0.0;
+    for (int q = 0; q < L->n; q++) {
+        double half_l = 0.25 * L->two_l_dt[q] * e->dt;
+        s += half_l * L->j[q] * L->j[q] + 0.5 * L->inv_c[q] * L->q[q] * L->q[q];
+    }
+    return s;
+}
+
 static int is_hsrc(const op_t *o) { return o->type == OP_SOURCE && (o->kind == 2 || o->kind == 3); }
 static int is_esrc(const op_t *o) { return o->type == OP_SOURCE && (o->kind == 0 || o->kind == 1); }
 static int is_iprobe(const op_t *o) { return o->type == OP_PROBE && o->quantity == 1; }
@@ -381,6 +510,7 @@ static int needs(const op_t *o, uint64_t n) {
         if (o->kind == 1 || o->kind == 3) return 1;
         return op_amp(o, n) != 0.0f;
     }
+    if (o->type == OP_LUMPED) return 1; /* pre_e / post_e every step */
     return n % o->interval == 0;
 }
 
@@ -392,9 +522,9 @@ static void one_step(orc_engine *e, int energy) {
     for (int q = 0; q < e->nops; q++) {
         op_t *o = &e->ops[q];
         if (!needs(o, n)) continue;
-        if (is_esrc(o) || is_vprobe(o)) need_post_e = 1;
+        if (is_esrc(o) || is_vprobe(o) || o->type == OP_LUMPED) need_post_e = 1;
         if (is_hsrc(o)) mid = 1, mid_movable = 0;
-        if (is_iprobe(o)) mid = 1;
+        if (is_iprobe(o) || o->type == OP_LUMPED) mid = 1;
     }
     int gap_b = mid && !mid_movable;
     int moved = mid && mid_movable;
@@ -409,6 +539,13 @@ static void one_step(orc_engine *e, int energy) {
             if (is_hsrc(o)) source_apply(e, o, n);
             if (is_iprobe(o) && n % o->interval == 0) probe_sample(e, o);
         }
+        for (int q = 0; q < e->nops; q++)
+            if (e->ops[q].type == OP_LUMPED) lumped_pre_e(e, &e->ops[q]);
+    } else {
+        /* moved pre_e hooks (mod.rs:966-971): u is unchanged since the
+         * previous post_e gap, so they read the same values here. */
+        for (int q = 0; q < e->nops; q++)
+            if (e->ops[q].type == OP_LUMPED) lumped_pre_e(e, &e->ops[q]);
     }
     parallel(e, 2);
     if (gap_a) {
@@ -421,6 +558,7 @@ static void one_step(orc_engine *e, int energy) {
             op_t *o = &e->ops[q];
             if (is_esrc(o)) source_apply(e, o, n);
             if (is_vprobe(o) && n % o->interval == 0) probe_sample(e, o);
+            if (o->type == OP_LUMPED) lumped_post_e(e, o, n);
         }
     }
     e->step++;
@@ -493,6 +631,14 @@ void orc_destroy(orc_engine *e) {
     for (int q = 0; q < e->nops; q++) {
         op_t *o = &e->ops[q];
         free(o->axis), free(o->idx), free(o->wf), free(o->amp), free(o->wd), free(o->vals);
+        lumped_t *L = o->lp;
+        if (L) {
+            for (int t = 0; t < L->nterm + L->ndrive; t++) free(L->tv[t]);
+            free(L->tv), free(L->tlen), free(L->idx), free(L->cb), free(L->gsrc), free(L->tw);
+            free(L->v), free(L->vd), free(L->two_l_dt), free(L->inv_c), free(L->z), free(L->j);
+            free(L->q), free(L->javg), free(L->uprev), free(L->rg), free(L->rc), free(L->rec);
+            free(L->pv), free(L->pi), free(L);
+        }
     }
     free(e->ops);
     free(e->zero), free(e->plane_energy), free(e->slab_start);
@@ -616,15 +762,105 @@ int orc_add_probe(orc_engine *e, int quantity, int n_entries, const int *axis, c
     return id;
 }
 
+static double *dup_d(const double *src, size_t n) {
+    double *d = (double *)calloc(n ? n : 1, sizeof(double));
+    if (src) memcpy(d, src, n * sizeof(double));
+    return d;
+}
+
+int orc_add_lumped(orc_engine *e, const orc_lumped_desc *d) {
+    if (d->axis < 0 || d->axis > 2 || d->n_edges < 1 || d->series < 1 || !d->cb || !d->g_src)
+        return -1;
+    if (d->branch < 0 || d->branch > 2 || (d->branch && (!d->two_l_dt || !d->inv_c || !d->z)))
+        return -1;
+    if (d->port_columns && !d->rec_g) return -1;
+    for (int q = 0; q < d->n_edges; q++) {
+        const int *nd = d->nodes + 3 * q;
+        if (!node_ok(e, nd) || nd[d->axis] + 1 >= e->n[d->axis]) return -1;
+    }
+    int id = add_op(e);
+    op_t *o = &e->ops[id];
+    o->type = OP_LUMPED;
+    lumped_t *L = (lumped_t *)calloc(1, sizeof(lumped_t));
+    o->lp = L;
+    const int ne = d->n_edges;
+    L->axis = d->axis, L->n = ne, L->series = d->series, L->src_in_branch = d->src_in_branch;
+    L->branch = d->branch, L->nterm = d->n_terms, L->ndrive = d->n_drive;
+    L->columns = d->port_columns;
+    L->idx = (size_t *)malloc(ne * sizeof(size_t));
+    for (int q = 0; q < ne; q++) {
+        const int *nd = d->nodes + 3 * q;
+        L->idx[q] = ((size_t)nd[0] * e->n[1] + nd[1]) * e->n[2] + nd[2];
+    }
+    L->cb = dup_d(d->cb, ne), L->gsrc = dup_d(d->g_src, ne);
+    int nt = d->n_terms + d->n_drive;
+    L->tw = (double *)calloc((size_t)(nt ? nt : 1) * ne, sizeof(double));
+    if (d->n_terms) memcpy(L->tw, d->term_weight, (size_t)d->n_terms * ne * sizeof(double));
+    if (d->n_drive)
+        memcpy(L->tw + (size_t)d->n_terms * ne, d->drive_weight,
+               (size_t)d->n_drive * ne * sizeof(double));
+    L->tv = (double **)calloc(nt ? nt : 1, sizeof(double *));
+    L->tlen = (uint64_t *)calloc(nt ? nt : 1, sizeof(uint64_t));
+    L->v = dup_d(NULL, ne), L->vd = dup_d(NULL, ne);
+    if (d->branch) {
+        L->two_l_dt = dup_d(d->two_l_dt, ne), L->inv_c = dup_d(d->inv_c, ne);
+        L->z = dup_d(d->z, ne);
+    }
+    L->j = dup_d(NULL, ne), L->q = dup_d(NULL, ne), L->javg = dup_d(NULL, ne);
+    L->uprev = dup_d(NULL, ne);
+    L->record = d->rec_g != NULL;
+    if (L->record) L->rg = dup_d(d->rec_g, ne), L->rc = dup_d(d->rec_c, ne);
+    return id;
+}
+
+int orc_lumped_table(orc_engine *e, int id, int term, const double *values, uint64_t count) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_LUMPED) return -1;
+    lumped_t *L = e->ops[id].lp;
+    if (term < 0 || term >= L->nterm + L->ndrive) return -1;
+    free(L->tv[term]);
+    L->tv[term] = dup_d(values, count);
+    L->tlen[term] = count;
+    return 0;
+}
+
+uint64_t orc_lumped_values(orc_engine *e, int id, int which, double *out, uint64_t cap) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_LUMPED) return 0;
+    lumped_t *L = e->ops[id].lp;
+    const double *src = which == 0 ? L->rec : which == 1 ? L->pv : L->pi;
+    if (!src || (which && !L->columns)) return 0;
+    uint64_t n = L->nrec < cap ? L->nrec : cap;
+    if (out) memcpy(out, src, n * sizeof(double));
+    return L->nrec;
+}
+
+int orc_lumped_branch(orc_engine *e, int id, double *j, double *q, double *javg) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_LUMPED) return -1;
+    lumped_t *L = e->ops[id].lp;
+    memcpy(j, L->j, L->n * sizeof(double));
+    memcpy(q, L->q, L->n * sizeof(double));
+    memcpy(javg, L->javg, L->n * sizeof(double));
+    return 0;
+}
+
 void orc_run(orc_engine *e, uint64_t n) {
     for (uint64_t t = 0; t < n; t++) one_step(e, 0);
 }
 
-/* Engine::step_with_energy (mod.rs:819-823): operator energies (all zero
- * for these kinds; Sum folds from -0.0) plus the in-step invariant. */
+/* Engine::operator_energy (mod.rs:753-755): Operator::energy in
+ * registration order, Sum folding from -0.0; 0 for every kind but a lumped
+ * element's branch. The CPML boundary counts as one operator. */
+static double ops_energy(const orc_engine *e) {
+    double w = -0.0;
+    if (e->ncpml) w += 0.0;
+    for (int q = 0; q < e->nops; q++)
+        w += e->ops[q].type == OP_LUMPED ? lumped_energy(e, &e->ops[q]) : 0.0;
+    return w;
+}
+
+/* Engine::step_with_energy (mod.rs:819-823): operator energies before the
+ * step plus the in-step invariant. */
 double orc_step_with_energy(orc_engine *e) {
-    double w_ops = -0.0;
-    for (int q = 0; q < e->nops; q++) w_ops += 0.0;
+    double w_ops = ops_energy(e);
     one_step(e, 1);
     return plane_sum(e) + w_ops;
 }
@@ -632,8 +868,7 @@ double orc_step_with_energy(orc_engine *e) {
 /* Engine::energy (mod.rs:722-726): read-only evaluation of the state. */
 double orc_energy(orc_engine *e) {
     parallel(e, 3);
-    double w_ops = -0.0;
-    for (int q = 0; q < e->nops; q++) w_ops += 0.0;
+    double w_ops = ops_energy(e);
     return plane_sum(e) + w_ops;
 }
 

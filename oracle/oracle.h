@@ -65,6 +65,35 @@ int orc_add_source(orc_engine *e, int kind, int n_edges, const int *axis,
 int orc_add_probe(orc_engine *e, int quantity, int n_entries, const int *axis,
                   const int *nodes, const double *w, uint64_t interval);
 
+/* Lumped element (lumped.rs:367-406 after install, :1106-1249) over edges
+ * along `axis` starting at nodes[3*q]: final cb and source conductance per
+ * edge, M = series, source (n_terms) and branch drive (n_drive) weight rows
+ * of n_edges each, branch 0 none / 1 trapezoidal series R+L+C / 2 explicit
+ * inductor with 2L/dt, 1/C, Z per edge, terminal-current record g, c per
+ * edge (rec_g NULL: none), port_columns > 0: a LumpedPort (port.rs:94-297)
+ * recording V = sum(u)/columns and I = -record. Returns the operator id. */
+typedef struct {
+    int axis, n_edges;
+    const int *nodes;
+    const double *cb, *g_src;
+    int series, src_in_branch;
+    int n_terms;
+    const double *term_weight;
+    int n_drive;
+    const double *drive_weight;
+    int branch;
+    const double *two_l_dt, *inv_c, *z;
+    const double *rec_g, *rec_c;
+    int port_columns;
+} orc_lumped_desc;
+int orc_add_lumped(orc_engine *e, const orc_lumped_desc *d);
+/* values[n] = Source::value((n + 1/2)dt) of term `term` (drive terms after
+ * the source terms) for steps n < count, 0 beyond. */
+int orc_lumped_table(orc_engine *e, int id, int term, const double *values, uint64_t count);
+/* which 0: terminal current record, 1: port V, 2: port I. Returns the count. */
+uint64_t orc_lumped_values(orc_engine *e, int id, int which, double *out, uint64_t cap);
+int orc_lumped_branch(orc_engine *e, int id, double *j, double *q, double *javg);
+
 void orc_run(orc_engine *e, uint64_t n);
 double orc_step_with_energy(orc_engine *e);
 double orc_energy(orc_engine *e);

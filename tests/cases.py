@@ -160,3 +160,94 @@ def staggered_energy(e, dt):
                 c = np.where(cb == 0.0, 0.0, dt * (1.0 + ca) / (2.0 * cb))
             w += float(np.sum(0.5 * c * x * x))
     return w
+
+
+# ---------------------------------------------------------------- lumped --
+# Models of the reference's lumped-element tests (tests/lumped.rs), for any
+# engine kind.
+from ems_amd.lumped import LumpedBox, LumpedElement, LumpedPort, Topology  # noqa: E402
+
+LOOP = {"port_x": 8, "load_x": 12, "y": 10, "z0": 9, "z1": 11}
+FREQS = [50e6, 100e6, 200e6, 500e6, 1e9, 2e9]
+Z0_PORT = 50.0
+
+
+def ramp(tr):
+    """tests/lumped.rs:147-159: raised-cosine 0 -> 1 over tr."""
+    def f(t):
+        if t >= tr:
+            return 1.0
+        if t <= 0.0:
+            return 0.0
+        return 0.5 * (1.0 - math.cos(math.pi * t / tr))
+    return f
+
+
+def loop_column(x):
+    return LumpedBox(E.Axis.Z, [x, LOOP["y"], LOOP["z0"]], [x, LOOP["y"], LOOP["z1"]])
+
+
+def loop_materials(g):
+    """The two PEC strips of the loop (tests/lumped.rs:59-65)."""
+    m = E.CellMaterials.vacuum(g)
+    for z in (LOOP["z0"], LOOP["z1"]):
+        m.fill_pec_box([(LOOP["port_x"], LOOP["load_x"] + 1), (LOOP["y"], LOOP["y"] + 1), (z, z + 1)])
+    return m
+
+
+def run_loop(kind, load, steps=4000, threads=2, freqs=FREQS):
+    """run_loop (tests/lumped.rs:105-132): load is "short", "open" or a
+    function of the load column returning a LumpedElement."""
+    g = E.Grid.uniform([20, 20, 20], [1e-3] * 3)
+    m = loop_materials(g)
+    if load == "short":
+        m.fill_pec_box([(LOOP["load_x"], LOOP["load_x"] + 1), (LOOP["y"], LOOP["y"] + 1),
+                        (LOOP["z0"], LOOP["z1"] + 1)])
+    e = make(kind, g, m, threads=threads)
+    p = e.add_operator(LumpedPort(loop_column(LOOP["port_x"]), Z0_PORT)
+                       .excite_waveform(E.Waveform.gaussian(0.0, 3e9), 1.0))
+    if callable(load):
+        e.add_operator(load(loop_column(LOOP["load_x"])))
+    e.run(steps)
+    return e, p
+
+
+def two_port(kind, excite, threads=2, steps=5000):
+    """two_port (tests/lumped.rs:452-496)."""
+    g = E.Grid.uniform([20, 20, 20], [1e-3] * 3)
+    y, z0, z1 = 10, 9, 11
+    m = E.CellMaterials.vacuum(g)
+    for z in (z0, z1):
+        m.fill_pec_box([(5, 16), (y, y + 1), (z, z + 1)])
+    col = lambda x: LumpedBox(E.Axis.Z, [x, y, z0], [x, y, z1])  # noqa: E731
+    e = make(kind, g, m, threads=threads)
+    w = E.Waveform.gaussian(0.0, 3e9)
+    p1 = LumpedPort(col(5), 50.0)
+    p2 = LumpedPort(col(15), 75.0)
+    if excite == 1:
+        p1.excite_waveform(w, 1.0)
+    else:
+        p2.excite_waveform(w, 1.0)
+    e.add_operator(p1)
+    e.add_operator(p2)
+    e.add_operator(LumpedElement.new(col(7)).c(0.5e-12))
+    e.add_operator(LumpedElement.new(col(10)).r(30.0).c(2e-12).topology(Topology.Series))
+    e.run(steps)
+    return e, p1, p2
+
+
+def lc_cavity(kind, threads=1):
+    """lc_loaded_cavity_conserves_total_energy (tests/lumped.rs:585-640):
+    parallel L||C and series L+C in a 12 mm PEC cube, the parallel element
+    charged to 0.25 V per edge."""
+    g = E.Grid.uniform([12, 12, 12], [1e-3] * 3)
+    e = make(kind, g, threads=threads)
+    par = LumpedBox(E.Axis.Z, [4, 6, 4], [4, 6, 8])
+    ser = LumpedBox(E.Axis.X, [6, 3, 7], [9, 3, 7])
+    hp = e.add_operator(LumpedElement.new(par).l(2e-9).c(0.2e-12))
+    hs = e.add_operator(LumpedElement.new(ser).l(3e-9).c(0.1e-12).topology(Topology.Series))
+    u = e.u_component(E.Axis.Z)
+    for p in par.edges():
+        u[p] = 0.25
+    e.set_component(0, E.Axis.Z, u)
+    return e, hp, hs
