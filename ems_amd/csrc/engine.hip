@@ -701,7 +701,7 @@ __global__ void __launch_bounds__(256) copy_f4(const float4 *__restrict__ a, flo
 
 // Serial operators (excite.rs, probe.rs, lumped.rs) and the gap rules of
 // Engine::advance (mod.rs:873-915).
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4 };
 
 // An installed lumped element (lumped.rs:367-406) and, with columns > 0,
 // its LumpedPort records (engine/port.rs:94-104).
@@ -756,6 +756,7 @@ __device__ __forceinline__ bool needs(const DevOp &o, uint64_t n) {
     return op_amp(o, n) != 0.0f;
   }
   if (o.type == OP_LUMPED) return true;  // post_e every step
+  if (o.type == OP_POLES) return false;  // slab hooks, not gap hooks
   return n % o.interval == 0;
 }
 
@@ -960,6 +961,82 @@ __global__ void __launch_bounds__(256) gap_post_e(Fields f, Fields fp, int captu
   if (threadIdx.x == 0) *step = n + 1;
 }
 
+// ---- dispersive media (dispersive.rs) ----
+
+struct DevPoleConst {
+  double a, inv_c, inv_z;
+  int kind;  // 0 trapezoidal, 1 leapfrog
+};
+// One side of a medium: edges (CSR over poles) and their state.
+struct DevPoleSide {
+  int n;
+  const int32_t *axis;
+  const uint64_t *idx;
+  const double *sum_inv_z;
+  const uint32_t *pole_end, *cidx;
+  double *j, *q;
+  float *prev;  // the field at the start of the update
+};
+
+// Side::correct / step_edge (dispersive.rs:1183-1343, 1392-1470) for
+// trapezoidal and leapfrog poles, a thread per edge: `fld` the field the
+// side corrects (u or i), lane 3 + axis (cb) or 9 + axis (db). With
+// `fp_prev` the saved value is read from `fp` (the state before a fused
+// step), else from `prev`.
+template <bool ESIDE>
+__global__ void __launch_bounds__(256) poles_correct(Fields f, Fields fp, int fp_prev, Lanes L,
+                                                     int nz, DevPoleSide s,
+                                                     const DevPoleConst *__restrict__ cs,
+                                                     double dt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= s.n) return;
+  const int ax = s.axis[e];
+  const uint64_t idx = s.idx[e];
+  float *fld = ESIDE ? f.u[ax] : f.i[ax];
+  const float *pf = ESIDE ? fp.u[ax] : fp.i[ax];
+  const size_t r = idx / (uint64_t)nz;
+  const int k = (int)(idx - r * (uint64_t)nz);
+  const double cb = (double)coef(L.l[(ESIDE ? 3 : 9) + ax], r, k, nz, idx);
+  const double prev = (double)(fp_prev ? pf[idx] : s.prev[e]);
+  const uint32_t p0 = e ? s.pole_end[e - 1] : 0u, p1 = s.pole_end[e];
+  const double u_star = (double)fld[idx];
+  const double half_prev = 0.5 * prev;
+  double sum = 0.0;
+  for (uint32_t p = p0; p < p1; p++) {
+    const DevPoleConst c = cs[s.cidx[p]];
+    if (c.kind == 1)
+      sum += c.a * s.j[p] + c.inv_z * (prev - s.q[p] * c.inv_c);
+    else
+      sum += (c.a * s.j[p] - s.q[p] * c.inv_c + half_prev) * c.inv_z;
+  }
+  const float u_new = (float)((u_star - cb * sum) / (1.0 + 0.5 * cb * s.sum_inv_z[e]));
+  const double half_new = 0.5 * (double)u_new;
+  for (uint32_t p = p0; p < p1; p++) {
+    const DevPoleConst c = cs[s.cidx[p]];
+    const double jo = s.j[p], qo = s.q[p];
+    if (c.kind == 1) {
+      const double jn = c.a * jo + c.inv_z * (prev - qo * c.inv_c);
+      s.j[p] = jn;
+      s.q[p] = qo + dt * jn;
+    } else {
+      const double kk = c.a * jo - qo * c.inv_c + half_prev;
+      const double jb = (kk + half_new) * c.inv_z;
+      s.j[p] = 2.0 * jb - jo;
+      s.q[p] = qo + dt * jb;
+    }
+  }
+  fld[idx] = u_new;
+}
+
+// Side::capture (dispersive.rs:1173-1179): the field of each edge.
+template <bool ESIDE>
+__global__ void __launch_bounds__(256) poles_capture(Fields f, DevPoleSide s) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= s.n) return;
+  const int ax = s.axis[e];
+  s.prev[e] = (ESIDE ? f.u[ax] : f.i[ax])[s.idx[e]];
+}
+
 // ---- energy invariant (kernel.rs:119-244, 364-423; pool.rs:673-692) ----
 
 // One H row's share, read-only (COMMIT = false): 4 threads per row, lane l
@@ -1103,6 +1180,11 @@ struct HostOp {
   uint64_t *d_tlen = nullptr;
   std::vector<double> two_l_dt, inv_c;  // host copies for the branch energy
   std::vector<void *> mem;              // every allocation of this op
+  // dispersive medium: sides (0 electric, 1 magnetic), constants
+  DevPoleSide side[2]{};
+  DevPoleConst *d_consts = nullptr;
+  std::vector<double> c_inv_c, c_half_l;  // host copies of the constants
+  std::vector<uint32_t> side_cidx[2];
 };
 
 }  // namespace
@@ -1133,7 +1215,7 @@ struct ems_gpu {
   uint64_t *d_step = nullptr;
   // fused schedule and its tiling
   bool fused = true, prefetch = true, rows = false;
-  int xcd_group = getenv("EMS_XCD_GROUP") ? atoi(getenv("EMS_XCD_GROUP")) : 2;
+  int xcd_group = 2;  // tile rows per XCD visit (2 measured best of 0, 1, 2, 4, 8)
   int tj = 7, nchunk = 0;
   // energy scratch
   double *d_rowsum = nullptr, *d_chains = nullptr, *d_planes = nullptr, *d_w = nullptr;
@@ -1197,10 +1279,47 @@ bool has_mid_ops(const ems_gpu *h) {
 // Whether steps run fused: not while an H source needs the serial gap
 // between the H and E updates (GAP_B, mod.rs:899-913; pool.rs:870-875).
 // The fused kernel keeps the slabs that reach a tile in a 32-bit mask.
+bool has_magnetic_poles(const ems_gpu *h) {
+  for (auto &o : h->ops)
+    if (o.type == OP_POLES && o.side[1].n > 0) return true;
+  return false;
+}
+
 bool use_fused(const ems_gpu *h) {
   size_t ns = 0;
   for (auto &s : h->slabs) ns += s.len != 0;
-  return h->fused && !has_hsrc(h) && ns <= 32;
+  return h->fused && !has_hsrc(h) && !has_magnetic_poles(h) && ns <= 32;
+}
+
+// The dispersive media as slab hooks (dispersive.rs:1700-1750), in
+// registration order. stage 0 (after the H update and CPML-H): correct the
+// magnetic poles, save u of the electric edges. stage 1 (after the E update
+// and CPML-E): correct the electric poles, save i of the magnetic edges;
+// with fp_prev the electric edges' u^n is read from `fp` instead.
+int launch_poles(ems_gpu *h, int stage, const Fields &f, const Fields &fp, bool fp_prev) {
+  for (auto &o : h->ops) {
+    if (o.type != OP_POLES) continue;
+    const DevPoleSide &se = o.side[0], &sh = o.side[1];
+    const int nz = h->n[2];
+    if (stage == 0) {
+      if (sh.n)
+        hipLaunchKernelGGL(poles_correct<false>, dim3((sh.n + 255) / 256), dim3(256), 0, h->stream,
+                           f, f, 0, lanes_arg(h), nz, sh, (const DevPoleConst *)o.d_consts, h->dt);
+      if (se.n)
+        hipLaunchKernelGGL(poles_capture<true>, dim3((se.n + 255) / 256), dim3(256), 0, h->stream,
+                           f, se);
+    } else {
+      if (se.n)
+        hipLaunchKernelGGL(poles_correct<true>, dim3((se.n + 255) / 256), dim3(256), 0, h->stream,
+                           f, fp, fp_prev ? 1 : 0, lanes_arg(h), nz, se,
+                           (const DevPoleConst *)o.d_consts, h->dt);
+      if (sh.n)
+        hipLaunchKernelGGL(poles_capture<false>, dim3((sh.n + 255) / 256), dim3(256), 0, h->stream,
+                           f, sh);
+    }
+    CK(h, hipGetLastError());
+  }
+  return 0;
 }
 
 int launch_h(ems_gpu *h, const Fields &f) {
@@ -1419,14 +1538,17 @@ int enqueue_step(ems_gpu *h, int &par) {
   if (use_fused(h)) {
     if ((rc = launch_fused(h, par))) return rc;
     par ^= 1;
+    if ((rc = launch_poles(h, 1, h->fb[par], h->fb[par ^ 1], true))) return rc;
     return launch_gap_post_e(h, h->fb[par], h->fb[par ^ 1], true);
   }
   const Fields &f = h->fb[par];
   if ((rc = launch_h(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 0, par))) return rc;
+  if ((rc = launch_poles(h, 0, f, f, false))) return rc;
   if ((rc = launch_gap_mid(h, f))) return rc;
   if ((rc = launch_e(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 1, par))) return rc;
+  if ((rc = launch_poles(h, 1, f, f, false))) return rc;
   return launch_gap_post_e(h, f, f, false);
 }
 
@@ -1795,6 +1917,11 @@ int ems_gpu_set_lane(ems_gpu *h, int lane, const float *rows, uint64_t nrows,
 
 int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *s, int n) {
   ALIVE(h);
+  for (auto &o : h->ops)
+    if (o.type == OP_POLES)
+      return fail(h, EMS_E_UNSUPPORTED,
+                  "CPML slabs must be registered before the dispersive media");
+  ALIVE(h);
   if (n > 0 && !s) return EMS_E_INVALID;
   for (int q = 0; q < n; q++) {
     const ems_cpml_slab &in = s[q];
@@ -2034,6 +2161,86 @@ int ems_gpu_lumped_branch(ems_gpu *h, int id, double *j, double *q, double *j_av
   return EMS_OK;
 }
 
+int ems_gpu_add_dispersive(ems_gpu *h, const ems_pole_side *electric,
+                           const ems_pole_side *magnetic, const ems_pole_consts *consts,
+                           int *id) {
+  ALIVE(h);
+  if (!electric || !magnetic || !consts || consts->n < 0 ||
+      (consts->n && (!consts->a || !consts->inv_c || !consts->inv_z || !consts->half_l ||
+                     !consts->kind)))
+    return fail(h, EMS_E_INVALID, "dispersive medium: null description");
+  for (int c = 0; c < consts->n; c++)
+    if (consts->kind[c] != 0 && consts->kind[c] != 1)
+      return fail(h, EMS_E_INVALID, "dispersive medium: pole kind must be 0 or 1");
+  HostOp o;
+  o.type = OP_POLES;
+  auto put = [&](const void *src, size_t bytes, void **dst) -> int {
+    int rc = upload(h, dst, src, bytes);
+    if (!rc) o.mem.push_back(*dst);
+    return rc;
+  };
+  const ems_pole_side *sd[2] = {electric, magnetic};
+  int rc;
+  for (int f = 0; f < 2; f++) {
+    const ems_pole_side *d = sd[f];
+    const int ne = d->n_edges;
+    if (ne < 0 || (ne && (!d->axis || !d->nodes || !d->sum_inv_z || !d->pole_end)) ||
+        (d->n_poles && !d->cidx))
+      return fail(h, EMS_E_INVALID, "dispersive side arrays");
+    std::vector<uint64_t> idx(ne);
+    uint32_t last = 0;
+    for (int e = 0; e < ne; e++) {
+      const int32_t *nd = d->nodes + 3 * e;
+      const int a = d->axis[e];
+      if (a < 0 || a > 2 || !node_ok(h, nd))
+        return fail(h, EMS_E_INVALID, "dispersive edge out of range");
+      if (d->pole_end[e] <= last || d->pole_end[e] > d->n_poles)
+        return fail(h, EMS_E_INVALID, "dispersive pole_end must increase up to n_poles");
+      last = d->pole_end[e];
+      idx[e] = ((uint64_t)nd[0] * h->n[1] + nd[1]) * h->n[2] + nd[2];
+    }
+    if (ne && last != d->n_poles) return fail(h, EMS_E_INVALID, "dispersive pole count");
+    for (uint64_t p = 0; p < d->n_poles; p++)
+      if ((int)d->cidx[p] >= consts->n) return fail(h, EMS_E_INVALID, "dispersive pole constant");
+    DevPoleSide &s = o.side[f];
+    s.n = ne;
+    const std::vector<double> zeros(d->n_poles ? d->n_poles : 1, 0.0);
+    const std::vector<float> zf(ne ? ne : 1, 0.0f);
+    if ((rc = put(d->axis, ne * sizeof(int32_t), (void **)&s.axis))) return rc;
+    if ((rc = put(idx.data(), ne * sizeof(uint64_t), (void **)&s.idx))) return rc;
+    if ((rc = put(d->sum_inv_z, ne * sizeof(double), (void **)&s.sum_inv_z))) return rc;
+    if ((rc = put(d->pole_end, ne * sizeof(uint32_t), (void **)&s.pole_end))) return rc;
+    if ((rc = put(d->cidx, d->n_poles * sizeof(uint32_t), (void **)&s.cidx))) return rc;
+    if ((rc = put(zeros.data(), zeros.size() * sizeof(double), (void **)&s.j))) return rc;
+    if ((rc = put(zeros.data(), zeros.size() * sizeof(double), (void **)&s.q))) return rc;
+    if ((rc = put(zf.data(), zf.size() * sizeof(float), (void **)&s.prev))) return rc;
+    o.side_cidx[f].assign(d->cidx, d->cidx + d->n_poles);
+  }
+  std::vector<DevPoleConst> cs(consts->n ? consts->n : 1);
+  for (int c = 0; c < consts->n; c++)
+    cs[c] = {consts->a[c], consts->inv_c[c], consts->inv_z[c], consts->kind[c]};
+  if ((rc = put(cs.data(), cs.size() * sizeof(DevPoleConst), (void **)&o.d_consts))) return rc;
+  o.c_inv_c.assign(consts->inv_c, consts->inv_c + consts->n);
+  o.c_half_l.assign(consts->half_l, consts->half_l + consts->n);
+  h->ops.push_back(o);
+  h->ops_dirty = true;
+  if (id) *id = (int)h->ops.size() - 1;
+  return EMS_OK;
+}
+
+int ems_gpu_dispersive_state(ems_gpu *h, int id, int field, double *j, double *q) {
+  ALIVE(h);
+  if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_POLES || field < 0 ||
+      field > 1)
+    return fail(h, EMS_E_INVALID, "not a dispersive medium id / field");
+  HostOp &o = h->ops[id];
+  CK(h, hipStreamSynchronize(h->stream));
+  const size_t nb = o.side_cidx[field].size() * sizeof(double);
+  if (nb && j) CK(h, hipMemcpy(j, o.side[field].j, nb, hipMemcpyDeviceToHost));
+  if (nb && q) CK(h, hipMemcpy(q, o.side[field].q, nb, hipMemcpyDeviceToHost));
+  return EMS_OK;
+}
+
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks) {
   ALIVE(h);
   // fused 2: the fused kernel without the next-plane prefetch; 3: the
@@ -2065,6 +2272,27 @@ static int ops_energy(ems_gpu *h, double *out) {
   if (!h->slabs.empty()) w += 0.0;
   bool synced = false;
   for (auto &o : h->ops) {
+    if (o.type == OP_POLES) {
+      // Dispersive::energy (dispersive.rs:2789-2794): electric + magnetic,
+      // each summed from 0.0 in pole order (Side::add_energy :1489-1517).
+      if (!synced) CK(h, hipStreamSynchronize(h->stream));
+      synced = true;
+      double side_w[2] = {0.0, 0.0};
+      for (int sd = 0; sd < 2; sd++) {
+        const std::vector<uint32_t> &ci = o.side_cidx[sd];
+        const size_t np = ci.size();
+        if (!np) continue;
+        std::vector<double> j(np), q(np);
+        CK(h, hipMemcpy(j.data(), o.side[sd].j, np * sizeof(double), hipMemcpyDeviceToHost));
+        CK(h, hipMemcpy(q.data(), o.side[sd].q, np * sizeof(double), hipMemcpyDeviceToHost));
+        double acc = 0.0;
+        for (size_t p = 0; p < np; p++)
+          acc += o.c_half_l[ci[p]] * j[p] * j[p] + 0.5 * o.c_inv_c[ci[p]] * q[p] * q[p];
+        side_w[sd] = acc;
+      }
+      w += side_w[0] + side_w[1];
+      continue;
+    }
     if (o.type != OP_LUMPED || !o.lp.branch) {
       w += 0.0;
       continue;
@@ -2180,6 +2408,7 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       if ((rc = launch_fused(h, h->par))) return rc;
       CK(h, hipEventRecord(ev[1], h->stream));
       h->par ^= 1;
+      if ((rc = launch_poles(h, 1, h->fb[h->par], h->fb[h->par ^ 1], true))) return rc;
       if ((rc = launch_gap_post_e(h, h->fb[h->par], h->fb[h->par ^ 1], true))) return rc;
       CK(h, hipEventRecord(ev[4], h->stream));
       CK(h, hipEventSynchronize(ev[4]));
@@ -2191,11 +2420,13 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       if ((rc = launch_h(h, f))) return rc;
       CK(h, hipEventRecord(ev[1], h->stream));
       if ((rc = launch_cpml(h, f, 0, h->par))) return rc;
+      if ((rc = launch_poles(h, 0, f, f, false))) return rc;
       if ((rc = launch_gap_mid(h, f))) return rc;
       CK(h, hipEventRecord(ev[2], h->stream));
       if ((rc = launch_e(h, f))) return rc;
       CK(h, hipEventRecord(ev[3], h->stream));
       if ((rc = launch_cpml(h, f, 1, h->par))) return rc;
+      if ((rc = launch_poles(h, 1, f, f, false))) return rc;
       if ((rc = launch_gap_post_e(h, f, f, false))) return rc;
       CK(h, hipEventRecord(ev[4], h->stream));
       CK(h, hipEventSynchronize(ev[4]));

@@ -1,0 +1,386 @@
+"""Dispersive (Drude, Lorentz, Debye) media by the auxiliary differential
+equation method (src/engine/dispersive.rs).
+
+Every pole is a series R-L-C branch across its edge (dispersive.rs:25-45),
+integrated with the semi-implicit trapezoidal rule or the explicit
+leapfrog (dispersive.rs:428-461). The host side of `Dispersive::install`
+(dispersive.rs:2063-2319) runs here: edges in (plane, field, axis, j, k)
+order, each pole's constants from its strength and the edge's geometry
+m = X0 * g, edges whose cb (db) is 0 skipped. The device keeps the branch
+state and corrects the pole edges after every E (H) update
+(dispersive.rs:1183-1343, 1392-1470).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from enum import IntEnum
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as nat
+from ._native import EmsError
+from .engine import EPS0, MU0, Axis
+
+
+class Field(IntEnum):
+    """Field (dispersive.rs:463-470)."""
+    Electric = 0
+    Magnetic = 1
+
+
+class Scheme(IntEnum):
+    """Scheme (dispersive.rs:428-461)."""
+    Trapezoidal = 0
+    Leapfrog = 1
+
+
+KIND_TRAPEZOIDAL, KIND_LEAPFROG = 0, 1
+
+
+class Pole:
+    """Pole (dispersive.rs:218-426). Frequencies in rad/s, times in s."""
+
+    def __init__(self, kind: str, a: float, b: float, c: float = 0.0):
+        self.kind = kind
+        self.p = (float(a), float(b), float(c))
+
+    @staticmethod
+    def lorentz(omega_p: float, omega_0: float, gamma: float) -> "Pole":
+        return Pole("lorentz", omega_p, omega_0, gamma)
+
+    @staticmethod
+    def drude(omega_p: float, gamma: float) -> "Pole":
+        return Pole("drude", omega_p, gamma)
+
+    @staticmethod
+    def debye(delta_eps: float, tau: float) -> "Pole":
+        return Pole("debye", delta_eps, tau)
+
+    def __eq__(self, o):
+        return isinstance(o, Pole) and (self.kind, self.p) == (o.kind, o.p)
+
+    def __hash__(self):
+        return hash((self.kind, self.p))
+
+    def __repr__(self):
+        return f"Pole.{self.kind}{self.p if self.kind == 'lorentz' else self.p[:2]}"
+
+    def susceptibility(self, omega: float) -> complex:
+        """chi(omega) (e^{jwt} convention, losses Im chi < 0)."""
+        if self.kind == "lorentz":
+            wp, w0, g = self.p
+            return wp * wp / complex(w0 * w0 - omega * omega, omega * g)
+        if self.kind == "drude":
+            wp, g = self.p[:2]
+            return -(wp * wp) / complex(omega * omega, -(omega * g))
+        de, tau = self.p[:2]
+        return de / complex(1.0, omega * tau)
+
+    def discrete_susceptibility(self, f: float, dt: float) -> complex:
+        """chi at the bilinear frequency (2/dt) tan(pi f dt) (dispersive.rs:288-290)."""
+        return self.susceptibility(2.0 / dt * math.tan(math.pi * f * dt))
+
+    def leapfrog_susceptibility(self, f: float, dt: float) -> complex:
+        """dispersive.rs:409-425."""
+        w = 2.0 * math.pi * f
+        c = math.cos(0.5 * w * dt)
+        om = 2.0 / dt * math.sin(0.5 * w * dt)
+        if self.kind == "lorentz":
+            wp, w0, g = self.p
+            return wp * wp / complex(w0 * w0 - om * om, om * g * c)
+        if self.kind == "drude":
+            wp, g = self.p[:2]
+            return wp * wp / complex(-(om * om), om * g * c)
+        return self.discrete_susceptibility(f, dt)
+
+    def scaled(self, s: float) -> "Pole":
+        """Susceptibility times s (dispersive.rs:295-316)."""
+        if self.kind == "lorentz":
+            return Pole("lorentz", self.p[0] * math.sqrt(s), self.p[1], self.p[2])
+        if self.kind == "drude":
+            return Pole("drude", self.p[0] * math.sqrt(s), self.p[1])
+        return Pole("debye", self.p[0] * s, self.p[1])
+
+    def is_null(self) -> bool:
+        return self.p[0] == 0.0
+
+    def validate(self) -> None:
+        n = 3 if self.kind == "lorentz" else 2
+        for v in self.p[:n]:
+            if not (math.isfinite(v) and v >= 0.0):
+                raise EmsError(nat.EMS_E_INVALID,
+                               f"dispersive pole {self!r}: {v} must be finite and >= 0")
+
+    def _lrc(self, m: float):
+        if self.kind == "lorentz":
+            wp, w0, g = self.p
+            l = 1.0 / (m * wp * wp)
+            return l, g * l, w0 * w0 * l
+        if self.kind == "drude":
+            wp, g = self.p[:2]
+            l = 1.0 / (m * wp * wp)
+            return l, g * l, 0.0
+        de, tau = self.p[:2]
+        c = m * de
+        return 0.0, tau / c, 1.0 / c
+
+    def branch(self, m: float, dt: float):
+        """(2L/dt, 1/C_p, z) (dispersive.rs:348-371)."""
+        l, r, inv_c = self._lrc(m)
+        a = 2.0 * l / dt
+        return a, inv_c, a + r + 0.5 * dt * inv_c
+
+    def leapfrog(self, m: float, dt: float):
+        """(k_a, k_b, 1/C_p, L/2) or None for Debye (dispersive.rs:375-393)."""
+        if self.kind == "debye":
+            return None
+        l, r, inv_c = self._lrc(m)
+        d = l / dt + 0.5 * r
+        return (l / dt - 0.5 * r) / d, 1.0 / d, inv_c, 0.5 * l
+
+
+class Dispersive:
+    """Dispersive (dispersive.rs:1790-2700): poles on E edges (and H dual
+    edges); install with ``engine.add_operator``."""
+
+    def __init__(self):
+        self.entries: List[Tuple[int, int, Tuple[int, int, int], Pole]] = []
+        self._scheme = Scheme.Trapezoidal
+        self._installed = None
+        self._skipped = 0
+
+    @staticmethod
+    def new() -> "Dispersive":
+        return Dispersive()
+
+    def with_scheme(self, s: Scheme) -> "Dispersive":
+        self._scheme = Scheme(s)
+        return self
+
+    def scheme(self) -> Scheme:
+        return self._scheme
+
+    def push(self, field: Field, axis: int, node, pole: Pole) -> None:
+        """dispersive.rs:1870-1880: null poles are ignored."""
+        if not pole.is_null():
+            self.entries.append((int(field), int(axis), tuple(int(v) for v in node), pole))
+
+    def fill_cells(self, grid, cells, field: Field, poles: Sequence[Pole]) -> None:
+        """dispersive.rs:1896-1986: every edge touching the cell box gets the
+        poles scaled by the share of its averaging volume inside."""
+        for p in poles:
+            p.validate()
+        nc = grid.cells()
+        cells = [(int(r[0]), int(r[1])) for r in cells]
+        if any(cells[a][1] > nc[a] or cells[a][0] > cells[a][1] for a in range(3)):
+            raise EmsError(nat.EMS_E_INVALID,
+                           f"dispersive cell box {cells} does not fit the grid ({nc} cells)")
+        if any(r[0] == r[1] for r in cells):
+            return
+        n = grid.nodes()
+        inside = lambda a, c: cells[a][0] <= c < cells[a][1]  # noqa: E731
+        for axis in Axis:
+            ab, ac = axis.others()
+            a, b, c = int(axis), int(ab), int(ac)
+
+            def rng(t, along):
+                return range(cells[t][0], cells[t][1]) if along else \
+                    range(cells[t][0], min(cells[t][1] + 1, n[t]))
+            if field == Field.Electric:
+                ra, rb, rc = rng(a, True), rng(b, False), rng(c, False)
+            else:
+                ra, rb, rc = rng(a, False), rng(b, True), rng(c, True)
+            for pa in ra:
+                for pb in rb:
+                    for pc in rc:
+                        node = [0, 0, 0]
+                        node[a], node[b], node[c] = pa, pb, pc
+                        w_in = w = 0.0
+                        if field == Field.Electric:
+                            for cb in (pb - 1, pb):
+                                if cb < 0 or cb >= n[b] - 1:
+                                    continue
+                                for cc in (pc - 1, pc):
+                                    if cc < 0 or cc >= n[c] - 1:
+                                        continue
+                                    area = grid.delta(b, cb) * grid.delta(c, cc)
+                                    w += area
+                                    if inside(b, cb) and inside(c, cc):
+                                        w_in += area
+                        else:
+                            for ca in (pa - 1, pa):
+                                if ca < 0 or ca >= n[a] - 1:
+                                    continue
+                                ln = grid.delta(a, ca)
+                                w += ln
+                                if inside(a, ca):
+                                    w_in += ln
+                        share = w_in / w if w > 0.0 else 0.0
+                        if share > 0.0:
+                            for p in poles:
+                                self.push(field, axis, node, p.scaled(share))
+
+    def __len__(self) -> int:
+        return len(self.entries)
+
+    def _plan(self, grid, dt: float, lanes: np.ndarray) -> dict:
+        """Dispersive::install (dispersive.rs:2063-2319) and the side layout
+        of slab_hooks (:2609-2770)."""
+        if self._installed is not None:
+            raise EmsError(nat.EMS_E_INVALID, "dispersive medium already installed")
+        n = grid.nodes()
+        for _, _, _, p in self.entries:
+            p.validate()
+        for field, axis, node, _ in self.entries:
+            a = axis
+            b, c = (int(x) for x in Axis(axis).others())
+            ok = all(node[t] < n[t] for t in range(3)) and (
+                node[a] + 1 < n[a] if field == Field.Electric else
+                (node[b] + 1 < n[b] and node[c] + 1 < n[c]))
+            if not ok:
+                raise EmsError(nat.EMS_E_INVALID,
+                               f"dispersive {Field(field).name} edge {Axis(axis).name}:{node} "
+                               f"is not in the grid ({n} nodes)")
+        key = lambda e: (e[2][0], e[0], e[1], e[2][1], e[2][2])  # noqa: E731
+        order = sorted(range(len(self.entries)), key=lambda q: key(self.entries[q]))
+        flat = [lanes[l].reshape(-1) for l in range(12)]
+        sides: Dict[int, dict] = {f: {"axis": [], "nodes": [], "sum_inv_z": [], "pole_end": [],
+                                      "cidx": []} for f in (0, 1)}
+        consts: List[tuple] = []
+        cmap: Dict[tuple, int] = {}
+        skipped = 0
+        last = None
+        for q in order:
+            field, axis, node, pole = self.entries[q]
+            k = key(self.entries[q])
+            ab, ac = (int(x) for x in Axis(axis).others())
+            idx = grid.index(node)
+            if field == Field.Electric:
+                coef = float(flat[3 + axis][idx])
+                geo = grid.dual_delta(ab, node[ab]) * grid.dual_delta(ac, node[ac]) / \
+                    grid.delta(axis, node[axis])
+                m = EPS0 * geo
+            else:
+                coef = float(flat[9 + axis][idx])
+                geo = grid.delta(ab, node[ab]) * grid.delta(ac, node[ac]) / \
+                    grid.dual_delta(axis, node[axis])
+                m = MU0 * geo
+            new_edge = last != k
+            last = k
+            if coef == 0.0:
+                if new_edge:
+                    skipped += 1
+                continue
+            s = sides[field]
+            if new_edge:
+                s["axis"].append(axis)
+                s["nodes"].append(node)
+                s["sum_inv_z"].append(0.0)
+                s["pole_end"].append(len(s["cidx"]))
+            leap = pole.leapfrog(m, dt) if self._scheme == Scheme.Leapfrog else None
+            if leap is not None:
+                ka, kb, inv_c, half_l = leap
+                cst = (ka, inv_c, kb, half_l, KIND_LEAPFROG)
+            else:
+                a_, inv_c, z = pole.branch(m, dt)
+                cst = (a_, inv_c, 1.0 / z, 0.25 * a_ * dt, KIND_TRAPEZOIDAL)
+                s["sum_inv_z"][-1] += cst[2]
+            ck = tuple(np.float64(v).tobytes() for v in cst[:4]) + (cst[4],)
+            if ck not in cmap:
+                cmap[ck] = len(consts)
+                consts.append(cst)
+            s["cidx"].append(cmap[ck])
+            s["pole_end"][-1] = len(s["cidx"])
+        self._skipped = skipped
+        return {"sides": sides, "consts": consts, "skipped": skipped}
+
+    # -- results
+    def _bind(self, engine, oid: int, plan: dict) -> None:
+        self._installed = (engine, oid, plan)
+
+    def edges(self) -> int:
+        if self._installed is None:
+            return 0
+        s = self._installed[2]["sides"]
+        return len(s[0]["axis"]) + len(s[1]["axis"])
+
+    def skipped(self) -> int:
+        return self._skipped
+
+    def state(self, field: Field):
+        """(j, q) per pole of one side, in pole order."""
+        eng, oid, _ = self._installed
+        return eng._dispersive_state(oid, int(field))
+
+    def _sum_energy(self, field: Field) -> float:
+        """Side::add_energy (dispersive.rs:1489-1517): from 0.0, pole order."""
+        if self._installed is None:
+            return 0.0
+        _, _, plan = self._installed
+        j, q = self.state(field)
+        consts = plan["consts"]
+        w = 0.0
+        for p, ci in enumerate(plan["sides"][int(field)]["cidx"]):
+            a, inv_c, inv_z, half_l, kind = consts[ci]
+            w += half_l * float(j[p]) * float(j[p]) + 0.5 * inv_c * float(q[p]) * float(q[p])
+        return w
+
+    def electric_energy(self) -> float:
+        return self._sum_energy(Field.Electric)
+
+    def magnetic_energy(self) -> float:
+        return self._sum_energy(Field.Magnetic)
+
+    def energy(self) -> float:
+        """Operator::energy (dispersive.rs:2789-2794)."""
+        return self.electric_energy() + self.magnetic_energy()
+
+
+class PoleSide(C.Structure):
+    """ems_pole_side (include/ems_gpu.h)."""
+    _fields_ = [("n_edges", C.c_int32), ("axis", C.POINTER(C.c_int32)),
+                ("nodes", C.POINTER(C.c_int32)), ("sum_inv_z", C.POINTER(C.c_double)),
+                ("pole_end", C.POINTER(C.c_uint32)), ("n_poles", C.c_uint64),
+                ("cidx", C.POINTER(C.c_uint32))]
+
+
+class PoleConsts(C.Structure):
+    """ems_pole_consts (include/ems_gpu.h)."""
+    _fields_ = [("n", C.c_int32), ("a", C.POINTER(C.c_double)), ("inv_c", C.POINTER(C.c_double)),
+                ("inv_z", C.POINTER(C.c_double)), ("half_l", C.POINTER(C.c_double)),
+                ("kind", C.POINTER(C.c_int32))]
+
+
+def make_desc(plan: dict):
+    """C descriptions of the two sides and the constants, and the arrays
+    they point into."""
+    keep = []
+
+    def arr(x, t):
+        a = np.ascontiguousarray(np.asarray(x, dtype=t))
+        if a.size == 0:
+            a = np.zeros(1, dtype=t)
+        keep.append(a)
+        return a
+
+    def p(a, ct):
+        return a.ctypes.data_as(C.POINTER(ct))
+
+    sides = []
+    for f in (0, 1):
+        s = plan["sides"][f]
+        ax = arr(s["axis"], np.int32)
+        nd = arr([v for n in s["nodes"] for v in n], np.int32)
+        sz = arr(s["sum_inv_z"], np.float64)
+        pe = arr(s["pole_end"], np.uint32)
+        ci = arr(s["cidx"], np.uint32)
+        sides.append(PoleSide(len(s["axis"]), p(ax, C.c_int32), p(nd, C.c_int32),
+                              p(sz, C.c_double), p(pe, C.c_uint32), len(s["cidx"]),
+                              p(ci, C.c_uint32)))
+    cs = plan["consts"]
+    cols = [arr([c[q] for c in cs], np.float64) for q in range(4)]
+    kind = arr([c[4] for c in cs], np.int32)
+    consts = PoleConsts(len(cs), *(p(c, C.c_double) for c in cols), p(kind, C.c_int32))
+    return sides[0], sides[1], consts, keep

@@ -119,6 +119,14 @@ class Grid:
         l = self.lines[axis]
         return float(l[i + 1] - l[i])
 
+    def dual_delta(self, axis: int, i: int) -> float:
+        """Grid::dual_delta (grid.rs:580-597): mean of the neighbouring
+        primary cells, a missing one (outside the grid) counting 0."""
+        n = self.lines[axis].size
+        lo = self.delta(axis, i - 1) if i > 0 else 0.0
+        hi = self.delta(axis, i) if i + 1 < n else 0.0
+        return 0.5 * (lo + hi)
+
     def _args(self):
         x, y, z = self.lines
         return (nat.ptr(x, C.c_double), x.size, nat.ptr(y, C.c_double), y.size,
@@ -660,6 +668,7 @@ class Engine:
         self.ops: list = []
         self._tables: list = []
         self._lumped_n: dict = {}
+        self._poles_n: dict = {}
         for l in range(12):
             self._upload_lane(l, lanes[l], pool)
 
@@ -731,6 +740,20 @@ class Engine:
             self.setup.append(("probe", int(op.quantity), axis, nodes, w, op._interval))
             op._engine = self
             op._id = oid.value
+        elif type(op).__name__ == "Dispersive":
+            # Dispersive::add_to (dispersive.rs:2479-2520): host install,
+            # then the device medium.
+            from . import dispersive as DP
+            if self._dense_lanes is None:
+                raise EmsError(nat.EMS_E_INVALID, "dispersive media need the host lanes")
+            plan = op._plan(self.grid, self.dt, self._dense_lanes)
+            se, sh, cs, keep = DP.make_desc(plan)
+            oid = C.c_int()
+            self._check(nat.ems_gpu_add_dispersive(h, C.byref(se), C.byref(sh), C.byref(cs),
+                                                   C.byref(oid)))
+            self._poles_n[oid.value] = (len(plan["sides"][0]["cidx"]), len(plan["sides"][1]["cidx"]))
+            self.setup.append(("dispersive", plan))
+            op._bind(self, oid.value, plan)
         elif hasattr(op, "_plan"):
             # LumpedElement / LumpedPort (lumped.rs:880-1012): fold into the
             # lanes, re-upload them, register the device operator.
@@ -790,6 +813,13 @@ class Engine:
         self._check(nat.ems_gpu_lumped_values(self._h, oid, which, nat.ptr(out, C.c_double),
                                               out.size, C.byref(cnt), C.byref(first)))
         return out, first.value
+
+    def _dispersive_state(self, oid: int, field: int):
+        n = self._poles_n[oid][field]
+        j, q = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        self._check(nat.ems_gpu_dispersive_state(self._h, oid, field, nat.ptr(j, C.c_double),
+                                                 nat.ptr(q, C.c_double)))
+        return j[:n], q[:n]
 
     def _lumped_branch(self, oid: int):
         n = self._lumped_n[oid]

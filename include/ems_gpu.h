@@ -147,6 +147,43 @@ int ems_gpu_lumped_values(ems_gpu *h, int id, int which, double *out, uint64_t c
  * branch_current, lumped.rs:1033-1087). */
 int ems_gpu_lumped_branch(ems_gpu *h, int id, double *j, double *q, double *j_avg);
 
+/* A dispersive medium (Dispersive, src/engine/dispersive.rs:1790-2794),
+ * after the host's `install` (:2063-2319): the distinct pole constants and
+ * the two sides (electric poles on E edges, magnetic poles on dual edges).
+ * A side lists its edges in (x plane, axis, j, k) order: axis, start node,
+ * sum of 1/z over its trapezoidal poles, and the end of its poles in the
+ * pole arrays (CSR); each pole an index into the constants:
+ *   a      2L/dt (trapezoidal) or k_a (leapfrog),
+ *   inv_c  1/C_p,
+ *   inv_z  1/z (trapezoidal) or k_b (leapfrog),
+ *   half_l L/2 (energy),
+ *   kind   0 trapezoidal, 1 leapfrog (Branch, dispersive.rs:1641-1663).
+ * The device keeps j, q (f64) per pole and the saved field per edge, and
+ * corrects the E side after the E update and the H side after the H
+ * update (Side::correct / step_edge, :1183-1343, :1392-1470), reading
+ * cb / db from the lanes every step. A medium with magnetic poles runs
+ * the unfused schedule. Media must be registered after every CPML slab
+ * (the order the reference recommends, dispersive.rs module docs). */
+typedef struct {
+    int32_t n;
+    const double *a, *inv_c, *inv_z, *half_l;
+    const int32_t *kind;
+} ems_pole_consts;
+typedef struct {
+    int32_t n_edges;
+    const int32_t *axis, *nodes;
+    const double *sum_inv_z;
+    const uint32_t *pole_end;
+    uint64_t n_poles;
+    const uint32_t *cidx;
+} ems_pole_side;
+int ems_gpu_add_dispersive(ems_gpu *h, const ems_pole_side *electric,
+                           const ems_pole_side *magnetic, const ems_pole_consts *consts,
+                           int *id);
+/* Branch state of one side (field 0 electric, 1 magnetic) in pole order:
+ * j^n, q^n per pole (Dispersive::electric_energy / magnetic_energy). */
+int ems_gpu_dispersive_state(ems_gpu *h, int id, int field, double *j, double *q);
+
 /* Engine::run (src/engine/mod.rs:825-828). */
 int ems_gpu_run(ems_gpu *h, uint64_t n);
 /* Engine::step_with_energy (src/engine/mod.rs:808-823): one step, *w = the

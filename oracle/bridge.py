@@ -48,6 +48,8 @@ orc_add_lumped = _sig("orc_add_lumped", _i, _p, _p)
 orc_lumped_table = _sig("orc_lumped_table", _i, _p, _i, _i, _dp, _u64)
 orc_lumped_values = _sig("orc_lumped_values", _u64, _p, _i, _i, _dp, _u64)
 orc_lumped_branch = _sig("orc_lumped_branch", _i, _p, _i, _dp, _dp, _dp)
+orc_add_dispersive = _sig("orc_add_dispersive", _i, _p, _p, _p, _p)
+orc_dispersive_state = _sig("orc_dispersive_state", _i, _p, _i, _i, _dp, _dp)
 orc_run = _sig("orc_run", None, _p, _u64)
 orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
 orc_energy = _sig("orc_energy", _d, _p)
@@ -91,6 +93,7 @@ class OracleEngine:
         self.ops = []
         self._tables = []
         self._lumped = {}
+        self._poles_n = {}
 
     def _set_lane(self, l, lane):
         n = self.grid.nodes()
@@ -116,6 +119,12 @@ class OracleEngine:
         out = np.empty(n, dtype=np.float64)
         orc_lumped_values(self._h, oid, which, P(out, C.c_double), n)
         return out, self._lumped[oid][1]
+
+    def _dispersive_state(self, oid, field):
+        n = self._poles_n[oid][field]
+        j, q = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        assert orc_dispersive_state(self._h, oid, field, P(j, C.c_double), P(q, C.c_double)) == 0
+        return j[:n], q[:n]
 
     def _lumped_branch(self, oid):
         n = self._lumped[oid][0]
@@ -160,6 +169,14 @@ class OracleEngine:
             op._oid = orc_add_probe(h, int(op.quantity), len(op.entries), P(axis, C.c_int),
                                     P(nodes, C.c_int), P(w, C.c_double), op._interval)
             assert op._oid >= 0
+        elif type(op).__name__ == "Dispersive":
+            from ems_amd import dispersive as DP
+            plan = op._plan(self.grid, self.dt, self._dense_lanes)
+            se, sh, cs, keep = DP.make_desc(plan)
+            oid = orc_add_dispersive(h, C.addressof(se), C.addressof(sh), C.addressof(cs))
+            assert oid >= 0
+            self._poles_n[oid] = (len(plan["sides"][0]["cidx"]), len(plan["sides"][1]["cidx"]))
+            op._bind(self, oid, plan)
         elif hasattr(op, "_plan"):
             from ems_amd import lumped as LM
             plan = op._plan(self.grid, self.dt, self._dense_lanes)

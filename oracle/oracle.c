@@ -69,7 +69,27 @@ typedef struct {
     float *b, *a, *k, *psi;
 } cpml_t;
 
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4 };
+
+/* One side of a dispersive medium (Side, dispersive.rs:977-1040): edges in
+ * (plane, axis, j, k) order, poles in CSR layout, f64 branch state. */
+typedef struct {
+    int n;
+    int *axis, *plane;
+    size_t *idx;
+    double *siz;
+    uint32_t *pole_end, *cidx;
+    uint64_t np;
+    double *j, *q;
+    float *prev;
+} poleside_t;
+
+typedef struct {
+    poleside_t side[2]; /* 0 electric, 1 magnetic */
+    int nc;
+    double *a, *inv_c, *inv_z, *half_l;
+    int *kind; /* 0 trapezoidal, 1 leapfrog */
+} poles_t;
 
 /* An installed lumped element (lumped.rs:367-406, Installed + Branch +
  * CurrentRecord), optionally a LumpedPort (engine/port.rs:94-104). */
@@ -107,6 +127,8 @@ typedef struct {
     uint64_t nvals, cap;
     /* lumped element */
     lumped_t *lp;
+    /* dispersive medium */
+    poles_t *pm;
 } op_t;
 
 struct orc_engine {
@@ -308,6 +330,103 @@ static void cpml_apply(orc_engine *e, cpml_t *s, int x0, int x1) {
 /* ------------------------------------------------------------------ */
 /* slab-parallel phases (pool.rs:314-403)                               */
 /* ------------------------------------------------------------------ */
+/* ------------------------------------------------------------------ */
+/* dispersive media as slab hooks (dispersive.rs:1700-1750)             */
+/* ------------------------------------------------------------------ */
+
+/* Side::edges_in (dispersive.rs:1166-1171): edges of the planes [p0, p1). */
+static void edges_in(const poleside_t *s, int p0, int p1, int *lo, int *hi) {
+    int a = 0, b = s->n;
+    while (a < b) {
+        int m = (a + b) / 2;
+        if (s->plane[m] < p0) a = m + 1; else b = m;
+    }
+    *lo = a;
+    b = s->n;
+    while (a < b) {
+        int m = (a + b) / 2;
+        if (s->plane[m] < p1) a = m + 1; else b = m;
+    }
+    *hi = a;
+}
+
+/* Side::correct through step_edge (dispersive.rs:1183-1343, 1392-1470)
+ * for trapezoidal and leapfrog poles; lane 3 + axis (cb) for the electric
+ * side on u, 9 + axis (db) for the magnetic side on i. */
+static void side_correct(orc_engine *e, const poles_t *m, int f, int p0, int p1) {
+    const poleside_t *s = &m->side[f];
+    int lo, hi;
+    edges_in(s, p0, p1, &lo, &hi);
+    const int ny = e->n[1], nz = e->n[2];
+    for (int ed = lo; ed < hi; ed++) {
+        const int ax = s->axis[ed];
+        const size_t idx = s->idx[ed];
+        float *fld = f == 0 ? e->u[ax] : e->i[ax];
+        const int gi = s->plane[ed];
+        const size_t off = idx - (size_t)gi * ny * nz;
+        const double cb = (double)row(e, (f == 0 ? 3 : 9) + ax, gi, (int)(off / nz))[off % nz];
+        const uint32_t q0 = ed ? s->pole_end[ed - 1] : 0, q1 = s->pole_end[ed];
+        const double prev = (double)s->prev[ed];
+        const double half_prev = 0.5 * prev;
+        const double u_star = (double)fld[idx];
+        double sum = 0.0;
+        for (uint32_t q = q0; q < q1; q++) {
+            const int c = s->cidx[q];
+            if (m->kind[c] == 1) /* leapfrog: j^{n+1/2} from u^n */
+                sum += m->a[c] * s->j[q] + m->inv_z[c] * (prev - s->q[q] * m->inv_c[c]);
+            else
+                sum += (m->a[c] * s->j[q] - s->q[q] * m->inv_c[c] + half_prev) * m->inv_z[c];
+        }
+        const float u_new = (float)((u_star - cb * sum) / (1.0 + 0.5 * cb * s->siz[ed]));
+        const double half_new = 0.5 * (double)u_new;
+        for (uint32_t q = q0; q < q1; q++) {
+            const int c = s->cidx[q];
+            const double jo = s->j[q], qo = s->q[q];
+            if (m->kind[c] == 1) {
+                const double jn = m->a[c] * jo + m->inv_z[c] * (prev - qo * m->inv_c[c]);
+                s->j[q] = jn;
+                s->q[q] = qo + e->dt * jn;
+            } else {
+                const double k = m->a[c] * jo - qo * m->inv_c[c] + half_prev;
+                const double jb = (k + half_new) * m->inv_z[c];
+                s->j[q] = 2.0 * jb - jo;
+                s->q[q] = qo + e->dt * jb;
+            }
+        }
+        fld[idx] = u_new;
+    }
+}
+
+/* Side::capture (dispersive.rs:1173-1179). */
+static void side_capture(orc_engine *e, const poles_t *m, int f, int p0, int p1) {
+    const poleside_t *s = &m->side[f];
+    int lo, hi;
+    edges_in(s, p0, p1, &lo, &hi);
+    for (int ed = lo; ed < hi; ed++)
+        s->prev[ed] = (f == 0 ? e->u : e->i)[s->axis[ed]][s->idx[ed]];
+}
+
+/* Parts::post_h_planes / post_e_planes (dispersive.rs:1700-1750). */
+static void poles_post_h(orc_engine *e, poles_t *m, int p0, int p1) {
+    side_correct(e, m, 1, p0, p1);
+    side_capture(e, m, 0, p0, p1);
+}
+static void poles_post_e(orc_engine *e, poles_t *m, int p0, int p1) {
+    side_correct(e, m, 0, p0, p1);
+    side_capture(e, m, 1, p0, p1);
+}
+
+/* Side::add_energy (dispersive.rs:1489-1517) over one side, from 0.0. */
+static double side_energy(const poles_t *m, int f) {
+    const poleside_t *s = &m->side[f];
+    double w = 0.0;
+    for (uint64_t q = 0; q < s->np; q++) {
+        const int c = s->cidx[q];
+        w += m->half_l[c] * s->j[q] * s->j[q] + 0.5 * m->inv_c[c] * s->q[q] * s->q[q];
+    }
+    return w;
+}
+
 static void phase(orc_engine *e, int cmd, int s) {
     int p0 = e->slab_start[s], p1 = e->slab_start[s + 1];
     if (cmd == 1) {
@@ -317,10 +436,14 @@ static void phase(orc_engine *e, int cmd, int s) {
         }
         for (int c = 0; c < e->ncpml; c++)
             if (!e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], p0, p1);
+        for (int q = 0; q < e->nops; q++)
+            if (e->ops[q].type == OP_POLES) poles_post_h(e, e->ops[q].pm, p0, p1);
     } else if (cmd == 2) {
         for (int gi = p0; gi < p1; gi++) e_plane(e, gi);
         for (int c = 0; c < e->ncpml; c++)
             if (e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], p0, p1);
+        for (int q = 0; q < e->nops; q++)
+            if (e->ops[q].type == OP_POLES) poles_post_e(e, e->ops[q].pm, p0, p1);
     } else if (cmd == 3) {
         for (int gi = p0; gi < p1; gi++) e->plane_energy[gi] = h_plane(e, gi, 1, 0);
     }
@@ -511,6 +634,7 @@ static int needs(const op_t *o, uint64_t n) {
         return op_amp(o, n) != 0.0f;
     }
     if (o->type == OP_LUMPED) return 1; /* pre_e / post_e every step */
+    if (o->type == OP_POLES) return 0;  /* slab hooks only */
     return n % o->interval == 0;
 }
 
@@ -631,6 +755,15 @@ void orc_destroy(orc_engine *e) {
     for (int q = 0; q < e->nops; q++) {
         op_t *o = &e->ops[q];
         free(o->axis), free(o->idx), free(o->wf), free(o->amp), free(o->wd), free(o->vals);
+        poles_t *m = o->pm;
+        if (m) {
+            for (int f = 0; f < 2; f++) {
+                poleside_t *ps = &m->side[f];
+                free(ps->axis), free(ps->plane), free(ps->idx), free(ps->siz);
+                free(ps->pole_end), free(ps->cidx), free(ps->j), free(ps->q), free(ps->prev);
+            }
+            free(m->a), free(m->inv_c), free(m->inv_z), free(m->half_l), free(m->kind), free(m);
+        }
         lumped_t *L = o->lp;
         if (L) {
             for (int t = 0; t < L->nterm + L->ndrive; t++) free(L->tv[t]);
@@ -842,6 +975,57 @@ int orc_lumped_branch(orc_engine *e, int id, double *j, double *q, double *javg)
     return 0;
 }
 
+int orc_add_dispersive(orc_engine *e, const orc_pole_side *el, const orc_pole_side *mg,
+                       const orc_pole_consts *cs) {
+    const orc_pole_side *sd[2] = {el, mg};
+    for (int f = 0; f < 2; f++)
+        for (int q = 0; q < sd[f]->n_edges; q++)
+            if (!node_ok(e, sd[f]->nodes + 3 * q)) return -1;
+    int id = add_op(e);
+    op_t *o = &e->ops[id];
+    o->type = OP_POLES;
+    poles_t *m = (poles_t *)calloc(1, sizeof(poles_t));
+    o->pm = m;
+    m->nc = cs->n;
+    m->a = dup_d(cs->a, cs->n), m->inv_c = dup_d(cs->inv_c, cs->n);
+    m->inv_z = dup_d(cs->inv_z, cs->n), m->half_l = dup_d(cs->half_l, cs->n);
+    m->kind = (int *)calloc(cs->n ? cs->n : 1, sizeof(int));
+    memcpy(m->kind, cs->kind, cs->n * sizeof(int));
+    for (int f = 0; f < 2; f++) {
+        const orc_pole_side *d = sd[f];
+        poleside_t *s = &m->side[f];
+        const int ne = d->n_edges;
+        s->n = ne;
+        s->np = d->n_poles;
+        s->axis = (int *)calloc(ne ? ne : 1, sizeof(int));
+        s->plane = (int *)calloc(ne ? ne : 1, sizeof(int));
+        s->idx = (size_t *)calloc(ne ? ne : 1, sizeof(size_t));
+        s->siz = dup_d(d->sum_inv_z, ne);
+        s->pole_end = (uint32_t *)calloc(ne ? ne : 1, sizeof(uint32_t));
+        s->cidx = (uint32_t *)calloc(d->n_poles ? d->n_poles : 1, sizeof(uint32_t));
+        s->j = dup_d(NULL, d->n_poles), s->q = dup_d(NULL, d->n_poles);
+        s->prev = (float *)calloc(ne ? ne : 1, sizeof(float));
+        for (int q = 0; q < ne; q++) {
+            const int *nd = d->nodes + 3 * q;
+            s->axis[q] = d->axis[q];
+            s->plane[q] = nd[0];
+            s->idx[q] = ((size_t)nd[0] * e->n[1] + nd[1]) * e->n[2] + nd[2];
+            s->pole_end[q] = d->pole_end[q];
+        }
+        memcpy(s->cidx, d->cidx, d->n_poles * sizeof(uint32_t));
+    }
+    return id;
+}
+
+int orc_dispersive_state(orc_engine *e, int id, int field, double *j, double *q) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_POLES || field < 0 || field > 1)
+        return -1;
+    const poleside_t *s = &e->ops[id].pm->side[field];
+    memcpy(j, s->j, s->np * sizeof(double));
+    memcpy(q, s->q, s->np * sizeof(double));
+    return 0;
+}
+
 void orc_run(orc_engine *e, uint64_t n) {
     for (uint64_t t = 0; t < n; t++) one_step(e, 0);
 }
@@ -852,8 +1036,13 @@ void orc_run(orc_engine *e, uint64_t n) {
 static double ops_energy(const orc_engine *e) {
     double w = -0.0;
     if (e->ncpml) w += 0.0;
-    for (int q = 0; q < e->nops; q++)
-        w += e->ops[q].type == OP_LUMPED ? lumped_energy(e, &e->ops[q]) : 0.0;
+    for (int q = 0; q < e->nops; q++) {
+        const op_t *o = &e->ops[q];
+        if (o->type == OP_POLES) /* Dispersive::energy (dispersive.rs:2789-2794) */
+            w += side_energy(o->pm, 0) + side_energy(o->pm, 1);
+        else
+            w += o->type == OP_LUMPED ? lumped_energy(e, o) : 0.0;
+    }
     return w;
 }
 

@@ -94,6 +94,27 @@ int orc_lumped_table(orc_engine *e, int id, int term, const double *values, uint
 uint64_t orc_lumped_values(orc_engine *e, int id, int which, double *out, uint64_t cap);
 int orc_lumped_branch(orc_engine *e, int id, double *j, double *q, double *javg);
 
+/* Dispersive medium (dispersive.rs) after the host's install: pole
+ * constants and the electric / magnetic sides, laid out as ems_pole_consts
+ * and ems_pole_side of include/ems_gpu.h. The corrections run as slab
+ * hooks after the CPML ones. Returns the operator id. */
+typedef struct {
+    int n;
+    const double *a, *inv_c, *inv_z, *half_l;
+    const int *kind;
+} orc_pole_consts;
+typedef struct {
+    int n_edges;
+    const int *axis, *nodes;
+    const double *sum_inv_z;
+    const uint32_t *pole_end;
+    uint64_t n_poles;
+    const uint32_t *cidx;
+} orc_pole_side;
+int orc_add_dispersive(orc_engine *e, const orc_pole_side *electric,
+                       const orc_pole_side *magnetic, const orc_pole_consts *consts);
+int orc_dispersive_state(orc_engine *e, int id, int field, double *j, double *q);
+
 void orc_run(orc_engine *e, uint64_t n);
 double orc_step_with_energy(orc_engine *e);
 double orc_energy(orc_engine *e);

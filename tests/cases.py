@@ -251,3 +251,79 @@ def lc_cavity(kind, threads=1):
         u[p] = 0.25
     e.set_component(0, E.Axis.Z, u)
     return e, hp, hs
+
+
+# ------------------------------------------------------------ dispersive --
+from ems_amd.dispersive import Dispersive, Field, Pole, Scheme  # noqa: E402
+
+TAU2PI = 2.0 * math.pi
+
+
+def disp_cavity(kind, threads=1, lossless=True, magnetic=False, scheme=None):
+    """cavity (tests/validation_dispersive.rs:377-425): 10x8x6 mm PEC box,
+    an eps_inf = 1.5 block with Lorentz + Drude + Debye poles (and a
+    magnetic Lorentz pole), four edges charged, no source."""
+    g = E.Grid.uniform([10, 8, 6], [1e-3] * 3)
+    m = E.CellMaterials.vacuum(g)
+    blk = [(2, 8), (1, 6), (1, 5)]
+    m.fill_box(blk, E.Material.dielectric(1.5))
+    e = make(kind, g, m, threads=threads)
+    gam = 0.0 if lossless else 3e9
+    w = TAU2PI * 20e9
+    d = Dispersive.new()
+    if scheme is not None:
+        d.with_scheme(scheme)
+    d.fill_cells(g, blk, Field.Electric, [Pole.lorentz(w, w, gam), Pole.drude(0.7 * w, gam),
+                                          Pole.debye(2.0, 0.0 if lossless else 5e-12)])
+    if magnetic:
+        d.fill_cells(g, blk, Field.Magnetic, [Pole.lorentz(0.5 * w, 1.5 * w, gam)])
+    h = e.add_operator(d)
+    for q, n in enumerate([[3, 3, 2], [5, 2, 3], [1, 4, 2], [6, 5, 3]]):
+        u = e.u_component(q % 3)
+        u[tuple(n)] = 1.0 + q
+        e.set_component(0, q % 3, u)
+    return e, h
+
+
+def sine_pulse(f0, bw):
+    """tests/common/mod.rs:24-31."""
+    tau = 1.0 / (math.pi * bw)
+    t0 = 4.0 * tau
+
+    def f(t):
+        x = (t - t0) / tau
+        return math.exp(-x * x) * math.sin(2.0 * math.pi * f0 * (t - t0))
+    return f
+
+
+def line(kind, cells, dz, pml_hi, src, probes, medium=None, steps=2400, f0=4.5e9, bw=3.5e9,
+         scheme=None):
+    """Line::build (tests/common/mod.rs:190-300): a 1-cell-wide TEM line
+    along z, x-polarised (PMC y faces, PEC x faces), CPML 8 cells at z_min
+    and `pml_hi` at z_max, soft source on the two x edges at z node `src`,
+    E probes; `medium` = (eps_inf, mu_inf, electric poles, magnetic poles,
+    first cell) fills z cells first.. with eps_inf/mu_inf and the poles."""
+    z = [q * dz for q in range(cells + 1)]
+    g = E.Grid([[0.0, 5e-3], [0.0, 5e-3], z])
+    faces = [E.FaceCondition.Pec] * 6
+    faces[E.Face.YMin] = faces[E.Face.YMax] = E.FaceCondition.Pmc
+    m = E.CellMaterials.vacuum(g)
+    if medium is not None:
+        eps_inf, mu_inf, ep, mp, c0 = medium
+        m.fill_box([(0, 1), (0, 1), (c0, cells)], E.Material.dielectric(eps_inf).with_mu_r(mu_inf))
+    e = make(kind, g, m, faces=faces, threads=1)
+    e.add_operator(E.CpmlBoundary.new(e, [(E.Face.ZMin, E.Cpml.with_cells(8)),
+                                          (E.Face.ZMax, E.Cpml.with_cells(pml_hi))]))
+    if medium is not None:
+        d = Dispersive.new()
+        if scheme is not None:
+            d.with_scheme(scheme)
+        d.fill_cells(g, [(0, 1), (0, 1), (c0, cells)], Field.Electric, ep)
+        d.fill_cells(g, [(0, 1), (0, 1), (c0, cells)], Field.Magnetic, mp)
+        e.add_operator(d)
+    s = E.EdgeSource(sine_pulse(f0, bw), horizon=steps)
+    for p in ([0, 0, src], [0, 1, src]):
+        s.edge(E.Axis.X, p, 1.0)
+    e.add_operator(s)
+    ps = [e.add_operator(E.Probe.voltage(E.Axis.X, [0, 0, k])) for k in probes]
+    return e, ps
