@@ -109,6 +109,9 @@ ems_gpu_lumped_values = _sig("ems_gpu_lumped_values", _i, _p, _i, _i, _dp, _u64,
 ems_gpu_lumped_branch = _sig("ems_gpu_lumped_branch", _i, _p, _i, _dp, _dp, _dp)
 ems_gpu_add_dispersive = _sig("ems_gpu_add_dispersive", _i, _p, _p, _p, _p, C.POINTER(_i))
 ems_gpu_dispersive_state = _sig("ems_gpu_dispersive_state", _i, _p, _i, _i, _dp, _dp)
+ems_gpu_add_planewave = _sig("ems_gpu_add_planewave", _i, _p, _p, C.POINTER(_i))
+ems_gpu_planewave_table = _sig("ems_gpu_planewave_table", _i, _p, _i, _dp, _u64)
+ems_gpu_planewave_aux = _sig("ems_gpu_planewave_aux", _i, _p, _i, _dp, _dp)
 ems_gpu_run = _sig("ems_gpu_run", _i, _p, _u64)
 ems_gpu_step_with_energy = _sig("ems_gpu_step_with_energy", _i, _p, _dp)
 ems_gpu_energy = _sig("ems_gpu_energy", _i, _p, _dp)

@@ -24,6 +24,7 @@
 //   energy_*: the invariant as f64 chains in exactly the reference's order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -316,6 +317,17 @@ __device__ __forceinline__ void stf(float *base, unsigned boff, float v) {
   *(float *)((char *)base + boff) = v;
 }
 
+// Coefficient l of a node row whose pool offsets are `ro` (croff + 16*r,
+// wave-uniform). ro[12] flags the lanes whose row holds one value: that
+// value is read once per wave through the scalar unit instead of a vector
+// load per lane (the bits are the same).
+__device__ __forceinline__ float ldc(const float *__restrict__ cpool, const uint32_t *ro, int l,
+                                     unsigned k4) {
+  const uint32_t o = ro[l];
+  if ((ro[12] >> l) & 1u) return *(const float *)((const char *)cpool + o);
+  return ldf(cpool, o + k4);
+}
+
 // Coefficient of lane L at row r (i*ny + j, wave-uniform) at byte offset
 // k4 of the row; `ob` is the row's first node ((i*ny + j)*nz).
 __device__ __forceinline__ float coef_row(const Lane &L, size_t r, size_t ob, int nz,
@@ -361,7 +373,7 @@ __device__ __forceinline__ void load_plane(PlaneIn &P, const float *F, size_t N,
   // no 64-bit address math.
   const uint32_t *ro = croff + 16 * r;
 #pragma unroll
-  for (int l = 0; l < 12; l++) P.c[l] = ldf(cpool, ro[l] + k4);
+  for (int l = 0; l < 12; l++) P.c[l] = ldc(cpool, ro, l, k4);
   P.uxc = uxc;
   P.ux_j1 = jv ? uxj : 0.0f;
   P.uz_j1 = jv ? uzj : 0.0f;
@@ -649,9 +661,9 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
           const float hz_jm = j > 0 ? sh[2][w - 1][64 * m + lane] : 0.0f;
           const uint32_t *ro = croff + 16 * r;
           float e0 = ldf(A + ob, k4), e1 = ldf(A + N + ob, k4), e2 = ldf(A + 2 * N + ob, k4);
-          const float ca0 = ldf(cpool, ro[0] + k4), ca1 = ldf(cpool, ro[1] + k4);
-          const float ca2 = ldf(cpool, ro[2] + k4), cb0 = ldf(cpool, ro[3] + k4);
-          const float cb1 = ldf(cpool, ro[4] + k4), cb2 = ldf(cpool, ro[5] + k4);
+          const float ca0 = ldc(cpool, ro, 0, k4), ca1 = ldc(cpool, ro, 1, k4);
+          const float ca2 = ldc(cpool, ro, 2, k4), cb0 = ldc(cpool, ro, 3, k4);
+          const float cb1 = ldc(cpool, ro, 4, k4), cb2 = ldc(cpool, ro, 5, k4);
           // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
           if (iv) e0 = e_expr<OE>(ca0, e0, cb0, hz, hz_jm, hy, hy_km);
           // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
@@ -674,20 +686,43 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
   }
 }
 
-// croff[16*r + l]: byte offset in the pool of lane l's row for node row r.
+// flags[p] = 1 when pool row p holds one value (bit for bit) in all nz
+// entries. A thread per row (set-up only).
+__global__ void pool_row_const(const float *pool, size_t prows, int nz, uint8_t *flags) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= prows) return;
+  const uint32_t *row = (const uint32_t *)(pool + p * (size_t)nz);
+  const uint32_t v = row[0];
+  uint8_t f = 1;
+  for (int k = 1; k < nz; k++)
+    if (row[k] != v) {
+      f = 0;
+      break;
+    }
+  flags[p] = f;
+}
+
+// croff[16*r + l]: byte offset in the pool of lane l's row for node row r;
+// croff[16*r + 12]: bit l set when that row is constant.
 __global__ void build_row_offsets(uint32_t *croff, const uint32_t *const *index,
-                                  const uint32_t *base, size_t rows, int nz) {
+                                  const uint32_t *base, const uint8_t *flags, size_t rows,
+                                  int nz) {
   const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
+  uint32_t cm = 0;
   for (int l = 0; l < 16; l++) {
     uint32_t v = 0;
     if (l < 12) {
       const uint32_t *ix = index[l];
       // Lanes without rows point at the pool's zero row (base[l] == 0).
-      if (base[l] != 0) v = 4u * (base[l] + (ix ? ix[r] : (uint32_t)r) * (uint32_t)nz);
+      uint32_t prow = 0;
+      if (base[l] != 0) prow = base[l] / (uint32_t)nz + (ix ? ix[r] : (uint32_t)r);
+      v = 4u * prow * (uint32_t)nz;
+      cm |= (uint32_t)flags[prow] << l;
     }
-    croff[16 * r + l] = v;
+    croff[16 * r + l] = l == 12 ? cm : v;
   }
+  croff[16 * r + 12] = cm;
 }
 
 // Plain device copy (read n4 float4, write n4 float4): the stream-bandwidth
@@ -701,7 +736,7 @@ __global__ void __launch_bounds__(256) copy_f4(const float4 *__restrict__ a, flo
 
 // Serial operators (excite.rs, probe.rs, lumped.rs) and the gap rules of
 // Engine::advance (mod.rs:873-915).
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5 };
 
 // An installed lumped element (lumped.rs:367-406) and, with columns > 0,
 // its LumpedPort records (engine/port.rs:94-104).
@@ -757,6 +792,7 @@ __device__ __forceinline__ bool needs(const DevOp &o, uint64_t n) {
   }
   if (o.type == OP_LUMPED) return true;  // post_e every step
   if (o.type == OP_POLES) return false;  // slab hooks, not gap hooks
+  if (o.type == OP_PW) return true;      // post_h and post_e every step
   return n % o.interval == 0;
 }
 
@@ -880,8 +916,8 @@ __device__ Gaps gaps_at(const DevOp *ops, int nops, uint64_t n) {
   for (int q = 0; q < nops; q++) {
     const DevOp &o = ops[q];
     if (!needs(o, n)) continue;
-    if (is_esrc(o) || is_vprobe(o) || is_lumped(o)) need_post_e = true;
-    if (is_hsrc(o)) mid = true, movable = false;
+    if (is_esrc(o) || is_vprobe(o) || is_lumped(o) || o.type == OP_PW) need_post_e = true;
+    if (is_hsrc(o) || o.type == OP_PW) mid = true, movable = false;
     if (is_iprobe(o) || is_lumped(o)) mid = true;  // lumped pre_e is movable
   }
   Gaps g;
@@ -900,8 +936,8 @@ __device__ Gaps gaps_at(const DevOp *ops, int nops, uint64_t n) {
 __device__ __forceinline__ bool gap_writer(const DevOp &o, int which) {
   return which == 0 ? is_hsrc(o) : (is_esrc(o) || is_lumped(o));
 }
-__device__ void run_gap(Fields f, const DevOp *ops, int nops, uint64_t n, int which) {
-  int q = 0;
+__device__ void run_gap(Fields f, const DevOp *ops, int q0, int nops, uint64_t n, int which) {
+  int q = q0;
   while (q < nops) {
     const DevOp &o = ops[q];
     if (gap_writer(o, which)) {
@@ -928,20 +964,24 @@ __device__ void run_gap(Fields f, const DevOp *ops, int nops, uint64_t n, int wh
 }
 
 // Unfused steps: the gap between the H and E updates (the fields are
-// updated in place, so pre_e captures u^n here).
-__global__ void __launch_bounds__(256) gap_mid(Fields f, const DevOp *ops, int nops,
-                                               const uint64_t *step) {
+// updated in place, so pre_e captures u^n here). The gap's ops run in
+// registration order; a plane wave in the list (its own kernels) splits
+// it into segments [q0, q1), the first of which also captures u^n.
+__global__ void __launch_bounds__(256) gap_mid(Fields f, const DevOp *ops, int nops, int q0,
+                                               int q1, const uint64_t *step) {
   const uint64_t n = *step;
   const Gaps g = gaps_at(ops, nops, n);
-  if (g.gap_b) run_gap(f, ops, nops, n, 0);
-  lumped_capture(f, ops, nops);
+  if (g.gap_b) run_gap(f, ops, q0, q1, n, 0);
+  if (q0 == 0) lumped_capture(f, ops, nops);
 }
 
 // The post-E gap on the step result `f`. Fused steps keep the state before
 // the step in `fp` (the other buffer), whose u is u^n as pre_e reads it;
-// capture = 0 when the mid gap has already captured it.
+// capture = 0 when the mid gap has already captured it. Segments as in
+// the mid gap; the last one advances the step counter.
 __global__ void __launch_bounds__(256) gap_post_e(Fields f, Fields fp, int capture,
-                                                  const DevOp *ops, int nops, uint64_t *step) {
+                                                  const DevOp *ops, int nops, int q0, int q1,
+                                                  int last, uint64_t *step) {
   const uint64_t n = *step;
   const Gaps g = gaps_at(ops, nops, n);
   if (capture) {
@@ -949,16 +989,74 @@ __global__ void __launch_bounds__(256) gap_post_e(Fields f, Fields fp, int captu
     __syncthreads();
   }
   if (g.gap_a) {
-    if (g.moved) {
+    if (g.moved && q0 == 0) {
       // Moved post_h hooks (current probes) first (mod.rs:936-942).
       for (int p = threadIdx.x; p < nops; p += blockDim.x)
         if (is_iprobe(ops[p]) && n % ops[p].interval == 0) probe_sample(f, ops[p]);
       __syncthreads();
     }
-    run_gap(f, ops, nops, n, 1);
+    run_gap(f, ops, q0, q1, n, 1);
   }
   __syncthreads();
-  if (threadIdx.x == 0) *step = n + 1;
+  if (last && threadIdx.x == 0) *step = n + 1;
+}
+
+// ---- plane wave, total-field / scattered-field (planewave.rs) ----
+
+// Corrections grouped by the edge they correct (in list order within a
+// group): field[axis][idx] += f32(coef * sum_q w_q aux[m + q]).
+struct DevPwCorr {
+  int ntgt;
+  const uint32_t *start;  // ntgt + 1
+  const uint64_t *idx;
+  const int32_t *axis;
+  const double *coef, *w;  // w: 4 per correction
+  const uint32_t *m;
+};
+struct DevPw {
+  int naux;
+  double *e, *h;  // auxiliary grid: naux E nodes, naux - 1 H nodes
+  const double *ca, *cb, *da, *db;
+  const double *src;  // e[0] after the E update of step n
+  uint64_t nsrc;
+};
+
+// PlaneWave::post_h / post_e corrections (planewave.rs:1513-1545), a
+// thread per corrected edge; interp is the 4-point Lagrange sum (:653-655).
+template <bool ESIDE>
+__global__ void __launch_bounds__(256) pw_correct(Fields f, DevPwCorr c, const double *aux) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c.ntgt) return;
+  const uint32_t q0 = c.start[t], q1 = c.start[t + 1];
+  float *x = (ESIDE ? f.u : f.i)[c.axis[q0]] + c.idx[q0];
+  float v = *x;
+  for (uint32_t q = q0; q < q1; q++) {
+    const double *w = c.w + 4 * (size_t)q;
+    const uint32_t m = c.m[q];
+    const double it = w[0] * aux[m] + w[1] * aux[m + 1] + w[2] * aux[m + 2] + w[3] * aux[m + 3];
+    v += (float)(c.coef[q] * it);
+  }
+  *x = v;
+}
+
+// The auxiliary H update (planewave.rs:1525-1528), a thread per node.
+__global__ void __launch_bounds__(256) pw_aux_h(DevPw p) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.naux - 1) return;
+  p.h[m] = p.da[m] * p.h[m] - p.db[m] * (p.e[m + 1] - p.e[m]);
+}
+
+// The auxiliary E update and the hard source at node 0
+// (planewave.rs:1553-1559): e[0] = gain * a((n + 1)dt + lead), tabulated.
+__global__ void __launch_bounds__(256) pw_aux_e(DevPw p, const uint64_t *step) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= p.naux - 1) return;
+  if (m == 0) {
+    const uint64_t n = *step;
+    p.e[0] = n < p.nsrc ? p.src[n] : 0.0;
+    return;
+  }
+  p.e[m] = p.ca[m] * p.e[m] - p.cb[m] * (p.h[m] - p.h[m - 1]);
 }
 
 // ---- dispersive media (dispersive.rs) ----
@@ -1185,6 +1283,10 @@ struct HostOp {
   DevPoleConst *d_consts = nullptr;
   std::vector<double> c_inv_c, c_half_l;  // host copies of the constants
   std::vector<uint32_t> side_cidx[2];
+  // plane wave: auxiliary grid, corrections (0 E, 1 H), source table
+  DevPw pw{};
+  DevPwCorr pwc[2]{};
+  double *pw_src = nullptr;
 };
 
 }  // namespace
@@ -1271,7 +1373,7 @@ bool has_mid_ops(const ems_gpu *h) {
   for (auto &o : h->ops)
     if ((o.type == OP_SOURCE && (o.kind == 2 || o.kind == 3)) ||
         (o.type == OP_PROBE && o.quantity == 1) ||
-        (o.type == OP_LUMPED && (o.lp.branch || o.lp.record)))
+        (o.type == OP_LUMPED && (o.lp.branch || o.lp.record)) || o.type == OP_PW)
       return true;
   return false;
 }
@@ -1285,10 +1387,16 @@ bool has_magnetic_poles(const ems_gpu *h) {
   return false;
 }
 
+bool has_planewave(const ems_gpu *h) {
+  for (auto &o : h->ops)
+    if (o.type == OP_PW) return true;
+  return false;
+}
+
 bool use_fused(const ems_gpu *h) {
   size_t ns = 0;
   for (auto &s : h->slabs) ns += s.len != 0;
-  return h->fused && !has_hsrc(h) && !has_magnetic_poles(h) && ns <= 32;
+  return h->fused && !has_hsrc(h) && !has_magnetic_poles(h) && !has_planewave(h) && ns <= 32;
 }
 
 // The dispersive media as slab hooks (dispersive.rs:1700-1750), in
@@ -1515,20 +1623,64 @@ int launch_fused(ems_gpu *h, int par) {
   return 0;
 }
 
-// Post-E gap on the result `f`; `fp` holds the state before a fused step
-// (prev = true), whose u the lumped pre_e hooks capture.
-int launch_gap_post_e(ems_gpu *h, const Fields &f, const Fields &fp, bool prev) {
-  hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(256), 0, h->stream, f, fp, prev ? 1 : 0,
-                     h->d_ops, (int)h->ops.size(), h->d_step);
+// A plane wave's hooks in a gap: the corrections, then its auxiliary grid
+// (stage 0 after the H update, 1 after the E update).
+int launch_pw(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
+  const DevPwCorr &c = o.pwc[stage == 0 ? 1 : 0];
+  if (c.ntgt) {
+    if (stage == 0)
+      hipLaunchKernelGGL(pw_correct<false>, dim3((c.ntgt + 255) / 256), dim3(256), 0, h->stream,
+                         f, c, (const double *)o.pw.e);
+    else
+      hipLaunchKernelGGL(pw_correct<true>, dim3((c.ntgt + 255) / 256), dim3(256), 0, h->stream,
+                         f, c, (const double *)o.pw.h);
+  }
+  const unsigned nb = (unsigned)((o.pw.naux + 254) / 256);
+  if (stage == 0)
+    hipLaunchKernelGGL(pw_aux_h, dim3(nb), dim3(256), 0, h->stream, o.pw);
+  else
+    hipLaunchKernelGGL(pw_aux_e, dim3(nb), dim3(256), 0, h->stream, o.pw,
+                       (const uint64_t *)h->d_step);
   CK(h, hipGetLastError());
+  return 0;
+}
+
+// Post-E gap on the result `f`; `fp` holds the state before a fused step
+// (prev = true), whose u the lumped pre_e hooks capture. Plane waves run
+// at their place among the gap's ops.
+int launch_gap_post_e(ems_gpu *h, const Fields &f, const Fields &fp, bool prev) {
+  const int nops = (int)h->ops.size();
+  int q0 = 0;
+  for (int q = 0; q <= nops; q++) {
+    if (q < nops && h->ops[q].type != OP_PW) continue;
+    const bool last = q == nops;
+    hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(256), 0, h->stream, f, fp,
+                       prev && q0 == 0 ? 1 : 0, h->d_ops, nops, q0, q, last ? 1 : 0, h->d_step);
+    CK(h, hipGetLastError());
+    if (!last) {
+      int rc = launch_pw(h, h->ops[q], f, 1);
+      if (rc) return rc;
+    }
+    q0 = q + 1;
+  }
   return 0;
 }
 
 int launch_gap_mid(ems_gpu *h, const Fields &f) {
   if (!has_mid_ops(h)) return 0;
-  hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops, (int)h->ops.size(),
-                     (const uint64_t *)h->d_step);
-  CK(h, hipGetLastError());
+  const int nops = (int)h->ops.size();
+  int q0 = 0;
+  for (int q = 0; q <= nops; q++) {
+    if (q < nops && h->ops[q].type != OP_PW) continue;
+    hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops, nops, q0, q,
+                       (const uint64_t *)h->d_step);
+    CK(h, hipGetLastError());
+    if (q < nops) {
+      int rc = launch_pw(h, h->ops[q], f, 0);
+      if (rc) return rc;
+    }
+    q0 = q + 1;
+  }
   return 0;
 }
 
@@ -1607,13 +1759,20 @@ int build_pool(ems_gpu *h) {
   CK(h, hipMalloc(&d_b, sizeof b32));
   CK(h, hipMemcpy(d_ix, ix, sizeof ix, hipMemcpyHostToDevice));
   CK(h, hipMemcpy(d_b, b32, sizeof b32, hipMemcpyHostToDevice));
+  const size_t prows = (size_t)(total / (uint64_t)nz);
+  uint8_t *d_flags = nullptr;
+  CK(h, hipMalloc(&d_flags, prows));
+  hipLaunchKernelGGL(pool_row_const, dim3((unsigned)((prows + 255) / 256)), dim3(256), 0,
+                     h->stream, (const float *)pool, prows, nz, d_flags);
+  CK(h, hipGetLastError());
   hipLaunchKernelGGL(build_row_offsets, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0,
                      h->stream, h->croff, (const uint32_t *const *)d_ix, (const uint32_t *)d_b,
-                     rows, nz);
+                     (const uint8_t *)d_flags, rows, nz);
   CK(h, hipGetLastError());
   CK(h, hipStreamSynchronize(h->stream));
   CK(h, hipFree(d_ix));
   CK(h, hipFree(d_b));
+  CK(h, hipFree(d_flags));
   // Point the lanes into the pool and free their own copies.
   for (int l = 0; l < 12; l++)
     if (h->lanes[l].rows) {
@@ -1876,6 +2035,7 @@ void ems_gpu_destroy(ems_gpu *h) {
     (void)hipFree(o.d_count);
     for (void *p : o.mem) (void)hipFree(p);
     for (double *p : o.tables) (void)hipFree(p);
+    if (o.pw_src) (void)hipFree(o.pw_src);
     if (o.lp.rec) (void)hipFree(o.lp.rec), (void)hipFree(o.lp.pv), (void)hipFree(o.lp.pi);
   }
   if (h->d_ops) (void)hipFree(h->d_ops);
@@ -2238,6 +2398,106 @@ int ems_gpu_dispersive_state(ems_gpu *h, int id, int field, double *j, double *q
   const size_t nb = o.side_cidx[field].size() * sizeof(double);
   if (nb && j) CK(h, hipMemcpy(j, o.side[field].j, nb, hipMemcpyDeviceToHost));
   if (nb && q) CK(h, hipMemcpy(q, o.side[field].q, nb, hipMemcpyDeviceToHost));
+  return EMS_OK;
+}
+
+int ems_gpu_add_planewave(ems_gpu *h, const ems_pw_desc *d, int *id) {
+  ALIVE(h);
+  if (!d || d->n_aux < 3 || !d->ca || !d->cb || !d->da || !d->db)
+    return fail(h, EMS_E_INVALID, "plane wave: auxiliary grid arrays");
+  HostOp o;
+  o.type = OP_PW;
+  auto put = [&](const void *src, size_t bytes, void **dst) -> int {
+    int rc = upload(h, dst, src, bytes);
+    if (!rc) o.mem.push_back(*dst);
+    return rc;
+  };
+  const int na = d->n_aux;
+  int rc;
+  const std::vector<double> z(na, 0.0);
+  DevPw &p = o.pw;
+  p.naux = na;
+  if ((rc = put(z.data(), na * sizeof(double), (void **)&p.e))) return rc;
+  if ((rc = put(z.data(), (na - 1) * sizeof(double), (void **)&p.h))) return rc;
+  if ((rc = put(d->ca, na * sizeof(double), (void **)&p.ca))) return rc;
+  if ((rc = put(d->cb, na * sizeof(double), (void **)&p.cb))) return rc;
+  if ((rc = put(d->da, (na - 1) * sizeof(double), (void **)&p.da))) return rc;
+  if ((rc = put(d->db, (na - 1) * sizeof(double), (void **)&p.db))) return rc;
+  const ems_pw_corrections *cs[2] = {&d->e_corr, &d->h_corr};
+  for (int side = 0; side < 2; side++) {
+    const ems_pw_corrections *c = cs[side];
+    const int nc = c->n;
+    if (nc < 0 || (nc && (!c->axis || !c->nodes || !c->coef || !c->m || !c->w)))
+      return fail(h, EMS_E_INVALID, "plane wave: correction arrays");
+    // Stable order by corrected edge: each edge's corrections stay in list
+    // order and one thread applies them.
+    std::vector<uint64_t> key(nc);
+    for (int q = 0; q < nc; q++) {
+      const int32_t *nd = c->nodes + 3 * q;
+      if (c->axis[q] < 0 || c->axis[q] > 2 || !node_ok(h, nd))
+        return fail(h, EMS_E_INVALID, "plane wave: corrected edge out of range");
+      if ((int64_t)c->m[q] + 3 >= (int64_t)(side == 0 ? na - 1 : na))
+        return fail(h, EMS_E_INVALID, "plane wave: stencil beyond the auxiliary grid");
+      key[q] = ((((uint64_t)nd[0] * h->n[1] + nd[1]) * h->n[2] + nd[2]) << 2) | (uint64_t)c->axis[q];
+    }
+    std::vector<int> ord(nc);
+    for (int q = 0; q < nc; q++) ord[q] = q;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+    std::vector<uint32_t> start, m(nc);
+    std::vector<uint64_t> idx;
+    std::vector<int32_t> axis(nc);
+    std::vector<double> coef(nc), w(4 * (size_t)(nc ? nc : 1));
+    for (int r = 0; r < nc; r++) {
+      const int q = ord[r];
+      if (r == 0 || key[q] != key[ord[r - 1]]) {
+        start.push_back((uint32_t)r);
+        idx.push_back(key[q] >> 2);
+      }
+      axis[r] = c->axis[q], coef[r] = c->coef[q], m[r] = c->m[q];
+      for (int t = 0; t < 4; t++) w[4 * (size_t)r + t] = c->w[4 * (size_t)q + t];
+    }
+    start.push_back((uint32_t)nc);
+    DevPwCorr &dc = o.pwc[side];
+    dc.ntgt = (int)idx.size();
+    if ((rc = put(start.data(), start.size() * sizeof(uint32_t), (void **)&dc.start))) return rc;
+    if ((rc = put(idx.data(), idx.size() * sizeof(uint64_t), (void **)&dc.idx))) return rc;
+    if ((rc = put(axis.data(), nc * sizeof(int32_t), (void **)&dc.axis))) return rc;
+    if ((rc = put(coef.data(), nc * sizeof(double), (void **)&dc.coef))) return rc;
+    if ((rc = put(m.data(), nc * sizeof(uint32_t), (void **)&dc.m))) return rc;
+    if ((rc = put(w.data(), w.size() * sizeof(double), (void **)&dc.w))) return rc;
+  }
+  h->ops.push_back(o);
+  h->ops_dirty = true;
+  drop_graphs(h);
+  if (id) *id = (int)h->ops.size() - 1;
+  return EMS_OK;
+}
+
+int ems_gpu_planewave_table(ems_gpu *h, int id, const double *values, uint64_t count) {
+  ALIVE(h);
+  if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_PW || (count && !values))
+    return fail(h, EMS_E_INVALID, "not a plane wave id / values");
+  HostOp &o = h->ops[id];
+  CK(h, hipStreamSynchronize(h->stream));
+  double *nt = nullptr;
+  int rc = upload(h, (void **)&nt, values, count * sizeof(double));
+  if (rc) return rc;
+  if (o.pw_src) CK(h, hipFree(o.pw_src));
+  o.pw_src = nt;
+  o.pw.src = nt;
+  o.pw.nsrc = count;
+  drop_graphs(h);  // the kernels take the table by value
+  return EMS_OK;
+}
+
+int ems_gpu_planewave_aux(ems_gpu *h, int id, double *e, double *hh) {
+  ALIVE(h);
+  if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_PW)
+    return fail(h, EMS_E_INVALID, "not a plane wave id");
+  const HostOp &o = h->ops[id];
+  CK(h, hipStreamSynchronize(h->stream));
+  if (e) CK(h, hipMemcpy(e, o.pw.e, o.pw.naux * sizeof(double), hipMemcpyDeviceToHost));
+  if (hh) CK(h, hipMemcpy(hh, o.pw.h, (o.pw.naux - 1) * sizeof(double), hipMemcpyDeviceToHost));
   return EMS_OK;
 }
 

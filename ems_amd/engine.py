@@ -669,6 +669,7 @@ class Engine:
         self._tables: list = []
         self._lumped_n: dict = {}
         self._poles_n: dict = {}
+        self._pw_tables: list = []
         for l in range(12):
             self._upload_lane(l, lanes[l], pool)
 
@@ -740,6 +741,16 @@ class Engine:
             self.setup.append(("probe", int(op.quantity), axis, nodes, w, op._interval))
             op._engine = self
             op._id = oid.value
+        elif type(op).__name__ == "PlaneWave":
+            from . import planewave as PW
+            d, keep = PW.make_desc(op)
+            oid = C.c_int()
+            self._check(nat.ems_gpu_add_planewave(h, C.byref(d), C.byref(oid)))
+            table = PW.SourceTable(op)
+            self._pw_tables.append((oid.value, table))
+            self.setup.append(("planewave", op))
+            op._bind(self, oid.value)
+            self._ensure_tables(self.timestep() + 1)
         elif type(op).__name__ == "Dispersive":
             # Dispersive::add_to (dispersive.rs:2479-2520): host install,
             # then the device medium.
@@ -797,6 +808,12 @@ class Engine:
             self.setup.append(("lane", l, rows, index))
 
     def _ensure_tables(self, upto: int) -> None:
+        for oid, t in self._pw_tables:
+            def up_pw(vals, oid=oid):
+                v = np.ascontiguousarray(vals)
+                self._check(nat.ems_gpu_planewave_table(self._h, oid, nat.ptr(v, C.c_double),
+                                                        v.size))
+            t.ensure(upto, up_pw)
         for oid, t in self._tables:
             def up(term, vals, oid=oid):
                 v = np.ascontiguousarray(vals)
@@ -813,6 +830,12 @@ class Engine:
         self._check(nat.ems_gpu_lumped_values(self._h, oid, which, nat.ptr(out, C.c_double),
                                               out.size, C.byref(cnt), C.byref(first)))
         return out, first.value
+
+    def _planewave_aux(self, oid: int, naux: int):
+        e, hh = np.zeros(naux), np.zeros(naux - 1)
+        self._check(nat.ems_gpu_planewave_aux(self._h, oid, nat.ptr(e, C.c_double),
+                                              nat.ptr(hh, C.c_double)))
+        return e, hh
 
     def _dispersive_state(self, oid: int, field: int):
         n = self._poles_n[oid][field]

@@ -184,6 +184,39 @@ int ems_gpu_add_dispersive(ems_gpu *h, const ems_pole_side *electric,
  * j^n, q^n per pole (Dispersive::electric_energy / magnetic_energy). */
 int ems_gpu_dispersive_state(ems_gpu *h, int id, int field, double *j, double *q);
 
+/* A total-field / scattered-field plane wave (PlaneWave,
+ * src/engine/planewave.rs:440-1561) in a homogeneous background, after the
+ * host's PlaneWave::new (:684-1049): the auxiliary 1-D grid's coefficients
+ * (ca, cb on n_aux E nodes; da, db on n_aux - 1 H nodes) and the two
+ * correction lists (Correction, :424-431): E edges on the box surface
+ * corrected after the E update from the auxiliary H, H edges just outside
+ * the box corrected after the H update from the auxiliary E, each
+ *   field[axis][node] += f32(coef * sum_q w[4q'+q] * aux[m + q]).
+ * Corrections of one edge apply in list order. Every step the device then
+ * advances the auxiliary grid (:1525-1528, :1553-1559), e[0] from the table
+ * of ems_gpu_planewave_table. A plane wave runs the unfused schedule (its
+ * post_h hook sits between the H and E updates), at its place among the
+ * other serial operators. */
+typedef struct {
+    int32_t n;
+    const int32_t *axis, *nodes;
+    const double *coef;
+    const uint32_t *m;
+    const double *w;
+} ems_pw_corrections;
+typedef struct {
+    int32_t n_aux;
+    const double *ca, *cb, *da, *db;
+    ems_pw_corrections e_corr, h_corr;
+} ems_pw_desc;
+int ems_gpu_add_planewave(ems_gpu *h, const ems_pw_desc *d, int *id);
+/* values[n] = gain * a((n + 1)dt + lead), the auxiliary source after the E
+ * update of step n, for n < count (0 beyond). Replaces the table. */
+int ems_gpu_planewave_table(ems_gpu *h, int id, const double *values, uint64_t count);
+/* The auxiliary grid's state: e (n_aux values), h (n_aux - 1) (for
+ * PlaneWave::incident_e / incident_h, :1446-1470). */
+int ems_gpu_planewave_aux(ems_gpu *h, int id, double *e, double *hh);
+
 /* Engine::run (src/engine/mod.rs:825-828). */
 int ems_gpu_run(ems_gpu *h, uint64_t n);
 /* Engine::step_with_energy (src/engine/mod.rs:808-823): one step, *w = the

@@ -50,6 +50,9 @@ orc_lumped_values = _sig("orc_lumped_values", _u64, _p, _i, _i, _dp, _u64)
 orc_lumped_branch = _sig("orc_lumped_branch", _i, _p, _i, _dp, _dp, _dp)
 orc_add_dispersive = _sig("orc_add_dispersive", _i, _p, _p, _p, _p)
 orc_dispersive_state = _sig("orc_dispersive_state", _i, _p, _i, _i, _dp, _dp)
+orc_add_planewave = _sig("orc_add_planewave", _i, _p, _p)
+orc_planewave_table = _sig("orc_planewave_table", _i, _p, _i, _dp, _u64)
+orc_planewave_aux = _sig("orc_planewave_aux", _i, _p, _i, _dp, _dp)
 orc_run = _sig("orc_run", None, _p, _u64)
 orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
 orc_energy = _sig("orc_energy", _d, _p)
@@ -94,6 +97,7 @@ class OracleEngine:
         self._tables = []
         self._lumped = {}
         self._poles_n = {}
+        self._pw_tables = []
 
     def _set_lane(self, l, lane):
         n = self.grid.nodes()
@@ -108,6 +112,11 @@ class OracleEngine:
                                 P(index, C.c_uint32)) == 0
 
     def _ensure_tables(self, upto):
+        for oid, t in self._pw_tables:
+            def up_pw(vals, oid=oid):
+                v = np.ascontiguousarray(vals)
+                assert orc_planewave_table(self._h, oid, P(v, C.c_double), v.size) == 0
+            t.ensure(upto, up_pw)
         for oid, t in self._tables:
             def up(term, vals, oid=oid):
                 v = np.ascontiguousarray(vals)
@@ -119,6 +128,11 @@ class OracleEngine:
         out = np.empty(n, dtype=np.float64)
         orc_lumped_values(self._h, oid, which, P(out, C.c_double), n)
         return out, self._lumped[oid][1]
+
+    def _planewave_aux(self, oid, naux):
+        ev, hv = np.zeros(naux), np.zeros(naux - 1)
+        assert orc_planewave_aux(self._h, oid, P(ev, C.c_double), P(hv, C.c_double)) == 0
+        return ev, hv
 
     def _dispersive_state(self, oid, field):
         n = self._poles_n[oid][field]
@@ -169,6 +183,13 @@ class OracleEngine:
             op._oid = orc_add_probe(h, int(op.quantity), len(op.entries), P(axis, C.c_int),
                                     P(nodes, C.c_int), P(w, C.c_double), op._interval)
             assert op._oid >= 0
+        elif type(op).__name__ == "PlaneWave":
+            from ems_amd import planewave as PW
+            d, keep = PW.make_desc(op)
+            oid = orc_add_planewave(h, C.addressof(d))
+            assert oid >= 0
+            self._pw_tables.append((oid, PW.SourceTable(op)))
+            op._bind(self, oid)
         elif type(op).__name__ == "Dispersive":
             from ems_amd import dispersive as DP
             plan = op._plan(self.grid, self.dt, self._dense_lanes)

@@ -69,7 +69,24 @@ typedef struct {
     float *b, *a, *k, *psi;
 } cpml_t;
 
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5 };
+
+/* Plane-wave corrections (Correction, planewave.rs:424-431), in list
+ * order, and the auxiliary grid (PlaneWave, :458-483). */
+typedef struct {
+    int n;
+    int *axis;
+    size_t *idx;
+    double *coef, *w;
+    uint32_t *m;
+} pwcorr_t;
+typedef struct {
+    int naux;
+    double *e, *h, *ca, *cb, *da, *db;
+    pwcorr_t ec, hc;
+    double *src;
+    uint64_t nsrc;
+} pw_t;
 
 /* One side of a dispersive medium (Side, dispersive.rs:977-1040): edges in
  * (plane, axis, j, k) order, poles in CSR layout, f64 branch state. */
@@ -129,6 +146,8 @@ typedef struct {
     lumped_t *lp;
     /* dispersive medium */
     poles_t *pm;
+    /* plane wave */
+    pw_t *pw;
 } op_t;
 
 struct orc_engine {
@@ -622,6 +641,33 @@ static double lumped_energy(const orc_engine *e, const op_t *o) {
     return s;
 }
 
+/* interp (planewave.rs:653-655): 4-point Lagrange sum. */
+static double pw_interp(const double *v, uint32_t m, const double *w) {
+    return w[0] * v[m] + w[1] * v[m + 1] + w[2] * v[m + 2] + w[3] * v[m + 3];
+}
+
+/* PlaneWave::post_h, auxiliary-grid branch (planewave.rs:1513-1529): the H
+ * corrections from the auxiliary E at n dt, then the auxiliary H update. */
+static void pw_post_h(orc_engine *e, pw_t *p) {
+    for (int c = 0; c < p->hc.n; c++)
+        e->i[p->hc.axis[c]][p->hc.idx[c]] +=
+            (float)(p->hc.coef[c] * pw_interp(p->e, p->hc.m[c], p->hc.w + 4 * (size_t)c));
+    for (int m = 0; m < p->naux - 1; m++)
+        p->h[m] = p->da[m] * p->h[m] - p->db[m] * (p->e[m + 1] - p->e[m]);
+}
+
+/* PlaneWave::post_e (planewave.rs:1533-1560): the E corrections from the
+ * auxiliary H at (n + 1/2)dt, the auxiliary E update, the hard source. */
+static void pw_post_e(orc_engine *e, pw_t *p, uint64_t n) {
+    for (int c = 0; c < p->ec.n; c++)
+        e->u[p->ec.axis[c]][p->ec.idx[c]] +=
+            (float)(p->ec.coef[c] * pw_interp(p->h, p->ec.m[c], p->ec.w + 4 * (size_t)c));
+    const int last = p->naux - 1;
+    for (int m = 1; m < last; m++)
+        p->e[m] = p->ca[m] * p->e[m] - p->cb[m] * (p->h[m] - p->h[m - 1]);
+    p->e[0] = n < p->nsrc ? p->src[n] : 0.0;
+}
+
 static int is_hsrc(const op_t *o) { return o->type == OP_SOURCE && (o->kind == 2 || o->kind == 3); }
 static int is_esrc(const op_t *o) { return o->type == OP_SOURCE && (o->kind == 0 || o->kind == 1); }
 static int is_iprobe(const op_t *o) { return o->type == OP_PROBE && o->quantity == 1; }
@@ -635,6 +681,7 @@ static int needs(const op_t *o, uint64_t n) {
     }
     if (o->type == OP_LUMPED) return 1; /* pre_e / post_e every step */
     if (o->type == OP_POLES) return 0;  /* slab hooks only */
+    if (o->type == OP_PW) return 1;     /* post_h and post_e every step */
     return n % o->interval == 0;
 }
 
@@ -646,8 +693,8 @@ static void one_step(orc_engine *e, int energy) {
     for (int q = 0; q < e->nops; q++) {
         op_t *o = &e->ops[q];
         if (!needs(o, n)) continue;
-        if (is_esrc(o) || is_vprobe(o) || o->type == OP_LUMPED) need_post_e = 1;
-        if (is_hsrc(o)) mid = 1, mid_movable = 0;
+        if (is_esrc(o) || is_vprobe(o) || o->type == OP_LUMPED || o->type == OP_PW) need_post_e = 1;
+        if (is_hsrc(o) || o->type == OP_PW) mid = 1, mid_movable = 0;
         if (is_iprobe(o) || o->type == OP_LUMPED) mid = 1;
     }
     int gap_b = mid && !mid_movable;
@@ -662,6 +709,7 @@ static void one_step(orc_engine *e, int energy) {
             op_t *o = &e->ops[q];
             if (is_hsrc(o)) source_apply(e, o, n);
             if (is_iprobe(o) && n % o->interval == 0) probe_sample(e, o);
+            if (o->type == OP_PW) pw_post_h(e, o->pw);
         }
         for (int q = 0; q < e->nops; q++)
             if (e->ops[q].type == OP_LUMPED) lumped_pre_e(e, &e->ops[q]);
@@ -683,6 +731,7 @@ static void one_step(orc_engine *e, int energy) {
             if (is_esrc(o)) source_apply(e, o, n);
             if (is_vprobe(o) && n % o->interval == 0) probe_sample(e, o);
             if (o->type == OP_LUMPED) lumped_post_e(e, o, n);
+            if (o->type == OP_PW) pw_post_e(e, o->pw, n);
         }
     }
     e->step++;
@@ -755,6 +804,14 @@ void orc_destroy(orc_engine *e) {
     for (int q = 0; q < e->nops; q++) {
         op_t *o = &e->ops[q];
         free(o->axis), free(o->idx), free(o->wf), free(o->amp), free(o->wd), free(o->vals);
+        pw_t *pwp = o->pw;
+        if (pwp) {
+            pwcorr_t *cc[2] = {&pwp->ec, &pwp->hc};
+            for (int t = 0; t < 2; t++)
+                free(cc[t]->axis), free(cc[t]->idx), free(cc[t]->coef), free(cc[t]->w), free(cc[t]->m);
+            free(pwp->e), free(pwp->h), free(pwp->ca), free(pwp->cb), free(pwp->da), free(pwp->db);
+            free(pwp->src), free(pwp);
+        }
         poles_t *m = o->pm;
         if (m) {
             for (int f = 0; f < 2; f++) {
@@ -1023,6 +1080,59 @@ int orc_dispersive_state(orc_engine *e, int id, int field, double *j, double *q)
     const poleside_t *s = &e->ops[id].pm->side[field];
     memcpy(j, s->j, s->np * sizeof(double));
     memcpy(q, s->q, s->np * sizeof(double));
+    return 0;
+}
+
+static void pwcorr_copy(orc_engine *e, pwcorr_t *d, const orc_pw_corrections *c) {
+    d->n = c->n;
+    d->axis = (int *)calloc(c->n ? c->n : 1, sizeof(int));
+    d->idx = (size_t *)calloc(c->n ? c->n : 1, sizeof(size_t));
+    d->coef = dup_d(c->coef, c->n);
+    d->w = dup_d(c->w, 4 * (size_t)c->n);
+    d->m = (uint32_t *)calloc(c->n ? c->n : 1, sizeof(uint32_t));
+    for (int q = 0; q < c->n; q++) {
+        const int *nd = c->nodes + 3 * q;
+        d->axis[q] = c->axis[q];
+        d->idx[q] = ((size_t)nd[0] * e->n[1] + nd[1]) * e->n[2] + nd[2];
+        d->m[q] = c->m[q];
+    }
+}
+
+int orc_add_planewave(orc_engine *e, const orc_pw_desc *d) {
+    if (d->n_aux < 3) return -1;
+    const orc_pw_corrections *cs[2] = {&d->e_corr, &d->h_corr};
+    for (int t = 0; t < 2; t++)
+        for (int q = 0; q < cs[t]->n; q++)
+            if (!node_ok(e, cs[t]->nodes + 3 * q) || cs[t]->m[q] + 3 >= (uint32_t)d->n_aux - t)
+                return -1;
+    int id = add_op(e);
+    op_t *o = &e->ops[id];
+    o->type = OP_PW;
+    pw_t *p = (pw_t *)calloc(1, sizeof(pw_t));
+    o->pw = p;
+    p->naux = d->n_aux;
+    p->e = dup_d(NULL, d->n_aux), p->h = dup_d(NULL, d->n_aux - 1);
+    p->ca = dup_d(d->ca, d->n_aux), p->cb = dup_d(d->cb, d->n_aux);
+    p->da = dup_d(d->da, d->n_aux - 1), p->db = dup_d(d->db, d->n_aux - 1);
+    pwcorr_copy(e, &p->ec, &d->e_corr);
+    pwcorr_copy(e, &p->hc, &d->h_corr);
+    return id;
+}
+
+int orc_planewave_table(orc_engine *e, int id, const double *values, uint64_t count) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_PW) return -1;
+    pw_t *p = e->ops[id].pw;
+    free(p->src);
+    p->src = dup_d(values, count);
+    p->nsrc = count;
+    return 0;
+}
+
+int orc_planewave_aux(orc_engine *e, int id, double *ev, double *hv) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_PW) return -1;
+    pw_t *p = e->ops[id].pw;
+    memcpy(ev, p->e, p->naux * sizeof(double));
+    memcpy(hv, p->h, (p->naux - 1) * sizeof(double));
     return 0;
 }
 

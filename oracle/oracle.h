@@ -115,6 +115,25 @@ int orc_add_dispersive(orc_engine *e, const orc_pole_side *electric,
                        const orc_pole_side *magnetic, const orc_pole_consts *consts);
 int orc_dispersive_state(orc_engine *e, int id, int field, double *j, double *q);
 
+/* Plane wave (planewave.rs) after the host's PlaneWave::new: laid out as
+ * ems_pw_desc of include/ems_gpu.h; post_h / post_e hooks in the gaps.
+ * The table gives the auxiliary source e[0] after the E update of step n. */
+typedef struct {
+    int n;
+    const int *axis, *nodes;
+    const double *coef;
+    const uint32_t *m;
+    const double *w;
+} orc_pw_corrections;
+typedef struct {
+    int n_aux;
+    const double *ca, *cb, *da, *db;
+    orc_pw_corrections e_corr, h_corr;
+} orc_pw_desc;
+int orc_add_planewave(orc_engine *e, const orc_pw_desc *d);
+int orc_planewave_table(orc_engine *e, int id, const double *values, uint64_t count);
+int orc_planewave_aux(orc_engine *e, int id, double *ev, double *hv);
+
 void orc_run(orc_engine *e, uint64_t n);
 double orc_step_with_energy(orc_engine *e);
 double orc_energy(orc_engine *e);

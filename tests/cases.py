@@ -327,3 +327,55 @@ def line(kind, cells, dz, pml_hi, src, probes, medium=None, steps=2400, f0=4.5e9
     e.add_operator(s)
     ps = [e.add_operator(E.Probe.voltage(E.Axis.X, [0, 0, k])) for k in probes]
     return e, ps
+
+
+# ------------------------------------------------------------- plane wave --
+from ems_amd import planewave as PWM  # noqa: E402
+
+
+def leakage(kind, n, lo, hi, d, pol, medium=None, f0=8e9, grid=None, record=None):
+    """leakage_in (tests/planewave.rs:48-143): cube of n cells with an
+    8-cell CPML, TF box lo..=hi; the worst scattered-field E (between the
+    box and the CPML, every second step) over the incident peak, in dB."""
+    medium = medium or PWM.Medium()
+    g = grid or E.Grid.uniform([n] * 3, [1e-3] * 3)
+    mat = E.Material.dielectric(medium.eps_r).with_mu_r(medium.mu_r) \
+        .with_kappa(medium.kappa).with_sigma_m(medium.sigma_m)
+    e = make(kind, g, E.CellMaterials(g, mat), threads=1)
+    e.add_operator(E.CpmlBoundary.new(e, [(f, E.Cpml.with_cells(8)) for f in E.Face]))
+    bw = 0.5 * f0
+    spec = PWM.PlaneWaveSpec.new([lo] * 3, [hi] * 3, d, pol).with_medium(medium) \
+        .with_match_frequency(f0)
+    pw = e.add_operator(PWM.PlaneWave.new(e, spec, sine_pulse(f0, bw)))
+    steps = int((8.0 / (math.pi * bw) + 3.0 * n * 1e-3 / medium.velocity()) / e.dt)
+    nn = g.nodes()
+    p0, p1 = 9, n - 9
+    lens = []
+    masks = []
+    for a in range(3):
+        m = np.zeros(nn, dtype=bool)
+        m[p0:p1, p0:p1, p0:p1] = True
+        sl = [slice(lo, hi + 1)] * 3
+        sl[a] = slice(lo, hi)
+        m[tuple(sl)] = False
+        masks.append(m)
+        ln = np.array([g.delta(a, i) if i + 1 < nn[a] else 1.0 for i in range(nn[a])])
+        shape = [1, 1, 1]
+        shape[a] = nn[a]
+        lens.append(np.broadcast_to(ln.reshape(shape), nn))
+    amp = math.sqrt(sum(x * x for x in pol))
+    centre = [0.5 * (g.lines[a][(lo + hi) // 2] + g.lines[a][(lo + hi) // 2 + 1]) for a in range(3)]
+    worst = inc_peak = 0.0
+    for step in range(steps):
+        e.run(1)
+        if record is not None:
+            record(e, pw, step)
+        if step % 2 == 1:
+            continue
+        for a in range(3):
+            u = e.u_component(a).astype(np.float64)
+            worst = max(worst, float(np.max(np.abs(u[masks[a]] / lens[a][masks[a]]))))
+        inc = pw.incident_e(centre)
+        inc_peak = max(inc_peak, math.sqrt(sum(x * x for x in inc)))
+    assert 0.5 < inc_peak / amp <= 1.0, inc_peak
+    return 20.0 * math.log10(worst / inc_peak), e, pw
