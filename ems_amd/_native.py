@@ -112,6 +112,11 @@ ems_gpu_dispersive_state = _sig("ems_gpu_dispersive_state", _i, _p, _i, _i, _dp,
 ems_gpu_add_planewave = _sig("ems_gpu_add_planewave", _i, _p, _p, C.POINTER(_i))
 ems_gpu_planewave_table = _sig("ems_gpu_planewave_table", _i, _p, _i, _dp, _u64)
 ems_gpu_planewave_aux = _sig("ems_gpu_planewave_aux", _i, _p, _i, _dp, _dp)
+ems_gpu_batch_create = _sig("ems_gpu_batch_create", _i, C.POINTER(_p), _i, C.POINTER(_p))
+ems_gpu_batch_destroy = _sig("ems_gpu_batch_destroy", None, _p)
+ems_gpu_batch_last_error = _sig("ems_gpu_batch_last_error", C.c_char_p, _p)
+ems_gpu_batch_set_schedule = _sig("ems_gpu_batch_set_schedule", _i, _p, _i, _i, _i)
+ems_gpu_batch_run = _sig("ems_gpu_batch_run", _i, _p, _u8p, _u64)
 ems_gpu_run = _sig("ems_gpu_run", _i, _p, _u64)
 ems_gpu_step_with_energy = _sig("ems_gpu_step_with_energy", _i, _p, _dp)
 ems_gpu_energy = _sig("ems_gpu_energy", _i, _p, _dp)
@@ -147,6 +152,8 @@ ems_fold_parallel = _sig("ems_fold_parallel", _i, C.c_float, C.c_float, _d, _d, 
 ems_dft = _sig("ems_dft", None, _d, _d, _dp, _u64, _dp, _i, _dp)
 ems_run_until = _sig("ems_run_until", _i, _p, C.POINTER(RunLimitsC), _ip, _i,
                      C.POINTER(RunOutcomeC))
+ems_run_until_batch = _sig("ems_run_until_batch", _i, _p, _i, C.POINTER(_p), C.POINTER(RunLimitsC),
+                          C.POINTER(RunOutcomeC))
 ems_drive_script = _sig("ems_drive_script", _i, _dp, _u64, _dp, _u64, C.POINTER(RunLimitsC),
                         C.POINTER(RunOutcomeC))
 

@@ -401,33 +401,17 @@ __device__ __forceinline__ void h_from(const PlaneIn &P, float uyc, float uzc, b
 }
 
 // ---- fused step kernel ----
+
+// Tile t of the fused step (k tiles fastest, then j tiles, then x chunks):
+// the body shared by the single-model and the batched kernels.
 template <bool OE, int TJ, bool PF>
-__global__ void __launch_bounds__(64 * (TJ + 1))
-    step_fused(const float *__restrict__ A, float *__restrict__ B, size_t N,
-               const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
-               const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
-               int par, int nx, int ny, int nz, int tk, int chunk, int ktiles, int jtiles,
-               int ntiles, int xcd_group) {
-  // XCD-aware order: workgroups go round-robin to the 8 XCDs, so give each
-  // XCD a contiguous range of tiles (k fastest, then j, then x chunk) and
-  // the halo rows its neighbours read stay in its own L2.
-  // Tiles come in rows of ktiles k-neighbours; rows are numbered
-  // R = c*jtiles + jt (j-neighbours adjacent). Groups of `xcd_group`
-  // consecutive rows go round-robin to the 8 XCDs (workgroups are handed to
-  // the XCDs round-robin), so k-neighbours and most j-neighbours share an
-  // L2 while every XCD gets a share of every x chunk. The grid is padded
-  // to whole groups of 8; padding blocks exit at once.
-  const int b = blockIdx.x;
-  int t;
-  if (xcd_group > 0) {
-    const int x8 = b % 8, sl = b / 8;
-    const int lr = sl / ktiles, ktl = sl % ktiles;
-    const int R = ((lr / xcd_group) * 8 + x8) * xcd_group + lr % xcd_group;
-    if (R >= ntiles / ktiles) return;
-    t = R * ktiles + ktl;
-  } else {
-    t = b;
-  }
+__device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *__restrict__ B,
+                                           size_t N, const float *__restrict__ cpool,
+                                           const uint32_t *__restrict__ croff,
+                                           const FSlab *__restrict__ slabs,
+                                           const uint32_t *__restrict__ pmask, int nslab, int par,
+                                           int nx, int ny, int nz, int tk, int chunk, int ktiles,
+                                           int jtiles, int t) {
   const int kt = t % ktiles;
   const int t2 = t / ktiles;
   const int jt = t2 % jtiles;
@@ -540,6 +524,69 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
         load_plane(P, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv);
     }
   }
+}
+
+// Block b -> tile. XCD-aware order: workgroups go round-robin to the 8
+// XCDs, so give each XCD a contiguous range of tiles (k fastest, then j,
+// then x chunk) and the halo rows its neighbours read stay in its own L2.
+// Tiles come in rows of ktiles k-neighbours; rows are numbered
+// R = c*jtiles + jt (j-neighbours adjacent). Groups of `xcd_group`
+// consecutive rows go round-robin to the 8 XCDs, so k-neighbours and most
+// j-neighbours share an L2 while every XCD gets a share of every x chunk.
+// The grid is padded to whole groups of 8; padding blocks get -1.
+__device__ __forceinline__ int tile_of_block(int b, int ktiles, int ntiles, int xcd_group) {
+  if (xcd_group <= 0) return b;
+  const int x8 = b % 8, sl = b / 8;
+  const int lr = sl / ktiles, ktl = sl % ktiles;
+  const int R = ((lr / xcd_group) * 8 + x8) * xcd_group + lr % xcd_group;
+  if (R >= ntiles / ktiles) return -1;
+  return R * ktiles + ktl;
+}
+
+template <bool OE, int TJ, bool PF>
+__global__ void __launch_bounds__(64 * (TJ + 1))
+    step_fused(const float *__restrict__ A, float *__restrict__ B, size_t N,
+               const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
+               const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
+               int par, int nx, int ny, int nz, int tk, int chunk, int ktiles, int jtiles,
+               int ntiles, int xcd_group) {
+  const int t = tile_of_block(blockIdx.x, ktiles, ntiles, xcd_group);
+  if (t < 0) return;
+  fused_tile<OE, TJ, PF>(A, B, N, cpool, croff, slabs, pmask, nslab, par, nx, ny, nz, tk, chunk,
+                         ktiles, jtiles, t);
+}
+
+// One model of a batched step: what step_fused takes as arguments.
+struct FusedArgs {
+  const float *A;
+  float *B;
+  size_t N;
+  const float *cpool;
+  const uint32_t *croff;
+  const FSlab *slabs;
+  const uint32_t *pmask;
+  int nslab, par, nx, ny, nz, tk, chunk, ktiles, jtiles, ntiles;
+};
+
+// Many models of uneven sizes in one launch: seg[m] is the first block of
+// model m (a multiple of 8, so the XCD order holds within each model),
+// seg[nm] the grid size.
+template <bool OE, int TJ, bool PF>
+__global__ void __launch_bounds__(64 * (TJ + 1))
+    step_fused_batch(const FusedArgs *__restrict__ ms, const uint32_t *__restrict__ seg, int nm,
+                     int xcd_group) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((int)seg[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const FusedArgs &a = ms[lo];
+  const int t = tile_of_block(b - (int)seg[lo], a.ktiles, a.ntiles, xcd_group);
+  if (t < 0) return;
+  fused_tile<OE, TJ, PF>(a.A, a.B, a.N, a.cpool, a.croff, a.slabs, a.pmask, a.nslab, a.par, a.nx,
+                         a.ny, a.nz, a.tk, a.chunk, a.ktiles, a.jtiles, t);
 }
 
 // ---- fused step, full-row variant ----
@@ -979,9 +1026,9 @@ __global__ void __launch_bounds__(256) gap_mid(Fields f, const DevOp *ops, int n
 // the step in `fp` (the other buffer), whose u is u^n as pre_e reads it;
 // capture = 0 when the mid gap has already captured it. Segments as in
 // the mid gap; the last one advances the step counter.
-__global__ void __launch_bounds__(256) gap_post_e(Fields f, Fields fp, int capture,
-                                                  const DevOp *ops, int nops, int q0, int q1,
-                                                  int last, uint64_t *step) {
+__device__ __forceinline__ void gap_post_e_body(Fields f, Fields fp, int capture, const DevOp *ops,
+                                                int nops, int q0, int q1, int last,
+                                                uint64_t *step) {
   const uint64_t n = *step;
   const Gaps g = gaps_at(ops, nops, n);
   if (capture) {
@@ -999,6 +1046,24 @@ __global__ void __launch_bounds__(256) gap_post_e(Fields f, Fields fp, int captu
   }
   __syncthreads();
   if (last && threadIdx.x == 0) *step = n + 1;
+}
+
+__global__ void __launch_bounds__(256) gap_post_e(Fields f, Fields fp, int capture,
+                                                  const DevOp *ops, int nops, int q0, int q1,
+                                                  int last, uint64_t *step) {
+  gap_post_e_body(f, fp, capture, ops, nops, q0, q1, last, step);
+}
+
+// The post-E gaps of a batch of fused models, a block per model.
+struct GapArgs {
+  Fields f, fp;
+  const DevOp *ops;
+  int nops;
+  uint64_t *step;
+};
+__global__ void __launch_bounds__(256) gap_post_e_batch(const GapArgs *__restrict__ gs) {
+  const GapArgs &g = gs[blockIdx.x];
+  gap_post_e_body(g.f, g.fp, 1, g.ops, g.nops, 0, g.nops, 1, g.step);
 }
 
 // ---- plane wave, total-field / scattered-field (planewave.rs) ----
@@ -1028,7 +1093,7 @@ __global__ void __launch_bounds__(256) pw_correct(Fields f, DevPwCorr c, const d
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= c.ntgt) return;
   const uint32_t q0 = c.start[t], q1 = c.start[t + 1];
-  float *x = (ESIDE ? f.u : f.i)[c.axis[q0]] + c.idx[q0];
+  float *x = (ESIDE ? f.u : f.i)[c.axis[q0]] + c.idx[t];
   float v = *x;
   for (uint32_t q = q0; q < q1; q++) {
     const double *w = c.w + 4 * (size_t)q;
@@ -2754,6 +2819,260 @@ int ems_gpu_copy_bandwidth(ems_gpu *h, uint64_t bytes, int reps, double *gbs) {
   CK(h, hipFree(a));
   CK(h, hipFree(b));
   *gbs = 2.0 * (double)(n4 * 16) * reps / (ms * 1e-3) / 1e9;
+  return EMS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batches --
+//
+// Many engines stepped together: one fused launch over the tiles of every
+// active model and one launch for all their post-E gaps per step, captured
+// in graphs. Each model computes exactly what its own engine computes (the
+// same tile kernel on its own arrays), so results are bit-identical to
+// stepping the models one by one. No reference counterpart: the reference
+// runs one model per engine.
+
+struct ems_gpu_batch {
+  std::vector<ems_gpu *> m;
+  hipStream_t stream = nullptr;
+  int tj = 7, nchunk = 0, oe = 0;
+  bool prefetch = true;
+  std::vector<int> active;        // the set the graphs were captured for
+  FusedArgs *d_args[2] = {nullptr, nullptr};
+  GapArgs *d_gap[2] = {nullptr, nullptr};
+  uint32_t *d_seg = nullptr;
+  size_t cap = 0;                 // models the device arrays hold
+  int nblocks = 0;
+  hipGraphExec_t g1[2] = {nullptr, nullptr}, gK[2] = {nullptr, nullptr};
+  int K = 16;                     // even
+  std::string err;
+};
+
+namespace {
+
+int bfail(ems_gpu_batch *b, int code, const std::string &msg) {
+  b->err = msg;
+  return code;
+}
+
+#define BCK(b, x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) return bfail((b), EMS_E_DEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+void batch_drop_graphs(ems_gpu_batch *b) {
+  for (int p = 0; p < 2; p++) {
+    if (b->g1[p]) (void)hipGraphExecDestroy(b->g1[p]);
+    if (b->gK[p]) (void)hipGraphExecDestroy(b->gK[p]);
+    b->g1[p] = b->gK[p] = nullptr;
+  }
+  b->active.clear();
+}
+
+// Writes the per-model launch arguments of `act` for both parities.
+int batch_args(ems_gpu_batch *b, const std::vector<int> &act) {
+  const size_t n = act.size();
+  if (n > b->cap) {
+    for (int p = 0; p < 2; p++) {
+      if (b->d_args[p]) BCK(b, hipFree(b->d_args[p]));
+      if (b->d_gap[p]) BCK(b, hipFree(b->d_gap[p]));
+      BCK(b, hipMalloc(&b->d_args[p], n * sizeof(FusedArgs)));
+      BCK(b, hipMalloc(&b->d_gap[p], n * sizeof(GapArgs)));
+    }
+    if (b->d_seg) BCK(b, hipFree(b->d_seg));
+    BCK(b, hipMalloc(&b->d_seg, (n + 1) * sizeof(uint32_t)));
+    b->cap = n;
+  }
+  std::vector<uint32_t> seg(n + 1, 0);
+  for (int p = 0; p < 2; p++) {
+    std::vector<FusedArgs> fa(n);
+    std::vector<GapArgs> ga(n);
+    for (size_t q = 0; q < n; q++) {
+      ems_gpu *h = b->m[act[q]];
+      const FusedGeom g = fused_geom(h);
+      FusedArgs &a = fa[q];
+      a.A = h->fbuf[p], a.B = h->fbuf[1 - p], a.N = h->N;
+      a.cpool = h->cpool, a.croff = h->croff, a.slabs = h->d_fslabs, a.pmask = h->d_pmask;
+      a.nslab = n_fslabs(h), a.par = p, a.nx = h->n[0], a.ny = h->n[1], a.nz = h->n[2];
+      a.tk = g.tk, a.chunk = g.chunk, a.ktiles = g.ktiles, a.jtiles = g.jtiles, a.ntiles = g.ntiles;
+      if (p == 0) seg[q + 1] = seg[q] + (uint32_t)g.nblocks;
+      GapArgs &gg = ga[q];
+      gg.f = h->fb[1 - p], gg.fp = h->fb[p], gg.ops = h->d_ops, gg.nops = (int)h->ops.size();
+      gg.step = h->d_step;
+    }
+    BCK(b, hipMemcpy(b->d_args[p], fa.data(), n * sizeof(FusedArgs), hipMemcpyHostToDevice));
+    BCK(b, hipMemcpy(b->d_gap[p], ga.data(), n * sizeof(GapArgs), hipMemcpyHostToDevice));
+  }
+  BCK(b, hipMemcpy(b->d_seg, seg.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  b->nblocks = (int)seg[n];
+  return 0;
+}
+
+template <bool OE, int TJ, bool PF>
+void batch_launch_tj(ems_gpu_batch *b, int par, int nm) {
+  hipLaunchKernelGGL((step_fused_batch<OE, TJ, PF>), dim3(b->nblocks), dim3(64, TJ + 1), 0,
+                     b->stream, (const FusedArgs *)b->d_args[par], (const uint32_t *)b->d_seg, nm,
+                     2);
+}
+
+// One batched step from parity `par` (every active model holds its state
+// in fbuf[par]).
+int batch_enqueue_step(ems_gpu_batch *b, const std::vector<int> &act, int par) {
+  const int nm = (int)act.size();
+  const int key = b->tj * 2 + (b->prefetch ? 1 : 0);
+#define BL(OE)                                                               \
+  switch (key) {                                                             \
+    case 6: batch_launch_tj<OE, 3, false>(b, par, nm); break;               \
+    case 7: batch_launch_tj<OE, 3, true>(b, par, nm); break;                \
+    case 14: batch_launch_tj<OE, 7, false>(b, par, nm); break;              \
+    case 15: batch_launch_tj<OE, 7, true>(b, par, nm); break;               \
+    case 30: batch_launch_tj<OE, 15, false>(b, par, nm); break;             \
+    case 31: batch_launch_tj<OE, 15, true>(b, par, nm); break;              \
+    default: return bfail(b, EMS_E_INVALID, "unsupported tile height");      \
+  }
+  if (b->oe) {
+    BL(true)
+  } else {
+    BL(false)
+  }
+#undef BL
+  BCK(b, hipGetLastError());
+  // Dispersive media of each model, on the batch stream.
+  for (int q : act) {
+    ems_gpu *h = b->m[q];
+    const hipStream_t own = h->stream;
+    h->stream = b->stream;
+    const int rc = launch_poles(h, 1, h->fb[par ^ 1], h->fb[par], true);
+    h->stream = own;
+    if (rc) return bfail(b, rc, h->err);
+  }
+  hipLaunchKernelGGL(gap_post_e_batch, dim3(nm), dim3(256), 0, b->stream,
+                     (const GapArgs *)b->d_gap[par]);
+  BCK(b, hipGetLastError());
+  return 0;
+}
+
+int batch_capture(ems_gpu_batch *b, const std::vector<int> &act, int steps, int par,
+                  hipGraphExec_t *out) {
+  hipGraph_t g;
+  BCK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int s = 0; s < steps && !rc; s++, par ^= 1) rc = batch_enqueue_step(b, act, par);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) return rc;
+  BCK(b, e);
+  BCK(b, hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+  BCK(b, hipGraphDestroy(g));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ems_gpu_batch_create(ems_gpu *const *models, int n, ems_gpu_batch **out) {
+  if (!models || n < 1 || !out) return EMS_E_INVALID;
+  *out = nullptr;
+  ems_gpu_batch *b = new ems_gpu_batch();
+  for (int q = 0; q < n; q++) {
+    if (!models[q] || models[q]->dead) {
+      delete b;
+      return EMS_E_INVALID;
+    }
+    b->m.push_back(models[q]);
+  }
+  b->oe = models[0]->oe;
+  for (auto *h : b->m)
+    if (h->oe != b->oe) {
+      delete b;
+      return EMS_E_UNSUPPORTED;
+    }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return EMS_E_DEVICE;
+  }
+  *out = b;
+  return EMS_OK;
+}
+
+void ems_gpu_batch_destroy(ems_gpu_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graphs(b);
+  for (int p = 0; p < 2; p++) {
+    if (b->d_args[p]) (void)hipFree(b->d_args[p]);
+    if (b->d_gap[p]) (void)hipFree(b->d_gap[p]);
+  }
+  if (b->d_seg) (void)hipFree(b->d_seg);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char *ems_gpu_batch_last_error(const ems_gpu_batch *b) {
+  return b ? b->err.c_str() : "null batch";
+}
+
+int ems_gpu_batch_set_schedule(ems_gpu_batch *b, int tile_rows, int x_chunks, int prefetch) {
+  if (!b) return EMS_E_INVALID;
+  if (tile_rows != 3 && tile_rows != 7 && tile_rows != 15)
+    return bfail(b, EMS_E_INVALID, "tile_rows must be 3, 7 or 15");
+  if (x_chunks < 0) return bfail(b, EMS_E_INVALID, "x_chunks must be >= 0");
+  b->tj = tile_rows, b->nchunk = x_chunks, b->prefetch = prefetch != 0;
+  batch_drop_graphs(b);
+  return EMS_OK;
+}
+
+int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
+  if (!b) return EMS_E_INVALID;
+  std::vector<int> act;
+  for (size_t q = 0; q < b->m.size(); q++)
+    if (!active || active[q]) act.push_back((int)q);
+  if (act.empty() || n == 0) return EMS_OK;
+  const int par = b->m[act[0]]->par;
+  for (int q : act) {
+    ems_gpu *h = b->m[q];
+    if (h->dead) return bfail(b, EMS_E_DEVICE, "model " + std::to_string(q) + ": " + h->err);
+    h->fused = true, h->rows = false, h->prefetch = b->prefetch, h->tj = b->tj;
+    h->nchunk = b->nchunk;
+    if (!use_fused(h))
+      return bfail(b, EMS_E_UNSUPPORTED,
+                   "model " + std::to_string(q) +
+                       " needs the unfused schedule (H source, magnetic poles, plane wave or "
+                       "more than 32 CPML slabs)");
+    if (h->par != par) return bfail(b, EMS_E_INVALID, "models of a batch step in lockstep");
+    const int rc = prepare(h, n);
+    if (rc) return bfail(b, rc, h->err);
+    BCK(b, hipStreamSynchronize(h->stream));
+  }
+  int rc = batch_args(b, act);
+  if (rc) return rc;
+  if (act != b->active) {
+    batch_drop_graphs(b);
+    for (int p = 0; p < 2; p++) {
+      if ((rc = batch_capture(b, act, 1, p, &b->g1[p]))) return rc;
+      if ((rc = batch_capture(b, act, b->K, p, &b->gK[p]))) return rc;
+    }
+    b->active = act;
+  }
+  int p = par;
+  uint64_t left = n;
+  while (left >= (uint64_t)b->K) {
+    BCK(b, hipGraphLaunch(b->gK[p], b->stream));
+    left -= b->K;
+  }
+  while (left > 0) {
+    BCK(b, hipGraphLaunch(b->g1[p], b->stream));
+    p ^= 1;
+    left--;
+  }
+  BCK(b, hipStreamSynchronize(b->stream));
+  for (int q : act) {
+    ems_gpu *h = b->m[q];
+    h->par = p;
+    advance_counts(h, n);
+  }
   return EMS_OK;
 }
 

@@ -698,6 +698,90 @@ int ems_run_until(ems_gpu *h, const ems_run_limits *limits, const int *probes, i
   return drive(s, *limits, *out);
 }
 
+int ems_run_until_batch(ems_gpu_batch *b, int n, ems_gpu *const *models,
+                        const ems_run_limits *lim, ems_run_outcome *out) {
+  if (!b || n < 1 || !models || !lim || !out) return EMS_E_INVALID;
+  if (lim->has_settle) return EMS_E_UNSUPPORTED;
+  if (lim->has_criteria && !(lim->energy_ratio >= 0.0)) return EMS_E_INVALID;
+  if (lim->has_excitation_end && std::isnan(lim->excitation_end)) return EMS_E_INVALID;
+  const uint64_t interval = lim->check_interval ? lim->check_interval : 50;
+  const uint64_t max = lim->max_timesteps;
+  const auto start = std::chrono::steady_clock::now();
+  struct St {
+    uint64_t steps = 0, met_steps = 0;
+    double w = 0.0, w_max = 0.0, ratio_db = 0.0;
+    bool met = false, done = false;
+    int reason = 1;
+  };
+  std::vector<St> st(n);
+  std::vector<uint8_t> act(n, 1);
+  int rc;
+  for (;;) {
+    // (All running models have taken the same number of steps.)
+    bool any = false;
+    for (int q = 0; q < n; q++) {
+      if (st[q].done) continue;
+      if (st[q].steps >= max) {
+        st[q].done = true, st[q].reason = 1;
+        act[q] = 0;
+        continue;
+      }
+      any = true;
+    }
+    if (!any) break;
+    // The energy step of drive(): W^n of the state before the step.
+    std::vector<uint64_t> t0(n);
+    for (int q = 0; q < n; q++) {
+      if (!act[q]) continue;
+      t0[q] = ems_gpu_timestep(models[q]);
+      if ((rc = ems_gpu_energy(models[q], &st[q].w))) return rc;
+    }
+    if ((rc = ems_gpu_batch_run(b, act.data(), 1))) return rc;
+    uint64_t steps_now = 0;
+    for (int q = 0; q < n; q++) {
+      if (!act[q]) continue;
+      St &s = st[q];
+      s.steps += 1;
+      steps_now = s.steps;
+      s.w_max = std::fmax(s.w_max, s.w);
+      const double ratio = s.w_max > 0.0 ? s.w / s.w_max : 1.0;
+      s.ratio_db = ratio > 0.0 ? 10.0 * std::log10(ratio) : -INFINITY;
+      const double t = (double)t0[q] * ems_gpu_dt(models[q]);
+      if (!std::isfinite(s.w) || s.w < -NEGATIVE_ENERGY * s.w_max) {
+        s.done = true, s.reason = 3, act[q] = 0;
+        continue;
+      }
+      if (lim->has_criteria && (!lim->has_excitation_end || t > lim->excitation_end) &&
+          s.w_max > 0.0 && ratio < lim->energy_ratio) {
+        if (!s.met) s.met = true, s.met_steps = s.steps;
+        s.done = true, s.reason = 0, act[q] = 0;
+      }
+    }
+    bool left = false;
+    for (int q = 0; q < n; q++) left = left || act[q];
+    if (!left) break;
+    const uint64_t chunk = std::min(interval - 1, max - steps_now);
+    if ((rc = ems_gpu_batch_run(b, act.data(), chunk))) return rc;
+    for (int q = 0; q < n; q++)
+      if (act[q]) st[q].steps += chunk;
+  }
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+  for (int q = 0; q < n; q++) {
+    ems_run_outcome &o = out[q];
+    o.reason = st[q].reason;
+    o.ratio_db = st[q].ratio_db;
+    o.steps = st[q].steps;
+    o.timestep = ems_gpu_timestep(models[q]);
+    o.time = (double)o.timestep * ems_gpu_dt(models[q]);
+    o.energy = st[q].w;
+    o.max_energy = st[q].w_max;
+    o.elapsed_s = el;
+    o.has_criterion_steps = st[q].met;
+    o.criterion_steps = st[q].met_steps;
+  }
+  return EMS_OK;
+}
+
 int ems_drive_script(const double *w, uint64_t nw, const double *signal, uint64_t ns,
                      const ems_run_limits *limits, ems_run_outcome *out) {
   if (!w || nw == 0 || !limits || !out) return EMS_E_INVALID;

@@ -965,3 +965,75 @@ class Engine:
         out = (C.c_double * 3)()
         self._check(nat.ems_gpu_traffic_model(self._h, out))
         return tuple(out)
+
+
+class EngineBatch:
+    """Engines stepped together (include/ems_gpu.h batches): one fused launch
+    over the tiles of every active model per step, each model bit-identical
+    to stepping it alone. No reference counterpart (the reference runs one
+    model per engine); the run loop per model is run::drive
+    (src/engine/run.rs:424-540)."""
+
+    def __init__(self, engines: Sequence[Engine]):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e._h for e in self.engines])
+        h = C.c_void_p()
+        rc = nat.ems_gpu_batch_create(arr, len(self.engines), C.byref(h))
+        if rc != 0:
+            raise EmsError(rc, "ems_gpu_batch_create failed (mixed curl orders or a dead engine?)")
+        self._h = h
+        self._arr = arr
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise EmsError(rc, nat.ems_gpu_batch_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            nat.ems_gpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_schedule(self, tile_rows: int = 7, x_chunks: int = 0, prefetch: bool = True) -> None:
+        self._check(nat.ems_gpu_batch_set_schedule(self._h, int(tile_rows), int(x_chunks),
+                                                   1 if prefetch else 0))
+
+    def run(self, n: int, active: Optional[Sequence[bool]] = None) -> None:
+        act = None
+        if active is not None:
+            act = np.ascontiguousarray(np.array([1 if a else 0 for a in active], dtype=np.uint8))
+        for q, e in enumerate(self.engines):
+            if active is None or active[q]:
+                e._ensure_tables(e.timestep() + int(n))
+        self._check(nat.ems_gpu_batch_run(self._h, nat.ptr(act, C.c_uint8), int(n)))
+
+    def run_until(self, limits: "RunLimits") -> List["RunOutcome"]:
+        for e in self.engines:
+            e._ensure_tables(e.timestep() + int(limits.max_timesteps) + 1)
+        outs = (nat.RunOutcomeC * len(self.engines))()
+        lim = limits.c()
+        self._check(nat.ems_run_until_batch(self._h, len(self.engines), self._arr, C.byref(lim),
+                                            outs))
+        return [RunOutcome.from_c(o) for o in outs]
+
+    def autotune(self, candidates=None, steps_each: int = 8):
+        """Times a few real steps of each (tile_rows, x_chunks) on the batch
+        (they advance every model) and keeps the fastest."""
+        import time as _t
+        if candidates is None:
+            candidates = [(t, c) for t in (3, 7) for c in (0, 1, 2, 4)]
+        results = []
+        for t, c in candidates:
+            self.set_schedule(t, c)
+            self.run(2)
+            t0 = _t.perf_counter()
+            self.run(steps_each)
+            results.append((t, c, (_t.perf_counter() - t0) / steps_each * 1e3))
+        best = min(results, key=lambda r: r[2])
+        self.set_schedule(best[0], best[1])
+        return best, results

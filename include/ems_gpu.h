@@ -217,6 +217,24 @@ int ems_gpu_planewave_table(ems_gpu *h, int id, const double *values, uint64_t c
  * PlaneWave::incident_e / incident_h, :1446-1470). */
 int ems_gpu_planewave_aux(ems_gpu *h, int id, double *e, double *hh);
 
+/* Batches (no reference counterpart; for geometry sweeps of many small
+ * models): engines stepped together, one fused launch over the tiles of
+ * every active model and one launch for all their post-E gaps per step,
+ * captured in graphs on the batch's own stream. Every model computes what
+ * its own engine would, bit for bit. The models must all run the fused
+ * schedule, with one curl order, and be in step with each other (same
+ * step parity). The batch holds the handles; it does not own them. */
+typedef struct ems_gpu_batch ems_gpu_batch;
+int ems_gpu_batch_create(ems_gpu *const *models, int n, ems_gpu_batch **out);
+void ems_gpu_batch_destroy(ems_gpu_batch *b);
+const char *ems_gpu_batch_last_error(const ems_gpu_batch *b);
+/* Tile height (3, 7, 15), x chunks per model (0 automatic), next-plane
+ * prefetch on or off, for every model of the batch. */
+int ems_gpu_batch_set_schedule(ems_gpu_batch *b, int tile_rows, int x_chunks, int prefetch);
+/* n steps of the models with active[q] != 0 (NULL: all); returns after
+ * they are done. */
+int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n);
+
 /* Engine::run (src/engine/mod.rs:825-828). */
 int ems_gpu_run(ems_gpu *h, uint64_t n);
 /* Engine::step_with_energy (src/engine/mod.rs:808-823): one step, *w = the

@@ -132,6 +132,14 @@ typedef struct {
 int ems_run_until(ems_gpu *h, const ems_run_limits *limits, const int *probes, int n_probes,
                   ems_run_outcome *out);
 
+/* Engine::run_until for every model of a batch, in lockstep: each model
+ * follows its own run::drive (run.rs:424-540) -- the same energy checks,
+ * stop decisions and step counts as ems_run_until on it alone -- and
+ * leaves the batch when it stops. out[q] per model. Settle is not
+ * supported here. */
+int ems_run_until_batch(ems_gpu_batch *b, int n_models, ems_gpu *const *models,
+                        const ems_run_limits *limits, ems_run_outcome *out);
+
 /* The same loop over a scripted stepper (run.rs:548-582): energy
  * w[min(n, nw-1)] at timestep n, dt = 1, and one probe sampled every step
  * with value signal[k] (k < ns; none if ns = 0). For testing the loop. */

@@ -1,0 +1,74 @@
+"""Batched stepping of uneven models: each model of a batch gives exactly
+what its own engine gives alone (port traces, fields, run-until outcome),
+for the lockstep run loop and for runs of a subset."""
+import numpy as np
+import pytest
+
+import ems_amd as E
+from ems_amd import workloads as W
+
+from test_gpu_parity import bits, same_fields
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(8.0, 20.0, 1.0, 8), (10.0, 24.0, 2.5, 10), (9.0, 22.0, 0.0, 9), (12.0, 30.0, 3.0, 8)]
+
+
+def models():
+    return [W.patch_model(*s) for s in SIZES]
+
+
+def engine(m):
+    e = E.Engine(m.grid, m.dt, m.lanes)
+    p = m.build_ops(e)
+    return e, p
+
+
+def limits():
+    return E.RunLimits(6000, E.EndCriteria.energy_decay_db(-40.0),
+                       excitation_end=E.Waveform.gaussian(2e9, 1e9).duration())
+
+
+@pytest.mark.parametrize("tile_rows", [3, 7])
+def test_batch_run_until_matches_single_runs(tile_rows):
+    ms = models()
+    single = [engine(m) for m in ms]
+    outs = []
+    for e, p in single:
+        e.set_schedule(1, tile_rows, 0)
+        outs.append(e.run_until(limits()))
+    batch = [engine(m) for m in ms]
+    b = E.EngineBatch([e for e, _ in batch])
+    b.set_schedule(tile_rows, 0)
+    bouts = b.run_until(limits())
+    for (e, p), (f, q), o, bo in zip(single, batch, outs, bouts):
+        assert (o.reason, o.steps, o.timestep) == (bo.reason, bo.steps, bo.timestep)
+        assert np.float64(o.energy).tobytes() == np.float64(bo.energy).tobytes()
+        assert np.array_equal(bits(p.voltage()), bits(q.voltage()))
+        assert np.array_equal(bits(p.current()), bits(q.current()))
+        same_fields(e, f)
+    assert len({o.steps for o in bouts}) >= 1 and all(o.reason == E.StopReason.EnergyDecay
+                                                      for o in bouts)
+    b.close()
+
+
+def test_batch_subset_runs():
+    ms = models()
+    single = [engine(m) for m in ms]
+    for e, _ in single:
+        e.run(77)
+    batch = [engine(m) for m in ms]
+    b = E.EngineBatch([e for e, _ in batch])
+    b.run(40)
+    b.run(37)
+    for (e, p), (f, q) in zip(single, batch):
+        same_fields(e, f)
+        assert np.array_equal(bits(p.voltage()), bits(q.voltage()))
+    # Only models 1 and 3 advance; the others stay put.
+    act = [False, True, False, True]
+    b.run(25, act)
+    for q, ((e, p), (f, _)) in enumerate(zip(single, batch)):
+        if act[q]:
+            e.run(25)
+        same_fields(e, f)
+    b.close()
