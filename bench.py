@@ -77,12 +77,85 @@ def cpu_baseline(wl, seconds: float, log) -> dict:
                       f"({dt:.1f} s; host CPU: {cpu_model()})"}
 
 
+def sweep(args, log) -> int:
+    """The patch-antenna sweep: every model solved to -40 dB energy decay
+    and analysed (S11 at 201 points over 1-3 GHz), first one after the other
+    on the GPU (each its own engine), then all together in one batch.
+    value = batched S11 solves per hour; both include set-up and analysis."""
+    import ems_amd as E
+    from ems_amd import workloads as W
+    freqs = [1e9 + 2e9 * q / 200.0 for q in range(201)]
+    lim = E.RunLimits(args.max_steps, E.EndCriteria.energy_decay_db(-40.0),
+                      excitation_end=E.Waveform.gaussian(2e9, 1e9).duration())
+    t0 = time.perf_counter()
+    models = W.patch_sweep(args.models, seed=args.seed)
+    log(f"{len(models)} patch models ({sum(m.cells for m in models)/len(models):.0f} cells on "
+        f"average) in {time.perf_counter()-t0:.1f} s")
+
+    def solve_one(m):
+        e = E.Engine(m.grid, m.dt, m.lanes)
+        p = m.build_ops(e)
+        e.release_dense_lanes()
+        o = e.run_until(lim)
+        s11 = p.analyze(freqs).s11()
+        e.close()
+        return o, s11
+
+    # warm-up: first engine, first graphs
+    solve_one(models[0])
+    t0 = time.perf_counter()
+    seq = [solve_one(m) for m in models]
+    t_seq = time.perf_counter() - t0
+    cell_steps = sum(m.cells * o.steps for m, (o, _) in zip(models, seq))
+    log(f"sequential: {t_seq:.2f} s, {cell_steps / t_seq / 1e9:.2f} Gcell/s")
+
+    t0 = time.perf_counter()
+    engines, ports = [], []
+    for m in models:
+        e = E.Engine(m.grid, m.dt, m.lanes)
+        ports.append(m.build_ops(e))
+        e.release_dense_lanes()
+        engines.append(e)
+    b = E.EngineBatch(engines)
+    b.set_schedule(args.tile_rows or 7, args.x_chunks)
+    outs = b.run_until(lim)
+    s11b = [p.analyze(freqs).s11() for p in ports]
+    t_bat = time.perf_counter() - t0
+    log(f"batched: {t_bat:.2f} s, {cell_steps / t_bat / 1e9:.2f} Gcell/s")
+    d = 0.0
+    same_steps = all(o.steps == so.steps for o, (so, _) in zip(outs, seq))
+    for a, (_, bb) in zip(s11b, seq):
+        d = max(d, max(abs(x - y) for x, y in zip(a, bb)))
+    b.close()
+    for e in engines:
+        e.close()
+    n = len(models)
+    line = {
+        "metric": "S11 solves/hr", "value": round(n / t_bat * 3600.0, 1), "unit": "solves/hr",
+        "n_gpus": 1, "steps": int(sum(o.steps for o in outs)), "warmup": 1,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded patch antennas)",
+        "config": {"workload": f"patch_sweep_{n}", "models": n, "seed": args.seed,
+                   "frequencies": 201, "end": "-40 dB energy decay",
+                   "schedule": {"tile_rows": args.tile_rows or 7, "x_chunks": args.x_chunks}},
+        "sequential_solves_per_hr": round(n / t_seq * 3600.0, 1),
+        "speedup_vs_sequential": round(t_seq / t_bat, 3),
+        "max_abs_dS11": d, "same_step_counts": same_steps,
+        "cell_updates": cell_steps, "batched_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cpml_box", choices=["cpml_box", "vacuum_box"])
+    ap.add_argument("--workload", default="cpml_box", choices=["cpml_box", "vacuum_box", "patch_sweep"])
+    ap.add_argument("--models", type=int, default=256, help="patch_sweep: models")
+    ap.add_argument("--seed", type=int, default=4242, help="patch_sweep: seed")
+    ap.add_argument("--max-steps", type=int, default=40000, help="patch_sweep: step limit")
     ap.add_argument("--n", type=int, default=None, help="cells per axis")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -102,6 +175,9 @@ def main() -> int:
 
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if args.workload == "patch_sweep":
+        return sweep(args, log)
 
     import ems_amd as E
     from ems_amd import workloads as W
