@@ -8,6 +8,11 @@ engine API over both.
 from ._native import EmsError, LIB_PATH  # noqa: F401  (fails loudly if unbuilt)
 from .engine import (  # noqa: F401
     C0, EPS0, MU0, Z0, Axis, CellMaterials, Cpml, CpmlBoundary, EdgeSource, EndCriteria, Engine,
-    EngineBuilder, Face, FaceCondition, Grid, Material, Probe, ProbeQuantity, RunLimits,
+    EngineBatch, EngineBuilder, Face, FaceCondition, Grid, Material, Probe, ProbeQuantity, RunLimits,
     RunOutcome, Settle, SourceKind, StopReason, TimeStep, Waveform, coefficients,
     courant_limit, cpml_coefficients, drive_script, pool_lane)
+from .lumped import (  # noqa: F401,E402
+    EdgeValues, LumpedBox, LumpedElement, LumpedPort, Topology, fold_parallel)
+from .dispersive import Dispersive, Field, Pole, Scheme  # noqa: F401,E402
+from .planewave import Medium, PlaneWave, PlaneWaveSpec  # noqa: F401,E402
+from .post import PortResult, dft, s_param  # noqa: F401,E402
