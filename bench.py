@@ -92,22 +92,34 @@ def sweep(args, log) -> int:
     log(f"{len(models)} patch models ({sum(m.cells for m in models)/len(models):.0f} cells on "
         f"average) in {time.perf_counter()-t0:.1f} s")
 
+    phase = {"setup": 0.0, "run": 0.0, "analysis": 0.0}
+
     def solve_one(m):
+        ta = time.perf_counter()
         e = E.Engine(m.grid, m.dt, m.lanes)
         p = m.build_ops(e)
         e.release_dense_lanes()
+        tb = time.perf_counter()
         o = e.run_until(lim)
+        tc = time.perf_counter()
         s11 = p.analyze(freqs).s11()
         e.close()
+        phase["setup"] += tb - ta
+        phase["run"] += tc - tb
+        phase["analysis"] += time.perf_counter() - tc
         return o, s11
 
     # warm-up: first engine, first graphs
     solve_one(models[0])
+    for k in phase:
+        phase[k] = 0.0
     t0 = time.perf_counter()
     seq = [solve_one(m) for m in models]
     t_seq = time.perf_counter() - t0
     cell_steps = sum(m.cells * o.steps for m, (o, _) in zip(models, seq))
-    log(f"sequential: {t_seq:.2f} s, {cell_steps / t_seq / 1e9:.2f} Gcell/s")
+    log(f"sequential: {t_seq:.2f} s, {cell_steps / t_seq / 1e9:.2f} Gcell/s "
+        f"(setup {phase['setup']:.2f} s, run {phase['run']:.2f} s, analysis {phase['analysis']:.2f} s)")
+    seq_phase = dict(phase)
 
     t0 = time.perf_counter()
     engines, ports = [], []
@@ -118,10 +130,15 @@ def sweep(args, log) -> int:
         engines.append(e)
     b = E.EngineBatch(engines)
     b.set_schedule(args.tile_rows or 7, args.x_chunks)
+    t1 = time.perf_counter()
     outs = b.run_until(lim)
+    t2 = time.perf_counter()
     s11b = [p.analyze(freqs).s11() for p in ports]
     t_bat = time.perf_counter() - t0
-    log(f"batched: {t_bat:.2f} s, {cell_steps / t_bat / 1e9:.2f} Gcell/s")
+    bat_phase = {"setup": t1 - t0, "run": t2 - t1, "analysis": time.perf_counter() - t2}
+    log(f"batched: {t_bat:.2f} s, {cell_steps / t_bat / 1e9:.2f} Gcell/s "
+        f"(setup {bat_phase['setup']:.2f} s, run {bat_phase['run']:.2f} s, "
+        f"analysis {bat_phase['analysis']:.2f} s)")
     d = 0.0
     same_steps = all(o.steps == so.steps for o, (so, _) in zip(outs, seq))
     for a, (_, bb) in zip(s11b, seq):
@@ -141,6 +158,8 @@ def sweep(args, log) -> int:
         "sequential_solves_per_hr": round(n / t_seq * 3600.0, 1),
         "speedup_vs_sequential": round(t_seq / t_bat, 3),
         "max_abs_dS11": d, "same_step_counts": same_steps,
+        "phases_s": {"sequential": {k: round(v, 3) for k, v in seq_phase.items()},
+                     "batched": {k: round(v, 3) for k, v in bat_phase.items()}},
         "cell_updates": cell_steps, "batched_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
     }
     print(json.dumps(line), flush=True)

@@ -147,6 +147,7 @@ ems_cpml_profile = _sig("ems_cpml_profile", _i, _dp, _i, _dp, _i, _dp, _i, _i,
 ems_gaussian = _sig("ems_gaussian", _d, _d, _d, _d)
 ems_gaussian_duration = _sig("ems_gaussian_duration", _d, _d)
 ems_gaussian_table = _sig("ems_gaussian_table", None, _d, _d, _d, _d, _u64, _fp)
+ems_gaussian_values = _sig("ems_gaussian_values", None, _d, _d, _d, _d, _u64, _dp)
 ems_fold_parallel = _sig("ems_fold_parallel", _i, C.c_float, C.c_float, _d, _d, _d,
                          C.POINTER(C.c_float), C.POINTER(C.c_float))
 ems_dft = _sig("ems_dft", None, _d, _d, _dp, _u64, _dp, _i, _dp)

@@ -814,6 +814,7 @@ struct DevOp {
   double *vals;
   uint64_t *count;
   DevLumped *lp;
+  int seg_end[2];  // per gap: the first writer at or after this op (nops: none)
 };
 
 __device__ __forceinline__ float op_amp(const DevOp &o, uint64_t n) {
@@ -958,18 +959,24 @@ __device__ void lumped_post_e(Fields f, const DevLumped &L, uint64_t n) {
 struct Gaps {
   bool gap_b, moved, gap_a;
 };
+// Evaluated by the whole block (every thread must call it): each thread
+// looks at a share of the operators, and the flags are or-ed over the
+// block, so the latency is one operator's, not a chain over all of them.
 __device__ Gaps gaps_at(const DevOp *ops, int nops, uint64_t n) {
-  bool need_post_e = false, mid = false, movable = true;
-  for (int q = 0; q < nops; q++) {
+  bool need_post_e = false, mid = false, fixed = false;
+  for (int q = threadIdx.x; q < nops; q += blockDim.x) {
     const DevOp &o = ops[q];
     if (!needs(o, n)) continue;
     if (is_esrc(o) || is_vprobe(o) || is_lumped(o) || o.type == OP_PW) need_post_e = true;
-    if (is_hsrc(o) || o.type == OP_PW) mid = true, movable = false;
+    if (is_hsrc(o) || o.type == OP_PW) mid = true, fixed = true;
     if (is_iprobe(o) || is_lumped(o)) mid = true;  // lumped pre_e is movable
   }
+  need_post_e = __syncthreads_or(need_post_e);
+  mid = __syncthreads_or(mid);
+  fixed = __syncthreads_or(fixed);
   Gaps g;
-  g.gap_b = mid && !movable;
-  g.moved = mid && movable;
+  g.gap_b = mid && fixed;
+  g.moved = mid && !fixed;
   g.gap_a = need_post_e || g.moved;
   return g;
 }
@@ -998,8 +1005,7 @@ __device__ void run_gap(Fields f, const DevOp *ops, int q0, int nops, uint64_t n
       q++;
       continue;
     }
-    int q1 = q;
-    while (q1 < nops && !gap_writer(ops[q1], which)) q1++;
+    const int q1 = min(o.seg_end[which], nops);
     for (int p = q + (int)threadIdx.x; p < q1; p += (int)blockDim.x) {
       const DevOp &pr = ops[p];
       const bool mine = which == 0 ? is_iprobe(pr) : is_vprobe(pr);
@@ -1932,6 +1938,19 @@ int prepare(ems_gpu *h, uint64_t steps) {
         o.lp.count = o.d_count;
         CK(h, hipMemcpy(o.d_lp, &o.lp, sizeof(DevLumped), hipMemcpyHostToDevice));
         d.lp = o.d_lp;
+      }
+    }
+    // Where each run of readers ends, per gap (run_gap jumps there).
+    const int nop = (int)h->ops.size();
+    for (int w = 0; w < 2; w++) {
+      int next = nop;
+      for (int q = nop - 1; q >= 0; q--) {
+        const HostOp &o = h->ops[q];
+        const bool writer = w == 0 ? (o.type == OP_SOURCE && (o.kind == 2 || o.kind == 3))
+                                   : ((o.type == OP_SOURCE && (o.kind == 0 || o.kind == 1)) ||
+                                      o.type == OP_LUMPED);
+        if (writer) next = q;
+        dv[q].seg_end[w] = next;
       }
     }
     if (h->d_ops) CK(h, hipFree(h->d_ops));

@@ -423,6 +423,11 @@ double ems_gaussian(double f0, double fc, double t) {
 
 double ems_gaussian_duration(double fc) { return 2.0 * gaussian_delay(fc); }
 
+void ems_gaussian_values(double f0, double fc, double dt, double offset, uint64_t count,
+                         double *out) {
+  for (uint64_t n = 0; n < count; n++) out[n] = ems_gaussian(f0, fc, ((double)n + offset) * dt);
+}
+
 void ems_gaussian_table(double f0, double fc, double dt, double offset, uint64_t count,
                         float *amp) {
   for (uint64_t n = 0; n < count; n++)

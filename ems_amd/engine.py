@@ -301,6 +301,12 @@ class Waveform:
     def duration(self) -> float:
         return float(nat.ems_gaussian_duration(self.fc))
 
+    def values(self, dt: float, offset: float, count: int) -> np.ndarray:
+        """value((n + offset) * dt) for n < count, in f64."""
+        out = np.empty(count, dtype=np.float64)
+        nat.ems_gaussian_values(self.f0, self.fc, dt, offset, count, nat.ptr(out, C.c_double))
+        return out
+
     def table(self, dt: float, offset: float, count: int) -> np.ndarray:
         out = np.empty(count, dtype=np.float32)
         nat.ems_gaussian_table(self.f0, self.fc, dt, offset, count, nat.ptr(out, C.c_float))

@@ -551,8 +551,12 @@ class SourceTables:
             old = self.vals[t]
             new = np.empty(want, dtype=np.float64)
             new[:old.size] = old
-            for n in range(old.size, want):
-                new[n] = s.value((n + 0.5) * self.dt, self.dt)
+            if s.fn is None:
+                # amplitude * waveform(t) with t = (n + 1/2)dt, tabulated in C++
+                new[old.size:] = s.amplitude * s.waveform.values(self.dt, 0.5, want)[old.size:]
+            else:
+                for n in range(old.size, want):
+                    new[n] = s.value((n + 0.5) * self.dt, self.dt)
             self.vals[t] = new
             self.len[t] = want
             upload(t, new)

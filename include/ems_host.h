@@ -87,6 +87,9 @@ int ems_cpml_layer(const double *x, int nx, const double *y, int ny, const doubl
 double ems_gaussian(double f0, double fc, double t);
 /* Its duration 2T (waveform.rs:506-513). */
 double ems_gaussian_duration(double fc);
+/* out[n] = gaussian((n + offset)*dt) in f64, n < count. */
+void ems_gaussian_values(double f0, double fc, double dt, double offset, uint64_t count,
+                         double *out);
 /* amp[n] = (gaussian((n + offset)*dt) as f32), n < count. */
 void ems_gaussian_table(double f0, double fc, double dt, double offset, uint64_t count,
                         float *amp);
