@@ -252,8 +252,18 @@ def main() -> int:
     else:
         name, dur, byt = "e_update", te, eb
     achieved = byt / (dur * 1e-3) / 1e9
+    # HBM bytes per launch from the committed rocprof counter passes
+    # (FETCH_SIZE, WRITE_SIZE, scaled by the copy-kernel calibration) of this
+    # workload's dominant kernel, when there are some.
+    traffic, traffic_cfg = None, None
+    tpath = os.path.join(ROOT, "profiles", f"{name}_{wl.name}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        traffic, traffic_cfg = tr["traffic_bytes"], tr["config"]
     roof = {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_measured_at": traffic_cfg,
             "bytes_per_launch": byt, "launch_ms": round(dur, 4),
             "step_ms_events": round(ts, 4), "device_copy_GBs": round(stream_gbs, 1),
             "frac_of_copy": round(achieved / stream_gbs, 4)}
