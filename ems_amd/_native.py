@@ -1,8 +1,9 @@
 """ctypes binding of the in-tree native library ``libems_gpu.so``.
 
-The library holds the HIP engine (include/ems_gpu.h) and the C++ host side
-(include/ems_host.h). There is no fallback: if the library is missing the
-import fails with the command that builds it.
+The library holds the HIP engine (include/ems_gpu.h), the C++ host side
+(include/ems_host.h) and the early-stop network (include/ems_learn.h).
+There is no fallback: if the library is missing the import fails with the
+command that builds it.
 """
 from __future__ import annotations
 
@@ -117,6 +118,18 @@ ems_gpu_batch_destroy = _sig("ems_gpu_batch_destroy", None, _p)
 ems_gpu_batch_last_error = _sig("ems_gpu_batch_last_error", C.c_char_p, _p)
 ems_gpu_batch_set_schedule = _sig("ems_gpu_batch_set_schedule", _i, _p, _i, _i, _i)
 ems_gpu_batch_run = _sig("ems_gpu_batch_run", _i, _p, _u8p, _u64)
+
+# include/ems_learn.h
+ems_cn_create = _sig("ems_cn_create", _i, _i, _i, _i, _i, _ip, _u64, C.POINTER(_p))
+ems_cn_destroy = _sig("ems_cn_destroy", None, _p)
+ems_cn_last_error = _sig("ems_cn_last_error", C.c_char_p, _p)
+ems_cn_param_count = _sig("ems_cn_param_count", _u64, _p)
+ems_cn_set_params = _sig("ems_cn_set_params", _i, _p, _fp, _u64)
+ems_cn_get_params = _sig("ems_cn_get_params", _i, _p, _fp, _u64)
+ems_cn_set_data = _sig("ems_cn_set_data", _i, _p, _fp, _fp, _i, _i)
+ems_cn_train_batch = _sig("ems_cn_train_batch", _i, _p, _ip, _i, _i, C.c_float, _fp)
+ems_cn_forward = _sig("ems_cn_forward", _i, _p, _fp, _i, _i, _fp)
+ems_cn_loss_grad = _sig("ems_cn_loss_grad", _i, _p, _fp, _fp, _i, _i, _i, _fp, _fp)
 ems_gpu_run = _sig("ems_gpu_run", _i, _p, _u64)
 ems_gpu_step_with_energy = _sig("ems_gpu_step_with_energy", _i, _p, _dp)
 ems_gpu_energy = _sig("ems_gpu_energy", _i, _p, _dp)

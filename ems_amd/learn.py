@@ -1,0 +1,462 @@
+"""The learned early stop (include/ems_learn.h).
+
+A causal dilated 1-D convolution network, trained and run on the device by
+the library's own kernels, that maps the first part of a lumped port's
+voltage/current traces (and the known excitation) to the whole trace. An
+early-stopped solve runs the solver for `observed` steps, predicts the
+rest of the horizon decimated by `decimation`, and assembles the port
+spectra from the observed full-rate samples plus the predicted tail. When
+the predicted port signal has not decayed by the end criterion within the
+horizon, the caller falls back to the full run.
+
+No counterpart in the reference: its run loop (src/engine/run.rs:424-540)
+always steps until the end criterion, and its port analysis
+(src/engine/port.rs:209-219) transforms the recorded traces; the tail
+assembly below is that same DFT (src/post/dft.rs:213-222) applied to the
+observed samples and to the predicted decimated samples at their times.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as nat
+from . import post
+from ._native import EmsError
+
+
+class ConvNet:
+    """An ems_cn network: `channels` wide, one layer per dilation (9 taps
+    each), f32 master weights and their bf16 copy on the device."""
+
+    def __init__(self, cin: int, cout: int, channels: int = 64,
+                 dilation: Sequence[int] = (1, 2, 4, 8, 16, 32), seed: int = 0):
+        self.cin, self.cout, self.channels = int(cin), int(cout), int(channels)
+        self.dilation = [int(d) for d in dilation]
+        dil = np.ascontiguousarray(np.array(self.dilation, dtype=np.int32))
+        h = C.c_void_p()
+        rc = nat.ems_cn_create(self.cin, self.cout, self.channels, len(self.dilation),
+                               nat.ptr(dil, C.c_int32), C.c_uint64(seed), C.byref(h))
+        if rc != 0:
+            raise EmsError(rc, "ems_cn_create failed")
+        self._h = h
+        self.param_count = int(nat.ems_cn_param_count(h))
+        self._data_T = 0
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise EmsError(rc, nat.ems_cn_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            nat.ems_cn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.param_count, np.float32)
+        self._check(nat.ems_cn_get_params(self._h, nat.ptr(p, C.c_float), p.size))
+        return p
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(np.asarray(p, dtype=np.float32).reshape(-1))
+        if p.size != self.param_count:
+            raise EmsError(nat.EMS_E_INVALID,
+                           f"{p.size} parameters given, the network has {self.param_count}")
+        self._check(nat.ems_cn_set_params(self._h, nat.ptr(p, C.c_float), p.size))
+
+    def _seq(self, x, ch: int, what: str) -> np.ndarray:
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+        if x.ndim != 3 or x.shape[1] != ch:
+            raise EmsError(nat.EMS_E_INVALID, f"{what}: expected [n][{ch}][T], got {x.shape}")
+        return x
+
+    def set_data(self, x, y) -> None:
+        x = self._seq(x, self.cin, "inputs")
+        y = self._seq(y, self.cout, "targets")
+        if x.shape[0] != y.shape[0] or x.shape[2] != y.shape[2]:
+            raise EmsError(nat.EMS_E_INVALID, f"inputs {x.shape} and targets {y.shape} differ")
+        self._check(nat.ems_cn_set_data(self._h, nat.ptr(x, C.c_float), nat.ptr(y, C.c_float),
+                                        x.shape[0], x.shape[2]))
+        self._data_n, self._data_T = x.shape[0], x.shape[2]
+
+    def train_batch(self, idx, t0: int, lr: float) -> float:
+        idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int32))
+        loss = C.c_float()
+        self._check(nat.ems_cn_train_batch(self._h, nat.ptr(idx, C.c_int32), idx.size, int(t0),
+                                           C.c_float(lr), C.byref(loss)))
+        return loss.value
+
+    def forward(self, x) -> np.ndarray:
+        x = self._seq(x, self.cin, "inputs")
+        out = np.empty((x.shape[0], self.cout, x.shape[2]), np.float32)
+        self._check(nat.ems_cn_forward(self._h, nat.ptr(x, C.c_float), x.shape[0], x.shape[2],
+                                       nat.ptr(out, C.c_float)))
+        return out
+
+    def loss_grad(self, x, y, t0: int) -> Tuple[float, np.ndarray]:
+        x = self._seq(x, self.cin, "inputs")
+        y = self._seq(y, self.cout, "targets")
+        g = np.empty(self.param_count, np.float32)
+        loss = C.c_float()
+        self._check(nat.ems_cn_loss_grad(self._h, nat.ptr(x, C.c_float), nat.ptr(y, C.c_float),
+                                         x.shape[0], x.shape[2], int(t0), C.byref(loss),
+                                         nat.ptr(g, C.c_float)))
+        return loss.value, g
+
+
+@dataclass
+class TraceFormat:
+    """How port traces become network sequences: `horizon` solver steps
+    decimated by `decimation` (sample m is step m*decimation), of which the
+    first `observed` steps are known. The reference resistance scales the
+    current to volts. Inputs per sample: observed V, observed R*I (both
+    times `gain`, zero after the observed part), the observed-part mask and
+    the excitation; outputs: V and R*I."""
+    horizon: int
+    decimation: int
+    observed: int
+    resistance: float = 50.0
+    gain: float = 1.0
+
+    @property
+    def T(self) -> int:
+        return self.horizon // self.decimation
+
+    @property
+    def N(self) -> int:
+        return self.observed // self.decimation
+
+    def check(self) -> None:
+        if self.decimation < 1 or self.horizon % self.decimation or self.observed % self.decimation:
+            raise EmsError(nat.EMS_E_INVALID, "horizon and observed steps must be multiples of "
+                                              "the decimation")
+        if not 0 < self.observed < self.horizon:
+            raise EmsError(nat.EMS_E_INVALID, "observed steps must lie inside the horizon")
+
+    def source(self, waveform, dt: float, amplitude: float = 1.0) -> np.ndarray:
+        """The excitation at the decimated steps (mid-step times)."""
+        D = self.decimation
+        return np.array([amplitude * waveform.value((m * D + 0.5) * dt) for m in range(self.T)],
+                        np.float32)
+
+    def inputs(self, v, i, src) -> np.ndarray:
+        """[n][4][T] network inputs from full-rate traces (at least
+        `observed` samples each) and the decimated excitation."""
+        v = np.atleast_2d(np.asarray(v, np.float64))
+        i = np.atleast_2d(np.asarray(i, np.float64))
+        T, N, D = self.T, self.N, self.decimation
+        x = np.zeros((v.shape[0], 4, T), np.float32)
+        x[:, 0, :N] = v[:, :N * D:D] * self.gain
+        x[:, 1, :N] = i[:, :N * D:D] * self.resistance * self.gain
+        x[:, 2, :N] = 1.0
+        x[:, 3, :] = src[None, :]
+        return x
+
+    def targets(self, v, i) -> np.ndarray:
+        """[n][2][T] decimated V and R*I over the horizon (traces padded
+        with zeros beyond their end)."""
+        v = np.atleast_2d(np.asarray(v, np.float64))
+        i = np.atleast_2d(np.asarray(i, np.float64))
+        T, D = self.T, self.decimation
+        y = np.zeros((v.shape[0], 2, T), np.float32)
+        nv = min(T, (v.shape[1] + D - 1) // D)
+        ni = min(T, (i.shape[1] + D - 1) // D)
+        y[:, 0, :nv] = v[:, :nv * D:D]
+        y[:, 1, :ni] = i[:, :ni * D:D] * self.resistance
+        return y
+
+
+def residual_db(pred: np.ndarray, tail: int = 16) -> np.ndarray:
+    """Port signal power (V^2 + (R I)^2) over the last `tail` samples of
+    the predicted horizon relative to its peak, in dB, per sample."""
+    p = pred[:, 0].astype(np.float64) ** 2 + pred[:, 1].astype(np.float64) ** 2
+    peak = p.max(axis=1)
+    end = p[:, -tail:].mean(axis=1)
+    with np.errstate(divide="ignore"):
+        return 10.0 * np.log10(np.where(peak > 0, end / np.where(peak > 0, peak, 1.0), 1.0))
+
+
+def early_port_result(fmt: TraceFormat, freqs, dt: float, v_obs, i_obs, pred,
+                      stop: Optional[int] = None) -> post.PortResult:
+    """Port spectra from the observed full-rate samples (V at (n+1)dt, I at
+    (n+1/2)dt, n < observed) plus the predicted decimated tail up to sample
+    `stop` (default: the horizon), each decimated sample standing for
+    `decimation` steps."""
+    n0, D = fmt.observed, fmt.decimation
+    stop = fmt.T if stop is None else stop
+    vh = pred[0, fmt.N:stop].astype(np.float64)
+    ih = pred[1, fmt.N:stop].astype(np.float64) / fmt.resistance
+    v = np.asarray(v_obs, np.float64)[:n0]
+    i = np.asarray(i_obs, np.float64)[:n0]
+    vf = np.array(post.dft(dt, dt, v, freqs)) + np.array(post.dft((n0 + 1) * dt, D * dt, vh, freqs))
+    i_f = np.array(post.dft(0.5 * dt, dt, i, freqs)) + \
+        np.array(post.dft((n0 + 0.5) * dt, D * dt, ih, freqs))
+    return post.PortResult(freqs, list(vf), list(i_f), complex(fmt.resistance, 0.0))
+
+
+def truncated_port_result(fmt: TraceFormat, freqs, dt: float, v_obs, i_obs) -> post.PortResult:
+    """The method without the network: the spectra of the observed samples
+    alone."""
+    n0 = fmt.observed
+    v = np.asarray(v_obs, np.float64)[:n0]
+    i = np.asarray(i_obs, np.float64)[:n0]
+    return post.PortResult(freqs, post.dft(dt, dt, v, freqs), post.dft(0.5 * dt, dt, i, freqs),
+                           complex(fmt.resistance, 0.0))
+
+
+# -- solver traces, training and the early-stopped sweep ---------------------
+
+FREQS_1_3_GHZ = [1e9 + 2e9 * q / 200.0 for q in range(201)]
+
+
+def _batch_engines(models):
+    from .engine import Engine
+    engines, ports = [], []
+    for m in models:
+        e = Engine(m.grid, m.dt, m.lanes)
+        ports.append(m.build_ops(e))
+        e.release_dense_lanes()
+        engines.append(e)
+    return engines, ports
+
+
+def solver_traces(models, steps: int, group: int = 256, tile_rows: int = 7, log=None):
+    """Runs the models `steps` steps in batches of `group` (one fused launch
+    per step over each batch) and returns the port voltage and current
+    traces [n][steps] (f32) and the time step (all models share it)."""
+    from .engine import EngineBatch
+    n = len(models)
+    v = np.zeros((n, steps), np.float32)
+    i = np.zeros((n, steps), np.float32)
+    dts = {m.dt for m in models}
+    if len(dts) != 1:
+        raise EmsError(nat.EMS_E_INVALID, "models with different time steps")
+    for g0 in range(0, n, group):
+        part = models[g0:g0 + group]
+        engines, ports = _batch_engines(part)
+        b = EngineBatch(engines)
+        b.set_schedule(tile_rows, 0)
+        b.run(steps)
+        for q, p in enumerate(ports):
+            v[g0 + q] = p.voltage()[:steps]
+            i[g0 + q] = p.current()[:steps]
+        b.close()
+        for e in engines:
+            e.close()
+        if log:
+            log(f"traces {g0 + len(part)}/{n}")
+    return v, i, dts.pop()
+
+
+@dataclass
+class EarlyStop:
+    """A trained network with its trace format and excitation."""
+    net: ConvNet
+    fmt: TraceFormat
+    src: np.ndarray
+    threshold_db: float = -40.0
+
+    def predict(self, v_obs, i_obs) -> np.ndarray:
+        """[n][2][T] predicted V and R*I from [n][>=observed] traces."""
+        return self.net.forward(self.fmt.inputs(v_obs, i_obs, self.src))
+
+    def save(self, path: str) -> None:
+        f = self.fmt
+        np.savez(path, params=self.net.params(), src=self.src,
+                 meta=np.array([f.horizon, f.decimation, f.observed, self.net.cin, self.net.cout,
+                                self.net.channels], np.int64),
+                 dilation=np.array(self.net.dilation, np.int64),
+                 floats=np.array([f.resistance, f.gain, self.threshold_db], np.float64))
+
+    @staticmethod
+    def load(path: str) -> "EarlyStop":
+        z = np.load(path, allow_pickle=False)
+        H, D, obs, cin, cout, ch = (int(q) for q in z["meta"])
+        r, gain, thr = (float(q) for q in z["floats"])
+        net = ConvNet(cin, cout, ch, [int(d) for d in z["dilation"]])
+        net.set_params(z["params"])
+        return EarlyStop(net, TraceFormat(H, D, obs, r, gain), z["src"].astype(np.float32), thr)
+
+
+def train(es: EarlyStop, v, i, steps: int, batch: int = 32, lr: float = 2e-3,
+          lr_end: float = 1e-4, seed: int = 0, log=None, every: int = 500):
+    """Trains es.net on full traces [n][horizon] with a cosine-decayed
+    learning rate; the loss counts the predicted part (t >= N) only.
+    Returns the loss history (one value per step)."""
+    fmt = es.fmt
+    x = fmt.inputs(v, i, es.src)
+    y = fmt.targets(v, i)
+    es.net.set_data(x, y)
+    rng = np.random.default_rng(seed)
+    hist = []
+    n = x.shape[0]
+    for s in range(steps):
+        a = 0.5 * (1.0 + math.cos(math.pi * s / max(steps - 1, 1)))
+        rate = lr_end + (lr - lr_end) * a
+        hist.append(es.net.train_batch(rng.integers(0, n, batch), fmt.N, rate))
+        if log and (s % every == 0 or s == steps - 1):
+            log(f"step {s} loss {np.mean(hist[-every:]):.3e} lr {rate:.2e}")
+    return np.array(hist)
+
+
+@dataclass
+class SweepResult:
+    s11: List[List[complex]]
+    fallback: List[bool]
+    steps: List[int]
+    residual_db: np.ndarray
+
+
+def early_stop_sweep(es: EarlyStop, models, limits, freqs=FREQS_1_3_GHZ, tile_rows: int = 7,
+                     truncated: bool = False) -> SweepResult:
+    """Every model runs `observed` steps in one batch; the network predicts
+    the rest of each trace; models whose predicted port power over the last
+    16 samples of the horizon stays above the threshold (relative to its
+    peak) are run again in full to `limits` (the fallback). With
+    `truncated`, the spectra come from the observed samples alone (the
+    method without the network) and nothing falls back."""
+    from .engine import EngineBatch
+    fmt = es.fmt
+    engines, ports = _batch_engines(models)
+    b = EngineBatch(engines)
+    b.set_schedule(tile_rows, 0)
+    b.run(fmt.observed)
+    v = np.stack([p.voltage()[:fmt.observed] for p in ports])
+    i = np.stack([p.current()[:fmt.observed] for p in ports])
+    dt = models[0].dt
+    b.close()
+    for e in engines:
+        e.close()
+    if truncated:
+        s11 = [truncated_port_result(fmt, freqs, dt, v[q], i[q]).s11() for q in range(len(models))]
+        return SweepResult(s11, [False] * len(models), [fmt.observed] * len(models),
+                           np.zeros(len(models)))
+    pred = es.predict(v, i)
+    rdb = residual_db(pred)
+    fb = [bool(r > es.threshold_db) for r in rdb]
+    s11 = [None] * len(models)
+    steps = [fmt.observed] * len(models)
+    for q in range(len(models)):
+        if not fb[q]:
+            s11[q] = early_port_result(fmt, freqs, dt, v[q], i[q], pred[q]).s11()
+    redo = [q for q in range(len(models)) if fb[q]]
+    if redo:
+        engines, ports = _batch_engines([models[q] for q in redo])
+        b = EngineBatch(engines)
+        b.set_schedule(tile_rows, 0)
+        outs = b.run_until(limits)
+        for k, q in enumerate(redo):
+            s11[q] = ports[k].analyze(freqs).s11()
+            steps[q] = fmt.observed + outs[k].steps
+        b.close()
+        for e in engines:
+            e.close()
+    return SweepResult(s11, fb, steps, rdb)
+
+
+def _s11_err(a, b) -> float:
+    return max(abs(x - y) for x, y in zip(a, b))
+
+
+def held_out_report(es: EarlyStop, v, i, dt: float, freqs=FREQS_1_3_GHZ) -> dict:
+    """|dS11| over 1-3 GHz of the early-stopped spectra (observed part plus
+    prediction) and of the truncated spectra (observed part alone) against
+    the spectra of the whole traces, per held-out trace."""
+    fmt = es.fmt
+    pred = es.predict(v, i)
+    net_err, trunc_err = [], []
+    for q in range(v.shape[0]):
+        full = post.PortResult(freqs, post.dft(dt, dt, v[q].astype(np.float64), freqs),
+                               post.dft(0.5 * dt, dt, i[q].astype(np.float64), freqs),
+                               complex(fmt.resistance, 0.0)).s11()
+        net_err.append(_s11_err(early_port_result(fmt, freqs, dt, v[q], i[q], pred[q]).s11(), full))
+        trunc_err.append(_s11_err(truncated_port_result(fmt, freqs, dt, v[q], i[q]).s11(), full))
+    y = fmt.targets(v, i)
+    rel = np.sqrt(np.mean((pred[:, :, fmt.N:] - y[:, :, fmt.N:]) ** 2) /
+                  np.mean(y[:, :, fmt.N:] ** 2))
+    return {"n": int(v.shape[0]), "trace_rel_rms": float(rel),
+            "net_dS11_max": float(np.max(net_err)), "net_dS11_median": float(np.median(net_err)),
+            "net_dS11_p90": float(np.percentile(net_err, 90)),
+            "trunc_dS11_max": float(np.max(trunc_err)),
+            "trunc_dS11_median": float(np.median(trunc_err)),
+            "residual_db_max": float(residual_db(pred).max()),
+            "net_err": [float(e) for e in net_err], "trunc_err": [float(e) for e in trunc_err]}
+
+
+def main(argv=None) -> int:
+    """Generates solver traces of seeded patch antennas on the device,
+    trains the network on them there, reports held-out accuracy and saves
+    the trained network."""
+    import argparse
+    import json
+    import sys
+    import time
+    from . import workloads as W
+    from .engine import Waveform
+    ap = argparse.ArgumentParser(prog="python -m ems_amd.learn")
+    ap.add_argument("--train", type=int, default=4096)
+    ap.add_argument("--held-out", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--horizon", type=int, default=4096)
+    ap.add_argument("--decimation", type=int, default=8)
+    ap.add_argument("--observed", type=int, default=1024)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--channels", type=int, default=64)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--traces", default="", help="save the traces (npz) here")
+    ap.add_argument("--report", default="", help="write the held-out report (json) here")
+    a = ap.parse_args(argv)
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    models = W.patch_sweep(a.train + a.held_out, seed=a.seed)
+    log(f"{len(models)} models, {sum(m.cells for m in models) / len(models):.0f} cells on average")
+    v, i, dt = solver_traces(models, a.horizon, log=log)
+    t_gen = time.perf_counter() - t0
+    cell_steps = sum(m.cells for m in models) * a.horizon
+    log(f"traces in {t_gen:.1f} s ({cell_steps / t_gen / 1e9:.2f} Gcell/s)")
+    fmt = TraceFormat(a.horizon, a.decimation, a.observed)
+    fmt.check()
+    src = fmt.source(Waveform.gaussian(2e9, 1e9), dt)
+    peak_obs = float(np.abs(src[:fmt.N]).max())
+    fmt.gain = 1.0 / peak_obs if peak_obs > 0 else 1.0
+    es = EarlyStop(ConvNet(4, 2, a.channels, (1, 2, 4, 8, 16, 32), seed=a.seed), fmt, src)
+    tr, ho = slice(0, a.train), slice(a.train, a.train + a.held_out)
+    if a.traces:
+        D = fmt.decimation
+        np.savez(a.traces, v_train=v[tr, ::D], i_train=i[tr, ::D], v_held=v[ho], i_held=i[ho],
+                 dt=dt)
+    t1 = time.perf_counter()
+    hist = train(es, v[tr], i[tr], a.steps, a.batch, a.lr, log=log)
+    t_train = time.perf_counter() - t1
+    log(f"trained {a.steps} steps in {t_train:.1f} s")
+    rep = held_out_report(es, v[ho], i[ho], dt)
+    rep.update({"generate_s": t_gen, "train_s": t_train, "steps": a.steps, "batch": a.batch,
+                "final_loss": float(np.mean(hist[-200:])), "horizon": a.horizon,
+                "observed": a.observed, "decimation": a.decimation, "train": a.train,
+                "seed": a.seed})
+    log(json.dumps({k: v for k, v in rep.items() if not isinstance(v, list)}))
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(rep, f)
+    if a.out:
+        es.save(a.out)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

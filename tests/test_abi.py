@@ -15,7 +15,7 @@ def declared(header):
 
 def test_library_exports_every_declared_symbol():
     lib = C.CDLL(os.path.join(ROOT, "ems_amd", "libems_gpu.so"))
-    names = declared("ems_gpu.h") + declared("ems_host.h")
+    names = declared("ems_gpu.h") + declared("ems_host.h") + declared("ems_learn.h")
     assert len(names) > 20
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
@@ -30,3 +30,7 @@ def test_invalid_arguments_are_rejected_without_a_device():
     d = nat.GridDesc((C.c_int32 * 3)(4, 4, 4), -1.0, 0)
     assert nat.ems_gpu_create(C.byref(d), C.byref(h)) == nat.EMS_E_INVALID
     assert nat.ems_gpu_run(None, 1) == nat.EMS_E_INVALID
+    dil = (C.c_int32 * 2)(1, 2)
+    assert nat.ems_cn_create(0, 2, 8, 2, dil, 0, C.byref(h)) == nat.EMS_E_INVALID
+    assert nat.ems_cn_create(4, 2, 8, 1, dil, 0, C.byref(h)) == nat.EMS_E_INVALID
+    assert nat.ems_cn_forward(None, None, 1, 1, None) == nat.EMS_E_INVALID
