@@ -694,6 +694,19 @@ class Engine:
         except Exception:
             pass
 
+    def patch_lanes(self, changed) -> None:
+        """Replaces whole coefficient lanes ({lane index: dense lane}) on the
+        host copy and the device (coefficient patches such as conformal
+        metal, before operators that read the lanes are added)."""
+        if self._dense_lanes is None:
+            raise EmsError(nat.EMS_E_INVALID, "the host lanes were released")
+        if not self._owns_lanes:  # the caller's array stays as it was
+            self._dense_lanes = self._dense_lanes.copy()
+            self._owns_lanes = True
+        for l, lane in sorted(changed.items()):
+            self._dense_lanes[l] = lane
+            self._upload_lane(l, self._dense_lanes[l], self._pool)
+
     def release_dense_lanes(self) -> None:
         """Drops the host copy of the dense coefficients (after setup)."""
         self._dense_lanes = None

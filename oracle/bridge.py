@@ -111,6 +111,13 @@ class OracleEngine:
             assert orc_set_lane(self._h, l, P(rows, C.c_float), rows.shape[0],
                                 P(index, C.c_uint32)) == 0
 
+    def patch_lanes(self, changed):
+        lanes = self._dense_lanes.copy()
+        for l, lane in sorted(changed.items()):
+            lanes[l] = lane
+            self._set_lane(l, lanes[l])
+        self._dense_lanes = lanes
+
     def _ensure_tables(self, upto):
         for oid, t in self._pw_tables:
             def up_pw(vals, oid=oid):

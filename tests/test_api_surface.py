@@ -8,5 +8,7 @@ def test_public_names():
                  "CpmlBoundary", "Cpml", "EdgeSource", "Probe", "Waveform", "RunLimits",
                  "EndCriteria", "LumpedBox", "LumpedElement", "LumpedPort", "Topology",
                  "EdgeValues", "fold_parallel", "Dispersive", "Pole", "Field", "Scheme",
-                 "PlaneWave", "PlaneWaveSpec", "Medium", "PortResult", "dft", "s_param"):
+                 "PlaneWave", "PlaneWaveSpec", "Medium", "PortResult", "dft", "s_param",
+                 "ConformalPec", "ConformalEdge", "ConformalFace", "ConformalOptions",
+                 "ConvNet", "EarlyStop", "TraceFormat"):
         assert hasattr(E, name), name

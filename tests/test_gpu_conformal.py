@@ -1,0 +1,76 @@
+"""Parity of conformal PEC (coefficient patches) on the device with the
+oracle: a circular cavity and a PEC rod in a CPML box with a dispersive
+sphere, fields, probes and energy bit for bit, fused and unfused."""
+import math
+
+import numpy as np
+import pytest
+
+import ems_amd as E
+from ems_amd import conformal as CF
+
+import cases as K
+from test_gpu_parity import bits, same_fields
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True, params=[(0, 0, 0), (1, 7, 0), (2, 3, 16)],
+                ids=lambda s: "mode%d_tj%d_xc%d" % s)
+def schedule(request):
+    f, t, c = request.param
+    old = dict(K.SCHEDULE)
+    K.SCHEDULE.update(fused=f, tile_rows=t, x_chunks=c)
+    yield
+    K.SCHEDULE.update(old)
+
+
+def test_conformal_cavity():
+    def build(kind):
+        g = E.Grid.uniform([17, 17, 3], [1e-3] * 3)
+        m = E.CellMaterials.vacuum(g)
+        pec = CF.cylinder_z(g, (8.5e-3, 8.5e-3), 7.3e-3, mats=m, metal="outside")
+        e = K.make(kind, g, m, threads=2)
+        rep = pec.apply(e)
+        assert rep.faces > 0 and rep.floored > 0
+        e.add_operator(E.EdgeSource(E.Waveform.gaussian(15e9, 10e9)).edge(E.Axis.Z, [7, 8, 1], 1.0))
+        p = e.add_operator(E.Probe.voltage(E.Axis.Z, [8, 8, 1]))
+        return e, p
+    (g, a), (o, b) = build("gpu"), build("oracle")
+    for n in (1, 300, 1200):
+        g.run(n)
+        o.run(n)
+        wg, wo = g.step_with_energy(), o.step_with_energy()
+        assert np.float64(wg).tobytes() == np.float64(wo).tobytes(), (wg, wo)
+    assert np.array_equal(bits(a.values()), bits(o.probe_values(b)))
+    same_fields(g, o)
+
+
+def test_conformal_rod_in_cpml_box():
+    def build(kind):
+        lines = []
+        for n in (28, 26, 20):
+            x, l = 0.0, []
+            for i in range(n + 1):
+                l.append(x)
+                x += 1e-3 * (1.0 + 0.25 * math.sin(0.7 * i))
+            lines.append(l)
+        g = E.Grid(lines)
+        m = E.CellMaterials.vacuum(g)
+        m.fill_box([(3, 10), (4, 12), (5, 15)], E.Material.dielectric(3.0).with_kappa(0.01))
+        xc, yc = g.lines[0][17] + 0.37e-3, g.lines[1][13] - 0.21e-3
+        pec = CF.cylinder_z(g, (xc, yc), 3.6e-3, mats=m, metal="inside")
+        e = K.make(kind, g, m, threads=2)
+        pec.apply(e)
+        e.add_operator(E.CpmlBoundary.new(e, [(f, E.Cpml.with_cells(5)) for f in E.Face]))
+        e.add_operator(E.EdgeSource(E.Waveform.gaussian(12e9, 8e9)).edge(E.Axis.Z, [8, 8, 10], 1.0))
+        p1 = e.add_operator(E.Probe.voltage(E.Axis.Z, [22, 13, 10]))
+        p2 = e.add_operator(E.Probe.current(E.Axis.Y, [16, 13, 10]))
+        return e, p1, p2
+    (g, a1, a2), (o, b1, b2) = build("gpu"), build("oracle")
+    for n in (1, 200, 250):
+        g.run(n)
+        o.run(n)
+    assert np.array_equal(bits(a1.values()), bits(o.probe_values(b1)))
+    assert np.array_equal(bits(a2.values()), bits(o.probe_values(b2)))
+    same_fields(g, o)
