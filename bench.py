@@ -94,9 +94,13 @@ def sweep(args, log) -> int:
 
     phase = {"setup": 0.0, "run": 0.0, "analysis": 0.0}
 
+    packed = args.kernel == "packed" and all(m.grid.nodes()[2] <= 64 for m in models)
+    tj = args.tile_rows or 7
+
     def solve_one(m):
         ta = time.perf_counter()
         e = E.Engine(m.grid, m.dt, m.lanes)
+        e.set_schedule(4 if packed else 1, tj, 0)
         p = m.build_ops(e)
         e.release_dense_lanes()
         tb = time.perf_counter()
@@ -129,7 +133,7 @@ def sweep(args, log) -> int:
         e.release_dense_lanes()
         engines.append(e)
     b = E.EngineBatch(engines)
-    b.set_schedule(args.tile_rows or 7, args.x_chunks)
+    b.set_schedule(tj, args.x_chunks, packed=packed)
     t1 = time.perf_counter()
     outs = b.run_until(lim)
     t2 = time.perf_counter()
@@ -154,7 +158,8 @@ def sweep(args, log) -> int:
         "dtype": "f32", "data": "synthetic (seeded patch antennas)",
         "config": {"workload": f"patch_sweep_{n}", "models": n, "seed": args.seed,
                    "frequencies": 201, "end": "-40 dB energy decay",
-                   "schedule": {"tile_rows": args.tile_rows or 7, "x_chunks": args.x_chunks}},
+                   "schedule": {"kernel": "packed rows" if packed else f"tiled, {tj} rows",
+                                "x_chunks": args.x_chunks}},
         "sequential_solves_per_hr": round(n / t_seq * 3600.0, 1),
         "speedup_vs_sequential": round(t_seq / t_bat, 3),
         "max_abs_dS11": d, "same_step_counts": same_steps,
@@ -166,15 +171,89 @@ def sweep(args, log) -> int:
     return 0
 
 
+def earlystop(args, log) -> int:
+    """The learned early stop on a fresh patch sweep: every model solved in
+    full to -40 dB energy decay (one batch), then early-stopped (one batch
+    of `observed` steps, the trained network predicting the rest of each
+    port trace, full runs for the models it flags); S11 at 201 points over
+    1-3 GHz compared with the full runs, and with the observed samples
+    alone (the method without the network). value = early-stopped S11
+    solves per hour, set-up and analysis included."""
+    import numpy as np
+    import ems_amd as E
+    from ems_amd import learn as L
+    from ems_amd import workloads as W
+    path = args.model or os.path.join(ROOT, "ems_amd", "earlystop_patch.npz")
+    es = L.EarlyStop.load(path)
+    freqs = L.FREQS_1_3_GHZ
+    lim = E.RunLimits(args.max_steps, E.EndCriteria.energy_decay_db(-40.0),
+                      excitation_end=E.Waveform.gaussian(2e9, 1e9).duration())
+    models = W.patch_sweep(args.models, seed=args.seed)
+    log(f"{len(models)} patch models, network {path} (horizon {es.fmt.horizon}, observed "
+        f"{es.fmt.observed}, decimation {es.fmt.decimation})")
+    # warm-up: one small batch through both paths
+    L.early_stop_sweep(es, models[:4], lim)
+
+    t0 = time.perf_counter()
+    engines, ports = L._batch_engines(models)
+    b = E.EngineBatch(engines)
+    L._schedule(b, models, args.tile_rows or 7)
+    outs = b.run_until(lim)
+    full = [p.analyze(freqs).s11() for p in ports]
+    b.close()
+    for e in engines:
+        e.close()
+    t_full = time.perf_counter() - t0
+    full_steps = [o.steps for o in outs]
+    log(f"full runs: {t_full:.2f} s, {np.mean(full_steps):.0f} steps on average")
+
+    t0 = time.perf_counter()
+    res = L.early_stop_sweep(es, models, lim, freqs, tile_rows=args.tile_rows or 7)
+    t_es = time.perf_counter() - t0
+    log(f"early-stopped: {t_es:.2f} s, {sum(res.fallback)} fallbacks")
+    err = [max(abs(x - y) for x, y in zip(a, f)) for a, f in zip(res.s11, full)]
+    tr = L.early_stop_sweep(es, models, lim, freqs, tile_rows=args.tile_rows or 7, truncated=True)
+    terr = [max(abs(x - y) for x, y in zip(a, f)) for a, f in zip(tr.s11, full)]
+    n = len(models)
+    line = {
+        "metric": "S11 solves/hr (learned early stop)", "value": round(n / t_es * 3600.0, 1),
+        "unit": "solves/hr", "n_gpus": 1, "steps": int(sum(res.steps)), "warmup": 1,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (network: bf16 weights, f32 accumulate)",
+        "data": "synthetic (seeded patch antennas)",
+        "config": {"workload": f"patch_earlystop_{n}", "models": n, "seed": args.seed,
+                   "frequencies": 201, "observed_steps": es.fmt.observed,
+                   "horizon": es.fmt.horizon, "decimation": es.fmt.decimation,
+                   "fallback": f"predicted port power over the last 16 samples > "
+                               f"{es.threshold_db:.0f} dB of its peak"},
+        "full_solves_per_hr": round(n / t_full * 3600.0, 1),
+        "speedup_vs_full_batched": round(t_full / t_es, 3),
+        "mean_full_steps": float(np.mean(full_steps)),
+        "fallbacks": int(sum(res.fallback)),
+        "max_abs_dS11": float(np.max(err)), "median_abs_dS11": float(np.median(err)),
+        "p90_abs_dS11": float(np.percentile(err, 90)),
+        "models_within_0.01": int(sum(e <= 0.01 for e in err)),
+        "truncated_max_abs_dS11": float(np.max(terr)),
+        "truncated_median_abs_dS11": float(np.median(terr)),
+        "full_s": round(t_full, 3), "early_s": round(t_es, 3),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cpml_box", choices=["cpml_box", "vacuum_box", "patch_sweep"])
+    ap.add_argument("--workload", default="cpml_box", choices=["cpml_box", "vacuum_box", "patch_sweep",
+                                                             "patch_earlystop"])
     ap.add_argument("--models", type=int, default=256, help="patch_sweep: models")
     ap.add_argument("--seed", type=int, default=4242, help="patch_sweep: seed")
     ap.add_argument("--max-steps", type=int, default=40000, help="patch_sweep: step limit")
+    ap.add_argument("--model", default="", help="patch_earlystop: trained network (npz)")
+    ap.add_argument("--kernel", default="packed", choices=["packed", "tiled"],
+                    help="patch workloads: fused kernel (packed short rows where nz <= 64)")
     ap.add_argument("--n", type=int, default=None, help="cells per axis")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -197,6 +276,8 @@ def main() -> int:
 
     if args.workload == "patch_sweep":
         return sweep(args, log)
+    if args.workload == "patch_earlystop":
+        return earlystop(args, log)
 
     import ems_amd as E
     from ems_amd import workloads as W

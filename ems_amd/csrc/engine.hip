@@ -589,6 +589,208 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
                          a.ny, a.nz, a.tk, a.chunk, a.ktiles, a.jtiles, t);
 }
 
+// ---- fused step, packed short rows ----
+//
+// For grids with nz <= 64: R = 64 / nz whole k-rows per wave (lane =
+// jr*nz + k), so short rows fill the wave instead of leaving most lanes
+// idle. A block of NW waves covers NW*R consecutive j rows, the first of
+// them the halo row j0-1 whose H the E update of row j0 needs (recomputed
+// read-only, as in the tiled kernel); rows are whole, so there is no k
+// halo. Rows of one x-plane are contiguous, so a wave's fields are one
+// contiguous run at a wave-uniform base. Same expressions and inputs as
+// the other schedules, so the same bits.
+
+// load_plane for a lane whose row differs from its wave's: fields at the
+// wave base `ob` plus the lane's byte offset kf, coefficients through the
+// lane's own row table entry (r) at byte offset kc within the row.
+__device__ __forceinline__ void load_plane_lane(PlaneIn &P, const float *F, size_t N,
+                                                const float *cpool, const uint32_t *croff,
+                                                size_t ob, size_t r, size_t plane, int nz,
+                                                unsigned kf, unsigned kc, bool iv, bool jv,
+                                                bool kv) {
+  asm volatile("" : "+v"(kf));
+  const unsigned kn = kf + (kv ? 4u : 0u);
+  const unsigned kj = kf + (jv ? 4u * (unsigned)nz : 0u);
+  const unsigned ki = kf + (iv ? 4u * (unsigned)plane : 0u);
+  const float *ux = F + ob, *uy = F + N + ob, *uz = F + 2 * N + ob;
+  const float uxc = ldf(ux, kf), uxj = ldf(ux, kj), uzj = ldf(uz, kj);
+  const float uxk = ldf(ux, kn), uyk = ldf(uy, kn);
+  const float uyi = ldf(uy, ki), uzi = ldf(uz, ki);
+  P.ix = ldf(F + 3 * N + ob, kf);
+  P.iy = ldf(F + 4 * N + ob, kf);
+  P.iz = ldf(F + 5 * N + ob, kf);
+  const uint32_t *ro = croff + 16 * r;
+  const uint32_t cflag = ro[12];
+#pragma unroll
+  for (int l = 0; l < 12; l++) {
+    // a constant row holds one value: read its first entry (same bits)
+    const unsigned off = ro[l] + (((cflag >> l) & 1u) ? 0u : kc);
+    P.c[l] = ldf(cpool, off);
+  }
+  P.uxc = uxc;
+  P.ux_j1 = jv ? uxj : 0.0f;
+  P.uz_j1 = jv ? uzj : 0.0f;
+  P.ux_k1 = kv ? uxk : 0.0f;
+  P.uy_k1 = kv ? uyk : 0.0f;
+  P.uyn = iv ? uyi : 0.0f;
+  P.uzn = iv ? uzi : 0.0f;
+}
+
+template <bool OE, int NW, bool PF>
+__device__ __forceinline__ void fused_packed(const float *__restrict__ A, float *__restrict__ B,
+                                             size_t N, const float *__restrict__ cpool,
+                                             const uint32_t *__restrict__ croff,
+                                             const FSlab *__restrict__ slabs,
+                                             const uint32_t *__restrict__ pmask, int nslab,
+                                             int par, int nx, int ny, int nz, int R, int chunk,
+                                             int jtiles, int t) {
+  const int jt = t % jtiles, c = t / jtiles;
+  const int lane = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int owned = NW * R - 1;
+  const int j0 = jt * owned;
+  const int jr = lane / nz, k = lane - jr * nz;
+  const bool lane_ok = jr < R;
+  const int rb = w * R + jr;  // row within the block, 0 = the halo row
+  const int j = j0 - 1 + rb;
+  const bool inb = lane_ok && j >= 0 && j < ny;
+  const bool own = inb && rb > 0;
+  // The wave's first row (clamped) is the base of every field address; a
+  // lane's row is clamped into [that row, ny - 1], so every load is valid.
+  const int jw = min(max(j0 - 1 + w * R, 0), ny - 1);
+  const int jc = min(max(j, jw), ny - 1);
+  const int kk = lane_ok ? k : 0;
+  const unsigned kf_ = (unsigned)(((jc - jw) * nz + kk) * 4);
+  const unsigned kc = (unsigned)kk * 4u;
+  const int i0 = c * chunk, i1 = min(nx, i0 + chunk);
+  const size_t plane = (size_t)ny * nz;
+  const bool jv = jc < ny - 1, kv = kk < nz - 1;
+  // CPML slabs whose j range meets the block's rows (k is checked per lane).
+  uint32_t mh = 0, me = 0;
+  for (int q = 0; q < nslab; q++) {
+    const FSlab &s = slabs[q];
+    if (s.lo1 < j0 - 1 + NW * R && s.hi1 > j0 - 1) {
+      if (s.e_side) me |= 1u << q;
+      else mh |= 1u << q;
+    }
+  }
+  __shared__ float sh[2][3][NW][64];
+
+  float uyc, uzc, hy_prev = 0.0f, hz_prev = 0.0f;
+  PlaneIn P;
+  {
+    const size_t obw = ((size_t)i0 * ny + jw) * nz;
+    const size_t r = (size_t)i0 * ny + jc;
+    const unsigned kf = kf_;
+    uyc = ldf(A + N + obw, kf);
+    uzc = ldf(A + 2 * N + obw, kf);
+    if (i0 > 0) {
+      // H of plane i0-1 at this node (read-only: its owner writes it).
+      PlaneIn Q;
+      load_plane_lane(Q, A, N, cpool, croff, obw - plane, r - ny, plane, nz, kf, kc, true, jv,
+                      kv);
+      const float uyp = ldf(A + N + (obw - plane), kf), uzp = ldf(A + 2 * N + (obw - plane), kf);
+      if (inb) {
+        float h[3];
+        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh & pmask[2 * (i0 - 1)], par, false, i0 - 1,
+                   j, k, h);
+        hy_prev = h[1];
+        hz_prev = h[2];
+      }
+    }
+    load_plane_lane(P, A, N, cpool, croff, obw, r, plane, nz, kf, kc, i0 + 1 < nx, jv, kv);
+  }
+  for (int i = i0; i < i1; i++) {
+    const int buf = i & 1;
+    const size_t r = (size_t)i * ny + jc;
+    const size_t obw = ((size_t)i * ny + jw) * nz;
+    const bool iv = i + 1 < nx;
+    unsigned kf = kf_;
+    asm volatile("" : "+v"(kf));
+    PlaneIn Nx;
+    if (PF && i + 1 < i1)
+      load_plane_lane(Nx, A, N, cpool, croff, obw + plane, r + ny, plane, nz, kf, kc, i + 2 < nx,
+                      jv, kv);
+    const uint32_t mhp = mh & pmask[2 * i], mep = me & pmask[2 * i + 1];
+    float h[3] = {0.0f, 0.0f, 0.0f};
+    if (inb) h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mhp, par, own, i, j, k, h);
+    sh[buf][0][w][lane] = h[0];
+    sh[buf][1][w][lane] = h[1];
+    sh[buf][2][w][lane] = h[2];
+    __syncthreads();
+    if (own) {
+      // k-1: the previous lane of the same row; j-1: the row before, in
+      // this wave or at the end of the previous wave's rows.
+      const int wp = jr > 0 ? w : w - 1;
+      const int lp = jr > 0 ? lane - nz : (R - 1) * nz + k;
+      const float hx_km = k > 0 ? sh[buf][0][w][lane - 1] : 0.0f;
+      const float hy_km = k > 0 ? sh[buf][1][w][lane - 1] : 0.0f;
+      const float hx_jm = j > 0 ? sh[buf][0][wp][lp] : 0.0f;
+      const float hz_jm = j > 0 ? sh[buf][2][wp][lp] : 0.0f;
+      float e0 = P.uxc, e1 = uyc, e2 = uzc;
+      // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
+      if (iv) e0 = e_expr<OE>(P.c[0], e0, P.c[3], h[2], hz_jm, h[1], hy_km);
+      // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
+      if (jv) e1 = e_expr<OE>(P.c[1], e1, P.c[4], h[0], hx_km, h[2], hz_prev);
+      // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
+      if (kv) e2 = e_expr<OE>(P.c[2], e2, P.c[5], h[1], hy_prev, h[0], hx_jm);
+      if (mep)
+        cpml_node<true>(slabs, mep, par, true, i, j, k, h[2] - hz_jm, h[1] - hy_km,
+                        h[2] - hz_prev, h[0] - hx_km, h[1] - hy_prev, h[0] - hx_jm, P.c[3],
+                        P.c[4], P.c[5], e0, e1, e2);
+      stf(B + obw, kf, e0);
+      stf(B + N + obw, kf, e1);
+      stf(B + 2 * N + obw, kf, e2);
+      stf(B + 3 * N + obw, kf, h[0]);
+      stf(B + 4 * N + obw, kf, h[1]);
+      stf(B + 5 * N + obw, kf, h[2]);
+    }
+    hy_prev = h[1];
+    hz_prev = h[2];
+    uyc = P.uyn;
+    uzc = P.uzn;
+    if (i + 1 < i1) {
+      if (PF)
+        P = Nx;
+      else
+        load_plane_lane(P, A, N, cpool, croff, obw + plane, r + ny, plane, nz, kf, kc, i + 2 < nx,
+                        jv, kv);
+    }
+  }
+}
+
+template <bool OE, int NW, bool PF>
+__global__ void __launch_bounds__(64 * NW)
+    step_packed(const float *__restrict__ A, float *__restrict__ B, size_t N,
+                const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
+                const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
+                int par, int nx, int ny, int nz, int R, int chunk, int jtiles, int ntiles,
+                int xcd_group) {
+  const int t = tile_of_block(blockIdx.x, 1, ntiles, xcd_group);
+  if (t < 0) return;
+  fused_packed<OE, NW, PF>(A, B, N, cpool, croff, slabs, pmask, nslab, par, nx, ny, nz, R, chunk,
+                           jtiles, t);
+}
+
+// The batched form: FusedArgs::tk holds R (rows per wave), ktiles is 1.
+template <bool OE, int NW, bool PF>
+__global__ void __launch_bounds__(64 * NW)
+    step_packed_batch(const FusedArgs *__restrict__ ms, const uint32_t *__restrict__ seg, int nm,
+                      int xcd_group) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((int)seg[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const FusedArgs &a = ms[lo];
+  const int t = tile_of_block(b - (int)seg[lo], 1, a.ntiles, xcd_group);
+  if (t < 0) return;
+  fused_packed<OE, NW, PF>(a.A, a.B, a.N, a.cpool, a.croff, a.slabs, a.pmask, a.nslab, a.par, a.nx,
+                           a.ny, a.nz, a.tk, a.chunk, a.jtiles, t);
+}
+
 // ---- fused step, full-row variant ----
 //
 // One wave per j-row covering all k in NSEG segments of 64 lanes (k = 64m +
@@ -1387,7 +1589,7 @@ struct ems_gpu {
   uint64_t step = 0;
   uint64_t *d_step = nullptr;
   // fused schedule and its tiling
-  bool fused = true, prefetch = true, rows = false;
+  bool fused = true, prefetch = true, rows = false, packed = false;
   int xcd_group = 2;  // tile rows per XCD visit (2 measured best of 0, 1, 2, 4, 8)
   int tj = 7, nchunk = 0;
   // energy scratch
@@ -1677,7 +1879,54 @@ int launch_rows(ems_gpu *h, int par) {
   return 0;
 }
 
+// Packed short rows (nz <= 64): R = 64 / nz rows per wave, PACK_NW waves
+// per block, the first row of a block its halo.
+constexpr int PACK_NW = 4;
+
+bool packed_ok(const ems_gpu *h) { return h->n[2] <= 64; }
+
+FusedGeom packed_geom(const ems_gpu *h, int nchunk) {
+  FusedGeom g;
+  const int nz = h->n[2], ny = h->n[1], nx = h->n[0];
+  g.tk = 64 / nz;  // rows per wave
+  g.ktiles = 1;
+  const int owned = PACK_NW * g.tk - 1;
+  g.jtiles = (ny + owned - 1) / owned;
+  int want = nchunk;
+  if (want <= 0) {
+    want = (2048 + g.jtiles - 1) / g.jtiles;
+    want = std::max(1, std::min(want, std::max(1, nx / 16)));
+  }
+  want = std::min(want, nx);
+  g.chunk = (nx + want - 1) / want;
+  g.nchunks = (nx + g.chunk - 1) / g.chunk;
+  g.ntiles = g.jtiles * g.nchunks;
+  const int G = h->xcd_group;
+  g.nblocks = G > 0 ? (g.ntiles + 8 * G - 1) / (8 * G) * (8 * G) : g.ntiles;
+  return g;
+}
+
+template <bool OE, bool PF>
+void launch_packed_t(ems_gpu *h, int par, const FusedGeom &g) {
+  hipLaunchKernelGGL((step_packed<OE, PACK_NW, PF>), dim3(g.nblocks), dim3(64, PACK_NW), 0,
+                     h->stream, h->fbuf[par], h->fbuf[1 - par], h->N, (const float *)h->cpool,
+                     (const uint32_t *)h->croff, h->d_fslabs, (const uint32_t *)h->d_pmask,
+                     n_fslabs(h), par, h->n[0], h->n[1], h->n[2], g.tk, g.chunk, g.jtiles,
+                     g.ntiles, h->xcd_group);
+}
+
+int launch_packed(ems_gpu *h, int par) {
+  const FusedGeom g = packed_geom(h, h->nchunk);
+  if (h->oe)
+    h->prefetch ? launch_packed_t<true, true>(h, par, g) : launch_packed_t<true, false>(h, par, g);
+  else
+    h->prefetch ? launch_packed_t<false, true>(h, par, g) : launch_packed_t<false, false>(h, par, g);
+  CK(h, hipGetLastError());
+  return 0;
+}
+
 int launch_fused(ems_gpu *h, int par) {
+  if (h->packed && packed_ok(h)) return launch_packed(h, par);
   if (h->rows && rows_ok(h)) return launch_rows(h, par);
   const FusedGeom g = fused_geom(h);
   const int key = h->tj * 2 + (h->prefetch ? 1 : 0);
@@ -2588,9 +2837,13 @@ int ems_gpu_planewave_aux(ems_gpu *h, int id, double *e, double *hh) {
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks) {
   ALIVE(h);
   // fused 2: the fused kernel without the next-plane prefetch; 3: the
-  // full-row fused kernel (nz <= 576, tile rows 3 or 7).
-  h->prefetch = fused != 2;
+  // full-row fused kernel (nz <= 576, tile rows 3 or 7); 4: packed short
+  // rows (nz <= 64; otherwise the tiled kernel), 5: packed without the
+  // prefetch.
+  if (fused < 0 || fused > 5) return fail(h, EMS_E_INVALID, "schedule mode must be 0-5");
+  h->prefetch = fused != 2 && fused != 5;
   h->rows = fused == 3;
+  h->packed = fused == 4 || fused == 5;
   if (tile_rows != 0 && tile_rows != 3 && tile_rows != 7 && tile_rows != 15)
     return fail(h, EMS_E_INVALID, "tile_rows must be 3, 7 or 15");
   if (x_chunks < 0) return fail(h, EMS_E_INVALID, "x_chunks must be >= 0");
@@ -2856,7 +3109,7 @@ struct ems_gpu_batch {
   std::vector<ems_gpu *> m;
   hipStream_t stream = nullptr;
   int tj = 7, nchunk = 0, oe = 0;
-  bool prefetch = true;
+  bool prefetch = true, packed = false;
   std::vector<int> active;        // the set the graphs were captured for
   FusedArgs *d_args[2] = {nullptr, nullptr};
   GapArgs *d_gap[2] = {nullptr, nullptr};
@@ -2910,7 +3163,7 @@ int batch_args(ems_gpu_batch *b, const std::vector<int> &act) {
     std::vector<GapArgs> ga(n);
     for (size_t q = 0; q < n; q++) {
       ems_gpu *h = b->m[act[q]];
-      const FusedGeom g = fused_geom(h);
+      const FusedGeom g = b->packed ? packed_geom(h, b->nchunk) : fused_geom(h);
       FusedArgs &a = fa[q];
       a.A = h->fbuf[p], a.B = h->fbuf[1 - p], a.N = h->N;
       a.cpool = h->cpool, a.croff = h->croff, a.slabs = h->d_fslabs, a.pmask = h->d_pmask;
@@ -2929,6 +3182,13 @@ int batch_args(ems_gpu_batch *b, const std::vector<int> &act) {
   return 0;
 }
 
+template <bool OE, bool PF>
+void batch_launch_packed(ems_gpu_batch *b, int par, int nm) {
+  hipLaunchKernelGGL((step_packed_batch<OE, PACK_NW, PF>), dim3(b->nblocks), dim3(64, PACK_NW), 0,
+                     b->stream, (const FusedArgs *)b->d_args[par], (const uint32_t *)b->d_seg, nm,
+                     2);
+}
+
 template <bool OE, int TJ, bool PF>
 void batch_launch_tj(ems_gpu_batch *b, int par, int nm) {
   hipLaunchKernelGGL((step_fused_batch<OE, TJ, PF>), dim3(b->nblocks), dim3(64, TJ + 1), 0,
@@ -2940,9 +3200,11 @@ void batch_launch_tj(ems_gpu_batch *b, int par, int nm) {
 // in fbuf[par]).
 int batch_enqueue_step(ems_gpu_batch *b, const std::vector<int> &act, int par) {
   const int nm = (int)act.size();
-  const int key = b->tj * 2 + (b->prefetch ? 1 : 0);
+  const int key = b->packed ? (b->prefetch ? 1 : 0) : b->tj * 2 + (b->prefetch ? 1 : 0);
 #define BL(OE)                                                               \
   switch (key) {                                                             \
+    case 0: batch_launch_packed<OE, false>(b, par, nm); break;              \
+    case 1: batch_launch_packed<OE, true>(b, par, nm); break;               \
     case 6: batch_launch_tj<OE, 3, false>(b, par, nm); break;               \
     case 7: batch_launch_tj<OE, 3, true>(b, par, nm); break;                \
     case 14: batch_launch_tj<OE, 7, false>(b, par, nm); break;              \
@@ -3033,12 +3295,19 @@ const char *ems_gpu_batch_last_error(const ems_gpu_batch *b) {
   return b ? b->err.c_str() : "null batch";
 }
 
-int ems_gpu_batch_set_schedule(ems_gpu_batch *b, int tile_rows, int x_chunks, int prefetch) {
+int ems_gpu_batch_set_schedule(ems_gpu_batch *b, int mode, int tile_rows, int x_chunks) {
   if (!b) return EMS_E_INVALID;
-  if (tile_rows != 3 && tile_rows != 7 && tile_rows != 15)
+  if (mode != 1 && mode != 2 && mode != 4 && mode != 5)
+    return bfail(b, EMS_E_INVALID, "batch schedule mode must be 1, 2, 4 or 5");
+  if (mode <= 2 && tile_rows != 3 && tile_rows != 7 && tile_rows != 15)
     return bfail(b, EMS_E_INVALID, "tile_rows must be 3, 7 or 15");
   if (x_chunks < 0) return bfail(b, EMS_E_INVALID, "x_chunks must be >= 0");
-  b->tj = tile_rows, b->nchunk = x_chunks, b->prefetch = prefetch != 0;
+  if (mode >= 4)
+    for (auto *h : b->m)
+      if (!packed_ok(h)) return bfail(b, EMS_E_INVALID, "packed rows need nz <= 64 in every model");
+  b->packed = mode >= 4, b->prefetch = mode == 1 || mode == 4;
+  if (mode <= 2) b->tj = tile_rows;
+  b->nchunk = x_chunks;
   batch_drop_graphs(b);
   return EMS_OK;
 }
@@ -3054,6 +3323,7 @@ int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
     ems_gpu *h = b->m[q];
     if (h->dead) return bfail(b, EMS_E_DEVICE, "model " + std::to_string(q) + ": " + h->err);
     h->fused = true, h->rows = false, h->prefetch = b->prefetch, h->tj = b->tj;
+    h->packed = b->packed;
     h->nchunk = b->nchunk;
     if (!use_fused(h))
       return bfail(b, EMS_E_UNSUPPORTED,

@@ -949,9 +949,10 @@ class Engine:
 
     def set_schedule(self, fused=True, tile_rows: int = 0, x_chunks: int = 0) -> None:
         """Fused (one pass per step, double-buffered fields; 2 = without the
-        next-plane prefetch) or unfused (H, CPML-H, E, CPML-E launches)
-        steps; the fused tile height (3, 7 or 15 rows) and the number of x
-        chunks (0 = automatic)."""
+        next-plane prefetch; 3 full rows; 4 packed short rows, nz <= 64,
+        5 the same without the prefetch) or unfused (H, CPML-H, E, CPML-E
+        launches) steps; the fused tile height (3, 7 or 15 rows) and the
+        number of x chunks (0 = automatic)."""
         mode = int(fused) if not isinstance(fused, bool) else (1 if fused else 0)
         self._check(nat.ems_gpu_set_schedule(self._h, mode, int(tile_rows), int(x_chunks)))
 
@@ -964,6 +965,8 @@ class Engine:
         and all timings."""
         if candidates is None:
             candidates = [(m, t, c) for m in (1, 2) for t in (3, 7, 15) for c in (0, 16, 32, 64)]
+            if self.grid.nodes()[2] <= 64:
+                candidates += [(m, 7, c) for m in (4, 5) for c in (0, 16, 32, 64)]
         results = []
         for m, t, c in candidates:
             self.set_schedule(m, t, c)
@@ -1018,9 +1021,13 @@ class EngineBatch:
         except Exception:
             pass
 
-    def set_schedule(self, tile_rows: int = 7, x_chunks: int = 0, prefetch: bool = True) -> None:
-        self._check(nat.ems_gpu_batch_set_schedule(self._h, int(tile_rows), int(x_chunks),
-                                                   1 if prefetch else 0))
+    def set_schedule(self, tile_rows: int = 7, x_chunks: int = 0, prefetch: bool = True,
+                     packed: bool = False) -> None:
+        """The fused kernel of every model: tiled (tile height 3, 7 or 15) or
+        packed short rows (64 // nz whole rows per wave, every nz <= 64);
+        x chunks per model (0 automatic); the next-plane prefetch."""
+        mode = (4 if prefetch else 5) if packed else (1 if prefetch else 2)
+        self._check(nat.ems_gpu_batch_set_schedule(self._h, mode, int(tile_rows), int(x_chunks)))
 
     def run(self, n: int, active: Optional[Sequence[bool]] = None) -> None:
         act = None
@@ -1046,13 +1053,21 @@ class EngineBatch:
         import time as _t
         if candidates is None:
             candidates = [(t, c) for t in (3, 7) for c in (0, 1, 2, 4)]
+            if all(e.grid.nodes()[2] <= 64 for e in self.engines):
+                candidates += [("packed", c) for c in (0, 1, 2, 4)]
         results = []
         for t, c in candidates:
-            self.set_schedule(t, c)
+            if t == "packed":
+                self.set_schedule(7, c, packed=True)
+            else:
+                self.set_schedule(t, c)
             self.run(2)
             t0 = _t.perf_counter()
             self.run(steps_each)
             results.append((t, c, (_t.perf_counter() - t0) / steps_each * 1e3))
         best = min(results, key=lambda r: r[2])
-        self.set_schedule(best[0], best[1])
+        if best[0] == "packed":
+            self.set_schedule(7, best[1], packed=True)
+        else:
+            self.set_schedule(best[0], best[1])
         return best, results

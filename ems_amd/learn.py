@@ -219,6 +219,15 @@ def truncated_port_result(fmt: TraceFormat, freqs, dt: float, v_obs, i_obs) -> p
 FREQS_1_3_GHZ = [1e9 + 2e9 * q / 200.0 for q in range(201)]
 
 
+def _schedule(b, models, tile_rows: int) -> None:
+    """Packed short rows when every model allows them (nz <= 64), else the
+    tiled kernel of height tile_rows."""
+    if all(m.grid.nodes()[2] <= 64 for m in models):
+        b.set_schedule(tile_rows, 0, packed=True)
+    else:
+        b.set_schedule(tile_rows, 0)
+
+
 def _batch_engines(models):
     from .engine import Engine
     engines, ports = [], []
@@ -233,19 +242,16 @@ def _batch_engines(models):
 def solver_traces(models, steps: int, group: int = 256, tile_rows: int = 7, log=None):
     """Runs the models `steps` steps in batches of `group` (one fused launch
     per step over each batch) and returns the port voltage and current
-    traces [n][steps] (f32) and the time step (all models share it)."""
+    traces [n][steps] (f32) and each model's time step."""
     from .engine import EngineBatch
     n = len(models)
     v = np.zeros((n, steps), np.float32)
     i = np.zeros((n, steps), np.float32)
-    dts = {m.dt for m in models}
-    if len(dts) != 1:
-        raise EmsError(nat.EMS_E_INVALID, "models with different time steps")
     for g0 in range(0, n, group):
         part = models[g0:g0 + group]
         engines, ports = _batch_engines(part)
         b = EngineBatch(engines)
-        b.set_schedule(tile_rows, 0)
+        _schedule(b, part, tile_rows)
         b.run(steps)
         for q, p in enumerate(ports):
             v[g0 + q] = p.voltage()[:steps]
@@ -255,7 +261,7 @@ def solver_traces(models, steps: int, group: int = 256, tile_rows: int = 7, log=
             e.close()
         if log:
             log(f"traces {g0 + len(part)}/{n}")
-    return v, i, dts.pop()
+    return v, i, np.array([m.dt for m in models])
 
 
 @dataclass
@@ -329,16 +335,16 @@ def early_stop_sweep(es: EarlyStop, models, limits, freqs=FREQS_1_3_GHZ, tile_ro
     fmt = es.fmt
     engines, ports = _batch_engines(models)
     b = EngineBatch(engines)
-    b.set_schedule(tile_rows, 0)
+    _schedule(b, models, tile_rows)
     b.run(fmt.observed)
     v = np.stack([p.voltage()[:fmt.observed] for p in ports])
     i = np.stack([p.current()[:fmt.observed] for p in ports])
-    dt = models[0].dt
     b.close()
     for e in engines:
         e.close()
     if truncated:
-        s11 = [truncated_port_result(fmt, freqs, dt, v[q], i[q]).s11() for q in range(len(models))]
+        s11 = [truncated_port_result(fmt, freqs, models[q].dt, v[q], i[q]).s11()
+               for q in range(len(models))]
         return SweepResult(s11, [False] * len(models), [fmt.observed] * len(models),
                            np.zeros(len(models)))
     pred = es.predict(v, i)
@@ -348,12 +354,12 @@ def early_stop_sweep(es: EarlyStop, models, limits, freqs=FREQS_1_3_GHZ, tile_ro
     steps = [fmt.observed] * len(models)
     for q in range(len(models)):
         if not fb[q]:
-            s11[q] = early_port_result(fmt, freqs, dt, v[q], i[q], pred[q]).s11()
+            s11[q] = early_port_result(fmt, freqs, models[q].dt, v[q], i[q], pred[q]).s11()
     redo = [q for q in range(len(models)) if fb[q]]
     if redo:
         engines, ports = _batch_engines([models[q] for q in redo])
         b = EngineBatch(engines)
-        b.set_schedule(tile_rows, 0)
+        _schedule(b, [models[q] for q in redo], tile_rows)
         outs = b.run_until(limits)
         for k, q in enumerate(redo):
             s11[q] = ports[k].analyze(freqs).s11()
@@ -368,14 +374,15 @@ def _s11_err(a, b) -> float:
     return max(abs(x - y) for x, y in zip(a, b))
 
 
-def held_out_report(es: EarlyStop, v, i, dt: float, freqs=FREQS_1_3_GHZ) -> dict:
+def held_out_report(es: EarlyStop, v, i, dts, freqs=FREQS_1_3_GHZ) -> dict:
     """|dS11| over 1-3 GHz of the early-stopped spectra (observed part plus
     prediction) and of the truncated spectra (observed part alone) against
-    the spectra of the whole traces, per held-out trace."""
+    the spectra of the whole traces, per held-out trace (time steps dts)."""
     fmt = es.fmt
     pred = es.predict(v, i)
     net_err, trunc_err = [], []
     for q in range(v.shape[0]):
+        dt = float(dts[q])
         full = post.PortResult(freqs, post.dft(dt, dt, v[q].astype(np.float64), freqs),
                                post.dft(0.5 * dt, dt, i[q].astype(np.float64), freqs),
                                complex(fmt.resistance, 0.0)).s11()
@@ -425,13 +432,15 @@ def main(argv=None) -> int:
     t0 = time.perf_counter()
     models = W.patch_sweep(a.train + a.held_out, seed=a.seed)
     log(f"{len(models)} models, {sum(m.cells for m in models) / len(models):.0f} cells on average")
-    v, i, dt = solver_traces(models, a.horizon, log=log)
+    v, i, dts = solver_traces(models, a.horizon, log=log)
     t_gen = time.perf_counter() - t0
     cell_steps = sum(m.cells for m in models) * a.horizon
     log(f"traces in {t_gen:.1f} s ({cell_steps / t_gen / 1e9:.2f} Gcell/s)")
     fmt = TraceFormat(a.horizon, a.decimation, a.observed)
     fmt.check()
-    src = fmt.source(Waveform.gaussian(2e9, 1e9), dt)
+    # (the models' time steps agree to rounding: the 0.5 mm mesh and the
+    # Courant factor fix them)
+    src = fmt.source(Waveform.gaussian(2e9, 1e9), float(dts[0]))
     peak_obs = float(np.abs(src[:fmt.N]).max())
     fmt.gain = 1.0 / peak_obs if peak_obs > 0 else 1.0
     es = EarlyStop(ConvNet(4, 2, a.channels, (1, 2, 4, 8, 16, 32), seed=a.seed), fmt, src)
@@ -439,12 +448,12 @@ def main(argv=None) -> int:
     if a.traces:
         D = fmt.decimation
         np.savez(a.traces, v_train=v[tr, ::D], i_train=i[tr, ::D], v_held=v[ho], i_held=i[ho],
-                 dt=dt)
+                 dt=dts)
     t1 = time.perf_counter()
     hist = train(es, v[tr], i[tr], a.steps, a.batch, a.lr, log=log)
     t_train = time.perf_counter() - t1
     log(f"trained {a.steps} steps in {t_train:.1f} s")
-    rep = held_out_report(es, v[ho], i[ho], dt)
+    rep = held_out_report(es, v[ho], i[ho], dts[ho])
     rep.update({"generate_s": t_gen, "train_s": t_train, "steps": a.steps, "batch": a.batch,
                 "final_loss": float(np.mean(hist[-200:])), "horizon": a.horizon,
                 "observed": a.observed, "decimation": a.decimation, "train": a.train,

@@ -228,9 +228,11 @@ typedef struct ems_gpu_batch ems_gpu_batch;
 int ems_gpu_batch_create(ems_gpu *const *models, int n, ems_gpu_batch **out);
 void ems_gpu_batch_destroy(ems_gpu_batch *b);
 const char *ems_gpu_batch_last_error(const ems_gpu_batch *b);
-/* Tile height (3, 7, 15), x chunks per model (0 automatic), next-plane
- * prefetch on or off, for every model of the batch. */
-int ems_gpu_batch_set_schedule(ems_gpu_batch *b, int tile_rows, int x_chunks, int prefetch);
+/* Kernel for every model of the batch: mode 1 the tiled fused kernel, 2
+ * the same without the next-plane prefetch, 4 packed short rows (every
+ * model nz <= 64), 5 packed without the prefetch; tile height (3, 7, 15;
+ * tiled modes only), x chunks per model (0 automatic). */
+int ems_gpu_batch_set_schedule(ems_gpu_batch *b, int mode, int tile_rows, int x_chunks);
 /* n steps of the models with active[q] != 0 (NULL: all); returns after
  * they are done. */
 int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n);
@@ -265,8 +267,10 @@ double ems_gpu_dt(const ems_gpu *h);
  * by itself, pool.rs:131-164): fused 1 runs H, CPML-H, E and CPML-E of a
  * step in one pass over HBM with double-buffered fields (bit-identical to
  * the unfused schedule; 2 is the same without the next-plane prefetch; 3
- * the full-row variant: one wave per row, nz <= 576, tile_rows 3 or 7);
- * tile_rows 3, 7 or 15 (0 keeps the current);
+ * the full-row variant: one wave per row, nz <= 576, tile_rows 3 or 7;
+ * 4 packed short rows: 64 / nz whole k-rows per wave, nz <= 64, else the
+ * tiled kernel; 5 the same without the prefetch); tile_rows 3, 7 or 15
+ * (0 keeps the current);
  * x_chunks 0 = automatic. Steps that need the serial gap between the H and
  * E updates (H sources) always run unfused. */
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks);

@@ -29,17 +29,20 @@ def limits():
                        excitation_end=E.Waveform.gaussian(2e9, 1e9).duration())
 
 
-@pytest.mark.parametrize("tile_rows", [3, 7])
+@pytest.mark.parametrize("tile_rows", [3, 7, "packed"])
 def test_batch_run_until_matches_single_runs(tile_rows):
     ms = models()
     single = [engine(m) for m in ms]
     outs = []
     for e, p in single:
-        e.set_schedule(1, tile_rows, 0)
+        e.set_schedule(1, 7 if tile_rows == "packed" else tile_rows, 0)
         outs.append(e.run_until(limits()))
     batch = [engine(m) for m in ms]
     b = E.EngineBatch([e for e, _ in batch])
-    b.set_schedule(tile_rows, 0)
+    if tile_rows == "packed":
+        b.set_schedule(7, 0, packed=True)
+    else:
+        b.set_schedule(tile_rows, 0)
     bouts = b.run_until(limits())
     for (e, p), (f, q), o, bo in zip(single, batch, outs, bouts):
         assert (o.reason, o.steps, o.timestep) == (bo.reason, bo.steps, bo.timestep)
@@ -52,13 +55,15 @@ def test_batch_run_until_matches_single_runs(tile_rows):
     b.close()
 
 
-def test_batch_subset_runs():
+@pytest.mark.parametrize("packed", [False, True])
+def test_batch_subset_runs(packed):
     ms = models()
     single = [engine(m) for m in ms]
     for e, _ in single:
         e.run(77)
     batch = [engine(m) for m in ms]
     b = E.EngineBatch([e for e, _ in batch])
+    b.set_schedule(7, 2 if packed else 0, packed=packed)
     b.run(40)
     b.run(37)
     for (e, p), (f, q) in zip(single, batch):
