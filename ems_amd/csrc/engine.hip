@@ -253,6 +253,23 @@ __global__ void __launch_bounds__(256) cpml_slab(Fields f, Lane cfl, SlabArg s, 
 // for bit (the reference's fused schedule rests on the same fact,
 // pool.rs:475-487).
 
+// Address spaces. Pointers the kernels load from memory (argument structs,
+// slab descriptors) are generic to the compiler, which then emits flat
+// accesses (no scalar-base addressing, waits on the LDS counter too) and
+// vector loads for uniform struct fields. Data pointers are cast to the
+// global space; read-only descriptors at uniform addresses to the constant
+// space, which the compiler reads through the scalar unit.
+#define GAS __attribute__((address_space(1)))
+#define CAS __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ T cload(const T *p, size_t i = 0) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const CAS T *)p)[i];
+#else
+  return p[i];  // (the host pass only checks the code)
+#endif
+}
+
 // One CPML slab as the fused kernel reads it (cpml.rs:394-420). H-side psi
 // is double-buffered with the fields; E-side psi is updated in place (only
 // the owner of a node computes its E update).
@@ -265,45 +282,114 @@ struct FSlab {
   float *psi0, *psi1;
 };
 
+// One slab's correction of component t at a node inside its box: the new
+// psi and the corrected value (cpml.rs:495-543).
+template <bool ESIDE>
+__device__ __forceinline__ void cpml_apply(const FSlab &s, float pin, float bz, float az, float kz,
+                                           float d01, float d02, float d10, float d12, float d20,
+                                           float d21, float cf0, float cf1, float cf2,
+                                           float &t0, float &t1, float &t2, float &ps) {
+  const int c = s.comp, n = s.normal;
+  float d = d01;
+  d = (c == 0 && n == 2) ? d02 : d;
+  d = (c == 1 && n == 0) ? d10 : d;
+  d = (c == 1 && n == 2) ? d12 : d;
+  d = (c == 2 && n == 0) ? d20 : d;
+  d = (c == 2 && n == 1) ? d21 : d;
+  const float cf = c == 0 ? cf0 : (c == 1 ? cf1 : cf2);
+  float t = c == 0 ? t0 : (c == 1 ? t1 : t2);
+  const float sv = s.sign * d;
+  ps = flush(bz * pin + az * sv);
+  const float scale = ESIDE ? 1.0f : -1.0f;
+  t = flush(t + scale * cf * (kz * sv + ps));
+  if (c == 0) t0 = t;
+  else if (c == 1) t1 = t;
+  else t2 = t;
+}
+
+__device__ __forceinline__ bool cpml_inside(const FSlab &s, int j, int k) {
+  return !(j < s.lo1 || j >= s.hi1 || k < s.lo2 || k >= s.hi2);
+}
+
+__device__ __forceinline__ int cpml_depth(const FSlab &s, int i, int j, int k) {
+  return s.normal == 0 ? i - s.lo0 : (s.normal == 1 ? j - s.lo1 : k - s.lo2);
+}
+
+__device__ __forceinline__ size_t cpml_index(const FSlab &s, int i, int j, int k) {
+  return ((size_t)(i - s.lo0) * s.ly + (j - s.lo1)) * s.lz + (k - s.lo2);
+}
+
 // CPML corrections of one side at node (i, j, k), in registration order,
 // over the slabs whose bits are set in `mask` (the slabs that can reach the
-// block's tile). d<c><n> is the difference h - l of the derivative along
-// normal n used by component c (cpml.rs:495-521); cf<c> the cb (E) or db (H)
-// of component c. (Scalars and selects, not arrays, so nothing goes to
-// scratch memory.)
+// block's tile and plane). d<c><n> is the difference h - l of the
+// derivative along normal n used by component c (cpml.rs:495-521); cf<c>
+// the cb (E) or db (H) of component c.
+//
+// Every slab has its own psi array, and the corrections of one component
+// add up in registration order, so the loads of all slabs can go first and
+// the psi stores last: up to CPML_SLOTS slabs (a node meets at most two
+// per face normal and side) every load is in flight at once instead of one
+// memory round trip per slab. More slabs take the plain loop.
+constexpr int CPML_SLOTS = 6;
+
 template <bool ESIDE>
 __device__ __forceinline__ void cpml_node(const FSlab *__restrict__ slabs, uint32_t mask, int par,
                                           bool write_psi, int i, int j, int k, float d01,
                                           float d02, float d10, float d12, float d20, float d21,
                                           float cf0, float cf1, float cf2, float &t0, float &t1,
                                           float &t2) {
+  if (__builtin_popcount(mask) <= CPML_SLOTS) {
+    float pv[CPML_SLOTS], bv[CPML_SLOTS], av[CPML_SLOTS], kv[CPML_SLOTS];
+    uint32_t m = mask;
+#pragma unroll
+    for (int q = 0; q < CPML_SLOTS; q++) {
+      pv[q] = bv[q] = av[q] = kv[q] = 0.0f;
+      if (m) {
+        const FSlab s = cload(slabs, __builtin_ctz(m));
+        m &= m - 1;
+        const bool in = cpml_inside(s, j, k);
+        // (outside the box: entry 0 of each array, a valid address)
+        const int z = in ? cpml_depth(s, i, j, k) : 0;
+        const size_t qq = in ? cpml_index(s, i, j, k) : 0;
+        const GAS float *pin = (const GAS float *)((ESIDE || par == 0) ? s.psi0 : s.psi1);
+        pv[q] = pin[qq];
+        bv[q] = ((const GAS float *)s.b)[z];
+        av[q] = ((const GAS float *)s.a)[z];
+        kv[q] = ((const GAS float *)s.k)[z];
+      }
+    }
+    m = mask;
+#pragma unroll
+    for (int q = 0; q < CPML_SLOTS; q++) {
+      if (m) {
+        const FSlab s = cload(slabs, __builtin_ctz(m));
+        m &= m - 1;
+        if (cpml_inside(s, j, k)) {
+          float ps;
+          cpml_apply<ESIDE>(s, pv[q], bv[q], av[q], kv[q], d01, d02, d10, d12, d20, d21, cf0, cf1,
+                            cf2, t0, t1, t2, ps);
+          GAS float *pout = (GAS float *)((ESIDE || par == 1) ? s.psi0 : s.psi1);
+          if (write_psi) pout[cpml_index(s, i, j, k)] = ps;
+        }
+      }
+    }
+    return;
+  }
   while (mask) {
     const int q = __builtin_ctz(mask);
     mask &= mask - 1;
-    const FSlab &s = slabs[q];
+    const FSlab s = cload(slabs, q);
     // (The plane is inside the slab's x range: the caller's mask says so.)
-    if (j < s.lo1 || j >= s.hi1 || k < s.lo2 || k >= s.hi2) continue;
-    const int z = s.normal == 0 ? i - s.lo0 : (s.normal == 1 ? j - s.lo1 : k - s.lo2);
-    const size_t qq = ((size_t)(i - s.lo0) * s.ly + (j - s.lo1)) * s.lz + (k - s.lo2);
-    const int c = s.comp, n = s.normal;
-    float d = d01;
-    d = (c == 0 && n == 2) ? d02 : d;
-    d = (c == 1 && n == 0) ? d10 : d;
-    d = (c == 1 && n == 2) ? d12 : d;
-    d = (c == 2 && n == 0) ? d20 : d;
-    d = (c == 2 && n == 1) ? d21 : d;
-    const float cf = c == 0 ? cf0 : (c == 1 ? cf1 : cf2);
-    float t = c == 0 ? t0 : (c == 1 ? t1 : t2);
-    const float sv = s.sign * d;
-    const float *pin = (ESIDE || par == 0) ? s.psi0 : s.psi1;
-    float *pout = (ESIDE || par == 1) ? s.psi0 : s.psi1;
-    const float ps = flush(s.b[z] * pin[qq] + s.a[z] * sv);
+    if (!cpml_inside(s, j, k)) continue;
+    const int z = cpml_depth(s, i, j, k);
+    const size_t qq = cpml_index(s, i, j, k);
+    const GAS float *pin = (const GAS float *)((ESIDE || par == 0) ? s.psi0 : s.psi1);
+    GAS float *pout = (GAS float *)((ESIDE || par == 1) ? s.psi0 : s.psi1);
+    float ps;
+    cpml_apply<ESIDE>(s, pin[qq], ((const GAS float *)s.b)[z], ((const GAS float *)s.a)[z],
+                      ((const GAS float *)s.k)[z], d01, d02, d10, d12, d20, d21, cf0, cf1, cf2, t0,
+                      t1, t2, ps);
     if (write_psi) pout[qq] = ps;
-    const float scale = ESIDE ? 1.0f : -1.0f;
-    t = flush(t + scale * cf * (s.k[z] * sv + ps));
-    if (c == 0) t0 = t;
-    else if (c == 1) t1 = t;
-    else t2 = t;
   }
 }
 
@@ -311,10 +397,10 @@ __device__ __forceinline__ void cpml_node(const FSlab *__restrict__ slabs, uint3
 // so the address is a scalar register pair plus one VGPR (saddr form)
 // instead of a 64-bit vector add per access.
 __device__ __forceinline__ float ldf(const float *base, unsigned boff) {
-  return *(const float *)((const char *)base + boff);
+  return *(const GAS float *)((const GAS char *)base + boff);
 }
 __device__ __forceinline__ void stf(float *base, unsigned boff, float v) {
-  *(float *)((char *)base + boff) = v;
+  *(GAS float *)((GAS char *)base + boff) = v;
 }
 
 // Coefficient l of a node row whose pool offsets are `ro` (croff + 16*r,
@@ -323,8 +409,8 @@ __device__ __forceinline__ void stf(float *base, unsigned boff, float v) {
 // load per lane (the bits are the same).
 __device__ __forceinline__ float ldc(const float *__restrict__ cpool, const uint32_t *ro, int l,
                                      unsigned k4) {
-  const uint32_t o = ro[l];
-  if ((ro[12] >> l) & 1u) return *(const float *)((const char *)cpool + o);
+  const uint32_t o = cload(ro, l);
+  if ((cload(ro, 12) >> l) & 1u) return cload((const float *)((const char *)cpool + o));
   return ldf(cpool, o + k4);
 }
 
@@ -436,7 +522,7 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
   // CPML slabs whose (j, k) box meets this tile (halo included).
   uint32_t mh = 0, me = 0;
   for (int q = 0; q < nslab; q++) {
-    const FSlab &s = slabs[q];
+    const FSlab s = cload(slabs, q);
     if (s.lo1 < j0 + TJ && s.hi1 > j0 - 1 && s.lo2 < k0 + tk && s.hi2 > k0 - 1) {
       if (s.e_side) me |= 1u << q;
       else mh |= 1u << q;
@@ -459,7 +545,7 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
       const float uyp = ldf(A + N + (ob - plane), k4), uzp = ldf(A + 2 * N + (ob - plane), k4);
       if (inb) {
         float h[3];
-        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh & pmask[2 * (i0 - 1)], par, false, i0 - 1,
+        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh & cload(pmask, 2 * (i0 - 1)), par, false, i0 - 1,
                    j, k, h);
         hy_prev = h[1];
         hz_prev = h[2];
@@ -482,7 +568,7 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
     if (PF && i + 1 < i1)
       load_plane(Nx, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv);
     // Slabs that reach this tile and this plane (bit masks, scalar).
-    const uint32_t mhp = mh & pmask[2 * i], mep = me & pmask[2 * i + 1];
+    const uint32_t mhp = mh & cload(pmask, 2 * i), mep = me & cload(pmask, 2 * i + 1);
     float h[3] = {0.0f, 0.0f, 0.0f};
     if (inb) h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mhp, par, own, i, j, k, h);
     sh[buf][0][ty][tx] = h[0];
@@ -579,11 +665,11 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
   int lo = 0, hi = nm;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if ((int)seg[mid] <= b) lo = mid;
+    if ((int)cload(seg, mid) <= b) lo = mid;
     else hi = mid;
   }
-  const FusedArgs &a = ms[lo];
-  const int t = tile_of_block(b - (int)seg[lo], a.ktiles, a.ntiles, xcd_group);
+  const FusedArgs a = cload(ms, lo);
+  const int t = tile_of_block(b - (int)cload(seg, lo), a.ktiles, a.ntiles, xcd_group);
   if (t < 0) return;
   fused_tile<OE, TJ, PF>(a.A, a.B, a.N, a.cpool, a.croff, a.slabs, a.pmask, a.nslab, a.par, a.nx,
                          a.ny, a.nz, a.tk, a.chunk, a.ktiles, a.jtiles, t);
@@ -619,7 +705,7 @@ __device__ __forceinline__ void load_plane_lane(PlaneIn &P, const float *F, size
   P.ix = ldf(F + 3 * N + ob, kf);
   P.iy = ldf(F + 4 * N + ob, kf);
   P.iz = ldf(F + 5 * N + ob, kf);
-  const uint32_t *ro = croff + 16 * r;
+  const GAS uint32_t *ro = (const GAS uint32_t *)croff + 16 * r;
   const uint32_t cflag = ro[12];
 #pragma unroll
   for (int l = 0; l < 12; l++) {
@@ -668,7 +754,7 @@ __device__ __forceinline__ void fused_packed(const float *__restrict__ A, float 
   // CPML slabs whose j range meets the block's rows (k is checked per lane).
   uint32_t mh = 0, me = 0;
   for (int q = 0; q < nslab; q++) {
-    const FSlab &s = slabs[q];
+    const FSlab s = cload(slabs, q);
     if (s.lo1 < j0 - 1 + NW * R && s.hi1 > j0 - 1) {
       if (s.e_side) me |= 1u << q;
       else mh |= 1u << q;
@@ -692,7 +778,7 @@ __device__ __forceinline__ void fused_packed(const float *__restrict__ A, float 
       const float uyp = ldf(A + N + (obw - plane), kf), uzp = ldf(A + 2 * N + (obw - plane), kf);
       if (inb) {
         float h[3];
-        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh & pmask[2 * (i0 - 1)], par, false, i0 - 1,
+        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh & cload(pmask, 2 * (i0 - 1)), par, false, i0 - 1,
                    j, k, h);
         hy_prev = h[1];
         hz_prev = h[2];
@@ -711,7 +797,7 @@ __device__ __forceinline__ void fused_packed(const float *__restrict__ A, float 
     if (PF && i + 1 < i1)
       load_plane_lane(Nx, A, N, cpool, croff, obw + plane, r + ny, plane, nz, kf, kc, i + 2 < nx,
                       jv, kv);
-    const uint32_t mhp = mh & pmask[2 * i], mep = me & pmask[2 * i + 1];
+    const uint32_t mhp = mh & cload(pmask, 2 * i), mep = me & cload(pmask, 2 * i + 1);
     float h[3] = {0.0f, 0.0f, 0.0f};
     if (inb) h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mhp, par, own, i, j, k, h);
     sh[buf][0][w][lane] = h[0];
@@ -781,11 +867,11 @@ __global__ void __launch_bounds__(64 * NW)
   int lo = 0, hi = nm;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if ((int)seg[mid] <= b) lo = mid;
+    if ((int)cload(seg, mid) <= b) lo = mid;
     else hi = mid;
   }
-  const FusedArgs &a = ms[lo];
-  const int t = tile_of_block(b - (int)seg[lo], 1, a.ntiles, xcd_group);
+  const FusedArgs a = cload(ms, lo);
+  const int t = tile_of_block(b - (int)cload(seg, lo), 1, a.ntiles, xcd_group);
   if (t < 0) return;
   fused_packed<OE, NW, PF>(a.A, a.B, a.N, a.cpool, a.croff, a.slabs, a.pmask, a.nslab, a.par, a.nx,
                            a.ny, a.nz, a.tk, a.chunk, a.jtiles, t);
@@ -833,7 +919,7 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
   // CPML slabs whose j range meets the tile (k is checked per lane).
   uint32_t mh = 0, me = 0;
   for (int q = 0; q < nslab; q++) {
-    const FSlab &s = slabs[q];
+    const FSlab s = cload(slabs, q);
     if (s.lo1 < j0 + TJ && s.hi1 > j0 - 1) {
       if (s.e_side) me |= 1u << q;
       else mh |= 1u << q;
@@ -846,7 +932,7 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
     // H of plane i0-1 (read-only: its owner writes it).
     const int ip = i0 - 1;
     const size_t ob = ((size_t)ip * ny + jc) * nz, r = (size_t)ip * ny + jc;
-    const uint32_t mhp = mh & pmask[2 * ip];
+    const uint32_t mhp = mh & cload(pmask, 2 * ip);
 #pragma unroll
     for (int m = 0; m < NSEG; m++) {
       const int k = 64 * m + lane;
@@ -868,7 +954,7 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
     const size_t r = (size_t)i * ny + jc;
     const size_t ob = r * nz;
     const bool iv = i + 1 < nx;
-    const uint32_t mhp = mh & pmask[2 * i], mep = me & pmask[2 * i + 1];
+    const uint32_t mhp = mh & cload(pmask, 2 * i), mep = me & cload(pmask, 2 * i + 1);
     // H of every segment of this row.
 #pragma unroll
     for (int m = 0; m < NSEG; m++) {
@@ -1270,7 +1356,7 @@ struct GapArgs {
   uint64_t *step;
 };
 __global__ void __launch_bounds__(256) gap_post_e_batch(const GapArgs *__restrict__ gs) {
-  const GapArgs &g = gs[blockIdx.x];
+  const GapArgs g = cload(gs, blockIdx.x);
   gap_post_e_body(g.f, g.fp, 1, g.ops, g.nops, 0, g.nops, 1, g.step);
 }
 
