@@ -183,7 +183,7 @@ def earlystop(args, log) -> int:
     import ems_amd as E
     from ems_amd import learn as L
     from ems_amd import workloads as W
-    path = args.model or os.path.join(ROOT, "ems_amd", "earlystop_patch.npz")
+    path = args.model or L.PATCH_MODEL
     es = L.EarlyStop.load(path)
     freqs = L.FREQS_1_3_GHZ
     lim = E.RunLimits(args.max_steps, E.EndCriteria.energy_decay_db(-40.0),
@@ -197,7 +197,7 @@ def earlystop(args, log) -> int:
     t0 = time.perf_counter()
     engines, ports = L._batch_engines(models)
     b = E.EngineBatch(engines)
-    L._schedule(b, models, args.tile_rows or 7)
+    L._schedule(b, models, args.tile_rows or 7, args.x_chunks)
     outs = b.run_until(lim)
     full = [p.analyze(freqs).s11() for p in ports]
     b.close()
@@ -262,7 +262,8 @@ def main() -> int:
     ap.add_argument("--mode", type=int, default=1,
                     help="fused kernel with --tile-rows: 1 tiled, 2 tiled without prefetch, 3 full rows")
     ap.add_argument("--tile-rows", type=int, default=0, help="fused tile height (3, 7, 15)")
-    ap.add_argument("--x-chunks", type=int, default=0)
+    ap.add_argument("--x-chunks", type=int, default=None,
+                    help="x chunks per model (0 automatic; patch workloads default 16)")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
 
@@ -274,6 +275,8 @@ def main() -> int:
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    if args.x_chunks is None:
+        args.x_chunks = 16 if args.workload.startswith("patch") else 0
     if args.workload == "patch_sweep":
         return sweep(args, log)
     if args.workload == "patch_earlystop":

@@ -180,14 +180,31 @@ __global__ void __launch_bounds__(256) cn_conv_bwd_w(const float *__restrict__ d
   if (threadIdx.x == CN_K && ci == 0) dbias[co] = red[CN_K][0];
 }
 
-// Adam on the f32 master parameters, then their bf16 copy (weights only).
+// Per-block sums of g^2 (the gradient norm, for clipping).
+__global__ void __launch_bounds__(256) cn_sqsum(const float *__restrict__ g, size_t n,
+                                                float *__restrict__ partial) {
+  __shared__ float red[256];
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const float x = i < n ? g[i] : 0.0f;
+  red[threadIdx.x] = x * x;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Adam on the f32 master parameters (gradient scaled by gscale, the
+// clipping factor), then their bf16 copy.
 __global__ void __launch_bounds__(256) cn_adam(float *__restrict__ p, const float *__restrict__ g,
                                                float *__restrict__ m, float *__restrict__ v,
                                                size_t n, float lr, float b1, float b2, float eps,
-                                               float c1, float c2, uint16_t *__restrict__ pbf) {
+                                               float c1, float c2, float gscale,
+                                               uint16_t *__restrict__ pbf) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float gi = g[i];
+  const float gi = g[i] * gscale;
   const float mi = b1 * m[i] + (1.0f - b1) * gi;
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
   m[i] = mi;
@@ -237,6 +254,7 @@ struct ems_cn {
   float *hy = nullptr;              // target batch
   std::vector<float *> h, a, z;     // per layer: input activation, tanh, pre-activation
   float *out = nullptr, *gout = nullptr, *gh = nullptr, *dz = nullptr, *part = nullptr;
+  float *gpart = nullptr;  // per-block sums of g^2
   int32_t *didx = nullptr;
   std::string err;
 };
@@ -306,8 +324,9 @@ int forward(ems_cn *n, int B, int T) {
   return 0;
 }
 
-// Loss over t >= t0 against n->hy and the parameter gradient in n->g.
-int backward(ems_cn *n, int B, int T, int t0, float *loss) {
+// Loss over t >= t0 against n->hy and the parameter gradient in n->g;
+// *gnorm (when not NULL) receives the gradient's 2-norm.
+int backward(ems_cn *n, int B, int T, int t0, float *loss, double *gnorm = nullptr) {
   const size_t BT = (size_t)B * T;
   const size_t no = BT * n->cout;
   const int Tv = T - t0 > 0 ? T - t0 : 1;
@@ -338,14 +357,25 @@ int backward(ems_cn *n, int B, int T, int t0, float *loss) {
                          (!last && l > 0) ? 1 : 0);
     }
   }
+  const unsigned ng = blocks(n->np);
+  if (gnorm)
+    hipLaunchKernelGGL(cn_sqsum, dim3(ng), dim3(256), 0, n->stream, n->g, n->np, n->gpart);
   CCK(n, hipGetLastError());
-  std::vector<float> part(nb);
+  std::vector<float> part(nb), gp(gnorm ? ng : 0);
   CCK(n, hipMemcpyAsync(part.data(), n->part, nb * sizeof(float), hipMemcpyDeviceToHost,
                         n->stream));
+  if (gnorm)
+    CCK(n, hipMemcpyAsync(gp.data(), n->gpart, ng * sizeof(float), hipMemcpyDeviceToHost,
+                          n->stream));
   CCK(n, hipStreamSynchronize(n->stream));
   double s = 0.0;
   for (float v : part) s += v;
   if (loss) *loss = (float)(s * inv);
+  if (gnorm) {
+    double g2 = 0.0;
+    for (float v : gp) g2 += v;
+    *gnorm = std::sqrt(g2);
+  }
   return 0;
 }
 
@@ -383,6 +413,7 @@ int ems_cn_create(int cin, int cout, int channels, int layers, const int32_t *di
     for (size_t q = 0; q < (size_t)n->lo[l] * n->li[l] * CN_K; q++) p[n->woff[l] + q] = nd(rng) * sc;
   }
   if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&n->gpart, blocks(n->np) * sizeof(float)) != hipSuccess ||
       hipMalloc(&n->p, n->np * 4) != hipSuccess || hipMalloc(&n->g, n->np * 4) != hipSuccess ||
       hipMalloc(&n->m, n->np * 4) != hipSuccess || hipMalloc(&n->v, n->np * 4) != hipSuccess ||
       hipMalloc(&n->pbf, n->np * 2) != hipSuccess) {
@@ -398,7 +429,7 @@ void ems_cn_destroy(ems_cn *n) {
   if (n->stream) (void)hipStreamSynchronize(n->stream);
   for (void *q : {(void *)n->p, (void *)n->g, (void *)n->m, (void *)n->v, (void *)n->pbf,
                   (void *)n->dx, (void *)n->dy, (void *)n->hx, (void *)n->hy, (void *)n->out,
-                  (void *)n->gout, (void *)n->gh, (void *)n->dz, (void *)n->part,
+                  (void *)n->gout, (void *)n->gh, (void *)n->dz, (void *)n->part, (void *)n->gpart,
                   (void *)n->didx})
     if (q) (void)hipFree(q);
   for (int l = 1; l < (int)n->h.size(); l++)
@@ -450,7 +481,8 @@ int ems_cn_set_data(ems_cn *n, const float *x, const float *y, int count, int T)
   return EMS_OK;
 }
 
-int ems_cn_train_batch(ems_cn *n, const int32_t *idx, int B, int t0, float lr, float *loss) {
+int ems_cn_train_batch(ems_cn *n, const int32_t *idx, int B, int t0, float lr, float clip,
+                       float *loss, float *grad_norm) {
   if (!n || !idx || B < 1 || !n->dx) return EMS_E_INVALID;
   for (int b = 0; b < B; b++)
     if (idx[b] < 0 || idx[b] >= n->dn) return cfail(n, EMS_E_INVALID, "sample index out of range");
@@ -463,12 +495,15 @@ int ems_cn_train_batch(ems_cn *n, const int32_t *idx, int B, int t0, float lr, f
   hipLaunchKernelGGL(cn_gather, dim3(blocks((size_t)B * n->cout * T)), dim3(256), 0, n->stream,
                      n->dy, n->didx, n->hy, n->cout, T, B);
   if ((rc = forward(n, B, T))) return rc;
-  if ((rc = backward(n, B, T, t0, loss))) return rc;
+  double gn = 0.0;
+  if ((rc = backward(n, B, T, t0, loss, &gn))) return rc;
+  if (grad_norm) *grad_norm = (float)gn;
   n->step++;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - std::pow(b1, (float)n->step), c2 = 1.0f - std::pow(b2, (float)n->step);
+  const float gscale = (clip > 0.0f && gn > clip) ? (float)(clip / gn) : 1.0f;
   hipLaunchKernelGGL(cn_adam, dim3(blocks(n->np)), dim3(256), 0, n->stream, n->p, n->g, n->m, n->v,
-                     n->np, lr, b1, b2, 1e-8f, c1, c2, n->pbf);
+                     n->np, lr, b1, b2, 1e-8f, c1, c2, gscale, n->pbf);
   CCK(n, hipGetLastError());
   return EMS_OK;
 }

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
 
@@ -89,11 +90,15 @@ class ConvNet:
                                         x.shape[0], x.shape[2]))
         self._data_n, self._data_T = x.shape[0], x.shape[2]
 
-    def train_batch(self, idx, t0: int, lr: float) -> float:
+    def train_batch(self, idx, t0: int, lr: float, clip: float = 0.0) -> float:
+        """One Adam step; returns the batch loss (before the step). The
+        gradient norm before clipping is kept in last_grad_norm."""
         idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int32))
-        loss = C.c_float()
+        loss, gn = C.c_float(), C.c_float()
         self._check(nat.ems_cn_train_batch(self._h, nat.ptr(idx, C.c_int32), idx.size, int(t0),
-                                           C.c_float(lr), C.byref(loss)))
+                                           C.c_float(lr), C.c_float(clip), C.byref(loss),
+                                           C.byref(gn)))
+        self.last_grad_norm = gn.value
         return loss.value
 
     def forward(self, x) -> np.ndarray:
@@ -149,12 +154,14 @@ class TraceFormat:
         return np.array([amplitude * waveform.value((m * D + 0.5) * dt) for m in range(self.T)],
                         np.float32)
 
-    def inputs(self, v, i, src) -> np.ndarray:
+    def inputs(self, v, i, src, decimated: bool = False) -> np.ndarray:
         """[n][4][T] network inputs from full-rate traces (at least
-        `observed` samples each) and the decimated excitation."""
+        `observed` samples each; already decimated with `decimated`) and
+        the decimated excitation."""
         v = np.atleast_2d(np.asarray(v, np.float64))
         i = np.atleast_2d(np.asarray(i, np.float64))
-        T, N, D = self.T, self.N, self.decimation
+        T, N = self.T, self.N
+        D = 1 if decimated else self.decimation
         x = np.zeros((v.shape[0], 4, T), np.float32)
         x[:, 0, :N] = v[:, :N * D:D] * self.gain
         x[:, 1, :N] = i[:, :N * D:D] * self.resistance * self.gain
@@ -162,12 +169,13 @@ class TraceFormat:
         x[:, 3, :] = src[None, :]
         return x
 
-    def targets(self, v, i) -> np.ndarray:
+    def targets(self, v, i, decimated: bool = False) -> np.ndarray:
         """[n][2][T] decimated V and R*I over the horizon (traces padded
         with zeros beyond their end)."""
         v = np.atleast_2d(np.asarray(v, np.float64))
         i = np.atleast_2d(np.asarray(i, np.float64))
-        T, D = self.T, self.decimation
+        T = self.T
+        D = 1 if decimated else self.decimation
         y = np.zeros((v.shape[0], 2, T), np.float32)
         nv = min(T, (v.shape[1] + D - 1) // D)
         ni = min(T, (i.shape[1] + D - 1) // D)
@@ -191,11 +199,17 @@ def early_port_result(fmt: TraceFormat, freqs, dt: float, v_obs, i_obs, pred,
     """Port spectra from the observed full-rate samples (V at (n+1)dt, I at
     (n+1/2)dt, n < observed) plus the predicted decimated tail up to sample
     `stop` (default: the horizon), each decimated sample standing for
-    `decimation` steps."""
+    `decimation` steps. The first tail sample is weighted (D+1)/2 steps
+    instead of D: the Euler-Maclaurin end terms of the full-rate sum and
+    the decimated one differ by (D-1)/2 steps of that sample, so the
+    junction adds no first-order error."""
     n0, D = fmt.observed, fmt.decimation
     stop = fmt.T if stop is None else stop
     vh = pred[0, fmt.N:stop].astype(np.float64)
     ih = pred[1, fmt.N:stop].astype(np.float64) / fmt.resistance
+    if vh.size:
+        vh[0] *= (D + 1) / (2.0 * D)
+        ih[0] *= (D + 1) / (2.0 * D)
     v = np.asarray(v_obs, np.float64)[:n0]
     i = np.asarray(i_obs, np.float64)[:n0]
     vf = np.array(post.dft(dt, dt, v, freqs)) + np.array(post.dft((n0 + 1) * dt, D * dt, vh, freqs))
@@ -218,14 +232,20 @@ def truncated_port_result(fmt: TraceFormat, freqs, dt: float, v_obs, i_obs) -> p
 
 FREQS_1_3_GHZ = [1e9 + 2e9 * q / 200.0 for q in range(201)]
 
+# The network trained on the patch-antenna family (python -m ems_amd.learn
+# --out ...): 4096 solver traces of seeded patches, horizon 4096 steps,
+# decimation 8, 1024 observed steps.
+PATCH_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "earlystop_patch.npz")
 
-def _schedule(b, models, tile_rows: int) -> None:
+
+def _schedule(b, models, tile_rows: int, x_chunks: int = 16) -> None:
     """Packed short rows when every model allows them (nz <= 64), else the
-    tiled kernel of height tile_rows."""
+    tiled kernel of height tile_rows; x_chunks per model (16 measured best
+    for the patch family, 0 automatic)."""
     if all(m.grid.nodes()[2] <= 64 for m in models):
-        b.set_schedule(tile_rows, 0, packed=True)
+        b.set_schedule(tile_rows, x_chunks, packed=True)
     else:
-        b.set_schedule(tile_rows, 0)
+        b.set_schedule(tile_rows, x_chunks)
 
 
 def _batch_engines(models):
@@ -285,7 +305,7 @@ class EarlyStop:
                  floats=np.array([f.resistance, f.gain, self.threshold_db], np.float64))
 
     @staticmethod
-    def load(path: str) -> "EarlyStop":
+    def load(path: str = PATCH_MODEL) -> "EarlyStop":
         z = np.load(path, allow_pickle=False)
         H, D, obs, cin, cout, ch = (int(q) for q in z["meta"])
         r, gain, thr = (float(q) for q in z["floats"])
@@ -294,24 +314,35 @@ class EarlyStop:
         return EarlyStop(net, TraceFormat(H, D, obs, r, gain), z["src"].astype(np.float32), thr)
 
 
-def train(es: EarlyStop, v, i, steps: int, batch: int = 32, lr: float = 2e-3,
-          lr_end: float = 1e-4, seed: int = 0, log=None, every: int = 500):
-    """Trains es.net on full traces [n][horizon] with a cosine-decayed
-    learning rate; the loss counts the predicted part (t >= N) only.
-    Returns the loss history (one value per step)."""
+def train(es: EarlyStop, v, i, steps: int, batch: int = 32, lr: float = 6e-4,
+          lr_end: float = 1e-5, warmup: int = 1000, seed: int = 0, log=None, every: int = 500,
+          decimated: bool = False):
+    """Trains es.net on full traces [n][horizon]: linear warm-up, then a
+    cosine-decayed learning rate; the loss counts the predicted part
+    (t >= N) only. Gradients longer than three times the running mean norm
+    are clipped to it (a guard against the rare large step that throws
+    Adam off). Returns the loss history (one value per step)."""
     fmt = es.fmt
-    x = fmt.inputs(v, i, es.src)
-    y = fmt.targets(v, i)
+    x = fmt.inputs(v, i, es.src, decimated)
+    y = fmt.targets(v, i, decimated)
     es.net.set_data(x, y)
     rng = np.random.default_rng(seed)
     hist = []
     n = x.shape[0]
+    mean_norm = 0.0
     for s in range(steps):
-        a = 0.5 * (1.0 + math.cos(math.pi * s / max(steps - 1, 1)))
-        rate = lr_end + (lr - lr_end) * a
-        hist.append(es.net.train_batch(rng.integers(0, n, batch), fmt.N, rate))
+        if s < warmup:
+            rate = lr * (s + 1) / warmup
+        else:
+            a = 0.5 * (1.0 + math.cos(math.pi * (s - warmup) / max(steps - warmup - 1, 1)))
+            rate = lr_end + (lr - lr_end) * a
+        clip = 3.0 * mean_norm if s >= 100 else 0.0
+        hist.append(es.net.train_batch(rng.integers(0, n, batch), fmt.N, rate, clip))
+        g = es.net.last_grad_norm
+        g = min(g, clip) if clip > 0 else g
+        mean_norm = g if s == 0 else 0.99 * mean_norm + 0.01 * g
         if log and (s % every == 0 or s == steps - 1):
-            log(f"step {s} loss {np.mean(hist[-every:]):.3e} lr {rate:.2e}")
+            log(f"step {s} loss {np.mean(hist[-every:]):.3e} lr {rate:.2e} |g| {mean_norm:.2e}")
     return np.array(hist)
 
 
@@ -417,12 +448,14 @@ def main(argv=None) -> int:
     ap.add_argument("--horizon", type=int, default=4096)
     ap.add_argument("--decimation", type=int, default=8)
     ap.add_argument("--observed", type=int, default=1024)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=40000)
     ap.add_argument("--batch", type=int, default=32)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=6e-4)
     ap.add_argument("--channels", type=int, default=64)
     ap.add_argument("--out", default="")
     ap.add_argument("--traces", default="", help="save the traces (npz) here")
+    ap.add_argument("--traces-in", default="",
+                    help="train on traces saved with --traces instead of running the solver")
     ap.add_argument("--report", default="", help="write the held-out report (json) here")
     a = ap.parse_args(argv)
 
@@ -430,12 +463,24 @@ def main(argv=None) -> int:
         print(m, file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
-    models = W.patch_sweep(a.train + a.held_out, seed=a.seed)
-    log(f"{len(models)} models, {sum(m.cells for m in models) / len(models):.0f} cells on average")
-    v, i, dts = solver_traces(models, a.horizon, log=log)
-    t_gen = time.perf_counter() - t0
-    cell_steps = sum(m.cells for m in models) * a.horizon
-    log(f"traces in {t_gen:.1f} s ({cell_steps / t_gen / 1e9:.2f} Gcell/s)")
+    tr, ho = slice(0, a.train), slice(a.train, a.train + a.held_out)
+    if a.traces_in:
+        z = np.load(a.traces_in, allow_pickle=False)
+        vt, it = z["v_train"][:a.train], z["i_train"][:a.train]
+        vh, ih = z["v_held"][:a.held_out], z["i_held"][:a.held_out]
+        dts = np.asarray(z["dt"], np.float64)
+        dts = dts if dts.ndim else np.full(a.train + a.held_out, float(dts))
+        dth = dts[-len(z["v_held"]):][:a.held_out]
+        t_gen = 0.0
+    else:
+        models = W.patch_sweep(a.train + a.held_out, seed=a.seed)
+        log(f"{len(models)} models, {sum(m.cells for m in models) / len(models):.0f} cells on average")
+        v, i, dts = solver_traces(models, a.horizon, log=log)
+        t_gen = time.perf_counter() - t0
+        cell_steps = sum(m.cells for m in models) * a.horizon
+        log(f"traces in {t_gen:.1f} s ({cell_steps / t_gen / 1e9:.2f} Gcell/s)")
+        vt, it = v[tr, ::a.decimation], i[tr, ::a.decimation]
+        vh, ih, dth = v[ho], i[ho], dts[ho]
     fmt = TraceFormat(a.horizon, a.decimation, a.observed)
     fmt.check()
     # (the models' time steps agree to rounding: the 0.5 mm mesh and the
@@ -444,16 +489,13 @@ def main(argv=None) -> int:
     peak_obs = float(np.abs(src[:fmt.N]).max())
     fmt.gain = 1.0 / peak_obs if peak_obs > 0 else 1.0
     es = EarlyStop(ConvNet(4, 2, a.channels, (1, 2, 4, 8, 16, 32), seed=a.seed), fmt, src)
-    tr, ho = slice(0, a.train), slice(a.train, a.train + a.held_out)
-    if a.traces:
-        D = fmt.decimation
-        np.savez(a.traces, v_train=v[tr, ::D], i_train=i[tr, ::D], v_held=v[ho], i_held=i[ho],
-                 dt=dts)
+    if a.traces and not a.traces_in:
+        np.savez(a.traces, v_train=vt, i_train=it, v_held=vh, i_held=ih, dt=dts)
     t1 = time.perf_counter()
-    hist = train(es, v[tr], i[tr], a.steps, a.batch, a.lr, log=log)
+    hist = train(es, vt, it, a.steps, a.batch, a.lr, log=log, decimated=True)
     t_train = time.perf_counter() - t1
     log(f"trained {a.steps} steps in {t_train:.1f} s")
-    rep = held_out_report(es, v[ho], i[ho], dts[ho])
+    rep = held_out_report(es, vh, ih, dth)
     rep.update({"generate_s": t_gen, "train_s": t_train, "steps": a.steps, "batch": a.batch,
                 "final_loss": float(np.mean(hist[-200:])), "horizon": a.horizon,
                 "observed": a.observed, "decimation": a.decimation, "train": a.train,

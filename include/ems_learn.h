@@ -20,7 +20,8 @@
  * then the biases [out].
  * Sequences: [sample][channel][time] f32, time fastest.
  * Loss: mean over samples, output channels and t >= t0 of (out - y)^2.
- * Optimizer: Adam (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected).
+ * Optimizer: Adam (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected),
+ * optionally on a norm-clipped gradient.
  *
  * Return codes as ems_gpu.h (EMS_OK, EMS_E_INVALID, EMS_E_DEVICE).
  */
@@ -51,9 +52,12 @@ int ems_cn_get_params(ems_cn *n, float *params, uint64_t count);
 int ems_cn_set_data(ems_cn *n, const float *x, const float *y, int count, int T);
 
 /* One optimizer step on the samples idx[0..B) of the training set, loss
- * over t >= t0; *loss (may be NULL) receives the batch loss before the
- * step. */
-int ems_cn_train_batch(ems_cn *n, const int32_t *idx, int B, int t0, float lr, float *loss);
+ * over t >= t0; the gradient is scaled to 2-norm `clip` when it is longer
+ * (clip <= 0: no clipping). *loss and *grad_norm (either may be NULL)
+ * receive the batch loss before the step and the gradient's norm before
+ * clipping. */
+int ems_cn_train_batch(ems_cn *n, const int32_t *idx, int B, int t0, float lr, float clip,
+                       float *loss, float *grad_norm);
 
 /* out [B][cout][T] = network(x [B][cin][T]). */
 int ems_cn_forward(ems_cn *n, const float *x, int B, int T, float *out);

@@ -120,7 +120,8 @@ def loss_and_grad(shape: Shape, p: np.ndarray, x: np.ndarray, y: np.ndarray, t0:
 
 
 class Adam:
-    """beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected."""
+    """beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected; optional norm
+    clipping of the gradient."""
 
     def __init__(self, n: int, lr: float):
         self.lr = lr
@@ -128,7 +129,11 @@ class Adam:
         self.v = np.zeros(n, np.float32)
         self.t = 0
 
-    def step(self, p: np.ndarray, g: np.ndarray) -> np.ndarray:
+    def step(self, p: np.ndarray, g: np.ndarray, clip: float = 0.0) -> np.ndarray:
+        """One step; the gradient scaled to 2-norm `clip` when longer."""
+        n = float(np.sqrt(np.sum(np.asarray(g, np.float64) ** 2)))
+        if clip > 0.0 and n > clip:
+            g = (g * np.float32(clip / n)).astype(np.float32)
         self.t += 1
         b1, b2 = np.float32(0.9), np.float32(0.999)
         c1 = np.float32(1.0 - 0.9 ** self.t)

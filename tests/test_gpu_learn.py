@@ -1,5 +1,6 @@
 """The early-stop network's device kernels against the CPU oracle
-(oracle/convnet.py): forward pass, loss gradient and one Adam step, on
+(oracle/convnet.py): forward pass, loss gradient and Adam steps (one with
+a clipped gradient), on
 the same parameters and inputs. The device accumulates in f32 in its own
 order (multiply-adds contracted), so values agree to a tolerance: 2e-5 of
 the largest magnitude for outputs, 2e-4 for gradients and 1e-5 for
@@ -72,12 +73,17 @@ def test_adam_step_matches_oracle():
     n.set_data(x, y)
     p = n.params()
     adam = O.Adam(s.count, 1e-3)
-    for step, idx in enumerate(([0, 2, 4], [5, 1, 3], [1, 1, 0])):
-        loss = n.train_batch(idx, 10, 1e-3)
+    # the third step clips the gradient to a tenth of its norm
+    for step, (idx, clip) in enumerate((([0, 2, 4], 0.0), ([5, 1, 3], 0.0), ([1, 1, 0], -1.0))):
         lo, go = O.loss_and_grad(s, p.astype(np.float64), x[idx].astype(np.float64),
                                  y[idx].astype(np.float64), 10)
+        gn = float(np.sqrt(np.sum(go ** 2)))
+        if clip < 0:
+            clip = 0.1 * gn
+        loss = n.train_batch(idx, 10, 1e-3, clip)
         assert abs(loss - lo) <= 1e-5 * abs(lo), step
-        p = adam.step(p, go.astype(np.float32))
+        assert abs(n.last_grad_norm - gn) <= 1e-4 * gn, (n.last_grad_norm, gn)
+        p = adam.step(p, go.astype(np.float32), clip)
         _close(n.params(), p, 1e-5)
     n.close()
 

@@ -1,0 +1,65 @@
+"""Host side of the learned early stop (ems_amd/learn.py) on the CPU: the
+trace format, the spectra assembled from observed samples plus a
+decimated tail, and the residual-power rule. Pinned by mathematics: with
+the true tail in place of a prediction, the assembled spectra equal the
+spectra of the whole trace up to the decimated sum's error, which is tiny
+for a band-limited signal sampled far above its band."""
+import math
+
+import numpy as np
+
+from ems_amd import learn as L
+from ems_amd import post
+
+DT = 1e-12
+
+
+def traces(n=4096, f0=2e9, tau=0.483e-9, t0=1.8e-9, r=50.0):
+    """A port pair whose voltage is a modulated Gaussian with a damped
+    ringing tail and whose current follows a 40-ohm load plus ringing."""
+    t = (np.arange(n) + 1.0) * DT
+    ring = 0.3 * np.exp(-(t - t0).clip(0) / 0.4e-9) * np.sin(2 * np.pi * 2.3e9 * (t - t0)) * (t > t0)
+    v = np.cos(2 * np.pi * f0 * (t - t0)) * np.exp(-((t - t0) / tau) ** 2) + ring
+    i = v / 40.0 - ring / r
+    return v, i
+
+
+def test_inputs_and_targets():
+    fmt = L.TraceFormat(4096, 8, 1024, gain=3.0)
+    fmt.check()
+    v, i = traces()
+    src = np.linspace(0, 1, fmt.T, dtype=np.float32)
+    x = fmt.inputs(v, i, src)
+    y = fmt.targets(v, i)
+    assert x.shape == (1, 4, 512) and y.shape == (1, 2, 512)
+    assert np.allclose(x[0, 0, :128], 3.0 * v[:1024:8].astype(np.float32))
+    assert np.allclose(x[0, 1, :128], 3.0 * 50.0 * i[:1024:8], rtol=1e-6)
+    assert not x[0, :2, 128:].any() and (x[0, 2, :128] == 1).all() and not x[0, 2, 128:].any()
+    assert np.array_equal(x[0, 3], src)
+    assert np.allclose(y[0, 0], v[::8].astype(np.float32))
+    assert np.allclose(y[0, 1], (50.0 * i[::8]).astype(np.float32))
+
+
+def test_true_tail_reproduces_the_full_spectra():
+    fmt = L.TraceFormat(4096, 8, 1024)
+    v, i = traces()
+    freqs = L.FREQS_1_3_GHZ
+    full = post.PortResult(freqs, post.dft(DT, DT, v, freqs), post.dft(0.5 * DT, DT, i, freqs),
+                           50.0).s11()
+    pred = fmt.targets(v, i)[0]
+    early = L.early_port_result(fmt, freqs, DT, v, i, pred).s11()
+    err = max(abs(a - b) for a, b in zip(early, full))
+    assert err < 2e-3, err
+    trunc = L.truncated_port_result(fmt, freqs, DT, v, i).s11()
+    terr = max(abs(a - b) for a, b in zip(trunc, full))
+    assert terr > 0.1, terr  # the observed part alone misses the pulse
+
+
+def test_residual_rule():
+    T = 512
+    t = np.arange(T)
+    fast = np.exp(-t / 20.0) * np.sin(t / 3.0)
+    slow = np.exp(-t / 400.0) * np.sin(t / 3.0)
+    pred = np.stack([np.stack([fast, fast]), np.stack([slow, slow])]).astype(np.float32)
+    r = L.residual_db(pred)
+    assert r[0] < -40.0 < r[1]
