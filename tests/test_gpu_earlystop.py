@@ -2,9 +2,9 @@
 method without it, on fresh solver traces: twelve patch antennas of a
 seed the network never saw, run on the device to the horizon. The
 early-stopped spectra (1024 observed steps plus the predicted tail) must
-be an order of magnitude closer to the whole-trace S11 than the spectra
-of the observed steps alone. (Held-out figures of the committed network
-on 512 traces: median |dS11| 0.038, 90th percentile 0.082; truncation:
+be far closer to the whole-trace S11 than the spectra of the observed
+steps alone. (Held-out figures of the committed network on 512 traces:
+median |dS11| 0.008, 90th percentile 0.026, largest 0.44; truncation:
 median 1.01.)"""
 import numpy as np
 import pytest
@@ -23,8 +23,8 @@ def test_network_gains_over_truncation():
     net = np.array(rep["net_err"])
     trunc = np.array(rep["trunc_err"])
     assert np.median(trunc) > 0.5, rep["trunc_dS11_median"]
-    assert np.median(net) < 0.1 * np.median(trunc), (np.median(net), np.median(trunc))
-    assert np.median(net) < 0.1 and net.max() < 0.6, net
+    assert np.median(net) < 0.05 * np.median(trunc), (np.median(net), np.median(trunc))
+    assert np.median(net) < 0.05 and net.max() < 0.6, net
     # the prediction decays within the horizon (no fallback on this family)
     assert rep["residual_db_max"] < -40.0
 
