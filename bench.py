@@ -117,13 +117,16 @@ def sweep(args, log) -> int:
     solve_one(models[0])
     for k in phase:
         phase[k] = 0.0
-    t0 = time.perf_counter()
-    seq = [solve_one(m) for m in models]
-    t_seq = time.perf_counter() - t0
-    cell_steps = sum(m.cells * o.steps for m, (o, _) in zip(models, seq))
-    log(f"sequential: {t_seq:.2f} s, {cell_steps / t_seq / 1e9:.2f} Gcell/s "
-        f"(setup {phase['setup']:.2f} s, run {phase['run']:.2f} s, analysis {phase['analysis']:.2f} s)")
-    seq_phase = dict(phase)
+    seq, t_seq, seq_phase = None, None, None
+    if not args.skip_sequential:
+        t0 = time.perf_counter()
+        seq = [solve_one(m) for m in models]
+        t_seq = time.perf_counter() - t0
+        cell_steps = sum(m.cells * o.steps for m, (o, _) in zip(models, seq))
+        log(f"sequential: {t_seq:.2f} s, {cell_steps / t_seq / 1e9:.2f} Gcell/s "
+            f"(setup {phase['setup']:.2f} s, run {phase['run']:.2f} s, "
+            f"analysis {phase['analysis']:.2f} s)")
+        seq_phase = dict(phase)
 
     t0 = time.perf_counter()
     engines, ports = [], []
@@ -140,13 +143,16 @@ def sweep(args, log) -> int:
     s11b = [p.analyze(freqs).s11() for p in ports]
     t_bat = time.perf_counter() - t0
     bat_phase = {"setup": t1 - t0, "run": t2 - t1, "analysis": time.perf_counter() - t2}
+    cell_steps = sum(m.cells * o.steps for m, o in zip(models, outs))
     log(f"batched: {t_bat:.2f} s, {cell_steps / t_bat / 1e9:.2f} Gcell/s "
         f"(setup {bat_phase['setup']:.2f} s, run {bat_phase['run']:.2f} s, "
         f"analysis {bat_phase['analysis']:.2f} s)")
-    d = 0.0
-    same_steps = all(o.steps == so.steps for o, (so, _) in zip(outs, seq))
-    for a, (_, bb) in zip(s11b, seq):
-        d = max(d, max(abs(x - y) for x, y in zip(a, bb)))
+    d, same_steps = None, None
+    if seq is not None:
+        d = 0.0
+        same_steps = all(o.steps == so.steps for o, (so, _) in zip(outs, seq))
+        for a, (_, bb) in zip(s11b, seq):
+            d = max(d, max(abs(x - y) for x, y in zip(a, bb)))
     b.close()
     for e in engines:
         e.close()
@@ -160,12 +166,14 @@ def sweep(args, log) -> int:
                    "frequencies": 201, "end": "-40 dB energy decay",
                    "schedule": {"kernel": "packed rows" if packed else f"tiled, {tj} rows",
                                 "x_chunks": args.x_chunks}},
-        "sequential_solves_per_hr": round(n / t_seq * 3600.0, 1),
-        "speedup_vs_sequential": round(t_seq / t_bat, 3),
+        "sequential_solves_per_hr": round(n / t_seq * 3600.0, 1) if t_seq else None,
+        "speedup_vs_sequential": round(t_seq / t_bat, 3) if t_seq else None,
         "max_abs_dS11": d, "same_step_counts": same_steps,
-        "phases_s": {"sequential": {k: round(v, 3) for k, v in seq_phase.items()},
+        "phases_s": {"sequential": ({k: round(v, 3) for k, v in seq_phase.items()}
+                                    if seq_phase else None),
                      "batched": {k: round(v, 3) for k, v in bat_phase.items()}},
-        "cell_updates": cell_steps, "batched_s": round(t_bat, 3), "sequential_s": round(t_seq, 3),
+        "cell_updates": cell_steps, "batched_s": round(t_bat, 3),
+        "sequential_s": round(t_seq, 3) if t_seq else None,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -252,6 +260,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=4242, help="patch_sweep: seed")
     ap.add_argument("--max-steps", type=int, default=40000, help="patch_sweep: step limit")
     ap.add_argument("--model", default="", help="patch_earlystop: trained network (npz)")
+    ap.add_argument("--skip-sequential", action="store_true",
+                    help="patch_sweep: the batched leg only (for profiling)")
     ap.add_argument("--kernel", default="packed", choices=["packed", "tiled"],
                     help="patch workloads: fused kernel (packed short rows where nz <= 64)")
     ap.add_argument("--n", type=int, default=None, help="cells per axis")

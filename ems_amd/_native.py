@@ -127,6 +127,7 @@ ems_cn_param_count = _sig("ems_cn_param_count", _u64, _p)
 ems_cn_set_params = _sig("ems_cn_set_params", _i, _p, _fp, _u64)
 ems_cn_get_params = _sig("ems_cn_get_params", _i, _p, _fp, _u64)
 ems_cn_set_data = _sig("ems_cn_set_data", _i, _p, _fp, _fp, _i, _i)
+ems_cn_set_spectral = _sig("ems_cn_set_spectral", _i, _p, _fp, _i, _i, C.c_float)
 ems_cn_train_batch = _sig("ems_cn_train_batch", _i, _p, _ip, _i, _i, C.c_float, C.c_float, _fp,
                           _fp)
 ems_cn_forward = _sig("ems_cn_forward", _i, _p, _fp, _i, _i, _fp)

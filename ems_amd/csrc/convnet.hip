@@ -110,6 +110,36 @@ __global__ void __launch_bounds__(256) cn_loss(const float *__restrict__ out,
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// The spectral term: per sample and output channel, the error d over
+// samples [t0, t0 + len) adds w d^T K d to the loss and 2 w K d to its
+// gradient (K symmetric, len x len, row-major). A thread per (sample,
+// channel, t); per-block sums of d_t (K d)_t.
+__global__ void __launch_bounds__(256) cn_spec(const float *__restrict__ out,
+                                               const float *__restrict__ y,
+                                               const float *__restrict__ K, float *__restrict__ g,
+                                               int BC, int T, int t0, int len, float w,
+                                               float *__restrict__ partial) {
+  __shared__ float red[256];
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float s = 0.0f;
+  if (id < (size_t)BC * len) {
+    const int t = (int)(id % len);
+    const size_t row = (id / len) * T + t0;
+    const float *kr = K + (size_t)t * len;
+    float acc = 0.0f;
+    for (int u = 0; u < len; u++) acc += kr[u] * (out[row + u] - y[row + u]);
+    g[row + t] += 2.0f * w * acc;
+    s = (out[row + t] - y[row + t]) * acc;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = w * red[0];
+}
+
 // dz = g (1 - a^2).
 __global__ void __launch_bounds__(256) cn_act_bwd(const float *__restrict__ g,
                                                   const float *__restrict__ a,
@@ -255,6 +285,11 @@ struct ems_cn {
   std::vector<float *> h, a, z;     // per layer: input activation, tanh, pre-activation
   float *out = nullptr, *gout = nullptr, *gh = nullptr, *dz = nullptr, *part = nullptr;
   float *gpart = nullptr;  // per-block sums of g^2
+  float *spart = nullptr;  // per-block sums of the spectral loss
+  // spectral loss term (len x len kernel over samples [st0, st0 + slen))
+  float *dK = nullptr;
+  int st0 = 0, slen = 0;
+  float sw = 0.0f;
   int32_t *didx = nullptr;
   std::string err;
 };
@@ -279,7 +314,7 @@ int ensure_work(ems_cn *n, int B, int T) {
     if (q) (void)hipFree(q);
     q = nullptr;
   };
-  fr(n->hx), fr(n->hy), fr(n->out), fr(n->gout), fr(n->gh), fr(n->dz), fr(n->part);
+  fr(n->hx), fr(n->hy), fr(n->out), fr(n->gout), fr(n->gh), fr(n->dz), fr(n->part), fr(n->spart);
   // h[0] is the input batch (hx), freed above
   for (size_t l = 1; l < n->h.size(); l++) fr(n->h[l]);
   for (auto *&q : n->a) fr(q);
@@ -293,6 +328,7 @@ int ensure_work(ems_cn *n, int B, int T) {
   CCK(n, hipMalloc(&n->gh, need * C * sizeof(float)));
   CCK(n, hipMalloc(&n->dz, need * C * sizeof(float)));
   CCK(n, hipMalloc(&n->part, (blocks(need * n->cout) + 1) * sizeof(float)));
+  CCK(n, hipMalloc(&n->spart, (blocks(need * n->cout) + 1) * sizeof(float)));
   CCK(n, hipMalloc(&n->didx, (size_t)B * sizeof(int32_t)));
   n->h.assign(n->L, nullptr), n->a.assign(n->L, nullptr), n->z.assign(n->L, nullptr);
   for (int l = 0; l < n->L; l++) {
@@ -334,6 +370,14 @@ int backward(ems_cn *n, int B, int T, int t0, float *loss, double *gnorm = nullp
   const unsigned nb = blocks(no);
   hipLaunchKernelGGL(cn_loss, dim3(nb), dim3(256), 0, n->stream, n->out, n->hy, n->gout, no, T, t0,
                      inv, n->part);
+  unsigned nsb = 0;
+  if (n->sw != 0.0f) {
+    if (n->st0 + n->slen > T) return cfail(n, EMS_E_INVALID, "spectral term past the sequence end");
+    const int BC = B * n->cout;
+    nsb = blocks((size_t)BC * n->slen);
+    hipLaunchKernelGGL(cn_spec, dim3(nsb), dim3(256), 0, n->stream, n->out, n->hy, n->dK, n->gout,
+                       BC, T, n->st0, n->slen, n->sw / (float)BC, n->spart);
+  }
   for (int l = n->L - 1; l >= 0; l--) {
     const bool last = l == n->L - 1;
     const size_t nl = BT * n->lo[l];
@@ -361,16 +405,21 @@ int backward(ems_cn *n, int B, int T, int t0, float *loss, double *gnorm = nullp
   if (gnorm)
     hipLaunchKernelGGL(cn_sqsum, dim3(ng), dim3(256), 0, n->stream, n->g, n->np, n->gpart);
   CCK(n, hipGetLastError());
-  std::vector<float> part(nb), gp(gnorm ? ng : 0);
+  std::vector<float> part(nb), gp(gnorm ? ng : 0), sp(nsb);
   CCK(n, hipMemcpyAsync(part.data(), n->part, nb * sizeof(float), hipMemcpyDeviceToHost,
                         n->stream));
+  if (nsb)
+    CCK(n, hipMemcpyAsync(sp.data(), n->spart, nsb * sizeof(float), hipMemcpyDeviceToHost,
+                          n->stream));
   if (gnorm)
     CCK(n, hipMemcpyAsync(gp.data(), n->gpart, ng * sizeof(float), hipMemcpyDeviceToHost,
                           n->stream));
   CCK(n, hipStreamSynchronize(n->stream));
   double s = 0.0;
   for (float v : part) s += v;
-  if (loss) *loss = (float)(s * inv);
+  double ss = 0.0;
+  for (float v : sp) ss += v;
+  if (loss) *loss = (float)(s * inv + ss);
   if (gnorm) {
     double g2 = 0.0;
     for (float v : gp) g2 += v;
@@ -430,6 +479,7 @@ void ems_cn_destroy(ems_cn *n) {
   for (void *q : {(void *)n->p, (void *)n->g, (void *)n->m, (void *)n->v, (void *)n->pbf,
                   (void *)n->dx, (void *)n->dy, (void *)n->hx, (void *)n->hy, (void *)n->out,
                   (void *)n->gout, (void *)n->gh, (void *)n->dz, (void *)n->part, (void *)n->gpart,
+                  (void *)n->spart, (void *)n->dK,
                   (void *)n->didx})
     if (q) (void)hipFree(q);
   for (int l = 1; l < (int)n->h.size(); l++)
@@ -463,6 +513,19 @@ int ems_cn_get_params(ems_cn *n, float *p, uint64_t count) {
   CCK(n, hipStreamSynchronize(n->stream));
   CCK(n, hipMemcpyAsync(p, n->p, n->np * 4, hipMemcpyDeviceToHost, n->stream));
   CCK(n, hipStreamSynchronize(n->stream));
+  return EMS_OK;
+}
+
+int ems_cn_set_spectral(ems_cn *n, const float *K, int t0, int len, float weight) {
+  if (!n || t0 < 0 || len < 0 || (weight != 0.0f && (!K || len < 1))) return EMS_E_INVALID;
+  if (n->dK) CCK(n, hipFree(n->dK));
+  n->dK = nullptr, n->sw = 0.0f, n->st0 = 0, n->slen = 0;
+  if (weight == 0.0f) return EMS_OK;
+  const size_t bytes = (size_t)len * len * sizeof(float);
+  CCK(n, hipMalloc(&n->dK, bytes));
+  CCK(n, hipMemcpyAsync(n->dK, K, bytes, hipMemcpyHostToDevice, n->stream));
+  CCK(n, hipStreamSynchronize(n->stream));
+  n->sw = weight, n->st0 = t0, n->slen = len;
   return EMS_OK;
 }
 

@@ -29,6 +29,8 @@ from . import _native as nat
 from . import post
 from ._native import EmsError
 
+FREQS_1_3_GHZ = [1e9 + 2e9 * q / 200.0 for q in range(201)]
+
 
 class ConvNet:
     """An ems_cn network: `channels` wide, one layer per dilation (9 taps
@@ -100,6 +102,19 @@ class ConvNet:
                                            C.byref(gn)))
         self.last_grad_norm = gn.value
         return loss.value
+
+    def set_spectral(self, K, t0: int, weight: float) -> None:
+        """Adds weight * d^T K d per sample and output channel (averaged) to
+        the loss, d the error over samples [t0, t0 + len(K)); weight 0
+        removes it."""
+        if weight == 0.0:
+            self._check(nat.ems_cn_set_spectral(self._h, None, 0, 0, C.c_float(0.0)))
+            return
+        K = np.ascontiguousarray(np.asarray(K, dtype=np.float32))
+        if K.ndim != 2 or K.shape[0] != K.shape[1]:
+            raise EmsError(nat.EMS_E_INVALID, f"spectral matrix must be square, got {K.shape}")
+        self._check(nat.ems_cn_set_spectral(self._h, nat.ptr(K, C.c_float), int(t0), K.shape[0],
+                                            C.c_float(weight)))
 
     def forward(self, x) -> np.ndarray:
         x = self._seq(x, self.cin, "inputs")
@@ -184,6 +199,25 @@ class TraceFormat:
         return y
 
 
+def spectral_matrix(fmt: TraceFormat, src_full, dt: float, freqs=FREQS_1_3_GHZ) -> np.ndarray:
+    """K for ConvNet.set_spectral over the predicted samples [N, T): the
+    error's power at `freqs`, each frequency weighted by 1 / |S(f)|^2 (S the
+    excitation's spectrum, so an error counts as it would in S11, whose
+    incident wave is S/2), weights normalised to mean 1 and K to the scale
+    of the mean squared error (a white error gives about the same value):
+    K[t][u] = sum_f w_f cos(2 pi f (t - u) D dt) / (F (T - N))."""
+    D, n = fmt.decimation, fmt.T - fmt.N
+    S = np.abs(np.array(post.dft(0.0, dt, np.asarray(src_full, np.float64), freqs)))
+    w = 1.0 / np.maximum(S, 1e-30) ** 2
+    w /= w.mean()
+    tau = np.arange(n) * (D * dt)
+    diff = tau[:, None] - tau[None, :]
+    K = np.zeros((n, n))
+    for f, wf in zip(freqs, w):
+        K += wf * np.cos(2.0 * np.pi * f * diff)
+    return (K / (len(freqs) * n)).astype(np.float32)
+
+
 def residual_db(pred: np.ndarray, tail: int = 16) -> np.ndarray:
     """Port signal power (V^2 + (R I)^2) over the last `tail` samples of
     the predicted horizon relative to its peak, in dB, per sample."""
@@ -230,11 +264,9 @@ def truncated_port_result(fmt: TraceFormat, freqs, dt: float, v_obs, i_obs) -> p
 
 # -- solver traces, training and the early-stopped sweep ---------------------
 
-FREQS_1_3_GHZ = [1e9 + 2e9 * q / 200.0 for q in range(201)]
-
 # The network trained on the patch-antenna family (python -m ems_amd.learn
-# --out ...): 4096 solver traces of seeded patches, horizon 4096 steps,
-# decimation 8, 1024 observed steps.
+# --spectral 4 --out ...): 4096 solver traces of seeded patches, horizon
+# 4096 steps, decimation 8, 1024 observed steps, 40 000 Adam steps.
 PATCH_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "earlystop_patch.npz")
 
 
@@ -316,7 +348,7 @@ class EarlyStop:
 
 def train(es: EarlyStop, v, i, steps: int, batch: int = 32, lr: float = 6e-4,
           lr_end: float = 1e-5, warmup: int = 1000, seed: int = 0, log=None, every: int = 500,
-          decimated: bool = False):
+          decimated: bool = False, spectral=None):
     """Trains es.net on full traces [n][horizon]: linear warm-up, then a
     cosine-decayed learning rate; the loss counts the predicted part
     (t >= N) only. Gradients longer than three times the running mean norm
@@ -326,6 +358,9 @@ def train(es: EarlyStop, v, i, steps: int, batch: int = 32, lr: float = 6e-4,
     x = fmt.inputs(v, i, es.src, decimated)
     y = fmt.targets(v, i, decimated)
     es.net.set_data(x, y)
+    if spectral is not None:
+        K, weight = spectral
+        es.net.set_spectral(K, fmt.N, weight)
     rng = np.random.default_rng(seed)
     hist = []
     n = x.shape[0]
@@ -452,6 +487,8 @@ def main(argv=None) -> int:
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--lr", type=float, default=6e-4)
     ap.add_argument("--channels", type=int, default=64)
+    ap.add_argument("--spectral", type=float, default=0.0,
+                    help="weight of the 1-3 GHz spectral error term in the loss (0: off)")
     ap.add_argument("--out", default="")
     ap.add_argument("--traces", default="", help="save the traces (npz) here")
     ap.add_argument("--traces-in", default="",
@@ -492,7 +529,12 @@ def main(argv=None) -> int:
     if a.traces and not a.traces_in:
         np.savez(a.traces, v_train=vt, i_train=it, v_held=vh, i_held=ih, dt=dts)
     t1 = time.perf_counter()
-    hist = train(es, vt, it, a.steps, a.batch, a.lr, log=log, decimated=True)
+    spec = None
+    if a.spectral > 0.0:
+        w = Waveform.gaussian(2e9, 1e9)
+        full = [w.value((m + 0.5) * float(dts[0])) for m in range(a.horizon)]
+        spec = (spectral_matrix(fmt, full, float(dts[0])), a.spectral)
+    hist = train(es, vt, it, a.steps, a.batch, a.lr, log=log, decimated=True, spectral=spec)
     t_train = time.perf_counter() - t1
     log(f"trained {a.steps} steps in {t_train:.1f} s")
     rep = held_out_report(es, vh, ih, dth)

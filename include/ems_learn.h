@@ -19,7 +19,8 @@
  * Parameter vector (f32): for each layer l, the weights [out][in][tap]
  * then the biases [out].
  * Sequences: [sample][channel][time] f32, time fastest.
- * Loss: mean over samples, output channels and t >= t0 of (out - y)^2.
+ * Loss: mean over samples, output channels and t >= t0 of (out - y)^2,
+ * plus an optional quadratic (spectral) term, ems_cn_set_spectral.
  * Optimizer: Adam (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected),
  * optionally on a norm-clipped gradient.
  *
@@ -47,6 +48,13 @@ uint64_t ems_cn_param_count(const ems_cn *n);
 /* Replaces the parameters (and resets the optimizer state). */
 int ems_cn_set_params(ems_cn *n, const float *params, uint64_t count);
 int ems_cn_get_params(ems_cn *n, float *params, uint64_t count);
+
+/* Adds a quadratic term to the loss: for every sample and output channel,
+ * weight * d^T K d / (samples * channels), with d the error (output -
+ * target) over samples [t0, t0 + len) and K a symmetric len x len matrix
+ * (row-major, copied). A K built from cosines at chosen frequencies makes
+ * it the error of the spectrum there. weight 0 removes the term. */
+int ems_cn_set_spectral(ems_cn *n, const float *K, int t0, int len, float weight);
 
 /* Uploads a training set: x [count][cin][T], y [count][cout][T]. */
 int ems_cn_set_data(ems_cn *n, const float *x, const float *y, int count, int T);

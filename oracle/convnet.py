@@ -96,16 +96,27 @@ def forward(shape: Shape, p: np.ndarray, x: np.ndarray, keep: bool = False, roun
 
 
 def loss_and_grad(shape: Shape, p: np.ndarray, x: np.ndarray, y: np.ndarray, t0: int,
-                  round_w=True):
-    """Mean over samples, channels and t >= t0 of (out - y)^2, and its
-    gradient with respect to the parameter vector (the bf16 rounding is
-    treated as the identity, as in straight-through training)."""
+                  round_w=True, spectral=None):
+    """Mean over samples, channels and t >= t0 of (out - y)^2, plus with
+    spectral = (K, s0, w) the mean over samples and channels of
+    w d^T K d (d the error over [s0, s0 + len(K))), and the gradient with
+    respect to the parameter vector (the bf16 rounding is treated as the
+    identity, as in straight-through training)."""
     out, cache = forward(shape, p, x, keep=True, round_w=round_w)
     diff = out[:, :, t0:] - y[:, :, t0:]
     n = diff.size
     loss = (diff * diff).sum() / n
     g = np.zeros_like(out)
     g[:, :, t0:] = 2.0 * diff / n
+    if spectral is not None:
+        K, s0, w = spectral
+        K = np.asarray(K, np.float64)
+        m = K.shape[0]
+        d = out[:, :, s0:s0 + m] - y[:, :, s0:s0 + m]
+        Kd = np.einsum("tu,bcu->bct", K, d)
+        bc = d.shape[0] * d.shape[1]
+        loss = loss + w * float((d * Kd).sum()) / bc
+        g[:, :, s0:s0 + m] += 2.0 * w * Kd / bc
     grads = [None] * shape.L
     for l in range(shape.L - 1, -1, -1):
         h, wq, a = cache[l]

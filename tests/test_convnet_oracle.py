@@ -42,14 +42,18 @@ def test_causality():
     assert not np.allclose(y0[:, :, 25:], y1[:, :, 25:])
 
 
-@pytest.mark.parametrize("t0", [0, 7])
-def test_gradient_matches_finite_differences(t0):
+@pytest.mark.parametrize("t0,spec", [(0, False), (7, False), (7, True)])
+def test_gradient_matches_finite_differences(t0, spec):
     rng = np.random.default_rng(11)
     s = O.Shape(3, 2, 4, (1, 2, 3))
     p = rng.standard_normal(s.count) * 0.4
     x = rng.standard_normal((2, 3, 24))
     y = rng.standard_normal((2, 2, 24))
-    loss, g = O.loss_and_grad(s, p, x, y, t0, round_w=False)
+    spectral = None
+    if spec:
+        M = rng.standard_normal((10, 10))
+        spectral = (M @ M.T / 10.0, 12, 0.7)   # symmetric, over samples [12, 22)
+    loss, g = O.loss_and_grad(s, p, x, y, t0, round_w=False, spectral=spectral)
     h = 1e-6
     idx = rng.choice(s.count, 60, replace=False)
     # every bias and a few weights of each layer as well
@@ -58,13 +62,18 @@ def test_gradient_matches_finite_differences(t0):
         pp, pm = p.copy(), p.copy()
         pp[q] += h
         pm[q] -= h
-        lp, _ = O.loss_and_grad(s, pp, x, y, t0, round_w=False)
-        lm, _ = O.loss_and_grad(s, pm, x, y, t0, round_w=False)
+        lp, _ = O.loss_and_grad(s, pp, x, y, t0, round_w=False, spectral=spectral)
+        lm, _ = O.loss_and_grad(s, pm, x, y, t0, round_w=False, spectral=spectral)
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[q]) <= 1e-6 * max(1.0, abs(fd)), (q, fd, g[q])
-    # the loss is the mean over t >= t0 only
+    # the loss is the mean over t >= t0 only (plus the quadratic term)
     out = O.forward(s, p, x, round_w=False)
-    assert np.isclose(loss, np.mean((out[:, :, t0:] - y[:, :, t0:]) ** 2), rtol=1e-13)
+    want = np.mean((out[:, :, t0:] - y[:, :, t0:]) ** 2)
+    if spectral is not None:
+        K, s0, w = spectral
+        d = out[:, :, s0:s0 + 10] - y[:, :, s0:s0 + 10]
+        want += w * np.mean([dd @ K @ dd for dd in d.reshape(-1, 10)])
+    assert np.isclose(loss, want, rtol=1e-12)
 
 
 def test_bf16_rounding_matches_torch():

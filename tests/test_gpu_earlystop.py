@@ -4,7 +4,7 @@ seed the network never saw, run on the device to the horizon. The
 early-stopped spectra (1024 observed steps plus the predicted tail) must
 be far closer to the whole-trace S11 than the spectra of the observed
 steps alone. (Held-out figures of the committed network on 512 traces:
-median |dS11| 0.008, 90th percentile 0.026, largest 0.44; truncation:
+median |dS11| 0.006, 90th percentile 0.026, largest 0.44; truncation:
 median 1.01.)"""
 import numpy as np
 import pytest

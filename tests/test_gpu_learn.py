@@ -63,6 +63,29 @@ def test_gradient_matches_oracle(t0):
     n.close()
 
 
+def test_spectral_term_gradient_matches_oracle():
+    dil = (1, 2, 4)
+    n = _net(4, 2, 16, dil, 2)
+    s = O.Shape(4, 2, 16, dil)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((3, 4, 96)).astype(np.float32)
+    y = rng.standard_normal((3, 2, 96)).astype(np.float32)
+    M = rng.standard_normal((64, 64))
+    K = (M @ M.T / 64.0).astype(np.float32)
+    n.set_spectral(K, 32, 0.8)
+    p = n.params()
+    loss, g = n.loss_grad(x, y, 20)
+    lo, go = O.loss_and_grad(s, p.astype(np.float64), x.astype(np.float64), y.astype(np.float64),
+                             20, spectral=(K.astype(np.float64), 32, 0.8))
+    assert abs(loss - lo) <= 1e-4 * abs(lo), (loss, lo)
+    _close(g, go, 2e-4)
+    n.set_spectral(None, 0, 0.0)
+    l2, _ = n.loss_grad(x, y, 20)
+    l2o, _ = O.loss_and_grad(s, p.astype(np.float64), x.astype(np.float64), y.astype(np.float64), 20)
+    assert abs(l2 - l2o) <= 1e-5 * abs(l2o)
+    n.close()
+
+
 def test_adam_step_matches_oracle():
     dil = (1, 2, 4)
     n = _net(2, 2, 16, dil, 3)

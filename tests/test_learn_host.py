@@ -63,3 +63,25 @@ def test_residual_rule():
     pred = np.stack([np.stack([fast, fast]), np.stack([slow, slow])]).astype(np.float32)
     r = L.residual_db(pred)
     assert r[0] < -40.0 < r[1]
+
+
+def test_spectral_matrix_is_the_weighted_spectrum():
+    """d^T K d equals the weighted power of d's DFT at the chosen
+    frequencies, with weights 1/|S(f)|^2 normalised to mean 1."""
+    fmt = L.TraceFormat(512, 8, 128)
+    w = 0.5 * np.cos(2 * np.pi * 2e9 * (np.arange(512) * DT - 0.2e-9)) * \
+        np.exp(-((np.arange(512) * DT - 0.2e-9) / 0.05e-9) ** 2)
+    freqs = [1e9, 2e9, 3e9]
+    K = L.spectral_matrix(fmt, w, DT, freqs).astype(np.float64)
+    n = fmt.T - fmt.N
+    assert K.shape == (n, n) and np.allclose(K, K.T)
+    ev = np.linalg.eigvalsh(K)
+    assert ev.min() > -1e-6 * ev.max()  # positive semidefinite (to f32 rounding)
+    rng = np.random.default_rng(1)
+    d = rng.standard_normal(n)
+    S = np.abs(np.array(post.dft(0.0, DT, w, freqs)))
+    wt = 1.0 / S ** 2
+    wt /= wt.mean()
+    t = np.arange(n) * 8 * DT
+    want = sum(wf * abs(np.sum(d * np.exp(-2j * np.pi * f * t))) ** 2 for f, wf in zip(freqs, wt))
+    assert np.isclose(d @ K @ d, want / (len(freqs) * n), rtol=1e-5)
