@@ -254,8 +254,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cpml_box", choices=["cpml_box", "vacuum_box", "patch_sweep",
-                                                             "patch_earlystop"])
+    ap.add_argument("--workload", default="cpml_box",
+                    choices=["cpml_box", "vacuum_box", "patch_sweep", "patch_earlystop",
+                             "dispersive_box"])
     ap.add_argument("--models", type=int, default=256, help="patch_sweep: models")
     ap.add_argument("--seed", type=int, default=4242, help="patch_sweep: seed")
     ap.add_argument("--max-steps", type=int, default=40000, help="patch_sweep: step limit")
@@ -296,15 +297,23 @@ def main() -> int:
     from ems_amd import workloads as W
 
     t0 = time.perf_counter()
+    energy_every = 0
     if args.workload == "cpml_box":
         wl = W.cpml_box(n=args.n or 512)
+    elif args.workload == "dispersive_box":
+        # Plane waves and the TF/SF gaps run the unfused schedule; the run
+        # records the energy every 50 steps.
+        wl = W.dispersive_box()
+        args.unfused, energy_every = True, 50
     else:
         wl = W.vacuum_box(n=args.n or 50)
     steps = args.steps if args.steps is not None else wl.steps
     log(f"workload {wl.name}: {wl.grid.nodes()} nodes, built in {time.perf_counter()-t0:.1f} s")
 
     eng = E.Engine(wl.grid, wl.dt, wl.lanes)
+    t0 = time.perf_counter()
     probes = wl.build_ops(eng)
+    log(f"operators in {time.perf_counter()-t0:.1f} s")
     stream_gbs = eng.copy_bandwidth(1 << 30, 10)
     log(f"device copy bandwidth: {stream_gbs:.0f} GB/s (read + write)")
     tuned = None
@@ -318,15 +327,24 @@ def main() -> int:
         t0 = eng.timestep()
         best, trials = eng.autotune()
         tuned = {"mode": best[0], "tile_rows": best[1], "x_chunks": best[2],
-                 "trials_ms": {f"m{m}_t{t}_c{c}": round(ms, 4) for m, t, c, ms in trials}}
+                 "xcd_group": getattr(eng, "xcd_group", None),
+                 "trials_ms": {f"m{m}_t{t}_c{c}": round(ms, 4) for m, t, c, ms in trials},
+                 "xcd_trials_ms": {str(g): round(ms, 4) for g, ms in getattr(eng, "xcd_trials", [])}}
         log(f"autotune: mode {best[0]}, tile_rows {best[1]}, x_chunks {best[2]} ({best[3]:.3f} ms)")
         warm = max(0, args.warmup - (eng.timestep() - t0))
     eng.run(warm)
     eng.synchronize()
     log(f"setup + warm-up done ({eng.timestep()} steps)")
 
+    energies = []
     t0 = time.perf_counter()
-    eng.run(steps)
+    if energy_every:
+        for _ in range(steps // energy_every):
+            eng.run(energy_every - 1)
+            energies.append(eng.step_with_energy())
+        eng.run(steps % energy_every)
+    else:
+        eng.run(steps)
     eng.synchronize()
     wall = time.perf_counter() - t0
     value = wl.cells * steps / wall / 1e9
@@ -373,10 +391,13 @@ def main() -> int:
         "metric": "Gcell-updates/s", "value": round(value, 3), "unit": "Gcell/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": round(wall / steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded graded mesh, FR-4 blocks, Gaussian source)"
-        if args.workload == "cpml_box" else "synthetic (vacuum PEC box, Gaussian source)",
+        "data": {"cpml_box": "synthetic (seeded graded mesh, FR-4 blocks, Gaussian source)",
+                 "dispersive_box": "synthetic (seeded Lorentz/Drude spheres, conformal PEC "
+                                   "cylinder, TF/SF plane wave 1-10 GHz)"}.get(
+            args.workload, "synthetic (vacuum PEC box, Gaussian source)"),
         "config": {"workload": wl.name, "cells": list(n), "nodes": list(wl.grid.nodes()),
-                   "cpml_cells": 8 if args.workload == "cpml_box" else 0,
+                   "cpml_cells": {"cpml_box": 8, "dispersive_box": 10}.get(args.workload, 0),
+                   "energy_every": energy_every or None,
                    "probes": len(probes), "parallelism": "single GPU",
                    "schedule": "unfused" if args.unfused else "fused",
                    "mode": None if args.unfused else (tuned["mode"] if tuned else args.mode),

@@ -145,6 +145,7 @@ ems_gpu_synchronize = _sig("ems_gpu_synchronize", _i, _p)
 ems_gpu_time_kernels = _sig("ems_gpu_time_kernels", _i, _p, _u64, _dp)
 ems_gpu_traffic_model = _sig("ems_gpu_traffic_model", _i, _p, _dp)
 ems_gpu_set_schedule = _sig("ems_gpu_set_schedule", _i, _p, _i, _i, _i)
+ems_gpu_set_xcd_group = _sig("ems_gpu_set_xcd_group", _i, _p, _i)
 ems_gpu_copy_bandwidth = _sig("ems_gpu_copy_bandwidth", _i, _p, _u64, _i, _dp)
 
 # include/ems_host.h

@@ -25,7 +25,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import Dict, List, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -209,17 +209,22 @@ def _corner(u: float, v: float, r: float) -> float:
 
 
 def cylinder_z(grid, center: Sequence[float], radius: float, mats=None,
-               options: ConformalOptions = None, metal: str = "inside") -> ConformalPec:
-    """A circular PEC boundary along z (all z) of `radius` about (x, y) =
-    `center`: metal inside the circle (a rod) or outside it (a cavity).
+               options: ConformalOptions = None, metal: str = "inside",
+               z_nodes: Optional[Tuple[int, int]] = None) -> ConformalPec:
+    """A circular PEC boundary along z of `radius` about (x, y) = `center`:
+    metal inside the circle (a rod) or outside it (a cavity), over all z or
+    between node planes z_nodes = (k0, k1) (flat end caps on those planes).
     Edges wholly in metal become PEC in `mats` (CellMaterials, per-edge
     mask); cut x/y edges and the faces the circle cuts become conformal
     edges and faces. Faces normal to x (y) take the open fraction of the y
-    (x) edge they stand on (the geometry is z-invariant)."""
+    (x) edge they stand on (the geometry is z-invariant between the caps)."""
     if metal not in ("inside", "outside"):
         raise EmsError(nat.EMS_E_INVALID, f"metal must be inside or outside, not {metal!r}")
     xs, ys, zs = (np.asarray(grid.lines[a], np.float64) for a in range(3))
     nx, ny, nz = len(xs), len(ys), len(zs)
+    k0, k1 = (0, nz - 1) if z_nodes is None else (int(z_nodes[0]), int(z_nodes[1]))
+    if not 0 <= k0 < k1 <= nz - 1:
+        raise EmsError(nat.EMS_E_INVALID, f"cylinder z nodes {z_nodes} outside 0..{nz - 1}")
     xc, yc, r = float(center[0]), float(center[1]), float(radius)
     pec = ConformalPec(options)
     in_disk = ((xs[:, None] - xc) ** 2 + (ys[None, :] - yc) ** 2) < r * r
@@ -253,9 +258,9 @@ def cylinder_z(grid, center: Sequence[float], radius: float, mats=None,
     fz = np.clip(fz, 0.0, 1.0)
     if mats is not None:
         for i, j in zip(*np.nonzero(zmetal)):
-            for k in range(nz - 1):
+            for k in range(k0, k1):
                 mats.set_pec_edge(2, (int(i), int(j), k))
-    for k in range(nz):
+    for k in range(k0, k1 + 1):
         for axis, f_e in ((0, fx), (1, fy)):
             for (i, j) in zip(*np.nonzero(f_e < 1.0)):
                 f = float(f_e[i, j])
@@ -266,7 +271,7 @@ def cylinder_z(grid, center: Sequence[float], radius: float, mats=None,
                     pec.push_edge(ConformalEdge(axis, (int(i), int(j), k), f))
         for (i, j) in zip(*np.nonzero((fz < 1.0) & (fz > 0.0))):
             pec.push_face(ConformalFace(2, (int(i), int(j), k), float(fz[i, j])))
-        if k + 1 < nz:
+        if k < k1:
             # x-normal faces span y [j, j+1] at xs[i]: the y edge's fraction
             for (i, j) in zip(*np.nonzero((fy < 1.0) & (fy > 0.0))):
                 pec.push_face(ConformalFace(0, (int(i), int(j), k), float(fy[i, j])))

@@ -2940,6 +2940,14 @@ int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks) {
   return EMS_OK;
 }
 
+int ems_gpu_set_xcd_group(ems_gpu *h, int group) {
+  ALIVE(h);
+  if (group < 0 || group > 64) return fail(h, EMS_E_INVALID, "xcd group must be 0..64");
+  h->xcd_group = group;
+  drop_graphs(h);
+  return EMS_OK;
+}
+
 int ems_gpu_run(ems_gpu *h, uint64_t n) {
   ALIVE(h);
   return run_steps(h, n);

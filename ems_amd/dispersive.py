@@ -222,6 +222,59 @@ class Dispersive:
                             for p in poles:
                                 self.push(field, axis, node, p.scaled(share))
 
+    def fill_mask(self, grid, mask, field: Field, poles: Sequence[Pole]) -> None:
+        """fill_cells for an arbitrary set of cells (mask: bool, cells-shaped):
+        every electric edge gets the poles scaled by the share of its
+        averaging area inside the set, summed in fill_cells' order (so a box
+        mask gives fill_cells' entries bit for bit, in its order)."""
+        if field != Field.Electric:
+            raise EmsError(nat.EMS_E_UNSUPPORTED, "fill_mask: electric poles only")
+        for p in poles:
+            p.validate()
+        nc = grid.cells()
+        mask = np.asarray(mask, dtype=bool)
+        if mask.shape != tuple(nc):
+            raise EmsError(nat.EMS_E_INVALID, f"mask shape {mask.shape}, grid cells {nc}")
+        if not mask.any():
+            return
+        n = grid.nodes()
+        lo = [int(v.min()) for v in np.nonzero(mask)]
+        hi = [int(v.max()) + 1 for v in np.nonzero(mask)]
+        deltas = [np.array([grid.delta(a, q) for q in range(nc[a])]) for a in range(3)]
+        for axis in Axis:
+            ab, ac = axis.others()
+            a, b, c = int(axis), int(ab), int(ac)
+            # edge nodes: pa over the mask's cells along a, pb / pc over the
+            # nodes that touch its cells
+            pa = np.arange(lo[a], hi[a])
+            pb = np.arange(lo[b], min(hi[b] + 1, n[b]))
+            pc = np.arange(lo[c], min(hi[c] + 1, n[c]))
+            m = np.transpose(mask, (a, b, c))
+
+            def cell(d, q, t):
+                ok = (q >= 0) & (q < nc[t])
+                return np.where(ok, d[np.clip(q, 0, nc[t] - 1)], 0.0), ok
+            w = np.zeros((len(pb), len(pc)))
+            terms = []
+            for cbo in (-1, 0):
+                db_, okb = cell(deltas[b], pb + cbo, b)
+                for cco in (-1, 0):
+                    dc_, okc = cell(deltas[c], pc + cco, c)
+                    area = db_[:, None] * dc_[None, :]
+                    w = w + area
+                    cbi = np.clip(pb + cbo, 0, nc[b] - 1)
+                    cci = np.clip(pc + cco, 0, nc[c] - 1)
+                    inside = m[np.ix_(pa, cbi, cci)] & (okb[:, None] & okc[None, :])[None]
+                    terms.append(np.where(inside, area[None], 0.0))
+            w_in = ((terms[0] + terms[1]) + terms[2]) + terms[3]
+            share = np.where(w[None] > 0.0, w_in / np.where(w[None] > 0.0, w[None], 1.0), 0.0)
+            for ia, ib, ic in zip(*np.nonzero(share > 0.0)):
+                node = [0, 0, 0]
+                node[a], node[b], node[c] = int(pa[ia]), int(pb[ib]), int(pc[ic])
+                sh = float(share[ia, ib, ic])
+                for p in poles:
+                    self.push(field, axis, node, p.scaled(sh))
+
     def __len__(self) -> int:
         return len(self.entries)
 

@@ -956,7 +956,12 @@ class Engine:
         mode = int(fused) if not isinstance(fused, bool) else (1 if fused else 0)
         self._check(nat.ems_gpu_set_schedule(self._h, mode, int(tile_rows), int(x_chunks)))
 
-    def autotune(self, candidates=None, steps_each: int = 4):
+    def set_xcd_group(self, group: int) -> None:
+        """Tile rows per XCD visit in the fused kernels' block order (0:
+        plain order)."""
+        self._check(nat.ems_gpu_set_xcd_group(self._h, int(group)))
+
+    def autotune(self, candidates=None, steps_each: int = 4, xcd_groups=(1, 2, 4, 8)):
         """Times a few real steps of each fused tiling on this engine (they
         advance the simulation like any other steps) and keeps the fastest.
         Candidates are (mode, tile_rows, x_chunks), mode 1 with and 2
@@ -975,6 +980,19 @@ class Engine:
             results.append((m, t, c, ms))
         best = min(results, key=lambda r: r[3])
         self.set_schedule(best[0], best[1], best[2])
+        # then the XCD grouping of the tile order for that tiling
+        groups = []
+        for g in xcd_groups:
+            self.set_xcd_group(g)
+            self.time_kernels(1)
+            groups.append((g, self.time_kernels(steps_each)[0]))
+        self.xcd_trials = groups
+        if groups:
+            g_best = min(groups, key=lambda r: r[1])
+            self.set_xcd_group(g_best[0])
+            self.xcd_group = g_best[0]
+            if g_best[1] < best[3]:
+                best = (best[0], best[1], best[2], g_best[1])
         return best, results
 
     def copy_bandwidth(self, nbytes: int = 1 << 30, reps: int = 10) -> float:

@@ -239,22 +239,24 @@ class PlaneWave:
         xi = lambda r: _dot(k_hat, [r[0] - r_ref[0], r[1] - r_ref[1], r[2] - r_ref[2]])  # noqa: E731
         dual_line = lambda a, i: 0.5 * (float(L[a][i]) + float(L[a][i + 1]))  # noqa: E731
 
+        oth = ((1, 2), (2, 0), (0, 1))  # Axis.others() as ints
+
         def tf_h(d, q):
-            e_, f_ = (int(x) for x in Axis(d).others())
+            e_, f_ = oth[d]
             return lo[d] <= q[d] <= hi[d] and lo[e_] <= q[e_] < hi[e_] and lo[f_] <= q[f_] < hi[f_]
 
         def h_valid(d, q):
-            e_, f_ = (int(x) for x in Axis(d).others())
+            e_, f_ = oth[d]
             return all(0 <= q[a] < n[a] for a in range(3)) and q[e_] + 1 < n[e_] and q[f_] + 1 < n[f_]
 
         def e_pos(a, p):
-            b, c = (int(x) for x in Axis(a).others())
+            b, c = oth[a]
             r = [0.0] * 3
             r[a], r[b], r[c] = dual_line(a, p[a]), float(L[b][p[b]]), float(L[c][p[c]])
             return r
 
         def h_pos(d, q):
-            e_, f_ = (int(x) for x in Axis(d).others())
+            e_, f_ = oth[d]
             r = [0.0] * 3
             r[d], r[e_], r[f_] = float(L[d][q[d]]), dual_line(e_, q[e_]), dual_line(f_, q[f_])
             return r
@@ -262,12 +264,12 @@ class PlaneWave:
         flat = [lanes[l].reshape(-1) for l in range(12)]
         e_raw, h_raw = [], []
         for a in range(3):
-            b, c = (int(x) for x in Axis(a).others())
+            b, c = oth[a]
             for pa in range(lo[a], hi[a]):
                 for pb in range(lo[b], hi[b] + 1):
-                    for pc in range(lo[c], hi[c] + 1):
-                        if not (pb == lo[b] or pb == hi[b] or pc == lo[c] or pc == hi[c]):
-                            continue
+                    # the surface: every pc on the two b faces, else the c faces
+                    pcs = range(lo[c], hi[c] + 1) if pb in (lo[b], hi[b]) else (lo[c], hi[c])
+                    for pc in pcs:
                         p = [0, 0, 0]
                         p[a], p[b], p[c] = pa, pb, pc
                         ip = grid.index(p)
@@ -292,7 +294,7 @@ class PlaneWave:
                             iq = grid.index(q)
                             dbv = float(flat[9 + d][iq])
                             if mismatch is None and dbv != 0.0:
-                                de, df = (int(x) for x in Axis(d).others())
+                                de, df = oth[d]
                                 geo = grid.delta(de, q[de]) * grid.delta(df, q[df]) / \
                                     grid.dual_delta(d, q[d])
                                 got = (flat[6 + d][iq], flat[9 + d][iq])

@@ -12,6 +12,7 @@
 from __future__ import annotations
 
 from dataclasses import dataclass
+import math
 from typing import List, Optional
 
 import numpy as np
@@ -77,6 +78,82 @@ def cpml_box(n: int = 512, steps: int = 5000, seed: int = 2026, blocks: int = 64
         return [e.add_operator(E.Probe.voltage(a, p)) for p in pts for a in range(3)]
 
     return Workload(f"cpml_box_{n}", g, dt, lanes, ops, steps, g.num_cells())
+
+
+def dispersive_box(n=(384, 384, 256), steps: int = 8000, seed: int = 2027, pml: int = 10,
+                   cell: float = 1e-3, lorentz: int = 32, drude: int = 16,
+                   cylinder_cells: float = 40.0, radii=(6.0, 14.0)) -> Workload:
+    """Dispersive media and conformal metal: an n-cell uniform grid
+    (cell mm), `pml`-cell CPML on all faces, `lorentz` seeded spheres of a
+    single-pole Lorentz medium (eps_inf 2, delta-eps 3, f0 8 GHz, damping
+    0.5 GHz: omega_p = sqrt(3) omega_0, gamma = 2 * 2 pi 0.5 GHz) and
+    `drude` of a Drude medium (f_p 20 GHz, gamma 2 pi 1 GHz), radii in
+    `radii` cells, outside a conformal PEC cylinder of radius `cylinder_cells`
+    along z at the centre (flat caps inside the TF/SF box, whose surface
+    stays in vacuum); an x-polarised TF/SF plane wave travelling +y,
+    Gaussian 1-10 GHz (gaussian(5.5e9, 4.5e9)); energy every 50 steps."""
+    from . import conformal as CF
+    from . import dispersive as DP
+    from . import planewave as PW
+    rng = np.random.default_rng(seed)
+    g = E.Grid.uniform(list(n), [cell] * 3)
+    m = E.CellMaterials.vacuum(g)
+    nc = g.cells()
+    c = [q / 2.0 for q in nc]
+    centres = np.indices(nc, dtype=np.float32)
+    centres += 0.5
+    lor_mask = np.zeros(nc, dtype=bool)
+    dru_mask = np.zeros(nc, dtype=bool)
+    lo_c = pml + 6
+    placed = tries = 0
+    while placed < lorentz + drude:
+        tries += 1
+        if tries > 100000:
+            raise ValueError(f"dispersive_box: cannot place {lorentz + drude} spheres of radius "
+                             f"{radii} in {n} cells off a {cylinder_cells}-cell cylinder")
+        r = float(rng.uniform(radii[0], radii[1]))
+        p = [float(rng.uniform(lo_c + r, nc[a] - lo_c - r)) for a in range(3)]
+        if math.hypot(p[0] - c[0], p[1] - c[1]) < cylinder_cells + r + 2.0:
+            continue  # keep the spheres off the cylinder
+        if any(p[a] - r < pml + 6 or p[a] + r > nc[a] - pml - 6 for a in range(3)):
+            continue
+        lo = [max(int(p[a] - r) - 1, 0) for a in range(3)]
+        hi = [min(int(p[a] + r) + 2, nc[a]) for a in range(3)]
+        sub = tuple(slice(lo[a], hi[a]) for a in range(3))
+        d2 = sum((centres[a][sub] - p[a]) ** 2 for a in range(3))
+        target = lor_mask if placed < lorentz else dru_mask
+        target[sub] |= d2 < r * r
+        placed += 1
+    lor_mask &= ~dru_mask
+    m_lor = E.Material.dielectric(2.0)
+    idx = np.nonzero(lor_mask)
+    for q in zip(*idx):
+        m.set(q, m_lor)
+    pw_lo, pw_hi = pml + 4, [q - pml - 4 for q in g.nodes()]
+    pec = CF.cylinder_z(g, (c[0] * cell, c[1] * cell), cylinder_cells * cell, mats=m,
+                        metal="inside", z_nodes=(pw_lo + 4, pw_hi[2] - 4))
+    dt = E.TimeStep.courant(0.99).resolve(g)
+    lanes = E.coefficients(g, m, [E.FaceCondition.Pec] * 6, dt)
+    w0 = 2.0 * math.pi * 8e9
+    lor = DP.Pole.lorentz(math.sqrt(3.0) * w0, w0, 2.0 * 2.0 * math.pi * 0.5e9)
+    dru = DP.Pole.drude(2.0 * math.pi * 20e9, 2.0 * math.pi * 1e9)
+    wave = E.Waveform.gaussian(5.5e9, 4.5e9)
+
+    def ops(e):
+        pec.apply(e)
+        d = DP.Dispersive.new()
+        d.fill_mask(g, lor_mask, DP.Field.Electric, [lor])
+        d.fill_mask(g, dru_mask, DP.Field.Electric, [dru])
+        e.add_operator(E.CpmlBoundary.new(e, [(f, E.Cpml.with_cells(pml)) for f in E.Face]))
+        e.add_operator(d)
+        spec = PW.PlaneWaveSpec.new([pw_lo] * 3, pw_hi, [0.0, 1.0, 0.0], [1.0, 0.0, 0.0])
+        e.add_operator(PW.PlaneWave.new(e, spec, lambda t: wave.value(t)))
+        # between the cylinder and the far face of the TF/SF box
+        y = int((c[1] + cylinder_cells + pw_hi[1]) / 2)
+        return [e.add_operator(E.Probe.voltage(E.Axis.X, [int(c[0]), y, int(c[2])]))]
+
+    return Workload(f"dispersive_box_{n[0]}x{n[1]}x{n[2]}", g, dt, lanes, ops, steps,
+                    g.num_cells())
 
 
 @dataclass

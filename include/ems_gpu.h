@@ -275,6 +275,11 @@ double ems_gpu_dt(const ems_gpu *h);
  * E updates (H sources) always run unfused. */
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks);
 
+/* Tile order of the fused kernels (no reference counterpart): groups of
+ * `group` consecutive tile rows go round-robin to the 8 XCDs (their L2s),
+ * 0 = plain block order. Default 2. */
+int ems_gpu_set_xcd_group(ems_gpu *h, int group);
+
 /* Waits for all queued device work. */
 int ems_gpu_synchronize(ems_gpu *h);
 

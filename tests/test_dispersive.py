@@ -228,3 +228,41 @@ def test_drude_half_space_reflection(scheme):
         ec, ed = max(ec, abs(g - gc)), max(ed, abs(g - gd))
     assert ec < 3e-3, ec
     assert ed < 5e-4, ed
+
+
+def test_fill_mask_matches_fill_cells_on_a_box():
+    """fill_mask over a box of cells gives fill_cells' entries bit for bit,
+    in its order, on a graded mesh; a sphere's edges get shares in (0, 1]."""
+    lines = []
+    for nn in (9, 8, 7):
+        x, l = 0.0, []
+        for q in range(nn + 1):
+            l.append(x)
+            x += 1e-3 * (1.0 + 0.3 * math.sin(1.3 * q))
+        lines.append(l)
+    g = E.Grid(lines)
+    blk = [(2, 7), (1, 5), (3, 6)]
+    poles = [Pole.lorentz(3e10, 2e10, 1e9), Pole.drude(5e10, 2e9)]
+    a = Dispersive.new()
+    a.fill_cells(g, blk, Field.Electric, poles)
+    m = np.zeros(g.cells(), dtype=bool)
+    m[2:7, 1:5, 3:6] = True
+    b = Dispersive.new()
+    b.fill_mask(g, m, Field.Electric, poles)
+    assert len(a.entries) == len(b.entries) > 0
+    for (fa, xa, na, pa), (fb, xb, nb, pb) in zip(a.entries, b.entries):
+        assert (fa, xa, na, pa.kind) == (fb, xb, nb, pb.kind)
+        assert np.array_equal(np.float64(pa.p), np.float64(pb.p))
+    c = np.indices(g.cells()).astype(float) + 0.5
+    sphere = ((c[0] - 4.5) ** 2 + (c[1] - 4.0) ** 2 + (c[2] - 3.5) ** 2) < 2.6 ** 2
+    d = Dispersive.new()
+    d.fill_mask(g, sphere, Field.Electric, [Pole.drude(5e10, 2e9)])
+    shares = [e[3].p[0] / 5e10 for e in d.entries]
+    assert len(shares) > 0 and min(shares) > 0.0 and max(shares) <= 1.0 + 1e-12
+
+
+def test_dispersive_box_rejects_impossible_sphere_placement():
+    from ems_amd import workloads as W
+    with pytest.raises(ValueError):
+        W.dispersive_box((40, 40, 32), pml=6, lorentz=2, drude=1, cylinder_cells=5.0,
+                         radii=(2.5, 3.5))
