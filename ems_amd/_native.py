@@ -113,6 +113,8 @@ ems_gpu_dispersive_state = _sig("ems_gpu_dispersive_state", _i, _p, _i, _i, _dp,
 ems_gpu_add_planewave = _sig("ems_gpu_add_planewave", _i, _p, _p, C.POINTER(_i))
 ems_gpu_planewave_table = _sig("ems_gpu_planewave_table", _i, _p, _i, _dp, _u64)
 ems_gpu_planewave_aux = _sig("ems_gpu_planewave_aux", _i, _p, _i, _dp, _dp)
+ems_gpu_add_merged = _sig("ems_gpu_add_merged", _i, _p, _p, C.POINTER(_i))
+ems_gpu_merged_state = _sig("ems_gpu_merged_state", _i, _p, _i, _dp, _dp)
 ems_gpu_batch_create = _sig("ems_gpu_batch_create", _i, C.POINTER(_p), _i, C.POINTER(_p))
 ems_gpu_batch_destroy = _sig("ems_gpu_batch_destroy", None, _p)
 ems_gpu_batch_last_error = _sig("ems_gpu_batch_last_error", C.c_char_p, _p)

@@ -1,4 +1,4 @@
-"""Conformal (sub-cell) PEC by coefficient patches (src/engine/conformal.rs).
+"""Conformal (sub-cell) PEC (src/engine/conformal.rs).
 
 The Dey-Mittra scheme in the integral form of the engine is a change of
 coefficients (conformal.rs:1-30): a cut E edge with open length fraction
@@ -6,30 +6,35 @@ f_e gets cb -> f_e cb, a cut face with open area fraction f_A gets
 db -> db / f_A; ca and da stay. The patches are written into the
 coefficient lanes, so the device step kernels run them unchanged.
 
-Arithmetic as ConformalPec::apply_known (conformal.rs:786-793: the edge
-scaling, (cb as f64 * f_e) as f32) and the face loop of construct
-(conformal.rs:1542-1566: (db as f64 / f_A) as f32, f_A below the merge
-threshold raised to it, the threshold at least 1e-6).
+A face below the merge threshold loses its own current (conformal.rs:
+34-95): interpolated from neighbouring faces at its open-area centroid, or
+merged with a neighbour where no stencil exists, each with a deviation
+current; the reduced currents are updated after every H half-step by the
+MergedFaces operator (ems_amd/merged.py builds it, the device runs it). A
+small face whose two cells pass the cell-wise stability bound keeps its
+current at its true area (keep_stable); one without an open neighbour is
+raised to the threshold.
 
-Scope: every cut face keeps its own current. The reference instead takes
-the current of a small face (f_A below the threshold) from its neighbours
-(interpolation with deviation currents, or merging, solved per step with
-envelope-Cholesky blocks by its MergedFaces operator, conformal.rs:2313)
-wherever a stencil exists, and raises f_A to the threshold only for faces
-without one. Here every small face is raised to the threshold (the
-reference's fallback), or kept at its true area with merge_below = 0
-(plain Dey-Mittra, which needs a shorter time step). The stability check
-(conformal.rs:1698-1760, power iteration) is not run.
+Arithmetic as ConformalPec::apply_known (conformal.rs:770-954): the edge
+scaling (cb as f64 * f_e) as f32 in edge order, the stable faces from the
+scaled lanes, construct at the threshold, then the patch on top of the
+scaled lanes (changed, :2704-2715).
+
+Scope: Cartesian grids; no thin PEC sheets; no reduced currents in
+absorbing layers (rejected); the power-iteration stability estimate
+(conformal.rs:2858-2925) and the threshold rounds it drives are not run:
+ConformalOptions.iterations is 0, the report's stability NaN.
 """
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
 from . import _native as nat
+from . import merged as M
 from ._native import EmsError
 
 CB, DB = 3, 9  # first cb and db lanes (x, y, z follow)
@@ -55,32 +60,42 @@ class ConformalFace:
 
 @dataclass
 class ConformalOptions:
-    """The merge threshold of ConformalOptions (conformal.rs:283-331):
-    faces below it are raised to it."""
+    """ConformalOptions (conformal.rs:283-331). iterations must be 0 (the
+    stability estimate is not run)."""
     merge_below: float = 0.5
+    max_stability: float = 0.999
+    iterations: int = 0
+    max_rounds: int = 4
+    deviation_area: float = 0.5
+    keep_stable: bool = True
+    keep_step: Optional[float] = None
 
 
 @dataclass
 class ConformalReport:
-    """Counts of ConformalReport (conformal.rs:350-424) that apply here."""
+    """ConformalReport (conformal.rs:350-424), the fields that apply here,
+    plus pec_edges (edges made PEC by a zero fraction)."""
     edges: int = 0
     faces: int = 0
+    interpolated: int = 0
+    hosts: int = 0
+    merged: int = 0
+    groups: int = 0
     floored: int = 0
+    kept: int = 0
+    bounded: bool = False
+    merge_below: float = 0.5
+    stability: float = float("nan")
+    verified: bool = True
     pec_edges: int = 0
 
-
-def _others(a: int) -> Tuple[int, int]:
-    return ((1, 2), (0, 2), (0, 1))[a]
-
-
-def _face_edges(a: int, p) -> List[Tuple[int, Tuple[int, int, int]]]:
-    """The four edges around the face normal to a with lowest corner p."""
-    b, c = _others(a)
-    pb = list(p)
-    pb[b] += 1
-    pc = list(p)
-    pc[c] += 1
-    return [(b, tuple(p)), (b, tuple(pc)), (c, tuple(p)), (c, tuple(pb))]
+    def warning(self) -> Optional[str]:
+        """conformal.rs:425-434."""
+        if self.verified:
+            return None
+        return (f"conformal PEC: estimated stability factor {self.stability:.4f} (> 1 is "
+                f"unstable) with merge threshold {self.merge_below:.3f}; reduce the time step "
+                "factor")
 
 
 class ConformalPec:
@@ -90,68 +105,105 @@ class ConformalPec:
         self.options = options or ConformalOptions()
         self.edges: List[ConformalEdge] = []
         self.faces: List[ConformalFace] = []
+        self.centroids: List[Optional[Tuple[float, float]]] = []
+        self.walls: List[Tuple[int, Tuple[int, int, int]]] = []
+        self.absorbing = [0] * 6
 
     def push_edge(self, e: ConformalEdge) -> None:
         self.edges.append(e)
 
     def push_face(self, f: ConformalFace) -> None:
+        """A cut face without a centroid: small, it is merged, not
+        interpolated (conformal.rs:518-524)."""
         self.faces.append(f)
+        self.centroids.append(None)
+
+    def push_face_with_centroid(self, f: ConformalFace, centroid: Sequence[float]) -> None:
+        """A cut face with the centroid of its open area, [t_b, t_c] in face
+        sides along (b, c) = (next, prev) of its axis (conformal.rs:527-533)."""
+        self.faces.append(f)
+        self.centroids.append((float(centroid[0]), float(centroid[1])))
+
+    def push_wall(self, axis: int, node) -> None:
+        """An edge closed to interpolation and merging, not PEC
+        (conformal.rs:509-515)."""
+        self.walls.append((int(axis), tuple(int(x) for x in node)))
+
+    def set_absorbing_layers(self, layers) -> None:
+        """(face index 0..5 = x-, x+, y-, y+, z-, z+, cells) pairs
+        (conformal.rs:537-541)."""
+        for f, cells in layers:
+            self.absorbing[f] = max(self.absorbing[f], int(cells))
 
     def validate(self, n) -> None:
-        """conformal.rs:1019-1035: fractions in [0, 1], edges and faces
-        inside the grid."""
+        """conformal.rs:1017-1036: fractions in [0, 1], edges and faces
+        inside the grid, centroids in [0, 1]^2."""
         for e in self.edges:
             a = e.axis
             if not (0.0 <= e.fraction <= 1.0) or any(e.node[d] >= n[d] for d in range(3)) \
                     or e.node[a] + 1 >= n[a]:
                 raise EmsError(nat.EMS_E_INVALID, f"invalid conformal edge {e}")
-        for f in self.faces:
-            b, c = _others(f.axis)
-            if not (0.0 <= f.fraction <= 1.0) or any(f.node[d] >= n[d] for d in range(3)) \
-                    or f.node[b] + 1 >= n[b] or f.node[c] + 1 >= n[c]:
+        for f, c in zip(self.faces, self.centroids):
+            bad_c = c is not None and not all(0.0 <= t <= 1.0 for t in c)
+            if not (0.0 <= f.fraction <= 1.0) or not M.face_exists(f.axis, f.node, n) or bad_c:
                 raise EmsError(nat.EMS_E_INVALID, f"invalid conformal face {f}")
 
-    def patched(self, lanes: np.ndarray) -> Tuple[Dict[int, np.ndarray], ConformalReport]:
-        """The changed lanes (index -> new dense lane) for coefficient lanes
-        `lanes` (12, nx, ny, nz), and the report."""
-        n = lanes.shape[1:]
+    def build(self, lanes: np.ndarray, grid, dt: float):
+        """The changed lanes (index -> new dense lane), the report and the
+        MergedFaces operator (None without reduced currents) for the
+        coefficient lanes `lanes` (12, nx, ny, nz) of a grid and time step."""
+        n = tuple(lanes.shape[1:])
         self.validate(n)
-        out: Dict[int, np.ndarray] = {}
-
-        def lane(l):
-            if l not in out:
-                out[l] = lanes[l].copy()
-            return out[l]
-
+        opt = self.options
+        if opt.iterations:
+            raise EmsError(nat.EMS_E_INVALID,
+                           "the conformal stability estimate is not available: iterations = 0")
+        flat = lanes.reshape(12, -1)
+        co = M.Overlay(flat)
         rep = ConformalReport(edges=len(self.edges))
         for e in self.edges:
-            cb = lane(CB + e.axis)
-            cb[e.node] = np.float32(float(cb[e.node]) * e.fraction)
+            k = M.index(n, e.node)
+            co[CB + e.axis, k] = np.float32(float(co[CB + e.axis, k]) * e.fraction)
             if e.fraction == 0.0:
                 rep.pec_edges += 1
-        # whether an edge is open after the edge scaling (cb > 0)
-        cbs = [out.get(CB + a, lanes[CB + a]) for a in range(3)]
-        tau = min(max(self.options.merge_below, 0.0), 1.0)
-        for f in self.faces:
-            if not any(cbs[b][q] > 0.0 for b, q in _face_edges(f.axis, f.node)):
-                continue  # not live: every edge closed
-            fa = f.fraction
-            if fa < tau:
-                fa = max(tau, 1e-6)
-                rep.floored += 1
-            if fa < 1.0:
-                db = lane(DB + f.axis)
-                db[f.node] = np.float32(float(db[f.node]) / fa)
-            rep.faces += 1
-        return out, rep
+        if opt.keep_stable:
+            at = opt.keep_step if (opt.keep_step is not None and 0.0 < opt.keep_step < dt) else None
+            kept, bounded = M.stable_faces(self.faces, co, grid, dt, opt.max_stability, at,
+                                           self.absorbing)
+        else:
+            kept, bounded = set(), False
+        tau = min(max(opt.merge_below, 0.0), 1.0)
+        b = M.construct(self.faces, self.centroids, self.edges, self.walls, co, grid, dt, tau,
+                        kept, min(max(opt.deviation_area, 0.0), 1.0), self.absorbing)
+        if b.driven_in_layer:
+            raise EmsError(nat.EMS_E_INVALID,
+                           "reduced currents of small cut faces inside an absorbing layer are not "
+                           "supported")
+        rep.faces, rep.interpolated, rep.hosts = b.faces, b.interpolated, b.hosts
+        rep.merged, rep.groups, rep.floored, rep.kept = b.merged_faces, b.groups, b.floored, b.kept
+        rep.bounded, rep.merge_below = bounded, tau
+        for (lane, k), v in b.patch.items():
+            co[lane, k] = v
+        # the values that differ from the lanes (bit patterns), by lane
+        out: Dict[int, np.ndarray] = {}
+        for (lane, k), v in co.over.items():
+            if np.float32(v).view(np.uint32) != flat[lane, k].view(np.uint32):
+                if lane not in out:
+                    out[lane] = flat[lane].copy()
+                out[lane][k] = v
+        out = {l: v.reshape(n) for l, v in out.items()}
+        return out, rep, (b.merged if len(b.merged) else None)
 
     def apply(self, engine) -> ConformalReport:
-        """Patches the engine's coefficient lanes (before operators that read
-        them are added, as ConformalPec::apply on a fresh engine)."""
+        """ConformalPec::apply (conformal.rs:755): patches the engine's
+        coefficient lanes and registers the reduced currents' operator
+        (before operators that read the lanes are added)."""
         if getattr(engine, "_dense_lanes", None) is None:
             raise EmsError(nat.EMS_E_INVALID, "conformal PEC needs the host lanes")
-        changed, rep = self.patched(engine._dense_lanes)
+        changed, rep, op = self.build(engine._dense_lanes, engine.grid, engine.dt)
         engine.patch_lanes(changed)
+        if op is not None:
+            engine.add_operator(op)
         return rep
 
 
@@ -208,16 +260,62 @@ def _corner(u: float, v: float, r: float) -> float:
     return total
 
 
+_GL = np.polynomial.legendre.leggauss(48)
+
+
+def _disk_rect_moments(x0, x1, y0, y1, xc, yc, r) -> Tuple[float, float]:
+    """First moments (integrals of x and of y) of the disk-rectangle
+    intersection, by Gauss-Legendre quadrature over the x intervals on which
+    the chord's clipping to [y0, y1] keeps one form."""
+    cuts = [x0, x1, xc - r, xc + r]
+    for y in (y0, y1):
+        d = r * r - (y - yc) ** 2
+        if d > 0.0:
+            cuts += [xc - math.sqrt(d), xc + math.sqrt(d)]
+    cuts = sorted(min(max(c, x0), x1) for c in cuts)
+    mx = my = 0.0
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        if b <= a:
+            continue
+        x = 0.5 * (a + b) + 0.5 * (b - a) * _GL[0]
+        h = np.sqrt(np.clip(r * r - (x - xc) ** 2, 0.0, None))
+        lo = np.maximum(y0, yc - h)
+        hi = np.minimum(y1, yc + h)
+        ln = np.clip(hi - lo, 0.0, None)
+        w = 0.5 * (b - a) * _GL[1]
+        mx += float(np.sum(w * x * ln))
+        my += float(np.sum(w * 0.5 * (hi * hi - lo * lo) * (ln > 0)))
+    return mx, my
+
+
+def _open_interval_centroid(a0, a1, c_lo, c_hi, metal_inside: bool) -> float:
+    """Centroid (as a fraction of [a0, a1]) of the open part of the segment,
+    metal on [c_lo, c_hi] (inside) or outside it."""
+    if metal_inside:
+        pieces = [(a0, min(a1, c_lo)), (max(a0, c_hi), a1)]
+    else:
+        pieces = [(max(a0, c_lo), min(a1, c_hi))]
+    num = den = 0.0
+    for u, v in pieces:
+        if v > u:
+            num += 0.5 * (u + v) * (v - u)
+            den += v - u
+    t = (num / den - a0) / (a1 - a0) if den > 0.0 else 0.5
+    return min(max(t, 0.0), 1.0)
+
+
 def cylinder_z(grid, center: Sequence[float], radius: float, mats=None,
                options: ConformalOptions = None, metal: str = "inside",
-               z_nodes: Optional[Tuple[int, int]] = None) -> ConformalPec:
+               z_nodes: Optional[Tuple[int, int]] = None, centroids: bool = True) -> ConformalPec:
     """A circular PEC boundary along z of `radius` about (x, y) = `center`:
     metal inside the circle (a rod) or outside it (a cavity), over all z or
     between node planes z_nodes = (k0, k1) (flat end caps on those planes).
     Edges wholly in metal become PEC in `mats` (CellMaterials, per-edge
     mask); cut x/y edges and the faces the circle cuts become conformal
     edges and faces. Faces normal to x (y) take the open fraction of the y
-    (x) edge they stand on (the geometry is z-invariant between the caps)."""
+    (x) edge they stand on (the geometry is z-invariant between the caps).
+    With `centroids` every cut face carries the centroid of its open area
+    (small ones are then interpolated, else merged)."""
     if metal not in ("inside", "outside"):
         raise EmsError(nat.EMS_E_INVALID, f"metal must be inside or outside, not {metal!r}")
     xs, ys, zs = (np.asarray(grid.lines[a], np.float64) for a in range(3))
@@ -253,9 +351,46 @@ def cylinder_z(grid, center: Sequence[float], radius: float, mats=None,
         fx, fy, fz, zmetal = 1.0 - gx, 1.0 - gy, 1.0 - gz, in_disk
     else:
         fx, fy, fz, zmetal = gx, gy, gz, ~in_disk
-    fx = np.clip(fx, 0.0, 1.0)
-    fy = np.clip(fy, 0.0, 1.0)
-    fz = np.clip(fz, 0.0, 1.0)
+    # fractions within 1e-9 of 0 or 1 are rounding of whole cells
+    fx, fy, fz = (np.where(f < 1e-9, 0.0, np.where(f > 1.0 - 1e-9, 1.0, f)) for f in (fx, fy, fz))
+    inside = metal == "inside"
+
+    def chord(c):
+        d = r * r - c * c
+        s = math.sqrt(d) if d > 0.0 else 0.0
+        return -s, s
+    cz: Dict[Tuple[int, int], Tuple[float, float]] = {}
+    cx: Dict[Tuple[int, int], float] = {}
+    cy: Dict[Tuple[int, int], float] = {}
+    if centroids:
+        for (i, j) in zip(*np.nonzero((fz < 1.0) & (fz > 0.0))):
+            x0, x1, y0, y1 = xs[i], xs[i + 1], ys[j], ys[j + 1]
+            area = (x1 - x0) * (y1 - y0)
+            a_in = _disk_rect_area_exact(x0, x1, y0, y1, xc, yc, r)
+            mx, my = _disk_rect_moments(x0, x1, y0, y1, xc, yc, r)
+            if inside:
+                a_open = area - a_in
+                mx = 0.5 * (x0 + x1) * area - mx
+                my = 0.5 * (y0 + y1) * area - my
+            else:
+                a_open = a_in
+            t = ((mx / a_open - x0) / (x1 - x0), (my / a_open - y0) / (y1 - y0))
+            cz[(int(i), int(j))] = tuple(min(max(v, 0.0), 1.0) for v in t)
+        for (i, j) in zip(*np.nonzero((fy < 1.0) & (fy > 0.0))):
+            lo, hi = chord(xs[i] - xc)
+            cy[(int(i), int(j))] = _open_interval_centroid(ys[j], ys[j + 1], yc + lo, yc + hi,
+                                                           inside)
+        for (i, j) in zip(*np.nonzero((fx < 1.0) & (fx > 0.0))):
+            lo, hi = chord(ys[j] - yc)
+            cx[(int(i), int(j))] = _open_interval_centroid(xs[i], xs[i + 1], xc + lo, xc + hi,
+                                                           inside)
+
+    def push(axis, node, frac, cen):
+        f = ConformalFace(axis, node, frac)
+        if centroids:
+            pec.push_face_with_centroid(f, cen)
+        else:
+            pec.push_face(f)
     if mats is not None:
         for i, j in zip(*np.nonzero(zmetal)):
             for k in range(k0, k1):
@@ -270,11 +405,12 @@ def cylinder_z(grid, center: Sequence[float], radius: float, mats=None,
                 else:
                     pec.push_edge(ConformalEdge(axis, (int(i), int(j), k), f))
         for (i, j) in zip(*np.nonzero((fz < 1.0) & (fz > 0.0))):
-            pec.push_face(ConformalFace(2, (int(i), int(j), k), float(fz[i, j])))
+            push(2, (int(i), int(j), k), float(fz[i, j]), cz.get((int(i), int(j))))
         if k < k1:
-            # x-normal faces span y [j, j+1] at xs[i]: the y edge's fraction
+            # x-normal faces span y [j, j+1] at xs[i]: the y edge's fraction;
+            # centroids along (b, c) = (y, z) for x faces, (z, x) for y faces
             for (i, j) in zip(*np.nonzero((fy < 1.0) & (fy > 0.0))):
-                pec.push_face(ConformalFace(0, (int(i), int(j), k), float(fy[i, j])))
+                push(0, (int(i), int(j), k), float(fy[i, j]), (cy.get((int(i), int(j))), 0.5))
             for (i, j) in zip(*np.nonzero((fx < 1.0) & (fx > 0.0))):
-                pec.push_face(ConformalFace(1, (int(i), int(j), k), float(fx[i, j])))
+                push(1, (int(i), int(j), k), float(fx[i, j]), (0.5, cx.get((int(i), int(j)))))
     return pec

@@ -32,6 +32,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/ems_gpu.h"
@@ -1071,7 +1072,7 @@ __global__ void __launch_bounds__(256) copy_f4(const float4 *__restrict__ a, flo
 
 // Serial operators (excite.rs, probe.rs, lumped.rs) and the gap rules of
 // Engine::advance (mod.rs:873-915).
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6 };
 
 // An installed lumped element (lumped.rs:367-406) and, with columns > 0,
 // its LumpedPort records (engine/port.rs:94-104).
@@ -1128,7 +1129,7 @@ __device__ __forceinline__ bool needs(const DevOp &o, uint64_t n) {
   }
   if (o.type == OP_LUMPED) return true;  // post_e every step
   if (o.type == OP_POLES) return false;  // slab hooks, not gap hooks
-  if (o.type == OP_PW) return true;      // post_h and post_e every step
+  if (o.type == OP_PW || o.type == OP_MERGED) return true;  // post_h and post_e every step
   return n % o.interval == 0;
 }
 
@@ -1255,8 +1256,9 @@ __device__ Gaps gaps_at(const DevOp *ops, int nops, uint64_t n) {
   for (int q = threadIdx.x; q < nops; q += blockDim.x) {
     const DevOp &o = ops[q];
     if (!needs(o, n)) continue;
-    if (is_esrc(o) || is_vprobe(o) || is_lumped(o) || o.type == OP_PW) need_post_e = true;
-    if (is_hsrc(o) || o.type == OP_PW) mid = true, fixed = true;
+    const bool own = o.type == OP_PW || o.type == OP_MERGED;
+    if (is_esrc(o) || is_vprobe(o) || is_lumped(o) || own) need_post_e = true;
+    if (is_hsrc(o) || own) mid = true, fixed = true;
     if (is_iprobe(o) || is_lumped(o)) mid = true;  // lumped pre_e is movable
   }
   need_post_e = __syncthreads_or(need_post_e);
@@ -1416,6 +1418,119 @@ __global__ void __launch_bounds__(256) pw_aux_e(DevPw p, const uint64_t *step) {
     return;
   }
   p.e[m] = p.ca[m] * p.e[m] - p.cb[m] * (p.h[m] - p.h[m - 1]);
+}
+
+// ---- reduced currents of small cut faces (MergedFaces, conformal.rs:2313-2700) ----
+
+// Flat tables of the operator (ems_merged_desc): circulation terms per
+// reduced current, blocks (rows in elimination order, envelope factor and
+// B rows, columns block-local), driven faces; and the state: g^{n-1/2}
+// (cur), the update's result (next), the circulations (circ) and the
+// blocks' right-hand sides (x, one per block row).
+struct DevMerged {
+  int ng, ns, nb, nf;
+  double dt;
+  const double *da, *db;
+  const uint32_t *ts;
+  const int32_t *tax;
+  const uint64_t *tidx;
+  const double *tw;
+  const uint32_t *bs, *mem, *ef;
+  const uint64_t *eo;
+  const double *ev;
+  const uint32_t *brs, *bc;
+  const double *bv;
+  const int32_t *fax;
+  const uint64_t *fidx;
+  const uint32_t *fs, *fg;
+  const double *fw;
+  double *cur, *next, *circ, *x;
+};
+
+__host__ __device__ __forceinline__ float flush_hd(float x) {
+  return fabsf(x) < FLT_MIN ? 0.0f : x;
+}
+
+// MergedFaces::advance for merge group q (conformal.rs:2570-2572).
+__host__ __device__ __forceinline__ void mf_scalar(const DevMerged &m, int q, const double *cur,
+                                                   const double *c, double *out) {
+  out[q] = (double)flush_hd((float)(m.da[q] * cur[q] - m.db[q] * c[q]));
+}
+
+// Block::advance (conformal.rs:2037-2052), its right-hand side row i of the
+// block whose rows start at r0: x_i = (B g)_i - dt c (the row's terms in
+// order, folding from -0.0).
+__host__ __device__ __forceinline__ double mf_rhs(const DevMerged &m, uint32_t r0, uint32_t i,
+                                                  const double *cur, const double *c) {
+  double s = -0.0;
+  for (uint32_t e = m.brs[r0 + i]; e < m.brs[r0 + i + 1]; e++)
+    s += m.bv[e] * cur[m.mem[r0 + m.bc[e]]];
+  return s - m.dt * c[m.mem[r0 + i]];
+}
+
+// Envelope::solve (conformal.rs:2101-2123) of the block's factor on its
+// nr rows, in place: L y = x forward, then L^T x' = y backward.
+__host__ __device__ inline void mf_solve(const DevMerged &m, uint32_t r0, uint32_t nr,
+                                         double *x) {
+  for (uint32_t i = 0; i < nr; i++) {
+    const uint32_t f = m.ef[r0 + i];
+    const double *row = m.ev + m.eo[r0 + i];
+    double s = x[i];
+    for (uint32_t j = f; j < i; j++) s -= row[j - f] * x[j];
+    x[i] = s / row[i - f];
+  }
+  for (uint32_t i = nr; i-- > 0;) {
+    const uint32_t f = m.ef[r0 + i];
+    const double *row = m.ev + m.eo[r0 + i];
+    x[i] /= row[i - f];
+    const double xi = x[i];
+    for (uint32_t j = f; j < i; j++) x[j] -= row[j - f] * xi;
+  }
+}
+
+// MergedFaces::circulations (conformal.rs:2541-2566): a thread per reduced
+// current, its terms in order, the sum folding from -0.0.
+__global__ void __launch_bounds__(256) mf_circ(Fields f, DevMerged m) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m.ng) return;
+  double c = -0.0;
+  for (uint32_t t = m.ts[g]; t < m.ts[g + 1]; t++)
+    c += m.tw[t] * (double)f.u[m.tax[t]][m.tidx[t]];
+  m.circ[g] = c;
+}
+
+// The update of every reduced current: a wave per block (its lanes form the
+// right-hand sides, lane 0 runs the solve's serial chain on them in LDS,
+// or in the global scratch for a block of more than lds_rows rows), then
+// a lane per merge group.
+__global__ void __launch_bounds__(64) mf_advance(DevMerged m, int lds_rows) {
+  extern __shared__ double xs[];
+  const int b = blockIdx.x;
+  if (b >= m.nb) {
+    const int q = (b - m.nb) * 64 + (int)threadIdx.x;
+    if (q < m.ns) mf_scalar(m, q, m.cur, m.circ, m.next);
+    return;
+  }
+  const uint32_t r0 = m.bs[b], nr = m.bs[b + 1] - r0;
+  double *x = nr <= (uint32_t)lds_rows ? xs : m.x + r0;
+  for (uint32_t i = threadIdx.x; i < nr; i += 64) x[i] = mf_rhs(m, r0, i, m.cur, m.circ);
+  __syncthreads();
+  if (threadIdx.x == 0) mf_solve(m, r0, nr, x);
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nr; i += 64)
+    m.next[m.mem[r0 + i]] = (double)flush_hd((float)x[i]);
+}
+
+// MergedFaces::post_h, the rest (conformal.rs:2662-2671): g = g', and every
+// driven face's current sum w g (from -0.0) as f32.
+__global__ void __launch_bounds__(256) mf_drive(Fields f, DevMerged m) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m.nf) {
+    double s = -0.0;
+    for (uint32_t e = m.fs[t]; e < m.fs[t + 1]; e++) s += m.fw[e] * m.next[m.fg[e]];
+    f.i[m.fax[t]][m.fidx[t]] = (float)s;
+  }
+  if (t < m.ng) m.cur[t] = m.next[t];
 }
 
 // ---- dispersive media (dispersive.rs) ----
@@ -1646,6 +1761,17 @@ struct HostOp {
   DevPw pw{};
   DevPwCorr pwc[2]{};
   double *pw_src = nullptr;
+  // reduced currents of cut faces: device tables and state, host copies
+  DevMerged mf{};
+  int mf_lds_rows = 0;  // rows of the largest block solved in LDS (at most 8192)
+  std::shared_ptr<struct MergedHost> mfh;
+};
+
+// Host copies of a reduced-current operator's tables (its energy).
+struct MergedHost {
+  std::vector<double> da, db, l, ev, bv, mval;
+  std::vector<uint32_t> bs, mem, ef, brs, bc, mrs, mcol;
+  std::vector<uint64_t> eo;
 };
 
 }  // namespace
@@ -1732,7 +1858,8 @@ bool has_mid_ops(const ems_gpu *h) {
   for (auto &o : h->ops)
     if ((o.type == OP_SOURCE && (o.kind == 2 || o.kind == 3)) ||
         (o.type == OP_PROBE && o.quantity == 1) ||
-        (o.type == OP_LUMPED && (o.lp.branch || o.lp.record)) || o.type == OP_PW)
+        (o.type == OP_LUMPED && (o.lp.branch || o.lp.record)) || o.type == OP_PW ||
+        o.type == OP_MERGED)
       return true;
   return false;
 }
@@ -1746,16 +1873,19 @@ bool has_magnetic_poles(const ems_gpu *h) {
   return false;
 }
 
-bool has_planewave(const ems_gpu *h) {
+// Operators with kernels of their own in the gaps (plane waves, reduced
+// currents of cut faces): they split the gaps and need the H-E gap.
+bool own_gap_kernels(const HostOp &o) { return o.type == OP_PW || o.type == OP_MERGED; }
+bool has_gap_kernels(const ems_gpu *h) {
   for (auto &o : h->ops)
-    if (o.type == OP_PW) return true;
+    if (own_gap_kernels(o)) return true;
   return false;
 }
 
 bool use_fused(const ems_gpu *h) {
   size_t ns = 0;
   for (auto &s : h->slabs) ns += s.len != 0;
-  return h->fused && !has_hsrc(h) && !has_magnetic_poles(h) && !has_planewave(h) && ns <= 32;
+  return h->fused && !has_hsrc(h) && !has_magnetic_poles(h) && !has_gap_kernels(h) && ns <= 32;
 }
 
 // The dispersive media as slab hooks (dispersive.rs:1700-1750), in
@@ -2051,6 +2181,30 @@ int launch_pw(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
   return 0;
 }
 
+// The reduced currents' hooks (MergedFaces::post_h / post_e,
+// conformal.rs:2655-2682): stage 0 the circulations, the update and the
+// driven faces; stage 1 the circulations of the energy look-ahead.
+int launch_merged(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
+  const DevMerged &m = o.mf;
+  if (m.ng)
+    hipLaunchKernelGGL(mf_circ, dim3((m.ng + 255) / 256), dim3(256), 0, h->stream, f, m);
+  if (stage == 0) {
+    const int na = m.nb + (m.ns + 63) / 64;
+    if (na)
+      hipLaunchKernelGGL(mf_advance, dim3(na), dim3(64), (size_t)o.mf_lds_rows * sizeof(double),
+                         h->stream, m, o.mf_lds_rows);
+    const int nd = std::max(m.nf, m.ng);
+    if (nd)
+      hipLaunchKernelGGL(mf_drive, dim3((nd + 255) / 256), dim3(256), 0, h->stream, f, m);
+  }
+  CK(h, hipGetLastError());
+  return 0;
+}
+
+int launch_gap_op(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
+  return o.type == OP_PW ? launch_pw(h, o, f, stage) : launch_merged(h, o, f, stage);
+}
+
 // Post-E gap on the result `f`; `fp` holds the state before a fused step
 // (prev = true), whose u the lumped pre_e hooks capture. Plane waves run
 // at their place among the gap's ops.
@@ -2058,13 +2212,13 @@ int launch_gap_post_e(ems_gpu *h, const Fields &f, const Fields &fp, bool prev) 
   const int nops = (int)h->ops.size();
   int q0 = 0;
   for (int q = 0; q <= nops; q++) {
-    if (q < nops && h->ops[q].type != OP_PW) continue;
+    if (q < nops && !own_gap_kernels(h->ops[q])) continue;
     const bool last = q == nops;
     hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(256), 0, h->stream, f, fp,
                        prev && q0 == 0 ? 1 : 0, h->d_ops, nops, q0, q, last ? 1 : 0, h->d_step);
     CK(h, hipGetLastError());
     if (!last) {
-      int rc = launch_pw(h, h->ops[q], f, 1);
+      int rc = launch_gap_op(h, h->ops[q], f, 1);
       if (rc) return rc;
     }
     q0 = q + 1;
@@ -2077,12 +2231,12 @@ int launch_gap_mid(ems_gpu *h, const Fields &f) {
   const int nops = (int)h->ops.size();
   int q0 = 0;
   for (int q = 0; q <= nops; q++) {
-    if (q < nops && h->ops[q].type != OP_PW) continue;
+    if (q < nops && !own_gap_kernels(h->ops[q])) continue;
     hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops, nops, q0, q,
                        (const uint64_t *)h->d_step);
     CK(h, hipGetLastError());
     if (q < nops) {
-      int rc = launch_pw(h, h->ops[q], f, 0);
+      int rc = launch_gap_op(h, h->ops[q], f, 0);
       if (rc) return rc;
     }
     q0 = q + 1;
@@ -2920,6 +3074,149 @@ int ems_gpu_planewave_aux(ems_gpu *h, int id, double *e, double *hh) {
   return EMS_OK;
 }
 
+int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
+  ALIVE(h);
+  if (!d || d->n_groups < 0 || d->n_scalar < 0 || d->n_scalar > d->n_groups ||
+      d->n_blocks < 0 || d->n_faces < 0 || !(d->dt > 0.0))
+    return fail(h, EMS_E_INVALID, "reduced currents: counts");
+  const uint32_t ng = (uint32_t)d->n_groups, nb = (uint32_t)d->n_blocks,
+                 nf = (uint32_t)d->n_faces;
+  if (ng && (!d->da || !d->db || !d->l))
+    return fail(h, EMS_E_INVALID, "reduced currents: groups");
+  if (!d->term_start || !d->blk_start || !d->env_off || !d->brow_start || !d->mrow_start ||
+      !d->face_start)
+    return fail(h, EMS_E_INVALID, "reduced currents: offset tables");
+  // Every index the kernels follow is checked here.
+  const uint32_t nt = ng ? d->term_start[ng] : 0;
+  for (uint32_t g = 0; g < ng; g++)
+    if (d->term_start[g] > d->term_start[g + 1])
+      return fail(h, EMS_E_INVALID, "reduced currents: term ranges");
+  if (nt && (!d->term_axis || !d->term_idx || !d->term_w))
+    return fail(h, EMS_E_INVALID, "reduced currents: terms");
+  for (uint32_t t = 0; t < nt; t++)
+    if (d->term_axis[t] < 0 || d->term_axis[t] > 2 || d->term_idx[t] >= h->N)
+      return fail(h, EMS_E_INVALID, "reduced currents: term edge out of range");
+  if (d->blk_start[0] != 0) return fail(h, EMS_E_INVALID, "reduced currents: block rows");
+  for (uint32_t b = 0; b < nb; b++)
+    if (d->blk_start[b] > d->blk_start[b + 1])
+      return fail(h, EMS_E_INVALID, "reduced currents: block rows");
+  const uint32_t nr = d->blk_start[nb];
+  if ((uint64_t)d->n_scalar + nr != ng)
+    return fail(h, EMS_E_INVALID, "reduced currents: every non-scalar current is one block row");
+  if (nr && (!d->members || !d->env_first || !d->env_val))
+    return fail(h, EMS_E_INVALID, "reduced currents: block tables");
+  std::vector<uint8_t> seen(ng, 0);
+  for (uint32_t b = 0; b < nb; b++) {
+    const uint32_t r0 = d->blk_start[b], r1 = d->blk_start[b + 1];
+    for (uint32_t r = r0; r < r1; r++) {
+      const uint32_t i = r - r0;
+      if (d->members[r] < (uint32_t)d->n_scalar || d->members[r] >= ng || seen[d->members[r]]++)
+        return fail(h, EMS_E_INVALID, "reduced currents: block members");
+      if (d->env_first[r] > i || d->env_off[r + 1] < d->env_off[r] ||
+          d->env_off[r + 1] - d->env_off[r] != (uint64_t)(i - d->env_first[r] + 1))
+        return fail(h, EMS_E_INVALID, "reduced currents: envelope rows");
+      for (int w = 0; w < 2; w++) {
+        const uint32_t *st = w ? d->mrow_start : d->brow_start;
+        const uint32_t *col = w ? d->mcol : d->bcol;
+        if (st[r + 1] < st[r]) return fail(h, EMS_E_INVALID, "reduced currents: B/M rows");
+        for (uint32_t e = st[r]; e < st[r + 1]; e++)
+          if (!col || col[e] >= r1 - r0)
+            return fail(h, EMS_E_INVALID, "reduced currents: B/M column out of the block");
+      }
+    }
+  }
+  if (d->env_off[0] != 0 || d->brow_start[0] != 0 || d->mrow_start[0] != 0)
+    return fail(h, EMS_E_INVALID, "reduced currents: row offsets");
+  if (d->face_start[0] != 0) return fail(h, EMS_E_INVALID, "reduced currents: face ranges");
+  for (uint32_t q = 0; q < nf; q++)
+    if (d->face_start[q] > d->face_start[q + 1])
+      return fail(h, EMS_E_INVALID, "reduced currents: face ranges");
+  const uint32_t nw = d->face_start[nf];
+  if (nf && (!d->face_axis || !d->face_idx || (nw && (!d->face_group || !d->face_w))))
+    return fail(h, EMS_E_INVALID, "reduced currents: faces");
+  for (uint32_t q = 0; q < nf; q++)
+    if (d->face_axis[q] < 0 || d->face_axis[q] > 2 || d->face_idx[q] >= h->N)
+      return fail(h, EMS_E_INVALID, "reduced currents: driven face out of range");
+  for (uint32_t e = 0; e < nw; e++)
+    if (d->face_group[e] >= ng) return fail(h, EMS_E_INVALID, "reduced currents: face weights");
+  const uint64_t nev = nr ? d->env_off[nr] : 0;
+  const uint32_t nbv = nr ? d->brow_start[nr] : 0, nmv = nr ? d->mrow_start[nr] : 0;
+
+  HostOp o;
+  o.type = OP_MERGED;
+  auto put = [&](const void *src, size_t bytes, void **dst) -> int {
+    int rc = upload(h, dst, src, bytes);
+    if (!rc) o.mem.push_back(*dst);
+    return rc;
+  };
+  DevMerged &m = o.mf;
+  m.ng = (int)ng, m.ns = d->n_scalar, m.nb = (int)nb, m.nf = (int)nf, m.dt = d->dt;
+  int rc;
+  const std::vector<double> z(std::max<size_t>(std::max<uint32_t>(ng, nr), 1), 0.0);
+#define PUT(dst, src, count, T)                                                           \
+  if ((rc = put((src), (size_t)(count) * sizeof(T), (void **)&(dst)))) return rc;
+  PUT(m.da, d->da, ng, double);
+  PUT(m.db, d->db, ng, double);
+  PUT(m.ts, d->term_start, ng + 1, uint32_t);
+  PUT(m.tax, d->term_axis, nt, int32_t);
+  PUT(m.tidx, d->term_idx, nt, uint64_t);
+  PUT(m.tw, d->term_w, nt, double);
+  PUT(m.bs, d->blk_start, nb + 1, uint32_t);
+  PUT(m.mem, d->members, nr, uint32_t);
+  PUT(m.ef, d->env_first, nr, uint32_t);
+  PUT(m.eo, d->env_off, nr + 1, uint64_t);
+  PUT(m.ev, d->env_val, nev, double);
+  PUT(m.brs, d->brow_start, nr + 1, uint32_t);
+  PUT(m.bc, d->bcol, nbv, uint32_t);
+  PUT(m.bv, d->bval, nbv, double);
+  PUT(m.fax, d->face_axis, nf, int32_t);
+  PUT(m.fidx, d->face_idx, nf, uint64_t);
+  PUT(m.fs, d->face_start, nf + 1, uint32_t);
+  PUT(m.fg, d->face_group, nw, uint32_t);
+  PUT(m.fw, d->face_w, nw, double);
+  PUT(m.cur, z.data(), ng, double);
+  PUT(m.next, z.data(), ng, double);
+  PUT(m.circ, z.data(), ng, double);
+  PUT(m.x, z.data(), nr, double);
+#undef PUT
+  auto mh = std::make_shared<MergedHost>();
+  auto cp = [](auto &v, const auto *src, size_t n) { v.assign(src, src + n); };
+  cp(mh->da, d->da, ng);
+  cp(mh->db, d->db, ng);
+  cp(mh->l, d->l, ng);
+  cp(mh->bs, d->blk_start, nb + 1);
+  cp(mh->mem, d->members, nr);
+  cp(mh->ef, d->env_first, nr);
+  cp(mh->eo, d->env_off, nr + 1);
+  cp(mh->ev, d->env_val, nev);
+  cp(mh->brs, d->brow_start, nr + 1);
+  cp(mh->bc, d->bcol, nbv);
+  cp(mh->bv, d->bval, nbv);
+  cp(mh->mrs, d->mrow_start, nr + 1);
+  cp(mh->mcol, d->mcol, nmv);
+  cp(mh->mval, d->mval, nmv);
+  o.mfh = mh;
+  uint32_t most = 0;
+  for (uint32_t b = 0; b < nb; b++) most = std::max(most, d->blk_start[b + 1] - d->blk_start[b]);
+  o.mf_lds_rows = (int)std::min<uint32_t>(most, 8192);
+  h->ops.push_back(o);
+  h->ops_dirty = true;
+  if (id) *id = (int)h->ops.size() - 1;
+  return EMS_OK;
+}
+
+int ems_gpu_merged_state(ems_gpu *h, int id, double *cur, double *circ) {
+  ALIVE(h);
+  if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_MERGED)
+    return fail(h, EMS_E_INVALID, "not a reduced-current operator id");
+  const HostOp &o = h->ops[id];
+  CK(h, hipStreamSynchronize(h->stream));
+  const size_t bytes = (size_t)o.mf.ng * sizeof(double);
+  if (cur && bytes) CK(h, hipMemcpy(cur, o.mf.cur, bytes, hipMemcpyDeviceToHost));
+  if (circ && bytes) CK(h, hipMemcpy(circ, o.mf.circ, bytes, hipMemcpyDeviceToHost));
+  return EMS_OK;
+}
+
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks) {
   ALIVE(h);
   // fused 2: the fused kernel without the next-plane prefetch; 3: the
@@ -2982,6 +3279,44 @@ static int ops_energy(ems_gpu *h, double *out) {
         side_w[sd] = acc;
       }
       w += side_w[0] + side_w[1];
+      continue;
+    }
+    if (o.type == OP_MERGED) {
+      // MergedFaces::energy (conformal.rs:2684-2698): the next update from
+      // the current g and the look-ahead circulations, evaluated here with
+      // the device's arithmetic, then 1/2 g^T M g'.
+      if (!synced) CK(h, hipStreamSynchronize(h->stream));
+      synced = true;
+      const MergedHost &mh = *o.mfh;
+      const int ng = o.mf.ng;
+      std::vector<double> cur(ng), circ(ng), next(ng, 0.0), x(mh.mem.size() + 1);
+      if (ng) {
+        CK(h, hipMemcpy(cur.data(), o.mf.cur, ng * sizeof(double), hipMemcpyDeviceToHost));
+        CK(h, hipMemcpy(circ.data(), o.mf.circ, ng * sizeof(double), hipMemcpyDeviceToHost));
+      }
+      DevMerged v = o.mf;
+      v.da = mh.da.data(), v.db = mh.db.data();
+      v.bs = mh.bs.data(), v.mem = mh.mem.data(), v.ef = mh.ef.data(), v.eo = mh.eo.data();
+      v.ev = mh.ev.data(), v.brs = mh.brs.data(), v.bc = mh.bc.data(), v.bv = mh.bv.data();
+      for (int q = 0; q < v.ns; q++) mf_scalar(v, q, cur.data(), circ.data(), next.data());
+      for (int b = 0; b < v.nb; b++) {
+        const uint32_t r0 = mh.bs[b], nr = mh.bs[b + 1] - r0;
+        for (uint32_t i = 0; i < nr; i++) x[i] = mf_rhs(v, r0, i, cur.data(), circ.data());
+        mf_solve(v, r0, nr, x.data());
+        for (uint32_t i = 0; i < nr; i++) next[mh.mem[r0 + i]] = (double)flush_hd((float)x[i]);
+      }
+      double e = 0.0;
+      for (int q = 0; q < v.ns; q++) e += 0.5 * mh.l[q] * cur[q] * next[q];
+      for (int b = 0; b < v.nb; b++) {
+        const uint32_t r0 = mh.bs[b];
+        for (uint32_t r = r0; r < mh.bs[b + 1]; r++) {
+          double mn = -0.0;
+          for (uint32_t k = mh.mrs[r]; k < mh.mrs[r + 1]; k++)
+            mn += mh.mval[k] * next[mh.mem[r0 + mh.mcol[k]]];
+          e += 0.5 * cur[mh.mem[r]] * mn;
+        }
+      }
+      w += e;
       continue;
     }
     if (o.type != OP_LUMPED || !o.lp.branch) {
@@ -3422,8 +3757,8 @@ int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
     if (!use_fused(h))
       return bfail(b, EMS_E_UNSUPPORTED,
                    "model " + std::to_string(q) +
-                       " needs the unfused schedule (H source, magnetic poles, plane wave or "
-                       "more than 32 CPML slabs)");
+                       " needs the unfused schedule (H source, magnetic poles, plane wave, "
+                       "cut-face currents or more than 32 CPML slabs)");
     if (h->par != par) return bfail(b, EMS_E_INVALID, "models of a batch step in lockstep");
     const int rc = prepare(h, n);
     if (rc) return bfail(b, rc, h->err);

@@ -770,6 +770,14 @@ class Engine:
             self.setup.append(("planewave", op))
             op._bind(self, oid.value)
             self._ensure_tables(self.timestep() + 1)
+        elif type(op).__name__ == "MergedFaces":
+            # the reduced currents of a conformal geometry (ConformalPec::apply)
+            from . import merged as MF
+            d, keep = MF.make_desc(op)
+            oid = C.c_int()
+            self._check(nat.ems_gpu_add_merged(h, C.byref(d), C.byref(oid)))
+            self.setup.append(("merged", op))
+            op._bind(self, oid.value)
         elif type(op).__name__ == "Dispersive":
             # Dispersive::add_to (dispersive.rs:2479-2520): host install,
             # then the device medium.
@@ -855,6 +863,12 @@ class Engine:
         self._check(nat.ems_gpu_planewave_aux(self._h, oid, nat.ptr(e, C.c_double),
                                               nat.ptr(hh, C.c_double)))
         return e, hh
+
+    def _merged_state(self, oid: int, ng: int):
+        cur, circ = np.zeros(ng), np.zeros(ng)
+        self._check(nat.ems_gpu_merged_state(self._h, oid, nat.ptr(cur, C.c_double),
+                                             nat.ptr(circ, C.c_double)))
+        return cur, circ
 
     def _dispersive_state(self, oid: int, field: int):
         n = self._poles_n[oid][field]

@@ -217,6 +217,50 @@ int ems_gpu_planewave_table(ems_gpu *h, int id, const double *values, uint64_t c
  * PlaneWave::incident_e / incident_h, :1446-1470). */
 int ems_gpu_planewave_aux(ems_gpu *h, int id, double *e, double *hh);
 
+/* Reduced currents of small cut faces: the MergedFaces operator of the
+ * conformal PEC (src/engine/conformal.rs:2313-2700; post_h :2655-2671,
+ * post_e :2673-2682, energy :2684-2698), built on the host (merge groups,
+ * interpolation hosts and deviation currents, their blocks' envelope
+ * Cholesky factors). After every H update: the circulations of u per
+ * reduced current (terms ts[g]..ts[g+1]), the update of each merge group
+ * g < n_scalar (g' = flush(f32(da g - db c))) and of each block (rows
+ * bs[b]..bs[b+1] in elimination order: x = B g - dt c, L L^T x' = x,
+ * g' = flush(f32(x'))), then every driven face's current sum w g' as f32.
+ * After every E update the circulations again (the energy look-ahead).
+ * Columns of B, M and the factor are block-local positions; env_off has
+ * one entry per row plus one. Runs the unfused schedule, at its place
+ * among the serial operators; the engine energy adds
+ * 1/2 g^T M g^(n+3/2). */
+typedef struct {
+    int32_t n_groups, n_scalar, n_blocks, n_faces;
+    double dt;
+    const double *da, *db, *l;  /* per group (da, db used for g < n_scalar) */
+    const uint32_t *term_start; /* n_groups + 1 */
+    const int32_t *term_axis;
+    const uint64_t *term_idx;   /* flat node index of the E edge */
+    const double *term_w;
+    const uint32_t *blk_start;  /* n_blocks + 1, into the rows */
+    const uint32_t *members;    /* per row: its reduced current */
+    const uint32_t *env_first;  /* per row: first column of its envelope */
+    const uint64_t *env_off;    /* rows + 1 */
+    const double *env_val;
+    const uint32_t *brow_start; /* rows + 1 */
+    const uint32_t *bcol;
+    const double *bval;
+    const uint32_t *mrow_start; /* rows + 1 */
+    const uint32_t *mcol;
+    const double *mval;
+    const int32_t *face_axis;
+    const uint64_t *face_idx;   /* flat node index of the driven H edge */
+    const uint32_t *face_start; /* n_faces + 1 */
+    const uint32_t *face_group;
+    const double *face_w;
+} ems_merged_desc;
+int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id);
+/* The reduced currents g (n_groups values) and the circulations of the
+ * last H or E update (either may be NULL). */
+int ems_gpu_merged_state(ems_gpu *h, int id, double *cur, double *circ);
+
 /* Batches (no reference counterpart; for geometry sweeps of many small
  * models): engines stepped together, one fused launch over the tiles of
  * every active model and one launch for all their post-E gaps per step,

@@ -53,6 +53,8 @@ orc_dispersive_state = _sig("orc_dispersive_state", _i, _p, _i, _i, _dp, _dp)
 orc_add_planewave = _sig("orc_add_planewave", _i, _p, _p)
 orc_planewave_table = _sig("orc_planewave_table", _i, _p, _i, _dp, _u64)
 orc_planewave_aux = _sig("orc_planewave_aux", _i, _p, _i, _dp, _dp)
+orc_add_merged = _sig("orc_add_merged", _i, _p, _p)
+orc_merged_state = _sig("orc_merged_state", _i, _p, _i, _dp, _dp)
 orc_run = _sig("orc_run", None, _p, _u64)
 orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
 orc_energy = _sig("orc_energy", _d, _p)
@@ -141,6 +143,11 @@ class OracleEngine:
         assert orc_planewave_aux(self._h, oid, P(ev, C.c_double), P(hv, C.c_double)) == 0
         return ev, hv
 
+    def _merged_state(self, oid, ng):
+        cur, circ = np.zeros(ng), np.zeros(ng)
+        assert orc_merged_state(self._h, oid, P(cur, C.c_double), P(circ, C.c_double)) == 0
+        return cur, circ
+
     def _dispersive_state(self, oid, field):
         n = self._poles_n[oid][field]
         j, q = np.zeros(max(n, 1)), np.zeros(max(n, 1))
@@ -196,6 +203,12 @@ class OracleEngine:
             oid = orc_add_planewave(h, C.addressof(d))
             assert oid >= 0
             self._pw_tables.append((oid, PW.SourceTable(op)))
+            op._bind(self, oid)
+        elif type(op).__name__ == "MergedFaces":
+            from ems_amd import merged as MF
+            d, keep = MF.make_desc(op)
+            oid = orc_add_merged(h, C.addressof(d))
+            assert oid >= 0
             op._bind(self, oid)
         elif type(op).__name__ == "Dispersive":
             from ems_amd import dispersive as DP

@@ -69,7 +69,7 @@ typedef struct {
     float *b, *a, *k, *psi;
 } cpml_t;
 
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6 };
 
 /* Plane-wave corrections (Correction, planewave.rs:424-431), in list
  * order, and the auxiliary grid (PlaneWave, :458-483). */
@@ -148,7 +148,22 @@ typedef struct {
     poles_t *pm;
     /* plane wave */
     pw_t *pw;
+    /* reduced currents of cut faces */
+    struct merged *mf;
 } op_t;
+
+/* MergedFaces (conformal.rs:2313-2340): the tables of orc_merged_desc and
+ * the state, Group::i per reduced current and the circulations. */
+typedef struct merged {
+    int ng, ns, nb, nf;
+    double dt;
+    double *da, *db, *l, *tw, *ev, *bv, *mv, *fw;
+    uint32_t *ts, *bs, *mem, *ef, *brs, *bc, *mrs, *mc, *fs, *fg;
+    uint64_t *eo;
+    int *tax, *fax;
+    size_t *tidx, *fidx;
+    double *i, *circ, *next, *x;
+} merged_t;
 
 struct orc_engine {
     int n[3];
@@ -668,6 +683,82 @@ static void pw_post_e(orc_engine *e, pw_t *p, uint64_t n) {
     p->e[0] = n < p->nsrc ? p->src[n] : 0.0;
 }
 
+/* MergedFaces::circulations (conformal.rs:2541-2566): per reduced current
+ * the weighted sum of its terms' voltages, folding from -0.0. */
+static void mf_circulations(const orc_engine *e, merged_t *m) {
+    for (int g = 0; g < m->ng; g++) {
+        double c = -0.0;
+        for (uint32_t t = m->ts[g]; t < m->ts[g + 1]; t++)
+            c += m->tw[t] * (double)e->u[m->tax[t]][m->tidx[t]];
+        m->circ[g] = c;
+    }
+}
+
+/* MergedFaces::advance (conformal.rs:2569-2576): the merge groups'
+ * flush(f32(da i - db c)), then each block (Block::advance :2037-2052):
+ * x = B i - dt c, Envelope::solve (:2101-2123), flush(f32(x)). */
+static void mf_advance(const merged_t *m, const double *c, double *x, double *out) {
+    for (int q = 0; q < m->ns; q++)
+        out[q] = (double)flush((float)(m->da[q] * m->i[q] - m->db[q] * c[q]));
+    for (int b = 0; b < m->nb; b++) {
+        const uint32_t r0 = m->bs[b], nr = m->bs[b + 1] - r0;
+        for (uint32_t i = 0; i < nr; i++) {
+            double s = -0.0;
+            for (uint32_t k = m->brs[r0 + i]; k < m->brs[r0 + i + 1]; k++)
+                s += m->bv[k] * m->i[m->mem[r0 + m->bc[k]]];
+            x[i] = s - m->dt * c[m->mem[r0 + i]];
+        }
+        for (uint32_t i = 0; i < nr; i++) {
+            const uint32_t f = m->ef[r0 + i];
+            const double *r = m->ev + m->eo[r0 + i];
+            double s = x[i];
+            for (uint32_t j = f; j < i; j++) s -= r[j - f] * x[j];
+            x[i] = s / r[i - f];
+        }
+        for (uint32_t i = nr; i-- > 0;) {
+            const uint32_t f = m->ef[r0 + i];
+            const double *r = m->ev + m->eo[r0 + i];
+            x[i] /= r[i - f];
+            const double xi = x[i];
+            for (uint32_t j = f; j < i; j++) x[j] -= r[j - f] * xi;
+        }
+        for (uint32_t i = 0; i < nr; i++) out[m->mem[r0 + i]] = (double)flush((float)x[i]);
+    }
+}
+
+/* MergedFaces::post_h (conformal.rs:2655-2671): circulations, the update,
+ * and every driven face's current sum w i (from -0.0) as f32. */
+static void mf_post_h(orc_engine *e, merged_t *m) {
+    mf_circulations(e, m);
+    mf_advance(m, m->circ, m->x, m->next);
+    memcpy(m->i, m->next, (size_t)m->ng * sizeof(double));
+    for (int q = 0; q < m->nf; q++) {
+        double s = -0.0;
+        for (uint32_t k = m->fs[q]; k < m->fs[q + 1]; k++) s += m->fw[k] * m->i[m->fg[k]];
+        e->i[m->fax[q]][m->fidx[q]] = (float)s;
+    }
+}
+
+/* Operator::energy of MergedFaces (conformal.rs:2684-2698). */
+static double mf_energy(const merged_t *m) {
+    double *next = (double *)calloc((size_t)m->ng + 1, sizeof(double));
+    double *x = (double *)calloc((size_t)(m->bs[m->nb]) + 1, sizeof(double));
+    mf_advance(m, m->circ, x, next);
+    double w = 0.0;
+    for (int q = 0; q < m->ns; q++) w += 0.5 * m->l[q] * m->i[q] * next[q];
+    for (int b = 0; b < m->nb; b++) {
+        const uint32_t r0 = m->bs[b];
+        for (uint32_t r = r0; r < m->bs[b + 1]; r++) {
+            double mn = -0.0;
+            for (uint32_t k = m->mrs[r]; k < m->mrs[r + 1]; k++) mn += m->mv[k] * next[m->mem[r0 + m->mc[k]]];
+            w += 0.5 * m->i[m->mem[r]] * mn;
+        }
+    }
+    free(next);
+    free(x);
+    return w;
+}
+
 static int is_hsrc(const op_t *o) { return o->type == OP_SOURCE && (o->kind == 2 || o->kind == 3); }
 static int is_esrc(const op_t *o) { return o->type == OP_SOURCE && (o->kind == 0 || o->kind == 1); }
 static int is_iprobe(const op_t *o) { return o->type == OP_PROBE && o->quantity == 1; }
@@ -681,7 +772,7 @@ static int needs(const op_t *o, uint64_t n) {
     }
     if (o->type == OP_LUMPED) return 1; /* pre_e / post_e every step */
     if (o->type == OP_POLES) return 0;  /* slab hooks only */
-    if (o->type == OP_PW) return 1;     /* post_h and post_e every step */
+    if (o->type == OP_PW || o->type == OP_MERGED) return 1; /* post_h and post_e every step */
     return n % o->interval == 0;
 }
 
@@ -693,8 +784,9 @@ static void one_step(orc_engine *e, int energy) {
     for (int q = 0; q < e->nops; q++) {
         op_t *o = &e->ops[q];
         if (!needs(o, n)) continue;
-        if (is_esrc(o) || is_vprobe(o) || o->type == OP_LUMPED || o->type == OP_PW) need_post_e = 1;
-        if (is_hsrc(o) || o->type == OP_PW) mid = 1, mid_movable = 0;
+        int own = o->type == OP_PW || o->type == OP_MERGED;
+        if (is_esrc(o) || is_vprobe(o) || o->type == OP_LUMPED || own) need_post_e = 1;
+        if (is_hsrc(o) || own) mid = 1, mid_movable = 0;
         if (is_iprobe(o) || o->type == OP_LUMPED) mid = 1;
     }
     int gap_b = mid && !mid_movable;
@@ -710,6 +802,7 @@ static void one_step(orc_engine *e, int energy) {
             if (is_hsrc(o)) source_apply(e, o, n);
             if (is_iprobe(o) && n % o->interval == 0) probe_sample(e, o);
             if (o->type == OP_PW) pw_post_h(e, o->pw);
+            if (o->type == OP_MERGED) mf_post_h(e, o->mf);
         }
         for (int q = 0; q < e->nops; q++)
             if (e->ops[q].type == OP_LUMPED) lumped_pre_e(e, &e->ops[q]);
@@ -732,6 +825,7 @@ static void one_step(orc_engine *e, int energy) {
             if (is_vprobe(o) && n % o->interval == 0) probe_sample(e, o);
             if (o->type == OP_LUMPED) lumped_post_e(e, o, n);
             if (o->type == OP_PW) pw_post_e(e, o->pw, n);
+            if (o->type == OP_MERGED) mf_circulations(e, o->mf);
         }
     }
     e->step++;
@@ -1136,6 +1230,62 @@ int orc_planewave_aux(orc_engine *e, int id, double *ev, double *hv) {
     return 0;
 }
 
+static void *dup_n(const void *src, size_t n, size_t sz) {
+    void *p = calloc(n + 1, sz);
+    if (src && n) memcpy(p, src, n * sz);
+    return p;
+}
+
+int orc_add_merged(orc_engine *e, const orc_merged_desc *d) {
+    if (d->n_groups < 0 || d->n_scalar < 0 || d->n_scalar > d->n_groups || d->n_blocks < 0 ||
+        d->n_faces < 0)
+        return -1;
+    const uint32_t ng = d->n_groups, nb = d->n_blocks, nf = d->n_faces;
+    const uint32_t nt = d->term_start[ng], nr = d->blk_start[nb], nw = d->face_start[nf];
+    for (uint32_t t = 0; t < nt; t++)
+        if (d->term_axis[t] < 0 || d->term_axis[t] > 2 || d->term_idx[t] >= e->N) return -1;
+    for (uint32_t q = 0; q < nf; q++)
+        if (d->face_axis[q] < 0 || d->face_axis[q] > 2 || d->face_idx[q] >= e->N) return -1;
+    for (uint32_t k = 0; k < nw; k++)
+        if (d->face_group[k] >= ng) return -1;
+    for (uint32_t r = 0; r < nr; r++)
+        if (d->members[r] >= ng) return -1;
+    int id = add_op(e);
+    op_t *o = &e->ops[id];
+    o->type = OP_MERGED;
+    merged_t *m = (merged_t *)calloc(1, sizeof(merged_t));
+    o->mf = m;
+    m->ng = ng, m->ns = d->n_scalar, m->nb = nb, m->nf = nf, m->dt = d->dt;
+    const uint64_t nev = d->env_off[nr];
+    const uint32_t nbv = d->brow_start[nr], nmv = d->mrow_start[nr];
+    m->da = dup_n(d->da, ng, 8), m->db = dup_n(d->db, ng, 8), m->l = dup_n(d->l, ng, 8);
+    m->ts = dup_n(d->term_start, ng + 1, 4), m->tw = dup_n(d->term_w, nt, 8);
+    m->tax = (int *)calloc(nt + 1, sizeof(int)), m->tidx = (size_t *)calloc(nt + 1, sizeof(size_t));
+    for (uint32_t t = 0; t < nt; t++) m->tax[t] = d->term_axis[t], m->tidx[t] = d->term_idx[t];
+    m->bs = dup_n(d->blk_start, nb + 1, 4), m->mem = dup_n(d->members, nr, 4);
+    m->ef = dup_n(d->env_first, nr, 4), m->eo = dup_n(d->env_off, nr + 1, 8);
+    m->ev = dup_n(d->env_val, nev, 8);
+    m->brs = dup_n(d->brow_start, nr + 1, 4), m->bc = dup_n(d->bcol, nbv, 4);
+    m->bv = dup_n(d->bval, nbv, 8);
+    m->mrs = dup_n(d->mrow_start, nr + 1, 4), m->mc = dup_n(d->mcol, nmv, 4);
+    m->mv = dup_n(d->mval, nmv, 8);
+    m->fax = (int *)calloc(nf + 1, sizeof(int)), m->fidx = (size_t *)calloc(nf + 1, sizeof(size_t));
+    for (uint32_t q = 0; q < nf; q++) m->fax[q] = d->face_axis[q], m->fidx[q] = d->face_idx[q];
+    m->fs = dup_n(d->face_start, nf + 1, 4), m->fg = dup_n(d->face_group, nw, 4);
+    m->fw = dup_n(d->face_w, nw, 8);
+    m->i = dup_n(NULL, ng, 8), m->circ = dup_n(NULL, ng, 8), m->next = dup_n(NULL, ng, 8);
+    m->x = dup_n(NULL, nr, 8);
+    return id;
+}
+
+int orc_merged_state(orc_engine *e, int id, double *cur, double *circ) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_MERGED) return -1;
+    const merged_t *m = e->ops[id].mf;
+    if (cur) memcpy(cur, m->i, (size_t)m->ng * sizeof(double));
+    if (circ) memcpy(circ, m->circ, (size_t)m->ng * sizeof(double));
+    return 0;
+}
+
 void orc_run(orc_engine *e, uint64_t n) {
     for (uint64_t t = 0; t < n; t++) one_step(e, 0);
 }
@@ -1150,6 +1300,8 @@ static double ops_energy(const orc_engine *e) {
         const op_t *o = &e->ops[q];
         if (o->type == OP_POLES) /* Dispersive::energy (dispersive.rs:2789-2794) */
             w += side_energy(o->pm, 0) + side_energy(o->pm, 1);
+        else if (o->type == OP_MERGED)
+            w += mf_energy(o->mf);
         else
             w += o->type == OP_LUMPED ? lumped_energy(e, o) : 0.0;
     }

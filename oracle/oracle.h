@@ -134,6 +134,34 @@ int orc_add_planewave(orc_engine *e, const orc_pw_desc *d);
 int orc_planewave_table(orc_engine *e, int id, const double *values, uint64_t count);
 int orc_planewave_aux(orc_engine *e, int id, double *ev, double *hv);
 
+/* Reduced currents of small cut faces (MergedFaces, conformal.rs:2313-2700),
+ * tables built on the host: per reduced current its circulation terms;
+ * merge groups (g < n_scalar) with their own da, db; blocks of rows in
+ * elimination order with the envelope Cholesky factor of A, and the rows of
+ * B and M (block-local columns); the driven faces and their weights. */
+typedef struct {
+    int32_t n_groups, n_scalar, n_blocks, n_faces;
+    double dt;
+    const double *da, *db, *l;
+    const uint32_t *term_start;
+    const int32_t *term_axis;
+    const uint64_t *term_idx;
+    const double *term_w;
+    const uint32_t *blk_start, *members, *env_first;
+    const uint64_t *env_off;
+    const double *env_val;
+    const uint32_t *brow_start, *bcol;
+    const double *bval;
+    const uint32_t *mrow_start, *mcol;
+    const double *mval;
+    const int32_t *face_axis;
+    const uint64_t *face_idx;
+    const uint32_t *face_start, *face_group;
+    const double *face_w;
+} orc_merged_desc;
+int orc_add_merged(orc_engine *e, const orc_merged_desc *d);
+int orc_merged_state(orc_engine *e, int id, double *cur, double *circ);
+
 void orc_run(orc_engine *e, uint64_t n);
 double orc_step_with_energy(orc_engine *e);
 double orc_energy(orc_engine *e);

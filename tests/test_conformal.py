@@ -61,6 +61,10 @@ def test_cut_fractions_add_up_to_the_disk():
 
 
 def test_patch_arithmetic():
+    """Edge scaling (cb f_e, PEC at 0), db / f_A for a face above the
+    threshold, and a small face without a centroid merged with the
+    neighbour across its most open edge: both frozen (db = 0), driven by
+    one group current with a deviation on the small member."""
     g = E.Grid.uniform([6, 6, 4], [1e-3] * 3)
     dt = E.TimeStep.courant(0.99).resolve(g)
     lanes = E.coefficients(g, None, [E.FaceCondition.Pec] * 6, dt)
@@ -69,25 +73,29 @@ def test_patch_arithmetic():
     pec.push_edge(K.ConformalEdge(2, (3, 3, 1), 0.0))
     pec.push_face(K.ConformalFace(2, (2, 2, 1), 0.8))
     pec.push_face(K.ConformalFace(1, (2, 3, 1), 0.2))      # below the threshold
-    changed, rep = pec.patched(lanes)
+    changed, rep, op = pec.build(lanes, g, dt)
     assert sorted(changed) == [3, 5, 10, 11]
     assert changed[3][2, 3, 1] == np.float32(float(lanes[3][2, 3, 1]) * 0.37)
     assert changed[5][3, 3, 1] == 0.0
     assert changed[11][2, 2, 1] == np.float32(float(lanes[11][2, 2, 1]) / 0.8)
-    assert changed[10][2, 3, 1] == np.float32(float(lanes[10][2, 3, 1]) / 0.5)
-    assert (rep.edges, rep.faces, rep.floored, rep.pec_edges) == (2, 2, 1, 1)
+    assert changed[10][2, 3, 1] == 0.0 and changed[10][1, 3, 1] == 0.0
+    assert (rep.edges, rep.faces, rep.merged, rep.groups, rep.floored, rep.pec_edges) == \
+        (2, 1, 2, 1, 0, 1)
+    # the group current drives both faces, the small one plus its deviation
+    assert op.faces == [(1, (1, 3, 1), [(0, 1.0)]), (1, (2, 3, 1), [(0, 1.0), (1, 1.0)])]
+    assert op.scalar == 0 and len(op.blocks) == 1 and len(op.blocks[0].members) == 2
     # everything else is untouched
     for l, lane in changed.items():
         diff = np.argwhere(lane != lanes[l])
-        assert len(diff) == 1
+        assert len(diff) == (2 if l == 10 else 1)
     bad = K.ConformalPec()
     bad.push_edge(K.ConformalEdge(0, (6, 0, 0), 0.5))   # runs off the grid
     with pytest.raises(E.EmsError):
-        bad.patched(lanes)
+        bad.build(lanes, g, dt)
     bad = K.ConformalPec()
     bad.push_face(K.ConformalFace(2, (1, 1, 1), 1.5))
     with pytest.raises(E.EmsError):
-        bad.patched(lanes)
+        bad.build(lanes, g, dt)
 
 
 def _cavity(mode, a_cells, steps, tau=0.5):

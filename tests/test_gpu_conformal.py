@@ -1,6 +1,10 @@
-"""Parity of conformal PEC (coefficient patches) on the device with the
-oracle: a circular cavity and a PEC rod in a CPML box with a dispersive
-sphere, fields, probes and energy bit for bit, fused and unfused."""
+"""Parity of conformal PEC on the device with the oracle: the coefficient
+patches and the reduced currents of small cut faces (merge groups,
+interpolation hosts, deviation currents, envelope-Cholesky blocks) on a
+circular cavity, a PEC rod in a graded CPML box, and the reference's
+sliver face merged and interpolated; fields, probes, reduced currents and
+energy bit for bit. Engines with reduced currents run the unfused
+schedule whatever is asked."""
 import math
 
 import numpy as np
@@ -32,11 +36,11 @@ def test_conformal_cavity():
         pec = CF.cylinder_z(g, (8.5e-3, 8.5e-3), 7.3e-3, mats=m, metal="outside")
         e = K.make(kind, g, m, threads=2)
         rep = pec.apply(e)
-        assert rep.faces > 0 and rep.floored > 0
+        assert rep.faces > 0 and rep.interpolated > 0 and rep.hosts > 0
         e.add_operator(E.EdgeSource(E.Waveform.gaussian(15e9, 10e9)).edge(E.Axis.Z, [7, 8, 1], 1.0))
         p = e.add_operator(E.Probe.voltage(E.Axis.Z, [8, 8, 1]))
-        return e, p
-    (g, a), (o, b) = build("gpu"), build("oracle")
+        return e, p, pec
+    (g, a, pg), (o, b, po) = build("gpu"), build("oracle")
     for n in (1, 300, 1200):
         g.run(n)
         o.run(n)
@@ -44,6 +48,51 @@ def test_conformal_cavity():
         assert np.float64(wg).tobytes() == np.float64(wo).tobytes(), (wg, wo)
     assert np.array_equal(bits(a.values()), bits(o.probe_values(b)))
     same_fields(g, o)
+    op_g = [x for x in g.setup if x[0] == "merged"][0][1]
+    op_o = [x for x in o.ops if type(x).__name__ == "MergedFaces"][0]
+    for x, y in zip(op_g.currents(), op_o.currents()):
+        assert np.array_equal(bits(x), bits(y))
+    assert np.abs(op_g.currents()[0]).max() > 0.0
+
+
+@pytest.mark.parametrize("centroid", [False, True], ids=["merged", "interpolated"])
+def test_sliver_face(centroid):
+    """The reference's sliver (2 % of a z face along its full x edge) with
+    fields set on both sides of it: merged across y - 1, or interpolated
+    from three hosts with a deviation current."""
+    def build(kind):
+        g = E.Grid.uniform([12, 12, 12], [1e-3] * 3)
+        e = K.make(kind, g, None)
+        pec = CF.ConformalPec()
+        f = CF.ConformalFace(2, (5, 5, 5), 0.02)
+        if centroid:
+            pec.push_face_with_centroid(f, (0.5, 0.01))
+        else:
+            pec.push_face(f)
+        for q in [(5, 5, 5), (6, 5, 5)]:
+            pec.push_edge(CF.ConformalEdge(1, q, 0.02))
+        pec.push_edge(CF.ConformalEdge(0, (5, 6, 5), 0.0))
+        rep = pec.apply(e)
+        assert (rep.merged, rep.interpolated) == ((0, 1) if centroid else (2, 0))
+        for axis, node, v in [(0, (5, 5, 5), 1.0), (1, (3, 7, 4), -0.5), (1, (4, 4, 4), 0.25)]:
+            u = e.u_component(axis)
+            u[node] = v
+            e.set_component(0, axis, u)
+        op = [x for x in (e.setup if kind == "gpu" else [(0, y) for y in e.ops])
+              if type(x[1]).__name__ == "MergedFaces"][0][1]
+        return e, op
+    (g, pg), (o, po) = build("gpu"), build("oracle")
+    for n in (1, 7, 400):
+        g.run(n)
+        o.run(n)
+        wg, wo = g.step_with_energy(), o.step_with_energy()
+        assert np.float64(wg).tobytes() == np.float64(wo).tobytes(), (n, wg, wo)
+        eg, eo = g.energy(), o.energy()
+        assert np.float64(eg).tobytes() == np.float64(eo).tobytes(), (n, eg, eo)
+        for x, y in zip(pg.currents(), po.currents()):
+            assert np.array_equal(bits(x), bits(y)), n
+    same_fields(g, o)
+    assert np.abs(pg.currents()[0]).max() > 0.0
 
 
 def test_conformal_rod_in_cpml_box():

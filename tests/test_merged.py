@@ -1,0 +1,236 @@
+"""Small cut faces without their own current (ems_amd/merged.py and the
+reduced-current operator in the oracle), on the CPU, by the reference's own
+cases (src/engine/conformal.rs tests, :3116-3570): the sliver face merged
+with the neighbour across its full edge, or interpolated from a host and
+its neighbours; the energy invariant held over 5000 steps with
+energy() equal to step_with_energy(); the cell-wise bound equal to the
+Courant number squared for cubic cells; small faces kept where flat cells
+make the time step short; affine-exact weights; the envelope Cholesky
+solve against a dense solve; plain Dey-Mittra diverging where the merge
+scheme is stable."""
+import math
+
+import numpy as np
+import pytest
+
+import ems_amd as E
+from ems_amd import conformal as K
+from ems_amd import merged as M
+from oracle.bridge import OracleEngine
+
+
+def _engine(s, dz=1.0):
+    g = E.Grid.uniform([12, 12, 12], [1e-3, 1e-3, dz * 1e-3])
+    dt = E.TimeStep.courant(s).resolve(g)
+    lanes = E.coefficients(g, None, [E.FaceCondition.Pec] * 6, dt)
+    return g, dt, lanes
+
+
+def _sliver(centroid=False, fraction=0.02, options=None):
+    """A z face keeping its full x edge at y = 5 and `fraction` of its area
+    (the y edges that open, the far x edge closed)."""
+    c = K.ConformalPec(options)
+    f = K.ConformalFace(2, (5, 5, 5), fraction)
+    if centroid:
+        c.push_face_with_centroid(f, (0.5, fraction / 2))
+    else:
+        c.push_face(f)
+    for q in [(5, 5, 5), (6, 5, 5)]:
+        c.push_edge(K.ConformalEdge(1, q, fraction))
+    c.push_edge(K.ConformalEdge(0, (5, 6, 5), 0.0))
+    return c
+
+
+def _energy_run(g, dt, lanes, pec, sets):
+    o = OracleEngine(g, dt, lanes, threads=1)
+    rep = pec.apply(o)
+    for axis, node, v in sets:
+        u = o.u_component(axis)
+        u[node] = v
+        o.set_component(0, axis, u)
+    o.run(1)
+    w0 = o.energy()
+    dev = 0.0
+    for _ in range(50):
+        o.run(100)
+        dev = max(dev, abs(o.energy() / w0 - 1.0))
+    a = o.energy()
+    b = o.step_with_energy()
+    o.close()
+    return rep, dev, a, b
+
+
+def test_sliver_face_is_merged():
+    g, dt, lanes = _engine(0.99)
+    changed, rep, op = _sliver().build(lanes, g, dt)
+    assert (rep.merged, rep.groups, rep.floored) == (2, 1, 0)
+    assert changed[11][5, 5, 5] == 0.0 and changed[11][5, 4, 5] == 0.0   # merged across y - 1
+    assert changed[3][5, 6, 5] == 0.0
+    assert abs(float(changed[4][5, 5, 5]) / (float(lanes[4][5, 5, 5]) * 0.02) - 1.0) < 1e-6
+    assert op.faces[1] == (2, (5, 5, 5), [(0, 1.0), (1, 1.0)])
+
+
+def test_sliver_face_is_interpolated():
+    g, dt, lanes = _engine(0.99)
+    changed, rep, op = _sliver(centroid=True).build(lanes, g, dt)
+    assert (rep.interpolated, rep.hosts, rep.merged) == (1, 3, 0)
+    for p in [(5, 5, 5), (5, 4, 5), (5, 3, 5)]:
+        assert changed[11][p] == 0.0, p
+    # the small face reads its host (y - 1) and the host's neighbour away from
+    # it (y - 2) with the linear weights at its centroid (y offset 0.49 of a
+    # cell below the host's centre), plus its deviation current
+    w = dict(op.faces[-1][2])
+    ids = {f[1]: f[2][0][0] for f in op.faces[:-1]}
+    assert op.faces[-1][0:2] == (2, (5, 5, 5))
+    assert math.isclose(w[ids[(5, 4, 5)]], 1.51, rel_tol=1e-12)
+    assert math.isclose(w[ids[(5, 3, 5)]], -0.51, rel_tol=1e-12)
+
+
+@pytest.mark.parametrize("centroid", [False, True], ids=["merged", "interpolated"])
+def test_reduced_schemes_conserve_energy(centroid):
+    g, dt, lanes = _engine(0.99)
+    sets = [(0, (5, 5, 5), 1.0), (1, (3, 7, 4), -0.5)]
+    if centroid:
+        sets += [(1, (4, 4, 4), -0.5), (1, (3, 7, 4), 0.25)]
+    rep, dev, a, b = _energy_run(g, dt, lanes, _sliver(centroid), sets)
+    assert dev < 1e-5, dev
+    assert a == b  # the in-step energy agrees with the look-ahead
+
+
+def test_plain_dey_mittra_sliver_diverges():
+    """Without the reduction (threshold 0) the 2 % face's own current has a
+    resonance far above the Courant limit: the run blows up."""
+    g, dt, lanes = _engine(0.99)
+    pec = _sliver(options=K.ConformalOptions(merge_below=0.0, keep_stable=False))
+    o = OracleEngine(g, dt, lanes, threads=1)
+    rep = pec.apply(o)
+    assert (rep.merged, rep.interpolated, rep.faces) == (0, 0, 1)
+    u = o.u_component(0)
+    u[5, 5, 5] = 1.0
+    o.set_component(0, 0, u)
+    o.run(1)
+    w0 = o.energy()
+    o.run(3000)
+    assert not (abs(o.energy()) < 1e6 * w0)
+    o.close()
+
+
+@pytest.mark.parametrize("s", [0.5, 0.8, 0.95])
+def test_cell_bound_is_the_courant_number(s):
+    g, dt, lanes = _engine(s)
+    co = lanes.reshape(12, -1)
+    assert abs(M.cell_stability(co, g, dt, (5, 5, 5), {}) - s * s) < 1e-6
+
+
+def test_small_faces_keep_their_current_where_the_time_step_allows():
+    # cubes at the Courant limit: the face loses its current
+    g, dt, lanes = _engine(0.99)
+    pec = _sliver(centroid=True, fraction=0.2)
+    _, rep, _ = pec.build(lanes, g, dt)
+    assert rep.kept == 0 and rep.interpolated + rep.merged > 0 and not rep.bounded
+    # cells a tenth as high as wide: the face keeps its current and area
+    g, dt, lanes = _engine(0.99, dz=0.1)
+    changed, rep, op = _sliver(centroid=True, fraction=0.2).build(lanes, g, dt)
+    assert (rep.kept, rep.interpolated, rep.merged, rep.floored) == (1, 0, 0, 0)
+    assert rep.bounded and op is None
+    assert abs(float(changed[11][5, 5, 5]) / float(lanes[11][5, 5, 5]) - 5.0) < 1e-5
+    rep, dev, a, b = _energy_run(g, dt, lanes, _sliver(centroid=True, fraction=0.2),
+                                 [(1, (5, 5, 5), 1.0), (0, (5, 5, 5), -0.5)])
+    assert dev < 1e-5, dev
+    # switched off: as on the cubes
+    pec = _sliver(centroid=True, fraction=0.2, options=K.ConformalOptions(keep_stable=False))
+    _, rep, _ = pec.build(lanes, g, dt)
+    assert rep.kept == 0 and rep.interpolated + rep.merged > 0
+
+
+def test_weights_are_affine_exact():
+    for d in ([(0.0, 0.0)], [(0.6, 0.0), (1.6, 0.0)], [(0.6, 0.2), (1.6, 0.25), (0.55, -0.8)]):
+        w = M.interpolation_weights(d)
+        assert abs(sum(w) - 1.0) < 1e-12
+        for ax in range(2):
+            assert abs(sum(wi * di[ax] for wi, di in zip(w, d))) < 1e-12
+    assert M.interpolation_weights([(0.7, 0.1)]) is None
+    assert M.interpolation_weights([(0.6, 0.1), (1.6, 0.0)]) is None
+    assert M.interpolation_weights([(5.0, 0.0), (6.0, 0.0)]) is None
+    assert M.interpolation_weights([(1.0, 0.0), (2.0, 0.0), (3.0, 0.0)]) is None
+
+
+def test_envelope_cholesky_solves():
+    n = 7
+    a = [[(i, 4.0 + i)] for i in range(n)]
+
+    def link(i, j, v):
+        a[i].append((j, v))
+        a[j].append((i, v))
+    for i in range(n - 1):
+        link(i, i + 1, -1.0)
+    link(0, n - 1, 0.5)
+    f = M.Envelope.factor(a)
+    x = [math.sin(i) + 0.3 for i in range(n)]
+    b = [sum(v * x[j] for j, v in row) for row in a]
+    f.solve(b)
+    assert max(abs(p - q) for p, q in zip(b, x)) < 1e-12
+    dense = np.zeros((n, n))
+    for i, row in enumerate(a):
+        for j, v in row:
+            dense[i, j] = v
+    L = np.zeros((n, n))
+    for i in range(n):
+        L[i, f.first[i]:i + 1] = f.rows[i]
+    assert np.allclose(L @ L.T, dense, rtol=0, atol=1e-12)
+    assert M.Envelope.factor([[(0, 1.0), (1, 2.0)], [(0, 2.0), (1, 1.0)]]) is None
+
+
+def test_rcm_order_keeps_a_path_banded():
+    """A path graph given in scrambled order comes out as the path: the
+    search starts at the end of smaller number (both ends have the minimum
+    degree), 2, and the reversed order ends there."""
+    perm = [3, 0, 5, 1, 4, 2]
+    adj = {perm[i]: [] for i in range(6)}
+    for i in range(5):
+        adj[perm[i]].append(perm[i + 1])
+        adj[perm[i + 1]].append(perm[i])
+    order = M.rcm_order(sorted(adj), lambda q: sorted(adj[q]))
+    assert order == perm
+
+
+def test_rejects_bad_input():
+    g, dt, lanes = _engine(0.9)
+    c = K.ConformalPec()
+    c.push_edge(K.ConformalEdge(0, (12, 0, 0), 0.5))
+    with pytest.raises(E.EmsError):
+        c.build(lanes, g, dt)
+    c = K.ConformalPec()
+    c.push_face(K.ConformalFace(0, (0, 0, 0), 1.5))
+    with pytest.raises(E.EmsError):
+        c.build(lanes, g, dt)
+    c = K.ConformalPec()
+    c.push_face_with_centroid(K.ConformalFace(2, (3, 3, 3), 0.3), (0.5, 1.2))
+    with pytest.raises(E.EmsError):
+        c.build(lanes, g, dt)
+    with pytest.raises(E.EmsError):
+        _sliver(options=K.ConformalOptions(iterations=300)).build(lanes, g, dt)
+
+
+def test_cell_bounds_at_once_equal_the_cell_by_cell_form():
+    """The batched bound of many cells (graded grid, cut edges and faces,
+    boundary cells) against cell_stability one cell at a time."""
+    rng = np.random.default_rng(3)
+    lines = [np.cumsum(np.r_[0.0, rng.uniform(0.5e-3, 1.5e-3, 9)]) for _ in range(3)]
+    g = E.Grid(lines)
+    dt = E.TimeStep.courant(0.9).resolve(g)
+    lanes = E.coefficients(g, None, [E.FaceCondition.Pec] * 6, dt)
+    co = M.Overlay(lanes.reshape(12, -1))
+    n = g.nodes()
+    for _ in range(40):
+        a = int(rng.integers(3))
+        p = [int(rng.integers(n[d] - 1)) for d in range(3)]
+        k = M.index(n, p)
+        co[M.CB + a, k] = np.float32(float(co[M.CB + a, k]) * rng.uniform(0.1, 1.0))
+    fr = {(int(rng.integers(3)), M.index(n, [int(rng.integers(n[d] - 1)) for d in range(3)])):
+          float(rng.uniform(0.05, 1.0)) for _ in range(40)}
+    cells = np.array([[i, j, k] for i in range(n[0]) for j in range(0, n[1], 3)
+                      for k in range(0, n[2], 2)])
+    got = M.cell_bounds(co, g, dt, cells, fr)
+    want = np.array([M.cell_stability(co, g, dt, tuple(c), fr) for c in cells])
+    assert np.allclose(got, want, rtol=1e-10, atol=0), np.abs(got - want).max()
