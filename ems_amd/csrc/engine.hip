@@ -1440,6 +1440,8 @@ struct DevMerged {
   const double *ev;
   const uint32_t *brs, *bc;
   const double *bv;
+  const uint32_t *cs, *cr;
+  const double *cv;
   const int32_t *fax;
   const uint64_t *fidx;
   const uint32_t *fs, *fg;
@@ -1499,23 +1501,81 @@ __global__ void __launch_bounds__(256) mf_circ(Fields f, DevMerged m) {
   m.circ[g] = c;
 }
 
-// The update of every reduced current: a wave per block (its lanes form the
-// right-hand sides, lane 0 runs the solve's serial chain on them in LDS,
-// or in the global scratch for a block of more than lds_rows rows), then
-// a lane per merge group.
-__global__ void __launch_bounds__(64) mf_advance(DevMerged m, int lds_rows) {
+// LDS bytes of a block's image in mf_advance: x, its envelope values and
+// columns (f64), per row the envelope's first column, the row and column
+// offsets, and the columns' rows (u32).
+__host__ __device__ __forceinline__ size_t mf_lds_need(uint32_t nr, uint64_t nev, uint32_t ncol) {
+  return 8 * ((size_t)nr + nev + ncol) + 4 * (3 * (size_t)nr + 2 + ncol);
+}
+
+// The update of the reduced currents: a wave per block of `list`, then (in
+// the launch with the merge groups) a lane per merge group. The block's lanes stage its factor in LDS and form the
+// right-hand sides; lane 0 runs the serial substitutions on LDS, the last
+// solved value kept in a register. The backward substitution goes column by
+// column (x_j = (y_j - sum over k > j, decreasing, of L_kj x_k) / L_jj), the
+// very operations of the row-by-row form in the same order. A block whose
+// image exceeds lds_bytes is solved in the global scratch row by row.
+__global__ void __launch_bounds__(64) mf_advance(DevMerged m, const uint32_t *__restrict__ list,
+                                                 int count, int lds_bytes) {
   extern __shared__ double xs[];
-  const int b = blockIdx.x;
-  if (b >= m.nb) {
-    const int q = (b - m.nb) * 64 + (int)threadIdx.x;
+  if ((int)blockIdx.x >= count) {
+    const int q = ((int)blockIdx.x - count) * 64 + (int)threadIdx.x;
     if (q < m.ns) mf_scalar(m, q, m.cur, m.circ, m.next);
     return;
   }
+  const uint32_t b = list[blockIdx.x];
   const uint32_t r0 = m.bs[b], nr = m.bs[b + 1] - r0;
-  double *x = nr <= (uint32_t)lds_rows ? xs : m.x + r0;
+  const uint64_t e0 = m.eo[r0], nev = m.eo[r0 + nr] - e0;
+  const uint32_t c0 = m.cs[r0], ncol = m.cs[r0 + nr] - c0;
+  if (mf_lds_need(nr, nev, ncol) > (size_t)lds_bytes) {
+    double *x = m.x + r0;
+    for (uint32_t i = threadIdx.x; i < nr; i += 64) x[i] = mf_rhs(m, r0, i, m.cur, m.circ);
+    __syncthreads();
+    if (threadIdx.x == 0) mf_solve(m, r0, nr, x);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nr; i += 64)
+      m.next[m.mem[r0 + i]] = (double)flush_hd((float)x[i]);
+    return;
+  }
+  double *x = xs, *env = xs + nr, *cv = env + nev;
+  uint32_t *ef = (uint32_t *)(cv + ncol), *eo = ef + nr, *cs = eo + nr + 1, *ck = cs + nr + 1;
+  for (uint32_t i = threadIdx.x; i < nr; i += 64) ef[i] = m.ef[r0 + i];
+  for (uint32_t i = threadIdx.x; i <= nr; i += 64) {
+    eo[i] = (uint32_t)(m.eo[r0 + i] - e0);
+    cs[i] = m.cs[r0 + i] - c0;
+  }
+  for (uint64_t e = threadIdx.x; e < nev; e += 64) env[e] = m.ev[e0 + e];
+  for (uint32_t e = threadIdx.x; e < ncol; e += 64) {
+    cv[e] = m.cv[c0 + e];
+    ck[e] = m.cr[c0 + e];
+  }
   for (uint32_t i = threadIdx.x; i < nr; i += 64) x[i] = mf_rhs(m, r0, i, m.cur, m.circ);
   __syncthreads();
-  if (threadIdx.x == 0) mf_solve(m, r0, nr, x);
+  if (threadIdx.x == 0) {
+    double xp = 0.0;
+    for (uint32_t i = 0; i < nr; i++) {
+      const uint32_t f = ef[i];
+      const double *row = env + eo[i];
+      double s = x[i];
+      if (f < i) {
+        for (uint32_t j = f; j + 1 < i; j++) s -= row[j - f] * x[j];
+        s -= row[i - 1 - f] * xp;
+      }
+      xp = s / row[i - f];
+      x[i] = xp;
+    }
+    for (uint32_t j = nr; j-- > 0;) {
+      double s = x[j];
+      uint32_t q = cs[j];
+      const uint32_t q1 = cs[j + 1];
+      if (q < q1) {
+        for (; q + 1 < q1; q++) s -= cv[q] * x[ck[q]];
+        s -= cv[q] * (ck[q] == j + 1 ? xp : x[ck[q]]);
+      }
+      xp = s / env[eo[j] + j - ef[j]];
+      x[j] = xp;
+    }
+  }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nr; i += 64)
     m.next[m.mem[r0 + i]] = (double)flush_hd((float)x[i]);
@@ -1763,7 +1823,10 @@ struct HostOp {
   double *pw_src = nullptr;
   // reduced currents of cut faces: device tables and state, host copies
   DevMerged mf{};
-  int mf_lds_rows = 0;  // rows of the largest block solved in LDS (at most 8192)
+  // blocks by the size of their LDS image: up to 8 KiB (small), above;
+  // the LDS of each launch (the largest image of its blocks, at most 64 KiB)
+  uint32_t *mf_small = nullptr, *mf_big = nullptr;
+  int mf_nsmall = 0, mf_nbig = 0, mf_lds_small = 0, mf_lds_big = 0;
   std::shared_ptr<struct MergedHost> mfh;
 };
 
@@ -2189,10 +2252,15 @@ int launch_merged(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
   if (m.ng)
     hipLaunchKernelGGL(mf_circ, dim3((m.ng + 255) / 256), dim3(256), 0, h->stream, f, m);
   if (stage == 0) {
-    const int na = m.nb + (m.ns + 63) / 64;
+    // small blocks (and the merge groups) with a small LDS image, so that
+    // many waves share a CU; then the large blocks
+    const int na = o.mf_nsmall + (m.ns + 63) / 64;
     if (na)
-      hipLaunchKernelGGL(mf_advance, dim3(na), dim3(64), (size_t)o.mf_lds_rows * sizeof(double),
-                         h->stream, m, o.mf_lds_rows);
+      hipLaunchKernelGGL(mf_advance, dim3(na), dim3(64), (size_t)o.mf_lds_small, h->stream, m,
+                         (const uint32_t *)o.mf_small, o.mf_nsmall, o.mf_lds_small);
+    if (o.mf_nbig)
+      hipLaunchKernelGGL(mf_advance, dim3(o.mf_nbig), dim3(64), (size_t)o.mf_lds_big, h->stream,
+                         m, (const uint32_t *)o.mf_big, o.mf_nbig, o.mf_lds_big);
     const int nd = std::max(m.nf, m.ng);
     if (nd)
       hipLaunchKernelGGL(mf_drive, dim3((nd + 255) / 256), dim3(256), 0, h->stream, f, m);
@@ -3125,6 +3193,18 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
       }
     }
   }
+  if (!d->col_start || d->col_start[0] != 0)
+    return fail(h, EMS_E_INVALID, "reduced currents: column offsets");
+  for (uint32_t b = 0; b < nb; b++) {
+    const uint32_t r0 = d->blk_start[b], r1 = d->blk_start[b + 1];
+    for (uint32_t r = r0; r < r1; r++) {
+      if (d->col_start[r + 1] < d->col_start[r])
+        return fail(h, EMS_E_INVALID, "reduced currents: column ranges");
+      for (uint32_t q = d->col_start[r]; q < d->col_start[r + 1]; q++)
+        if (!d->col_row || !d->col_val || d->col_row[q] <= r - r0 || d->col_row[q] >= r1 - r0)
+          return fail(h, EMS_E_INVALID, "reduced currents: column row out of the block");
+    }
+  }
   if (d->env_off[0] != 0 || d->brow_start[0] != 0 || d->mrow_start[0] != 0)
     return fail(h, EMS_E_INVALID, "reduced currents: row offsets");
   if (d->face_start[0] != 0) return fail(h, EMS_E_INVALID, "reduced currents: face ranges");
@@ -3169,6 +3249,10 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
   PUT(m.brs, d->brow_start, nr + 1, uint32_t);
   PUT(m.bc, d->bcol, nbv, uint32_t);
   PUT(m.bv, d->bval, nbv, double);
+  const uint32_t ncv = nr ? d->col_start[nr] : 0;
+  PUT(m.cs, d->col_start, nr + 1, uint32_t);
+  PUT(m.cr, d->col_row, ncv, uint32_t);
+  PUT(m.cv, d->col_val, ncv, double);
   PUT(m.fax, d->face_axis, nf, int32_t);
   PUT(m.fidx, d->face_idx, nf, uint64_t);
   PUT(m.fs, d->face_start, nf + 1, uint32_t);
@@ -3196,9 +3280,24 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
   cp(mh->mcol, d->mcol, nmv);
   cp(mh->mval, d->mval, nmv);
   o.mfh = mh;
-  uint32_t most = 0;
-  for (uint32_t b = 0; b < nb; b++) most = std::max(most, d->blk_start[b + 1] - d->blk_start[b]);
-  o.mf_lds_rows = (int)std::min<uint32_t>(most, 8192);
+  std::vector<uint32_t> small, big;
+  size_t lds_small = 0, lds_big = 0;
+  for (uint32_t b = 0; b < nb; b++) {
+    const uint32_t r0 = d->blk_start[b], r1 = d->blk_start[b + 1];
+    const size_t need = mf_lds_need(r1 - r0, d->env_off[r1] - d->env_off[r0],
+                                    d->col_start[r1] - d->col_start[r0]);
+    if (need <= 8192) {
+      small.push_back(b);
+      lds_small = std::max(lds_small, need);
+    } else {
+      big.push_back(b);
+      if (need <= 65536) lds_big = std::max(lds_big, need);
+    }
+  }
+  if ((rc = put(small.data(), small.size() * sizeof(uint32_t), (void **)&o.mf_small))) return rc;
+  if ((rc = put(big.data(), big.size() * sizeof(uint32_t), (void **)&o.mf_big))) return rc;
+  o.mf_nsmall = (int)small.size(), o.mf_nbig = (int)big.size();
+  o.mf_lds_small = (int)lds_small, o.mf_lds_big = (int)lds_big;
   h->ops.push_back(o);
   h->ops_dirty = true;
   if (id) *id = (int)h->ops.size() - 1;

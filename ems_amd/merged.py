@@ -365,7 +365,22 @@ class MergedFaces:
         bs = [0]
         mem, ef, eo, ev, brs, bc, bv = [], [], [0], [], [0], [], []
         mrs, mc, mv = [0], [], []
+        cs, cr, cv = [0], [], []
         for blk in self.blocks:
+            # column j of the factor: (k, L[k][j]) over the rows k > j whose
+            # envelope holds j, in decreasing k (the order in which the
+            # backward substitution applies them)
+            nr = len(blk.members)
+            cols = [[] for _ in range(nr)]
+            for k in range(nr - 1, -1, -1):
+                f = blk.factor.first[k]
+                for j in range(f, k):
+                    cols[j].append((k, blk.factor.rows[k][j - f]))
+            for j in range(nr):
+                for k, v in cols[j]:
+                    cr.append(k)
+                    cv.append(v)
+                cs.append(len(cr))
             mem += blk.members
             for i in range(len(blk.members)):
                 ef.append(blk.factor.first[i])
@@ -394,6 +409,8 @@ class MergedFaces:
             l=np.array([x.l for x in g], np.float64),
             mrow_start=np.array(mrs, np.uint32), mcol=np.array(mc, np.uint32),
             mval=np.array(mv, np.float64),
+            col_start=np.array(cs, np.uint32), col_row=np.array(cr, np.uint32),
+            col_val=np.array(cv, np.float64),
             term_start=ts, term_axis=np.array(tax, np.int32), term_idx=np.array(tidx, np.uint64),
             term_w=np.array(tw, np.float64),
             blk_start=np.array(bs, np.uint32), members=np.array(mem, np.uint32),
@@ -1042,6 +1059,7 @@ class MergedDesc(C.Structure):
                 ("env_off", _u64p), ("env_val", _dp),
                 ("brow_start", _u32p), ("bcol", _u32p), ("bval", _dp),
                 ("mrow_start", _u32p), ("mcol", _u32p), ("mval", _dp),
+                ("col_start", _u32p), ("col_row", _u32p), ("col_val", _dp),
                 ("face_axis", _i32p), ("face_idx", _u64p), ("face_start", _u32p),
                 ("face_group", _u32p), ("face_w", _dp)]
 
@@ -1054,7 +1072,7 @@ def make_desc(op: MergedFaces):
     d.n_groups, d.n_scalar = len(op.groups), op.scalar
     d.n_blocks, d.n_faces, d.dt = len(op.blocks), len(op.faces), op.dt
     types = {"da": _dp, "db": _dp, "l": _dp, "term_w": _dp, "env_val": _dp, "bval": _dp,
-             "mval": _dp, "face_w": _dp, "term_axis": _i32p, "face_axis": _i32p,
+             "mval": _dp, "col_val": _dp, "face_w": _dp, "term_axis": _i32p, "face_axis": _i32p,
              "term_idx": _u64p, "env_off": _u64p, "face_idx": _u64p}
     for k, v in arr.items():
         setattr(d, k, v.ctypes.data_as(types.get(k, _u32p)))

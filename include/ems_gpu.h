@@ -250,6 +250,12 @@ typedef struct {
     const uint32_t *mrow_start; /* rows + 1 */
     const uint32_t *mcol;
     const double *mval;
+    /* the factor by columns: for row j the rows k > j whose envelope holds
+     * j, in decreasing k, and L[k][j] (the backward substitution taken
+     * column by column; the same operations in the same order) */
+    const uint32_t *col_start;  /* rows + 1 */
+    const uint32_t *col_row;
+    const double *col_val;
     const int32_t *face_axis;
     const uint64_t *face_idx;   /* flat node index of the driven H edge */
     const uint32_t *face_start; /* n_faces + 1 */

@@ -154,6 +154,8 @@ typedef struct {
     const double *bval;
     const uint32_t *mrow_start, *mcol;
     const double *mval;
+    const uint32_t *col_start, *col_row; /* unused here (a layout of the device's) */
+    const double *col_val;
     const int32_t *face_axis;
     const uint64_t *face_idx;
     const uint32_t *face_start, *face_group;

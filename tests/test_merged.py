@@ -234,3 +234,39 @@ def test_cell_bounds_at_once_equal_the_cell_by_cell_form():
     got = M.cell_bounds(co, g, dt, cells, fr)
     want = np.array([M.cell_stability(co, g, dt, tuple(c), fr) for c in cells])
     assert np.allclose(got, want, rtol=1e-10, atol=0), np.abs(got - want).max()
+
+
+def test_columns_give_the_row_solve_bit_for_bit():
+    """The device's backward substitution by columns (the factor's column
+    lists of the descriptor, decreasing rows) performs the row-by-row
+    solve's operations in the same order: equal to Envelope.solve bit for
+    bit on the blocks of a cut cylinder."""
+    g = E.Grid.uniform([30, 30, 8], [1e-3] * 3)
+    m = E.CellMaterials.vacuum(g)
+    pec = K.cylinder_z(g, (15.2e-3, 14.9e-3), 9.3e-3, mats=m, metal="inside", z_nodes=(1, 7))
+    dt = E.TimeStep.courant(0.99).resolve(g)
+    lanes = E.coefficients(g, m, [E.FaceCondition.Pec] * 6, dt)
+    _, rep, op = pec.build(lanes, g, dt)
+    d = op.desc_arrays()
+    rng = np.random.default_rng(5)
+    big = 0
+    for b, blk in enumerate(op.blocks):
+        r0, nr = int(d["blk_start"][b]), len(blk.members)
+        big = max(big, nr)
+        y = list(rng.standard_normal(nr))
+        want = list(y)
+        blk.factor.solve(want)
+        x = list(y)
+        for i in range(nr):   # forward, as the row form
+            f, row = blk.factor.first[i], blk.factor.rows[i]
+            s = x[i]
+            for j in range(f, i):
+                s -= row[j - f] * x[j]
+            x[i] = s / row[i - f]
+        for j in range(nr - 1, -1, -1):
+            s = x[j]
+            for q in range(int(d["col_start"][r0 + j]), int(d["col_start"][r0 + j + 1])):
+                s -= float(d["col_val"][q]) * x[int(d["col_row"][q])]
+            x[j] = s / blk.factor.rows[j][j - blk.factor.first[j]]
+        assert np.array_equal(np.array(x).view(np.uint64), np.array(want).view(np.uint64)), b
+    assert big > 10 and rep.interpolated > 0
