@@ -1508,6 +1508,102 @@ __host__ __device__ __forceinline__ size_t mf_lds_need(uint32_t nr, uint64_t nev
   return 8 * ((size_t)nr + nev + ncol) + 4 * (3 * (size_t)nr + 2 + ncol);
 }
 
+// LDS bytes of a narrow block's image (envelopes at most 3 below the
+// diagonal): per row 4 envelope values, 3 column values, x, a packed word.
+__host__ __device__ __forceinline__ size_t mf_narrow_need(uint32_t nr) {
+  return 68 * (size_t)nr;
+}
+
+// One narrow block by a wave: the lanes stage each row's envelope (len =
+// i - f + 1 <= 4 values), its column (up to 3 rows k = j + dk, decreasing)
+// and right-hand side; lane 0 then substitutes with the last three solved
+// values in registers and the next row's data loaded one row ahead, the
+// terms in the row form's order (ascending j forward, decreasing k
+// backward).
+__device__ __forceinline__ void mf_narrow(const DevMerged &m, uint32_t r0, uint32_t nr,
+                                          double *xs) {
+  double *rv = xs, *cw = rv + 4 * (size_t)nr, *x = cw + 3 * (size_t)nr;
+  uint32_t *meta = (uint32_t *)(x + nr);
+  for (uint32_t i = threadIdx.x; i < nr; i += 64) {
+    const uint32_t f = m.ef[r0 + i], len = i - f + 1;
+    const uint64_t e = m.eo[r0 + i];
+    for (uint32_t t = 0; t < len; t++) rv[4 * (size_t)i + t] = m.ev[e + t];
+    const uint32_t q0 = m.cs[r0 + i], nc = m.cs[r0 + i + 1] - q0;
+    uint32_t w = len | (nc << 4);
+    for (uint32_t t = 0; t < nc; t++) {
+      cw[3 * (size_t)i + t] = m.cv[q0 + t];
+      w |= (m.cr[q0 + t] - i) << (8 + 2 * t);
+    }
+    meta[i] = w;
+    x[i] = mf_rhs(m, r0, i, m.cur, m.circ);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double w1 = 0.0, w2 = 0.0, w3 = 0.0;
+    uint32_t mt = meta[0];
+    double a0 = rv[0], a1 = rv[1], a2 = rv[2], a3 = rv[3], y = x[0];
+    for (uint32_t i = 0; i < nr; i++) {
+      const uint32_t len = mt & 7;
+      const double b0 = a0, b1 = a1, b2 = a2, b3 = a3;
+      double s = y;
+      if (i + 1 < nr) {
+        const double *r = rv + 4 * (size_t)(i + 1);
+        mt = meta[i + 1], a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3], y = x[i + 1];
+      }
+      double d;
+      if (len == 4) {
+        s -= b0 * w3;
+        s -= b1 * w2;
+        s -= b2 * w1;
+        d = b3;
+      } else if (len == 3) {
+        s -= b0 * w2;
+        s -= b1 * w1;
+        d = b2;
+      } else if (len == 2) {
+        s -= b0 * w1;
+        d = b1;
+      } else {
+        d = b0;
+      }
+      const double xi = s / d;
+      x[i] = xi;
+      w3 = w2, w2 = w1, w1 = xi;
+    }
+    // backward: w1, w2, w3 = x[j + 1], x[j + 2], x[j + 3]
+    w1 = w2 = w3 = 0.0;
+    const uint32_t jl = nr - 1;
+    mt = meta[jl];
+    double c0 = cw[3 * (size_t)jl], c1 = cw[3 * (size_t)jl + 1], c2 = cw[3 * (size_t)jl + 2];
+    double dg = rv[4 * (size_t)jl + (mt & 7) - 1];
+    y = x[jl];
+    for (uint32_t j = nr; j-- > 0;) {
+      const uint32_t nc = (mt >> 4) & 3, dk = mt >> 8;
+      const double e0 = c0, e1 = c1, e2 = c2, dj = dg;
+      double s = y;
+      if (j > 0) {
+        const uint32_t jn = j - 1;
+        mt = meta[jn];
+        c0 = cw[3 * (size_t)jn], c1 = cw[3 * (size_t)jn + 1], c2 = cw[3 * (size_t)jn + 2];
+        dg = rv[4 * (size_t)jn + (mt & 7) - 1];
+        y = x[jn];
+      }
+      const double ev[3] = {e0, e1, e2};
+      for (uint32_t t = 0; t < 3; t++) {
+        if (t >= nc) break;
+        const uint32_t k = (dk >> (2 * t)) & 3;
+        s -= ev[t] * (k == 1 ? w1 : (k == 2 ? w2 : w3));
+      }
+      const double xj = s / dj;
+      x[j] = xj;
+      w3 = w2, w2 = w1, w1 = xj;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nr; i += 64)
+    m.next[m.mem[r0 + i]] = (double)flush_hd((float)x[i]);
+}
+
 // The update of the reduced currents: a wave per block of `list`, then (in
 // the launch with the merge groups) a lane per merge group. The block's lanes stage its factor in LDS and form the
 // right-hand sides; lane 0 runs the serial substitutions on LDS, the last
@@ -1523,8 +1619,12 @@ __global__ void __launch_bounds__(64) mf_advance(DevMerged m, const uint32_t *__
     if (q < m.ns) mf_scalar(m, q, m.cur, m.circ, m.next);
     return;
   }
-  const uint32_t b = list[blockIdx.x];
+  const uint32_t b = list[blockIdx.x] & 0x7fffffffu;
   const uint32_t r0 = m.bs[b], nr = m.bs[b + 1] - r0;
+  if (list[blockIdx.x] >> 31) {
+    mf_narrow(m, r0, nr, xs);
+    return;
+  }
   const uint64_t e0 = m.eo[r0], nev = m.eo[r0 + nr] - e0;
   const uint32_t c0 = m.cs[r0], ncol = m.cs[r0 + nr] - c0;
   if (mf_lds_need(nr, nev, ncol) > (size_t)lds_bytes) {
@@ -3284,13 +3384,20 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
   size_t lds_small = 0, lds_big = 0;
   for (uint32_t b = 0; b < nb; b++) {
     const uint32_t r0 = d->blk_start[b], r1 = d->blk_start[b + 1];
-    const size_t need = mf_lds_need(r1 - r0, d->env_off[r1] - d->env_off[r0],
-                                    d->col_start[r1] - d->col_start[r0]);
+    bool narrow = true;
+    for (uint32_t r = r0; r < r1 && narrow; r++) narrow = r - r0 - d->env_first[r] <= 3;
+    size_t need = mf_lds_need(r1 - r0, d->env_off[r1] - d->env_off[r0],
+                              d->col_start[r1] - d->col_start[r0]);
+    uint32_t tag = b;
+    if (narrow && mf_narrow_need(r1 - r0) <= 65536) {
+      need = mf_narrow_need(r1 - r0);
+      tag |= 0x80000000u;
+    }
     if (need <= 8192) {
-      small.push_back(b);
+      small.push_back(tag);
       lds_small = std::max(lds_small, need);
     } else {
-      big.push_back(b);
+      big.push_back(tag);
       if (need <= 65536) lds_big = std::max(lds_big, need);
     }
   }
