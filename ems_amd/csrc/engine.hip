@@ -1771,100 +1771,173 @@ __global__ void __launch_bounds__(256) poles_capture(Fields f, DevPoleSide s) {
 
 // ---- energy invariant (kernel.rs:119-244, 364-423; pool.rs:673-692) ----
 
-// One H row's share, read-only (COMMIT = false): 4 threads per row, lane l
-// sums entries k = l mod 4 in order; lane 0 returns (a0 + a1) + (a2 + a3).
-// Rows are enumerated per plane as x rows (ny-1), y rows (ny), z rows (ny-1).
+// The chains below are the reference's sums, each in its own order; what is
+// parallel is the terms. A wave computes a round of up to 256 consecutive
+// terms at once (coalesced loads, one latency per round) into LDS, then
+// the lanes that own the chains add them in order.
+
+// acc + b[t] + b[t + st] + ... over indices < m, in that order; the LDS
+// reads issued 16 at a time so that one latency serves 16 additions.
+__device__ __forceinline__ double chain_add(double acc, const double *b, int t, int m, int st) {
+  int q = t;
+  for (; q + 15 * st < m; q += 16 * st) {
+    double v[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++) v[s] = b[q + st * s];
+#pragma unroll
+    for (int s = 0; s < 16; s++) acc += v[s];
+  }
+  for (; q < m; q += st) acc += b[q];
+  return acc;
+}
+
+// A lane or field array by a wave-uniform runtime index, through selects
+// with constant indices (a dynamic index into a kernel argument would go
+// through memory).
+__device__ __forceinline__ Lane pick_lane(const Lanes &L, int l) {
+  Lane x = L.l[0];
+#pragma unroll
+  for (int q = 1; q < 12; q++)
+    if (l == q) x = L.l[q];
+  return x;
+}
+__device__ __forceinline__ const float *pick3(float *const *a, int c) {
+  return c == 0 ? a[0] : (c == 1 ? a[1] : a[2]);
+}
+
+// H rows, a wave per row (rows per plane: x rows ny-1, y rows ny, z rows
+// ny-1, enumerated in that order): four accumulators over k = 0, 1, 2, 3
+// mod 4 in increasing k (h_energy_row, kernel.rs:119-216: the weighted
+// i^n i^(n+1), the update evaluated read-only), combined as
+// (a0 + a1) + (a2 + a3). Rounds of 512 terms, 8 per lane.
 template <bool OE>
 __global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, int ny, int nz,
                                                      double *rowsum) {
+  constexpr int NT = 8, RND = 64 * NT;
+  __shared__ double buf[4][RND];
+  const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
   const int rows_per_plane = 3 * ny - 2;
-  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = (int)(tid & 3);
-  const size_t rid = tid >> 2;
-  __shared__ double part[256];
-  double acc = 0.0;
+  const size_t rid = (size_t)blockIdx.x * 4 + wv;
   const size_t total = (size_t)nx * rows_per_plane;
-  const bool live = rid < total;
-  if (live) {
-    const int gi = (int)(rid / rows_per_plane);
-    const int rr = (int)(rid % rows_per_plane);
-    int comp, j;
-    if (rr < ny - 1) {
-      comp = 0, j = rr;
-    } else if (rr < 2 * ny - 1) {
-      comp = 1, j = rr - (ny - 1);
-    } else {
-      comp = 2, j = rr - (2 * ny - 1);
-    }
-    const size_t plane = (size_t)ny * nz;
-    const size_t base = (size_t)gi * plane + (size_t)j * nz;
-    const size_t r = (size_t)gi * ny + j;
-    const bool has_next = gi + 1 < nx;
-    if (comp == 0 || has_next) {
-      const int n = comp == 2 ? nz : nz - 1;
-      const float *ux = f.u[0], *uy = f.u[1], *uz = f.u[2];
-      const float *h = f.i[comp];
-      for (int k = lane; k < n; k += 4) {
-        const size_t o = base + k;
-        float a, b, c, d;
-        if (comp == 0) {
-          a = uz[o + nz], b = uz[o], c = uy[o + 1], d = uy[o];
-        } else if (comp == 1) {
-          a = ux[o + 1], b = ux[o], c = uz[o + plane], d = uz[o];
-        } else {
-          a = uy[o + plane], b = uy[o], c = ux[o + nz], d = ux[o];
-        }
-        const float da = coef(L.l[6 + comp], r, k, nz, o);
-        const float db = coef(L.l[9 + comp], r, k, nz, o);
-        const float old = h[o];
-        const float nw = h_expr<OE>(da, old, db, a, b, c, d);
-        acc += weight(da, (double)old * (double)nw, db);
-      }
-    }
+  if (rid >= total) return;  // uniform per wave; no block barrier below
+  const int gi = (int)(rid / rows_per_plane);
+  const int rr = (int)(rid % rows_per_plane);
+  int comp, j;
+  if (rr < ny - 1) {
+    comp = 0, j = rr;
+  } else if (rr < 2 * ny - 1) {
+    comp = 1, j = rr - (ny - 1);
+  } else {
+    comp = 2, j = rr - (2 * ny - 1);
   }
-  part[threadIdx.x] = acc;
-  __syncthreads();
-  if (live && lane == 0) {
-    const double *pp = part + threadIdx.x;
-    rowsum[rid] = (pp[0] + pp[1]) + (pp[2] + pp[3]);
-  }
-}
-
-// E part: per (plane, component, lane) the chain over in-plane indices
-// q = lane mod 4 in increasing order (e_energy_row, kernel.rs:218-244).
-__global__ void __launch_bounds__(64) energy_e_chains(Fields f, Lanes L, int nx, int ny, int nz,
-                                                      double *chains) {
-  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)nx * 3 * 4;
-  if (tid >= total) return;
-  const int lane = (int)(tid & 3);
-  const int a = (int)((tid >> 2) % 3);
-  const int gi = (int)((tid >> 2) / 3);
   const size_t plane = (size_t)ny * nz;
-  const float *u = f.u[a] + (size_t)gi * plane;
-  double acc = 0.0;
-  for (size_t q = lane; q < plane; q += 4) {
-    const int j = (int)(q / nz), k = (int)(q - (size_t)j * nz);
-    const size_t r = (size_t)gi * ny + j;
-    const size_t o = (size_t)gi * plane + q;
-    const float ca = coef(L.l[a], r, k, nz, o), cb = coef(L.l[3 + a], r, k, nz, o);
-    const double x = (double)u[q];
-    acc += weight(ca, x * x, cb);
+  const size_t base = (size_t)gi * plane + (size_t)j * nz;
+  const size_t r = (size_t)gi * ny + j;
+  const int n = (comp == 0 || gi + 1 < nx) ? (comp == 2 ? nz : nz - 1) : 0;
+  // the curl's four voltages: a(o + sa) - b(o) - (c(o + sc) - d(o))
+  const int ca_ = comp == 0 ? 2 : (comp == 1 ? 0 : 1), cc_ = comp == 0 ? 1 : (comp == 1 ? 2 : 0);
+  const float *A = pick3(f.u, ca_), *Cc = pick3(f.u, cc_), *H = pick3(f.i, comp);
+  const size_t sa = comp == 0 ? (size_t)nz : (comp == 1 ? 1 : plane);
+  const size_t sc = comp == 0 ? 1 : (comp == 1 ? plane : (size_t)nz);
+  const Lane lda = pick_lane(L, 6 + comp), ldb = pick_lane(L, 9 + comp);
+  double acc = 0.0;  // lanes 0..3: the accumulator of k = lane mod 4
+  double *b = buf[wv];
+  for (int k0 = 0; k0 < n; k0 += RND) {
+    float va[NT], vb[NT], vc[NT], vd[NT], da[NT], db[NT], old[NT];
+#pragma unroll
+    for (int s = 0; s < NT; s++) {
+      const int k = min(k0 + 64 * s + t, n - 1);
+      const size_t o = base + k;
+      va[s] = A[o + sa], vb[s] = A[o], vc[s] = Cc[o + sc], vd[s] = Cc[o], old[s] = H[o];
+      da[s] = coef(lda, r, k, nz, o), db[s] = coef(ldb, r, k, nz, o);
+    }
+#pragma unroll
+    for (int s = 0; s < NT; s++) {
+      const float nw = h_expr<OE>(da[s], old[s], db[s], va[s], vb[s], vc[s], vd[s]);
+      b[64 * s + t] = k0 + 64 * s + t < n ? weight(da[s], (double)old[s] * (double)nw, db[s])
+                                          : 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (t < 4) acc = chain_add(acc, b, t, min(RND, n - k0), 4);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
-  chains[tid] = acc;
+  const double a1 = __shfl(acc, 1), a2 = __shfl(acc, 2), a3 = __shfl(acc, 3);
+  if (t == 0) rowsum[rid] = (acc + a1) + (a2 + a3);
 }
 
-// Plane values we + wh (h_slab, kernel.rs:404-420) and their sum in plane
-// order (pool.rs:675-692); one block.
-__global__ void __launch_bounds__(256) energy_combine(int nx, int ny, const double *rowsum,
-                                                      const double *chains, double *planes,
-                                                      double dt, double *out) {
+// E part: per (plane, component) four chains over the in-plane index
+// q = 0, 1, 2, 3 mod 4 in increasing q (e_energy_row, kernel.rs:218-244),
+// a wave per (plane, component), lanes 0..3 the chains. Rounds of 1024
+// terms, 16 per lane.
+__global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx, int ny, int nz,
+                                                       double *chains) {
+  constexpr int NT = 16, RND = 64 * NT;
+  __shared__ double buf[4][RND];
+  const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
+  const size_t w = (size_t)blockIdx.x * 4 + wv;
+  if (w >= (size_t)nx * 3) return;  // uniform per wave
+  const int a = (int)(w % 3), gi = (int)(w / 3);
+  const size_t plane = (size_t)ny * nz;
+  const float *u = pick3(f.u, a) + (size_t)gi * plane;
+  const Lane la = pick_lane(L, a), lb = pick_lane(L, 3 + a);
+  double acc = 0.0;
+  double *b = buf[wv];
+  for (size_t q0 = 0; q0 < plane; q0 += RND) {
+    float x[NT], ca[NT], cb[NT];
+#pragma unroll
+    for (int s = 0; s < NT; s++) {
+      const size_t q = min(q0 + 64 * s + t, plane - 1);
+      const int j = (int)(q / nz), k = (int)(q - (size_t)j * nz);
+      const size_t r = (size_t)gi * ny + j;
+      const size_t o = (size_t)gi * plane + q;
+      x[s] = u[q];
+      ca[s] = coef(la, r, k, nz, o), cb[s] = coef(lb, r, k, nz, o);
+    }
+#pragma unroll
+    for (int s = 0; s < NT; s++) {
+      const double xd = (double)x[s];
+      b[64 * s + t] = q0 + 64 * s + t < plane ? weight(ca[s], xd * xd, cb[s]) : 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (t < 4) acc = chain_add(acc, b, t, (int)min((size_t)RND, plane - q0), 4);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (t < 4) chains[w * 4 + t] = acc;
+}
+
+// Plane values we + wh (h_slab, kernel.rs:404-420): a wave per plane, the
+// plane's row sums added in row order by lane 0 from LDS.
+__global__ void __launch_bounds__(256) energy_planes(int nx, int ny, const double *rowsum,
+                                                     const double *chains, double *planes) {
+  __shared__ double buf[4][256];
+  const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
+  const int gi = (int)blockIdx.x * 4 + wv;
+  if (gi >= nx) return;  // uniform per wave
   const int rows_per_plane = 3 * ny - 2;
-  for (int gi = threadIdx.x; gi < nx; gi += blockDim.x) {
-    const double *rs = rowsum + (size_t)gi * rows_per_plane;
-    double wh = 0.0;
-    const int nrows = gi + 1 < nx ? rows_per_plane : ny - 1;
-    for (int q = 0; q < nrows; q++) wh += rs[q];
+  const double *rs = rowsum + (size_t)gi * rows_per_plane;
+  const int nrows = gi + 1 < nx ? rows_per_plane : ny - 1;
+  double wh = 0.0;
+  double *b = buf[wv];
+  for (int q0 = 0; q0 < nrows; q0 += 256) {
+    double v[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int q = q0 + 64 * s + t;
+      v[s] = q < nrows ? rs[q] : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; s++) b[64 * s + t] = v[s];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (t == 0) wh = chain_add(wh, b, 0, min(256, nrows - q0), 1);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (t == 0) {
     double we = 0.0;
     for (int a = 0; a < 3; a++) {
       const double *c = chains + ((size_t)gi * 3 + a) * 4;
@@ -1872,12 +1945,26 @@ __global__ void __launch_bounds__(256) energy_combine(int nx, int ny, const doub
     }
     planes[gi] = we + wh;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = -0.0;
-    for (int gi = 0; gi < nx; gi++) s += planes[gi];
-    *out = 0.25 * dt * s;
+}
+
+// The plane values summed in plane order (pool.rs:675-692), folding from
+// -0.0, scaled by dt/4; one wave.
+__global__ void __launch_bounds__(64) energy_combine(int nx, const double *planes, double dt,
+                                                     double *out) {
+  __shared__ double buf[256];
+  const int t = threadIdx.x;
+  double s = -0.0;
+  for (int g0 = 0; g0 < nx; g0 += 256) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int g = g0 + 64 * k + t;
+      buf[64 * k + t] = g < nx ? planes[g] : 0.0;
+    }
+    __syncthreads();
+    if (t == 0) s = chain_add(s, buf, 0, min(256, nx - g0), 1);
+    __syncthreads();
   }
+  if (t == 0) *out = 0.25 * dt * s;
 }
 
 // ------------------------------------------------------------------ host --
@@ -2642,8 +2729,8 @@ int energy_now(ems_gpu *h, double *w) {
     CK(h, hipMalloc(&h->d_planes, (size_t)nx * sizeof(double)));
     CK(h, hipMalloc(&h->d_w, sizeof(double)));
   }
-  const size_t threads = rows * 4;
-  dim3 g1((unsigned)((threads + 255) / 256));
+  // a wave per H row, per (plane, component) E chains and per plane
+  dim3 g1((unsigned)((rows + 3) / 4));
   if (h->oe)
     hipLaunchKernelGGL(energy_h_rows<true>, g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny,
                        nz, h->d_rowsum);
@@ -2651,12 +2738,15 @@ int energy_now(ems_gpu *h, double *w) {
     hipLaunchKernelGGL(energy_h_rows<false>, g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny,
                        nz, h->d_rowsum);
   CK(h, hipGetLastError());
-  dim3 g2((unsigned)(((size_t)nx * 12 + 63) / 64));
-  hipLaunchKernelGGL(energy_e_chains, g2, dim3(64), 0, h->stream, f, lanes_arg(h), nx, ny, nz,
+  dim3 g2((unsigned)(((size_t)nx * 3 + 3) / 4));
+  hipLaunchKernelGGL(energy_e_chains, g2, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny, nz,
                      h->d_chains);
   CK(h, hipGetLastError());
-  hipLaunchKernelGGL(energy_combine, dim3(1), dim3(256), 0, h->stream, nx, ny, h->d_rowsum,
-                     h->d_chains, h->d_planes, h->dt, h->d_w);
+  hipLaunchKernelGGL(energy_planes, dim3((unsigned)((nx + 3) / 4)), dim3(256), 0, h->stream, nx,
+                     ny, h->d_rowsum, h->d_chains, h->d_planes);
+  CK(h, hipGetLastError());
+  hipLaunchKernelGGL(energy_combine, dim3(1), dim3(64), 0, h->stream, nx, h->d_planes, h->dt,
+                     h->d_w);
   CK(h, hipGetLastError());
   double v = 0;
   CK(h, hipMemcpyAsync(&v, h->d_w, sizeof(double), hipMemcpyDeviceToHost, h->stream));
