@@ -2032,6 +2032,10 @@ struct ems_gpu {
   double dt = 0;
   int oe = 0;
   hipStream_t stream = nullptr;
+  // the reduced currents' update beside the H half-step (created with the
+  // first such operator; default priority)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   Fields fb[2]{};  // ping-pong field buffers (views into fbuf)
   float *fbuf[2] = {nullptr, nullptr};  // u_x..u_z, i_x..i_z, N nodes each
   // coefficient pool: a zero row, then every lane's rows; croff[16*r + l]
@@ -2129,6 +2133,12 @@ bool own_gap_kernels(const HostOp &o) { return o.type == OP_PW || o.type == OP_M
 bool has_gap_kernels(const ems_gpu *h) {
   for (auto &o : h->ops)
     if (own_gap_kernels(o)) return true;
+  return false;
+}
+
+bool has_merged(const ems_gpu *h) {
+  for (auto &o : h->ops)
+    if (o.type == OP_MERGED) return true;
   return false;
 }
 
@@ -2432,32 +2442,41 @@ int launch_pw(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
 }
 
 // The reduced currents' hooks (MergedFaces::post_h / post_e,
-// conformal.rs:2655-2682): stage 0 the circulations, the update and the
-// driven faces; stage 1 the circulations of the energy look-ahead.
-int launch_merged(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
+// conformal.rs:2655-2682), split by what they read: stage 2 the
+// circulations of u^n and the update (they read u, which the H half-step
+// leaves alone, and the operator's own state, so they run beside the H
+// update on the side stream); stage 0, at the operator's place in the mid
+// gap, writes the driven faces' currents; stage 1 the circulations of the
+// energy look-ahead in the post-E gap.
+int launch_merged(ems_gpu *h, const HostOp &o, const Fields &f, int stage, hipStream_t st) {
   const DevMerged &m = o.mf;
-  if (m.ng)
-    hipLaunchKernelGGL(mf_circ, dim3((m.ng + 255) / 256), dim3(256), 0, h->stream, f, m);
-  if (stage == 0) {
+  if (stage != 0 && m.ng)
+    hipLaunchKernelGGL(mf_circ, dim3((m.ng + 255) / 256), dim3(256), 0, st, f, m);
+  if (stage == 2) {
     // small blocks (and the merge groups) with a small LDS image, so that
     // many waves share a CU; then the large blocks
     const int na = o.mf_nsmall + (m.ns + 63) / 64;
     if (na)
-      hipLaunchKernelGGL(mf_advance, dim3(na), dim3(64), (size_t)o.mf_lds_small, h->stream, m,
+      hipLaunchKernelGGL(mf_advance, dim3(na), dim3(64), (size_t)o.mf_lds_small, st, m,
                          (const uint32_t *)o.mf_small, o.mf_nsmall, o.mf_lds_small);
     if (o.mf_nbig)
-      hipLaunchKernelGGL(mf_advance, dim3(o.mf_nbig), dim3(64), (size_t)o.mf_lds_big, h->stream,
-                         m, (const uint32_t *)o.mf_big, o.mf_nbig, o.mf_lds_big);
+      hipLaunchKernelGGL(mf_advance, dim3(o.mf_nbig), dim3(64), (size_t)o.mf_lds_big, st, m,
+                         (const uint32_t *)o.mf_big, o.mf_nbig, o.mf_lds_big);
+  }
+  if (stage == 0) {
     const int nd = std::max(m.nf, m.ng);
-    if (nd)
-      hipLaunchKernelGGL(mf_drive, dim3((nd + 255) / 256), dim3(256), 0, h->stream, f, m);
+    if (nd) hipLaunchKernelGGL(mf_drive, dim3((nd + 255) / 256), dim3(256), 0, st, f, m);
   }
   CK(h, hipGetLastError());
   return 0;
 }
 
+// An own-kernel operator's hooks at its place in a gap: stage 0 the mid
+// gap, 1 the post-E gap. The reduced currents' update itself (stage 2 of
+// launch_merged) ran before, beside the H update (enqueue_step).
 int launch_gap_op(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
-  return o.type == OP_PW ? launch_pw(h, o, f, stage) : launch_merged(h, o, f, stage);
+  return o.type == OP_PW ? launch_pw(h, o, f, stage)
+                         : launch_merged(h, o, f, stage, h->stream);
 }
 
 // Post-E gap on the result `f`; `fp` holds the state before a fused step
@@ -2499,6 +2518,26 @@ int launch_gap_mid(ems_gpu *h, const Fields &f) {
   return 0;
 }
 
+// Unfused steps with reduced currents: their update forks onto the side
+// stream before the H update and joins before the mid gap.
+int merged_fork(ems_gpu *h, const Fields &f) {
+  if (!h->side || !has_merged(h)) return 0;
+  CK(h, hipEventRecord(h->ev_fork, h->stream));
+  CK(h, hipStreamWaitEvent(h->side, h->ev_fork, 0));
+  for (auto &o : h->ops) {
+    if (o.type != OP_MERGED) continue;
+    int rc = launch_merged(h, o, f, 2, h->side);
+    if (rc) return rc;
+  }
+  CK(h, hipEventRecord(h->ev_join, h->side));
+  return 0;
+}
+
+int merged_join(ems_gpu *h) {
+  if (h->side && has_merged(h)) CK(h, hipStreamWaitEvent(h->stream, h->ev_join, 0));
+  return 0;
+}
+
 // One step starting from buffer `par`; returns the buffer of the result.
 int enqueue_step(ems_gpu *h, int &par) {
   int rc;
@@ -2509,9 +2548,11 @@ int enqueue_step(ems_gpu *h, int &par) {
     return launch_gap_post_e(h, h->fb[par], h->fb[par ^ 1], true);
   }
   const Fields &f = h->fb[par];
+  if ((rc = merged_fork(h, f))) return rc;
   if ((rc = launch_h(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 0, par))) return rc;
   if ((rc = launch_poles(h, 0, f, f, false))) return rc;
+  if ((rc = merged_join(h))) return rc;
   if ((rc = launch_gap_mid(h, f))) return rc;
   if ((rc = launch_e(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 1, par))) return rc;
@@ -2876,6 +2917,9 @@ void ems_gpu_destroy(ems_gpu *h) {
     (void)hipFree(h->d_planes), (void)hipFree(h->d_w);
   }
   if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->side) (void)hipStreamDestroy(h->side);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   delete h;
 }
 
@@ -3493,6 +3537,11 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
   }
   if ((rc = put(small.data(), small.size() * sizeof(uint32_t), (void **)&o.mf_small))) return rc;
   if ((rc = put(big.data(), big.size() * sizeof(uint32_t), (void **)&o.mf_big))) return rc;
+  if (!h->side) {
+    CK(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    CK(h, hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    CK(h, hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+  }
   o.mf_nsmall = (int)small.size(), o.mf_nbig = (int)big.size();
   o.mf_lds_small = (int)lds_small, o.mf_lds_big = (int)lds_big;
   h->ops.push_back(o);
@@ -3739,10 +3788,12 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
     } else {
       const Fields &f = h->fb[h->par];
       CK(h, hipEventRecord(ev[0], h->stream));
+      if ((rc = merged_fork(h, f))) return rc;
       if ((rc = launch_h(h, f))) return rc;
       CK(h, hipEventRecord(ev[1], h->stream));
       if ((rc = launch_cpml(h, f, 0, h->par))) return rc;
       if ((rc = launch_poles(h, 0, f, f, false))) return rc;
+      if ((rc = merged_join(h))) return rc;
       if ((rc = launch_gap_mid(h, f))) return rc;
       CK(h, hipEventRecord(ev[2], h->stream));
       if ((rc = launch_e(h, f))) return rc;
