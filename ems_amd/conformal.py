@@ -20,8 +20,9 @@ scaling (cb as f64 * f_e) as f32 in edge order, the stable faces from the
 scaled lanes, construct at the threshold, then the patch on top of the
 scaled lanes (changed, :2704-2715).
 
-Scope: Cartesian grids; no thin PEC sheets; no reduced currents in
-absorbing layers (rejected); the power-iteration stability estimate
+Inside CPML layers (set_cpml) small faces are merged and the merge groups
+apply the layer's H-side correction themselves. Scope: Cartesian grids;
+no thin PEC sheets; no cylindrical PML; the power-iteration stability estimate
 (conformal.rs:2858-2925) and the threshold rounds it drives are not run:
 ConformalOptions.iterations is 0, the report's stability NaN.
 """
@@ -108,6 +109,7 @@ class ConformalPec:
         self.centroids: List[Optional[Tuple[float, float]]] = []
         self.walls: List[Tuple[int, Tuple[int, int, int]]] = []
         self.absorbing = [0] * 6
+        self.cpml_h: list = []
 
     def push_edge(self, e: ConformalEdge) -> None:
         self.edges.append(e)
@@ -128,6 +130,18 @@ class ConformalPec:
         """An edge closed to interpolation and merging, not PEC
         (conformal.rs:509-515)."""
         self.walls.append((int(axis), tuple(int(x) for x in node)))
+
+    def set_cpml(self, cpml) -> None:
+        """ConformalPec::set_cpml (conformal.rs:555-571): the CPML's layers
+        become absorbing layers (small faces there are merged) and their
+        H-side grading is kept for the merge groups' own correction."""
+        for face, params, _nf in cpml.faces:
+            f = int(face)
+            self.set_absorbing_layers([(f, params.cells)])
+            h = cpml.h_profile(face)
+            if h is not None:
+                self.cpml_h = [l for l in self.cpml_h if l[0] != f]
+                self.cpml_h.append((f, params.cells, h))
 
     def set_absorbing_layers(self, layers) -> None:
         """(face index 0..5 = x-, x+, y-, y+, z-, z+, cells) pairs
@@ -174,11 +188,8 @@ class ConformalPec:
             kept, bounded = set(), False
         tau = min(max(opt.merge_below, 0.0), 1.0)
         b = M.construct(self.faces, self.centroids, self.edges, self.walls, co, grid, dt, tau,
-                        kept, min(max(opt.deviation_area, 0.0), 1.0), self.absorbing)
-        if b.driven_in_layer:
-            raise EmsError(nat.EMS_E_INVALID,
-                           "reduced currents of small cut faces inside an absorbing layer are not "
-                           "supported")
+                        kept, min(max(opt.deviation_area, 0.0), 1.0), self.absorbing,
+                        self.cpml_h)
         rep.faces, rep.interpolated, rep.hosts = b.faces, b.interpolated, b.hosts
         rep.merged, rep.groups, rep.floored, rep.kept = b.merged_faces, b.groups, b.floored, b.kept
         rep.bounded, rep.merge_below = bounded, tau

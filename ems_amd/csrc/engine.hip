@@ -1442,6 +1442,13 @@ struct DevMerged {
   const double *bv;
   const uint32_t *cs, *cr;
   const double *cv;
+  int np;
+  const int32_t *pax;
+  const uint64_t *pi0, *pi1;
+  const double *pc;
+  const uint32_t *pst, *pt;
+  const double *pw;
+  double *psi, *pcon;
   const int32_t *fax;
   const uint64_t *fidx;
   const uint32_t *fs, *fg;
@@ -1602,6 +1609,29 @@ __device__ __forceinline__ void mf_narrow(const DevMerged &m, uint32_t r0, uint3
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nr; i += 64)
     m.next[m.mem[r0 + i]] = (double)flush_hd((float)x[i]);
+}
+
+// MergedFaces::add_pml (conformal.rs:2582-2638): a thread per term, T and
+// psi (kept when commit: the H update; the energy look-ahead leaves the
+// state), the contribution k T + psi; then a thread per reduced current
+// adds w times its terms' contributions in term order.
+__global__ void __launch_bounds__(256) mf_pml_terms(Fields f, DevMerged m, int commit) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= m.np) return;
+  const double *c = m.pc + 4 * (size_t)q;
+  const float *u = f.u[m.pax[q]];
+  const double d = c[0] * ((double)u[m.pi0[q]] - (double)u[m.pi1[q]]);
+  const double psi = c[1] * m.psi[q] + c[2] * d;
+  if (commit) m.psi[q] = psi;
+  m.pcon[q] = c[3] * d + psi;
+}
+
+__global__ void __launch_bounds__(256) mf_pml_add(DevMerged m) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= m.ng || m.pst[g] == m.pst[g + 1]) return;
+  double c = m.circ[g];
+  for (uint32_t e = m.pst[g]; e < m.pst[g + 1]; e++) c += m.pw[e] * m.pcon[m.pt[e]];
+  m.circ[g] = c;
 }
 
 // The update of the reduced currents: a wave per block of `list`, then (in
@@ -2452,6 +2482,11 @@ int launch_merged(ems_gpu *h, const HostOp &o, const Fields &f, int stage, hipSt
   const DevMerged &m = o.mf;
   if (stage != 0 && m.ng)
     hipLaunchKernelGGL(mf_circ, dim3((m.ng + 255) / 256), dim3(256), 0, st, f, m);
+  if (stage != 0 && m.np) {
+    hipLaunchKernelGGL(mf_pml_terms, dim3((m.np + 255) / 256), dim3(256), 0, st, f, m,
+                       stage == 2 ? 1 : 0);
+    hipLaunchKernelGGL(mf_pml_add, dim3((m.ng + 255) / 256), dim3(256), 0, st, m);
+  }
   if (stage == 2) {
     // small blocks (and the merge groups) with a small LDS image, so that
     // many waves share a CU; then the large blocks
@@ -3439,6 +3474,22 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
           return fail(h, EMS_E_INVALID, "reduced currents: column row out of the block");
     }
   }
+  if (d->n_pml < 0 || !d->pml_start || d->pml_start[0] != 0)
+    return fail(h, EMS_E_INVALID, "reduced currents: CPML term tables");
+  const uint32_t npml = (uint32_t)d->n_pml;
+  for (uint32_t g = 0; g < ng; g++)
+    if (d->pml_start[g + 1] < d->pml_start[g])
+      return fail(h, EMS_E_INVALID, "reduced currents: CPML term ranges");
+  const uint32_t npw = d->pml_start[ng];
+  if ((npml && (!d->pml_axis || !d->pml_idx0 || !d->pml_idx1 || !d->pml_coef)) ||
+      (npw && (!d->pml_term || !d->pml_w)))
+    return fail(h, EMS_E_INVALID, "reduced currents: CPML terms");
+  for (uint32_t q = 0; q < npml; q++)
+    if (d->pml_axis[q] < 0 || d->pml_axis[q] > 2 || d->pml_idx0[q] >= h->N ||
+        d->pml_idx1[q] >= h->N)
+      return fail(h, EMS_E_INVALID, "reduced currents: CPML term voltage out of range");
+  for (uint32_t e = 0; e < npw; e++)
+    if (d->pml_term[e] >= npml) return fail(h, EMS_E_INVALID, "reduced currents: CPML target");
   if (d->env_off[0] != 0 || d->brow_start[0] != 0 || d->mrow_start[0] != 0)
     return fail(h, EMS_E_INVALID, "reduced currents: row offsets");
   if (d->face_start[0] != 0) return fail(h, EMS_E_INVALID, "reduced currents: face ranges");
@@ -3466,7 +3517,7 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
   DevMerged &m = o.mf;
   m.ng = (int)ng, m.ns = d->n_scalar, m.nb = (int)nb, m.nf = (int)nf, m.dt = d->dt;
   int rc;
-  const std::vector<double> z(std::max<size_t>(std::max<uint32_t>(ng, nr), 1), 0.0);
+  const std::vector<double> z(std::max<size_t>(std::max({ng, nr, npml}), 1), 0.0);
 #define PUT(dst, src, count, T)                                                           \
   if ((rc = put((src), (size_t)(count) * sizeof(T), (void **)&(dst)))) return rc;
   PUT(m.da, d->da, ng, double);
@@ -3487,6 +3538,16 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id) {
   PUT(m.cs, d->col_start, nr + 1, uint32_t);
   PUT(m.cr, d->col_row, ncv, uint32_t);
   PUT(m.cv, d->col_val, ncv, double);
+  m.np = (int)npml;
+  PUT(m.pax, d->pml_axis, npml, int32_t);
+  PUT(m.pi0, d->pml_idx0, npml, uint64_t);
+  PUT(m.pi1, d->pml_idx1, npml, uint64_t);
+  PUT(m.pc, d->pml_coef, 4 * (size_t)npml, double);
+  PUT(m.pst, d->pml_start, ng + 1, uint32_t);
+  PUT(m.pt, d->pml_term, npw, uint32_t);
+  PUT(m.pw, d->pml_w, npw, double);
+  PUT(m.psi, z.data(), npml, double);
+  PUT(m.pcon, z.data(), npml, double);
   PUT(m.fax, d->face_axis, nf, int32_t);
   PUT(m.fidx, d->face_idx, nf, uint64_t);
   PUT(m.fs, d->face_start, nf + 1, uint32_t);

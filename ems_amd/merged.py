@@ -26,8 +26,10 @@ here): the stencils and merge groups, the mass rows, the blocks and their
 factors, and the coefficient patch. The per-step update runs on the device
 (ems_gpu_add_merged) and in the oracle.
 
-Scope: Cartesian grids, no thin sheets, no faces in absorbing layers (the
-reference applies the CPML's H correction to merge groups there); the
+Inside CPML layers (set_cpml) small faces are merged, and the layer's
+H-side correction of every driven face there is added to the circulations
+of the reduced currents driving it (PmlTerm, conformal.rs:2398-2412).
+Scope: Cartesian grids, no thin sheets, no cylindrical PML; the
 power-iteration stability estimate (conformal.rs:2858-2925) is not run
 (ConformalOptions.iterations must be 0).
 """
@@ -346,6 +348,7 @@ class MergedFaces:
     scalar: int
     blocks: List[Block]
     faces: list
+    pml: list = None
 
     def __len__(self):
         return len(self.groups)
@@ -404,7 +407,24 @@ class MergedFaces:
                 dg.append(q)
                 dw.append(x)
             ds.append(len(dg))
+        pml = self.pml or []
+        per = [[] for _ in g]
+        for q, t in enumerate(pml):
+            for (gq, w) in t.targets:
+                per[gq].append((q, w))
+        gps, gpt, gpw = [0], [], []
+        for lst in per:
+            for q, w in lst:
+                gpt.append(q)
+                gpw.append(w)
+            gps.append(len(gpt))
         return dict(
+            pml_axis=np.array([t.other for t in pml], np.int32),
+            pml_idx0=np.array([index(n, t.nodes[0]) for t in pml], np.uint64),
+            pml_idx1=np.array([index(n, t.nodes[1]) for t in pml], np.uint64),
+            pml_coef=np.array([[t.sign, t.b, t.a, t.k] for t in pml], np.float64).reshape(-1),
+            pml_start=np.array(gps, np.uint32), pml_term=np.array(gpt, np.uint32),
+            pml_w=np.array(gpw, np.float64),
             da=np.array([x.da for x in g], np.float64), db=np.array([x.db for x in g], np.float64),
             l=np.array([x.l for x in g], np.float64),
             mrow_start=np.array(mrs, np.uint32), mcol=np.array(mc, np.uint32),
@@ -501,7 +521,6 @@ class Built:
     floored: int = 0
     kept: int = 0
     faces: int = 0
-    driven_in_layer: bool = False
 
 
 def cell_stability(co: np.ndarray, grid, dt: float, q, fractions: Dict[Key, float]) -> float:
@@ -722,10 +741,46 @@ def stable_faces(faces, co, grid, dt: float, max_stability: float,
     return out, every
 
 
+def cpml_h_at(cpml_h, a: int, p, n):
+    """ConformalPec::cpml_h_at (conformal.rs:576-598): (d, coefficients) of
+    each layer whose H positions hold the face along its derivative
+    direction d != a; cpml_h lists (face index, cells, H profile)."""
+    out = []
+    for face, cells, h in cpml_h:
+        d = face // 2
+        if d == a:
+            continue
+        np_, k = n[d], p[d]
+        if face % 2 == 1:
+            inner = np_ - 1 - cells
+            pos = k - inner if k >= inner else None
+        else:
+            pos = k if k < cells else None
+        if pos is not None and pos < len(h):
+            out.append((d, h[pos]))
+    return out
+
+
+@dataclass
+class PmlTerm:
+    """The CPML correction of one driven face in a layer, one derivative
+    direction (conformal.rs:2398-2412): psi <- b psi + a T, circulation
+    += (k T + psi) w for every (group, w) target, T = sign (u_other(hi) -
+    u_other(p))."""
+    targets: list
+    other: int
+    nodes: tuple
+    sign: float
+    b: float
+    a: float
+    k: float
+
+
 def construct(faces, centroids, edges, walls, co: np.ndarray, grid, dt: float, tau: float,
-              kept: Set[Key], deviation: float, absorbing) -> Built:
+              kept: Set[Key], deviation: float, absorbing, cpml_h=()) -> Built:
     """ConformalPec::construct (conformal.rs:1041-1589) without thin sheets:
-    `co` is the lane array (12, N) flat with the edge scaling applied."""
+    `co` is the lane array (12, N) flat with the edge scaling applied;
+    cpml_h the Cartesian layers' H grading (set_cpml)."""
     n = grid.nodes()
     idx = lambda p: index(n, p)  # noqa: E731
     patch: Dict[Tuple[int, int], np.float32] = {}
@@ -1023,7 +1078,18 @@ def construct(faces, centroids, edges, walls, co: np.ndarray, grid, dt: float, t
             w.append((first_host + nh + ns, 1.0))
         driven.append((a, p, w))
         freeze(k)
-    out.driven_in_layer = any(in_abs(a, p) for (a, p, _w) in driven)
+    # the CPML's H-side correction of the driven faces in the layers, on
+    # every reduced current driving the face (conformal.rs:1502-1539)
+    pml = []
+    for (a, p, targets) in driven:
+        if not in_abs(a, p):
+            continue
+        for d, c in cpml_h_at(cpml_h, a, p, n):
+            sign, other = (1.0, (a + 2) % 3) if d == (a + 1) % 3 else (-1.0, (a + 1) % 3)
+            hi = list(p)
+            hi[d] += 1
+            pml.append(PmlTerm(list(targets), other, (tuple(hi), tuple(p)), sign,
+                               float(c["b"]), float(c["a"]), float(c["direct"])))
 
     for f in faces:
         k = idx(f.node)
@@ -1041,7 +1107,7 @@ def construct(faces, centroids, edges, walls, co: np.ndarray, grid, dt: float, t
             db = patch.get(key, co[DB + f.axis, k])
             patch[key] = np.float32(float(db) / fa)
         out.faces += 1
-    out.merged = MergedFaces(tuple(n), dt, groups, scalar, blocks, driven)
+    out.merged = MergedFaces(tuple(n), dt, groups, scalar, blocks, driven, pml)
     return out
 
 
@@ -1060,6 +1126,9 @@ class MergedDesc(C.Structure):
                 ("brow_start", _u32p), ("bcol", _u32p), ("bval", _dp),
                 ("mrow_start", _u32p), ("mcol", _u32p), ("mval", _dp),
                 ("col_start", _u32p), ("col_row", _u32p), ("col_val", _dp),
+                ("n_pml", C.c_int32), ("pml_axis", _i32p), ("pml_idx0", _u64p),
+                ("pml_idx1", _u64p), ("pml_coef", _dp), ("pml_start", _u32p),
+                ("pml_term", _u32p), ("pml_w", _dp),
                 ("face_axis", _i32p), ("face_idx", _u64p), ("face_start", _u32p),
                 ("face_group", _u32p), ("face_w", _dp)]
 
@@ -1071,9 +1140,11 @@ def make_desc(op: MergedFaces):
     d = MergedDesc()
     d.n_groups, d.n_scalar = len(op.groups), op.scalar
     d.n_blocks, d.n_faces, d.dt = len(op.blocks), len(op.faces), op.dt
+    d.n_pml = len(op.pml or [])
     types = {"da": _dp, "db": _dp, "l": _dp, "term_w": _dp, "env_val": _dp, "bval": _dp,
              "mval": _dp, "col_val": _dp, "face_w": _dp, "term_axis": _i32p, "face_axis": _i32p,
-             "term_idx": _u64p, "env_off": _u64p, "face_idx": _u64p}
+             "term_idx": _u64p, "env_off": _u64p, "face_idx": _u64p, "pml_axis": _i32p,
+             "pml_idx0": _u64p, "pml_idx1": _u64p, "pml_coef": _dp, "pml_w": _dp}
     for k, v in arr.items():
         setattr(d, k, v.ctypes.data_as(types.get(k, _u32p)))
     return d, arr

@@ -256,6 +256,19 @@ typedef struct {
     const uint32_t *col_start;  /* rows + 1 */
     const uint32_t *col_row;
     const double *col_val;
+    /* the CPML's H-side correction of driven faces inside layers
+     * (MergedFaces::add_pml, conformal.rs:2582-2638): per term
+     * T = sign (u_axis[idx0] - u_axis[idx1]), psi <- b psi + a T (kept after
+     * the H update, not after E), contribution k T + psi, added to the
+     * circulation of each reduced current driving the face, w times, in
+     * term order (per current: pml_start, pml_term, pml_w) */
+    int32_t n_pml;
+    const int32_t *pml_axis;
+    const uint64_t *pml_idx0, *pml_idx1;
+    const double *pml_coef;     /* 4 per term: sign, b, a, k */
+    const uint32_t *pml_start;  /* n_groups + 1 */
+    const uint32_t *pml_term;
+    const double *pml_w;
     const int32_t *face_axis;
     const uint64_t *face_idx;   /* flat node index of the driven H edge */
     const uint32_t *face_start; /* n_faces + 1 */

@@ -163,6 +163,13 @@ typedef struct merged {
     int *tax, *fax;
     size_t *tidx, *fidx;
     double *i, *circ, *next, *x;
+    /* CPML terms of driven faces in layers (PmlTerm, conformal.rs:2398-2412) */
+    int np;
+    int *pax;
+    size_t *pi0, *pi1;
+    double *pc, *psi;
+    uint32_t *pst, *pt;
+    double *pw;
 } merged_t;
 
 struct orc_engine {
@@ -694,6 +701,27 @@ static void mf_circulations(const orc_engine *e, merged_t *m) {
     }
 }
 
+/* MergedFaces::add_pml (conformal.rs:2582-2638), Cartesian terms: per term
+ * in order T = sign (u(hi) - u(p)), psi = b psi + a T (kept if commit), and
+ * circ[g] += w (k T + psi) for its targets in order. The terms' targets are
+ * held by reduced current; the loop over a current's entries in term order
+ * visits them as the term loop does. */
+static void mf_add_pml(const orc_engine *e, merged_t *m, int commit) {
+    if (!m->np) return;
+    double *con = (double *)calloc((size_t)m->np, sizeof(double));
+    for (int q = 0; q < m->np; q++) {
+        const double *c = m->pc + 4 * (size_t)q;
+        const float *u = e->u[m->pax[q]];
+        const double d = c[0] * ((double)u[m->pi0[q]] - (double)u[m->pi1[q]]);
+        const double psi = c[1] * m->psi[q] + c[2] * d;
+        if (commit) m->psi[q] = psi;
+        con[q] = c[3] * d + psi;
+    }
+    for (int g = 0; g < m->ng; g++)
+        for (uint32_t k = m->pst[g]; k < m->pst[g + 1]; k++) m->circ[g] += m->pw[k] * con[m->pt[k]];
+    free(con);
+}
+
 /* MergedFaces::advance (conformal.rs:2569-2576): the merge groups'
  * flush(f32(da i - db c)), then each block (Block::advance :2037-2052):
  * x = B i - dt c, Envelope::solve (:2101-2123), flush(f32(x)). */
@@ -730,6 +758,7 @@ static void mf_advance(const merged_t *m, const double *c, double *x, double *ou
  * and every driven face's current sum w i (from -0.0) as f32. */
 static void mf_post_h(orc_engine *e, merged_t *m) {
     mf_circulations(e, m);
+    mf_add_pml(e, m, 1);
     mf_advance(m, m->circ, m->x, m->next);
     memcpy(m->i, m->next, (size_t)m->ng * sizeof(double));
     for (int q = 0; q < m->nf; q++) {
@@ -825,7 +854,10 @@ static void one_step(orc_engine *e, int energy) {
             if (is_vprobe(o) && n % o->interval == 0) probe_sample(e, o);
             if (o->type == OP_LUMPED) lumped_post_e(e, o, n);
             if (o->type == OP_PW) pw_post_e(e, o->pw, n);
-            if (o->type == OP_MERGED) mf_circulations(e, o->mf);
+            if (o->type == OP_MERGED) {
+                mf_circulations(e, o->mf);
+                mf_add_pml(e, o->mf, 0);
+            }
         }
     }
     e->step++;
@@ -1273,6 +1305,20 @@ int orc_add_merged(orc_engine *e, const orc_merged_desc *d) {
     for (uint32_t q = 0; q < nf; q++) m->fax[q] = d->face_axis[q], m->fidx[q] = d->face_idx[q];
     m->fs = dup_n(d->face_start, nf + 1, 4), m->fg = dup_n(d->face_group, nw, 4);
     m->fw = dup_n(d->face_w, nw, 8);
+    const uint32_t npml = d->n_pml < 0 ? 0 : (uint32_t)d->n_pml, npw = d->pml_start[ng];
+    for (uint32_t q = 0; q < npml; q++)
+        if (d->pml_axis[q] < 0 || d->pml_axis[q] > 2 || d->pml_idx0[q] >= e->N ||
+            d->pml_idx1[q] >= e->N)
+            return -1;
+    m->np = npml;
+    m->pax = (int *)calloc(npml + 1, sizeof(int));
+    m->pi0 = (size_t *)calloc(npml + 1, sizeof(size_t));
+    m->pi1 = (size_t *)calloc(npml + 1, sizeof(size_t));
+    for (uint32_t q = 0; q < npml; q++)
+        m->pax[q] = d->pml_axis[q], m->pi0[q] = d->pml_idx0[q], m->pi1[q] = d->pml_idx1[q];
+    m->pc = dup_n(d->pml_coef, 4 * (size_t)npml, 8), m->psi = dup_n(NULL, npml, 8);
+    m->pst = dup_n(d->pml_start, ng + 1, 4), m->pt = dup_n(d->pml_term, npw, 4);
+    m->pw = dup_n(d->pml_w, npw, 8);
     m->i = dup_n(NULL, ng, 8), m->circ = dup_n(NULL, ng, 8), m->next = dup_n(NULL, ng, 8);
     m->x = dup_n(NULL, nr, 8);
     return id;

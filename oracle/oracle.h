@@ -156,6 +156,12 @@ typedef struct {
     const double *mval;
     const uint32_t *col_start, *col_row; /* unused here (a layout of the device's) */
     const double *col_val;
+    int32_t n_pml; /* CPML terms of driven faces in layers, 4 coefficients each */
+    const int32_t *pml_axis;
+    const uint64_t *pml_idx0, *pml_idx1;
+    const double *pml_coef;
+    const uint32_t *pml_start, *pml_term; /* the terms' targets by reduced current */
+    const double *pml_w;
     const int32_t *face_axis;
     const uint64_t *face_idx;
     const uint32_t *face_start, *face_group;

@@ -95,6 +95,43 @@ def test_sliver_face(centroid):
     assert np.abs(pg.currents()[0]).max() > 0.0
 
 
+def test_sliver_face_in_cpml():
+    """The sliver inside an x+ CPML layer: merged, its group applying the
+    layer's H-side correction (committed after H, evaluated for the energy
+    look-ahead after E)."""
+    def build(kind):
+        g = E.Grid.uniform([12, 12, 12], [1e-3] * 3)
+        e = K.make(kind, g, None)
+        cpml = E.CpmlBoundary.new(e, [(E.Face.XMax, E.Cpml.with_cells(6))])
+        pec = CF.ConformalPec()
+        pec.push_face_with_centroid(CF.ConformalFace(2, (8, 5, 5), 0.02), (0.5, 0.01))
+        for q in [(8, 5, 5), (9, 5, 5)]:
+            pec.push_edge(CF.ConformalEdge(1, q, 0.02))
+        pec.push_edge(CF.ConformalEdge(0, (8, 6, 5), 0.0))
+        pec.set_cpml(cpml)
+        rep = pec.apply(e)
+        assert rep.groups == 1
+        e.add_operator(cpml)
+        for axis, node, v in [(0, (8, 5, 5), 1.0), (1, (7, 7, 4), -0.5)]:
+            u = e.u_component(axis)
+            u[node] = v
+            e.set_component(0, axis, u)
+        op = [x for x in (e.setup if kind == "gpu" else [(0, y) for y in e.ops])
+              if type(x[1]).__name__ == "MergedFaces"][0][1]
+        assert len(op.pml) == 2
+        return e, op
+    (g, pg), (o, po) = build("gpu"), build("oracle")
+    for n in (1, 7, 300):
+        g.run(n)
+        o.run(n)
+        wg, wo = g.step_with_energy(), o.step_with_energy()
+        assert np.float64(wg).tobytes() == np.float64(wo).tobytes(), (n, wg, wo)
+        for x, y in zip(pg.currents(), po.currents()):
+            assert np.array_equal(bits(x), bits(y)), n
+    same_fields(g, o)
+    assert np.abs(pg.currents()[0]).max() > 0.0
+
+
 def test_conformal_rod_in_cpml_box():
     def build(kind):
         lines = []

@@ -270,3 +270,44 @@ def test_columns_give_the_row_solve_bit_for_bit():
             x[j] = s / blk.factor.rows[j][j - blk.factor.first[j]]
         assert np.array_equal(np.array(x).view(np.uint64), np.array(want).view(np.uint64)), b
     assert big > 10 and rep.interpolated > 0
+
+
+def test_energy_lookahead_includes_the_pml():
+    """The sliver inside an x+ CPML layer (conformal.rs:3294-3327): merged
+    (no interpolation in layers), its group applies the layer's H-side
+    correction, and the look-ahead update from the circulations after the
+    E half-step is exactly the next step's currents."""
+    g, dt, lanes = _engine(0.99)
+    o = OracleEngine(g, dt, lanes, threads=1)
+    cpml = E.CpmlBoundary.new(o, [(E.Face.XMax, E.Cpml.with_cells(6))])
+    pec = K.ConformalPec()
+    f = K.ConformalFace(2, (8, 5, 5), 0.02)
+    pec.push_face_with_centroid(f, (0.5, 0.01))
+    for q in [(8, 5, 5), (9, 5, 5)]:
+        pec.push_edge(K.ConformalEdge(1, q, 0.02))
+    pec.push_edge(K.ConformalEdge(0, (8, 6, 5), 0.0))
+    pec.set_cpml(cpml)
+    rep = pec.apply(o)
+    assert rep.groups == 1 and rep.interpolated == 0
+    o.add_operator(cpml)
+    op = [x for x in o.ops if type(x).__name__ == "MergedFaces"][0]
+    assert len(op.pml) == 2 and all(t.other == 1 and t.sign == 1.0 for t in op.pml)
+    for axis, node, v in [(0, (8, 5, 5), 1.0), (1, (7, 7, 4), -0.5)]:
+        u = o.u_component(axis)
+        u[node] = v
+        o.set_component(0, axis, u)
+    for _ in range(5):
+        o.run(7)
+        cur, circ = op.currents()
+        ahead = op.advance(list(cur), list(circ))
+        o.run(1)
+        now, _ = op.currents()
+        assert np.array_equal(np.array(ahead), now)
+        assert np.any(now != 0.0)
+    # the layer's correction is in the circulations: they differ from the
+    # plain sums of the groups' terms over u
+    _, circ = op.currents()
+    u = [o.u_component(a) for a in range(3)]
+    plain = [sum(w * float(u[a][p]) for a, p, w in gr.terms) for gr in op.groups]
+    assert any(abs(c - q) > 1e-6 * (abs(q) + 1e-30) for c, q in zip(circ, plain))
+    o.close()
