@@ -169,6 +169,7 @@ ems_gaussian_values = _sig("ems_gaussian_values", None, _d, _d, _d, _d, _u64, _d
 ems_fold_parallel = _sig("ems_fold_parallel", _i, C.c_float, C.c_float, _d, _d, _d,
                          C.POINTER(C.c_float), C.POINTER(C.c_float))
 ems_dft = _sig("ems_dft", None, _d, _d, _dp, _u64, _dp, _i, _dp)
+ems_conformal_stability = _sig("ems_conformal_stability", _d, _p)
 ems_run_until = _sig("ems_run_until", _i, _p, C.POINTER(RunLimitsC), _ip, _i,
                      C.POINTER(RunOutcomeC))
 ems_run_until_batch = _sig("ems_run_until_batch", _i, _p, _i, C.POINTER(_p), C.POINTER(RunLimitsC),

@@ -17,14 +17,15 @@ raised to the threshold.
 
 Arithmetic as ConformalPec::apply_known (conformal.rs:770-954): the edge
 scaling (cb as f64 * f_e) as f32 in edge order, the stable faces from the
-scaled lanes, construct at the threshold, then the patch on top of the
-scaled lanes (changed, :2704-2715).
+scaled lanes, then rounds: construct at the threshold, the patch on top of
+the scaled lanes (changed, :2704-2715), the stability check (the cell-wise
+bound where every current is a grid face's, else the power-iteration
+estimate, ems_conformal_stability), the threshold raised by 1.25 while the
+check fails and rounds remain.
 
 Inside CPML layers (set_cpml) small faces are merged and the merge groups
 apply the layer's H-side correction themselves. Scope: Cartesian grids;
-no thin PEC sheets; no cylindrical PML; the power-iteration stability estimate
-(conformal.rs:2858-2925) and the threshold rounds it drives are not run:
-ConformalOptions.iterations is 0, the report's stability NaN.
+no thin PEC sheets; no cylindrical PML.
 """
 from __future__ import annotations
 
@@ -61,11 +62,11 @@ class ConformalFace:
 
 @dataclass
 class ConformalOptions:
-    """ConformalOptions (conformal.rs:283-331). iterations must be 0 (the
-    stability estimate is not run)."""
+    """ConformalOptions (conformal.rs:283-331): iterations of the stability
+    estimate per round (0 skips the check), rounds of threshold raising."""
     merge_below: float = 0.5
     max_stability: float = 0.999
-    iterations: int = 0
+    iterations: int = 300
     max_rounds: int = 4
     deviation_area: float = 0.5
     keep_stable: bool = True
@@ -87,6 +88,7 @@ class ConformalReport:
     bounded: bool = False
     merge_below: float = 0.5
     stability: float = float("nan")
+    proven: bool = False
     verified: bool = True
     pec_edges: int = 0
 
@@ -169,9 +171,6 @@ class ConformalPec:
         n = tuple(lanes.shape[1:])
         self.validate(n)
         opt = self.options
-        if opt.iterations:
-            raise EmsError(nat.EMS_E_INVALID,
-                           "the conformal stability estimate is not available: iterations = 0")
         flat = lanes.reshape(12, -1)
         co = M.Overlay(flat)
         rep = ConformalReport(edges=len(self.edges))
@@ -186,24 +185,58 @@ class ConformalPec:
                                            self.absorbing)
         else:
             kept, bounded = set(), False
+        seeds = [(a, M.index(n, p)) for f in self.faces for a, p in M.face_edges(f.axis, f.node)]
         tau = min(max(opt.merge_below, 0.0), 1.0)
-        b = M.construct(self.faces, self.centroids, self.edges, self.walls, co, grid, dt, tau,
-                        kept, min(max(opt.deviation_area, 0.0), 1.0), self.absorbing,
-                        self.cpml_h)
-        rep.faces, rep.interpolated, rep.hosts = b.faces, b.interpolated, b.hosts
-        rep.merged, rep.groups, rep.floored, rep.kept = b.merged_faces, b.groups, b.floored, b.kept
-        rep.bounded, rep.merge_below = bounded, tau
-        for (lane, k), v in b.patch.items():
-            co[lane, k] = v
+        rounds = max(opt.max_rounds, 1)
+        limit = opt.max_stability * opt.max_stability
+        for rnd in range(rounds):
+            b = M.construct(self.faces, self.centroids, self.edges, self.walls, co, grid, dt,
+                            tau, kept, min(max(opt.deviation_area, 0.0), 1.0), self.absorbing,
+                            self.cpml_h)
+            r = ConformalReport(edges=rep.edges, pec_edges=rep.pec_edges, faces=b.faces,
+                                interpolated=b.interpolated, hosts=b.hosts,
+                                merged=b.merged_faces, groups=b.groups, floored=b.floored,
+                                kept=b.kept, bounded=bounded, merge_below=tau)
+            cr = M.Overlay(flat)
+            cr.over = dict(co.over)
+            for (lane, k), v in b.patch.items():
+                cr[lane, k] = v
+            last = rnd + 1 == rounds or tau >= 1.0
+            op = b.merged if len(b.merged) else None
+            if opt.iterations > 0:
+                # apply_known (conformal.rs:850-900): the cell-wise bound where
+                # every current is a grid face's, else the power iteration
+                bound = M.grid_bound(cr, grid, dt, limit) if op is None else None
+                if bound is not None:
+                    r.stability, r.verified, r.proven = math.sqrt(bound), True, True
+                else:
+                    dense = self._dense(flat, cr, (CB, CB + 1, CB + 2, DB, DB + 1, DB + 2))
+                    r.stability = M.stability(dense, n, op, seeds, opt.iterations)
+                    r.verified = r.stability <= opt.max_stability
+            if r.verified or last:
+                break
+            tau = min(tau * 1.25, 1.0)
+        rep = r
         # the values that differ from the lanes (bit patterns), by lane
         out: Dict[int, np.ndarray] = {}
-        for (lane, k), v in co.over.items():
+        for (lane, k), v in cr.over.items():
             if np.float32(v).view(np.uint32) != flat[lane, k].view(np.uint32):
                 if lane not in out:
                     out[lane] = flat[lane].copy()
                 out[lane][k] = v
         out = {l: v.reshape(n) for l, v in out.items()}
-        return out, rep, (b.merged if len(b.merged) else None)
+        return out, rep, op
+
+    @staticmethod
+    def _dense(flat, cr, lanes):
+        """The lanes `lanes` with the overlay's changes, dense (others None)."""
+        out = [None] * 12
+        for l in lanes:
+            out[l] = flat[l].copy()
+        for (lane, k), v in cr.over.items():
+            if out[lane] is not None:
+                out[lane][k] = v
+        return out
 
     def apply(self, engine) -> ConformalReport:
         """ConformalPec::apply (conformal.rs:755): patches the engine's

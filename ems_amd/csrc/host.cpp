@@ -796,3 +796,169 @@ int ems_drive_script(const double *w, uint64_t nw, const double *signal, uint64_
 }
 
 }  // extern "C"
+
+// ---- conformal stability estimate (conformal.rs:2858-3055) ----
+
+namespace {
+
+struct Stab {
+  size_t N;
+  int n[3];
+  std::vector<double> sq[3];
+  const ems_stability_desc *d;
+};
+
+// apply_s (conformal.rs:2927-2995): w = S v.
+void stab_apply(const Stab &st, const std::vector<double> (&v)[3], std::vector<double> (&w)[3]) {
+  const ems_stability_desc &d = *st.d;
+  const int *n = st.n;
+  for (int a = 0; a < 3; a++) std::fill(w[a].begin(), w[a].end(), 0.0);
+  const size_t stride[3] = {(size_t)n[1] * n[2], (size_t)n[2], 1};
+  for (int a = 0; a < 3; a++) {
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    const size_t sb = stride[b], sc = stride[c];
+    const std::vector<double> &vb = v[b], &vc = v[c], &rb = st.sq[b], &rc = st.sq[c];
+    std::vector<double> &wb = w[b], &wc = w[c];
+    auto last = [&](int dd, int p) { return (b == dd || c == dd) && p + 1 >= n[dd]; };
+    const int nk = (b == 2 || c == 2) ? std::max(n[2] - 1, 0) : n[2];
+    const float *dbl = d.db[a];
+    for (int i = 0; i < n[0]; i++) {
+      if (last(0, i)) continue;
+      for (int j = 0; j < n[1]; j++) {
+        if (last(1, j)) continue;
+        const size_t base = ((size_t)i * n[1] + j) * n[2];
+        for (int k = 0; k < nk; k++) {
+          const double dv = (double)dbl[base + k];
+          if (dv == 0.0) continue;
+          const size_t q = base + k;
+          const double curl =
+              vc[q + sb] * rc[q + sb] - vc[q] * rc[q] - vb[q + sc] * rb[q + sc] + vb[q] * rb[q];
+          const double f = dv * curl;
+          if (f == 0.0) continue;
+          wc[q + sb] += f;
+          wc[q] -= f;
+          wb[q + sc] -= f;
+          wb[q] += f;
+        }
+      }
+    }
+  }
+  auto x = [&](int a, uint64_t k) { return v[a][k] * st.sq[a][k]; };
+  for (int g = 0; g < d.n_groups; g++) {
+    double curl = -0.0;
+    for (uint32_t t = d.g_term_start[g]; t < d.g_term_start[g + 1]; t++)
+      curl += d.g_term_w[t] * x(d.g_term_axis[t], d.g_term_idx[t]);
+    const double f = d.g_dt_over_l[g] * curl;
+    for (uint32_t t = d.g_term_start[g]; t < d.g_term_start[g + 1]; t++)
+      w[d.g_term_axis[t]][d.g_term_idx[t]] += d.g_term_w[t] * f;
+  }
+  std::vector<double> y;
+  for (int b = 0; b < d.n_blocks; b++) {
+    const uint32_t r0 = d.blk_start[b], nr = d.blk_start[b + 1] - r0;
+    y.assign(nr, 0.0);
+    for (uint32_t i = 0; i < nr; i++) {
+      double s = -0.0;
+      for (uint32_t t = d.row_term_start[r0 + i]; t < d.row_term_start[r0 + i + 1]; t++)
+        s += d.row_term_w[t] * x(d.row_term_axis[t], d.row_term_idx[t]);
+      y[i] = s;
+    }
+    for (uint32_t i = 0; i < nr; i++) {  // Envelope::solve
+      const uint32_t f = d.env_first[r0 + i];
+      const double *row = d.env_val + d.env_off[r0 + i];
+      double s = y[i];
+      for (uint32_t jj = f; jj < i; jj++) s -= row[jj - f] * y[jj];
+      y[i] = s / row[i - f];
+    }
+    for (uint32_t i = nr; i-- > 0;) {
+      const uint32_t f = d.env_first[r0 + i];
+      const double *row = d.env_val + d.env_off[r0 + i];
+      y[i] /= row[i - f];
+      const double yi = y[i];
+      for (uint32_t jj = f; jj < i; jj++) y[jj] -= row[jj - f] * yi;
+    }
+    for (uint32_t i = 0; i < nr; i++)
+      for (uint32_t t = d.row_term_start[r0 + i]; t < d.row_term_start[r0 + i + 1]; t++)
+        w[d.row_term_axis[t]][d.row_term_idx[t]] += d.row_term_w[t] * d.blk_dt[b] * y[i];
+  }
+  for (int a = 0; a < 3; a++)
+    for (size_t k = 0; k < st.N; k++) w[a][k] *= st.sq[a][k];
+}
+
+}  // namespace
+
+extern "C" {
+
+double ems_conformal_stability(const ems_stability_desc *d) {
+  if (!d || d->nodes[0] < 2 || d->nodes[1] < 2 || d->nodes[2] < 2 || d->n_groups < 0 ||
+      d->n_blocks < 0 || d->n_seeds < 0 || d->iterations < 0)
+    return EMS_E_INVALID;
+  for (int a = 0; a < 3; a++)
+    if (!d->cb[a] || !d->db[a]) return EMS_E_INVALID;
+  Stab st;
+  st.d = d;
+  for (int a = 0; a < 3; a++) st.n[a] = d->nodes[a];
+  st.N = (size_t)st.n[0] * st.n[1] * st.n[2];
+  // every index the loops follow
+  const uint32_t ng = (uint32_t)d->n_groups, nb = (uint32_t)d->n_blocks;
+  auto edge_ok = [&](int32_t a, uint64_t k) { return a >= 0 && a < 3 && k < st.N; };
+  if (ng && (!d->g_dt_over_l || !d->g_term_start)) return EMS_E_INVALID;
+  for (uint32_t g = 0; g < ng; g++)
+    for (uint32_t t = d->g_term_start[g]; t < d->g_term_start[g + 1]; t++)
+      if (!edge_ok(d->g_term_axis[t], d->g_term_idx[t])) return EMS_E_INVALID;
+  if (nb && (!d->blk_start || !d->blk_dt || !d->row_term_start || !d->env_first || !d->env_off ||
+             !d->env_val))
+    return EMS_E_INVALID;
+  for (uint32_t b = 0; b < nb; b++) {
+    const uint32_t r0 = d->blk_start[b], r1 = d->blk_start[b + 1];
+    if (r1 < r0) return EMS_E_INVALID;
+    for (uint32_t r = r0; r < r1; r++) {
+      if (d->env_first[r] > r - r0 ||
+          d->env_off[r + 1] - d->env_off[r] != (uint64_t)(r - r0 - d->env_first[r] + 1))
+        return EMS_E_INVALID;
+      for (uint32_t t = d->row_term_start[r]; t < d->row_term_start[r + 1]; t++)
+        if (!edge_ok(d->row_term_axis[t], d->row_term_idx[t])) return EMS_E_INVALID;
+    }
+  }
+  for (int q = 0; q < d->n_seeds; q++)
+    if (!edge_ok(d->seed_axis[q], d->seed_idx[q])) return EMS_E_INVALID;
+  for (int a = 0; a < 3; a++) {
+    st.sq[a].resize(st.N);
+    for (size_t k = 0; k < st.N; k++) st.sq[a][k] = std::sqrt((double)d->cb[a][k]);
+  }
+  std::vector<double> v[3], w[3];
+  for (int a = 0; a < 3; a++) {
+    // xorshift: a fixed sequence per array
+    uint64_t state = 0x9e3779b97f4a7c15ull ^ (uint64_t)(a + 1);
+    v[a].resize(st.N);
+    w[a].assign(st.N, 0.0);
+    for (size_t k = 0; k < st.N; k++) {
+      state ^= state << 13;
+      state ^= state >> 7;
+      state ^= state << 17;
+      const double r = (double)(state >> 11) / (double)(1ull << 53);
+      v[a][k] = st.sq[a][k] > 0.0 ? 1e-3 * (r - 0.5) : 0.0;
+    }
+  }
+  for (int q = 0; q < d->n_seeds; q++)
+    if (st.sq[d->seed_axis[q]][d->seed_idx[q]] > 0.0) v[d->seed_axis[q]][d->seed_idx[q]] += 1.0;
+  double lambda = 0.0;
+  const int it = std::max(d->iterations, 1);
+  for (int s = 0; s < it; s++) {
+    double nn = -0.0;
+    for (int a = 0; a < 3; a++)
+      for (size_t k = 0; k < st.N; k++) nn += v[a][k] * v[a][k];
+    const double norm = std::sqrt(nn);
+    if (norm == 0.0) return 0.0;
+    for (int a = 0; a < 3; a++)
+      for (size_t k = 0; k < st.N; k++) v[a][k] /= norm;
+    stab_apply(st, v, w);
+    double l = -0.0;
+    for (int a = 0; a < 3; a++)
+      for (size_t k = 0; k < st.N; k++) l += v[a][k] * w[a][k];
+    lambda = l;
+    for (int a = 0; a < 3; a++) std::swap(v[a], w[a]);
+  }
+  return std::sqrt(std::max(lambda, 0.0)) / 2.0;
+}
+
+}  // extern "C"

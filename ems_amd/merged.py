@@ -29,9 +29,10 @@ factors, and the coefficient patch. The per-step update runs on the device
 Inside CPML layers (set_cpml) small faces are merged, and the layer's
 H-side correction of every driven face there is added to the circulations
 of the reduced currents driving it (PmlTerm, conformal.rs:2398-2412).
-Scope: Cartesian grids, no thin sheets, no cylindrical PML; the
-power-iteration stability estimate (conformal.rs:2858-2925) is not run
-(ConformalOptions.iterations must be 0).
+Scope: Cartesian grids, no thin sheets, no cylindrical PML. The stability
+estimate (power iteration) runs in the host library
+(ems_conformal_stability); `stability` and `grid_bound` feed the rounds of
+ConformalPec.build.
 """
 from __future__ import annotations
 
@@ -311,6 +312,7 @@ class Block:
     m: List[List[Tuple[int, float]]]
     b: List[List[Tuple[int, float]]]
     dt: float
+    mfactor: Optional[Envelope] = None  # the factor of M where it differs from A (lossy)
 
     @staticmethod
     def build(comp: Sequence[int], ml: Sequence[Dict[int, Tuple[float, float]]], first: int,
@@ -333,7 +335,8 @@ class Block:
         fac = Envelope.factor(a)
         if fac is None:
             raise ValueError("P^T L P is not positive definite")
-        return Block([first + q for q in order], fac, m, b, dt)
+        mfac = None if a == m else Envelope.factor(m)
+        return Block([first + q for q in order], fac, m, b, dt, mfac)
 
 
 @dataclass
@@ -1148,3 +1151,101 @@ def make_desc(op: MergedFaces):
     for k, v in arr.items():
         setattr(d, k, v.ctypes.data_as(types.get(k, _u32p)))
     return d, arr
+
+
+def grid_bound(co, grid, dt: float, limit: float, chunk: int = 1 << 18) -> Optional[float]:
+    """grid_bound (conformal.rs:1707-1731): the largest cell-wise bound over
+    every cell of the grid on the round's coefficients, or None as soon as
+    a cell reaches the limit (taken a part in 1e9 lower). Each cell enters
+    with its own largest eigenvalue (the reference first tries the row
+    sums, an upper bound, and reports those where they pass)."""
+    n = grid.nodes()
+    lim = limit * (1.0 - 1e-9)
+    c = [n[a] - 1 for a in range(3)]
+    total = c[0] * c[1] * c[2]
+    worst = 0.0
+    for s0 in range(0, total, chunk):
+        ids = np.arange(s0, min(s0 + chunk, total), dtype=np.int64)
+        cells = np.stack([ids // (c[1] * c[2]), (ids // c[2]) % c[1], ids % c[2]], axis=1)
+        v = cell_bounds(co, grid, dt, cells, {})
+        if not np.all(v < lim):
+            return None
+        worst = max(worst, float(v.max()))
+    return worst
+
+
+_fp = C.POINTER(C.c_float)
+
+
+class StabilityDesc(C.Structure):
+    """ems_stability_desc (include/ems_host.h)."""
+    _fields_ = [("nodes", C.c_int32 * 3), ("cb", _fp * 3), ("db", _fp * 3),
+                ("n_groups", C.c_int32), ("g_dt_over_l", _dp), ("g_term_start", _u32p),
+                ("g_term_axis", _i32p), ("g_term_idx", _u64p), ("g_term_w", _dp),
+                ("n_blocks", C.c_int32), ("blk_start", _u32p), ("blk_dt", _dp),
+                ("row_term_start", _u32p), ("row_term_axis", _i32p), ("row_term_idx", _u64p),
+                ("row_term_w", _dp), ("env_first", _u32p), ("env_off", _u64p),
+                ("env_val", _dp), ("n_seeds", C.c_int32), ("seed_axis", _i32p),
+                ("seed_idx", _u64p), ("iterations", C.c_int32)]
+
+
+def stability(lanes, n, op: Optional[MergedFaces], seeds, iterations: int) -> float:
+    """The stability estimate s of the round's coefficients `lanes` (the 12
+    lanes (12, N) flat, the patch applied) with the reduced currents of
+    `op` (ems_conformal_stability, the reference's power iteration)."""
+    from . import _native as nat
+    groups = op.groups[:op.scalar] if op is not None else []
+    blocks = op.blocks if op is not None else []
+
+    def terms_arrays(lists):
+        st, ax, ix, w = [0], [], [], []
+        for terms in lists:
+            for a, p, x in terms:
+                ax.append(a)
+                ix.append(index(n, p))
+                w.append(x)
+            st.append(len(ax))
+        return (np.array(st, np.uint32), np.array(ax, np.int32), np.array(ix, np.uint64),
+                np.array(w, np.float64))
+    keep = []
+    d = StabilityDesc()
+    d.nodes[:] = list(n)
+    for a in range(3):
+        cb = np.ascontiguousarray(lanes[CB + a], np.float32)
+        db = np.ascontiguousarray(lanes[DB + a], np.float32)
+        keep += [cb, db]
+        d.cb[a] = cb.ctypes.data_as(_fp)
+        d.db[a] = db.ctypes.data_as(_fp)
+    gs, ga, gi, gw = terms_arrays([g.terms for g in groups])
+    gd = np.array([op.dt / g.l for g in groups], np.float64)
+    rows, bs, bdt, ef, eo, ev = [], [0], [], [], [0], []
+    for b in blocks:
+        fac = b.mfactor if b.mfactor is not None else b.factor
+        for i, q in enumerate(b.members):
+            rows.append(op.groups[q].terms)
+            ef.append(fac.first[i])
+            ev += fac.rows[i]
+            eo.append(len(ev))
+        bs.append(len(rows))
+        bdt.append(b.dt)
+    rs, ra, ri, rw = terms_arrays(rows)
+    sa = np.array([a for a, _ in seeds], np.int32)
+    si = np.array([k for _, k in seeds], np.uint64)
+    arrs = dict(g_dt_over_l=gd, g_term_start=gs, g_term_axis=ga, g_term_idx=gi, g_term_w=gw,
+                blk_start=np.array(bs, np.uint32), blk_dt=np.array(bdt, np.float64),
+                row_term_start=rs, row_term_axis=ra, row_term_idx=ri, row_term_w=rw,
+                env_first=np.array(ef, np.uint32), env_off=np.array(eo, np.uint64),
+                env_val=np.array(ev, np.float64), seed_axis=sa, seed_idx=si)
+    types = {"g_dt_over_l": _dp, "g_term_w": _dp, "blk_dt": _dp, "row_term_w": _dp,
+             "env_val": _dp, "g_term_axis": _i32p, "row_term_axis": _i32p, "seed_axis": _i32p,
+             "g_term_idx": _u64p, "row_term_idx": _u64p, "env_off": _u64p, "seed_idx": _u64p}
+    for k, v in arrs.items():
+        v = np.ascontiguousarray(v)
+        keep.append(v)
+        setattr(d, k, v.ctypes.data_as(types.get(k, _u32p)))
+    d.n_groups, d.n_blocks, d.n_seeds = len(groups), len(blocks), len(seeds)
+    d.iterations = int(iterations)
+    r = nat.ems_conformal_stability(C.byref(d))
+    if r < 0:
+        raise ValueError("stability estimate: invalid tables")
+    return r

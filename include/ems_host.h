@@ -106,6 +106,41 @@ int ems_fold_parallel(float ca, float cb, double dt, double g, double c, float *
 void ems_dft(double t0, double dt, const double *x, uint64_t n, const double *freqs, int nf,
              double *out);
 
+/* The conformal PEC's stability estimate (stability, src/engine/
+ * conformal.rs:2858-2925; apply_s :2927-2995; faces :3012-3055): s =
+ * sqrt(lambda_max(S)) / 2 by `iterations` power iterations, S = sqrt(cb)
+ * (PK)^T dt (P^T L P)^-1 (PK) sqrt(cb) over the whole grid (the kernel's
+ * faces with db != 0, merge groups with dt/L, blocks with the factor of
+ * their M), from the fixed xorshift start vector plus 1 on the seed
+ * edges; f64, one thread, the reference's loop and summation orders.
+ * Per block row: the terms of its reduced current (`row_*`, block-local
+ * rows). Returns s, or a negative code for invalid input. */
+typedef struct {
+    int32_t nodes[3];
+    const float *cb[3], *db[3];   /* dense lanes, N each */
+    int32_t n_groups;             /* merge groups with their own update */
+    const double *g_dt_over_l;
+    const uint32_t *g_term_start; /* n_groups + 1 */
+    const int32_t *g_term_axis;
+    const uint64_t *g_term_idx;
+    const double *g_term_w;
+    int32_t n_blocks;
+    const uint32_t *blk_start;    /* n_blocks + 1 rows */
+    const double *blk_dt;
+    const uint32_t *row_term_start; /* rows + 1 */
+    const int32_t *row_term_axis;
+    const uint64_t *row_term_idx;
+    const double *row_term_w;
+    const uint32_t *env_first;    /* the factor of M by rows */
+    const uint64_t *env_off;      /* rows + 1 */
+    const double *env_val;
+    int32_t n_seeds;
+    const int32_t *seed_axis;
+    const uint64_t *seed_idx;
+    int32_t iterations;
+} ems_stability_desc;
+double ems_conformal_stability(const ems_stability_desc *d);
+
 /* RunLimits + EndCriteria (src/engine/run.rs:22-139, src/fdtd.rs:428-605). */
 typedef struct {
     uint64_t max_timesteps;

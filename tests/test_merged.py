@@ -208,8 +208,6 @@ def test_rejects_bad_input():
     c.push_face_with_centroid(K.ConformalFace(2, (3, 3, 3), 0.3), (0.5, 1.2))
     with pytest.raises(E.EmsError):
         c.build(lanes, g, dt)
-    with pytest.raises(E.EmsError):
-        _sliver(options=K.ConformalOptions(iterations=300)).build(lanes, g, dt)
 
 
 def test_cell_bounds_at_once_equal_the_cell_by_cell_form():
@@ -311,3 +309,44 @@ def test_energy_lookahead_includes_the_pml():
     plain = [sum(w * float(u[a][p]) for a, p, w in gr.terms) for gr in op.groups]
     assert any(abs(c - q) > 1e-6 * (abs(q) + 1e-30) for c, q in zip(circ, plain))
     o.close()
+
+
+def test_uncut_grid_stability_is_the_courant_factor():
+    """The power iteration on an uncut 12^3 PEC box at Courant factor 0.9
+    (conformal.rs:3126-3133): its highest mode is slightly below the
+    infinite grid's limit."""
+    g, dt, lanes = _engine(0.9)
+    s = M.stability(list(lanes.reshape(12, -1)), g.nodes(), None, [], 400)
+    assert 0.85 < s < 0.9, s
+
+
+def test_stability_check_of_the_sliver():
+    """conformal.rs:3160-3189, 3232-3246: plain Dey-Mittra (threshold 0,
+    one round) reads unstable with a warning; merged and interpolated read
+    stable, a little below the limit."""
+    g, dt, lanes = _engine(0.99)
+    raw = _sliver(options=K.ConformalOptions(merge_below=0.0, max_rounds=1))
+    _, r, _ = raw.build(lanes, g, dt)
+    assert not r.verified and r.stability > 1.5 and r.warning() and r.merged == 0
+    for cen in (False, True):
+        _, r, _ = _sliver(centroid=cen).build(lanes, g, dt)
+        assert r.verified and 0.9 < r.stability < 0.999 and not r.proven, r
+        assert r.warning() is None
+
+
+def test_flat_cells_are_proven_by_the_cell_bound():
+    """conformal.rs:3432-3455: with the face kept on flat cells (no reduced
+    currents) the grid-wide cell bound stands in for the estimate, and the
+    power iteration finds no more than it; on cubes the face is
+    interpolated and the estimate is made."""
+    g, dt, lanes = _engine(0.99, dz=0.1)
+    changed, r, op = _sliver(centroid=True, fraction=0.2).build(lanes, g, dt)
+    assert r.proven and r.verified and r.stability < 0.999 and op is None, r
+    full = lanes.reshape(12, -1).copy()
+    for l, v in changed.items():
+        full[l] = v.reshape(-1)
+    est = M.stability(list(full), g.nodes(), None, [(2, M.index(g.nodes(), (5, 5, 5)))], 300)
+    assert est <= r.stability, (est, r.stability)
+    g, dt, lanes = _engine(0.99)
+    _, r, _ = _sliver(centroid=True, fraction=0.2).build(lanes, g, dt)
+    assert not r.proven and r.verified
