@@ -1839,11 +1839,12 @@ __device__ __forceinline__ const float *pick3(float *const *a, int c) {
 // ny-1, enumerated in that order): four accumulators over k = 0, 1, 2, 3
 // mod 4 in increasing k (h_energy_row, kernel.rs:119-216: the weighted
 // i^n i^(n+1), the update evaluated read-only), combined as
-// (a0 + a1) + (a2 + a3). Rounds of 512 terms, 8 per lane.
-template <bool OE>
+// (a0 + a1) + (a2 + a3). Rounds of 64 NT terms, NT per lane (NT sized to
+// the rows: 1 for rows of at most 64 nodes, up to 8).
+template <bool OE, int NT>
 __global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, int ny, int nz,
                                                      double *rowsum) {
-  constexpr int NT = 8, RND = 64 * NT;
+  constexpr int RND = 64 * NT;
   __shared__ double buf[4][RND];
   const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
   const int rows_per_plane = 3 * ny - 2;
@@ -2807,12 +2808,18 @@ int energy_now(ems_gpu *h, double *w) {
   }
   // a wave per H row, per (plane, component) E chains and per plane
   dim3 g1((unsigned)((rows + 3) / 4));
-  if (h->oe)
-    hipLaunchKernelGGL(energy_h_rows<true>, g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny,
-                       nz, h->d_rowsum);
-  else
-    hipLaunchKernelGGL(energy_h_rows<false>, g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny,
-                       nz, h->d_rowsum);
+  const int nt = nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8));
+#define EH(OE, NT)                                                                           \
+  hipLaunchKernelGGL((energy_h_rows<OE, NT>), g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, \
+                     ny, nz, h->d_rowsum)
+  if (h->oe) {
+    if (nt == 1) EH(true, 1); else if (nt == 2) EH(true, 2); else if (nt == 4) EH(true, 4);
+    else EH(true, 8);
+  } else {
+    if (nt == 1) EH(false, 1); else if (nt == 2) EH(false, 2); else if (nt == 4) EH(false, 4);
+    else EH(false, 8);
+  }
+#undef EH
   CK(h, hipGetLastError());
   dim3 g2((unsigned)(((size_t)nx * 3 + 3) / 4));
   hipLaunchKernelGGL(energy_e_chains, g2, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny, nz,

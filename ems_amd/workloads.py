@@ -130,8 +130,14 @@ def dispersive_box(n=(384, 384, 256), steps: int = 8000, seed: int = 2027, pml: 
     for q in zip(*idx):
         m.set(q, m_lor)
     pw_lo, pw_hi = pml + 4, [q - pml - 4 for q in g.nodes()]
+    # The stability estimate (300 power iterations over every edge, one
+    # host core) takes about nine minutes at 385 x 385 x 257 nodes; this
+    # cylinder passes it in the first round (s = 0.9907 at the default
+    # size), so the engine is the same without it. Smaller boxes run it.
+    big = g.num_nodes() > 8_000_000
+    opts = CF.ConformalOptions(iterations=0) if big else None
     pec = CF.cylinder_z(g, (c[0] * cell, c[1] * cell), cylinder_cells * cell, mats=m,
-                        metal="inside", z_nodes=(pw_lo + 4, pw_hi[2] - 4))
+                        options=opts, metal="inside", z_nodes=(pw_lo + 4, pw_hi[2] - 4))
     dt = E.TimeStep.courant(0.99).resolve(g)
     lanes = E.coefficients(g, m, [E.FaceCondition.Pec] * 6, dt)
     w0 = 2.0 * math.pi * 8e9
