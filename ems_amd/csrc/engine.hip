@@ -41,6 +41,12 @@ namespace {
 
 // ---------------------------------------------------------------- device --
 
+// Global (1) and constant (4) address spaces: pointers the kernels load
+// from memory are cast so that the compiler emits global / scalar loads
+// instead of flat ones.
+#define GAS __attribute__((address_space(1)))
+#define CAS __attribute__((address_space(4)))
+
 // kernel.rs:30-33
 __device__ __forceinline__ float flush(float x) { return fabsf(x) < FLT_MIN ? 0.0f : x; }
 
@@ -91,6 +97,14 @@ struct Fields {
   float *i[3];
 };
 
+// coef through global-address-space pointers.
+__device__ __forceinline__ float coefg(const Lane &L, size_t r, int k, int nz, size_t flat) {
+  if (L.rows == nullptr) return 0.0f;
+  const GAS float *rows = (const GAS float *)L.rows;
+  if (L.index == nullptr) return rows[flat];
+  return rows[(size_t)((const GAS uint32_t *)L.index)[r] * nz + k];
+}
+
 // Faraday update (kernel.rs:443-511, 566-618): plane-wise in x, all three
 // components at node (i, j, k); padding entries are never written.
 template <bool OE>
@@ -103,12 +117,12 @@ __global__ void __launch_bounds__(256) h_update(Fields f, Lanes L, int nx, int n
   const int i0 = blockIdx.y * chunk;
   const int i1 = min(nx, i0 + chunk);
   const bool jv = j < ny - 1, kv = k < nz - 1;
-  const float *__restrict__ ux = f.u[0];
-  const float *__restrict__ uy = f.u[1];
-  const float *__restrict__ uz = f.u[2];
-  float *__restrict__ ix = f.i[0];
-  float *__restrict__ iy = f.i[1];
-  float *__restrict__ iz = f.i[2];
+  const GAS float *__restrict__ ux = (const GAS float *)f.u[0];
+  const GAS float *__restrict__ uy = (const GAS float *)f.u[1];
+  const GAS float *__restrict__ uz = (const GAS float *)f.u[2];
+  GAS float *__restrict__ ix = (GAS float *)f.i[0];
+  GAS float *__restrict__ iy = (GAS float *)f.i[1];
+  GAS float *__restrict__ iz = (GAS float *)f.i[2];
   size_t o = (size_t)i0 * plane + p;
   float uyc = uy[o], uzc = uz[o];
   for (int i = i0; i < i1; i++, o += plane) {
@@ -118,15 +132,15 @@ __global__ void __launch_bounds__(256) h_update(Fields f, Lanes L, int nx, int n
     const float uyn = iv ? uy[o + plane] : 0.0f;
     const float uzn = iv ? uz[o + plane] : 0.0f;
     if (jv && kv) {  // curl_x = (uz[j+1] - uz[j]) - (uy[k+1] - uy[k])
-      float da = coef(L.l[6], r, k, nz, o), db = coef(L.l[9], r, k, nz, o);
+      float da = coefg(L.l[6], r, k, nz, o), db = coefg(L.l[9], r, k, nz, o);
       ix[o] = h_expr<OE>(da, ix[o], db, uz[o + nz], uzc, uy[o + 1], uyc);
     }
     if (iv && kv) {  // curl_y = (ux[k+1] - ux[k]) - (uz[i+1] - uz[i])
-      float da = coef(L.l[7], r, k, nz, o), db = coef(L.l[10], r, k, nz, o);
+      float da = coefg(L.l[7], r, k, nz, o), db = coefg(L.l[10], r, k, nz, o);
       iy[o] = h_expr<OE>(da, iy[o], db, ux[o + 1], uxc, uzn, uzc);
     }
     if (iv && jv) {  // curl_z = (uy[i+1] - uy[i]) - (ux[j+1] - ux[j])
-      float da = coef(L.l[8], r, k, nz, o), db = coef(L.l[11], r, k, nz, o);
+      float da = coefg(L.l[8], r, k, nz, o), db = coefg(L.l[11], r, k, nz, o);
       iz[o] = h_expr<OE>(da, iz[o], db, uyn, uyc, ux[o + nz], uxc);
     }
     uyc = uyn;
@@ -144,12 +158,12 @@ __global__ void __launch_bounds__(256) e_update(Fields f, Lanes L, int nx, int n
   const int j = p / nz, k = p - j * nz;
   const int i0 = blockIdx.y * chunk;
   const int i1 = min(nx, i0 + chunk);
-  const float *__restrict__ ix = f.i[0];
-  const float *__restrict__ iy = f.i[1];
-  const float *__restrict__ iz = f.i[2];
-  float *__restrict__ ux = f.u[0];
-  float *__restrict__ uy = f.u[1];
-  float *__restrict__ uz = f.u[2];
+  const GAS float *__restrict__ ix = (const GAS float *)f.i[0];
+  const GAS float *__restrict__ iy = (const GAS float *)f.i[1];
+  const GAS float *__restrict__ iz = (const GAS float *)f.i[2];
+  GAS float *__restrict__ ux = (GAS float *)f.u[0];
+  GAS float *__restrict__ uy = (GAS float *)f.u[1];
+  GAS float *__restrict__ uz = (GAS float *)f.u[2];
   size_t o = (size_t)i0 * plane + p;
   float iyp = i0 > 0 ? iy[o - plane] : 0.0f;
   float izp = i0 > 0 ? iz[o - plane] : 0.0f;
@@ -157,18 +171,18 @@ __global__ void __launch_bounds__(256) e_update(Fields f, Lanes L, int nx, int n
     const size_t r = (size_t)i * ny + j;
     const float ixc = ix[o], iyc = iy[o], izc = iz[o];
     if (i + 1 < nx) {  // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
-      float ca = coef(L.l[0], r, k, nz, o), cb = coef(L.l[3], r, k, nz, o);
+      float ca = coefg(L.l[0], r, k, nz, o), cb = coefg(L.l[3], r, k, nz, o);
       float b = j > 0 ? iz[o - nz] : 0.0f;
       float d = k > 0 ? iy[o - 1] : 0.0f;
       ux[o] = e_expr<OE>(ca, ux[o], cb, izc, b, iyc, d);
     }
     if (j + 1 < ny) {  // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
-      float ca = coef(L.l[1], r, k, nz, o), cb = coef(L.l[4], r, k, nz, o);
+      float ca = coefg(L.l[1], r, k, nz, o), cb = coefg(L.l[4], r, k, nz, o);
       float b = k > 0 ? ix[o - 1] : 0.0f;
       uy[o] = e_expr<OE>(ca, uy[o], cb, ixc, b, izc, izp);
     }
     if (k + 1 < nz) {  // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
-      float ca = coef(L.l[2], r, k, nz, o), cb = coef(L.l[5], r, k, nz, o);
+      float ca = coefg(L.l[2], r, k, nz, o), cb = coefg(L.l[5], r, k, nz, o);
       float d = j > 0 ? ix[o - nz] : 0.0f;
       uz[o] = e_expr<OE>(ca, uz[o], cb, iyc, iyp, ixc, d);
     }
@@ -200,8 +214,13 @@ __global__ void __launch_bounds__(256) cpml_slab(Fields f, Lane cfl, SlabArg s, 
   const size_t plane = (size_t)ny * nz;
   const size_t o = (size_t)i * plane + (size_t)j * nz + k;
   const bool es = s.e_side != 0;
-  const float *cur = es ? f.i[s.other] : f.u[s.other];
-  float *t = es ? f.u[s.comp] : f.i[s.comp];
+  // the two arrays by selects on the uniform component numbers (a dynamic
+  // index into the argument would go through memory)
+  float *const *src = es ? f.i : f.u;
+  float *const *dst = es ? f.u : f.i;
+  const GAS float *cur =
+      (const GAS float *)(s.other == 0 ? src[0] : (s.other == 1 ? src[1] : src[2]));
+  GAS float *t = (GAS float *)(s.comp == 0 ? dst[0] : (s.comp == 1 ? dst[1] : dst[2]));
   float h, l;
   int z;
   if (s.normal == 0) {
@@ -232,12 +251,13 @@ __global__ void __launch_bounds__(256) cpml_slab(Fields f, Lane cfl, SlabArg s, 
     }
     z = kk;
   }
-  const float cf = coef(cfl, (size_t)i * ny + j, k, nz, o);
+  const float cf = coefg(cfl, (size_t)i * ny + j, k, nz, o);
   const float scale = es ? 1.0f : -1.0f;
   const float sv = s.sign * (h - l);
-  const float ps = flush(s.b[z] * s.psi[q] + s.a[z] * sv);
-  s.psi[q] = ps;
-  t[o] = flush(t[o] + scale * cf * (s.k[z] * sv + ps));
+  GAS float *psi = (GAS float *)s.psi;
+  const float ps = flush(((const GAS float *)s.b)[z] * psi[q] + ((const GAS float *)s.a)[z] * sv);
+  psi[q] = ps;
+  t[o] = flush(t[o] + scale * cf * (((const GAS float *)s.k)[z] * sv + ps));
   (void)nx;
 }
 
@@ -260,8 +280,6 @@ __global__ void __launch_bounds__(256) cpml_slab(Fields f, Lane cfl, SlabArg s, 
 // vector loads for uniform struct fields. Data pointers are cast to the
 // global space; read-only descriptors at uniform addresses to the constant
 // space, which the compiler reads through the scalar unit.
-#define GAS __attribute__((address_space(1)))
-#define CAS __attribute__((address_space(4)))
 template <class T>
 __device__ __forceinline__ T cload(const T *p, size_t i = 0) {
 #if defined(__HIP_DEVICE_COMPILE__)
