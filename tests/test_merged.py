@@ -122,6 +122,25 @@ def test_cell_bound_is_the_courant_number(s):
     assert abs(M.cell_stability(co, g, dt, (5, 5, 5), {}) - s * s) < 1e-6
 
 
+@pytest.mark.parametrize("size,s", [((1e-3,) * 3, 0.9), ((1e-3, 0.5e-3, 0.2e-3), 0.7)])
+def test_cell_bound_of_cubes_and_bricks(size, s):
+    """conformal.rs:3329-3351: for cubes or bricks of a uniform grid the cell
+    bound is c^2 dt^2 (1/dx^2 + 1/dy^2 + 1/dz^2) over its limit, the Courant
+    factor squared, in every cell; a face of a fifth of its area raises it;
+    the batched form agrees."""
+    g = E.Grid.uniform([6, 6, 6], list(size))
+    dt = E.TimeStep.courant(s).resolve(g)
+    lanes = E.coefficients(g, None, [E.FaceCondition.Pec] * 6, dt)
+    co = lanes.reshape(12, -1)
+    cells = [(2, 3, 2), (1, 1, 1), (4, 2, 3)]
+    for q in cells:
+        assert abs(M.cell_stability(co, g, dt, q, {}) - s * s) < 1e-5, q
+    got = M.cell_bounds(co, g, dt, np.array(cells), {})
+    assert np.allclose(got, s * s, atol=1e-5)
+    cut = {(2, M.index(g.nodes(), (2, 3, 2))): 0.2}
+    assert M.cell_stability(co, g, dt, (2, 3, 2), cut) > s * s * 1.05
+
+
 def test_small_faces_keep_their_current_where_the_time_step_allows():
     # cubes at the Courant limit: the face loses its current
     g, dt, lanes = _engine(0.99)
