@@ -369,3 +369,13 @@ def test_flat_cells_are_proven_by_the_cell_bound():
     g, dt, lanes = _engine(0.99)
     _, r, _ = _sliver(centroid=True, fraction=0.2).build(lanes, g, dt)
     assert not r.proven and r.verified
+
+
+def test_stability_rejects_invalid_tables():
+    """ems_conformal_stability checks every index it follows: a seed edge
+    outside the grid is refused, not read."""
+    g, dt, lanes = _engine(0.9)
+    with pytest.raises(ValueError):
+        M.stability(list(lanes.reshape(12, -1)), g.nodes(), None, [(1, 10 ** 9)], 10)
+    with pytest.raises(ValueError):
+        M.stability(list(lanes.reshape(12, -1)), g.nodes(), None, [(5, 0)], 10)
