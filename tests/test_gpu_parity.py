@@ -179,8 +179,12 @@ def test_pooled_and_dense_lanes_agree():
     same_fields(b, o)
 
 
-@pytest.mark.parametrize("n", [(2, 2, 2), (3, 2, 5), (2, 7, 3), (5, 1 + 256, 2), (4, 3, 300)])
+@pytest.mark.parametrize("n", [(2, 2, 2), (3, 2, 5), (2, 7, 3), (5, 1 + 256, 2), (4, 3, 300),
+                               (2, 40, 70), (3, 2, 700)])
 def test_odd_shapes(n):
+    """Every kernel on odd shapes; the last three also run the energy's
+    several term widths (rows of 71, 301, 701 nodes: 2, 8 and 8 terms per
+    lane, the last in two rounds) and E chains over several rounds."""
     def build(kind):
         g = E.Grid.uniform([n[0], n[1], n[2]], [1e-3] * 3)
         faces = [E.FaceCondition.Pmc] * 6
