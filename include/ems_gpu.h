@@ -228,9 +228,11 @@ int ems_gpu_planewave_aux(ems_gpu *h, int id, double *e, double *hh);
  * g' = flush(f32(x'))), then every driven face's current sum w g' as f32.
  * After every E update the circulations again (the energy look-ahead).
  * Columns of B, M and the factor are block-local positions; env_off has
- * one entry per row plus one. Runs the unfused schedule, at its place
- * among the serial operators; the engine energy adds
- * 1/2 g^T M g^(n+3/2). */
+ * one entry per row plus one. Runs the unfused schedule: the circulations
+ * and the update (they read u and the operator's own state only) run on a
+ * second stream of the engine beside the H update, the driven faces are
+ * written at the operator's place among the serial operators of the mid
+ * gap; the engine energy adds 1/2 g^T M g^(n+3/2). */
 typedef struct {
     int32_t n_groups, n_scalar, n_blocks, n_faces;
     double dt;
