@@ -160,3 +160,77 @@ def test_conformal_rod_in_cpml_box():
     assert np.array_equal(bits(a1.values()), bits(o.probe_values(b1)))
     assert np.array_equal(bits(a2.values()), bits(o.probe_values(b2)))
     same_fields(g, o)
+
+
+def _synthetic_merged(grid, dt, sizes, seed=3):
+    """A MergedFaces operator built by hand: one merge group, then one block
+    per entry of `sizes` with a full (dense) factor, so that the device runs
+    its general LDS path (a small dense block), its global-scratch path (a
+    block whose image exceeds 64 KiB) and its narrow path (a tridiagonal
+    block); circulation terms on random edges, driven faces with two weights."""
+    from ems_amd import merged as MF
+    rng = np.random.default_rng(seed)
+    n = grid.nodes()
+
+    def edge():
+        a = int(rng.integers(3))
+        p = [int(rng.integers(n[d] - (1 if d == a else 0))) for d in range(3)]
+        return a, tuple(p)
+    groups = []
+    total = 1 + sum(abs(s) for s in sizes)
+    for q in range(total):
+        terms = sorted({edge(): float(rng.choice([-1.0, 1.0]) * rng.uniform(0.5, 1.5))
+                        for _ in range(4)}.items(), key=lambda t: (t[0][0], MF.index(n, t[0][1])))
+        terms = [(a, p, w) for (a, p), w in terms]
+        groups.append(MF.Group.new(terms, 1e-9 * (1 + q % 3), 0.0, dt))
+    blocks, first = [], 1
+    for sz in sizes:
+        m = abs(sz)
+        rows = []
+        for i in range(m):
+            row = [(i, 4.0e-9 + 1e-10 * i)]
+            for j in range(m):
+                if j != i and (sz > 0 or abs(i - j) == 1):
+                    row.append((j, -1.0e-9 / m if sz > 0 else -1.0e-9))
+            rows.append(sorted(row))
+        fac = MF.Envelope.factor(rows)
+        assert fac is not None
+        blocks.append(MF.Block(list(range(first, first + m)), fac, rows, rows, dt))
+        first += m
+    faces, used = [], set()
+    while len(faces) < 12:
+        a = int(rng.integers(3))
+        b, c = MF.others(a)
+        p = [int(rng.integers(n[d] - (1 if d in (b, c) else 0))) for d in range(3)]
+        if (a, tuple(p)) in used:
+            continue
+        used.add((a, tuple(p)))
+        faces.append((a, tuple(p), [(int(rng.integers(total)), 0.7), (0, 0.3)]))
+    return MF.MergedFaces(tuple(n), dt, groups, 1, blocks, faces, [])
+
+
+def test_reduced_current_block_paths():
+    """Every device path of the block solve against the oracle, bit for bit:
+    a dense 20-row block (general LDS path of the small-block launch), a
+    dense 40-row block (general LDS path of the large-block launch), a
+    dense 140-row block (image above 64 KiB: global scratch), a tridiagonal
+    300-row block (narrow path of the large-block launch)."""
+    def build(kind):
+        g = E.Grid.uniform([10, 9, 8], [1e-3] * 3)
+        e = K.make(kind, g, None)
+        op = _synthetic_merged(g, e.dt, [20, 40, 140, -300])
+        e.add_operator(op)
+        rng = np.random.default_rng(11)
+        for a in range(3):
+            e.set_component(0, a, rng.standard_normal(g.nodes()).astype(np.float32))
+        return e, op
+    (g, pg), (o, po) = build("gpu"), build("oracle")
+    for n in (1, 3, 6):
+        g.run(n)
+        o.run(n)
+        for x, y in zip(pg.currents(), po.currents()):
+            assert np.array_equal(bits(x), bits(y)), n
+        wg, wo = g.energy(), o.energy()
+        assert np.float64(wg).tobytes() == np.float64(wo).tobytes(), (n, wg, wo)
+    same_fields(g, o)
+    assert np.abs(pg.currents()[0]).max() > 0.0
