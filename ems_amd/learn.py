@@ -252,6 +252,38 @@ def early_port_result(fmt: TraceFormat, freqs, dt: float, v_obs, i_obs, pred,
     return post.PortResult(freqs, list(vf), list(i_f), complex(fmt.resistance, 0.0))
 
 
+def _dft_matrix(t0: float, step: float, n: int, freqs) -> np.ndarray:
+    """E[f, k] = exp(-j 2 pi f (t0 + k step)) step: the DFT of post.dft as a
+    matrix, so that many traces of one time step transform at once."""
+    t = t0 + np.arange(n) * step
+    return np.exp(-2j * np.pi * np.outer(np.asarray(freqs, np.float64), t)) * step
+
+
+def early_s11(fmt: TraceFormat, freqs, dts, v_obs, i_obs, pred) -> List[List[complex]]:
+    """S11 of early_port_result for many models at once (models of one time
+    step share their DFT matrices); the same sums as the per-model form up to
+    the rounding of the transform."""
+    n0, D, R = fmt.observed, fmt.decimation, fmt.resistance
+    vh = pred[:, 0, fmt.N:fmt.T].astype(np.float64)
+    ih = pred[:, 1, fmt.N:fmt.T].astype(np.float64) / R
+    vh[:, 0] *= (D + 1) / (2.0 * D)
+    ih[:, 0] *= (D + 1) / (2.0 * D)
+    v = np.asarray(v_obs, np.float64)[:, :n0]
+    i = np.asarray(i_obs, np.float64)[:, :n0]
+    out: List[List[complex]] = [None] * len(dts)
+    for dt in sorted(set(float(d) for d in dts)):
+        qs = [q for q, d in enumerate(dts) if float(d) == dt]
+        ev = _dft_matrix(dt, dt, n0, freqs)
+        eh = _dft_matrix((n0 + 1) * dt, D * dt, vh.shape[1], freqs)
+        ei = _dft_matrix(0.5 * dt, dt, n0, freqs)
+        eih = _dft_matrix((n0 + 0.5) * dt, D * dt, ih.shape[1], freqs)
+        vf = v[qs] @ ev.T + vh[qs] @ eh.T
+        i_f = i[qs] @ ei.T + ih[qs] @ eih.T
+        for k, q in enumerate(qs):
+            out[q] = post.PortResult(freqs, list(vf[k]), list(i_f[k]), complex(R, 0.0)).s11()
+    return out
+
+
 def truncated_port_result(fmt: TraceFormat, freqs, dt: float, v_obs, i_obs) -> post.PortResult:
     """The method without the network: the spectra of the observed samples
     alone."""
@@ -418,9 +450,11 @@ def early_stop_sweep(es: EarlyStop, models, limits, freqs=FREQS_1_3_GHZ, tile_ro
     fb = [bool(r > es.threshold_db) for r in rdb]
     s11 = [None] * len(models)
     steps = [fmt.observed] * len(models)
-    for q in range(len(models)):
-        if not fb[q]:
-            s11[q] = early_port_result(fmt, freqs, models[q].dt, v[q], i[q], pred[q]).s11()
+    keep = [q for q in range(len(models)) if not fb[q]]
+    if keep:
+        got = early_s11(fmt, freqs, [models[q].dt for q in keep], v[keep], i[keep], pred[keep])
+        for q, x in zip(keep, got):
+            s11[q] = x
     redo = [q for q in range(len(models)) if fb[q]]
     if redo:
         engines, ports = _batch_engines([models[q] for q in redo])

@@ -85,3 +85,21 @@ def test_spectral_matrix_is_the_weighted_spectrum():
     t = np.arange(n) * 8 * DT
     want = sum(wf * abs(np.sum(d * np.exp(-2j * np.pi * f * t))) ** 2 for f, wf in zip(freqs, wt))
     assert np.isclose(d @ K @ d, want / (len(freqs) * n), rtol=1e-5)
+
+
+def test_batched_early_spectra_match_the_per_model_form():
+    """early_s11 (many models, DFT matrices shared by models of one time
+    step) against early_port_result model by model: the same sums up to the
+    transform's rounding."""
+    fmt = L.TraceFormat(4096, 8, 1024)
+    rng = np.random.default_rng(3)
+    v0, i0 = traces()
+    v = np.stack([v0 * (1 + 0.1 * q) for q in range(4)])
+    i = np.stack([i0 * (1 - 0.05 * q) for q in range(4)])
+    pred = np.stack([fmt.targets(v[q], i[q])[0] for q in range(4)])
+    pred += 1e-3 * rng.standard_normal(pred.shape).astype(np.float32)
+    dts = [DT, DT, 2 * DT, DT]
+    got = L.early_s11(fmt, L.FREQS_1_3_GHZ, dts, v, i, pred)
+    for q in range(4):
+        ref = L.early_port_result(fmt, L.FREQS_1_3_GHZ, dts[q], v[q], i[q], pred[q]).s11()
+        assert max(abs(a - b) for a, b in zip(got[q], ref)) < 1e-9
