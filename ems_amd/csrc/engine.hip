@@ -2109,6 +2109,7 @@ struct ems_gpu {
   int tj = 7, nchunk = 0;
   // energy scratch
   double *d_rowsum = nullptr, *d_chains = nullptr, *d_planes = nullptr, *d_w = nullptr;
+  double *h_w = nullptr;  // pinned host copy of W^n
   // graphs of whole steps, per starting buffer
   hipGraphExec_t g1[2] = {nullptr, nullptr}, gK[2] = {nullptr, nullptr};
   int K = 32;  // even: a K-step graph ends on the buffer it started from
@@ -2815,7 +2816,9 @@ void advance_counts(ems_gpu *h, uint64_t steps) {
   h->step += steps;
 }
 
-int energy_now(ems_gpu *h, double *w) {
+// The invariant's kernels and the copy of W^n into pinned host memory
+// (h->h_w), queued on the engine's stream; energy_now waits for it.
+int energy_enqueue(ems_gpu *h) {
   const int nx = h->n[0], ny = h->n[1], nz = h->n[2];
   const size_t rows = (size_t)nx * (3 * ny - 2);
   const Fields &f = h->fb[h->par];
@@ -2824,6 +2827,7 @@ int energy_now(ems_gpu *h, double *w) {
     CK(h, hipMalloc(&h->d_chains, (size_t)nx * 12 * sizeof(double)));
     CK(h, hipMalloc(&h->d_planes, (size_t)nx * sizeof(double)));
     CK(h, hipMalloc(&h->d_w, sizeof(double)));
+    CK(h, hipHostMalloc((void **)&h->h_w, sizeof(double), hipHostMallocDefault));
   }
   // a wave per H row, per (plane, component) E chains and per plane
   dim3 g1((unsigned)((rows + 3) / 4));
@@ -2850,10 +2854,15 @@ int energy_now(ems_gpu *h, double *w) {
   hipLaunchKernelGGL(energy_combine, dim3(1), dim3(64), 0, h->stream, nx, h->d_planes, h->dt,
                      h->d_w);
   CK(h, hipGetLastError());
-  double v = 0;
-  CK(h, hipMemcpyAsync(&v, h->d_w, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  CK(h, hipMemcpyAsync(h->h_w, h->d_w, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  return 0;
+}
+
+int energy_now(ems_gpu *h, double *w) {
+  int rc = energy_enqueue(h);
+  if (rc) return rc;
   CK(h, hipStreamSynchronize(h->stream));
-  *w = v;
+  *w = *h->h_w;
   return 0;
 }
 
@@ -2981,6 +2990,7 @@ void ems_gpu_destroy(ems_gpu *h) {
   if (h->side) (void)hipStreamDestroy(h->side);
   if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->h_w) (void)hipHostFree(h->h_w);
   delete h;
 }
 
@@ -3793,6 +3803,27 @@ int ems_gpu_energy(ems_gpu *h, double *w) {
   if (rc) return rc;
   if ((rc = ops_energy(h, &w_ops))) return rc;
   *w = wf + w_ops;
+  return EMS_OK;
+}
+
+int ems_gpu_energy_many(ems_gpu *const *models, int n, const uint8_t *active, double *out) {
+  if (!models || n < 0 || !out) return EMS_E_INVALID;
+  for (int q = 0; q < n; q++) {
+    if (active && !active[q]) continue;
+    ems_gpu *h = models[q];
+    ALIVE(h);
+    int rc = energy_enqueue(h);
+    if (rc) return rc;
+  }
+  for (int q = 0; q < n; q++) {
+    if (active && !active[q]) continue;
+    ems_gpu *h = models[q];
+    CK(h, hipStreamSynchronize(h->stream));
+    double w_ops = 0;
+    int rc = ops_energy(h, &w_ops);
+    if (rc) return rc;
+    out[q] = *h->h_w + w_ops;
+  }
   return EMS_OK;
 }
 

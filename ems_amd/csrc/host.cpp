@@ -736,11 +736,12 @@ int ems_run_until_batch(ems_gpu_batch *b, int n, ems_gpu *const *models,
     if (!any) break;
     // The energy step of drive(): W^n of the state before the step.
     std::vector<uint64_t> t0(n);
-    for (int q = 0; q < n; q++) {
-      if (!act[q]) continue;
-      t0[q] = ems_gpu_timestep(models[q]);
-      if ((rc = ems_gpu_energy(models[q], &st[q].w))) return rc;
-    }
+    std::vector<double> wq(n, 0.0);
+    for (int q = 0; q < n; q++)
+      if (act[q]) t0[q] = ems_gpu_timestep(models[q]);
+    if ((rc = ems_gpu_energy_many(models, n, act.data(), wq.data()))) return rc;
+    for (int q = 0; q < n; q++)
+      if (act[q]) st[q].w = wq[q];
     if ((rc = ems_gpu_batch_run(b, act.data(), 1))) return rc;
     uint64_t steps_now = 0;
     for (int q = 0; q < n; q++) {

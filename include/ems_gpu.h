@@ -278,6 +278,12 @@ typedef struct {
     const double *face_w;
 } ems_merged_desc;
 int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id);
+/* ems_gpu_energy of many engines at once (no reference counterpart; the
+ * batch run loop's energy checks): every active engine's evaluation is
+ * queued before any is waited for. out[q] for active[q] != 0 (all when
+ * active is NULL). */
+int ems_gpu_energy_many(ems_gpu *const *models, int n, const uint8_t *active, double *out);
+
 /* The reduced currents g (n_groups values) and the circulations of the
  * last H or E update (either may be NULL). */
 int ems_gpu_merged_state(ems_gpu *h, int id, double *cur, double *circ);
