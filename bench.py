@@ -83,6 +83,7 @@ def sweep(args, log) -> int:
     on the GPU (each its own engine), then all together in one batch.
     value = batched S11 solves per hour; both include set-up and analysis."""
     import ems_amd as E
+    from ems_amd import learn as L
     from ems_amd import workloads as W
     freqs = [1e9 + 2e9 * q / 200.0 for q in range(201)]
     lim = E.RunLimits(args.max_steps, E.EndCriteria.energy_decay_db(-40.0),
@@ -129,12 +130,7 @@ def sweep(args, log) -> int:
         seq_phase = dict(phase)
 
     t0 = time.perf_counter()
-    engines, ports = [], []
-    for m in models:
-        e = E.Engine(m.grid, m.dt, m.lanes)
-        ports.append(m.build_ops(e))
-        e.release_dense_lanes()
-        engines.append(e)
+    engines, ports = L._batch_engines(models)
     b = E.EngineBatch(engines)
     b.set_schedule(tj, args.x_chunks, packed=packed)
     t1 = time.perf_counter()

@@ -312,15 +312,25 @@ def _schedule(b, models, tile_rows: int, x_chunks: int = 16) -> None:
         b.set_schedule(tile_rows, x_chunks)
 
 
-def _batch_engines(models):
+def _batch_engines(models, workers: int = 8):
+    """One engine per model with its operators. The models are independent
+    and the native calls (coefficient pooling, CPML layers, uploads) release
+    the interpreter lock, so they are built by a few threads at once; the
+    order of the result is the models'."""
+    from concurrent.futures import ThreadPoolExecutor
     from .engine import Engine
-    engines, ports = [], []
-    for m in models:
+
+    def one(m):
         e = Engine(m.grid, m.dt, m.lanes)
-        ports.append(m.build_ops(e))
+        p = m.build_ops(e)
         e.release_dense_lanes()
-        engines.append(e)
-    return engines, ports
+        return e, p
+    if workers <= 1 or len(models) < 2:
+        built = [one(m) for m in models]
+    else:
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            built = list(ex.map(one, models))
+    return [b[0] for b in built], [b[1] for b in built]
 
 
 def solver_traces(models, steps: int, group: int = 256, tile_rows: int = 7, log=None):
