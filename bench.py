@@ -136,7 +136,10 @@ def sweep(args, log) -> int:
     t1 = time.perf_counter()
     outs = b.run_until(lim)
     t2 = time.perf_counter()
-    s11b = [p.analyze(freqs).s11() for p in ports]
+    # the ports' transforms are native and release the interpreter lock
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        s11b = list(ex.map(lambda p: p.analyze(freqs).s11(), ports))
     t_bat = time.perf_counter() - t0
     bat_phase = {"setup": t1 - t0, "run": t2 - t1, "analysis": time.perf_counter() - t2}
     cell_steps = sum(m.cells * o.steps for m, o in zip(models, outs))
