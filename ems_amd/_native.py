@@ -18,6 +18,7 @@ EMS_E_INVALID = -1
 EMS_E_UNSUPPORTED = -2
 EMS_E_IO = -3
 EMS_E_DEVICE = -4
+EMS_PROBE_F32_SUM = 4  # or-ed into a probe's quantity
 
 
 class EmsError(RuntimeError):
@@ -166,6 +167,9 @@ ems_gaussian = _sig("ems_gaussian", _d, _d, _d, _d)
 ems_gaussian_duration = _sig("ems_gaussian_duration", _d, _d)
 ems_gaussian_table = _sig("ems_gaussian_table", None, _d, _d, _d, _d, _u64, _fp)
 ems_gaussian_values = _sig("ems_gaussian_values", None, _d, _d, _d, _d, _u64, _dp)
+ems_openems_gaussian = _sig("ems_openems_gaussian", _d, _d, _d, _d, _d)
+ems_openems_gaussian_duration = _sig("ems_openems_gaussian_duration", _d, _d)
+ems_openems_source_table = _sig("ems_openems_source_table", _i, _d, _d, _d, _i, _d, _u64, _fp)
 ems_fold_parallel = _sig("ems_fold_parallel", _i, C.c_float, C.c_float, _d, _d, _d,
                          C.POINTER(C.c_float), C.POINTER(C.c_float))
 ems_dft = _sig("ems_dft", None, _d, _d, _dp, _u64, _dp, _i, _dp)

@@ -30,13 +30,32 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
+    """Compiles every source to an object in parallel (one hipcc process
+    each), then links the shared library."""
+    objs = [os.path.join(HERE, "csrc", os.path.basename(src) + ".o") for src in SOURCES]
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    procs = []
+    for src, obj in zip(SOURCES, objs):
+        cmd = [hipcc(), *compile_flags, "-c", "-o", obj, src]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                            text=True)))
+    failed = []
+    for cmd, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(" ".join(cmd))
+    if failed:
+        raise RuntimeError("hipcc failed: " + "; ".join(failed))
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *SOURCES]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+        raise RuntimeError("hipcc link failed: " + " ".join(cmd))
     os.replace(tmp, OUT)
+    for obj in objs:
+        os.remove(obj)
     if verbose:
         print("built", OUT)
     return OUT

@@ -1111,6 +1111,7 @@ struct DevLumped {
 
 struct DevOp {
   int type, kind, quantity, n;
+  int f32sum;  // probes: the f32 sum of openEMS's current probes
   uint64_t interval;
   const int32_t *axis;
   const uint64_t *idx;
@@ -1167,7 +1168,14 @@ __device__ void source_apply(Fields f, const DevOp &o, uint64_t n) {
 __device__ void probe_sample(Fields f, const DevOp &o) {
   float *const *fld = o.quantity == 1 ? f.i : f.u;
   double v = -0.0;
-  for (int q = 0; q < o.n; q++) v += o.wd[q] * (double)fld[o.axis[q]][o.idx[q]];
+  if (o.f32sum) {
+    // Probe::f32_sum (probe.rs:398-404): (w as f32) * x summed in f32 from 0.
+    float s = 0.0f;
+    for (int q = 0; q < o.n; q++) s = s + (float)o.wd[q] * fld[o.axis[q]][o.idx[q]];
+    v = (double)s;
+  } else {
+    for (int q = 0; q < o.n; q++) v += o.wd[q] * (double)fld[o.axis[q]][o.idx[q]];
+  }
   uint64_t c = *o.count;
   o.vals[c] = v;
   *o.count = c + 1;
@@ -2026,7 +2034,7 @@ struct HostSlab {
 };
 
 struct HostOp {
-  int type = 0, kind = 0, quantity = 0, n = 0;
+  int type = 0, kind = 0, quantity = 0, n = 0, f32sum = 0;
   uint64_t interval = 1;
   int32_t *d_axis = nullptr;
   uint64_t *d_idx = nullptr;
@@ -2765,6 +2773,7 @@ int prepare(ems_gpu *h, uint64_t steps) {
       HostOp &o = h->ops[q];
       DevOp &d = dv[q];
       d.type = o.type, d.kind = o.kind, d.quantity = o.quantity, d.n = o.n;
+      d.f32sum = o.f32sum;
       d.interval = o.interval;
       d.axis = o.d_axis, d.idx = o.d_idx, d.wf = o.d_wf, d.wd = o.d_wd;
       d.amp = o.d_amp, d.namp = o.namp, d.vals = o.d_vals, d.count = o.d_count;
@@ -3105,6 +3114,8 @@ int ems_gpu_add_source(ems_gpu *h, int kind, int n_edges, const int32_t *axis,
 int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *axis,
                       const int32_t *nodes, const double *w, uint64_t interval, int *id) {
   ALIVE(h);
+  const int f32sum = (quantity & EMS_PROBE_F32_SUM) != 0;
+  quantity &= ~EMS_PROBE_F32_SUM;
   if (quantity < 0 || quantity > 1) return fail(h, EMS_E_INVALID, "probe quantity");
   if (n_entries < 0 || (n_entries && (!axis || !nodes || !w)))
     return fail(h, EMS_E_INVALID, "probe arrays");
@@ -3117,6 +3128,7 @@ int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *ax
   HostOp o;
   o.type = OP_PROBE;
   o.quantity = quantity;
+  o.f32sum = f32sum;
   o.n = n_entries;
   o.interval = interval ? interval : 1;
   o.first = (h->step + o.interval - 1) / o.interval * o.interval;

@@ -434,6 +434,48 @@ void ems_gaussian_table(double f0, double fc, double dt, double offset, uint64_t
     amp[n] = (float)ems_gaussian(f0, fc, ((double)n + offset) * dt);
 }
 
+// Waveform::OpenEmsGaussian (src/waveform.rs:454-466; oe_tau / oe_delay
+// :260-268): tau = 3/(2 pi fc), T = 3 tau. Sampled (dt > 0): sample index
+// k = floor(t/dt + 1/4 + 1e-9) (either grid), non-zero for
+// 1 <= k <= floor(2T/dt + 1e-9); continuous (dt = 0): 0 < t <= 2T.
+double ems_openems_gaussian(double f0, double fc, double t, double dt) {
+  const double tau = 3.0 / (2.0 * M_PI * fc), delay = 3.0 * tau;
+  bool inside;
+  if (dt > 0.0) {
+    const double k = std::floor(t / dt + 0.25 + 1e-9);
+    inside = k >= 1.0 && k <= std::floor(2.0 * delay / dt + 1e-9);
+  } else {
+    inside = t > 0.0 && t <= 2.0 * delay;
+  }
+  if (!inside) return 0.0;
+  const double x = (t - delay) / tau;
+  return std::cos(2.0 * M_PI * f0 * (t - delay)) * std::exp(-x * x);
+}
+
+double ems_openems_gaussian_duration(double fc) { return 6.0 * 3.0 / (2.0 * M_PI * fc); }
+
+// openems_signal / openems_lag (src/sim/map.rs:278-305): the amplitude of a
+// model excitation with openEMS timing, tabulated per step at the engine's
+// sample time of the source kind (SoftE (n+1/2)dt, HardE (n+1)dt, SoftH
+// n dt, HardH (n+1/2)dt): the openEMS Gaussian at tau = t - lag - D dt
+// (lag dt/2 for soft, dt for hard kinds; D = floor(delay/dt)), 0 for
+// tau < -dt/4; as f32.
+int ems_openems_source_table(double f0, double fc, double delay, int kind, double dt,
+                             uint64_t count, float *amp) {
+  if (kind < 0 || kind > 3 || !(dt > 0.0) || (count && !amp)) return EMS_E_INVALID;
+  static const double offset[4] = {0.5, 1.0, 0.0, 0.5};
+  const bool hard = kind == 1 || kind == 3;
+  const double lag = hard ? dt : 0.5 * dt;
+  const double steps = delay > 0.0 ? std::floor(delay / dt) : 0.0;
+  const double shift = lag + steps * dt;
+  for (uint64_t n = 0; n < count; n++) {
+    const double t = ((double)n + offset[kind]) * dt;
+    const double tau = t - shift;
+    amp[n] = tau < -0.25 * dt ? 0.0f : (float)ems_openems_gaussian(f0, fc, tau, dt);
+  }
+  return EMS_OK;
+}
+
 // fold_parallel (src/engine/lumped.rs:275-282): g and c in parallel with the
 // edge's recovered C = dt(1 + ca)/(2cb) and G = (1 - ca)/cb.
 int ems_fold_parallel(float ca, float cb, double dt, double g, double c, float *ca_out,

@@ -286,30 +286,67 @@ def pool_lane(dense: np.ndarray):
 # ------------------------------------------------------------ waveform --
 
 class Waveform:
-    """src/waveform.rs: the ems Gaussian (cos * exp on [0, 2T])."""
+    """src/waveform.rs: the ems Gaussian (cos * exp on [0, 2T]) or openEMS's
+    Gaussian pulse (``openems_gaussian``: tau = 3/(2 pi fc), T = 3 tau, its
+    samples k = 1 .. floor(2T/dt))."""
 
-    def __init__(self, f0: float, fc: float):
+    def __init__(self, f0: float, fc: float, openems: bool = False):
         self.f0, self.fc = float(f0), float(fc)
+        self.openems = bool(openems)
 
     @staticmethod
     def gaussian(f0: float, fc: float) -> "Waveform":
         return Waveform(f0, fc)
 
+    @staticmethod
+    def openems_gaussian(f0: float, fc: float) -> "Waveform":
+        return Waveform(f0, fc, openems=True)
+
     def value(self, t: float, dt: float = 0.0) -> float:
+        if self.openems:
+            return float(nat.ems_openems_gaussian(self.f0, self.fc, t, dt))
         return float(nat.ems_gaussian(self.f0, self.fc, t))
 
     def duration(self) -> float:
+        if self.openems:
+            return float(nat.ems_openems_gaussian_duration(self.fc))
         return float(nat.ems_gaussian_duration(self.fc))
 
     def values(self, dt: float, offset: float, count: int) -> np.ndarray:
         """value((n + offset) * dt) for n < count, in f64."""
         out = np.empty(count, dtype=np.float64)
+        if self.openems:
+            for n in range(count):
+                out[n] = self.value((n + offset) * dt, dt)
+            return out
         nat.ems_gaussian_values(self.f0, self.fc, dt, offset, count, nat.ptr(out, C.c_double))
         return out
 
     def table(self, dt: float, offset: float, count: int) -> np.ndarray:
+        if self.openems:
+            return self.values(dt, offset, count).astype(np.float32)
         out = np.empty(count, dtype=np.float32)
         nat.ems_gaussian_table(self.f0, self.fc, dt, offset, count, nat.ptr(out, C.c_float))
+        return out
+
+
+class OpenEmsSignal:
+    """The amplitude of a model excitation with openEMS's timing
+    (``openems_signal`` / ``openems_lag``, src/sim/map.rs:278-305): a field
+    at time T receives the openEMS Gaussian's sample at T - dt (soft
+    sources sample mid-update, so their lag is dt/2; hard ones dt), delayed
+    by floor(delay/dt) whole steps."""
+
+    def __init__(self, waveform: "Waveform", delay: float = 0.0):
+        if not waveform.openems:
+            raise EmsError(nat.EMS_E_INVALID, "OpenEmsSignal takes an openEMS Gaussian")
+        self.waveform, self.delay = waveform, float(delay)
+
+    def table(self, kind: int, dt: float) -> np.ndarray:
+        count = int(math.ceil((self.waveform.duration() + self.delay) / dt)) + 3
+        out = np.empty(count, dtype=np.float32)
+        nat.check(nat.ems_openems_source_table(self.waveform.f0, self.waveform.fc, self.delay,
+                                               int(kind), dt, count, nat.ptr(out, C.c_float)))
         return out
 
 
@@ -349,6 +386,8 @@ class EdgeSource:
 
     def _table(self, dt: float) -> np.ndarray:
         off = _SOURCE_OFFSET[self._kind]
+        if isinstance(self.amplitude, OpenEmsSignal):
+            return self.amplitude.table(self._kind, dt)
         if isinstance(self.amplitude, Waveform):
             count = int(math.ceil(self.amplitude.duration() / dt)) + 2
             return self.amplitude.table(dt, off, count)
@@ -370,6 +409,7 @@ class Probe:
         self.quantity = ProbeQuantity(quantity)
         self.entries: List[Tuple[int, Tuple[int, int, int], float]] = []
         self._interval = 1
+        self._f32_sum = False
         self._engine = None
         self._id = -1
 
@@ -388,6 +428,15 @@ class Probe:
     def interval(self, n: int) -> "Probe":
         self._interval = max(int(n), 1)
         return self
+
+    def f32_sum(self, on: bool = True) -> "Probe":
+        """Probe::f32_sum (probe.rs:398-404): the sum of the weighted
+        entries in f32 (openEMS's current probes)."""
+        self._f32_sum = bool(on)
+        return self
+
+    def _c_quantity(self) -> int:
+        return int(self.quantity) | (nat.EMS_PROBE_F32_SUM if self._f32_sum else 0)
 
     def values(self) -> np.ndarray:
         return self._engine._probe_values(self._id)
@@ -754,10 +803,11 @@ class Engine:
         elif isinstance(op, Probe):
             axis, nodes, w = self._edges(op.entries, np.float64)
             oid = C.c_int()
-            self._check(nat.ems_gpu_add_probe(h, int(op.quantity), len(op.entries),
+            q = op._c_quantity()
+            self._check(nat.ems_gpu_add_probe(h, q, len(op.entries),
                                               nat.ptr(axis, C.c_int32), nat.ptr(nodes, C.c_int32),
                                               nat.ptr(w, C.c_double), op._interval, C.byref(oid)))
-            self.setup.append(("probe", int(op.quantity), axis, nodes, w, op._interval))
+            self.setup.append(("probe", q, axis, nodes, w, op._interval))
             op._engine = self
             op._id = oid.value
         elif type(op).__name__ == "PlaneWave":

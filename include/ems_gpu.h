@@ -88,7 +88,10 @@ int ems_gpu_add_source(ems_gpu *h, int kind, int n_edges, const int32_t *axis,
 
 /* Probe (src/engine/probe.rs:240-513): quantity 0 voltage (u after the E
  * update, t = (n+1)dt), 1 current (i after the H update, t = (n+1/2)dt);
- * value = sum w*f64(x) in entry order, every `interval` steps. */
+ * value = sum w*f64(x) in entry order, every `interval` steps. With
+ * EMS_PROBE_F32_SUM or-ed into the quantity, Probe::f32_sum (probe.rs:398-404,
+ * the sum of openEMS's current probes): f64(sum of f32(w)*x in f32 from 0). */
+#define EMS_PROBE_F32_SUM 4
 int ems_gpu_add_probe(ems_gpu *h, int quantity, int n_entries, const int32_t *axis,
                       const int32_t *nodes, const double *w, uint64_t interval, int *id);
 

@@ -94,6 +94,20 @@ void ems_gaussian_values(double f0, double fc, double dt, double offset, uint64_
 void ems_gaussian_table(double f0, double fc, double dt, double offset, uint64_t count,
                         float *amp);
 
+/* Waveform::openems_gaussian(f0, fc).value(t, dt) (src/waveform.rs:454-466):
+ * tau = 3/(2 pi fc), T = 3 tau; with dt > 0 openEMS's sampled signal (sample
+ * k = floor(t/dt + 1/4) non-zero for 1 <= k <= floor(2T/dt)). */
+double ems_openems_gaussian(double f0, double fc, double t, double dt);
+/* Its duration 2T (waveform.rs:506-513). */
+double ems_openems_gaussian_duration(double fc);
+/* The f32 amplitude table of an EdgeSource of `kind` (0 SoftE .. 3 HardH)
+ * driven with openEMS timing (openems_signal / openems_lag,
+ * src/sim/map.rs:278-305): amp[n] = the openEMS Gaussian at the kind's
+ * sample time of step n minus the lag (dt/2 soft, dt hard) and
+ * floor(delay/dt) steps, 0 before its first sample. */
+int ems_openems_source_table(double f0, double fc, double delay, int kind, double dt,
+                             uint64_t count, float *amp);
+
 /* fold_parallel (src/engine/lumped.rs:275-282): adds conductance g [S] and
  * capacitance c [F] in parallel to an E edge with coefficients (ca, cb).
  * EMS_E_INVALID for cb = 0 (a PEC edge). */

@@ -194,7 +194,7 @@ class OracleEngine:
             axis, nodes, w = E.Engine._edges(op.entries, np.float64)
             axis = axis.astype(np.intc)
             nodes = nodes.astype(np.intc)
-            op._oid = orc_add_probe(h, int(op.quantity), len(op.entries), P(axis, C.c_int),
+            op._oid = orc_add_probe(h, op._c_quantity(), len(op.entries), P(axis, C.c_int),
                                     P(nodes, C.c_int), P(w, C.c_double), op._interval)
             assert op._oid >= 0
         elif type(op).__name__ == "PlaneWave":

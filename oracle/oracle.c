@@ -137,7 +137,7 @@ typedef struct {
     float *amp;
     uint64_t namp;
     /* probe */
-    int quantity;
+    int quantity, f32sum;
     double *wd;
     uint64_t interval;
     double *vals;
@@ -546,7 +546,14 @@ static void source_apply(orc_engine *e, op_t *o, uint64_t n) {
 static void probe_sample(orc_engine *e, op_t *o) {
     float **f = o->quantity == 1 ? e->i : e->u;
     double v = -0.0;
-    for (int q = 0; q < o->n; q++) v += o->wd[q] * (double)f[o->axis[q]][o->idx[q]];
+    if (o->f32sum) {
+        /* Probe::f32_sum (probe.rs:398-404): (w as f32)*x folded in f32 from 0. */
+        float s = 0.0f;
+        for (int q = 0; q < o->n; q++) s = s + (float)o->wd[q] * f[o->axis[q]][o->idx[q]];
+        v = (double)s;
+    } else {
+        for (int q = 0; q < o->n; q++) v += o->wd[q] * (double)f[o->axis[q]][o->idx[q]];
+    }
     if (o->nvals == o->cap) {
         o->cap = o->cap ? 2 * o->cap : 1024;
         o->vals = (double *)realloc(o->vals, o->cap * sizeof(double));
@@ -1064,7 +1071,8 @@ int orc_add_probe(orc_engine *e, int quantity, int n_entries, const int *axis, c
     int id = add_op(e);
     op_t *o = &e->ops[id];
     o->type = OP_PROBE;
-    o->quantity = quantity;
+    o->f32sum = (quantity & 4) != 0;
+    o->quantity = quantity & ~4;
     o->n = n_entries;
     o->interval = interval ? interval : 1;
     o->axis = (int *)malloc(n_entries * sizeof(int) + 1);
