@@ -138,6 +138,7 @@ ems_cn_loss_grad = _sig("ems_cn_loss_grad", _i, _p, _fp, _fp, _i, _i, _i, _fp, _
 ems_gpu_run = _sig("ems_gpu_run", _i, _p, _u64)
 ems_gpu_step_with_energy = _sig("ems_gpu_step_with_energy", _i, _p, _dp)
 ems_gpu_energy = _sig("ems_gpu_energy", _i, _p, _dp)
+ems_gpu_energy_noalloc = _sig("ems_gpu_energy_noalloc", _i, _p, _dp)
 ems_gpu_timestep = _sig("ems_gpu_timestep", _u64, _p)
 ems_gpu_read_component = _sig("ems_gpu_read_component", _i, _p, _i, _i, _fp)
 ems_gpu_write_component = _sig("ems_gpu_write_component", _i, _p, _i, _i, _fp)

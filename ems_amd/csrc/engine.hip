@@ -33,9 +33,17 @@
 #include <cstdlib>
 #include <string>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ems_gpu.h"
+
+// Scratch of ems_gpu_energy_noalloc: one static device array per GPU the
+// library runs on, held until the process ends (16 MiB). Host code never
+// passes this name to a kernel or a copy (on the host it is a stand-in, not
+// a device address): it uses the address hipGetSymbolAddress returns for
+// the current device (noalloc_scratch below).
+static __device__ __align__(256) double scratch[EMS_ENERGY_NOALLOC_DOUBLES];
 
 namespace {
 
@@ -2825,25 +2833,26 @@ void advance_counts(ems_gpu *h, uint64_t steps) {
   h->step += steps;
 }
 
-// The invariant's kernels and the copy of W^n into pinned host memory
-// (h->h_w), queued on the engine's stream; energy_now waits for it.
-int energy_enqueue(ems_gpu *h) {
+// Scratch of one evaluation, in doubles: a sum per H row (nx * (3 ny - 2)),
+// four chains per plane and E component (12 nx), a value per plane (nx)
+// and W^n itself (1).
+size_t energy_scratch_doubles(const ems_gpu *h) {
+  const size_t nx = (size_t)h->n[0], ny = (size_t)h->n[1];
+  return nx * (3 * ny - 2) + 12 * nx + nx + 1;
+}
+
+// The invariant's kernels on the engine's stream, with the scratch arrays
+// given; W^n lands in *d_w (device memory).
+int energy_kernels(ems_gpu *h, double *rowsum, double *chains, double *planes, double *d_w) {
   const int nx = h->n[0], ny = h->n[1], nz = h->n[2];
   const size_t rows = (size_t)nx * (3 * ny - 2);
   const Fields &f = h->fb[h->par];
-  if (!h->d_rowsum) {
-    CK(h, hipMalloc(&h->d_rowsum, rows * sizeof(double)));
-    CK(h, hipMalloc(&h->d_chains, (size_t)nx * 12 * sizeof(double)));
-    CK(h, hipMalloc(&h->d_planes, (size_t)nx * sizeof(double)));
-    CK(h, hipMalloc(&h->d_w, sizeof(double)));
-    CK(h, hipHostMalloc((void **)&h->h_w, sizeof(double), hipHostMallocDefault));
-  }
   // a wave per H row, per (plane, component) E chains and per plane
   dim3 g1((unsigned)((rows + 3) / 4));
   const int nt = nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8));
 #define EH(OE, NT)                                                                           \
   hipLaunchKernelGGL((energy_h_rows<OE, NT>), g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, \
-                     ny, nz, h->d_rowsum)
+                     ny, nz, rowsum)
   if (h->oe) {
     if (nt == 1) EH(true, 1); else if (nt == 2) EH(true, 2); else if (nt == 4) EH(true, 4);
     else EH(true, 8);
@@ -2855,14 +2864,31 @@ int energy_enqueue(ems_gpu *h) {
   CK(h, hipGetLastError());
   dim3 g2((unsigned)(((size_t)nx * 3 + 3) / 4));
   hipLaunchKernelGGL(energy_e_chains, g2, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny, nz,
-                     h->d_chains);
+                     chains);
   CK(h, hipGetLastError());
   hipLaunchKernelGGL(energy_planes, dim3((unsigned)((nx + 3) / 4)), dim3(256), 0, h->stream, nx,
-                     ny, h->d_rowsum, h->d_chains, h->d_planes);
+                     ny, rowsum, chains, planes);
   CK(h, hipGetLastError());
-  hipLaunchKernelGGL(energy_combine, dim3(1), dim3(64), 0, h->stream, nx, h->d_planes, h->dt,
-                     h->d_w);
+  hipLaunchKernelGGL(energy_combine, dim3(1), dim3(64), 0, h->stream, nx, planes, h->dt, d_w);
   CK(h, hipGetLastError());
+  return 0;
+}
+
+// The invariant's kernels and the copy of W^n into pinned host memory
+// (h->h_w), queued on the engine's stream; energy_now waits for it. The
+// scratch is the engine's own, allocated at the first evaluation.
+int energy_enqueue(ems_gpu *h) {
+  const int nx = h->n[0], ny = h->n[1];
+  const size_t rows = (size_t)nx * (3 * ny - 2);
+  if (!h->d_rowsum) {
+    CK(h, hipMalloc(&h->d_rowsum, rows * sizeof(double)));
+    CK(h, hipMalloc(&h->d_chains, (size_t)nx * 12 * sizeof(double)));
+    CK(h, hipMalloc(&h->d_planes, (size_t)nx * sizeof(double)));
+    CK(h, hipMalloc(&h->d_w, sizeof(double)));
+    CK(h, hipHostMalloc((void **)&h->h_w, sizeof(double), hipHostMallocDefault));
+  }
+  int rc = energy_kernels(h, h->d_rowsum, h->d_chains, h->d_planes, h->d_w);
+  if (rc) return rc;
   CK(h, hipMemcpyAsync(h->h_w, h->d_w, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   return 0;
 }
@@ -3836,6 +3862,70 @@ int ems_gpu_energy_many(ems_gpu *const *models, int n, const uint8_t *active, do
     if (rc) return rc;
     out[q] = *h->h_w + w_ops;
   }
+  return EMS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Device address of `scratch` per device, looked up once per device with
+// that device current (each GPU holds its own copy at its own address).
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[64] = {};
+
+int noalloc_scratch(ems_gpu *h, double **out) {
+  int dev = -1;
+  CK(h, hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(h, EMS_E_UNSUPPORTED, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(h, EMS_E_DEVICE,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ems_gpu_energy_noalloc(ems_gpu *h, double *w) {
+  ALIVE(h);
+  if (!w) return EMS_E_INVALID;
+  const size_t need = energy_scratch_doubles(h);
+  if (need > (size_t)EMS_ENERGY_NOALLOC_DOUBLES)
+    return fail(h, EMS_E_INVALID,
+                "energy_noalloc: the grid needs " + std::to_string(need) +
+                    " doubles of scratch, the static array holds " +
+                    std::to_string((size_t)EMS_ENERGY_NOALLOC_DOUBLES) +
+                    " (nx*(3*ny-2) + 13*nx + 1 at most)");
+  // Operator energies read back from device state need host buffers.
+  for (auto &o : h->ops)
+    if (o.type == OP_POLES || o.type == OP_MERGED || (o.type == OP_LUMPED && o.lp.branch))
+      return fail(h, EMS_E_UNSUPPORTED,
+                  "energy_noalloc: dispersive media, reduced currents and lumped branches "
+                  "carry energies read back into host buffers; use ems_gpu_energy");
+  double *s = nullptr;
+  int rc = noalloc_scratch(h, &s);
+  if (rc) return rc;
+  const size_t nx = (size_t)h->n[0], rows = nx * (3 * (size_t)h->n[1] - 2);
+  double *rowsum = s, *chains = s + rows, *planes = chains + 12 * nx, *d_w = planes + nx;
+  if ((rc = energy_kernels(h, rowsum, chains, planes, d_w))) return rc;
+  double wf = 0.0;
+  CK(h, hipMemcpyAsync(&wf, d_w, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  CK(h, hipStreamSynchronize(h->stream));
+  // Engine::operator_energy of the remaining kinds (every one 0, folded
+  // from -0.0 as ops_energy does).
+  double w_ops = -0.0;
+  if (!h->slabs.empty()) w_ops += 0.0;
+  for (size_t q = 0; q < h->ops.size(); q++) w_ops += 0.0;
+  *w = wf + w_ops;
   return EMS_OK;
 }
 

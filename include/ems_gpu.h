@@ -318,6 +318,19 @@ int ems_gpu_run(ems_gpu *h, uint64_t n);
 int ems_gpu_step_with_energy(ems_gpu *h, double *w);
 /* Engine::energy (src/engine/mod.rs:722-726): read-only W^n. */
 int ems_gpu_energy(ems_gpu *h, double *w);
+/* No counterpart in the reference: ems_gpu_energy for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same value, bit for bit, but it calls no
+ * hipMalloc, hipFree or hipHostMalloc: its scratch (row sums, plane chains,
+ * plane values) lives in one static device array per GPU of
+ * EMS_ENERGY_NOALLOC_DOUBLES doubles (16 MiB), so grids with
+ * nx*(3*ny - 2) + 13*nx + 1 above that are refused with EMS_E_INVALID, and
+ * engines holding dispersive media, reduced currents or lumped branches
+ * (whose energies are read back into host buffers) with EMS_E_UNSUPPORTED.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (from several host threads) race; each call waits for its own result. */
+#define EMS_ENERGY_NOALLOC_DOUBLES (2u * 1024u * 1024u)
+int ems_gpu_energy_noalloc(ems_gpu *h, double *w);
 uint64_t ems_gpu_timestep(const ems_gpu *h);
 
 /* Engine::u_component / i_component / fields_mut (src/engine/mod.rs:659-690):
