@@ -56,6 +56,7 @@ orc_planewave_aux = _sig("orc_planewave_aux", _i, _p, _i, _dp, _dp)
 orc_add_merged = _sig("orc_add_merged", _i, _p, _p)
 orc_merged_state = _sig("orc_merged_state", _i, _p, _i, _dp, _dp)
 orc_run = _sig("orc_run", None, _p, _u64)
+orc_set_fused = _sig("orc_set_fused", None, _p, _i)
 orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
 orc_energy = _sig("orc_energy", _d, _p)
 orc_timestep = _sig("orc_timestep", _u64, _p)
@@ -79,12 +80,14 @@ class OracleEngine:
     """The CPU restatement, fed the same arrays as :class:`ems_amd.Engine`."""
 
     def __init__(self, grid: E.Grid, dt: float, lanes: np.ndarray, curl_order: int = 0,
-                 threads: int = 1, pool: bool = True):
+                 threads: int = 1, pool: bool = True, fused: int = 0):
         self.grid, self.dt = grid, float(dt)
         self._dense_lanes = lanes
         n = grid.nodes()
         self._h = orc_create(n[0], n[1], n[2], self.dt, curl_order, threads)
         assert self._h, "orc_create failed"
+        # FusedSteps: 0 Auto (the reference's default), 1 Always, 2 Never
+        orc_set_fused(self._h, fused)
         for l in range(12):
             lane = np.ascontiguousarray(lanes[l])
             pooled = E.pool_lane(lane) if pool else None

@@ -191,7 +191,8 @@ struct orc_engine {
     int *slab_start;
     pthread_t *workers;
     pthread_barrier_t bar;
-    int cmd; /* 1 H, 2 E, 3 energy (read only), 9 exit */
+    int cmd; /* 1 H, 2 E, 3 energy (read only), 4/5 fused phases, 9 exit */
+    int fuse; /* FusedSteps: 0 Auto, 1 Always, 2 Never (pool.rs:131-164) */
     int energy_flag;
     char err[256];
 };
@@ -487,6 +488,29 @@ static void phase(orc_engine *e, int cmd, int s) {
             if (e->ops[q].type == OP_POLES) poles_post_e(e, e->ops[q].pm, p0, p1);
     } else if (cmd == 3) {
         for (int gi = p0; gi < p1; gi++) e->plane_energy[gi] = h_plane(e, gi, 1, 0);
+    } else if (cmd == 4) {
+        /* Pool::fused_a (pool.rs:405-473): the H update of the slab's last
+         * plane (it reads the next slab's first voltages before that slab
+         * updates them) and its CPML H corrections (post_h_planes). */
+        const int g = p1 - 1;
+        h_plane(e, g, 0, 1);
+        for (int c = 0; c < e->ncpml; c++)
+            if (!e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], g, g + 1);
+    } else if (cmd == 5) {
+        /* Pool::fused_b (pool.rs:475-594): plane by plane, H of plane g
+         * (all but the last) with its CPML H corrections, then E of plane g
+         * with its CPML E corrections. The currents of plane g - 1 are final
+         * by then (in this slab, or the previous slab's fused_a). */
+        for (int g = p0; g < p1; g++) {
+            if (g + 1 < p1) {
+                h_plane(e, g, 0, 1);
+                for (int c = 0; c < e->ncpml; c++)
+                    if (!e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], g, g + 1);
+            }
+            e_plane(e, g);
+            for (int c = 0; c < e->ncpml; c++)
+                if (e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], g, g + 1);
+        }
     }
 }
 
@@ -829,6 +853,21 @@ static void one_step(orc_engine *e, int energy) {
     int moved = mid && mid_movable;
     int gap_a = need_post_e || moved;
 
+    /* The fused schedule (pool.rs:884-940): no energy step, no serial hook
+     * between the half-steps and only plane-wise slab hooks (CPML; the
+     * dispersive media keep the unfused one here); FusedSteps::Auto when the
+     * fields of a slab (24 bytes per node) reach FUSE_MIN_BYTES = 4 MiB. */
+    int fused = !energy && !gap_b && e->fuse != 2;
+    for (int q = 0; q < e->nops && fused; q++)
+        if (e->ops[q].type == OP_POLES) fused = 0;
+    if (fused && e->fuse == 0) fused = 24 * e->N / (size_t)e->threads >= ((size_t)4 << 20);
+    if (fused) {
+        for (int q = 0; q < e->nops; q++)
+            if (e->ops[q].type == OP_LUMPED) lumped_pre_e(e, &e->ops[q]);
+        parallel(e, 4);
+        parallel(e, 5);
+        goto post_e;
+    }
     e->energy_flag = energy;
     parallel(e, 1);
     e->energy_flag = 0;
@@ -849,6 +888,7 @@ static void one_step(orc_engine *e, int energy) {
             if (e->ops[q].type == OP_LUMPED) lumped_pre_e(e, &e->ops[q]);
     }
     parallel(e, 2);
+post_e:
     if (gap_a) {
         if (moved)
             for (int q = 0; q < e->nops; q++) {
@@ -1378,6 +1418,8 @@ double orc_energy(orc_engine *e) {
 }
 
 uint64_t orc_timestep(orc_engine *e) { return e->step; }
+
+void orc_set_fused(orc_engine *e, int mode) { e->fuse = mode < 0 || mode > 2 ? 0 : mode; }
 
 int orc_read(orc_engine *e, int field, int axis, float *out) {
     if (axis < 0 || axis > 2) return -1;

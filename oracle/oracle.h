@@ -171,6 +171,10 @@ int orc_add_merged(orc_engine *e, const orc_merged_desc *d);
 int orc_merged_state(orc_engine *e, int id, double *cur, double *circ);
 
 void orc_run(orc_engine *e, uint64_t n);
+/* FusedSteps (pool.rs:131-164): 0 Auto (fused when 24 bytes per node of a
+ * thread's slab reach 4 MiB and the step allows it), 1 Always, 2 Never.
+ * Results are bit-identical either way. */
+void orc_set_fused(orc_engine *e, int mode);
 double orc_step_with_energy(orc_engine *e);
 double orc_energy(orc_engine *e);
 uint64_t orc_timestep(orc_engine *e);
