@@ -139,6 +139,7 @@ ems_gpu_run = _sig("ems_gpu_run", _i, _p, _u64)
 ems_gpu_step_with_energy = _sig("ems_gpu_step_with_energy", _i, _p, _dp)
 ems_gpu_energy = _sig("ems_gpu_energy", _i, _p, _dp)
 ems_gpu_energy_noalloc = _sig("ems_gpu_energy_noalloc", _i, _p, _dp)
+ems_gpu_energy_in = _sig("ems_gpu_energy_in", _i, _p, _i, _dp)
 ems_gpu_timestep = _sig("ems_gpu_timestep", _u64, _p)
 ems_gpu_read_component = _sig("ems_gpu_read_component", _i, _p, _i, _i, _fp)
 ems_gpu_write_component = _sig("ems_gpu_write_component", _i, _p, _i, _i, _fp)
@@ -150,6 +151,7 @@ ems_gpu_time_kernels = _sig("ems_gpu_time_kernels", _i, _p, _u64, _dp)
 ems_gpu_traffic_model = _sig("ems_gpu_traffic_model", _i, _p, _dp)
 ems_gpu_set_schedule = _sig("ems_gpu_set_schedule", _i, _p, _i, _i, _i)
 ems_gpu_set_xcd_group = _sig("ems_gpu_set_xcd_group", _i, _p, _i)
+ems_gpu_set_cpml_split = _sig("ems_gpu_set_cpml_split", _i, _p, _i)
 ems_gpu_copy_bandwidth = _sig("ems_gpu_copy_bandwidth", _i, _p, _u64, _i, _dp)
 
 # include/ems_host.h

@@ -105,6 +105,16 @@ struct Fields {
   float *i[3];
 };
 
+// A lane by a wave-uniform runtime index, through selects with constant
+// indices (a dynamic index into a kernel argument would go through memory).
+__device__ __forceinline__ Lane pick_lane(const Lanes &L, int l) {
+  Lane x = L.l[0];
+#pragma unroll
+  for (int q = 1; q < 12; q++)
+    if (l == q) x = L.l[q];
+  return x;
+}
+
 // coef through global-address-space pointers.
 __device__ __forceinline__ float coefg(const Lane &L, size_t r, int k, int nz, size_t flat) {
   if (L.rows == nullptr) return 0.0f;
@@ -295,6 +305,118 @@ __device__ __forceinline__ T cload(const T *p, size_t i = 0) {
 #else
   return p[i];  // (the host pass only checks the code)
 #endif
+}
+
+// ---- CPML slabs outside the fused kernel ----
+//
+// A slab's H-side correction t = flush(t + (-db)*(k*s + psi')) with
+// psi' = flush(b*psi + a*s) depends on t only through the final add: s is a
+// difference of old voltages u^n, which the fused step only reads. So the
+// increment (-db)*(k*s + psi') can be formed before the step (cpml_delta_h)
+// and added by the fused kernel where the slab's turn comes, with the same
+// bits. The E-side correction reads the new currents and is the last write
+// to its component, so a slab whose corrections follow every other slab's
+// at its nodes (a suffix of the registration order) can run after the fused
+// kernel (cpml_post_e). The fused step moves the slabs normal to z there:
+// in a k-tile only the few lanes inside such a slab would run its code, and
+// the whole wave would pay for it.
+struct DSlab {
+  int comp, normal, other;
+  float sign;
+  int lo0, hi0, lo1, hi1, lo2, hi2, ly, lz;
+  const float *b, *a, *k;
+  float *psi0, *psi1;  // H side: read psi[par], write psi[1 - par]; E side: psi0
+  float *delta;        // H side: the step's increments (box order)
+  uint32_t first;      // the slab's first thread in its launch
+};
+
+__device__ __forceinline__ int dslab_of(const DSlab *ds, int nds, uint32_t q) {
+  int s = 0;
+  while (s + 1 < nds && cload(ds, s + 1).first <= q) s++;
+  return s;
+}
+
+// The increments of the H-side slabs ds (cpml.rs:495-543 with scale -1),
+// from u^n in A; psi read from buffer par, written to the other.
+__global__ void __launch_bounds__(256) cpml_delta_h(const float *__restrict__ A, size_t N, Lanes L,
+                                                    const DSlab *__restrict__ ds, int nds,
+                                                    uint32_t total, int par, int nx, int ny,
+                                                    int nz) {
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  if (q >= total) return;
+  const DSlab d = cload(ds, dslab_of(ds, nds, q));
+  const uint32_t p = q - d.first;
+  const int kk = (int)(p % (uint32_t)d.lz);
+  const uint32_t t2 = p / (uint32_t)d.lz;
+  const int jj = (int)(t2 % (uint32_t)d.ly), ii = (int)(t2 / (uint32_t)d.ly);
+  const int i = d.lo0 + ii, j = d.lo1 + jj, k = d.lo2 + kk;
+  const size_t plane = (size_t)ny * nz;
+  const size_t o = (size_t)i * plane + (size_t)j * nz + k;
+  const GAS float *u = (const GAS float *)(A + (size_t)d.other * N);
+  // s = u_other[node + e_n] - u_other[node], 0 beyond the grid
+  float hv;
+  int z;
+  if (d.normal == 0) hv = i + 1 < nx ? u[o + plane] : 0.0f, z = ii;
+  else if (d.normal == 1) hv = j + 1 < ny ? u[o + nz] : 0.0f, z = jj;
+  else hv = k + 1 < nz ? u[o + 1] : 0.0f, z = kk;
+  const float sv = d.sign * (hv - u[o]);
+  const float cf = coefg(pick_lane(L, 9 + d.comp), (size_t)i * ny + j, k, nz, o);
+  const GAS float *pin = (const GAS float *)(par == 0 ? d.psi0 : d.psi1);
+  GAS float *pout = (GAS float *)(par == 1 ? d.psi0 : d.psi1);
+  const float ps = flush(((const GAS float *)d.b)[z] * pin[p] + ((const GAS float *)d.a)[z] * sv);
+  pout[p] = ps;
+  const float scale = -1.0f;
+  ((GAS float *)d.delta)[p] = scale * cf * (((const GAS float *)d.k)[z] * sv + ps);
+}
+
+// The E-side slabs ds after the fused kernel, on its output B (new u and i);
+// no two of them correct one component at one node.
+__global__ void __launch_bounds__(256) cpml_post_e(float *__restrict__ B, size_t N, Lanes L,
+                                                   const DSlab *__restrict__ ds, int nds,
+                                                   uint32_t total, int ny, int nz) {
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  if (q >= total) return;
+  const DSlab d = cload(ds, dslab_of(ds, nds, q));
+  const uint32_t p = q - d.first;
+  const int kk = (int)(p % (uint32_t)d.lz);
+  const uint32_t t2 = p / (uint32_t)d.lz;
+  const int jj = (int)(t2 % (uint32_t)d.ly), ii = (int)(t2 / (uint32_t)d.ly);
+  const int i = d.lo0 + ii, j = d.lo1 + jj, k = d.lo2 + kk;
+  const size_t plane = (size_t)ny * nz;
+  const size_t o = (size_t)i * plane + (size_t)j * nz + k;
+  const GAS float *cur = (const GAS float *)(B + (size_t)(3 + d.other) * N);
+  // s = i_other[node] - i_other[node - e_n], 0 below the grid
+  float lv;
+  int z;
+  if (d.normal == 0) lv = i > 0 ? cur[o - plane] : 0.0f, z = ii;
+  else if (d.normal == 1) lv = j > 0 ? cur[o - nz] : 0.0f, z = jj;
+  else lv = k > 0 ? cur[o - 1] : 0.0f, z = kk;
+  const float sv = d.sign * (cur[o] - lv);
+  const float cf = coefg(pick_lane(L, 3 + d.comp), (size_t)i * ny + j, k, nz, o);
+  GAS float *psi = (GAS float *)d.psi0;
+  const float ps = flush(((const GAS float *)d.b)[z] * psi[p] + ((const GAS float *)d.a)[z] * sv);
+  psi[p] = ps;
+  GAS float *t = (GAS float *)(B + (size_t)d.comp * N);
+  const float scale = 1.0f;
+  t[o] = flush(t[o] + scale * cf * (((const GAS float *)d.k)[z] * sv + ps));
+}
+
+// The increments of the H-side slabs handled by cpml_delta_h at node
+// (i, j, k), in registration order, over the slabs whose bits are set in
+// `mask` (those reaching the tile and the plane).
+__device__ __forceinline__ void h_out_apply(const DSlab *__restrict__ ds, uint32_t mask, int i,
+                                            int j, int k, float h[3]) {
+  while (mask) {
+    const int q = __builtin_ctz(mask);
+    mask &= mask - 1;
+    const DSlab d = cload(ds, q);
+    if (j < d.lo1 || j >= d.hi1 || k < d.lo2 || k >= d.hi2) continue;
+    const size_t p = ((size_t)(i - d.lo0) * d.ly + (j - d.lo1)) * d.lz + (k - d.lo2);
+    const float dv = ((const GAS float *)d.delta)[p];
+    if (d.comp == 0) h[0] = flush(h[0] + dv);
+    else if (d.comp == 1) h[1] = flush(h[1] + dv);
+    else h[2] = flush(h[2] + dv);
+  }
 }
 
 // One CPML slab as the fused kernel reads it (cpml.rs:394-420). H-side psi
@@ -523,6 +645,8 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
                                            const uint32_t *__restrict__ croff,
                                            const FSlab *__restrict__ slabs,
                                            const uint32_t *__restrict__ pmask, int nslab, int par,
+                                           const DSlab *__restrict__ hout,
+                                           const uint32_t *__restrict__ pmask_o, int nhout,
                                            int nx, int ny, int nz, int tk, int chunk, int ktiles,
                                            int jtiles, int t) {
   const int kt = t % ktiles;
@@ -555,6 +679,12 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
       else mh |= 1u << q;
     }
   }
+  // H-side slabs whose increments come from cpml_delta_h.
+  uint32_t mo = 0;
+  for (int q = 0; q < nhout; q++) {
+    const DSlab s = cload(hout, q);
+    if (s.lo1 < j0 + TJ && s.hi1 > j0 - 1 && s.lo2 < k0 + tk && s.hi2 > k0 - 1) mo |= 1u << q;
+  }
   __shared__ float sh[2][3][TJ + 1][64];
 
   float uyc, uzc, hy_prev = 0.0f, hz_prev = 0.0f;
@@ -574,6 +704,7 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
         float h[3];
         h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh & cload(pmask, 2 * (i0 - 1)), par, false, i0 - 1,
                    j, k, h);
+        if (mo) h_out_apply(hout, mo & cload(pmask_o, i0 - 1), i0 - 1, j, k, h);
         hy_prev = h[1];
         hz_prev = h[2];
       }
@@ -597,7 +728,10 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
     // Slabs that reach this tile and this plane (bit masks, scalar).
     const uint32_t mhp = mh & cload(pmask, 2 * i), mep = me & cload(pmask, 2 * i + 1);
     float h[3] = {0.0f, 0.0f, 0.0f};
-    if (inb) h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mhp, par, own, i, j, k, h);
+    if (inb) {
+      h_from<OE>(P, uyc, uzc, iv, jv, kv, slabs, mhp, par, own, i, j, k, h);
+      if (mo) h_out_apply(hout, mo & cload(pmask_o, i), i, j, k, h);
+    }
     sh[buf][0][ty][tx] = h[0];
     sh[buf][1][ty][tx] = h[1];
     sh[buf][2][ty][tx] = h[2];
@@ -661,12 +795,164 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
     step_fused(const float *__restrict__ A, float *__restrict__ B, size_t N,
                const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
                const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
-               int par, int nx, int ny, int nz, int tk, int chunk, int ktiles, int jtiles,
+               int par, const DSlab *__restrict__ hout, const uint32_t *__restrict__ pmask_o,
+               int nhout, int nx, int ny, int nz, int tk, int chunk, int ktiles, int jtiles,
                int ntiles, int xcd_group) {
   const int t = tile_of_block(blockIdx.x, ktiles, ntiles, xcd_group);
   if (t < 0) return;
-  fused_tile<OE, TJ, PF>(A, B, N, cpool, croff, slabs, pmask, nslab, par, nx, ny, nz, tk, chunk,
-                         ktiles, jtiles, t);
+  fused_tile<OE, TJ, PF>(A, B, N, cpool, croff, slabs, pmask, nslab, par, hout, pmask_o, nhout, nx,
+                         ny, nz, tk, chunk, ktiles, jtiles, t);
+}
+
+// ---- fused step, bands of rows per wave ----
+//
+// A wave owns a k-window of 63 nodes (lane 0 is the halo node k0-1, whose
+// H the window's first E update needs) and a band of TB consecutive rows
+// j0..j0+TB-1, and marches a chunk of x-planes. Per plane it walks the
+// band's rows in order: first the halo row j0-1 (H only), then each row
+// (H, then E with the H of the row before it, of the lane before it and of
+// the plane before it). So the j- and x-neighbours come from registers
+// (carried from the previous row and the previous plane) and the
+// k-neighbour from the lane below: no LDS, no barrier, waves independent.
+// Only one halo row per TB rows, one halo lane per 63 and the plane before
+// a chunk are computed twice (read-only; their owners write them).
+// Same expressions on the same inputs as the other schedules: same bits.
+template <bool OE, int TB>
+__device__ __forceinline__ void band_tile(const float *__restrict__ A, float *__restrict__ B,
+                                          size_t N, const float *__restrict__ cpool,
+                                          const uint32_t *__restrict__ croff,
+                                          const FSlab *__restrict__ slabs,
+                                          const uint32_t *__restrict__ pmask, int nslab, int par,
+                                          int nx, int ny, int nz, int chunk, int kw, int nbands,
+                                          int t) {
+  const int w = t % kw;
+  const int t2 = t / kw;
+  const int b = t2 % nbands;
+  const int c = t2 / nbands;
+  const int lane = threadIdx.x;
+  const int k0 = 63 * w, k = k0 - 1 + lane;
+  const bool kin = k >= 0 && k < nz;  // a node of the grid
+  const bool own = kin && lane > 0;    // owned (E written)
+  const unsigned ku = (unsigned)min(max(k, 0), nz - 1);
+  const unsigned k4_ = ku * 4u;
+  const bool kv = (int)ku < nz - 1;
+  const int j0 = b * TB;
+  const int i0 = c * chunk, i1 = min(nx, i0 + chunk);
+  const size_t plane = (size_t)ny * nz;
+  // CPML slabs whose (j, k) box meets this wave's rows (halo included).
+  uint32_t mh = 0, me = 0;
+  for (int q = 0; q < nslab; q++) {
+    const FSlab s = cload(slabs, q);
+    if (s.lo1 < j0 + TB && s.hi1 > j0 - 1 && s.lo2 < k0 + 63 && s.hi2 > k0 - 1) {
+      if (s.e_side) me |= 1u << q;
+      else mh |= 1u << q;
+    }
+  }
+  float hyp[TB], hzp[TB], uyc[TB], uzc[TB];
+#pragma unroll
+  for (int r = 0; r < TB; r++) hyp[r] = hzp[r] = uyc[r] = uzc[r] = 0.0f;
+#pragma unroll
+  for (int r = 0; r < TB; r++) {
+    const int j = j0 + r;
+    if (j >= ny) break;
+    unsigned k4 = k4_;
+    asm volatile("" : "+v"(k4));
+    const size_t ob = ((size_t)i0 * ny + j) * nz;
+    uyc[r] = ldf(A + N + ob, k4);
+    uzc[r] = ldf(A + 2 * N + ob, k4);
+    if (i0 > 0) {
+      // H of plane i0-1 (read-only: its owner writes it).
+      const bool jv = j < ny - 1;
+      PlaneIn Q;
+      load_plane(Q, A, N, cpool, croff, ob - plane, (size_t)(i0 - 1) * ny + j, plane, nz, k4, true,
+                 jv, kv);
+      const float uyp = ldf(A + N + (ob - plane), k4), uzp = ldf(A + 2 * N + (ob - plane), k4);
+      if (kin) {
+        float h[3];
+        h_from<OE>(Q, uyp, uzp, true, jv, kv, slabs, mh & cload(pmask, 2 * (i0 - 1)), par, false,
+                   i0 - 1, j, k, h);
+        hyp[r] = h[1];
+        hzp[r] = h[2];
+      }
+    }
+  }
+  for (int i = i0; i < i1; i++) {
+    const bool iv = i + 1 < nx;
+    const uint32_t mhp = mh & cload(pmask, 2 * i), mep = me & cload(pmask, 2 * i + 1);
+    // H of the halo row j0-1 (read-only): hx and hz feed row j0's E.
+    float hxj = 0.0f, hzj = 0.0f;
+    if (j0 > 0) {
+      unsigned k4 = k4_;
+      asm volatile("" : "+v"(k4));
+      const int j = j0 - 1;
+      const size_t r = (size_t)i * ny + j, ob = r * nz;
+      PlaneIn Q;
+      load_plane(Q, A, N, cpool, croff, ob, r, plane, nz, k4, iv, true, kv);
+      const float uyh = ldf(A + N + ob, k4), uzh = ldf(A + 2 * N + ob, k4);
+      if (kin) {
+        float h[3];
+        h_from<OE>(Q, uyh, uzh, iv, true, kv, slabs, mhp, par, false, i, j, k, h);
+        hxj = h[0];
+        hzj = h[2];
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < TB; rr++) {
+      const int j = j0 + rr;
+      if (j >= ny) break;
+      // An opaque copy of the lane offset per row: "scalar base + 32-bit
+      // offset" addressing at every access.
+      unsigned k4 = k4_;
+      asm volatile("" : "+v"(k4));
+      const bool jv = j < ny - 1;
+      const size_t r = (size_t)i * ny + j, ob = r * nz;
+      PlaneIn P;
+      load_plane(P, A, N, cpool, croff, ob, r, plane, nz, k4, iv, jv, kv);
+      float h[3] = {P.ix, P.iy, P.iz};
+      if (kin) h_from<OE>(P, uyc[rr], uzc[rr], iv, jv, kv, slabs, mhp, par, own, i, j, k, h);
+      // H of the lane below (k - 1); zero below k = 0.
+      float hx_km = __shfl_up(h[0], 1), hy_km = __shfl_up(h[1], 1);
+      if (k <= 0) hx_km = hy_km = 0.0f;
+      if (own) {
+        float e0 = P.uxc, e1 = uyc[rr], e2 = uzc[rr];
+        // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
+        if (iv) e0 = e_expr<OE>(P.c[0], e0, P.c[3], h[2], hzj, h[1], hy_km);
+        // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
+        if (jv) e1 = e_expr<OE>(P.c[1], e1, P.c[4], h[0], hx_km, h[2], hzp[rr]);
+        // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
+        if (kv) e2 = e_expr<OE>(P.c[2], e2, P.c[5], h[1], hyp[rr], h[0], hxj);
+        if (mep)
+          cpml_node<true>(slabs, mep, par, true, i, j, k, h[2] - hzj, h[1] - hy_km,
+                          h[2] - hzp[rr], h[0] - hx_km, h[1] - hyp[rr], h[0] - hxj, P.c[3],
+                          P.c[4], P.c[5], e0, e1, e2);
+        stf(B + ob, k4, e0);
+        stf(B + N + ob, k4, e1);
+        stf(B + 2 * N + ob, k4, e2);
+        stf(B + 3 * N + ob, k4, h[0]);
+        stf(B + 4 * N + ob, k4, h[1]);
+        stf(B + 5 * N + ob, k4, h[2]);
+      }
+      hxj = h[0];
+      hzj = h[2];
+      hyp[rr] = h[1];
+      hzp[rr] = h[2];
+      uyc[rr] = P.uyn;
+      uzc[rr] = P.uzn;
+    }
+  }
+}
+
+template <bool OE, int TB>
+__global__ void __launch_bounds__(64, 4)
+    step_band(const float *__restrict__ A, float *__restrict__ B, size_t N,
+              const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
+              const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
+              int par, int nx, int ny, int nz, int chunk, int kw, int nbands, int ntiles,
+              int xcd_group) {
+  const int t = tile_of_block(blockIdx.x, kw, ntiles, xcd_group);
+  if (t < 0) return;
+  band_tile<OE, TB>(A, B, N, cpool, croff, slabs, pmask, nslab, par, nx, ny, nz, chunk, kw, nbands,
+                    t);
 }
 
 // One model of a batched step: what step_fused takes as arguments.
@@ -698,8 +984,9 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
   const FusedArgs a = cload(ms, lo);
   const int t = tile_of_block(b - (int)cload(seg, lo), a.ktiles, a.ntiles, xcd_group);
   if (t < 0) return;
-  fused_tile<OE, TJ, PF>(a.A, a.B, a.N, a.cpool, a.croff, a.slabs, a.pmask, a.nslab, a.par, a.nx,
-                         a.ny, a.nz, a.tk, a.chunk, a.ktiles, a.jtiles, t);
+  fused_tile<OE, TJ, PF>(a.A, a.B, a.N, a.cpool, a.croff, a.slabs, a.pmask, a.nslab, a.par,
+                         nullptr, nullptr, 0, a.nx, a.ny, a.nz, a.tk, a.chunk, a.ktiles, a.jtiles,
+                         t);
 }
 
 // ---- fused step, packed short rows ----
@@ -1855,16 +2142,7 @@ __device__ __forceinline__ double chain_add(double acc, const double *b, int t, 
   return acc;
 }
 
-// A lane or field array by a wave-uniform runtime index, through selects
-// with constant indices (a dynamic index into a kernel argument would go
-// through memory).
-__device__ __forceinline__ Lane pick_lane(const Lanes &L, int l) {
-  Lane x = L.l[0];
-#pragma unroll
-  for (int q = 1; q < 12; q++)
-    if (l == q) x = L.l[q];
-  return x;
-}
+// A field array by a wave-uniform runtime index, through selects.
 __device__ __forceinline__ const float *pick3(float *const *a, int c) {
   return c == 0 ? a[0] : (c == 1 ? a[1] : a[2]);
 }
@@ -2038,7 +2316,9 @@ struct HostSlab {
   SlabArg arg;  // arg.psi is set per launch (parity)
   float *d_b = nullptr, *d_a = nullptr, *d_k = nullptr;
   float *d_psi[2] = {nullptr, nullptr};  // H side: double-buffered; E side: [0] only
+  float *d_delta = nullptr;  // H side, run outside the fused kernel: its increments
   size_t len = 0;
+  bool out = false;  // corrected outside the fused kernel (cpml_delta_h / cpml_post_e)
 };
 
 struct HostOp {
@@ -2114,13 +2394,21 @@ struct ems_gpu {
   std::vector<HostSlab> slabs;
   FSlab *d_fslabs = nullptr;
   uint32_t *d_pmask = nullptr;  // per x-plane: slabs (H, E) whose x range holds it
+  // Slabs normal to z corrected outside the tiled fused kernel: their
+  // records, the per-x-plane mask of the H-side ones, and thread counts.
+  DSlab *d_hout = nullptr, *d_eout = nullptr;
+  uint32_t *d_pmask_o = nullptr;
+  int n_hout = 0, n_eout = 0;
+  uint32_t hout_total = 0, eout_total = 0;
+  bool cpml_split = true;  // allowed (ems_gpu_set_cpml_split)
+  int split_state = -1;    // what the uploaded tables hold (-1: nothing yet)
   std::vector<HostOp> ops;
   DevOp *d_ops = nullptr;
   bool ops_dirty = true, slabs_dirty = true;
   uint64_t step = 0;
   uint64_t *d_step = nullptr;
   // fused schedule and its tiling
-  bool fused = true, prefetch = true, rows = false, packed = false;
+  bool fused = true, prefetch = true, rows = false, packed = false, band = false;
   int xcd_group = 2;  // tile rows per XCD visit (2 measured best of 0, 1, 2, 4, 8)
   int tj = 7, nchunk = 0;
   // energy scratch
@@ -2314,13 +2602,92 @@ FusedGeom fused_geom(const ems_gpu *h) {
   return g;
 }
 
-int upload_fslabs(ems_gpu *h) {
-  if (!h->slabs_dirty) return 0;
+// Marks the slabs the tiled fused step corrects outside its kernel: per
+// side the longest suffix of that side's slabs (registration order) that
+// are normal to z, so that every correction of a node's component outside
+// the kernel follows those inside it. E-side ones must not correct one
+// component at one node twice (they run in one launch); otherwise that
+// side keeps all its slabs inside.
+void mark_out_slabs(ems_gpu *h, bool split) {
+  for (auto &s : h->slabs) s.out = false;
+  if (!split) return;
+  for (int side = 0; side < 2; side++) {
+    std::vector<HostSlab *> suf;
+    for (auto it = h->slabs.rbegin(); it != h->slabs.rend(); ++it) {
+      if (it->len == 0 || it->arg.e_side != side) continue;
+      if (it->arg.normal != 2) break;
+      suf.push_back(&*it);
+    }
+    bool ok = !suf.empty() && suf.size() <= 32;
+    for (size_t a = 0; ok && side == 1 && a < suf.size(); a++)
+      for (size_t b = a + 1; ok && b < suf.size(); b++) {
+        const SlabArg &x = suf[a]->arg, &y = suf[b]->arg;
+        bool meet = x.comp == y.comp;
+        for (int d = 0; d < 3 && meet; d++) meet = x.lo[d] < y.hi[d] && y.lo[d] < x.hi[d];
+        ok = !meet;
+      }
+    if (ok)
+      for (HostSlab *q : suf) q->out = true;
+  }
+}
+
+int upload_out_slabs(ems_gpu *h) {
+  if (h->d_hout) CK(h, hipFree(h->d_hout));
+  if (h->d_eout) CK(h, hipFree(h->d_eout));
+  if (h->d_pmask_o) CK(h, hipFree(h->d_pmask_o));
+  h->d_hout = h->d_eout = nullptr;
+  h->d_pmask_o = nullptr;
+  h->n_hout = h->n_eout = 0;
+  h->hout_total = h->eout_total = 0;
+  std::vector<DSlab> hv, ev;
+  for (auto &s : h->slabs) {
+    if (!s.out) continue;
+    const SlabArg &a = s.arg;
+    if (!a.e_side && !s.d_delta) CK(h, hipMalloc(&s.d_delta, s.len * sizeof(float)));
+    if ((a.e_side ? h->eout_total : h->hout_total) + s.len >= (1ull << 31))
+      return fail(h, EMS_E_UNSUPPORTED, "CPML slabs over 2^31 nodes");
+    DSlab d;
+    d.comp = a.comp, d.normal = a.normal, d.other = a.other, d.sign = a.sign;
+    d.lo0 = a.lo[0], d.hi0 = a.hi[0], d.lo1 = a.lo[1], d.hi1 = a.hi[1];
+    d.lo2 = a.lo[2], d.hi2 = a.hi[2];
+    d.ly = a.hi[1] - a.lo[1], d.lz = a.hi[2] - a.lo[2];
+    d.b = s.d_b, d.a = s.d_a, d.k = s.d_k;
+    d.psi0 = s.d_psi[0];
+    d.psi1 = a.e_side ? s.d_psi[0] : s.d_psi[1];
+    d.delta = s.d_delta;
+    uint32_t &tot = a.e_side ? h->eout_total : h->hout_total;
+    d.first = tot;
+    tot += (uint32_t)s.len;
+    (a.e_side ? ev : hv).push_back(d);
+  }
+  h->n_hout = (int)hv.size(), h->n_eout = (int)ev.size();
+  if (!hv.empty()) {
+    CK(h, hipMalloc(&h->d_hout, hv.size() * sizeof(DSlab)));
+    CK(h, hipMemcpy(h->d_hout, hv.data(), hv.size() * sizeof(DSlab), hipMemcpyHostToDevice));
+    std::vector<uint32_t> pm((size_t)h->n[0], 0u);
+    for (size_t q = 0; q < hv.size(); q++)
+      for (int i = hv[q].lo0; i < hv[q].hi0; i++) pm[i] |= 1u << q;
+    CK(h, hipMalloc(&h->d_pmask_o, pm.size() * sizeof(uint32_t)));
+    CK(h, hipMemcpy(h->d_pmask_o, pm.data(), pm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  if (!ev.empty()) {
+    CK(h, hipMalloc(&h->d_eout, ev.size() * sizeof(DSlab)));
+    CK(h, hipMemcpy(h->d_eout, ev.data(), ev.size() * sizeof(DSlab), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int upload_fslabs(ems_gpu *h, bool split) {
+  if (!h->slabs_dirty && h->split_state == (split ? 1 : 0)) return 0;
+  mark_out_slabs(h, split);
+  int rc = upload_out_slabs(h);
+  if (rc) return rc;
+  h->split_state = split ? 1 : 0;
   if (h->d_fslabs) CK(h, hipFree(h->d_fslabs));
   h->d_fslabs = nullptr;
   std::vector<FSlab> v;
   for (auto &s : h->slabs) {
-    if (s.len == 0) continue;
+    if (s.len == 0 || s.out) continue;
     FSlab f;
     f.e_side = s.arg.e_side, f.comp = s.arg.comp, f.normal = s.arg.normal, f.sign = s.arg.sign;
     f.lo0 = s.arg.lo[0], f.hi0 = s.arg.hi[0], f.lo1 = s.arg.lo[1], f.hi1 = s.arg.hi[1];
@@ -2347,8 +2714,23 @@ int upload_fslabs(ems_gpu *h) {
 
 int n_fslabs(const ems_gpu *h) {
   int n = 0;
-  for (auto &s : h->slabs) n += s.len != 0;
+  for (auto &s : h->slabs) n += s.len != 0 && !s.out;
   return n;
+}
+
+// The passes of the slabs corrected outside the fused kernel: before it
+// (increments from u^n in buffer par) and after it (on buffer 1 - par).
+int launch_cpml_out(ems_gpu *h, int par, bool after) {
+  if (!after && h->n_hout)
+    hipLaunchKernelGGL(cpml_delta_h, dim3((h->hout_total + 255) / 256), dim3(256), 0, h->stream,
+                       (const float *)h->fbuf[par], h->N, lanes_arg(h), (const DSlab *)h->d_hout,
+                       h->n_hout, h->hout_total, par, h->n[0], h->n[1], h->n[2]);
+  if (after && h->n_eout)
+    hipLaunchKernelGGL(cpml_post_e, dim3((h->eout_total + 255) / 256), dim3(256), 0, h->stream,
+                       h->fbuf[1 - par], h->N, lanes_arg(h), (const DSlab *)h->d_eout, h->n_eout,
+                       h->eout_total, h->n[1], h->n[2]);
+  CK(h, hipGetLastError());
+  return 0;
 }
 
 template <int TJ, bool PF>
@@ -2360,12 +2742,14 @@ void launch_fused_tj(ems_gpu *h, int par, const FusedGeom &g) {
   if (h->oe)
     hipLaunchKernelGGL((step_fused<true, TJ, PF>), dim3(g.nblocks), block, 0, h->stream, A, B, h->N,
                        (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
-                       (const uint32_t *)h->d_pmask, ns, par, h->n[0], h->n[1], h->n[2], g.tk,
+                       (const uint32_t *)h->d_pmask, ns, par, (const DSlab *)h->d_hout,
+                       (const uint32_t *)h->d_pmask_o, h->n_hout, h->n[0], h->n[1], h->n[2], g.tk,
                        g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group);
   else
     hipLaunchKernelGGL((step_fused<false, TJ, PF>), dim3(g.nblocks), block, 0, h->stream, A, B,
                        h->N, (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
-                       (const uint32_t *)h->d_pmask, ns, par, h->n[0], h->n[1], h->n[2], g.tk,
+                       (const uint32_t *)h->d_pmask, ns, par, (const DSlab *)h->d_hout,
+                       (const uint32_t *)h->d_pmask_o, h->n_hout, h->n[0], h->n[1], h->n[2], g.tk,
                        g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group);
 }
 
@@ -2468,11 +2852,60 @@ int launch_packed(ems_gpu *h, int par) {
   return 0;
 }
 
+// Band kernel: windows of 63 k-nodes per row, bands of TB rows, x chunks
+// sized for about 8 waves per SIMD unless chunks get very short.
+FusedGeom band_geom(const ems_gpu *h) {
+  FusedGeom g;
+  const int nz = h->n[2], ny = h->n[1], nx = h->n[0];
+  g.ktiles = (nz + 62) / 63;
+  g.tk = 63;
+  g.jtiles = (ny + h->tj - 1) / h->tj;
+  int want = h->nchunk;
+  if (want <= 0) {
+    const int per = g.ktiles * g.jtiles;
+    want = (8192 + per - 1) / per;
+    want = std::max(1, std::min(want, std::max(1, nx / 8)));
+  }
+  want = std::min(want, nx);
+  g.chunk = (nx + want - 1) / want;
+  g.nchunks = (nx + g.chunk - 1) / g.chunk;
+  g.ntiles = g.ktiles * g.jtiles * g.nchunks;
+  const int rows = g.jtiles * g.nchunks, G = h->xcd_group;
+  const int prow = G > 0 ? (rows + 8 * G - 1) / (8 * G) * (8 * G) : rows;
+  g.nblocks = prow * g.ktiles;
+  return g;
+}
+
+template <bool OE, int TB>
+void launch_band_t(ems_gpu *h, int par, const FusedGeom &g) {
+  hipLaunchKernelGGL((step_band<OE, TB>), dim3(g.nblocks), dim3(64), 0, h->stream, h->fbuf[par],
+                     h->fbuf[1 - par], h->N, (const float *)h->cpool, (const uint32_t *)h->croff,
+                     h->d_fslabs, (const uint32_t *)h->d_pmask, n_fslabs(h), par, h->n[0],
+                     h->n[1], h->n[2], g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group);
+}
+
+int launch_band(ems_gpu *h, int par) {
+  const FusedGeom g = band_geom(h);
+  switch (h->tj) {
+    case 4: h->oe ? launch_band_t<true, 4>(h, par, g) : launch_band_t<false, 4>(h, par, g); break;
+    case 8: h->oe ? launch_band_t<true, 8>(h, par, g) : launch_band_t<false, 8>(h, par, g); break;
+    case 16:
+      h->oe ? launch_band_t<true, 16>(h, par, g) : launch_band_t<false, 16>(h, par, g);
+      break;
+    default: return fail(h, EMS_E_INVALID, "band kernel: tile_rows must be 4, 8 or 16");
+  }
+  CK(h, hipGetLastError());
+  return 0;
+}
+
 int launch_fused(ems_gpu *h, int par) {
+  if (h->band) return launch_band(h, par);
   if (h->packed && packed_ok(h)) return launch_packed(h, par);
   if (h->rows && rows_ok(h)) return launch_rows(h, par);
   const FusedGeom g = fused_geom(h);
   const int key = h->tj * 2 + (h->prefetch ? 1 : 0);
+  int rc = launch_cpml_out(h, par, false);
+  if (rc) return rc;
   switch (key) {
     case 6: launch_fused_tj<3, false>(h, par, g); break;
     case 7: launch_fused_tj<3, true>(h, par, g); break;
@@ -2483,7 +2916,7 @@ int launch_fused(ems_gpu *h, int par) {
     default: return fail(h, EMS_E_INVALID, "unsupported tile height");
   }
   CK(h, hipGetLastError());
-  return 0;
+  return launch_cpml_out(h, par, true);
 }
 
 // A plane wave's hooks in a gap: the corrections, then its auxiliary grid
@@ -2719,7 +3152,13 @@ int build_pool(ems_gpu *h) {
   return 0;
 }
 
-int prepare(ems_gpu *h, uint64_t steps) {
+// Whether steps run the tiled fused kernel (the one that corrects the
+// z-normal CPML slabs outside itself).
+bool tiled_fused(const ems_gpu *h) {
+  return use_fused(h) && !h->band && !(h->packed && packed_ok(h)) && !(h->rows && rows_ok(h));
+}
+
+int prepare(ems_gpu *h, uint64_t steps, bool allow_split = true) {
   if (h->lanes_dirty) {
     drop_graphs(h);
     int rc = build_pool(h);
@@ -2769,9 +3208,10 @@ int prepare(ems_gpu *h, uint64_t steps) {
     }
   }
   if (grown) h->ops_dirty = true;
-  if (h->slabs_dirty) {
+  const bool split = allow_split && h->cpml_split && tiled_fused(h);
+  if (h->slabs_dirty || h->split_state != (split ? 1 : 0)) {
     drop_graphs(h);
-    int rc = upload_fslabs(h);
+    int rc = upload_fslabs(h, split);
     if (rc) return rc;
   }
   if (h->ops_dirty) {
@@ -3003,9 +3443,13 @@ void ems_gpu_destroy(ems_gpu *h) {
     (void)hipFree(s.d_b), (void)hipFree(s.d_a), (void)hipFree(s.d_k);
     (void)hipFree(s.d_psi[0]);
     if (s.d_psi[1]) (void)hipFree(s.d_psi[1]);
+    if (s.d_delta) (void)hipFree(s.d_delta);
   }
   if (h->d_fslabs) (void)hipFree(h->d_fslabs);
   if (h->d_pmask) (void)hipFree(h->d_pmask);
+  if (h->d_hout) (void)hipFree(h->d_hout);
+  if (h->d_eout) (void)hipFree(h->d_eout);
+  if (h->d_pmask_o) (void)hipFree(h->d_pmask_o);
   for (auto &o : h->ops) {
     (void)hipFree(o.d_axis), (void)hipFree(o.d_idx), (void)hipFree(o.d_wf);
     (void)hipFree(o.d_wd), (void)hipFree(o.d_amp), (void)hipFree(o.d_vals);
@@ -3703,16 +4147,32 @@ int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks) {
   // full-row fused kernel (nz <= 576, tile rows 3 or 7); 4: packed short
   // rows (nz <= 64; otherwise the tiled kernel), 5: packed without the
   // prefetch.
-  if (fused < 0 || fused > 5) return fail(h, EMS_E_INVALID, "schedule mode must be 0-5");
+  // 6: bands of rows per wave (tile rows 4, 8 or 16: the band height).
+  if (fused < 0 || fused > 6) return fail(h, EMS_E_INVALID, "schedule mode must be 0-6");
+  if (fused == 6) {
+    if (tile_rows != 4 && tile_rows != 8 && tile_rows != 16)
+      return fail(h, EMS_E_INVALID, "band kernel: tile_rows must be 4, 8 or 16");
+  } else if (tile_rows != 0 && tile_rows != 3 && tile_rows != 7 && tile_rows != 15) {
+    return fail(h, EMS_E_INVALID, "tile_rows must be 3, 7 or 15");
+  } else if (tile_rows == 0 && h->band) {
+    h->tj = 7;  // leaving the band kernel: back to a tile height
+  }
   h->prefetch = fused != 2 && fused != 5;
   h->rows = fused == 3;
   h->packed = fused == 4 || fused == 5;
-  if (tile_rows != 0 && tile_rows != 3 && tile_rows != 7 && tile_rows != 15)
-    return fail(h, EMS_E_INVALID, "tile_rows must be 3, 7 or 15");
+  h->band = fused == 6;
   if (x_chunks < 0) return fail(h, EMS_E_INVALID, "x_chunks must be >= 0");
   h->fused = fused != 0;
   if (tile_rows) h->tj = tile_rows;
   h->nchunk = x_chunks;
+  drop_graphs(h);
+  return EMS_OK;
+}
+
+int ems_gpu_set_cpml_split(ems_gpu *h, int on) {
+  ALIVE(h);
+  if (on != 0 && on != 1) return fail(h, EMS_E_INVALID, "cpml split must be 0 or 1");
+  h->cpml_split = on != 0;
   drop_graphs(h);
   return EMS_OK;
 }
@@ -3842,6 +4302,16 @@ int ems_gpu_energy(ems_gpu *h, double *w) {
   if ((rc = ops_energy(h, &w_ops))) return rc;
   *w = wf + w_ops;
   return EMS_OK;
+}
+
+int ems_gpu_energy_in(ems_gpu *h, int measure, double *w) {
+  ALIVE(h);
+  if (!w || measure < 0 || measure > 1) return EMS_E_INVALID;
+  if (measure == 1)
+    return fail(h, EMS_E_UNSUPPORTED,
+                "energy_in: openEMS's measure (EnergyMeasure::OpenEms) is not held by the "
+                "device engine; the end criterion of the openEMS-mode runs needs the CPU engine");
+  return ems_gpu_energy(h, w);
 }
 
 int ems_gpu_energy_many(ems_gpu *const *models, int n, const uint8_t *active, double *out) {
@@ -4254,7 +4724,11 @@ int ems_gpu_batch_create(ems_gpu *const *models, int n, ems_gpu_batch **out) {
   *out = nullptr;
   ems_gpu_batch *b = new ems_gpu_batch();
   for (int q = 0; q < n; q++) {
-    if (!models[q] || models[q]->dead) {
+    // (a handle listed twice would be stepped twice per launch, its CPML
+    // psi updated in place by two blocks)
+    bool dup = false;
+    for (int r = 0; r < q && !dup; r++) dup = models[r] == models[q];
+    if (!models[q] || models[q]->dead || dup) {
       delete b;
       return EMS_E_INVALID;
     }
@@ -4318,7 +4792,7 @@ int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
   for (int q : act) {
     ems_gpu *h = b->m[q];
     if (h->dead) return bfail(b, EMS_E_DEVICE, "model " + std::to_string(q) + ": " + h->err);
-    h->fused = true, h->rows = false, h->prefetch = b->prefetch, h->tj = b->tj;
+    h->fused = true, h->rows = false, h->band = false, h->prefetch = b->prefetch, h->tj = b->tj;
     h->packed = b->packed;
     h->nchunk = b->nchunk;
     if (!use_fused(h))
@@ -4327,7 +4801,7 @@ int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
                        " needs the unfused schedule (H source, magnetic poles, plane wave, "
                        "cut-face currents or more than 32 CPML slabs)");
     if (h->par != par) return bfail(b, EMS_E_INVALID, "models of a batch step in lockstep");
-    const int rc = prepare(h, n);
+    const int rc = prepare(h, n, false);
     if (rc) return bfail(b, rc, h->err);
     BCK(b, hipStreamSynchronize(h->stream));
   }

@@ -1020,6 +1020,12 @@ class Engine:
         mode = int(fused) if not isinstance(fused, bool) else (1 if fused else 0)
         self._check(nat.ems_gpu_set_schedule(self._h, mode, int(tile_rows), int(x_chunks)))
 
+    def set_cpml_split(self, on: bool) -> None:
+        """Whether the tiled fused step corrects its z-normal CPML slabs in
+        passes of their own (before the kernel for H, after it for E) rather
+        than inside it; the same values either way."""
+        self._check(nat.ems_gpu_set_cpml_split(self._h, 1 if on else 0))
+
     def set_xcd_group(self, group: int) -> None:
         """Tile rows per XCD visit in the fused kernels' block order (0:
         plain order)."""
@@ -1084,7 +1090,8 @@ class EngineBatch:
         h = C.c_void_p()
         rc = nat.ems_gpu_batch_create(arr, len(self.engines), C.byref(h))
         if rc != 0:
-            raise EmsError(rc, "ems_gpu_batch_create failed (mixed curl orders or a dead engine?)")
+            raise EmsError(rc, "ems_gpu_batch_create failed (mixed curl orders, a dead engine "
+                                "or an engine listed twice)")
         self._h = h
         self._arr = arr
 

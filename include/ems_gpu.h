@@ -297,7 +297,12 @@ int ems_gpu_merged_state(ems_gpu *h, int id, double *cur, double *circ);
  * captured in graphs on the batch's own stream. Every model computes what
  * its own engine would, bit for bit. The models must all run the fused
  * schedule, with one curl order, and be in step with each other (same
- * step parity). The batch holds the handles; it does not own them. */
+ * step parity); a handle listed twice is refused. The batch holds the
+ * handles; it does not own them. ems_run_until_batch with an odd check
+ * interval can stop models at checks an odd number of steps apart: a later
+ * ems_gpu_batch_run over all of them is then refused with EMS_E_INVALID
+ * ("models of a batch step in lockstep"); run such models in separate
+ * batches, or step the odd ones once on their own. */
 typedef struct ems_gpu_batch ems_gpu_batch;
 int ems_gpu_batch_create(ems_gpu *const *models, int n, ems_gpu_batch **out);
 void ems_gpu_batch_destroy(ems_gpu_batch *b);
@@ -318,6 +323,12 @@ int ems_gpu_run(ems_gpu *h, uint64_t n);
 int ems_gpu_step_with_energy(ems_gpu *h, double *w);
 /* Engine::energy (src/engine/mod.rs:722-726): read-only W^n. */
 int ems_gpu_energy(ems_gpu *h, double *w);
+/* Engine::energy_in (src/engine/mod.rs:747-752) and Stepper::energy_now
+ * (src/engine/run.rs:296-299): measure 0 EnergyMeasure::Invariant (=
+ * ems_gpu_energy); 1 EnergyMeasure::OpenEms is refused with
+ * EMS_E_UNSUPPORTED (the engine keeps only the invariant). */
+int ems_gpu_energy_in(ems_gpu *h, int measure, double *w);
+
 /* No counterpart in the reference: ems_gpu_energy for callers that cannot
  * allocate (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc). Same value, bit for bit, but it calls no
@@ -361,6 +372,13 @@ double ems_gpu_dt(const ems_gpu *h);
  * x_chunks 0 = automatic. Steps that need the serial gap between the H and
  * E updates (H sources) always run unfused. */
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks);
+
+/* CPML slabs of the tiled fused step (no reference counterpart): with
+ * on = 1 (the default) the slabs normal to z that follow every other slab
+ * of their side in registration order are corrected outside the fused
+ * kernel, H-side by increments formed from u^n before it, E-side after it
+ * (the same values, bit for bit); 0 keeps every slab inside. */
+int ems_gpu_set_cpml_split(ems_gpu *h, int on);
 
 /* Tile order of the fused kernels (no reference counterpart): groups of
  * `group` consecutive tile rows go round-robin to the 8 XCDs (their L2s),

@@ -31,6 +31,7 @@ def test_invalid_arguments_are_rejected_without_a_device():
     assert nat.ems_gpu_create(C.byref(d), C.byref(h)) == nat.EMS_E_INVALID
     assert nat.ems_gpu_run(None, 1) == nat.EMS_E_INVALID
     assert nat.ems_gpu_energy_noalloc(None, None) == nat.EMS_E_INVALID
+    assert nat.ems_gpu_energy_in(None, 0, None) == nat.EMS_E_INVALID
     dil = (C.c_int32 * 2)(1, 2)
     assert nat.ems_cn_create(0, 2, 8, 2, dil, 0, C.byref(h)) == nat.EMS_E_INVALID
     assert nat.ems_cn_create(4, 2, 8, 1, dil, 0, C.byref(h)) == nat.EMS_E_INVALID

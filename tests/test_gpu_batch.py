@@ -25,7 +25,10 @@ def engine(m):
 
 
 def limits():
-    return E.RunLimits(6000, E.EndCriteria.energy_decay_db(-40.0),
+    # -80 dB checked every 20 steps: these four models stop at three
+    # different checks (3781, 3801 and 4501 steps on the oracle), so the
+    # staggered stops of the batch run loop are exercised.
+    return E.RunLimits(6000, E.EndCriteria(1e-8, check_interval=20),
                        excitation_end=E.Waveform.gaussian(2e9, 1e9).duration())
 
 
@@ -50,8 +53,8 @@ def test_batch_run_until_matches_single_runs(tile_rows):
         assert np.array_equal(bits(p.voltage()), bits(q.voltage()))
         assert np.array_equal(bits(p.current()), bits(q.current()))
         same_fields(e, f)
-    assert len({o.steps for o in bouts}) >= 1 and all(o.reason == E.StopReason.EnergyDecay
-                                                      for o in bouts)
+    assert len({o.steps for o in bouts}) > 1 and all(o.reason == E.StopReason.EnergyDecay
+                                                     for o in bouts)
     b.close()
 
 
@@ -76,4 +79,26 @@ def test_batch_subset_runs(packed):
         if act[q]:
             e.run(25)
         same_fields(e, f)
+    b.close()
+
+
+def test_batch_refuses_models_out_of_step_and_duplicates():
+    """Models of a batch step in lockstep: two models whose step parities
+    differ (one has taken an odd number of steps more, as after a run loop
+    with an odd check interval that stopped them at different checks) are
+    refused, and so is one engine listed twice."""
+    ms = models()[:2]
+    (a, _), (b, _) = engine(ms[0]), engine(ms[1])
+    a.run(1)
+    batch = E.EngineBatch([a, b])
+    with pytest.raises(E.EmsError, match="lockstep"):
+        batch.run(2)
+    batch.close()
+    b.run(1)  # back in step: the same batch object works
+    batch = E.EngineBatch([a, b])
+    batch.run(2)
+    batch.close()
+    with pytest.raises(E.EmsError):
+        E.EngineBatch([a, a])
+    a.close()
     b.close()

@@ -43,7 +43,8 @@ def same_fields(g, o, label):
                                      f"({bad.shape[0]} entries)")
 
 
-VACUUM_SCHEDULES = [(0, 0, 0), (1, 7, 0), (2, 7, 16), (3, 7, 0), (4, 7, 0), (5, 7, 4)]
+VACUUM_SCHEDULES = [(0, 0, 0), (1, 7, 0), (2, 7, 16), (3, 7, 0), (4, 7, 0), (5, 7, 4), (6, 8, 0),
+                    (6, 16, 7)]
 
 
 @pytest.fixture(scope="module")
@@ -96,13 +97,16 @@ def cpml_box_512():
     return wl, seeded_fields(wl.grid.nodes(), 512)
 
 
-@pytest.mark.parametrize("sched", [None, (0, 0, 0)], ids=["default", "unfused"])
+@pytest.mark.parametrize("sched", [None, (0, 0, 0), (6, 16, 0), "inline"],
+                         ids=["default", "unfused", "band16", "cpml_inline"])
 def test_cpml_box_512_full_size(cpml_box_512, sched):
     from oracle.bridge import OracleEngine
     wl, init = cpml_box_512
     steps = 24
     e = E.Engine(wl.grid, wl.dt, wl.lanes)
-    if sched is not None:
+    if sched == "inline":
+        e.set_cpml_split(False)  # every CPML slab inside the tiled kernel
+    elif sched is not None:
         e.set_schedule(*sched)
     pe = wl.build_ops(e)
     for field, a, v in init:

@@ -10,7 +10,7 @@ import openems_case as OC
 
 pytestmark = pytest.mark.gpu
 
-SCHEDULES = [(0, 0, 0), (1, 3, 0), (2, 7, 4), (3, 7, 0), (4, 7, 0), (5, 7, 2)]
+SCHEDULES = [(0, 0, 0), (1, 3, 0), (2, 7, 4), (3, 7, 0), (4, 7, 0), (5, 7, 2), (6, 4, 2), (6, 8, 0)]
 
 
 def bits(a):

@@ -16,9 +16,10 @@ pytestmark = pytest.mark.gpu
 # Every parity test runs on the unfused schedule and on the fused one with
 # each tile height and with many x chunks (chunk seams inside the box), the
 # full-row and the packed-row kernels (packed: grids with nz <= 64; the
-# others run the tiled kernel).
+# others run the tiled kernel) and the band kernel (bands of 4, 8, 16 rows).
 SCHEDULES = [(0, 0, 0), (1, 3, 0), (1, 7, 0), (1, 15, 0), (1, 7, 5), (2, 7, 3), (2, 15, 0),
-             (3, 3, 0), (3, 7, 0), (3, 7, 4), (4, 7, 0), (4, 7, 3), (5, 7, 0)]
+             (3, 3, 0), (3, 7, 0), (3, 7, 4), (4, 7, 0), (4, 7, 3), (5, 7, 0),
+             (6, 4, 0), (6, 8, 3), (6, 16, 0), (6, 16, 5)]
 
 
 @pytest.fixture(autouse=True, params=SCHEDULES, ids=lambda s: "mode%d_tj%d_xc%d" % s)
