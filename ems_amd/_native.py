@@ -11,7 +11,13 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libems_gpu.so")
+# EMS_AMD_FP_CONTRACT=fast selects the build whose step kernels use fused
+# multiply-adds (results within rounding of the reference engine instead of
+# bit-identical).
+FP_CONTRACT = os.environ.get("EMS_AMD_FP_CONTRACT", "off")
+if FP_CONTRACT not in ("off", "fast"):
+    raise ImportError(f"EMS_AMD_FP_CONTRACT must be 'off' or 'fast', not {FP_CONTRACT!r}")
+LIB_PATH = os.path.join(_HERE, "libems_gpu.so" if FP_CONTRACT == "off" else "libems_gpu_fma.so")
 
 EMS_OK = 0
 EMS_E_INVALID = -1
