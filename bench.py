@@ -91,7 +91,8 @@ def cpu_baseline(wl, seconds: float, log) -> dict:
 
 def sweep(args, log) -> int:
     """The patch-antenna sweep: every model solved to -40 dB energy decay
-    and analysed (S11 at 201 points over 1-3 GHz), first one after the other
+    and analysed (S11 at 201 points over 1-3 GHz, the port spectra from
+    running DFTs on the device), first one after the other
     on the GPU (each its own engine), then all together in one batch.
     value = batched S11 solves per hour; both include set-up and analysis."""
     import ems_amd as E
@@ -119,7 +120,7 @@ def sweep(args, log) -> int:
         tb = time.perf_counter()
         o = e.run_until(lim)
         tc = time.perf_counter()
-        s11 = p.analyze(freqs).s11()
+        s11 = p.analyze_device(freqs).s11()
         e.close()
         phase["setup"] += tb - ta
         phase["run"] += tc - tb
@@ -148,10 +149,11 @@ def sweep(args, log) -> int:
     t1 = time.perf_counter()
     outs = b.run_until(lim)
     t2 = time.perf_counter()
-    # the ports' transforms are native and release the interpreter lock
-    from concurrent.futures import ThreadPoolExecutor
-    with ThreadPoolExecutor(max_workers=8) as ex:
-        s11b = list(ex.map(lambda p: p.analyze(freqs).s11(), ports))
+    # every port's V and I spectra from the device's running DFTs (the
+    # series stay on the device)
+    for p in ports:
+        p.update_device(freqs)
+    s11b = [p.analyze_device(freqs).s11() for p in ports]
     t_bat = time.perf_counter() - t0
     bat_phase = {"setup": t1 - t0, "run": t2 - t1, "analysis": time.perf_counter() - t2}
     cell_steps = sum(m.cells * o.steps for m, o in zip(models, outs))
@@ -218,7 +220,9 @@ def earlystop(args, log) -> int:
     b = E.EngineBatch(engines)
     L._schedule(b, models, args.tile_rows or 7, args.x_chunks)
     outs = b.run_until(lim)
-    full = [p.analyze(freqs).s11() for p in ports]
+    for p in ports:
+        p.update_device(freqs)
+    full = [p.analyze_device(freqs).s11() for p in ports]
     b.close()
     for e in engines:
         e.close()

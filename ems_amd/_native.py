@@ -115,6 +115,9 @@ ems_gpu_lumped_table = _sig("ems_gpu_lumped_table", _i, _p, _i, _i, _dp, _u64)
 ems_gpu_lumped_values = _sig("ems_gpu_lumped_values", _i, _p, _i, _i, _dp, _u64, C.POINTER(_u64),
                              C.POINTER(_u64))
 ems_gpu_lumped_branch = _sig("ems_gpu_lumped_branch", _i, _p, _i, _dp, _dp, _dp)
+ems_gpu_add_dft = _sig("ems_gpu_add_dft", _i, _p, _i, _i, _dp, _i, _d, _d, C.POINTER(_i))
+ems_gpu_dft_update = _sig("ems_gpu_dft_update", _i, _p, _i)
+ems_gpu_dft_result = _sig("ems_gpu_dft_result", _i, _p, _i, _dp, C.POINTER(_u64))
 ems_gpu_add_dispersive = _sig("ems_gpu_add_dispersive", _i, _p, _p, _p, _p, C.POINTER(_i))
 ems_gpu_dispersive_state = _sig("ems_gpu_dispersive_state", _i, _p, _i, _i, _dp, _dp)
 ems_gpu_add_planewave = _sig("ems_gpu_add_planewave", _i, _p, _p, C.POINTER(_i))
@@ -182,6 +185,8 @@ ems_openems_source_table = _sig("ems_openems_source_table", _i, _d, _d, _d, _i, 
 ems_fold_parallel = _sig("ems_fold_parallel", _i, C.c_float, C.c_float, _d, _d, _d,
                          C.POINTER(C.c_float), C.POINTER(C.c_float))
 ems_dft = _sig("ems_dft", None, _d, _d, _dp, _u64, _dp, _i, _dp)
+ems_dft_phasors = _sig("ems_dft_phasors", None, _dp, _i, _d, _d, _u64, _dp)
+ems_dft_steps = _sig("ems_dft_steps", None, _dp, _i, _d, _dp)
 ems_conformal_stability = _sig("ems_conformal_stability", _d, _p)
 ems_run_until = _sig("ems_run_until", _i, _p, C.POINTER(RunLimitsC), _ip, _i,
                      C.POINTER(RunOutcomeC))

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/ems_gpu.h"
+#include "../../include/ems_host.h"
 
 // Scratch of ems_gpu_energy_noalloc: one static device array per GPU the
 // library runs on, held until the process ends (16 MiB). Host code never
@@ -2147,6 +2148,34 @@ __device__ __forceinline__ const float *pick3(float *const *a, int c) {
   return c == 0 ? a[0] : (c == 1 ? a[1] : a[2]);
 }
 
+// RunningDft::push (post/dft.rs:125-136) over samples [n0, n1) of one
+// recorded series, a thread per frequency: acc += phasor * x, phasor *=
+// step, and at every 1024th sample the phasor re-anchored from the exact
+// phase (the anchors are the host's, ems_dft_phasors). State per
+// frequency: acc (re, im), phasor (re, im). Same operations in the same
+// order as the host's ems_dft, so the same bits.
+__global__ void __launch_bounds__(256) dft_run(const double *__restrict__ x, uint64_t n0,
+                                               uint64_t n1, int nf, double *__restrict__ st,
+                                               const double *__restrict__ stepv,
+                                               const double *__restrict__ anchors) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= nf) return;
+  double ar = st[4 * f], ai = st[4 * f + 1], pr = st[4 * f + 2], pi = st[4 * f + 3];
+  const double sr = stepv[2 * f], si = stepv[2 * f + 1];
+  for (uint64_t s = n0; s < n1; s++) {
+    const double xv = ((const GAS double *)x)[s];
+    ar = ar + pr * xv;
+    ai = ai + pi * xv;
+    const double nr = pr * sr - pi * si, ni = pr * si + pi * sr;
+    pr = nr, pi = ni;
+    if ((s + 1) % 1024 == 0) {
+      const double *a = anchors + ((s + 1) / 1024 - 1) * 2 * (uint64_t)nf;
+      pr = a[2 * f], pi = a[2 * f + 1];
+    }
+  }
+  st[4 * f] = ar, st[4 * f + 1] = ai, st[4 * f + 2] = pr, st[4 * f + 3] = pi;
+}
+
 // H rows, a wave per row (rows per plane: x rows ny-1, y rows ny, z rows
 // ny-1, enumerated in that order): four accumulators over k = 0, 1, 2, 3
 // mod 4 in increasing k (h_energy_row, kernel.rs:119-216: the weighted
@@ -2363,6 +2392,17 @@ struct HostOp {
 };
 
 // Host copies of a reduced-current operator's tables (its energy).
+// A running DFT of a lumped element's record series on the device.
+struct HostDft {
+  int op = -1, which = 0, nf = 0;
+  double t0 = 0.0, dt = 0.0;
+  uint64_t done = 0;     // samples accumulated
+  uint64_t anchors = 0;  // anchors uploaded (sample 1024 m for m = 1..anchors)
+  uint64_t anchor_cap = 0;
+  std::vector<double> freqs;
+  double *d_state = nullptr, *d_step = nullptr, *d_anchor = nullptr;
+};
+
 struct MergedHost {
   std::vector<double> da, db, l, ev, bv, mval;
   std::vector<uint32_t> bs, mem, ef, brs, bc, mrs, mcol;
@@ -2403,6 +2443,7 @@ struct ems_gpu {
   bool cpml_split = true;  // allowed (ems_gpu_set_cpml_split)
   int split_state = -1;    // what the uploaded tables hold (-1: nothing yet)
   std::vector<HostOp> ops;
+  std::vector<HostDft> dfts;
   DevOp *d_ops = nullptr;
   bool ops_dirty = true, slabs_dirty = true;
   uint64_t step = 0;
@@ -3447,6 +3488,11 @@ void ems_gpu_destroy(ems_gpu *h) {
   }
   if (h->d_fslabs) (void)hipFree(h->d_fslabs);
   if (h->d_pmask) (void)hipFree(h->d_pmask);
+  for (auto &d : h->dfts) {
+    if (d.d_state) (void)hipFree(d.d_state);
+    if (d.d_step) (void)hipFree(d.d_step);
+    if (d.d_anchor) (void)hipFree(d.d_anchor);
+  }
   if (h->d_hout) (void)hipFree(h->d_hout);
   if (h->d_eout) (void)hipFree(h->d_eout);
   if (h->d_pmask_o) (void)hipFree(h->d_pmask_o);
@@ -3734,6 +3780,86 @@ int ems_gpu_lumped_values(ems_gpu *h, int id, int which, double *out, uint64_t c
   if (n && out) CK(h, hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost));
   if (count) *count = o->count;
   if (first_step) *first_step = o->first;
+  return EMS_OK;
+}
+
+int ems_gpu_add_dft(ems_gpu *h, int lumped_id, int which, const double *freqs, int nf, double t0,
+                    double dt, int *dft_id) {
+  ALIVE(h);
+  HostOp *o = lumped_op(h, lumped_id);
+  if (!o) return fail(h, EMS_E_INVALID, "not a lumped element id");
+  if (which < 0 || which > 2 || (which && !o->lp.columns))
+    return fail(h, EMS_E_INVALID, "dft series: 0 current, 1/2 port V/I (ports only)");
+  if (!o->lp.record) return fail(h, EMS_E_INVALID, "dft: the element records no series");
+  if (nf < 1 || !freqs || !(dt > 0.0) || !std::isfinite(t0)) return fail(h, EMS_E_INVALID, "dft args");
+  HostDft d;
+  d.op = lumped_id, d.which = which, d.nf = nf, d.t0 = t0, d.dt = dt;
+  d.freqs.assign(freqs, freqs + nf);
+  std::vector<double> st(4 * (size_t)nf, 0.0), step(2 * (size_t)nf), ph(2 * (size_t)nf);
+  // RunningDft::new: acc 0, phasor anchored at n = 0 (dft.rs:90-108)
+  ems_dft_phasors(freqs, nf, t0, dt, 0, ph.data());
+  ems_dft_steps(freqs, nf, dt, step.data());
+  for (int k = 0; k < nf; k++) st[4 * k + 2] = ph[2 * k], st[4 * k + 3] = ph[2 * k + 1];
+  int rc;
+  if ((rc = upload(h, (void **)&d.d_state, st.data(), st.size() * sizeof(double)))) return rc;
+  if ((rc = upload(h, (void **)&d.d_step, step.data(), step.size() * sizeof(double)))) return rc;
+  h->dfts.push_back(std::move(d));
+  if (dft_id) *dft_id = (int)h->dfts.size() - 1;
+  return EMS_OK;
+}
+
+int ems_gpu_dft_update(ems_gpu *h, int dft_id) {
+  ALIVE(h);
+  if (dft_id < 0 || dft_id >= (int)h->dfts.size()) return fail(h, EMS_E_INVALID, "not a dft id");
+  HostDft &d = h->dfts[dft_id];
+  HostOp *o = lumped_op(h, d.op);
+  const uint64_t n = o->count;
+  if (n <= d.done) return EMS_OK;
+  // the anchors of samples 1024 m <= n, from the host's routine
+  const uint64_t need = n / 1024;
+  if (need > d.anchors) {
+    const size_t per = 2 * (size_t)d.nf;
+    if (need > d.anchor_cap) {
+      uint64_t cap = d.anchor_cap ? 2 * d.anchor_cap : 16;
+      while (cap < need) cap *= 2;
+      double *na = nullptr;
+      CK(h, hipMalloc(&na, cap * per * sizeof(double)));
+      if (d.d_anchor) {
+        CK(h, hipMemcpyAsync(na, d.d_anchor, d.anchors * per * sizeof(double),
+                             hipMemcpyDeviceToDevice, h->stream));
+        CK(h, hipStreamSynchronize(h->stream));
+        CK(h, hipFree(d.d_anchor));
+      }
+      d.d_anchor = na, d.anchor_cap = cap;
+    }
+    std::vector<double> a((need - d.anchors) * per);
+    for (uint64_t m = d.anchors + 1; m <= need; m++)
+      ems_dft_phasors(d.freqs.data(), d.nf, d.t0, d.dt, 1024 * m,
+                      a.data() + (m - d.anchors - 1) * per);
+    CK(h, hipMemcpyAsync(d.d_anchor + d.anchors * per, a.data(), a.size() * sizeof(double),
+                         hipMemcpyHostToDevice, h->stream));
+    CK(h, hipStreamSynchronize(h->stream));  // (a is a host temporary)
+    d.anchors = need;
+  }
+  const double *src = d.which == 0 ? o->lp.rec : d.which == 1 ? o->lp.pv : o->lp.pi;
+  hipLaunchKernelGGL(dft_run, dim3((d.nf + 255) / 256), dim3(256), 0, h->stream, src, d.done, n,
+                     d.nf, d.d_state, (const double *)d.d_step, (const double *)d.d_anchor);
+  CK(h, hipGetLastError());
+  d.done = n;
+  return EMS_OK;
+}
+
+int ems_gpu_dft_result(ems_gpu *h, int dft_id, double *out, uint64_t *samples) {
+  ALIVE(h);
+  if (dft_id < 0 || dft_id >= (int)h->dfts.size() || !out)
+    return fail(h, EMS_E_INVALID, "not a dft id");
+  const HostDft &d = h->dfts[dft_id];
+  std::vector<double> st(4 * (size_t)d.nf);
+  CK(h, hipStreamSynchronize(h->stream));
+  CK(h, hipMemcpy(st.data(), d.d_state, st.size() * sizeof(double), hipMemcpyDeviceToHost));
+  // RunningDft::result (dft.rs:170-178): acc * dt
+  for (int k = 0; k < d.nf; k++) out[2 * k] = st[4 * k] * d.dt, out[2 * k + 1] = st[4 * k + 1] * d.dt;
+  if (samples) *samples = d.done;
   return EMS_OK;
 }
 

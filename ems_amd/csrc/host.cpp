@@ -543,6 +543,23 @@ void ems_dft(double t0, double dt, const double *x, uint64_t n, const double *fr
   }
 }
 
+// RunningDft's phasors at sample n (anchor, dft.rs:116-122): e^{-j2 pi f t_n}
+// from the exact phase, per frequency, as (re, im) pairs.
+void ems_dft_phasors(const double *freqs, int nf, double t0, double dt, uint64_t n, double *out) {
+  for (int k = 0; k < nf; k++) {
+    const Cx p = cis(-TAU * phase_cycles(freqs[k], t0, dt, (double)n));
+    out[2 * k] = p.re, out[2 * k + 1] = p.im;
+  }
+}
+
+// RunningDft's per-sample rotation (dft.rs:93-97): e^{-j2 pi f dt}.
+void ems_dft_steps(const double *freqs, int nf, double dt, double *out) {
+  for (int k = 0; k < nf; k++) {
+    const Cx p = cis(-TAU * freqs[k] * dt);
+    out[2 * k] = p.re, out[2 * k + 1] = p.im;
+  }
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------- run loop --

@@ -908,6 +908,22 @@ class Engine:
                                               out.size, C.byref(cnt), C.byref(first)))
         return out, first.value
 
+    def _add_dft(self, oid: int, which: int, freqs, t0: float, dt: float) -> int:
+        f = np.ascontiguousarray(np.asarray(freqs, dtype=np.float64))
+        did = C.c_int()
+        self._check(nat.ems_gpu_add_dft(self._h, oid, which, nat.ptr(f, C.c_double), f.size,
+                                        float(t0), float(dt), C.byref(did)))
+        return did.value
+
+    def _dft_update(self, did: int) -> None:
+        self._check(nat.ems_gpu_dft_update(self._h, did))
+
+    def _dft_result(self, did: int, nf: int):
+        out = np.empty(2 * nf, dtype=np.float64)
+        n = C.c_uint64()
+        self._check(nat.ems_gpu_dft_result(self._h, did, nat.ptr(out, C.c_double), C.byref(n)))
+        return [complex(out[2 * k], out[2 * k + 1]) for k in range(nf)], n.value
+
     def _planewave_aux(self, oid: int, naux: int):
         e, hh = np.zeros(naux), np.zeros(naux - 1)
         self._check(nat.ems_gpu_planewave_aux(self._h, oid, nat.ptr(e, C.c_double),

@@ -472,7 +472,7 @@ def early_stop_sweep(es: EarlyStop, models, limits, freqs=FREQS_1_3_GHZ, tile_ro
         _schedule(b, [models[q] for q in redo], tile_rows)
         outs = b.run_until(limits)
         for k, q in enumerate(redo):
-            s11[q] = ports[k].analyze(freqs).s11()
+            s11[q] = ports[k].analyze_device(freqs).s11()
             steps[q] = fmt.observed + outs[k].steps
         b.close()
         for e in engines:

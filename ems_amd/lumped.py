@@ -421,6 +421,7 @@ class LumpedPort:
         self.element = LumpedElement.resistor(region, r).record_current()
         self.resistance = float(r)
         self.dt = 0.0
+        self._dfts = {}  # frequencies -> (V, I) running DFT ids on the device
 
     @staticmethod
     def new(region: LumpedBox, r: float) -> "LumpedPort":
@@ -485,6 +486,39 @@ class LumpedPort:
         vf = post.dft(self.voltage_t0(), self.dt, v, freqs)
         i_f = post.dft(self.current_t0(), self.dt, self.current(), freqs)
         return post.PortResult(freqs, vf, i_f, z_ref)
+
+    def track(self, freqs) -> None:
+        """Starts running DFTs of the port voltage and current on the device
+        at `freqs` (ems_gpu_add_dft); `analyze_device` then reads them."""
+        eng, oid = self._installed
+        key = tuple(float(f) for f in freqs)
+        if key not in self._dfts:
+            self._dfts[key] = (eng._add_dft(oid, 1, key, self.voltage_t0(), self.dt),
+                               eng._add_dft(oid, 2, key, self.current_t0(), self.dt))
+
+    def update_device(self, freqs) -> None:
+        """Queues the device DFTs' accumulation of the samples recorded so
+        far (no wait): queue every port's first, then read them."""
+        eng, _ = self._installed
+        self.track(freqs)
+        for did in self._dfts[tuple(float(f) for f in freqs)]:
+            eng._dft_update(did)
+
+    def analyze_device(self, freqs, z_ref=None) -> post.PortResult:
+        """analyze_with from the device's running DFTs of the port V and I:
+        the same spectra as the host's transform of the recorded series, bit
+        for bit, without copying the series back."""
+        eng, _ = self._installed
+        self.track(freqs)
+        vid, iid = self._dfts[tuple(float(f) for f in freqs)]
+        eng._dft_update(vid)
+        eng._dft_update(iid)
+        vf, nv = eng._dft_result(vid, len(freqs))
+        i_f, ni = eng._dft_result(iid, len(freqs))
+        if nv == 0:
+            raise EmsError(nat.EMS_E_INVALID, "port has no samples yet")
+        zr = complex(self.resistance, 0.0) if z_ref is None else z_ref
+        return post.PortResult(list(freqs), vf, i_f, zr)
 
 
 class LumpedDesc(C.Structure):

@@ -146,6 +146,24 @@ int ems_gpu_lumped_table(ems_gpu *h, int id, int term, const double *values, uin
 int ems_gpu_lumped_values(ems_gpu *h, int id, int which, double *out, uint64_t cap,
                           uint64_t *count, uint64_t *first_step);
 
+/* A running DFT on the device (RunningDft, src/post/dft.rs:75-190) of a
+ * lumped element's record series `which` (as ems_gpu_lumped_values) at nf
+ * frequencies, with sample n at t0 + n dt counted from the element's first
+ * record: X(f) = sum x(t_n) e^{-j 2 pi f t_n} dt, the phasor advanced by
+ * one complex product per sample and re-anchored from the exact phase
+ * every 1024 samples (the anchors are ems_dft_phasors'), so the result is
+ * ems_dft's on the same series, bit for bit, and the series never leaves
+ * the device. ems_gpu_dft_update queues the accumulation of the samples
+ * recorded since the last update (any number of updates give the same
+ * result); ems_gpu_dft_result waits for it and writes 2*nf doubles (Re,
+ * Im per frequency) and the samples accumulated. No reference counterpart
+ * for the device placement: the reference runs LumpedPort::analyze_with
+ * (src/port/mod.rs:501-544) on the host over the recorded series. */
+int ems_gpu_add_dft(ems_gpu *h, int lumped_id, int which, const double *freqs, int nf, double t0,
+                    double dt, int *dft_id);
+int ems_gpu_dft_update(ems_gpu *h, int dft_id);
+int ems_gpu_dft_result(ems_gpu *h, int dft_id, double *out, uint64_t *samples);
+
 /* Branch state j^n, q^n and the last j-bar per edge (branch_energy,
  * branch_current, lumped.rs:1033-1087). */
 int ems_gpu_lumped_branch(ems_gpu *h, int id, double *j, double *q, double *j_avg);

@@ -119,6 +119,11 @@ int ems_fold_parallel(float ca, float cb, double dt, double g, double c, float *
  * X(freqs[k]). */
 void ems_dft(double t0, double dt, const double *x, uint64_t n, const double *freqs, int nf,
              double *out);
+/* RunningDft's anchor phasors e^{-j 2 pi f_k t_n} at sample n from the
+ * exact phase (src/post/dft.rs:116-122, phase_cycles :204-212), and its
+ * per-sample rotations e^{-j 2 pi f_k dt} (:93-97); (re, im) per frequency. */
+void ems_dft_phasors(const double *freqs, int nf, double t0, double dt, uint64_t n, double *out);
+void ems_dft_steps(const double *freqs, int nf, double dt, double *out);
 
 /* The conformal PEC's stability estimate (stability, src/engine/
  * conformal.rs:2858-2925; apply_s :2927-2995; faces :3012-3055): s =
