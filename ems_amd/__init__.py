@@ -17,5 +17,6 @@ from .dispersive import Dispersive, Field, Pole, Scheme  # noqa: F401,E402
 from .planewave import Medium, PlaneWave, PlaneWaveSpec  # noqa: F401,E402
 from .post import PortResult, dft, s_param  # noqa: F401,E402
 from .learn import ConvNet, EarlyStop, TraceFormat  # noqa: F401,E402
+from .mur import MurBoundary  # noqa: F401,E402
 from .conformal import (  # noqa: F401,E402
     ConformalEdge, ConformalFace, ConformalOptions, ConformalPec, ConformalReport)

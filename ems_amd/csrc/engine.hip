@@ -1386,7 +1386,7 @@ __global__ void __launch_bounds__(256) copy_f4(const float4 *__restrict__ a, flo
 
 // Serial operators (excite.rs, probe.rs, lumped.rs) and the gap rules of
 // Engine::advance (mod.rs:873-915).
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6, OP_MUR = 7 };
 
 // An installed lumped element (lumped.rs:367-406) and, with columns > 0,
 // its LumpedPort records (engine/port.rs:94-104).
@@ -1421,6 +1421,76 @@ struct DevOp {
   int seg_end[2];  // per gap: the first writer at or after this op (nops: none)
 };
 
+// ---- Mur absorbing boundary (boundary/mur.rs) ----
+//
+// Face edges with one 1-D update each, then the corners (edges on two Mur
+// faces), which read face edges already updated. u_b^n and u_1^n are saved
+// before the E update (mur_capture; the H update does not change u), the
+// updates run after the E update's slab corrections (mur_edges,
+// mur_corners). No face edge's interior neighbour is a Mur edge, so the
+// edges of one launch are independent.
+struct DevMur {
+  int ne, nc, oe_order, oe_corners;
+  const int32_t *eax;
+  const uint64_t *enode, *einner, *ehold;
+  const float *ek;
+  const int32_t *cax;
+  const uint64_t *cnode, *cin0, *cin1, *chold;
+  const float *ck0, *ck1;
+  float *saved;  // 2 per edge (u_b^n, u_1^n), then 3 per corner
+};
+
+// mur_1d (mur.rs:313-320): ems's order or openEMS's.
+__device__ __forceinline__ float mur_1d(float i, float k, float x, float b, int oe) {
+  return oe ? (i - k * b) + k * x : i + k * (x - b);
+}
+
+__device__ __forceinline__ float *pick_u(const Fields &f, int a) {
+  return a == 0 ? f.u[0] : (a == 1 ? f.u[1] : f.u[2]);
+}
+
+// pre_e / post_h_slab (mur.rs:358-373, 806-827): the saved values.
+__global__ void __launch_bounds__(256) mur_capture(Fields f, DevMur m) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q < m.ne) {
+    const GAS float *u = (const GAS float *)pick_u(f, m.eax[q]);
+    m.saved[2 * q] = u[m.enode[q]];
+    m.saved[2 * q + 1] = u[m.einner[q]];
+  } else if (q < m.ne + m.nc) {
+    const int c = q - m.ne;
+    const GAS float *u = (const GAS float *)pick_u(f, m.cax[c]);
+    float *sv = m.saved + 2 * (size_t)m.ne + 3 * (size_t)c;
+    sv[0] = u[m.cnode[c]], sv[1] = u[m.cin0[c]], sv[2] = u[m.cin1[c]];
+  }
+}
+
+// update (mur.rs:322-355), face edges.
+__global__ void __launch_bounds__(256) mur_edges(Fields f, DevMur m, const uint64_t *step) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= m.ne || *step < m.ehold[q]) return;
+  GAS float *u = (GAS float *)pick_u(f, m.eax[q]);
+  u[m.enode[q]] = mur_1d(m.saved[2 * q + 1], m.ek[q], u[m.einner[q]], m.saved[2 * q], m.oe_order);
+}
+
+// update (mur.rs:322-355), corners: the average of the two 1-D updates, or
+// openEMS's rule (the lower axis's face alone, its new neighbour taken as 0).
+__global__ void __launch_bounds__(256) mur_corners(Fields f, DevMur m, const uint64_t *step) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= m.nc || *step < m.chold[c]) return;
+  GAS float *u = (GAS float *)pick_u(f, m.cax[c]);
+  const float *sv = m.saved + 2 * (size_t)m.ne + 3 * (size_t)c;
+  const float b = sv[0], i0 = sv[1], i1 = sv[2];
+  float v;
+  if (m.oe_corners) {
+    v = i0 - m.ck0[c] * b;
+  } else {
+    const float v0 = mur_1d(i0, m.ck0[c], u[m.cin0[c]], b, m.oe_order);
+    const float v1 = mur_1d(i1, m.ck1[c], u[m.cin1[c]], b, m.oe_order);
+    v = 0.5f * (v0 + v1);
+  }
+  u[m.cnode[c]] = v;
+}
+
 __device__ __forceinline__ float op_amp(const DevOp &o, uint64_t n) {
   return n < o.namp ? o.amp[n] : 0.0f;
 }
@@ -1444,6 +1514,7 @@ __device__ __forceinline__ bool needs(const DevOp &o, uint64_t n) {
   }
   if (o.type == OP_LUMPED) return true;  // post_e every step
   if (o.type == OP_POLES) return false;  // slab hooks, not gap hooks
+  if (o.type == OP_MUR) return false;    // its own kernels before the gap
   if (o.type == OP_PW || o.type == OP_MERGED) return true;  // post_h and post_e every step
   return n % o.interval == 0;
 }
@@ -2389,6 +2460,8 @@ struct HostOp {
   uint32_t *mf_small = nullptr, *mf_big = nullptr;
   int mf_nsmall = 0, mf_nbig = 0, mf_lds_small = 0, mf_lds_big = 0;
   std::shared_ptr<struct MergedHost> mfh;
+  // Mur boundary
+  DevMur mur{};
 };
 
 // Host copies of a reduced-current operator's tables (its energy).
@@ -3064,6 +3137,29 @@ int launch_gap_mid(ems_gpu *h, const Fields &f) {
   return 0;
 }
 
+// The Mur boundaries: stage 0 saves u_b^n, u_1^n from `f` (before the E
+// update), stage 1 updates the face edges and then the corners of `f`.
+int launch_mur(ems_gpu *h, const Fields &f, int stage) {
+  for (auto &o : h->ops) {
+    if (o.type != OP_MUR) continue;
+    const DevMur &m = o.mur;
+    if (stage == 0) {
+      if (m.ne + m.nc)
+        hipLaunchKernelGGL(mur_capture, dim3((m.ne + m.nc + 255) / 256), dim3(256), 0, h->stream, f,
+                           m);
+    } else {
+      if (m.ne)
+        hipLaunchKernelGGL(mur_edges, dim3((m.ne + 255) / 256), dim3(256), 0, h->stream, f, m,
+                           (const uint64_t *)h->d_step);
+      if (m.nc)
+        hipLaunchKernelGGL(mur_corners, dim3((m.nc + 255) / 256), dim3(256), 0, h->stream, f, m,
+                           (const uint64_t *)h->d_step);
+    }
+    CK(h, hipGetLastError());
+  }
+  return 0;
+}
+
 // Unfused steps with reduced currents: their update forks onto the side
 // stream before the H update and joins before the mid gap.
 int merged_fork(ems_gpu *h, const Fields &f) {
@@ -3091,6 +3187,9 @@ int enqueue_step(ems_gpu *h, int &par) {
     if ((rc = launch_fused(h, par))) return rc;
     par ^= 1;
     if ((rc = launch_poles(h, 1, h->fb[par], h->fb[par ^ 1], true))) return rc;
+    // (u^n is still in the previous buffer)
+    if ((rc = launch_mur(h, h->fb[par ^ 1], 0))) return rc;
+    if ((rc = launch_mur(h, h->fb[par], 1))) return rc;
     return launch_gap_post_e(h, h->fb[par], h->fb[par ^ 1], true);
   }
   const Fields &f = h->fb[par];
@@ -3098,11 +3197,13 @@ int enqueue_step(ems_gpu *h, int &par) {
   if ((rc = launch_h(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 0, par))) return rc;
   if ((rc = launch_poles(h, 0, f, f, false))) return rc;
+  if ((rc = launch_mur(h, f, 0))) return rc;
   if ((rc = merged_join(h))) return rc;
   if ((rc = launch_gap_mid(h, f))) return rc;
   if ((rc = launch_e(h, f))) return rc;
   if ((rc = launch_cpml(h, f, 1, par))) return rc;
   if ((rc = launch_poles(h, 1, f, f, false))) return rc;
+  if ((rc = launch_mur(h, f, 1))) return rc;
   return launch_gap_post_e(h, f, f, false);
 }
 
@@ -3595,6 +3696,77 @@ int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *s, int n) {
   }
   h->slabs_dirty = true;
   drop_graphs(h);
+  return EMS_OK;
+}
+
+int ems_gpu_add_mur(ems_gpu *h, const ems_mur_desc *d, int *id) {
+  ALIVE(h);
+  if (!d || d->n_edges < 0 || d->n_corners < 0) return EMS_E_INVALID;
+  // Slab mode of the reference (mur.rs:553-562): no operator registered
+  // before it acts on the voltages serially around the E update.
+  for (auto &o : h->ops)
+    if ((o.type == OP_SOURCE && (o.kind == 0 || o.kind == 1)) ||
+        (o.type == OP_PROBE && o.quantity == 0) || o.type == OP_LUMPED || o.type == OP_PW ||
+        o.type == OP_MERGED)
+      return fail(h, EMS_E_UNSUPPORTED,
+                  "register the Mur boundary before E sources, voltage probes, lumped "
+                  "elements, plane waves and cut-face currents");
+  const int ne = d->n_edges, nc = d->n_corners;
+  if ((ne && (!d->edge_axis || !d->edge_node || !d->edge_inner || !d->edge_k)) ||
+      (nc && (!d->corner_axis || !d->corner_node || !d->corner_inner0 || !d->corner_inner1 ||
+              !d->corner_k0 || !d->corner_k1)))
+    return fail(h, EMS_E_INVALID, "Mur arrays");
+  auto flat = [&](const int32_t *nd, uint64_t *out) {
+    if (!node_ok(h, nd)) return false;
+    *out = ((uint64_t)nd[0] * h->n[1] + nd[1]) * h->n[2] + nd[2];
+    return true;
+  };
+  std::vector<int32_t> eax(ne), cax(nc);
+  std::vector<uint64_t> en(ne), ei(ne), eh(ne, 0), cn(nc), c0(nc), c1(nc), ch(nc, 0);
+  for (int q = 0; q < ne; q++) {
+    eax[q] = d->edge_axis[q];
+    if (eax[q] < 0 || eax[q] > 2 || !flat(d->edge_node + 3 * q, &en[q]) ||
+        !flat(d->edge_inner + 3 * q, &ei[q]))
+      return fail(h, EMS_E_INVALID, "Mur edge out of range");
+    if (d->edge_hold) eh[q] = d->edge_hold[q];
+  }
+  for (int q = 0; q < nc; q++) {
+    cax[q] = d->corner_axis[q];
+    if (cax[q] < 0 || cax[q] > 2 || !flat(d->corner_node + 3 * q, &cn[q]) ||
+        !flat(d->corner_inner0 + 3 * q, &c0[q]) || !flat(d->corner_inner1 + 3 * q, &c1[q]))
+      return fail(h, EMS_E_INVALID, "Mur corner out of range");
+    if (d->corner_hold) ch[q] = d->corner_hold[q];
+  }
+  HostOp o;
+  o.type = OP_MUR;
+  DevMur &m = o.mur;
+  m.ne = ne, m.nc = nc, m.oe_order = d->openems_order ? 1 : 0;
+  m.oe_corners = d->openems_corners ? 1 : 0;
+  int rc;
+  void *p = nullptr;
+#define UP(dst, src, bytes)                                       \
+  if ((rc = upload(h, &p, (src), (bytes)))) return rc;           \
+  o.mem.push_back(p);                                             \
+  dst = (decltype(dst))p;
+  UP(m.eax, eax.data(), ne * sizeof(int32_t));
+  UP(m.enode, en.data(), ne * sizeof(uint64_t));
+  UP(m.einner, ei.data(), ne * sizeof(uint64_t));
+  UP(m.ehold, eh.data(), ne * sizeof(uint64_t));
+  UP(m.ek, d->edge_k, ne * sizeof(float));
+  UP(m.cax, cax.data(), nc * sizeof(int32_t));
+  UP(m.cnode, cn.data(), nc * sizeof(uint64_t));
+  UP(m.cin0, c0.data(), nc * sizeof(uint64_t));
+  UP(m.cin1, c1.data(), nc * sizeof(uint64_t));
+  UP(m.chold, ch.data(), nc * sizeof(uint64_t));
+  UP(m.ck0, d->corner_k0, nc * sizeof(float));
+  UP(m.ck1, d->corner_k1, nc * sizeof(float));
+  std::vector<float> zero(2 * (size_t)ne + 3 * (size_t)nc, 0.0f);
+  UP(m.saved, zero.data(), zero.size() * sizeof(float));
+#undef UP
+  h->ops.push_back(std::move(o));
+  h->ops_dirty = true;
+  drop_graphs(h);
+  if (id) *id = (int)h->ops.size() - 1;
   return EMS_OK;
 }
 
@@ -4596,6 +4768,8 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       CK(h, hipEventRecord(ev[1], h->stream));
       h->par ^= 1;
       if ((rc = launch_poles(h, 1, h->fb[h->par], h->fb[h->par ^ 1], true))) return rc;
+      if ((rc = launch_mur(h, h->fb[h->par ^ 1], 0))) return rc;
+      if ((rc = launch_mur(h, h->fb[h->par], 1))) return rc;
       if ((rc = launch_gap_post_e(h, h->fb[h->par], h->fb[h->par ^ 1], true))) return rc;
       CK(h, hipEventRecord(ev[4], h->stream));
       CK(h, hipEventSynchronize(ev[4]));
@@ -4609,6 +4783,7 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       CK(h, hipEventRecord(ev[1], h->stream));
       if ((rc = launch_cpml(h, f, 0, h->par))) return rc;
       if ((rc = launch_poles(h, 0, f, f, false))) return rc;
+      if ((rc = launch_mur(h, f, 0))) return rc;
       if ((rc = merged_join(h))) return rc;
       if ((rc = launch_gap_mid(h, f))) return rc;
       CK(h, hipEventRecord(ev[2], h->stream));
@@ -4616,6 +4791,7 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       CK(h, hipEventRecord(ev[3], h->stream));
       if ((rc = launch_cpml(h, f, 1, h->par))) return rc;
       if ((rc = launch_poles(h, 1, f, f, false))) return rc;
+      if ((rc = launch_mur(h, f, 1))) return rc;
       if ((rc = launch_gap_post_e(h, f, f, false))) return rc;
       CK(h, hipEventRecord(ev[4], h->stream));
       CK(h, hipEventSynchronize(ev[4]));
@@ -4921,6 +5097,10 @@ int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
     h->fused = true, h->rows = false, h->band = false, h->prefetch = b->prefetch, h->tj = b->tj;
     h->packed = b->packed;
     h->nchunk = b->nchunk;
+    for (auto &o : h->ops)
+      if (o.type == OP_MUR)
+        return bfail(b, EMS_E_UNSUPPORTED,
+                     "model " + std::to_string(q) + " has a Mur boundary (not batched)");
     if (!use_fused(h))
       return bfail(b, EMS_E_UNSUPPORTED,
                    "model " + std::to_string(q) +

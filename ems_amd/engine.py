@@ -842,6 +842,11 @@ class Engine:
             self._poles_n[oid.value] = (len(plan["sides"][0]["cidx"]), len(plan["sides"][1]["cidx"]))
             self.setup.append(("dispersive", plan))
             op._bind(self, oid.value, plan)
+        elif type(op).__name__ == "MurBoundary":
+            d, keep = op.desc()
+            oid = C.c_int()
+            self._check(nat.ems_gpu_add_mur(h, C.byref(d), C.byref(oid)))
+            self.setup.append(("mur", op))
         elif hasattr(op, "_plan"):
             # LumpedElement / LumpedPort (lumped.rs:880-1012): fold into the
             # lanes, re-upload them, register the device operator.

@@ -77,6 +77,33 @@ typedef struct {
 } ems_cpml_slab;
 int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *slabs, int n);
 
+/* MurBoundary (src/engine/boundary/mur.rs:207-640): the first-order Mur
+ * absorbing boundary on outer faces built as PEC. Edges (face edges with one
+ * 1-D update each): axis, node[3], interior neighbour inner[3] one node
+ * inward along the face normal, coefficient k = (v dt - d)/(v dt + d) as
+ * f32, and a hold step (no update before it, MurBoundary::hold_faces;
+ * NULL = none). Corners (edges on two Mur faces): the two interior
+ * neighbours and coefficients. Before the E update u_b^n and u_1^n are
+ * saved; after it (and after the CPML and dispersive corrections) every
+ * edge takes u_1^n + k (u_1^{n+1} - u_b^n) (openems_order: (u_1^n - k u_b^n)
+ * + k u_1^{n+1}), then every corner the average of its two updates
+ * (openems_corners: u_1^n - k0 u_b^n of the first). Refused after an E
+ * source, a voltage probe, a lumped element, a plane wave or cut-face
+ * currents: the reference's slab mode (mur.rs:553-562), which this
+ * placement follows, needs the boundary registered before them. */
+typedef struct {
+    int32_t n_edges;
+    const int32_t *edge_axis, *edge_node, *edge_inner;
+    const float *edge_k;
+    const uint64_t *edge_hold;
+    int32_t n_corners;
+    const int32_t *corner_axis, *corner_node, *corner_inner0, *corner_inner1;
+    const float *corner_k0, *corner_k1;
+    const uint64_t *corner_hold;
+    int32_t openems_order, openems_corners;
+} ems_mur_desc;
+int ems_gpu_add_mur(ems_gpu *h, const ems_mur_desc *d, int *id);
+
 /* EdgeSource (src/engine/excite.rs:10-173): kind 0 SoftE, 1 HardE,
  * 2 SoftH, 3 HardH; edges (axis, node[3], weight f32). The host tabulates
  * the amplitude `a(t) as f32` at each step's sample time (SoftE (n+1/2)dt,

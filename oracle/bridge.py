@@ -57,6 +57,10 @@ orc_add_merged = _sig("orc_add_merged", _i, _p, _p)
 orc_merged_state = _sig("orc_merged_state", _i, _p, _i, _dp, _dp)
 orc_run = _sig("orc_run", None, _p, _u64)
 orc_set_fused = _sig("orc_set_fused", None, _p, _i)
+_ip2 = C.POINTER(C.c_int)
+orc_add_mur = _sig("orc_add_mur", _i, _p, _i, _ip2, _ip2, _ip2, C.POINTER(C.c_float),
+                   C.POINTER(C.c_uint64), _i, _ip2, _ip2, _ip2, _ip2, C.POINTER(C.c_float),
+                   C.POINTER(C.c_float), C.POINTER(C.c_uint64), _i, _i)
 orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
 orc_energy = _sig("orc_energy", _d, _p)
 orc_timestep = _sig("orc_timestep", _u64, _p)
@@ -103,6 +107,9 @@ class OracleEngine:
         self._lumped = {}
         self._poles_n = {}
         self._pw_tables = []
+
+    def coefficients(self) -> np.ndarray:
+        return self._dense_lanes
 
     def _set_lane(self, l, lane):
         n = self.grid.nodes()
@@ -177,6 +184,18 @@ class OracleEngine:
 
     def add_operator(self, op):
         h = self._h
+        if type(op).__name__ == "MurBoundary":
+            a = op.arrays()
+            I = C.c_int
+            r = orc_add_mur(h, len(op.edges), P(a["eax"], I), P(a["enode"], I), P(a["einner"], I),
+                            P(a["ek"], C.c_float), P(a["ehold"], C.c_uint64), len(op.corners),
+                            P(a["cax"], I), P(a["cnode"], I), P(a["cin0"], I), P(a["cin1"], I),
+                            P(a["ck0"], C.c_float), P(a["ck1"], C.c_float),
+                            P(a["chold"], C.c_uint64), 1 if op._openems_order else 0,
+                            1 if op._openems_corners else 0)
+            assert r >= 0, "orc_add_mur failed"
+            self.ops.append(op)
+            return op
         if isinstance(op, E.CpmlBoundary):
             for s in op.slabs:
                 lo = (C.c_int * 3)(*s["lo"])

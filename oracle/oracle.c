@@ -69,7 +69,18 @@ typedef struct {
     float *b, *a, *k, *psi;
 } cpml_t;
 
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6 };
+enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6, OP_MUR = 7 };
+
+/* MurBoundary (boundary/mur.rs:94-140): face edges (Edge) and domain edges
+ * on two Mur faces (Corner), with the values saved before the E update. */
+typedef struct {
+    int ne, nc, oe_order, oe_corners;
+    int *eax, *cax;
+    size_t *enode, *einner, *cnode, *cin0, *cin1;
+    float *ek, *ck0, *ck1;
+    uint64_t *ehold, *chold;
+    float *saved; /* 2 per edge, then 3 per corner */
+} mur_t;
 
 /* Plane-wave corrections (Correction, planewave.rs:424-431), in list
  * order, and the auxiliary grid (PlaneWave, :458-483). */
@@ -150,6 +161,8 @@ typedef struct {
     pw_t *pw;
     /* reduced currents of cut faces */
     struct merged *mf;
+    /* Mur boundary */
+    mur_t *mur;
 } op_t;
 
 /* MergedFaces (conformal.rs:2313-2340): the tables of orc_merged_desc and
@@ -832,12 +845,67 @@ static int needs(const op_t *o, uint64_t n) {
     }
     if (o->type == OP_LUMPED) return 1; /* pre_e / post_e every step */
     if (o->type == OP_POLES) return 0;  /* slab hooks only */
+    if (o->type == OP_MUR) return 0;    /* slab hooks only (mur.rs:799-853) */
     if (o->type == OP_PW || o->type == OP_MERGED) return 1; /* post_h and post_e every step */
     return n % o->interval == 0;
 }
 
 /* One step: H phase, mid gap, E phase, post-E gap, with the gap rules of
  * Engine::advance (mod.rs:873-915) for these operator kinds. */
+/* mur_1d (mur.rs:313-320) */
+static float mur_1d(float i, float k, float x, float b, int oe) {
+    return oe ? (i - k * b) + k * x : i + k * (x - b);
+}
+
+/* MurSlabs::post_h_slab (mur.rs:799-815): u_b^n, u_1^n of every edge and
+ * corner, before the E update. */
+static void mur_capture(orc_engine *e, mur_t *m) {
+    for (int q = 0; q < m->ne; q++) {
+        const float *u = e->u[m->eax[q]];
+        m->saved[2 * q] = u[m->enode[q]];
+        m->saved[2 * q + 1] = u[m->einner[q]];
+    }
+    for (int c = 0; c < m->nc; c++) {
+        const float *u = e->u[m->cax[c]];
+        float *sv = m->saved + 2 * (size_t)m->ne + 3 * (size_t)c;
+        sv[0] = u[m->cnode[c]], sv[1] = u[m->cin0[c]], sv[2] = u[m->cin1[c]];
+    }
+}
+
+/* update (mur.rs:322-355): the face edges, then the corners (whose
+ * neighbours are face edges updated here first). */
+static void mur_update(orc_engine *e, mur_t *m, uint64_t step) {
+    for (int q = 0; q < m->ne; q++) {
+        if (step < m->ehold[q]) continue;
+        float *u = e->u[m->eax[q]];
+        u[m->enode[q]] = mur_1d(m->saved[2 * q + 1], m->ek[q], u[m->einner[q]], m->saved[2 * q],
+                                m->oe_order);
+    }
+    for (int c = 0; c < m->nc; c++) {
+        if (step < m->chold[c]) continue;
+        float *u = e->u[m->cax[c]];
+        const float *sv = m->saved + 2 * (size_t)m->ne + 3 * (size_t)c;
+        const float b = sv[0], i0 = sv[1], i1 = sv[2];
+        float v;
+        if (m->oe_corners) {
+            v = i0 - m->ck0[c] * b;
+        } else {
+            const float v0 = mur_1d(i0, m->ck0[c], u[m->cin0[c]], b, m->oe_order);
+            const float v1 = mur_1d(i1, m->ck1[c], u[m->cin1[c]], b, m->oe_order);
+            v = 0.5f * (v0 + v1);
+        }
+        u[m->cnode[c]] = v;
+    }
+}
+
+static void murs(orc_engine *e, int stage, uint64_t step) {
+    for (int q = 0; q < e->nops; q++)
+        if (e->ops[q].type == OP_MUR) {
+            if (stage == 0) mur_capture(e, e->ops[q].mur);
+            else mur_update(e, e->ops[q].mur, step);
+        }
+}
+
 static void one_step(orc_engine *e, int energy) {
     uint64_t n = e->step;
     int need_post_e = 0, mid = 0, mid_movable = 1;
@@ -864,13 +932,16 @@ static void one_step(orc_engine *e, int energy) {
     if (fused) {
         for (int q = 0; q < e->nops; q++)
             if (e->ops[q].type == OP_LUMPED) lumped_pre_e(e, &e->ops[q]);
+        murs(e, 0, n); /* (u is unchanged since the previous post_e) */
         parallel(e, 4);
         parallel(e, 5);
+        murs(e, 1, n); /* Fusion::Voltages: after the slab's E hooks */
         goto post_e;
     }
     e->energy_flag = energy;
     parallel(e, 1);
     e->energy_flag = 0;
+    murs(e, 0, n);
     if (gap_b) {
         for (int q = 0; q < e->nops; q++) {
             op_t *o = &e->ops[q];
@@ -888,6 +959,7 @@ static void one_step(orc_engine *e, int energy) {
             if (e->ops[q].type == OP_LUMPED) lumped_pre_e(e, &e->ops[q]);
     }
     parallel(e, 2);
+    murs(e, 1, n);
 post_e:
     if (gap_a) {
         if (moved)
@@ -1418,6 +1490,51 @@ double orc_energy(orc_engine *e) {
 }
 
 uint64_t orc_timestep(orc_engine *e) { return e->step; }
+
+int orc_add_mur(orc_engine *e, int ne, const int *eax, const int *enode, const int *einner,
+                const float *ek, const uint64_t *ehold, int nc, const int *cax, const int *cnode,
+                const int *cin0, const int *cin1, const float *ck0, const float *ck1,
+                const uint64_t *chold, int oe_order, int oe_corners) {
+    if (ne < 0 || nc < 0) return -1;
+    for (int q = 0; q < ne; q++)
+        if (eax[q] < 0 || eax[q] > 2 || !node_ok(e, enode + 3 * q) || !node_ok(e, einner + 3 * q))
+            return -1;
+    for (int q = 0; q < nc; q++)
+        if (cax[q] < 0 || cax[q] > 2 || !node_ok(e, cnode + 3 * q) || !node_ok(e, cin0 + 3 * q) ||
+            !node_ok(e, cin1 + 3 * q))
+            return -1;
+    int id = add_op(e);
+    op_t *o = &e->ops[id];
+    o->type = OP_MUR;
+    mur_t *m = (mur_t *)calloc(1, sizeof(mur_t));
+    o->mur = m;
+    m->ne = ne, m->nc = nc, m->oe_order = oe_order, m->oe_corners = oe_corners;
+    m->eax = (int *)calloc(ne + 1, sizeof(int));
+    m->enode = (size_t *)calloc(ne + 1, sizeof(size_t));
+    m->einner = (size_t *)calloc(ne + 1, sizeof(size_t));
+    m->ek = (float *)calloc(ne + 1, sizeof(float));
+    m->ehold = (uint64_t *)calloc(ne + 1, sizeof(uint64_t));
+    m->cax = (int *)calloc(nc + 1, sizeof(int));
+    m->cnode = (size_t *)calloc(nc + 1, sizeof(size_t));
+    m->cin0 = (size_t *)calloc(nc + 1, sizeof(size_t));
+    m->cin1 = (size_t *)calloc(nc + 1, sizeof(size_t));
+    m->ck0 = (float *)calloc(nc + 1, sizeof(float));
+    m->ck1 = (float *)calloc(nc + 1, sizeof(float));
+    m->chold = (uint64_t *)calloc(nc + 1, sizeof(uint64_t));
+    m->saved = (float *)calloc(2 * (size_t)ne + 3 * (size_t)nc + 1, sizeof(float));
+#define FLAT(p) (((size_t)(p)[0] * e->n[1] + (p)[1]) * e->n[2] + (p)[2])
+    for (int q = 0; q < ne; q++) {
+        m->eax[q] = eax[q], m->enode[q] = FLAT(enode + 3 * q), m->einner[q] = FLAT(einner + 3 * q);
+        m->ek[q] = ek[q], m->ehold[q] = ehold ? ehold[q] : 0;
+    }
+    for (int q = 0; q < nc; q++) {
+        m->cax[q] = cax[q], m->cnode[q] = FLAT(cnode + 3 * q);
+        m->cin0[q] = FLAT(cin0 + 3 * q), m->cin1[q] = FLAT(cin1 + 3 * q);
+        m->ck0[q] = ck0[q], m->ck1[q] = ck1[q], m->chold[q] = chold ? chold[q] : 0;
+    }
+#undef FLAT
+    return id;
+}
 
 void orc_set_fused(orc_engine *e, int mode) { e->fuse = mode < 0 || mode > 2 ? 0 : mode; }
 

@@ -175,6 +175,13 @@ void orc_run(orc_engine *e, uint64_t n);
  * thread's slab reach 4 MiB and the step allows it), 1 Always, 2 Never.
  * Results are bit-identical either way. */
 void orc_set_fused(orc_engine *e, int mode);
+/* MurBoundary (boundary/mur.rs): face edges (axis, node[3], inner[3], k,
+ * hold) and corners (axis, node[3], inner0[3], inner1[3], k0, k1, hold);
+ * holds may be NULL. Returns the operator id or -1. */
+int orc_add_mur(orc_engine *e, int ne, const int *eax, const int *enode, const int *einner,
+                const float *ek, const uint64_t *ehold, int nc, const int *cax, const int *cnode,
+                const int *cin0, const int *cin1, const float *ck0, const float *ck1,
+                const uint64_t *chold, int oe_order, int oe_corners);
 double orc_step_with_energy(orc_engine *e);
 double orc_energy(orc_engine *e);
 uint64_t orc_timestep(orc_engine *e);
