@@ -2513,7 +2513,7 @@ struct ems_gpu {
   uint32_t *d_pmask_o = nullptr;
   int n_hout = 0, n_eout = 0;
   uint32_t hout_total = 0, eout_total = 0;
-  bool cpml_split = true;  // allowed (ems_gpu_set_cpml_split)
+  bool cpml_split = false;  // ems_gpu_set_cpml_split (measured no faster: off)
   int split_state = -1;    // what the uploaded tables hold (-1: nothing yet)
   std::vector<HostOp> ops;
   std::vector<HostDft> dfts;

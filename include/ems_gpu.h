@@ -419,7 +419,7 @@ double ems_gpu_dt(const ems_gpu *h);
 int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks);
 
 /* CPML slabs of the tiled fused step (no reference counterpart): with
- * on = 1 (the default) the slabs normal to z that follow every other slab
+ * on = 1 (default 0) the slabs normal to z that follow every other slab
  * of their side in registration order are corrected outside the fused
  * kernel, H-side by increments formed from u^n before it, E-side after it
  * (the same values, bit for bit); 0 keeps every slab inside. */

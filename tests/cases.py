@@ -30,8 +30,9 @@ def graded(length: float, cells: int, alpha: float):
             for q in range(cells + 1)]
 
 
-# Schedule of the device engines the tests build: (fused, tile_rows, x_chunks).
-SCHEDULE = {"fused": 1, "tile_rows": 0, "x_chunks": 0}
+# Schedule of the device engines the tests build: (fused, tile_rows, x_chunks)
+# and whether the tiled step corrects its z-normal CPML slabs outside itself.
+SCHEDULE = {"fused": 1, "tile_rows": 0, "x_chunks": 0, "split": 0}
 
 
 def make(kind: str, grid, mats=None, faces=None, ts=None, threads=1, curl_order=0, pool=True):
@@ -42,6 +43,7 @@ def make(kind: str, grid, mats=None, faces=None, ts=None, threads=1, curl_order=
     if kind == "gpu":
         e = E.Engine(grid, dt, lanes, curl_order=curl_order, pool=pool)
         e.set_schedule(SCHEDULE["fused"], SCHEDULE["tile_rows"], SCHEDULE["x_chunks"])
+        e.set_cpml_split(bool(SCHEDULE.get("split", 0)))
         return e
     from oracle.bridge import OracleEngine
     return OracleEngine(grid, dt, lanes, curl_order=curl_order, threads=threads, pool=pool)

@@ -97,15 +97,15 @@ def cpml_box_512():
     return wl, seeded_fields(wl.grid.nodes(), 512)
 
 
-@pytest.mark.parametrize("sched", [None, (0, 0, 0), (6, 16, 0), "inline"],
-                         ids=["default", "unfused", "band16", "cpml_inline"])
+@pytest.mark.parametrize("sched", [None, (0, 0, 0), (6, 16, 0), "split"],
+                         ids=["default", "unfused", "band16", "cpml_split"])
 def test_cpml_box_512_full_size(cpml_box_512, sched):
     from oracle.bridge import OracleEngine
     wl, init = cpml_box_512
     steps = 24
     e = E.Engine(wl.grid, wl.dt, wl.lanes)
-    if sched == "inline":
-        e.set_cpml_split(False)  # every CPML slab inside the tiled kernel
+    if sched == "split":
+        e.set_cpml_split(True)  # the z-normal slabs in passes of their own
     elif sched is not None:
         e.set_schedule(*sched)
     pe = wl.build_ops(e)

@@ -16,17 +16,19 @@ pytestmark = pytest.mark.gpu
 # Every parity test runs on the unfused schedule and on the fused one with
 # each tile height and with many x chunks (chunk seams inside the box), the
 # full-row and the packed-row kernels (packed: grids with nz <= 64; the
-# others run the tiled kernel) and the band kernel (bands of 4, 8, 16 rows).
+# others run the tiled kernel), the band kernel (bands of 4, 8, 16 rows) and
+# the tiled kernel with its z-normal CPML slabs in passes of their own.
 SCHEDULES = [(0, 0, 0), (1, 3, 0), (1, 7, 0), (1, 15, 0), (1, 7, 5), (2, 7, 3), (2, 15, 0),
              (3, 3, 0), (3, 7, 0), (3, 7, 4), (4, 7, 0), (4, 7, 3), (5, 7, 0),
-             (6, 4, 0), (6, 8, 3), (6, 16, 0), (6, 16, 5)]
+             (6, 4, 0), (6, 8, 3), (6, 16, 0), (6, 16, 5), (1, 7, 0, 1), (2, 3, 5, 1)]
 
 
-@pytest.fixture(autouse=True, params=SCHEDULES, ids=lambda s: "mode%d_tj%d_xc%d" % s)
+@pytest.fixture(autouse=True, params=SCHEDULES,
+                ids=lambda s: "mode%d_tj%d_xc%d" % s[:3] + ("_split" if s[3:] else ""))
 def schedule(request):
-    f, t, c = request.param
+    f, t, c, *split = request.param
     old = dict(cases.SCHEDULE)
-    cases.SCHEDULE.update(fused=f, tile_rows=t, x_chunks=c)
+    cases.SCHEDULE.update(fused=f, tile_rows=t, x_chunks=c, split=split[0] if split else 0)
     yield
     cases.SCHEDULE.update(old)
 
