@@ -18,5 +18,6 @@ from .planewave import Medium, PlaneWave, PlaneWaveSpec  # noqa: F401,E402
 from .post import PortResult, dft, s_param  # noqa: F401,E402
 from .learn import ConvNet, EarlyStop, TraceFormat  # noqa: F401,E402
 from .mur import MurBoundary  # noqa: F401,E402
+from .upml import OpenEmsPml, openems_pml_sigma  # noqa: F401,E402
 from .conformal import (  # noqa: F401,E402
     ConformalEdge, ConformalFace, ConformalOptions, ConformalPec, ConformalReport)

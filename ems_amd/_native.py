@@ -107,6 +107,8 @@ ems_gpu_last_error = _sig("ems_gpu_last_error", C.c_char_p, _p)
 ems_gpu_set_lane = _sig("ems_gpu_set_lane", _i, _p, _i, _fp, _u64, _u32p)
 ems_gpu_add_cpml = _sig("ems_gpu_add_cpml", _i, _p, C.POINTER(CpmlSlab), _i)
 ems_gpu_add_mur = _sig("ems_gpu_add_mur", _i, _p, _p, C.POINTER(_i))
+ems_gpu_add_upml = _sig("ems_gpu_add_upml", _i, _p, _p, C.POINTER(_i))
+ems_gpu_upml_state = _sig("ems_gpu_upml_state", _i, _p, _i, _i, _fp, _fp)
 ems_gpu_add_source = _sig("ems_gpu_add_source", _i, _p, _i, _i, _ip, _ip, _fp, _fp, _u64,
                           C.POINTER(_i))
 ems_gpu_add_probe = _sig("ems_gpu_add_probe", _i, _p, _i, _i, _ip, _ip, _dp, _u64,

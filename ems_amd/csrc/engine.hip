@@ -1386,7 +1386,10 @@ __global__ void __launch_bounds__(256) copy_f4(const float4 *__restrict__ a, flo
 
 // Serial operators (excite.rs, probe.rs, lumped.rs) and the gap rules of
 // Engine::advance (mod.rs:873-915).
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6, OP_MUR = 7 };
+enum {
+  OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6, OP_MUR = 7,
+  OP_UPML = 8
+};
 
 // An installed lumped element (lumped.rs:367-406) and, with columns > 0,
 // its LumpedPort records (engine/port.rs:94-104).
@@ -1491,6 +1494,50 @@ __global__ void __launch_bounds__(256) mur_corners(Fields f, DevMur m, const uin
   u[m.cnode[c]] = v;
 }
 
+// ---- openEMS's PML_n layers (boundary/upml.rs) ----
+//
+// The layer edges carry coefficients rewritten by the host (ca = 0 and
+// cb = dt/((A/l)(1 + a_p)), da and db likewise), so the H and E updates
+// leave the flux increment t in the field. Edges are grouped by side and
+// axis (off[g] is the first edge of group g = 3 side + axis: E x, y, z,
+// then H x, y, z); every edge is independent of the others.
+struct DevUpml {
+  int ne, nh;
+  int off[7];
+  const uint64_t *idx;  // node index per edge
+  const float4 *co;     // (1-a_p)/(1+a_p), (1-a_q)/(1+a_q), (1+a_a)/((1+a_q)m), (1-a_a)/((1+a_q)m)
+  float2 *fv;           // flux F, start value v
+};
+
+__device__ __forceinline__ float *pick_i(const Fields &f, int a) {
+  return a == 0 ? f.i[0] : (a == 1 ? f.i[1] : f.i[2]);
+}
+
+// Parts::post_h_slab / post_e_slab (upml.rs:442-470). stage 0 (after the H
+// update): E edges take v from u (Run::refresh, :391-400), H edges complete
+// (Run::apply, :404-426); stage 1 (after the E update) the other way round.
+__global__ void __launch_bounds__(256) upml_stage(Fields f, DevUpml m, int stage) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= m.ne + m.nh) return;
+  int g = 0;
+  while (g < 5 && q >= m.off[g + 1]) g++;
+  const bool eside = g < 3;
+  const int a = eside ? g : g - 3;
+  GAS float *x = (GAS float *)(eside ? pick_u(f, a) : pick_i(f, a));
+  const uint64_t n = m.idx[q];
+  float2 s = m.fv[q];
+  if (eside == (stage == 0)) {
+    s.y = x[n];
+  } else {
+    const float4 c = m.co[q];
+    const float f1 = flush(flush(c.x * s.x) + x[n]);
+    const float v = flush(flush(flush(c.y * s.y) - flush(c.w * s.x)) + flush(c.z * f1));
+    s.x = f1, s.y = v;
+    x[n] = v;
+  }
+  m.fv[q] = s;
+}
+
 __device__ __forceinline__ float op_amp(const DevOp &o, uint64_t n) {
   return n < o.namp ? o.amp[n] : 0.0f;
 }
@@ -1515,6 +1562,7 @@ __device__ __forceinline__ bool needs(const DevOp &o, uint64_t n) {
   if (o.type == OP_LUMPED) return true;  // post_e every step
   if (o.type == OP_POLES) return false;  // slab hooks, not gap hooks
   if (o.type == OP_MUR) return false;    // its own kernels before the gap
+  if (o.type == OP_UPML) return false;   // slab corrections, not gap hooks
   if (o.type == OP_PW || o.type == OP_MERGED) return true;  // post_h and post_e every step
   return n % o.interval == 0;
 }
@@ -2462,6 +2510,9 @@ struct HostOp {
   std::shared_ptr<struct MergedHost> mfh;
   // Mur boundary
   DevMur mur{};
+  // openEMS's PML layers; upml_pos: device slot of each edge in the caller's order
+  DevUpml upml{};
+  std::vector<uint32_t> upml_pos;
 };
 
 // Host copies of a reduced-current operator's tables (its energy).
@@ -2611,10 +2662,17 @@ bool has_merged(const ems_gpu *h) {
   return false;
 }
 
+bool has_upml(const ems_gpu *h) {
+  for (auto &o : h->ops)
+    if (o.type == OP_UPML) return true;
+  return false;
+}
+
 bool use_fused(const ems_gpu *h) {
   size_t ns = 0;
   for (auto &s : h->slabs) ns += s.len != 0;
-  return h->fused && !has_hsrc(h) && !has_magnetic_poles(h) && !has_gap_kernels(h) && ns <= 32;
+  return h->fused && !has_hsrc(h) && !has_magnetic_poles(h) && !has_gap_kernels(h) &&
+         !has_upml(h) && ns <= 32;
 }
 
 // The dispersive media as slab hooks (dispersive.rs:1700-1750), in
@@ -3160,6 +3218,20 @@ int launch_mur(ems_gpu *h, const Fields &f, int stage) {
   return 0;
 }
 
+// openEMS's PML layers: stage 0 right after the H update, stage 1 right
+// after the E update (first among the slab corrections).
+int launch_upml(ems_gpu *h, const Fields &f, int stage) {
+  for (auto &o : h->ops) {
+    if (o.type != OP_UPML) continue;
+    const int n = o.upml.ne + o.upml.nh;
+    if (n)
+      hipLaunchKernelGGL(upml_stage, dim3((n + 255) / 256), dim3(256), 0, h->stream, f, o.upml,
+                         stage);
+    CK(h, hipGetLastError());
+  }
+  return 0;
+}
+
 // Unfused steps with reduced currents: their update forks onto the side
 // stream before the H update and joins before the mid gap.
 int merged_fork(ems_gpu *h, const Fields &f) {
@@ -3195,12 +3267,14 @@ int enqueue_step(ems_gpu *h, int &par) {
   const Fields &f = h->fb[par];
   if ((rc = merged_fork(h, f))) return rc;
   if ((rc = launch_h(h, f))) return rc;
+  if ((rc = launch_upml(h, f, 0))) return rc;
   if ((rc = launch_cpml(h, f, 0, par))) return rc;
   if ((rc = launch_poles(h, 0, f, f, false))) return rc;
   if ((rc = launch_mur(h, f, 0))) return rc;
   if ((rc = merged_join(h))) return rc;
   if ((rc = launch_gap_mid(h, f))) return rc;
   if ((rc = launch_e(h, f))) return rc;
+  if ((rc = launch_upml(h, f, 1))) return rc;
   if ((rc = launch_cpml(h, f, 1, par))) return rc;
   if ((rc = launch_poles(h, 1, f, f, false))) return rc;
   if ((rc = launch_mur(h, f, 1))) return rc;
@@ -3649,10 +3723,13 @@ int ems_gpu_set_lane(ems_gpu *h, int lane, const float *rows, uint64_t nrows,
 
 int ems_gpu_add_cpml(ems_gpu *h, const ems_cpml_slab *s, int n) {
   ALIVE(h);
-  for (auto &o : h->ops)
+  for (auto &o : h->ops) {
     if (o.type == OP_POLES)
       return fail(h, EMS_E_UNSUPPORTED,
                   "CPML slabs must be registered before the dispersive media");
+    if (o.type == OP_UPML)
+      return fail(h, EMS_E_UNSUPPORTED, "openEMS PML layers and CPML slabs on one engine");
+  }
   ALIVE(h);
   if (n > 0 && !s) return EMS_E_INVALID;
   for (int q = 0; q < n; q++) {
@@ -3767,6 +3844,103 @@ int ems_gpu_add_mur(ems_gpu *h, const ems_mur_desc *d, int *id) {
   h->ops_dirty = true;
   drop_graphs(h);
   if (id) *id = (int)h->ops.size() - 1;
+  return EMS_OK;
+}
+
+int ems_gpu_add_upml(ems_gpu *h, const ems_upml_desc *d, int *id) {
+  ALIVE(h);
+  if (!d || d->n_e < 0 || d->n_h < 0) return EMS_E_INVALID;
+  if (!h->slabs.empty())
+    return fail(h, EMS_E_UNSUPPORTED, "openEMS PML layers and CPML slabs on one engine");
+  for (auto &o : h->ops) {
+    if (o.type == OP_UPML) return fail(h, EMS_E_UNSUPPORTED, "a second PML layer operator");
+    if (o.type == OP_POLES || o.type == OP_MUR || o.type == OP_PW || o.type == OP_MERGED)
+      return fail(h, EMS_E_UNSUPPORTED,
+                  "register the PML layers before dispersive media, Mur boundaries, plane "
+                  "waves and cut-face currents (they run first among the slab corrections)");
+  }
+  const int cnt[2] = {d->n_e, d->n_h};
+  const int32_t *ax[2] = {d->e_axis, d->h_axis}, *nd[2] = {d->e_node, d->h_node};
+  const float *cf[2] = {d->e_coef, d->h_coef};
+  for (int s = 0; s < 2; s++)
+    if (cnt[s] && (!ax[s] || !nd[s] || !cf[s])) return fail(h, EMS_E_INVALID, "PML arrays");
+  const uint64_t N = (uint64_t)h->n[0] * h->n[1] * h->n[2];
+  // Grouped by (side, axis), each group in the given order.
+  std::vector<int> order[6];
+  std::vector<uint64_t> flat[2], keys;
+  for (int s = 0; s < 2; s++) {
+    flat[s].resize(cnt[s]);
+    for (int q = 0; q < cnt[s]; q++) {
+      const int a = ax[s][q];
+      const int32_t *p = nd[s] + 3 * (size_t)q;
+      if (a < 0 || a > 2 || !node_ok(h, p)) return fail(h, EMS_E_INVALID, "PML edge out of range");
+      for (int c = 0; c < 4; c++)
+        if (!std::isfinite(cf[s][4 * (size_t)q + c]))
+          return fail(h, EMS_E_INVALID, "PML coefficient not finite");
+      flat[s][q] = ((uint64_t)p[0] * h->n[1] + p[1]) * h->n[2] + p[2];
+      order[3 * s + a].push_back(q);
+      keys.push_back((uint64_t)(3 * s + a) * N + flat[s][q]);
+    }
+  }
+  std::sort(keys.begin(), keys.end());
+  if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
+    return fail(h, EMS_E_INVALID, "PML edge listed twice");
+  HostOp o;
+  o.type = OP_UPML;
+  DevUpml &m = o.upml;
+  m.ne = cnt[0], m.nh = cnt[1];
+  const size_t n = (size_t)m.ne + m.nh;
+  std::vector<uint64_t> idx;
+  std::vector<float4> co;
+  idx.reserve(n), co.reserve(n);
+  m.off[0] = 0;
+  for (int g = 0; g < 6; g++) {
+    const int s = g / 3;
+    for (int q : order[g]) {
+      idx.push_back(flat[s][q]);
+      const float *c = cf[s] + 4 * (size_t)q;
+      co.push_back(make_float4(c[0], c[1], c[2], c[3]));
+    }
+    m.off[g + 1] = (int)idx.size();
+  }
+  int rc;
+  void *p = nullptr;
+  if ((rc = upload(h, &p, idx.data(), n * sizeof(uint64_t)))) return rc;
+  o.mem.push_back(p), m.idx = (const uint64_t *)p;
+  if ((rc = upload(h, &p, co.data(), n * sizeof(float4)))) return rc;
+  o.mem.push_back(p), m.co = (const float4 *)p;
+  std::vector<float2> zero(n, make_float2(0.0f, 0.0f));
+  if ((rc = upload(h, &p, zero.data(), n * sizeof(float2)))) return rc;
+  o.mem.push_back(p), m.fv = (float2 *)p;
+  // The caller's order of each side, for ems_gpu_upml_state.
+  std::vector<uint32_t> pos(n);
+  for (int g = 0; g < 6; g++)
+    for (size_t r = 0; r < order[g].size(); r++)
+      pos[(g < 3 ? 0 : m.ne) + order[g][r]] = (uint32_t)(m.off[g] + r);
+  o.upml_pos = std::move(pos);
+  h->ops.push_back(std::move(o));
+  h->ops_dirty = true;
+  drop_graphs(h);
+  if (id) *id = (int)h->ops.size() - 1;
+  return EMS_OK;
+}
+
+int ems_gpu_upml_state(ems_gpu *h, int id, int side, float *flux, float *value) {
+  ALIVE(h);
+  if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_UPML || side < 0 || side > 1)
+    return fail(h, EMS_E_INVALID, "not a PML layer operator");
+  const HostOp &o = h->ops[id];
+  const int cnt = side ? o.upml.nh : o.upml.ne;
+  if (cnt && (!flux || !value)) return EMS_E_INVALID;
+  const size_t n = (size_t)o.upml.ne + o.upml.nh;
+  std::vector<float2> fv(n);
+  CK(h, hipStreamSynchronize(h->stream));
+  if (n) CK(h, hipMemcpy(fv.data(), o.upml.fv, n * sizeof(float2), hipMemcpyDeviceToHost));
+  const size_t base = side ? o.upml.ne : 0;
+  for (int q = 0; q < cnt; q++) {
+    const float2 s = fv[o.upml_pos[base + q]];
+    flux[q] = s.x, value[q] = s.y;
+  }
   return EMS_OK;
 }
 
@@ -4781,6 +4955,7 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       if ((rc = merged_fork(h, f))) return rc;
       if ((rc = launch_h(h, f))) return rc;
       CK(h, hipEventRecord(ev[1], h->stream));
+      if ((rc = launch_upml(h, f, 0))) return rc;
       if ((rc = launch_cpml(h, f, 0, h->par))) return rc;
       if ((rc = launch_poles(h, 0, f, f, false))) return rc;
       if ((rc = launch_mur(h, f, 0))) return rc;
@@ -4789,6 +4964,7 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       CK(h, hipEventRecord(ev[2], h->stream));
       if ((rc = launch_e(h, f))) return rc;
       CK(h, hipEventRecord(ev[3], h->stream));
+      if ((rc = launch_upml(h, f, 1))) return rc;
       if ((rc = launch_cpml(h, f, 1, h->par))) return rc;
       if ((rc = launch_poles(h, 1, f, f, false))) return rc;
       if ((rc = launch_mur(h, f, 1))) return rc;
@@ -5105,7 +5281,7 @@ int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
       return bfail(b, EMS_E_UNSUPPORTED,
                    "model " + std::to_string(q) +
                        " needs the unfused schedule (H source, magnetic poles, plane wave, "
-                       "cut-face currents or more than 32 CPML slabs)");
+                       "cut-face currents, openEMS PML layers or more than 32 CPML slabs)");
     if (h->par != par) return bfail(b, EMS_E_INVALID, "models of a batch step in lockstep");
     const int rc = prepare(h, n, false);
     if (rc) return bfail(b, rc, h->err);

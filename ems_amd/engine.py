@@ -847,6 +847,13 @@ class Engine:
             oid = C.c_int()
             self._check(nat.ems_gpu_add_mur(h, C.byref(d), C.byref(oid)))
             self.setup.append(("mur", op))
+        elif type(op).__name__ == "OpenEmsPml":
+            d, keep = op.desc()
+            oid = C.c_int()
+            self._check(nat.ems_gpu_add_upml(h, C.byref(d), C.byref(oid)))
+            op._oid = getattr(op, "_oid", {})
+            op._oid[id(self)] = oid.value
+            self.setup.append(("upml", op))
         elif hasattr(op, "_plan"):
             # LumpedElement / LumpedPort (lumped.rs:880-1012): fold into the
             # lanes, re-upload them, register the device operator.
@@ -1025,6 +1032,15 @@ class Engine:
         self._check(nat.ems_gpu_probe_times(self._h, pid, nat.ptr(out, C.c_double), out.size,
                                             C.byref(cnt)))
         return out
+
+    def upml_state(self, op, side: int) -> Tuple[np.ndarray, np.ndarray]:
+        """The state (flux, value) of a registered OpenEmsPml's E edges
+        (side 0) or H dual edges (side 1), in its edge order."""
+        n = op.edge_counts()[side]
+        f, v = np.empty(n, np.float32), np.empty(n, np.float32)
+        self._check(nat.ems_gpu_upml_state(self._h, op._oid[id(self)], int(side),
+                                           nat.ptr(f, C.c_float), nat.ptr(v, C.c_float)))
+        return f, v
 
     # -- measurement
     def time_kernels(self, steps: int) -> Tuple[float, float, float]:

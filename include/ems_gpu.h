@@ -104,6 +104,39 @@ typedef struct {
 } ems_mur_desc;
 int ems_gpu_add_mur(ems_gpu *h, const ems_mur_desc *d, int *id);
 
+/* OpenEmsPml (src/engine/boundary/upml.rs:541-760): openEMS's PML_n layers,
+ * a two-stage uniaxial PML through a flux F per layer edge. The caller has
+ * rewritten the coefficients of every layer edge (ca = 0, cb = dt/((A/l)
+ * (1 + a_p)); da, db likewise), so the H and E updates leave the flux
+ * increment t in the field (OpenEmsPml::build, upml.rs:602-700). Per edge:
+ * axis, node[3] and four f32 coefficients c = ((1 - a_p)/(1 + a_p),
+ * (1 - a_q)/(1 + a_q), (1 + a_a)/((1 + a_q) m), (1 - a_a)/((1 + a_q) m)),
+ * the E edges first, then the H dual edges. After the H update every E
+ * edge takes its start value v from u and every H edge completes its
+ * update; after the E update the roles swap (Parts::post_h_slab /
+ * post_e_slab, upml.rs:442-470). The update (Run::apply, upml.rs:404-426),
+ * every f32 operation flushed:
+ *   F' = c0 F + t;  x = (c1 v - c3 F) + c2 F';  F = F', v = x, field = x.
+ * It runs first among the slab corrections, as the reference registers it
+ * (compile/mod.rs:1224-1236), so it is refused after dispersive media, Mur
+ * boundaries, plane waves or cut-face currents; it never shares an engine
+ * with CPML slabs (refused either way round). An edge listed twice and a
+ * second layer operator are refused. The steps run unfused (the H update
+ * has to be complete before the E update reads it). */
+typedef struct {
+    int32_t n_e;
+    const int32_t *e_axis, *e_node; /* node: 3 per edge */
+    const float *e_coef;            /* 4 per edge */
+    int32_t n_h;
+    const int32_t *h_axis, *h_node;
+    const float *h_coef;
+} ems_upml_desc;
+int ems_gpu_add_upml(ems_gpu *h, const ems_upml_desc *d, int *id);
+
+/* The layer state in the given edge order: side 0 the E edges, 1 the H
+ * dual edges; flux and value each hold the side's count of floats. */
+int ems_gpu_upml_state(ems_gpu *h, int id, int side, float *flux, float *value);
+
 /* EdgeSource (src/engine/excite.rs:10-173): kind 0 SoftE, 1 HardE,
  * 2 SoftH, 3 HardH; edges (axis, node[3], weight f32). The host tabulates
  * the amplitude `a(t) as f32` at each step's sample time (SoftE (n+1/2)dt,

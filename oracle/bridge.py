@@ -61,6 +61,9 @@ _ip2 = C.POINTER(C.c_int)
 orc_add_mur = _sig("orc_add_mur", _i, _p, _i, _ip2, _ip2, _ip2, C.POINTER(C.c_float),
                    C.POINTER(C.c_uint64), _i, _ip2, _ip2, _ip2, _ip2, C.POINTER(C.c_float),
                    C.POINTER(C.c_float), C.POINTER(C.c_uint64), _i, _i)
+orc_add_upml = _sig("orc_add_upml", _i, _p, _i, _ip2, _ip2, C.POINTER(C.c_float), _i, _ip2, _ip2,
+                    C.POINTER(C.c_float))
+orc_upml_state = _sig("orc_upml_state", _i, _p, _i, _i, C.POINTER(C.c_float), C.POINTER(C.c_float))
 orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
 orc_energy = _sig("orc_energy", _d, _p)
 orc_timestep = _sig("orc_timestep", _u64, _p)
@@ -196,6 +199,16 @@ class OracleEngine:
             assert r >= 0, "orc_add_mur failed"
             self.ops.append(op)
             return op
+        if type(op).__name__ == "OpenEmsPml":
+            (ea, en, ec), (ha, hn, hc) = op.e, op.h
+            I, F = C.c_int, C.c_float
+            r = orc_add_upml(h, ea.size, P(ea, I), P(en, I), P(ec, F), ha.size, P(ha, I), P(hn, I),
+                             P(hc, F))
+            assert r >= 0, "orc_add_upml refused the layers"
+            op._oid = getattr(op, "_oid", {})
+            op._oid[id(self)] = r
+            self.ops.append(op)
+            return op
         if isinstance(op, E.CpmlBoundary):
             for s in op.slabs:
                 lo = (C.c_int * 3)(*s["lo"])
@@ -287,6 +300,14 @@ class OracleEngine:
     def set_component(self, field, axis, values):
         v = np.ascontiguousarray(np.asarray(values, dtype=np.float32).reshape(self.grid.nodes()))
         assert orc_write(self._h, field, int(axis), P(v, C.c_float)) == 0
+
+    def upml_state(self, op, side):
+        n = op.edge_counts()[side]
+        f, v = np.empty(n, np.float32), np.empty(n, np.float32)
+        r = orc_upml_state(self._h, op._oid[id(self)], int(side), P(f, C.c_float),
+                           P(v, C.c_float))
+        assert r == 0
+        return f, v
 
     def probe_values(self, op):
         n = orc_probe_count(self._h, op._oid)

@@ -69,7 +69,10 @@ typedef struct {
     float *b, *a, *k, *psi;
 } cpml_t;
 
-enum { OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6, OP_MUR = 7 };
+enum {
+    OP_SOURCE = 1, OP_PROBE = 2, OP_LUMPED = 3, OP_POLES = 4, OP_PW = 5, OP_MERGED = 6, OP_MUR = 7,
+    OP_UPML = 8
+};
 
 /* MurBoundary (boundary/mur.rs:94-140): face edges (Edge) and domain edges
  * on two Mur faces (Corner), with the values saved before the E update. */
@@ -81,6 +84,21 @@ typedef struct {
     uint64_t *ehold, *chold;
     float *saved; /* 2 per edge, then 3 per corner */
 } mur_t;
+
+/* OpenEmsPml (boundary/upml.rs:541-760): per side (0 E edges, 1 H dual
+ * edges) the layer edges in the caller's order with their coefficients
+ * ((1-a_p)/(1+a_p), (1-a_q)/(1+a_q), (1+a_a)/((1+a_q)m), (1-a_a)/((1+a_q)m))
+ * and state (flux F, start value v), and the edges by x plane for the
+ * slabs. */
+typedef struct {
+    int n[2];
+    int *ax[2];
+    size_t *idx[2];
+    float *co[2];
+    float *f[2], *v[2];
+    int *perm[2];      /* edge numbers sorted by plane, stable */
+    size_t *pstart[2]; /* nx + 1 offsets into perm */
+} upml_t;
 
 /* Plane-wave corrections (Correction, planewave.rs:424-431), in list
  * order, and the auxiliary grid (PlaneWave, :458-483). */
@@ -163,6 +181,8 @@ typedef struct {
     struct merged *mf;
     /* Mur boundary */
     mur_t *mur;
+    /* openEMS's PML layers */
+    upml_t *upml;
 } op_t;
 
 /* MergedFaces (conformal.rs:2313-2340): the tables of orc_merged_desc and
@@ -482,6 +502,42 @@ static double side_energy(const poles_t *m, int f) {
     return w;
 }
 
+/* Run::refresh (upml.rs:391-400): side s's edges in planes [p0, p1) take
+ * their start value from the field. */
+static void upml_refresh(orc_engine *e, upml_t *m, int s, int p0, int p1) {
+    float *const *fld = s ? e->i : e->u;
+    for (size_t r = m->pstart[s][p0]; r < m->pstart[s][p1]; r++) {
+        const int q = m->perm[s][r];
+        m->v[s][q] = fld[m->ax[s][q]][m->idx[s][q]];
+    }
+}
+
+/* Run::apply (upml.rs:404-426): the update left the flux increment t in the
+ * field; F' = c0 F + t, x = (c1 v - c3 F) + c2 F', every operation flushed. */
+static void upml_apply(orc_engine *e, upml_t *m, int s, int p0, int p1) {
+    float *const *fld = s ? e->i : e->u;
+    for (size_t r = m->pstart[s][p0]; r < m->pstart[s][p1]; r++) {
+        const int q = m->perm[s][r];
+        float *x = &fld[m->ax[s][q]][m->idx[s][q]];
+        const float *c = m->co[s] + 4 * (size_t)q;
+        const float f0 = m->f[s][q];
+        const float f1 = flush(flush(c[0] * f0) + *x);
+        const float v = flush(flush(flush(c[1] * m->v[s][q]) - flush(c[3] * f0)) + flush(c[2] * f1));
+        m->f[s][q] = f1, m->v[s][q] = v, *x = v;
+    }
+}
+
+/* Parts::post_h_slab / post_e_slab (upml.rs:442-470): after the H update
+ * the E edges refresh and the H edges complete; after the E update the
+ * other way round. */
+static void upml_hook(orc_engine *e, int e_half, int p0, int p1) {
+    for (int q = 0; q < e->nops; q++) {
+        if (e->ops[q].type != OP_UPML) continue;
+        upml_refresh(e, e->ops[q].upml, e_half ? 1 : 0, p0, p1);
+        upml_apply(e, e->ops[q].upml, e_half ? 0 : 1, p0, p1);
+    }
+}
+
 static void phase(orc_engine *e, int cmd, int s) {
     int p0 = e->slab_start[s], p1 = e->slab_start[s + 1];
     if (cmd == 1) {
@@ -489,12 +545,14 @@ static void phase(orc_engine *e, int cmd, int s) {
             double w = h_plane(e, gi, e->energy_flag, 1);
             if (e->energy_flag) e->plane_energy[gi] = w;
         }
+        upml_hook(e, 0, p0, p1);
         for (int c = 0; c < e->ncpml; c++)
             if (!e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], p0, p1);
         for (int q = 0; q < e->nops; q++)
             if (e->ops[q].type == OP_POLES) poles_post_h(e, e->ops[q].pm, p0, p1);
     } else if (cmd == 2) {
         for (int gi = p0; gi < p1; gi++) e_plane(e, gi);
+        upml_hook(e, 1, p0, p1);
         for (int c = 0; c < e->ncpml; c++)
             if (e->cpml[c].e_side) cpml_apply(e, &e->cpml[c], p0, p1);
         for (int q = 0; q < e->nops; q++)
@@ -846,6 +904,7 @@ static int needs(const op_t *o, uint64_t n) {
     if (o->type == OP_LUMPED) return 1; /* pre_e / post_e every step */
     if (o->type == OP_POLES) return 0;  /* slab hooks only */
     if (o->type == OP_MUR) return 0;    /* slab hooks only (mur.rs:799-853) */
+    if (o->type == OP_UPML) return 0;   /* slab hooks only (upml.rs:442-500) */
     if (o->type == OP_PW || o->type == OP_MERGED) return 1; /* post_h and post_e every step */
     return n % o->interval == 0;
 }
@@ -927,7 +986,7 @@ static void one_step(orc_engine *e, int energy) {
      * fields of a slab (24 bytes per node) reach FUSE_MIN_BYTES = 4 MiB. */
     int fused = !energy && !gap_b && e->fuse != 2;
     for (int q = 0; q < e->nops && fused; q++)
-        if (e->ops[q].type == OP_POLES) fused = 0;
+        if (e->ops[q].type == OP_POLES || e->ops[q].type == OP_UPML) fused = 0;
     if (fused && e->fuse == 0) fused = 24 * e->N / (size_t)e->threads >= ((size_t)4 << 20);
     if (fused) {
         for (int q = 0; q < e->nops; q++)
@@ -1066,6 +1125,19 @@ void orc_destroy(orc_engine *e) {
             }
             free(m->a), free(m->inv_c), free(m->inv_z), free(m->half_l), free(m->kind), free(m);
         }
+        mur_t *mu = o->mur;
+        if (mu) {
+            free(mu->eax), free(mu->cax), free(mu->enode), free(mu->einner), free(mu->cnode);
+            free(mu->cin0), free(mu->cin1), free(mu->ek), free(mu->ck0), free(mu->ck1);
+            free(mu->ehold), free(mu->chold), free(mu->saved), free(mu);
+        }
+        upml_t *up = o->upml;
+        if (up) {
+            for (int t = 0; t < 2; t++)
+                free(up->ax[t]), free(up->idx[t]), free(up->co[t]), free(up->f[t]), free(up->v[t]),
+                    free(up->perm[t]), free(up->pstart[t]);
+            free(up);
+        }
         lumped_t *L = o->lp;
         if (L) {
             for (int t = 0; t < L->nterm + L->ndrive; t++) free(L->tv[t]);
@@ -1113,6 +1185,8 @@ int orc_add_cpml(orc_engine *e, int e_side, int comp, int normal, const int lo[3
     if (comp == normal || comp < 0 || comp > 2 || normal < 0 || normal > 2) return -1;
     for (int d = 0; d < 3; d++)
         if (lo[d] < 0 || hi[d] > e->n[d] || lo[d] > hi[d]) return -1;
+    for (int q = 0; q < e->nops; q++)
+        if (e->ops[q].type == OP_UPML) return -1;
     e->cpml = (cpml_t *)realloc(e->cpml, (e->ncpml + 1) * sizeof(cpml_t));
     cpml_t *s = &e->cpml[e->ncpml++];
     memset(s, 0, sizeof *s);
@@ -1534,6 +1608,59 @@ int orc_add_mur(orc_engine *e, int ne, const int *eax, const int *enode, const i
     }
 #undef FLAT
     return id;
+}
+
+int orc_add_upml(orc_engine *e, int ne, const int *eax, const int *enode, const float *eco,
+                 int nh, const int *hax, const int *hnode, const float *hco) {
+    if (ne < 0 || nh < 0) return -1;
+    if (e->ncpml) return -1;
+    for (int q = 0; q < e->nops; q++) {
+        const int t = e->ops[q].type;
+        if (t == OP_UPML || t == OP_POLES || t == OP_MUR || t == OP_PW || t == OP_MERGED) return -1;
+    }
+    const int cnt[2] = {ne, nh};
+    const int *ax[2] = {eax, hax}, *nd[2] = {enode, hnode};
+    const float *co[2] = {eco, hco};
+    for (int s = 0; s < 2; s++)
+        for (int q = 0; q < cnt[s]; q++)
+            if (ax[s][q] < 0 || ax[s][q] > 2 || !node_ok(e, nd[s] + 3 * q)) return -1;
+    int id = add_op(e);
+    op_t *o = &e->ops[id];
+    o->type = OP_UPML;
+    upml_t *m = (upml_t *)calloc(1, sizeof(upml_t));
+    o->upml = m;
+    for (int s = 0; s < 2; s++) {
+        const int n = cnt[s];
+        m->n[s] = n;
+        m->ax[s] = (int *)calloc(n + 1, sizeof(int));
+        m->idx[s] = (size_t *)calloc(n + 1, sizeof(size_t));
+        m->co[s] = (float *)calloc(4 * (size_t)n + 1, sizeof(float));
+        m->f[s] = (float *)calloc(n + 1, sizeof(float));
+        m->v[s] = (float *)calloc(n + 1, sizeof(float));
+        m->perm[s] = (int *)calloc(n + 1, sizeof(int));
+        m->pstart[s] = (size_t *)calloc(e->n[0] + 1, sizeof(size_t));
+        for (int q = 0; q < n; q++) {
+            const int *p = nd[s] + 3 * q;
+            m->ax[s][q] = ax[s][q];
+            m->idx[s][q] = ((size_t)p[0] * e->n[1] + p[1]) * e->n[2] + p[2];
+            memcpy(m->co[s] + 4 * (size_t)q, co[s] + 4 * (size_t)q, 4 * sizeof(float));
+            m->pstart[s][p[0] + 1]++;
+        }
+        for (int g = 0; g < e->n[0]; g++) m->pstart[s][g + 1] += m->pstart[s][g];
+        size_t *fill = (size_t *)calloc(e->n[0] + 1, sizeof(size_t));
+        memcpy(fill, m->pstart[s], e->n[0] * sizeof(size_t));
+        for (int q = 0; q < n; q++) m->perm[s][fill[nd[s][3 * q]]++] = q;
+        free(fill);
+    }
+    return id;
+}
+
+int orc_upml_state(orc_engine *e, int id, int side, float *flux, float *value) {
+    if (id < 0 || id >= e->nops || e->ops[id].type != OP_UPML || side < 0 || side > 1) return -1;
+    const upml_t *m = e->ops[id].upml;
+    memcpy(flux, m->f[side], m->n[side] * sizeof(float));
+    memcpy(value, m->v[side], m->n[side] * sizeof(float));
+    return 0;
 }
 
 void orc_set_fused(orc_engine *e, int mode) { e->fuse = mode < 0 || mode > 2 ? 0 : mode; }

@@ -182,6 +182,16 @@ int orc_add_mur(orc_engine *e, int ne, const int *eax, const int *enode, const i
                 const float *ek, const uint64_t *ehold, int nc, const int *cax, const int *cnode,
                 const int *cin0, const int *cin1, const float *ck0, const float *ck1,
                 const uint64_t *chold, int oe_order, int oe_corners);
+/* OpenEmsPml (boundary/upml.rs:541-760): the layer edges, E then H dual
+ * edges, each with axis, node[3] and four coefficients
+ * ((1-a_p)/(1+a_p), (1-a_q)/(1+a_q), (1+a_a)/((1+a_q)m), (1-a_a)/((1+a_q)m));
+ * the caller has already rewritten their update coefficients. Refused
+ * with CPML slabs, after dispersive media, Mur boundaries, plane waves,
+ * cut-face currents or another layer operator. Returns the id or -1. */
+int orc_add_upml(orc_engine *e, int ne, const int *eax, const int *enode, const float *eco,
+                 int nh, const int *hax, const int *hnode, const float *hco);
+/* The layer state (flux, value) of side 0 (E) or 1 (H), in edge order. */
+int orc_upml_state(orc_engine *e, int id, int side, float *flux, float *value);
 double orc_step_with_energy(orc_engine *e);
 double orc_energy(orc_engine *e);
 uint64_t orc_timestep(orc_engine *e);
