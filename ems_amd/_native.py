@@ -83,7 +83,9 @@ class RunLimitsC(C.Structure):
                 ("has_excitation_end", C.c_int32), ("excitation_end", C.c_double),
                 ("has_settle", C.c_int32), ("settle_frequency", C.c_double),
                 ("settle_tolerance", C.c_double), ("settle_window", C.c_double),
-                ("settle_max_extension", C.c_double)]
+                ("settle_max_extension", C.c_double), ("check_after_interval", C.c_int32),
+                ("has_extra_check", C.c_int32), ("extra_check", C.c_uint64),
+                ("energy_measure", C.c_int32)]
 
 
 class RunOutcomeC(C.Structure):
@@ -170,6 +172,8 @@ ems_gpu_copy_bandwidth = _sig("ems_gpu_copy_bandwidth", _i, _p, _u64, _i, _dp)
 ems_courant_limit = _sig("ems_courant_limit", _d, _dp, _i, _dp, _i, _dp, _i)
 ems_coefficients_from_cells = _sig("ems_coefficients_from_cells", _i, _dp, _i, _dp, _i, _dp, _i,
                                    _u16p, _dp, _i, _u8p, _ip, _d, _i, _i, _fp)
+ems_edge_materials_from_cells = _sig("ems_edge_materials_from_cells", _i, _dp, _i, _dp, _i, _dp,
+                                     _i, _u16p, _dp, _i, _i, _dp, _dp)
 ems_pool_lane = _sig("ems_pool_lane", _i, _fp, _i, _i, _i, _fp, _u32p, C.POINTER(_u64))
 ems_cpml_coefficients = _sig("ems_cpml_coefficients", None, C.POINTER(CpmlParams), _d, _d, _d,
                              _d, _dp)

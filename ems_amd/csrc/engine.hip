@@ -85,6 +85,18 @@ __device__ __forceinline__ double weight(float a, double p, float b) {
   return b != 0.0f ? w : 0.0;
 }
 
+// src/constants.rs
+constexpr double C0_D = 299792458.0;
+constexpr double MU0_D = 1.25663706127e-6;
+constexpr double EPS0_D = 1.0 / (MU0_D * C0_D * C0_D);
+
+// oe_square (kernel.rs:198-210): x^2 in f64, infinite where its f32 would
+// be (round to nearest: from the midpoint between FLT_MAX and 2^128).
+__device__ __forceinline__ double oe_square(double x) {
+  const double s = x * x;
+  return s >= 0x1.ffffffp+127 ? __builtin_huge_val() : s;
+}
+
 // One coefficient lane: rows + per-(i,j) row index (pooled), rows only
 // (dense, index == nullptr) or nothing (all zero).
 struct Lane {
@@ -2301,7 +2313,7 @@ __global__ void __launch_bounds__(256) dft_run(const double *__restrict__ x, uin
 // i^n i^(n+1), the update evaluated read-only), combined as
 // (a0 + a1) + (a2 + a3). Rounds of 64 NT terms, NT per lane (NT sized to
 // the rows: 1 for rows of at most 64 nodes, up to 8).
-template <bool OE, int NT>
+template <bool OE, int NT, bool MEAS>
 __global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, int ny, int nz,
                                                      double *rowsum) {
   constexpr int RND = 64 * NT;
@@ -2345,8 +2357,13 @@ __global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, 
 #pragma unroll
     for (int s = 0; s < NT; s++) {
       const float nw = h_expr<OE>(da[s], old[s], db[s], va[s], vb[s], vc[s], vd[s]);
-      b[64 * s + t] = k0 + 64 * s + t < n ? weight(da[s], (double)old[s] * (double)nw, db[s])
-                                          : 0.0;
+      const int k = k0 + 64 * s + t;
+      double term;
+      if (MEAS)  // openEMS's measure: the ends of a z row weigh 4 (kernel.rs:153-156, 497-507)
+        term = (comp == 2 && (k == 0 || k == n - 1) ? 4.0 : 1.0) * oe_square((double)nw);
+      else
+        term = weight(da[s], (double)old[s] * (double)nw, db[s]);
+      b[64 * s + t] = k < n ? term : 0.0;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -2355,13 +2372,19 @@ __global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, 
     __builtin_amdgcn_wave_barrier();
   }
   const double a1 = __shfl(acc, 1), a2 = __shfl(acc, 2), a3 = __shfl(acc, 3);
-  if (t == 0) rowsum[rid] = (acc + a1) + (a2 + a3);
+  // openEMS's measure: x rows of the outer x planes and y rows j = 0, ny - 1
+  // weigh 4 (h_plane, kernel.rs:457-493)
+  const double rw = MEAS && ((comp == 0 && (gi == 0 || gi + 1 == nx)) ||
+                             (comp == 1 && (j == 0 || j + 1 == ny)))
+                        ? 4.0 : 1.0;
+  if (t == 0) rowsum[rid] = MEAS ? rw * ((acc + a1) + (a2 + a3)) : (acc + a1) + (a2 + a3);
 }
 
 // E part: per (plane, component) four chains over the in-plane index
 // q = 0, 1, 2, 3 mod 4 in increasing q (e_energy_row, kernel.rs:218-244),
 // a wave per (plane, component), lanes 0..3 the chains. Rounds of 1024
 // terms, 16 per lane.
+template <bool MEAS>
 __global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx, int ny, int nz,
                                                        double *chains) {
   constexpr int NT = 16, RND = 64 * NT;
@@ -2377,6 +2400,7 @@ __global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx
   double *b = buf[wv];
   for (size_t q0 = 0; q0 < plane; q0 += RND) {
     float x[NT], ca[NT], cb[NT];
+    bool in[NT];
 #pragma unroll
     for (int s = 0; s < NT; s++) {
       const size_t q = min(q0 + 64 * s + t, plane - 1);
@@ -2384,12 +2408,20 @@ __global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx
       const size_t r = (size_t)gi * ny + j;
       const size_t o = (size_t)gi * plane + q;
       x[s] = u[q];
-      ca[s] = coef(la, r, k, nz, o), cb[s] = coef(lb, r, k, nz, o);
+      // openEMS's measure leaves out the E on the three upper faces
+      // (kernel.rs:410-418); the zero terms leave the chains unchanged.
+      in[s] = gi + 1 < nx && j + 1 < ny && k + 1 < nz;
+      if (!MEAS) ca[s] = coef(la, r, k, nz, o), cb[s] = coef(lb, r, k, nz, o);
     }
 #pragma unroll
     for (int s = 0; s < NT; s++) {
       const double xd = (double)x[s];
-      b[64 * s + t] = q0 + 64 * s + t < plane ? weight(ca[s], xd * xd, cb[s]) : 0.0;
+      double term;
+      if (MEAS)
+        term = in[s] ? oe_square(xd) : 0.0;
+      else
+        term = weight(ca[s], xd * xd, cb[s]);
+      b[64 * s + t] = q0 + 64 * s + t < plane ? term : 0.0;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -2402,6 +2434,7 @@ __global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx
 
 // Plane values we + wh (h_slab, kernel.rs:404-420): a wave per plane, the
 // plane's row sums added in row order by lane 0 from LDS.
+template <bool MEAS>
 __global__ void __launch_bounds__(256) energy_planes(int nx, int ny, const double *rowsum,
                                                      const double *chains, double *planes) {
   __shared__ double buf[4][256];
@@ -2434,12 +2467,109 @@ __global__ void __launch_bounds__(256) energy_planes(int nx, int ny, const doubl
       const double *c = chains + ((size_t)gi * 3 + a) * 4;
       we += (c[0] + c[1]) + (c[2] + c[3]);
     }
-    planes[gi] = we + wh;
+    planes[gi] = MEAS ? EPS0_D * we + MU0_D * wh : we + wh;
   }
+}
+
+// Parts::energy_openems_slab (upml.rs:505-537): per H run of the layers
+// (component, plane) mu0 sum (v^2 - inc^2) over its edges in index order,
+// inc the kernel's flux increment evaluated in the Paired order as written
+// there, v the completed layer value; added to the plane in component
+// order. A wave per plane, the terms a round at a time into LDS, lane 0
+// adding them in order. hplane[c (nx + 1) + gi]: the plane's first H edge
+// of component c (device order: by component, then node index).
+__global__ void __launch_bounds__(256) upml_energy_planes(Fields f, Lanes L, DevUpml m,
+                                                          const uint32_t *hplane, int nx, int ny,
+                                                          int nz, double *planes) {
+  __shared__ double buf[4][256];
+  const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
+  const int gi = (int)blockIdx.x * 4 + wv;
+  if (gi >= nx) return;  // uniform per wave
+  const size_t plane = (size_t)ny * nz;
+  double *b = buf[wv];
+  double pv = t == 0 ? planes[gi] : 0.0;
+  for (int comp = 0; comp < 3; comp++) {
+    const uint32_t r0 = hplane[comp * (nx + 1) + gi], r1 = hplane[comp * (nx + 1) + gi + 1];
+    if (r0 == r1) continue;  // no run
+    const Lane ldb = pick_lane(L, 9 + comp);
+    double sum = 0.0;
+    for (uint32_t q0 = r0; q0 < r1; q0 += 256) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const uint32_t q = q0 + 64 * s + t;
+        double term = 0.0;
+        if (q < r1) {
+          const size_t idx = m.idx[q], o = idx - (size_t)gi * plane;
+          const int j = (int)(o / nz), k = (int)(o - (size_t)j * nz);
+          const float *ux = f.u[0], *uy = f.u[1], *uz = f.u[2];
+          float c;
+          if (comp == 0)
+            c = (uz[idx + nz] - uz[idx]) - (uy[idx + 1] - uy[idx]);
+          else if (comp == 1)
+            c = (ux[idx + 1] - ux[idx]) - (uz[idx + plane] - uz[idx]);
+          else
+            c = (uy[idx + plane] - uy[idx]) - (ux[idx + nz] - ux[idx]);
+          const float db = coef(ldb, (size_t)gi * ny + j, k, nz, idx);
+          const float2 st = m.fv[q];
+          const float4 co = m.co[q];
+          const float inc = flush(0.0f * st.y - db * c);
+          const float f1 = flush(flush(co.x * st.x) + inc);
+          const float v = flush(flush(flush(co.y * st.y) - flush(co.w * st.x)) + flush(co.z * f1));
+          term = (double)v * (double)v - (double)inc * (double)inc;
+        }
+        b[64 * s + t] = term;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      if (t == 0) sum = chain_add(sum, b, 0, (int)min(256u, r1 - q0), 1);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+    if (t == 0) pv += MU0_D * sum;
+  }
+  if (t == 0) planes[gi] = pv;
+}
+
+// The kernel's look-ahead i^{n+1/2} of listed dual edges (dump.rs:1830-1845),
+// completed by the layers where the edge is one of theirs (binary search
+// in the layers' H edges of its component, sorted by node index).
+template <bool OE>
+__global__ void __launch_bounds__(64) lookahead_edges(Fields f, Lanes L, int n, const int32_t *axis,
+                                                      const uint64_t *idx, int nx, int ny, int nz,
+                                                      DevUpml m, int has_upml, float *out) {
+  const int q = blockIdx.x * 64 + threadIdx.x;
+  if (q >= n) return;
+  const int c = axis[q], a = (c + 1) % 3, bb = (c + 2) % 3;
+  const uint64_t o = idx[q];
+  const int p[3] = {(int)(o / ((uint64_t)ny * nz)), (int)((o / nz) % ny), (int)(o % nz)};
+  const int nn[3] = {nx, ny, nz};
+  const size_t st[3] = {(size_t)ny * nz, (size_t)nz, 1};
+  const float *ua = pick3(f.u, a), *ub = pick3(f.u, bb);
+  const float t0 = p[a] + 1 < nn[a] ? ub[o + st[a]] : 0.0f, t1 = ub[o];
+  const float t2 = p[bb] + 1 < nn[bb] ? ua[o + st[bb]] : 0.0f, t3 = ua[o];
+  const size_t r = (size_t)p[0] * ny + p[1];
+  const float da = coef(pick_lane(L, 6 + c), r, p[2], nz, o);
+  const float db = coef(pick_lane(L, 9 + c), r, p[2], nz, o);
+  float k = h_expr<OE>(da, pick3(f.i, c)[o], db, t0, t1, t2, t3);
+  if (has_upml) {
+    int lo = m.off[3 + c], hi = m.off[4 + c];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (m.idx[mid] < o) lo = mid + 1; else hi = mid;
+    }
+    if (lo < m.off[4 + c] && m.idx[lo] == o) {
+      const float2 s = m.fv[lo];
+      const float4 co = m.co[lo];
+      const float f1 = flush(flush(co.x * s.x) + k);
+      k = flush(flush(flush(co.y * s.y) - flush(co.w * s.x)) + flush(co.z * f1));
+    }
+  }
+  out[q] = k;
 }
 
 // The plane values summed in plane order (pool.rs:675-692), folding from
 // -0.0, scaled by dt/4; one wave.
+template <bool MEAS>
 __global__ void __launch_bounds__(64) energy_combine(int nx, const double *planes, double dt,
                                                      double *out) {
   __shared__ double buf[256];
@@ -2455,7 +2585,7 @@ __global__ void __launch_bounds__(64) energy_combine(int nx, const double *plane
     if (t == 0) s = chain_add(s, buf, 0, min(256, nx - g0), 1);
     __syncthreads();
   }
-  if (t == 0) *out = 0.25 * dt * s;
+  if (t == 0) *out = MEAS ? s : 0.25 * dt * s;  // scale_energy (pool.rs:684-689)
 }
 
 // ------------------------------------------------------------------ host --
@@ -2513,6 +2643,11 @@ struct HostOp {
   // openEMS's PML layers; upml_pos: device slot of each edge in the caller's order
   DevUpml upml{};
   std::vector<uint32_t> upml_pos;
+  uint32_t *d_hplane = nullptr;  // per H component the first edge of each x plane
+  // sources: host copies of the edges, weights and amplitude table
+  std::vector<int32_t> h_axis;
+  std::vector<uint64_t> h_idx;
+  std::vector<float> h_wf, h_amp;
 };
 
 // Host copies of a reduced-current operator's tables (its energy).
@@ -3499,16 +3634,17 @@ size_t energy_scratch_doubles(const ems_gpu *h) {
 
 // The invariant's kernels on the engine's stream, with the scratch arrays
 // given; W^n lands in *d_w (device memory).
-int energy_kernels(ems_gpu *h, double *rowsum, double *chains, double *planes, double *d_w) {
+template <bool MEAS>
+int energy_kernels_t(ems_gpu *h, double *rowsum, double *chains, double *planes, double *d_w) {
   const int nx = h->n[0], ny = h->n[1], nz = h->n[2];
   const size_t rows = (size_t)nx * (3 * ny - 2);
   const Fields &f = h->fb[h->par];
   // a wave per H row, per (plane, component) E chains and per plane
   dim3 g1((unsigned)((rows + 3) / 4));
   const int nt = nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8));
-#define EH(OE, NT)                                                                           \
-  hipLaunchKernelGGL((energy_h_rows<OE, NT>), g1, dim3(256), 0, h->stream, f, lanes_arg(h), nx, \
-                     ny, nz, rowsum)
+#define EH(OE, NT)                                                                        \
+  hipLaunchKernelGGL((energy_h_rows<OE, NT, MEAS>), g1, dim3(256), 0, h->stream, f,       \
+                     lanes_arg(h), nx, ny, nz, rowsum)
   if (h->oe) {
     if (nt == 1) EH(true, 1); else if (nt == 2) EH(true, 2); else if (nt == 4) EH(true, 4);
     else EH(true, 8);
@@ -3519,21 +3655,39 @@ int energy_kernels(ems_gpu *h, double *rowsum, double *chains, double *planes, d
 #undef EH
   CK(h, hipGetLastError());
   dim3 g2((unsigned)(((size_t)nx * 3 + 3) / 4));
-  hipLaunchKernelGGL(energy_e_chains, g2, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny, nz,
-                     chains);
+  hipLaunchKernelGGL(energy_e_chains<MEAS>, g2, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny,
+                     nz, chains);
   CK(h, hipGetLastError());
-  hipLaunchKernelGGL(energy_planes, dim3((unsigned)((nx + 3) / 4)), dim3(256), 0, h->stream, nx,
-                     ny, rowsum, chains, planes);
+  const dim3 gp((unsigned)((nx + 3) / 4));
+  hipLaunchKernelGGL(energy_planes<MEAS>, gp, dim3(256), 0, h->stream, nx, ny, rowsum, chains,
+                     planes);
   CK(h, hipGetLastError());
-  hipLaunchKernelGGL(energy_combine, dim3(1), dim3(64), 0, h->stream, nx, planes, h->dt, d_w);
+  if (MEAS)
+    for (auto &o : h->ops)
+      if (o.type == OP_UPML && o.upml.nh) {
+        hipLaunchKernelGGL(upml_energy_planes, gp, dim3(256), 0, h->stream, f, lanes_arg(h), o.upml,
+                           (const uint32_t *)o.d_hplane, nx, ny, nz, planes);
+        CK(h, hipGetLastError());
+      }
+  hipLaunchKernelGGL(energy_combine<MEAS>, dim3(1), dim3(64), 0, h->stream, nx, planes, h->dt,
+                     d_w);
   CK(h, hipGetLastError());
   return 0;
+}
+
+// The energy's kernels on the engine's stream, with the scratch arrays
+// given: the invariant (measure 0) or the field part of openEMS's measure
+// (1, with the layers' corrections); the value lands in *d_w (device).
+int energy_kernels(ems_gpu *h, double *rowsum, double *chains, double *planes, double *d_w,
+                   int measure = 0) {
+  return measure ? energy_kernels_t<true>(h, rowsum, chains, planes, d_w)
+                 : energy_kernels_t<false>(h, rowsum, chains, planes, d_w);
 }
 
 // The invariant's kernels and the copy of W^n into pinned host memory
 // (h->h_w), queued on the engine's stream; energy_now waits for it. The
 // scratch is the engine's own, allocated at the first evaluation.
-int energy_enqueue(ems_gpu *h) {
+int energy_enqueue(ems_gpu *h, int measure = 0) {
   const int nx = h->n[0], ny = h->n[1];
   const size_t rows = (size_t)nx * (3 * ny - 2);
   if (!h->d_rowsum) {
@@ -3543,14 +3697,14 @@ int energy_enqueue(ems_gpu *h) {
     CK(h, hipMalloc(&h->d_w, sizeof(double)));
     CK(h, hipHostMalloc((void **)&h->h_w, sizeof(double), hipHostMallocDefault));
   }
-  int rc = energy_kernels(h, h->d_rowsum, h->d_chains, h->d_planes, h->d_w);
+  int rc = energy_kernels(h, h->d_rowsum, h->d_chains, h->d_planes, h->d_w, measure);
   if (rc) return rc;
   CK(h, hipMemcpyAsync(h->h_w, h->d_w, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   return 0;
 }
 
-int energy_now(ems_gpu *h, double *w) {
-  int rc = energy_enqueue(h);
+int energy_now(ems_gpu *h, double *w, int measure = 0) {
+  int rc = energy_enqueue(h, measure);
   if (rc) return rc;
   CK(h, hipStreamSynchronize(h->stream));
   *w = *h->h_w;
@@ -3885,6 +4039,10 @@ int ems_gpu_add_upml(ems_gpu *h, const ems_upml_desc *d, int *id) {
   std::sort(keys.begin(), keys.end());
   if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
     return fail(h, EMS_E_INVALID, "PML edge listed twice");
+  // each group by node index (the layer runs' order, upml.rs:290-330)
+  for (int g = 0; g < 6; g++)
+    std::stable_sort(order[g].begin(), order[g].end(),
+                     [&](int x, int y) { return flat[g / 3][x] < flat[g / 3][y]; });
   HostOp o;
   o.type = OP_UPML;
   DevUpml &m = o.upml;
@@ -3918,6 +4076,17 @@ int ems_gpu_add_upml(ems_gpu *h, const ems_upml_desc *d, int *id) {
     for (size_t r = 0; r < order[g].size(); r++)
       pos[(g < 3 ? 0 : m.ne) + order[g][r]] = (uint32_t)(m.off[g] + r);
   o.upml_pos = std::move(pos);
+  const int nx = h->n[0];
+  const uint64_t plane = (uint64_t)h->n[1] * h->n[2];
+  std::vector<uint32_t> hp(3 * (size_t)(nx + 1));
+  for (int c = 0; c < 3; c++)
+    for (int gi = 0; gi <= nx; gi++)
+      hp[c * (size_t)(nx + 1) + gi] =
+          (uint32_t)(std::lower_bound(idx.begin() + m.off[3 + c], idx.begin() + m.off[4 + c],
+                                      (uint64_t)gi * plane) -
+                     idx.begin());
+  if ((rc = upload(h, &p, hp.data(), hp.size() * sizeof(uint32_t)))) return rc;
+  o.mem.push_back(p), o.d_hplane = (uint32_t *)p;
   h->ops.push_back(std::move(o));
   h->ops_dirty = true;
   drop_graphs(h);
@@ -3967,6 +4136,12 @@ int ems_gpu_add_source(ems_gpu *h, int kind, int n_edges, const int32_t *axis,
   if ((rc = upload(h, (void **)&o.d_wf, w, n_edges * sizeof(float)))) return rc;
   if ((rc = upload(h, (void **)&o.d_amp, amp, n_amp * sizeof(float)))) return rc;
   o.namp = n_amp;
+  if (kind == 2 || kind == 3) {  // for the H sources' share of openEMS's energy
+    o.h_axis.assign(axis, axis + n_edges);
+    o.h_idx = idx;
+    o.h_wf.assign(w, w + n_edges);
+    o.h_amp.assign(amp, amp + n_amp);
+  }
   h->ops.push_back(o);
   h->ops_dirty = true;
   if (id) *id = (int)h->ops.size() - 1;
@@ -4776,14 +4951,93 @@ int ems_gpu_energy(ems_gpu *h, double *w) {
   return EMS_OK;
 }
 
+// lookahead_source_energy (dump.rs:1735-1782): mu0 sum w ((k + s)^2 - k^2)
+// over the H sources' dual edges (sorted by axis and node, without
+// repeats; w = 4 on the outer planes normal to the edge), k the kernel's
+// look-ahead (completed by the layers on their edges, on the device), k + s
+// with the next step's H sources applied in registration order.
+static int hsource_energy(ems_gpu *h, double *out) {
+  *out = 0.0;
+  std::vector<std::pair<int32_t, uint64_t>> ed;
+  for (auto &o : h->ops)
+    if (o.type == OP_SOURCE && (o.kind == 2 || o.kind == 3))
+      for (size_t q = 0; q < o.h_idx.size(); q++) ed.push_back({o.h_axis[q], o.h_idx[q]});
+  if (ed.empty()) return 0;
+  std::sort(ed.begin(), ed.end());
+  ed.erase(std::unique(ed.begin(), ed.end()), ed.end());
+  const int n = (int)ed.size();
+  std::vector<int32_t> ax(n);
+  std::vector<uint64_t> ix(n);
+  for (int q = 0; q < n; q++) ax[q] = ed[q].first, ix[q] = ed[q].second;
+  const DevUpml *pm = nullptr;
+  for (auto &o : h->ops)
+    if (o.type == OP_UPML) pm = &o.upml;
+  int32_t *d_ax = nullptr;
+  uint64_t *d_ix = nullptr;
+  float *d_k = nullptr;
+  int rc = 0;
+  auto cleanup = [&]() {
+    if (d_ax) (void)hipFree(d_ax);
+    if (d_ix) (void)hipFree(d_ix);
+    if (d_k) (void)hipFree(d_k);
+  };
+  if ((rc = upload(h, (void **)&d_ax, ax.data(), n * sizeof(int32_t))) ||
+      (rc = upload(h, (void **)&d_ix, ix.data(), n * sizeof(uint64_t)))) {
+    cleanup();
+    return rc;
+  }
+  if (hipMalloc(&d_k, n * sizeof(float)) != hipSuccess) {
+    cleanup();
+    return fail(h, EMS_E_DEVICE, "hipMalloc (H source look-ahead)");
+  }
+  const Fields &f = h->fb[h->par];
+  const DevUpml m = pm ? *pm : DevUpml{};
+  if (h->oe)
+    hipLaunchKernelGGL(lookahead_edges<true>, dim3((n + 63) / 64), dim3(64), 0, h->stream, f,
+                       lanes_arg(h), n, d_ax, d_ix, h->n[0], h->n[1], h->n[2], m, pm ? 1 : 0, d_k);
+  else
+    hipLaunchKernelGGL(lookahead_edges<false>, dim3((n + 63) / 64), dim3(64), 0, h->stream, f,
+                       lanes_arg(h), n, d_ax, d_ix, h->n[0], h->n[1], h->n[2], m, pm ? 1 : 0, d_k);
+  std::vector<float> k(n);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(k.data(), d_k, n * sizeof(float), hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess) return fail(h, EMS_E_DEVICE, hipGetErrorString(e));
+  double sum = -0.0;
+  for (int q = 0; q < n; q++) {
+    const int c = ax[q];
+    float v = k[q];
+    for (auto &o : h->ops) {
+      if (!(o.type == OP_SOURCE && (o.kind == 2 || o.kind == 3))) continue;
+      const float a = h->step < o.h_amp.size() ? o.h_amp[h->step] : 0.0f;
+      for (size_t r = 0; r < o.h_idx.size(); r++)
+        if (o.h_axis[r] == c && o.h_idx[r] == ix[q]) {
+          const float x = o.h_wf[r] * a;
+          v = o.kind == 3 ? x : v + x;
+        }
+    }
+    const uint64_t o_ = ix[q];
+    const int p[3] = {(int)(o_ / ((uint64_t)h->n[1] * h->n[2])), (int)((o_ / h->n[2]) % h->n[1]),
+                      (int)(o_ % h->n[2])};
+    const double wt = (p[c] == 0 || p[c] + 1 == h->n[c]) ? 4.0 : 1.0;
+    sum += (double)v * (double)v * wt - (double)k[q] * (double)k[q] * wt;
+  }
+  *out = MU0_D * sum;
+  return 0;
+}
+
 int ems_gpu_energy_in(ems_gpu *h, int measure, double *w) {
   ALIVE(h);
   if (!w || measure < 0 || measure > 1) return EMS_E_INVALID;
-  if (measure == 1)
-    return fail(h, EMS_E_UNSUPPORTED,
-                "energy_in: openEMS's measure (EnergyMeasure::OpenEms) is not held by the "
-                "device engine; the end criterion of the openEMS-mode runs needs the CPU engine");
-  return ems_gpu_energy(h, w);
+  if (measure == 0) return ems_gpu_energy(h, w);
+  // Engine::energy_in(EnergyMeasure::OpenEms) (mod.rs:747-752)
+  double wf = 0.0, ws = 0.0;
+  int rc = energy_now(h, &wf, 1);
+  if (rc) return rc;
+  if ((rc = hsource_energy(h, &ws))) return rc;
+  *w = wf + ws;
+  return EMS_OK;
 }
 
 int ems_gpu_energy_many(ems_gpu *const *models, int n, const uint8_t *active, double *out) {

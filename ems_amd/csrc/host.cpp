@@ -87,6 +87,62 @@ double ems_courant_limit(const double *x, int nx, const double *y, int ny, const
   return 1.0 / (C0 * std::sqrt(s));
 }
 
+extern "C++" {
+// EdgeMaterials::from_cells for the E edge along a at node p
+// (edge_material.rs:396-401): the cells' capacitance and conductance
+// around it (primary_edge_cg, coeffs.rs:264-295) back to eps_r, kappa.
+template <class Cell>
+void edge_eps(const Grid &g, const Cell &cell, int a, const int p[3], double *eps_e,
+              double *kap_e) {
+  const int b = nxt(a), c = prv(a);
+  const double l = g.delta(a, p[a]);
+  double cap = 0.0, cond = 0.0;
+  for (int cb : {p[b] - 1, p[b]}) {
+    if (cb < 0 || cb >= g.n[b] - 1) continue;
+    for (int cc : {p[c] - 1, p[c]}) {
+      if (cc < 0 || cc >= g.n[c] - 1) continue;
+      int ce[3];
+      ce[a] = p[a], ce[b] = cb, ce[c] = cc;
+      const double area = 0.25 * g.delta(b, cb) * g.delta(c, cc);
+      const Mat &m = cell(ce);
+      cap += EPS0 * m.eps_r[a] * area;
+      cond += m.kappa[a] * area;
+    }
+  }
+  cap = cap / l, cond = cond / l;
+  const double kk = g.delta(a, p[a]) / (g.dual_delta(b, p[b]) * g.dual_delta(c, p[c]));
+  *eps_e = cap * kk / EPS0;
+  *kap_e = cond * kk;
+}
+
+// EdgeMaterials::from_cells for the H dual edge along a at node p
+// (edge_material.rs:404-410): the series inductance and resistance of its
+// two half cells (dual_edge_lr, coeffs.rs:303-325) back to mu_r, sigma_m.
+template <class Cell>
+void edge_mu(const Grid &g, const Cell &cell, int a, const int p[3], double *mu_e,
+             double *sig_e) {
+  const int b = nxt(a), c = prv(a);
+  const double area = g.delta(b, p[b]) * g.delta(c, p[c]);
+  double inv_l = 0.0, r_over_l2 = 0.0;
+  for (int ca : {p[a] - 1, p[a]}) {
+    if (ca < 0 || ca >= g.n[a] - 1) continue;
+    int ce[3] = {p[0], p[1], p[2]};
+    ce[a] = ca;
+    const double half = 0.5 * g.delta(a, ca);
+    const Mat &m = cell(ce);
+    const double lc = MU0 * m.mu_r[a] * area / half;
+    const double rc = m.sigma_m[a] * area / half;
+    inv_l += 1.0 / lc;
+    r_over_l2 += rc / (lc * lc);
+  }
+  const double L = 1.0 / inv_l;
+  const double R = L * L * r_over_l2;
+  const double kk = g.dual_delta(a, p[a]) / (g.delta(b, p[b]) * g.delta(c, p[c]));
+  *mu_e = L * kk / MU0;
+  *sig_e = R * kk;
+}
+}  // extern "C++"
+
 int ems_coefficients_from_cells(const double *x, int nx, const double *y, int ny,
                                 const double *z, int nz, const uint16_t *cell_mat,
                                 const double *mat_table, int n_mat, const uint8_t *pec,
@@ -137,26 +193,8 @@ int ems_coefficients_from_cells(const double *x, int nx, const double *y, int ny
                                  (p[c] == g.n[c] - 1 && pec_face[c][1]);
             const bool pm = pec && pec[(size_t)a * N + idx];
             if (!(on_face || pm)) {
-              // primary_edge_cg (coeffs.rs:264-295)
-              const double l = g.delta(a, p[a]);
-              double cap = 0.0, cond = 0.0;
-              for (int cb : {p[b] - 1, p[b]}) {
-                if (cb < 0 || cb >= g.n[b] - 1) continue;
-                for (int cc : {p[c] - 1, p[c]}) {
-                  if (cc < 0 || cc >= g.n[c] - 1) continue;
-                  int ce[3];
-                  ce[a] = p[a], ce[b] = cb, ce[c] = cc;
-                  const double area = 0.25 * g.delta(b, cb) * g.delta(c, cc);
-                  const Mat &m = cell(ce);
-                  cap += EPS0 * m.eps_r[a] * area;
-                  cond += m.kappa[a] * area;
-                }
-              }
-              cap = cap / l, cond = cond / l;
-              // EdgeMaterials::from_cells (edge_material.rs:396-401)
-              const double kk = g.delta(a, p[a]) / (g.dual_delta(b, p[b]) * g.dual_delta(c, p[c]));
-              const double eps_e = cap * kk / EPS0;
-              const double kap_e = cond * kk;
+              double eps_e, kap_e;
+              edge_eps(g, cell, a, p, &eps_e, &kap_e);
               // e_edge_geometry + e_coefficients (coeffs.rs:766-774, 817-820, 930-933)
               const double geo = g.dual_delta(b, p[b]) * g.dual_delta(c, p[c]) / g.delta(a, p[a]);
               const double C = store(eps0 * eps_e * geo);
@@ -167,26 +205,8 @@ int ems_coefficients_from_cells(const double *x, int nx, const double *y, int ny
             }
           }
           if (p[b] + 1 < g.n[b] && p[c] + 1 < g.n[c]) {
-            // dual_edge_lr (coeffs.rs:303-325)
-            const double area = g.delta(b, p[b]) * g.delta(c, p[c]);
-            double inv_l = 0.0, r_over_l2 = 0.0;
-            for (int ca : {p[a] - 1, p[a]}) {
-              if (ca < 0 || ca >= g.n[a] - 1) continue;
-              int ce[3] = {p[0], p[1], p[2]};
-              ce[a] = ca;
-              const double half = 0.5 * g.delta(a, ca);
-              const Mat &m = cell(ce);
-              const double lc = MU0 * m.mu_r[a] * area / half;
-              const double rc = m.sigma_m[a] * area / half;
-              inv_l += 1.0 / lc;
-              r_over_l2 += rc / (lc * lc);
-            }
-            const double L = 1.0 / inv_l;
-            const double R = L * L * r_over_l2;
-            // EdgeMaterials::from_cells (edge_material.rs:404-410)
-            const double kk = g.dual_delta(a, p[a]) / (g.delta(b, p[b]) * g.delta(c, p[c]));
-            const double mu_e = L * kk / MU0;
-            const double sig_e = R * kk;
+            double mu_e, sig_e;
+            edge_mu(g, cell, a, p, &mu_e, &sig_e);
             // coeffs.rs:939-955
             const double geo = g.delta(b, p[b]) * g.delta(c, p[c]) / g.dual_delta(a, p[a]);
             const double Ls = store(mu0 * mu_e * geo);
@@ -194,6 +214,56 @@ int ems_coefficients_from_cells(const double *x, int nx, const double *y, int ny
             const double den = Ls + 0.5 * Rs * dt;
             lanes[(size_t)(6 + a) * N + idx] = (float)((Ls - 0.5 * Rs * dt) / den);
             lanes[(size_t)(9 + a) * N + idx] = (float)(dt / den);
+          }
+        }
+    }
+  });
+  return EMS_OK;
+}
+
+int ems_edge_materials_from_cells(const double *x, int nx, const double *y, int ny,
+                                  const double *z, int nz, const uint16_t *cell_mat,
+                                  const double *mat_table, int n_mat, int threads,
+                                  double *eps_r, double *mu_r) {
+  if (nx < 2 || ny < 2 || nz < 2 || !mat_table || n_mat < 1 || !eps_r || !mu_r)
+    return EMS_E_INVALID;
+  const Grid g = make_grid(x, nx, y, ny, z, nz);
+  const size_t N = (size_t)nx * ny * nz;
+  const int cn[3] = {nx - 1, ny - 1, nz - 1};
+  std::vector<Mat> mats(n_mat);
+  for (int m = 0; m < n_mat; m++) {
+    const double *t = mat_table + 12 * m;
+    for (int a = 0; a < 3; a++) {
+      mats[m].eps_r[a] = t[a], mats[m].mu_r[a] = t[3 + a];
+      mats[m].kappa[a] = t[6 + a], mats[m].sigma_m[a] = t[9 + a];
+    }
+  }
+  if (cell_mat) {
+    const size_t nc = (size_t)cn[0] * cn[1] * cn[2];
+    for (size_t c = 0; c < nc; c++)
+      if (cell_mat[c] >= n_mat) return EMS_E_INVALID;
+  }
+  auto cell = [&](const int c[3]) -> const Mat & {
+    if (!cell_mat) return mats[0];
+    return mats[cell_mat[((size_t)c[0] * cn[1] + c[1]) * cn[2] + c[2]]];
+  };
+  std::memset(eps_r, 0, 3 * N * sizeof(double));
+  std::memset(mu_r, 0, 3 * N * sizeof(double));
+  parallel_for(nx, threads, [&](int i) {
+    for (int a = 0; a < 3; a++) {
+      const int b = nxt(a), c = prv(a);
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          const int p[3] = {i, j, k};
+          const size_t idx = g.index(i, j, k);
+          double v, w;
+          if (p[a] + 1 < g.n[a]) {
+            edge_eps(g, cell, a, p, &v, &w);
+            eps_r[(size_t)a * N + idx] = v;
+          }
+          if (p[b] + 1 < g.n[b] && p[c] + 1 < g.n[c]) {
+            edge_mu(g, cell, a, p, &v, &w);
+            mu_r[(size_t)a * N + idx] = v;
           }
         }
     }
@@ -571,6 +641,13 @@ struct Stepper {
   virtual double dt() = 0;
   virtual int step_with_energy(double *w) = 0;
   virtual int run(uint64_t n) = 0;
+  // Stepper::energy_now (run.rs:295-299): the energy of the current state
+  // in `measure` without stepping, where the stepper has it.
+  virtual bool has_energy_now() { return false; }
+  virtual int energy_now(int measure, double *w) {
+    (void)measure, (void)w;
+    return EMS_E_UNSUPPORTED;
+  }
   // per probe: sample times and values (run.rs:302-306)
   virtual int signals(std::function<void(const std::vector<double> &, const std::vector<double> &)>) {
     return 0;
@@ -643,6 +720,7 @@ constexpr double NEGATIVE_ENERGY = 1e-12;  // run.rs:419
 int drive(Stepper &s, const ems_run_limits &lim, ems_run_outcome &out) {
   if (lim.has_criteria && !(lim.energy_ratio >= 0.0)) return EMS_E_INVALID;
   if (lim.has_excitation_end && std::isnan(lim.excitation_end)) return EMS_E_INVALID;
+  if (lim.energy_measure < 0 || lim.energy_measure > 1) return EMS_E_INVALID;
   const uint64_t interval = lim.check_interval ? lim.check_interval : 50;
   const uint64_t max = lim.max_timesteps;
   const auto start = std::chrono::steady_clock::now();
@@ -652,16 +730,43 @@ int drive(Stepper &s, const ems_run_limits &lim, ems_run_outcome &out) {
   uint64_t met_steps = 0;
   Settling settle{lim.settle_frequency, lim.settle_tolerance, lim.settle_window,
                   lim.settle_max_extension, {}, {}};
+  // run.rs:436-442: openEMS's schedule evaluates the state reached after
+  // the interval; another measure than the invariant needs the state's own
+  // evaluation before the step.
+  const int measure = lim.energy_measure;
+  const bool supported = s.has_energy_now();
+  const bool after = lim.check_after_interval && supported;
+  const bool before = !after && measure != 0 && supported;
   int reason;
   for (;;) {
     if (steps >= max) {
       reason = 1;
       break;
     }
-    const uint64_t n = s.timestep();
-    int rc = s.step_with_energy(&w);
-    if (rc) return rc;
-    steps += 1;
+    uint64_t n;
+    int rc;
+    if (after) {
+      // run.rs:456-470: the next regular check or the extra one before it
+      const uint64_t regular = (steps / interval + 1) * interval;
+      uint64_t next = regular;
+      if (lim.has_extra_check && lim.extra_check > steps && lim.extra_check < regular)
+        next = lim.extra_check;
+      next = std::min(next, max);
+      if ((rc = s.run(next - steps))) return rc;
+      steps = next;
+      if (steps % interval != 0 && !(lim.has_extra_check && lim.extra_check == steps)) continue;
+      n = s.timestep();
+      if ((rc = s.energy_now(measure, &w))) return rc;
+    } else if (before) {
+      n = s.timestep();
+      if ((rc = s.energy_now(measure, &w))) return rc;
+      if ((rc = s.run(1))) return rc;
+      steps += 1;
+    } else {
+      n = s.timestep();
+      if ((rc = s.step_with_energy(&w))) return rc;
+      steps += 1;
+    }
     w_max = std::fmax(w_max, w);
     const double ratio = w_max > 0.0 ? w / w_max : 1.0;
     ratio_db = ratio > 0.0 ? 10.0 * std::log10(ratio) : -INFINITY;
@@ -683,9 +788,11 @@ int drive(Stepper &s, const ems_run_limits &lim, ems_run_outcome &out) {
         break;
       }
     }
-    const uint64_t chunk = std::min(interval - 1, max - steps);
-    if ((rc = s.run(chunk))) return rc;
-    steps += chunk;
+    if (!after) {
+      const uint64_t chunk = std::min(interval - 1, max - steps);
+      if ((rc = s.run(chunk))) return rc;
+      steps += chunk;
+    }
   }
   out.reason = reason;
   out.ratio_db = ratio_db;
@@ -710,6 +817,8 @@ struct GpuStepper : Stepper {
   double dt() override { return dtv; }
   int step_with_energy(double *w) override { return ems_gpu_step_with_energy(h, w); }
   int run(uint64_t n) override { return ems_gpu_run(h, n); }
+  bool has_energy_now() override { return true; }
+  int energy_now(int measure, double *w) override { return ems_gpu_energy_in(h, measure, w); }
   int signals(std::function<void(const std::vector<double> &, const std::vector<double> &)> f)
       override {
     for (int id : probes) {
@@ -765,7 +874,7 @@ int ems_run_until(ems_gpu *h, const ems_run_limits *limits, const int *probes, i
 int ems_run_until_batch(ems_gpu_batch *b, int n, ems_gpu *const *models,
                         const ems_run_limits *lim, ems_run_outcome *out) {
   if (!b || n < 1 || !models || !lim || !out) return EMS_E_INVALID;
-  if (lim->has_settle) return EMS_E_UNSUPPORTED;
+  if (lim->has_settle || lim->check_after_interval || lim->energy_measure) return EMS_E_UNSUPPORTED;
   if (lim->has_criteria && !(lim->energy_ratio >= 0.0)) return EMS_E_INVALID;
   if (lim->has_excitation_end && std::isnan(lim->excitation_end)) return EMS_E_INVALID;
   const uint64_t interval = lim->check_interval ? lim->check_interval : 50;

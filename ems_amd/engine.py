@@ -267,6 +267,27 @@ def coefficients(grid: Grid, mats: Optional[CellMaterials], faces: Sequence[int]
     return lanes
 
 
+def edge_materials(grid: Grid, mats: Optional[CellMaterials], threads: int = 0):
+    """EdgeMaterials::from_cells (edge_material.rs:376-417): (eps_r, mu_r),
+    each shape (3, nx, ny, nz): the relative permittivity of every E edge
+    and permeability of every H dual edge by axis (0 on padding)."""
+    n = grid.nodes()
+    if mats is None:
+        mats = CellMaterials.vacuum(grid)
+    table = np.ascontiguousarray(np.array([m.row() for m in mats.table], dtype=np.float64))
+    ids = np.ascontiguousarray(mats.ids)
+    eps = np.empty((3,) + n, dtype=np.float64)
+    mu = np.empty((3,) + n, dtype=np.float64)
+    if threads <= 0:
+        import os
+        threads = min(32, os.cpu_count() or 1)
+    nat.check(nat.ems_edge_materials_from_cells(*grid._args(), nat.ptr(ids, C.c_uint16),
+                                                nat.ptr(table, C.c_double), len(mats.table),
+                                                threads, nat.ptr(eps, C.c_double),
+                                                nat.ptr(mu, C.c_double)))
+    return eps, mu
+
+
 def pool_lane(dense: np.ndarray):
     """Lane::pool: (rows, index) if pooled, else None."""
     nx, ny, nz = dense.shape
@@ -610,10 +631,14 @@ class EndCriteria:
 
 @dataclass
 class RunLimits:
-    """src/engine/run.rs:22-139"""
+    """src/engine/run.rs:22-139. energy_measure: 0 EnergyMeasure::Invariant,
+    1 EnergyMeasure::OpenEms."""
     max_timesteps: int
     end_criteria: Optional[EndCriteria] = None
     excitation_end: Optional[float] = None
+    check_after_interval: bool = False
+    extra_check: Optional[int] = None
+    energy_measure: int = 0
 
     def c(self) -> nat.RunLimitsC:
         ec = self.end_criteria
@@ -623,7 +648,9 @@ class RunLimits:
             ec.check_interval if ec else 50, 0 if self.excitation_end is None else 1,
             self.excitation_end if self.excitation_end is not None else 0.0,
             1 if st else 0, st.frequency if st else 0.0, st.tolerance if st else 0.0,
-            st.window if st else 0.0, st.max_extension if st else 0.0)
+            st.window if st else 0.0, st.max_extension if st else 0.0,
+            1 if self.check_after_interval else 0, 0 if self.extra_check is None else 1,
+            0 if self.extra_check is None else int(self.extra_check), int(self.energy_measure))
 
 
 @dataclass
@@ -987,6 +1014,17 @@ class Engine:
         w = C.c_double()
         self._check(nat.ems_gpu_energy(self._h, C.byref(w)))
         return w.value
+
+    def energy_in(self, measure: int) -> float:
+        """Engine::energy_in (mod.rs:747-752): 0 the invariant, 1 openEMS's
+        measure."""
+        w = C.c_double()
+        self._check(nat.ems_gpu_energy_in(self._h, int(measure), C.byref(w)))
+        return w.value
+
+    def energy_openems(self) -> float:
+        """Engine::energy_openems (mod.rs:727-741)."""
+        return self.energy_in(1)
 
     def run_until(self, limits: RunLimits, watch: Sequence[Probe] = ()) -> RunOutcome:
         self._ensure_tables(self.timestep() + int(limits.max_timesteps) + 1)

@@ -403,8 +403,13 @@ int ems_gpu_step_with_energy(ems_gpu *h, double *w);
 int ems_gpu_energy(ems_gpu *h, double *w);
 /* Engine::energy_in (src/engine/mod.rs:747-752) and Stepper::energy_now
  * (src/engine/run.rs:296-299): measure 0 EnergyMeasure::Invariant (=
- * ems_gpu_energy); 1 EnergyMeasure::OpenEms is refused with
- * EMS_E_UNSUPPORTED (the engine keeps only the invariant). */
+ * ems_gpu_energy); 1 EnergyMeasure::OpenEms (Engine::energy_openems,
+ * mod.rs:727-741): eps0 sum u^2 + mu0 sum w i'^2 over every edge with
+ * i' the read-only look-ahead, the dual edges of the outer planes normal
+ * to their own axis weighted 4 and the E of the three upper faces left out
+ * (kernel.rs:364-423, 132-173, 218-244), the PML layers' corrections of
+ * their H edges (upml.rs:505-537), plus the H sources' look-ahead share
+ * (dump.rs:1735-1782). Cartesian grids (the only ones held). */
 int ems_gpu_energy_in(ems_gpu *h, int measure, double *w);
 
 /* No counterpart in the reference: ems_gpu_energy for callers that cannot

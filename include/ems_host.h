@@ -43,6 +43,15 @@ int ems_coefficients_from_cells(const double *x, int nx, const double *y, int ny
                                 const int face_pec[6], double dt, int rounding, int threads,
                                 float *lanes);
 
+/* EdgeMaterials::from_cells (src/engine/edge_material.rs:376-417): the
+ * relative permittivity of every E edge and permeability of every H dual
+ * edge (3*N doubles each, axis major; 0 where the edge is padding), from
+ * the same cells and table as ems_coefficients_from_cells. */
+int ems_edge_materials_from_cells(const double *x, int nx, const double *y, int ny,
+                                  const double *z, int nz, const uint16_t *cell_mat,
+                                  const double *mat_table, int n_mat, int threads,
+                                  double *eps_r, double *mu_r);
+
 /* Lane::pool (src/engine/coeffs.rs:560-598): pools the nx*ny z-rows of a
  * dense lane if at most max(count/4, 1) are distinct. Returns 1 and fills
  * rows_out (<= count/4 rows, first-seen order) / index_out (nx*ny) when
@@ -171,6 +180,15 @@ typedef struct {
     /* Settle (fdtd.rs:440-490); has_settle 0 = none */
     int32_t has_settle;
     double settle_frequency, settle_tolerance, settle_window, settle_max_extension;
+    /* RunLimits::check_after_interval, extra_check, energy_measure
+     * (run.rs:38-52): check the state reached after every interval (and
+     * once more after extra_check steps) instead of the one before a step;
+     * the measure 0 Invariant, 1 OpenEms (see ems_gpu_energy_in). A
+     * stepper without its own state evaluation (the scripted one) ignores
+     * the first two and the measure, as the reference's does. */
+    int32_t check_after_interval, has_extra_check;
+    uint64_t extra_check;
+    int32_t energy_measure;
 } ems_run_limits;
 
 /* RunOutcome / StopReason (run.rs:141-214): reason 0 EnergyDecay,
@@ -192,8 +210,8 @@ int ems_run_until(ems_gpu *h, const ems_run_limits *limits, const int *probes, i
 /* Engine::run_until for every model of a batch, in lockstep: each model
  * follows its own run::drive (run.rs:424-540) -- the same energy checks,
  * stop decisions and step counts as ems_run_until on it alone -- and
- * leaves the batch when it stops. out[q] per model. Settle is not
- * supported here. */
+ * leaves the batch when it stops. out[q] per model. Settle, the checks
+ * after the interval and openEMS's measure are not supported here. */
 int ems_run_until_batch(ems_gpu_batch *b, int n_models, ems_gpu *const *models,
                         const ems_run_limits *limits, ems_run_outcome *out);
 

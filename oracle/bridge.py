@@ -66,6 +66,8 @@ orc_add_upml = _sig("orc_add_upml", _i, _p, _i, _ip2, _ip2, C.POINTER(C.c_float)
 orc_upml_state = _sig("orc_upml_state", _i, _p, _i, _i, C.POINTER(C.c_float), C.POINTER(C.c_float))
 orc_step_with_energy = _sig("orc_step_with_energy", _d, _p)
 orc_energy = _sig("orc_energy", _d, _p)
+orc_energy_openems = _sig("orc_energy_openems", _d, _p)
+orc_energy_openems_ex = _sig("orc_energy_openems_ex", _d, _p, _i)
 orc_timestep = _sig("orc_timestep", _u64, _p)
 orc_read = _sig("orc_read", _i, _p, _i, _i, _fp)
 orc_write = _sig("orc_write", _i, _p, _i, _i, _fp)
@@ -279,6 +281,9 @@ class OracleEngine:
     def step_with_energy(self):
         self._ensure_tables(self.timestep() + 1)
         return orc_step_with_energy(self._h)
+
+    def energy_openems(self, layers=True):
+        return orc_energy_openems_ex(self._h, 1 if layers else 0)
 
     def energy(self):
         return orc_energy(self._h)

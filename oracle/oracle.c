@@ -1563,6 +1563,226 @@ double orc_energy(orc_engine *e) {
     return plane_sum(e) + w_ops;
 }
 
+/* ------------------------------------------------------------------ */
+/* openEMS's energy measure (EnergyMeasure::OpenEms, run.rs:66-75)      */
+/* ------------------------------------------------------------------ */
+
+/* oe_square (kernel.rs:198-210): x^2 in f64, infinite where its f32 would
+ * be (round to nearest: from the midpoint between FLT_MAX and 2^128). */
+static inline double oe_square(double x) {
+    const double s = x * x;
+    return s >= 0x1.ffffffp+127 ? INFINITY : s;
+}
+
+/* h_row_energy with OE (kernel.rs:132-173), read-only: the new values
+ * squared, the row's two ends weighted by `ends`, 4 accumulators. */
+static double hrow_oe(const orc_engine *e, const float *h, const float *da, const float *db,
+                      const float *a, const float *b, const float *c, const float *d, int n,
+                      double ends) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < n; k++) {
+        float nw = e->oe ? oe_h(da[k], h[k], db[k], a[k], b[k], c[k], d[k])
+                         : pair_h(da[k], h[k], db[k], a[k], b[k], c[k], d[k]);
+        const double w = (k == 0 || k + 1 == n) ? ends : 1.0;
+        acc[k & 3] += w * oe_square((double)nw);
+    }
+    return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+/* The plane value of h_slab with OE on a Cartesian grid (kernel.rs:364-423,
+ * 447-511): eps0 sum u^2 + mu0 sum w i'^2; the dual edges of the outer
+ * planes normal to their own axis weigh 4, the E of the last node plane
+ * along any axis is left out. */
+static double plane_openems(const orc_engine *e, int gi) {
+    const int nx = e->n[0], ny = e->n[1], nz = e->n[2];
+    const size_t p = e->plane, off = (size_t)gi * p;
+    const float *ix = e->i[0] + off, *iy = e->i[1] + off, *iz = e->i[2] + off;
+    const float *ux = e->u[0] + off, *uy = e->u[1] + off, *uz = e->u[2] + off;
+    const int m = nz - 1;
+    const double wx = (gi == 0 || gi + 1 == nx) ? 4.0 : 1.0;
+    double wh = 0.0;
+    for (int j = 0; j < ny - 1; j++) {
+        size_t r = (size_t)j * nz;
+        wh += wx * hrow_oe(e, ix + r, row(e, 6, gi, j), row(e, 9, gi, j), uz + r + nz, uz + r,
+                           uy + r + 1, uy + r, m, 1.0);
+    }
+    if (gi + 1 < nx) {
+        const float *uyn = uy + p, *uzn = uz + p;
+        for (int j = 0; j < ny; j++) {
+            size_t r = (size_t)j * nz;
+            const double wy = (j == 0 || j + 1 == ny) ? 4.0 : 1.0;
+            wh += wy * hrow_oe(e, iy + r, row(e, 7, gi, j), row(e, 10, gi, j), ux + r + 1, ux + r,
+                               uzn + r, uz + r, m, 1.0);
+        }
+        for (int j = 0; j < ny - 1; j++) {
+            size_t r = (size_t)j * nz;
+            wh += hrow_oe(e, iz + r, row(e, 8, gi, j), row(e, 11, gi, j), uyn + r, uy + r,
+                          ux + r + nz, ux + r, nz, 4.0);
+        }
+    }
+    double we = 0.0;
+    if (gi + 1 < nx) {
+        const float *uc[3] = {ux, uy, uz};
+        for (int a = 0; a < 3; a++) {
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int j = 0; j < ny - 1; j++)
+                for (int k = 0; k < nz - 1; k++) {
+                    const size_t q = (size_t)j * nz + k;
+                    acc[q & 3] += oe_square((double)uc[a][q]);
+                }
+            we += (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        }
+    }
+    return EPS0 * we + MU0 * wh;
+}
+
+/* The layer step of an H edge from the kernel's value k (Run::apply and
+ * OpenEmsPml::lookahead_current, upml.rs:404-426, 719-731), read-only. */
+static float upml_complete(const upml_t *m, int q, float k) {
+    const float *c = m->co[1] + 4 * (size_t)q;
+    const float f0 = m->f[1][q];
+    const float f1 = flush(flush(c[0] * f0) + k);
+    return flush(flush(flush(c[1] * m->v[1][q]) - flush(c[3] * f0)) + flush(c[2] * f1));
+}
+
+/* Parts::energy_openems_slab (upml.rs:505-537): per H run (component,
+ * plane) mu0 sum (v^2 - inc^2) over its edges in order, inc the kernel's
+ * flux increment evaluated in the Paired order as written there, v the
+ * completed layer value; added to the plane in component order. */
+static void upml_energy_openems(const orc_engine *e, const upml_t *m, double *planes) {
+    const int nz = e->n[2];
+    const size_t p = e->plane;
+    for (int gi = 0; gi < e->n[0]; gi++)
+        for (int comp = 0; comp < 3; comp++) {
+            int any = 0;
+            double sum = 0.0;
+            for (size_t r = m->pstart[1][gi]; r < m->pstart[1][gi + 1]; r++) {
+                const int q = m->perm[1][r];
+                if (m->ax[1][q] != comp) continue;
+                any = 1;
+                const size_t idx = m->idx[1][q], o = idx - (size_t)gi * p;
+                const int j = (int)(o / nz), k = (int)(o % nz);
+                const float *ux = e->u[0], *uy = e->u[1], *uz = e->u[2];
+                float t;
+                if (comp == 0)
+                    t = (uz[idx + nz] - uz[idx]) - (uy[idx + 1] - uy[idx]);
+                else if (comp == 1)
+                    t = (ux[idx + 1] - ux[idx]) - (uz[idx + p] - uz[idx]);
+                else
+                    t = (uy[idx + p] - uy[idx]) - (ux[idx + nz] - ux[idx]);
+                const float db = row(e, 9 + comp, gi, j)[k];
+                const float inc = flush(0.0f * m->v[1][q] - db * t);
+                const float v = upml_complete(m, q, inc);
+                sum += (double)v * (double)v - (double)inc * (double)inc;
+            }
+            if (any) planes[gi] += MU0 * sum;
+        }
+}
+
+/* The kernel's look-ahead i^{n+1/2} of one dual edge (dump.rs:1830-1845). */
+static float lookahead_h(const orc_engine *e, int c, const int p[3]) {
+    const int a = axis_next(c), b = axis_prev(c);
+    const size_t idx = ((size_t)p[0] * e->n[1] + p[1]) * e->n[2] + p[2];
+    int pa[3] = {p[0], p[1], p[2]}, pb[3] = {p[0], p[1], p[2]};
+    pa[a]++, pb[b]++;
+    const float t0 = pa[a] < e->n[a]
+                         ? e->u[b][((size_t)pa[0] * e->n[1] + pa[1]) * e->n[2] + pa[2]] : 0.0f;
+    const float t2 = pb[b] < e->n[b]
+                         ? e->u[a][((size_t)pb[0] * e->n[1] + pb[1]) * e->n[2] + pb[2]] : 0.0f;
+    const float t1 = e->u[b][idx], t3 = e->u[a][idx];
+    const float da = row(e, 6 + c, p[0], p[1])[p[2]], db = row(e, 9 + c, p[0], p[1])[p[2]];
+    const float h = e->i[c][idx];
+    return e->oe ? oe_h(da, h, db, t0, t1, t2, t3)
+                 : flush(da * h - db * ((t0 - t1) - (t2 - t3)));
+}
+
+typedef struct {
+    int axis;
+    int node[3];
+} hedge_t;
+
+static int hedge_cmp(const void *x, const void *y) {
+    const hedge_t *a = (const hedge_t *)x, *b = (const hedge_t *)y;
+    if (a->axis != b->axis) return a->axis < b->axis ? -1 : 1;
+    for (int d = 0; d < 3; d++)
+        if (a->node[d] != b->node[d]) return a->node[d] < b->node[d] ? -1 : 1;
+    return 0;
+}
+
+/* lookahead_source_energy (dump.rs:1735-1782): mu0 sum w ((k + s)^2 - k^2)
+ * over the H sources' edges (sorted by axis and node, without repeats), k
+ * the look-ahead (completed by the layers on their edges), k + s with the
+ * next step's H sources applied in registration order. */
+static double hsource_energy(const orc_engine *e) {
+    size_t cnt = 0;
+    for (int q = 0; q < e->nops; q++)
+        if (is_hsrc(&e->ops[q])) cnt += e->ops[q].n;
+    if (!cnt) return 0.0;
+    hedge_t *ed = (hedge_t *)malloc(cnt * sizeof(hedge_t));
+    size_t w = 0;
+    for (int q = 0; q < e->nops; q++) {
+        const op_t *o = &e->ops[q];
+        if (!is_hsrc(o)) continue;
+        for (int r = 0; r < o->n; r++) {
+            size_t idx = o->idx[r];
+            ed[w].axis = o->axis[r];
+            ed[w].node[2] = (int)(idx % e->n[2]);
+            idx /= e->n[2];
+            ed[w].node[1] = (int)(idx % e->n[1]);
+            ed[w].node[0] = (int)(idx / e->n[1]);
+            w++;
+        }
+    }
+    qsort(ed, cnt, sizeof(hedge_t), hedge_cmp);
+    const upml_t *pm = NULL;
+    for (int q = 0; q < e->nops; q++)
+        if (e->ops[q].type == OP_UPML) pm = e->ops[q].upml;
+    double sum = -0.0;
+    for (size_t r = 0; r < cnt; r++) {
+        if (r && hedge_cmp(&ed[r], &ed[r - 1]) == 0) continue;
+        const int c = ed[r].axis;
+        const int *p = ed[r].node;
+        const size_t idx = ((size_t)p[0] * e->n[1] + p[1]) * e->n[2] + p[2];
+        float k = lookahead_h(e, c, p);
+        if (pm)
+            for (int q = 0; q < pm->n[1]; q++)
+                if (pm->ax[1][q] == c && pm->idx[1][q] == idx) {
+                    k = upml_complete(pm, q, k);
+                    break;
+                }
+        float v = k;
+        for (int q = 0; q < e->nops; q++) {
+            const op_t *o = &e->ops[q];
+            if (!is_hsrc(o)) continue;
+            const float a = op_amp(o, e->step);
+            for (int s = 0; s < o->n; s++)
+                if (o->axis[s] == c && o->idx[s] == idx) {
+                    const float x = o->wf[s] * a;
+                    v = (o->kind == 3) ? x : v + x;
+                }
+        }
+        const double wt = (p[c] == 0 || p[c] + 1 == e->n[c]) ? 4.0 : 1.0;
+        const double with = (double)v * (double)v * wt, without = (double)k * (double)k * wt;
+        sum += with - without;
+    }
+    free(ed);
+    return MU0 * sum;
+}
+
+double orc_energy_openems_ex(orc_engine *e, int layers) {
+    const int nx = e->n[0];
+    double *planes = (double *)calloc(nx, sizeof(double));
+    for (int gi = 0; gi < nx; gi++) planes[gi] = plane_openems(e, gi);
+    for (int q = 0; q < e->nops && layers; q++)
+        if (e->ops[q].type == OP_UPML) upml_energy_openems(e, e->ops[q].upml, planes);
+    double s = -0.0;
+    for (int gi = 0; gi < nx; gi++) s += planes[gi];
+    free(planes);
+    return s + hsource_energy(e);
+}
+
+double orc_energy_openems(orc_engine *e) { return orc_energy_openems_ex(e, 1); }
+
 uint64_t orc_timestep(orc_engine *e) { return e->step; }
 
 int orc_add_mur(orc_engine *e, int ne, const int *eax, const int *enode, const int *einner,

@@ -192,6 +192,13 @@ int orc_add_upml(orc_engine *e, int ne, const int *eax, const int *enode, const 
                  int nh, const int *hax, const int *hnode, const float *hco);
 /* The layer state (flux, value) of side 0 (E) or 1 (H), in edge order. */
 int orc_upml_state(orc_engine *e, int id, int side, float *flux, float *value);
+/* Engine::energy_in(EnergyMeasure::OpenEms) (mod.rs:727-752): openEMS's
+ * measure eps0 sum u^2 + mu0 sum w i'^2 of the current state (Cartesian
+ * boundary rules), the layers' corrections and the H sources' look-ahead
+ * energy; read-only. */
+double orc_energy_openems(orc_engine *e);
+/* The same without the layers' corrections (layers = 0), for the tests. */
+double orc_energy_openems_ex(orc_engine *e, int layers);
 double orc_step_with_energy(orc_engine *e);
 double orc_energy(orc_engine *e);
 uint64_t orc_timestep(orc_engine *e);

@@ -46,7 +46,7 @@ def test_pml_face_fixture_on_device():
     o.close()
 
 
-def graded_box(kind, curl, materials):
+def graded_box(kind, curl, materials, h_in_layer=False):
     cells = (14, 12, 13)
 
     def lines(n, s, w):
@@ -82,8 +82,10 @@ def graded_box(kind, curl, materials):
         return math.exp(-x * x)
     e.add_operator(E.EdgeSource(pulse, horizon=3000).edge(E.Axis.Z, [6, 6, 6], 1e-3)
                    .edge(E.Axis.X, [5, 7, 6], 3e-4).edge(E.Axis.Y, [11, 2, 10], 5e-4))
-    e.add_operator(E.EdgeSource(pulse, horizon=3000).kind(E.SourceKind.SoftH)
-                   .edge(E.Axis.Y, [4, 5, 5], 2e-3))
+    hs = E.EdgeSource(pulse, horizon=3000).kind(E.SourceKind.SoftH).edge(E.Axis.Y, [4, 5, 5], 2e-3)
+    if h_in_layer:  # H_x at j = 1: inside the y_min layer
+        hs.edge(E.Axis.X, [6, 1, 6], 1e-3)
+    e.add_operator(hs)
     probes = [e.add_operator(E.Probe.voltage(E.Axis.Z, [12, 2, 11])),
               e.add_operator(E.Probe.current(E.Axis.X, [12, 1, 11])),
               e.add_operator(E.Probe.voltage(E.Axis.Y, [7, 6, 6]))]
