@@ -599,7 +599,7 @@ struct PlaneIn {
 __device__ __forceinline__ void load_plane(PlaneIn &P, const float *F, size_t N,
                                            const float *cpool, const uint32_t *croff, size_t ob,
                                            size_t r, size_t plane, int nz, unsigned k4, bool iv,
-                                           bool jv, bool kv) {
+                                           bool jv, bool kv, bool need_e = true) {
   // Branch-free, every address valid (neighbour offsets fall back to the
   // node itself where the neighbour does not exist; the value is then
   // replaced by 0). One wave-uniform base per component array; the
@@ -620,8 +620,10 @@ __device__ __forceinline__ void load_plane(PlaneIn &P, const float *F, size_t N,
   // kept under 4 GiB; croff holds byte offsets), so the coefficients need
   // no 64-bit address math.
   const uint32_t *ro = croff + 16 * r;
+  // Rows that only update H (a tile's halo row, the plane before a chunk;
+  // need_e is wave-uniform) skip the E coefficients.
 #pragma unroll
-  for (int l = 0; l < 12; l++) P.c[l] = ldc(cpool, ro, l, k4);
+  for (int l = 0; l < 12; l++) P.c[l] = (l >= 6 || need_e) ? ldc(cpool, ro, l, k4) : 0.0f;
   P.uxc = uxc;
   P.ux_j1 = jv ? uxj : 0.0f;
   P.uz_j1 = jv ? uzj : 0.0f;
@@ -652,7 +654,7 @@ __device__ __forceinline__ void h_from(const PlaneIn &P, float uyc, float uzc, b
 
 // Tile t of the fused step (k tiles fastest, then j tiles, then x chunks):
 // the body shared by the single-model and the batched kernels.
-template <bool OE, int TJ, bool PF>
+template <bool OE, int TJ, bool PF, bool CP = true>
 __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *__restrict__ B,
                                            size_t N, const float *__restrict__ cpool,
                                            const uint32_t *__restrict__ croff,
@@ -662,6 +664,9 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
                                            const uint32_t *__restrict__ pmask_o, int nhout,
                                            int nx, int ny, int nz, int tk, int chunk, int ktiles,
                                            int jtiles, int t) {
+  // Tiles that no slab reaches run an instantiation without the slab code
+  // (CP false: 44 VGPRs instead of 73, eight waves per SIMD instead of six).
+  if (!CP) nslab = 0, nhout = 0;
   const int kt = t % ktiles;
   const int t2 = t / ktiles;
   const int jt = t2 % jtiles;
@@ -711,7 +716,7 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
       // H of plane i0-1 at this node (read-only: its owner writes it).
       PlaneIn Q;
       load_plane(Q, A, N, cpool, croff, ob - plane, (size_t)(i0 - 1) * ny + jc, plane, nz, k4,
-                 true, jv, kv);
+                 true, jv, kv, false);
       const float uyp = ldf(A + N + (ob - plane), k4), uzp = ldf(A + 2 * N + (ob - plane), k4);
       if (inb) {
         float h[3];
@@ -723,7 +728,7 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
       }
     }
     load_plane(P, A, N, cpool, croff, ob, (size_t)i0 * ny + jc, plane, nz, k4, i0 + 1 < nx, jv,
-               kv);
+               kv, ty > 0);
   }
   for (int i = i0; i < i1; i++) {
     const int buf = i & 1;
@@ -737,7 +742,8 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
     asm volatile("" : "+v"(k4));
     PlaneIn Nx;
     if (PF && i + 1 < i1)
-      load_plane(Nx, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv);
+      load_plane(Nx, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv,
+                 ty > 0);
     // Slabs that reach this tile and this plane (bit masks, scalar).
     const uint32_t mhp = mh & cload(pmask, 2 * i), mep = me & cload(pmask, 2 * i + 1);
     float h[3] = {0.0f, 0.0f, 0.0f};
@@ -781,7 +787,8 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
       if (PF)
         P = Nx;
       else
-        load_plane(P, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv);
+        load_plane(P, A, N, cpool, croff, ob + plane, r + ny, plane, nz, k4, i + 2 < nx, jv, kv,
+                   ty > 0);
     }
   }
 }
@@ -794,7 +801,8 @@ __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *_
 // consecutive rows go round-robin to the 8 XCDs, so k-neighbours and most
 // j-neighbours share an L2 while every XCD gets a share of every x chunk.
 // The grid is padded to whole groups of 8; padding blocks get -1.
-__device__ __forceinline__ int tile_of_block(int b, int ktiles, int ntiles, int xcd_group) {
+__host__ __device__ __forceinline__ int tile_of_block(int b, int ktiles, int ntiles,
+                                                      int xcd_group) {
   if (xcd_group <= 0) return b;
   const int x8 = b % 8, sl = b / 8;
   const int lr = sl / ktiles, ktl = sl % ktiles;
@@ -803,18 +811,20 @@ __device__ __forceinline__ int tile_of_block(int b, int ktiles, int ntiles, int 
   return R * ktiles + ktl;
 }
 
-template <bool OE, int TJ, bool PF>
+// tiles: the tile of each block (one class of them), or null for all tiles
+// in tile_of_block's order.
+template <bool OE, int TJ, bool PF, bool CP>
 __global__ void __launch_bounds__(64 * (TJ + 1))
     step_fused(const float *__restrict__ A, float *__restrict__ B, size_t N,
                const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
                const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
                int par, const DSlab *__restrict__ hout, const uint32_t *__restrict__ pmask_o,
                int nhout, int nx, int ny, int nz, int tk, int chunk, int ktiles, int jtiles,
-               int ntiles, int xcd_group) {
-  const int t = tile_of_block(blockIdx.x, ktiles, ntiles, xcd_group);
+               int ntiles, int xcd_group, const int *__restrict__ tiles) {
+  const int t = tiles ? cload(tiles, blockIdx.x) : tile_of_block(blockIdx.x, ktiles, ntiles, xcd_group);
   if (t < 0) return;
-  fused_tile<OE, TJ, PF>(A, B, N, cpool, croff, slabs, pmask, nslab, par, hout, pmask_o, nhout, nx,
-                         ny, nz, tk, chunk, ktiles, jtiles, t);
+  fused_tile<OE, TJ, PF, CP>(A, B, N, cpool, croff, slabs, pmask, nslab, par, hout, pmask_o, nhout,
+                             nx, ny, nz, tk, chunk, ktiles, jtiles, t);
 }
 
 // ---- fused step, bands of rows per wave ----
@@ -878,7 +888,7 @@ __device__ __forceinline__ void band_tile(const float *__restrict__ A, float *__
       const bool jv = j < ny - 1;
       PlaneIn Q;
       load_plane(Q, A, N, cpool, croff, ob - plane, (size_t)(i0 - 1) * ny + j, plane, nz, k4, true,
-                 jv, kv);
+                 jv, kv, false);
       const float uyp = ldf(A + N + (ob - plane), k4), uzp = ldf(A + 2 * N + (ob - plane), k4);
       if (kin) {
         float h[3];
@@ -900,7 +910,7 @@ __device__ __forceinline__ void band_tile(const float *__restrict__ A, float *__
       const int j = j0 - 1;
       const size_t r = (size_t)i * ny + j, ob = r * nz;
       PlaneIn Q;
-      load_plane(Q, A, N, cpool, croff, ob, r, plane, nz, k4, iv, true, kv);
+      load_plane(Q, A, N, cpool, croff, ob, r, plane, nz, k4, iv, true, kv, false);
       const float uyh = ldf(A + N + ob, k4), uzh = ldf(A + 2 * N + ob, k4);
       if (kin) {
         float h[3];
@@ -1267,7 +1277,7 @@ __global__ void __launch_bounds__(64 * (TJ + 1))
       const unsigned ku = (unsigned)min(k, nz - 1);
       const bool kv = (int)ku < nz - 1;
       PlaneIn Q;
-      load_plane(Q, A, N, cpool, croff, ob, r, plane, nz, ku * 4u, true, jv, kv);
+      load_plane(Q, A, N, cpool, croff, ob, r, plane, nz, ku * 4u, true, jv, kv, false);
       const float uyc = ldf(A + N + ob, ku * 4u), uzc = ldf(A + 2 * N + ob, ku * 4u);
       if (inb) {
         float h[3];
@@ -2711,6 +2721,13 @@ struct ems_gpu {
   bool fused = true, prefetch = true, rows = false, packed = false, band = false;
   int xcd_group = 2;  // tile rows per XCD visit (2 measured best of 0, 1, 2, 4, 8)
   int tj = 7, nchunk = 0;
+  // tile classes of the tiled step (ems_gpu_set_tile_classes): the tiles a
+  // CPML slab reaches and the others, each list padded so that a tile keeps
+  // the XCD tile_of_block gives it; d_tiles holds both lists
+  bool tile_class = false;
+  int *d_tiles = nullptr;
+  int ntl[2] = {0, 0};
+  uint64_t tiles_key = 0, slab_gen = 0;
   // energy scratch
   double *d_rowsum = nullptr, *d_chains = nullptr, *d_planes = nullptr, *d_w = nullptr;
   double *h_w = nullptr;  // pinned host copy of W^n
@@ -2990,6 +3007,7 @@ int upload_fslabs(ems_gpu *h, bool split) {
   int rc = upload_out_slabs(h);
   if (rc) return rc;
   h->split_state = split ? 1 : 0;
+  h->slab_gen++;
   if (h->d_fslabs) CK(h, hipFree(h->d_fslabs));
   h->d_fslabs = nullptr;
   std::vector<FSlab> v;
@@ -3040,24 +3058,38 @@ int launch_cpml_out(ems_gpu *h, int par, bool after) {
   return 0;
 }
 
-template <int TJ, bool PF>
-void launch_fused_tj(ems_gpu *h, int par, const FusedGeom &g) {
+template <bool OE, int TJ, bool PF, bool CP>
+void launch_fused_one(ems_gpu *h, int par, const FusedGeom &g, int nblocks, const int *tiles) {
+  if (nblocks <= 0) return;
   const float *A = h->fbuf[par];
   float *B = h->fbuf[1 - par];
-  dim3 block(64, TJ + 1);
-  const int ns = n_fslabs(h);
+  hipLaunchKernelGGL((step_fused<OE, TJ, PF, CP>), dim3(nblocks), dim3(64, TJ + 1), 0, h->stream,
+                     A, B, h->N, (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
+                     (const uint32_t *)h->d_pmask, n_fslabs(h), par, (const DSlab *)h->d_hout,
+                     (const uint32_t *)h->d_pmask_o, h->n_hout, h->n[0], h->n[1], h->n[2], g.tk,
+                     g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group, tiles);
+}
+
+// The tiled step: one launch over every tile, or with tile classes the
+// tiles a slab reaches first (the slower ones), then the others with the
+// instantiation without slab code. The launches read A and write B at
+// disjoint tiles, so their order does not matter for the result.
+template <bool OE, int TJ, bool PF>
+void launch_fused_oe(ems_gpu *h, int par, const FusedGeom &g) {
+  if (h->tile_class && h->d_tiles) {
+    launch_fused_one<OE, TJ, PF, true>(h, par, g, h->ntl[1], h->d_tiles + h->ntl[0]);
+    launch_fused_one<OE, TJ, PF, false>(h, par, g, h->ntl[0], h->d_tiles);
+  } else {
+    launch_fused_one<OE, TJ, PF, true>(h, par, g, g.nblocks, nullptr);
+  }
+}
+
+template <int TJ, bool PF>
+void launch_fused_tj(ems_gpu *h, int par, const FusedGeom &g) {
   if (h->oe)
-    hipLaunchKernelGGL((step_fused<true, TJ, PF>), dim3(g.nblocks), block, 0, h->stream, A, B, h->N,
-                       (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
-                       (const uint32_t *)h->d_pmask, ns, par, (const DSlab *)h->d_hout,
-                       (const uint32_t *)h->d_pmask_o, h->n_hout, h->n[0], h->n[1], h->n[2], g.tk,
-                       g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group);
+    launch_fused_oe<true, TJ, PF>(h, par, g);
   else
-    hipLaunchKernelGGL((step_fused<false, TJ, PF>), dim3(g.nblocks), block, 0, h->stream, A, B,
-                       h->N, (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
-                       (const uint32_t *)h->d_pmask, ns, par, (const DSlab *)h->d_hout,
-                       (const uint32_t *)h->d_pmask_o, h->n_hout, h->n[0], h->n[1], h->n[2], g.tk,
-                       g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group);
+    launch_fused_oe<false, TJ, PF>(h, par, g);
 }
 
 // Full-row kernel: one wave per row, rows of up to 9 segments of 64.
@@ -3509,6 +3541,60 @@ bool tiled_fused(const ems_gpu *h) {
   return use_fused(h) && !h->band && !(h->packed && packed_ok(h)) && !(h->rows && rows_ok(h));
 }
 
+// The tile classes of the tiled step for the present geometry and slabs
+// (host side of launch_fused_oe): a tile needs the slab code when a slab
+// the kernel corrects (E or H side) or whose H increments it adds meets its
+// (j, k) box with the halo row and lane and its planes i0 - 1 .. i1 - 1.
+int build_tile_classes(ems_gpu *h) {
+  const FusedGeom g = fused_geom(h);
+  uint64_t key = 1469598103934665603ull;
+  for (uint64_t v : {(uint64_t)g.tk, (uint64_t)g.ktiles, (uint64_t)g.jtiles, (uint64_t)g.chunk,
+                     (uint64_t)g.nblocks, (uint64_t)h->tj, (uint64_t)h->xcd_group, h->slab_gen,
+                     (uint64_t)(h->split_state + 1)})
+    key = (key ^ v) * 1099511628211ull;
+  if (key == h->tiles_key && h->d_tiles) return 0;
+  drop_graphs(h);
+  struct Box {
+    int lo[3], hi[3];
+  };
+  std::vector<Box> boxes;
+  for (auto &sl : h->slabs)
+    if (sl.len != 0 && (!sl.out || !sl.arg.e_side)) {
+      Box b;
+      for (int d = 0; d < 3; d++) b.lo[d] = sl.arg.lo[d], b.hi[d] = sl.arg.hi[d];
+      boxes.push_back(b);
+    }
+  std::vector<int> q[2][8];
+  for (int b = 0; b < g.nblocks; b++) {
+    const int t = tile_of_block(b, g.ktiles, g.ntiles, h->xcd_group);
+    if (t < 0) continue;
+    const int kt = t % g.ktiles, t2 = t / g.ktiles, jt = t2 % g.jtiles, c = t2 / g.jtiles;
+    const int k0 = kt * g.tk, j0 = jt * h->tj, i0 = c * g.chunk;
+    const int i1 = std::min(h->n[0], i0 + g.chunk), ia = std::max(i0 - 1, 0);
+    bool cp = false;
+    for (const Box &x : boxes)
+      cp = cp || (x.lo[1] < j0 + h->tj && x.hi[1] > j0 - 1 && x.lo[2] < k0 + g.tk &&
+                  x.hi[2] > k0 - 1 && x.lo[0] < i1 && x.hi[0] > ia);
+    q[cp ? 1 : 0][b % 8].push_back(t);
+  }
+  std::vector<int> all;
+  for (int cl = 0; cl < 2; cl++) {
+    size_t m = 0;
+    for (int x = 0; x < 8; x++) m = std::max(m, q[cl][x].size());
+    for (size_t r = 0; r < m; r++)
+      for (int x = 0; x < 8; x++) all.push_back(r < q[cl][x].size() ? q[cl][x][r] : -1);
+    h->ntl[cl] = (int)(8 * m);
+  }
+  if (h->d_tiles) CK(h, hipFree(h->d_tiles));
+  h->d_tiles = nullptr;
+  if (!all.empty()) {
+    CK(h, hipMalloc(&h->d_tiles, all.size() * sizeof(int)));
+    CK(h, hipMemcpy(h->d_tiles, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  h->tiles_key = key;
+  return 0;
+}
+
 int prepare(ems_gpu *h, uint64_t steps, bool allow_split = true) {
   if (h->lanes_dirty) {
     drop_graphs(h);
@@ -3563,6 +3649,10 @@ int prepare(ems_gpu *h, uint64_t steps, bool allow_split = true) {
   if (h->slabs_dirty || h->split_state != (split ? 1 : 0)) {
     drop_graphs(h);
     int rc = upload_fslabs(h, split);
+    if (rc) return rc;
+  }
+  if (h->tile_class && tiled_fused(h)) {
+    int rc = build_tile_classes(h);
     if (rc) return rc;
   }
   if (h->ops_dirty) {
@@ -3817,6 +3907,7 @@ void ems_gpu_destroy(ems_gpu *h) {
   }
   if (h->d_fslabs) (void)hipFree(h->d_fslabs);
   if (h->d_pmask) (void)hipFree(h->d_pmask);
+  if (h->d_tiles) (void)hipFree(h->d_tiles);
   for (auto &d : h->dfts) {
     if (d.d_state) (void)hipFree(d.d_state);
     if (d.d_step) (void)hipFree(d.d_step);
@@ -4820,6 +4911,13 @@ int ems_gpu_set_cpml_split(ems_gpu *h, int on) {
   ALIVE(h);
   if (on != 0 && on != 1) return fail(h, EMS_E_INVALID, "cpml split must be 0 or 1");
   h->cpml_split = on != 0;
+  drop_graphs(h);
+  return EMS_OK;
+}
+
+int ems_gpu_set_tile_classes(ems_gpu *h, int on) {
+  ALIVE(h);
+  h->tile_class = on != 0;
   drop_graphs(h);
   return EMS_OK;
 }

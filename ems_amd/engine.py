@@ -1101,6 +1101,12 @@ class Engine:
         than inside it; the same values either way."""
         self._check(nat.ems_gpu_set_cpml_split(self._h, 1 if on else 0))
 
+    def set_tile_classes(self, on: bool) -> None:
+        """Whether the tiled fused step runs the tiles a CPML slab reaches
+        and the others in two launches (the latter without the slab code);
+        the same values either way."""
+        self._check(nat.ems_gpu_set_tile_classes(self._h, 1 if on else 0))
+
     def set_xcd_group(self, group: int) -> None:
         """Tile rows per XCD visit in the fused kernels' block order (0:
         plain order)."""

@@ -462,6 +462,11 @@ int ems_gpu_set_schedule(ems_gpu *h, int fused, int tile_rows, int x_chunks);
  * kernel, H-side by increments formed from u^n before it, E-side after it
  * (the same values, bit for bit); 0 keeps every slab inside. */
 int ems_gpu_set_cpml_split(ems_gpu *h, int on);
+/* Tile classes of the tiled fused step (no reference counterpart): with
+ * on != 0 the tiles that a CPML slab reaches run in one launch and the
+ * others in a second one, with a kernel instantiation without the slab
+ * code (fewer registers, more waves per SIMD). Same results either way. */
+int ems_gpu_set_tile_classes(ems_gpu *h, int on);
 
 /* Tile order of the fused kernels (no reference counterpart): groups of
  * `group` consecutive tile rows go round-robin to the 8 XCDs (their L2s),

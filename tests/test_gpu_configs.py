@@ -97,8 +97,8 @@ def cpml_box_512():
     return wl, seeded_fields(wl.grid.nodes(), 512)
 
 
-@pytest.mark.parametrize("sched", [None, (0, 0, 0), (6, 16, 0), "split"],
-                         ids=["default", "unfused", "band16", "cpml_split"])
+@pytest.mark.parametrize("sched", [None, (0, 0, 0), (6, 16, 0), "split", "classes"],
+                         ids=["default", "unfused", "band16", "cpml_split", "tile_classes"])
 def test_cpml_box_512_full_size(cpml_box_512, sched):
     from oracle.bridge import OracleEngine
     wl, init = cpml_box_512
@@ -106,6 +106,8 @@ def test_cpml_box_512_full_size(cpml_box_512, sched):
     e = E.Engine(wl.grid, wl.dt, wl.lanes)
     if sched == "split":
         e.set_cpml_split(True)  # the z-normal slabs in passes of their own
+    elif sched == "classes":
+        e.set_tile_classes(True)  # slab tiles and the others in two launches
     elif sched is not None:
         e.set_schedule(*sched)
     pe = wl.build_ops(e)

@@ -17,18 +17,26 @@ pytestmark = pytest.mark.gpu
 # each tile height and with many x chunks (chunk seams inside the box), the
 # full-row and the packed-row kernels (packed: grids with nz <= 64; the
 # others run the tiled kernel), the band kernel (bands of 4, 8, 16 rows) and
-# the tiled kernel with its z-normal CPML slabs in passes of their own.
+# the tiled kernel with its z-normal CPML slabs in passes of their own
+# (flag 1) and with its tiles in two classes (flag 2).
 SCHEDULES = [(0, 0, 0), (1, 3, 0), (1, 7, 0), (1, 15, 0), (1, 7, 5), (2, 7, 3), (2, 15, 0),
              (3, 3, 0), (3, 7, 0), (3, 7, 4), (4, 7, 0), (4, 7, 3), (5, 7, 0),
-             (6, 4, 0), (6, 8, 3), (6, 16, 0), (6, 16, 5), (1, 7, 0, 1), (2, 3, 5, 1)]
+             (6, 4, 0), (6, 8, 3), (6, 16, 0), (6, 16, 5), (1, 7, 0, 1), (2, 3, 5, 1),
+             (1, 7, 0, 2), (2, 7, 3, 2), (2, 15, 4, 2), (2, 3, 5, 3)]
 
 
-@pytest.fixture(autouse=True, params=SCHEDULES,
-                ids=lambda s: "mode%d_tj%d_xc%d" % s[:3] + ("_split" if s[3:] else ""))
+def schedule_id(s):
+    flags = s[3] if len(s) > 3 else 0
+    return ("mode%d_tj%d_xc%d" % s[:3] + ("_split" if flags & 1 else "")
+            + ("_classes" if flags & 2 else ""))
+
+
+@pytest.fixture(autouse=True, params=SCHEDULES, ids=schedule_id)
 def schedule(request):
-    f, t, c, *split = request.param
+    f, t, c, *flags = request.param
+    fl = flags[0] if flags else 0
     old = dict(cases.SCHEDULE)
-    cases.SCHEDULE.update(fused=f, tile_rows=t, x_chunks=c, split=split[0] if split else 0)
+    cases.SCHEDULE.update(fused=f, tile_rows=t, x_chunks=c, split=fl & 1, classes=(fl >> 1) & 1)
     yield
     cases.SCHEDULE.update(old)
 
