@@ -88,6 +88,40 @@ class RunLimitsC(C.Structure):
                 ("energy_measure", C.c_int32)]
 
 
+class PoleEntryC(C.Structure):
+    """ems_pole_entry (include/ems_host.h)."""
+    _fields_ = [("field", C.c_int32), ("axis", C.c_int32), ("node", C.c_int32 * 3),
+                ("kind", C.c_int32), ("p", C.c_double * 3)]
+
+
+class PolePlanC(C.Structure):
+    """ems_pole_plan (include/ems_host.h)."""
+    _fields_ = [("n_edges", C.c_int32 * 2), ("axis", C.POINTER(C.c_int32) * 2),
+                ("nodes", C.POINTER(C.c_int32) * 2), ("sum_inv_z", C.POINTER(C.c_double) * 2),
+                ("pole_end", C.POINTER(C.c_uint32) * 2), ("n_poles", C.c_uint64 * 2),
+                ("cidx", C.POINTER(C.c_uint32) * 2), ("n_consts", C.c_int32),
+                ("a", C.POINTER(C.c_double)), ("inv_c", C.POINTER(C.c_double)),
+                ("inv_z", C.POINTER(C.c_double)), ("half_l", C.POINTER(C.c_double)),
+                ("kind", C.POINTER(C.c_int32)), ("skipped", C.c_int32)]
+
+
+class LumpedElementC(C.Structure):
+    """ems_lumped_element (include/ems_host.h)."""
+    _fields_ = [("axis", C.c_int32), ("n_edges", C.c_int32), ("nodes", C.POINTER(C.c_int32)),
+                ("series", C.c_int32), ("columns", C.c_int32), ("r", C.c_double),
+                ("l", C.c_double), ("c", C.c_double), ("edge_r", C.POINTER(C.c_double)),
+                ("edge_l", C.POINTER(C.c_double)), ("edge_c", C.POINTER(C.c_double)),
+                ("topology", C.c_int32), ("explicit_inductor", C.c_int32)]
+
+
+class LumpedInstallC(C.Structure):
+    """ems_lumped_install (include/ems_host.h)."""
+    _fields_ = [("cb_e", C.POINTER(C.c_double)), ("g_src", C.POINTER(C.c_double)),
+                ("rec_g", C.POINTER(C.c_double)), ("rec_c", C.POINTER(C.c_double)),
+                ("two_l_dt", C.POINTER(C.c_double)), ("inv_c", C.POINTER(C.c_double)),
+                ("z", C.POINTER(C.c_double)), ("branch", C.c_int32), ("src_in_branch", C.c_int32)]
+
+
 class RunOutcomeC(C.Structure):
     _fields_ = [("reason", C.c_int32), ("ratio_db", C.c_double), ("steps", C.c_uint64),
                 ("timestep", C.c_uint64), ("time", C.c_double), ("energy", C.c_double),
@@ -176,6 +210,13 @@ ems_coefficients_from_cells = _sig("ems_coefficients_from_cells", _i, _dp, _i, _
 ems_edge_materials_from_cells = _sig("ems_edge_materials_from_cells", _i, _dp, _i, _dp, _i, _dp,
                                      _i, _u16p, _dp, _i, _i, _dp, _dp)
 ems_pool_lane = _sig("ems_pool_lane", _i, _fp, _i, _i, _i, _fp, _u32p, C.POINTER(_u64))
+ems_dispersive_fill_cells = _sig("ems_dispersive_fill_cells", _i, _dp, _i, _dp, _i, _dp, _i,
+                                 C.POINTER(C.c_int32), _i, C.POINTER(PoleEntryC), _i,
+                                 C.POINTER(PoleEntryC), _u64, C.POINTER(_u64))
+ems_dispersive_plan = _sig("ems_dispersive_plan", _i, _dp, _i, _dp, _i, _dp, _i, _d, _fp, _i,
+                           C.POINTER(PoleEntryC), _i, C.POINTER(PolePlanC))
+ems_lumped_plan = _sig("ems_lumped_plan", _i, C.POINTER(C.c_int32), _d,
+                       C.POINTER(LumpedElementC), _fp, _fp, C.POINTER(LumpedInstallC))
 ems_cpml_coefficients = _sig("ems_cpml_coefficients", None, C.POINTER(CpmlParams), _d, _d, _d,
                              _d, _dp)
 ems_cpml_layer_index = _sig("ems_cpml_layer_index", _d, _dp, _i, _dp, _i, _dp, _i, _fp, _d, _i, _i)

@@ -3,11 +3,13 @@
 // and the end-criterion run loop, restated from the reference in C++ with
 // the same f64 expressions (compiled without FP contraction).
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -1128,6 +1130,300 @@ double ems_conformal_stability(const ems_stability_desc *d) {
     for (int a = 0; a < 3; a++) std::swap(v[a], w[a]);
   }
   return std::sqrt(std::max(lambda, 0.0)) / 2.0;
+}
+
+}  // extern "C"
+
+// ---- setup producers: dispersive media and lumped elements ----
+//
+// What the reference's compile step runs on the host before the engine
+// steps: the edges a dispersive cell box reaches with their pole shares
+// (Dispersive::fill_cells), the install plan of a dispersive medium
+// (Dispersive::install) and of a lumped element (LumpedElement::install),
+// producing the descriptions ems_gpu_add_dispersive and ems_gpu_add_lumped
+// take.
+
+namespace {
+
+bool pole_valid(const ems_pole_entry &e) {
+  if (e.kind < 0 || e.kind > 2) return false;
+  const int np = e.kind == 0 ? 3 : 2;
+  for (int q = 0; q < np; q++)
+    if (!(std::isfinite(e.p[q]) && e.p[q] >= 0.0)) return false;
+  return true;
+}
+
+// Pole::scaled (dispersive.rs:186-204): a share s of the pole's strength.
+void pole_scale(ems_pole_entry &e, double s) {
+  if (e.kind == 2)
+    e.p[0] = e.p[0] * s;
+  else
+    e.p[0] = e.p[0] * std::sqrt(s);
+}
+
+// The pole as a series L, R and 1/C at edge capacitance (inductance) m
+// (dispersive.rs:300-345): Lorentz L = 1/(m wp^2), R = gamma L,
+// 1/C = w0^2 L; Drude the same without 1/C; Debye C = m d_eps, R = tau/C.
+void pole_lrc(const ems_pole_entry &e, double m, double *l, double *r, double *inv_c) {
+  if (e.kind == 0) {
+    const double wp = e.p[0], w0 = e.p[1], g = e.p[2];
+    *l = 1.0 / (m * wp * wp), *r = g * *l, *inv_c = w0 * w0 * *l;
+  } else if (e.kind == 1) {
+    const double wp = e.p[0], g = e.p[1];
+    *l = 1.0 / (m * wp * wp), *r = g * *l, *inv_c = 0.0;
+  } else {
+    const double c = m * e.p[0];
+    *l = 0.0, *r = e.p[1] / c, *inv_c = 1.0 / c;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ems_dispersive_fill_cells(const double *x, int nx, const double *y, int ny, const double *z,
+                              int nz, const int32_t cells[6], int field, const ems_pole_entry *poles,
+                              int n_poles, ems_pole_entry *out, uint64_t cap, uint64_t *count) {
+  if (!cells || !count || n_poles < 0 || (n_poles && !poles) || field < 0 || field > 1)
+    return EMS_E_INVALID;
+  for (int q = 0; q < n_poles; q++)
+    if (!pole_valid(poles[q])) return EMS_E_INVALID;
+  const Grid g = make_grid(x, nx, y, ny, z, nz);
+  const int lo[3] = {cells[0], cells[2], cells[4]}, hi[3] = {cells[1], cells[3], cells[5]};
+  for (int a = 0; a < 3; a++)
+    if (lo[a] < 0 || hi[a] > g.n[a] - 1 || lo[a] > hi[a]) return EMS_E_INVALID;
+  uint64_t w_out = 0;
+  *count = 0;
+  if (lo[0] == hi[0] || lo[1] == hi[1] || lo[2] == hi[2]) return EMS_OK;
+  auto inside = [&](int a, int c) { return lo[a] <= c && c < hi[a]; };
+  for (int a = 0; a < 3; a++) {
+    const int b = nxt(a), c = prv(a);
+    // along the edge's axis the cells, across it the nodes that touch them
+    auto lim = [&](int t, bool along) { return along ? hi[t] : std::min(hi[t] + 1, g.n[t]); };
+    const bool e = field == 0;
+    const int ea = lim(a, e), eb = lim(b, !e), ec = lim(c, !e);
+    for (int pa = lo[a]; pa < ea; pa++)
+      for (int pb = lo[b]; pb < eb; pb++)
+        for (int pc = lo[c]; pc < ec; pc++) {
+          double w_in = 0.0, w = 0.0;
+          if (e) {
+            for (int cb : {pb - 1, pb}) {
+              if (cb < 0 || cb >= g.n[b] - 1) continue;
+              for (int cc : {pc - 1, pc}) {
+                if (cc < 0 || cc >= g.n[c] - 1) continue;
+                const double area = g.delta(b, cb) * g.delta(c, cc);
+                w += area;
+                if (inside(b, cb) && inside(c, cc)) w_in += area;
+              }
+            }
+          } else {
+            for (int ca : {pa - 1, pa}) {
+              if (ca < 0 || ca >= g.n[a] - 1) continue;
+              const double ln = g.delta(a, ca);
+              w += ln;
+              if (inside(a, ca)) w_in += ln;
+            }
+          }
+          const double share = w > 0.0 ? w_in / w : 0.0;
+          if (!(share > 0.0)) continue;
+          for (int q = 0; q < n_poles; q++) {
+            ems_pole_entry en = poles[q];
+            pole_scale(en, share);
+            if (en.p[0] == 0.0) continue;  // null poles are not pushed
+            en.field = field, en.axis = a;
+            en.node[a] = pa, en.node[b] = pb, en.node[c] = pc;
+            if (out && w_out < cap) out[w_out] = en;
+            w_out++;
+          }
+        }
+  }
+  *count = w_out;
+  return out && w_out > cap ? EMS_E_INVALID : EMS_OK;
+}
+
+int ems_dispersive_plan(const double *x, int nx, const double *y, int ny, const double *z, int nz,
+                        double dt, const float *lanes, int scheme, const ems_pole_entry *entries,
+                        int n, ems_pole_plan *out) {
+  if (!lanes || !out || n < 0 || (n && !entries) || !(dt > 0.0) || scheme < 0 || scheme > 1)
+    return EMS_E_INVALID;
+  const Grid g = make_grid(x, nx, y, ny, z, nz);
+  const size_t N = (size_t)nx * ny * nz;
+  std::vector<int> ord;
+  for (int q = 0; q < n; q++) {
+    const ems_pole_entry &e = entries[q];
+    if (!pole_valid(e) || e.field < 0 || e.field > 1 || e.axis < 0 || e.axis > 2)
+      return EMS_E_INVALID;
+    const int a = e.axis, b = nxt(a), c = prv(a);
+    for (int t = 0; t < 3; t++)
+      if (e.node[t] < 0 || e.node[t] >= g.n[t]) return EMS_E_INVALID;
+    const bool fits = e.field == 0 ? e.node[a] + 1 < g.n[a]
+                                   : (e.node[b] + 1 < g.n[b] && e.node[c] + 1 < g.n[c]);
+    if (!fits) return EMS_E_INVALID;
+    if (e.p[0] == 0.0) continue;
+    ord.push_back(q);
+  }
+  // the edges in plane order (dispersive.rs:2090-2105): x, field, axis, y, z
+  auto key = [&](int q) {
+    const ems_pole_entry &e = entries[q];
+    return std::array<int, 5>{e.node[0], e.field, e.axis, e.node[1], e.node[2]};
+  };
+  std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return key(p) < key(q); });
+  std::map<std::array<uint64_t, 5>, int> cmap;
+  int32_t ne[2] = {0, 0};
+  uint64_t np[2] = {0, 0};
+  int32_t nc = 0, skipped = 0;
+  bool have_last = false;
+  std::array<int, 5> last{};
+  for (int q : ord) {
+    const ems_pole_entry &e = entries[q];
+    const int a = e.axis, b = nxt(a), c = prv(a), f = e.field;
+    const size_t idx = g.index(e.node[0], e.node[1], e.node[2]);
+    double coef, m;
+    if (f == 0) {
+      coef = (double)lanes[(size_t)(3 + a) * N + idx];
+      const double geo = g.dual_delta(b, e.node[b]) * g.dual_delta(c, e.node[c]) /
+                         g.delta(a, e.node[a]);
+      m = EPS0 * geo;
+    } else {
+      coef = (double)lanes[(size_t)(9 + a) * N + idx];
+      const double geo = g.delta(b, e.node[b]) * g.delta(c, e.node[c]) / g.dual_delta(a, e.node[a]);
+      m = MU0 * geo;
+    }
+    const std::array<int, 5> k = key(q);
+    const bool new_edge = !have_last || k != last;
+    have_last = true, last = k;
+    if (coef == 0.0) {  // PEC or padding: no medium there (dispersive.rs:2110-2118)
+      if (new_edge) skipped++;
+      continue;
+    }
+    if (new_edge) {
+      const int32_t s = ne[f]++;
+      out->axis[f][s] = a;
+      for (int t = 0; t < 3; t++) out->nodes[f][3 * s + t] = e.node[t];
+      out->sum_inv_z[f][s] = 0.0;
+      out->pole_end[f][s] = (uint32_t)np[f];
+    }
+    double l, r, inv_c, cst[4];
+    int32_t kind;
+    pole_lrc(e, m, &l, &r, &inv_c);
+    if (scheme == 1 && e.kind != 2) {
+      // leapfrog (dispersive.rs:375-393): k_a, 1/C, k_b, L/2
+      const double d = l / dt + 0.5 * r;
+      cst[0] = (l / dt - 0.5 * r) / d, cst[1] = inv_c, cst[2] = 1.0 / d, cst[3] = 0.5 * l;
+      kind = 1;
+    } else {
+      // trapezoidal branch (dispersive.rs:348-371): 2L/dt, 1/C, 1/z, (2L/dt) dt/4
+      const double a2 = 2.0 * l / dt;
+      const double zz = a2 + r + 0.5 * dt * inv_c;
+      cst[0] = a2, cst[1] = inv_c, cst[2] = 1.0 / zz, cst[3] = 0.25 * a2 * dt;
+      kind = 0;
+      out->sum_inv_z[f][ne[f] - 1] += cst[2];
+    }
+    std::array<uint64_t, 5> ck;
+    for (int t = 0; t < 4; t++) std::memcpy(&ck[t], &cst[t], sizeof(double));
+    ck[4] = (uint64_t)kind;
+    auto it = cmap.find(ck);
+    int ci;
+    if (it == cmap.end()) {
+      ci = nc++;
+      cmap.emplace(ck, ci);
+      out->a[ci] = cst[0], out->inv_c[ci] = cst[1], out->inv_z[ci] = cst[2];
+      out->half_l[ci] = cst[3], out->kind[ci] = kind;
+    } else {
+      ci = it->second;
+    }
+    out->cidx[f][np[f]++] = (uint32_t)ci;
+    out->pole_end[f][ne[f] - 1] = (uint32_t)np[f];
+  }
+  for (int f = 0; f < 2; f++) out->n_edges[f] = ne[f], out->n_poles[f] = np[f];
+  out->n_consts = nc;
+  out->skipped = skipped;
+  return EMS_OK;
+}
+
+int ems_lumped_plan(const int32_t nodes[3], double dt, const ems_lumped_element *el, float *ca,
+                    float *cb, ems_lumped_install *out) {
+  if (!nodes || !el || !ca || !cb || !out || !(dt > 0.0) || el->axis < 0 || el->axis > 2 ||
+      el->n_edges < 1 || !el->nodes || el->topology < 0 || el->topology > 1)
+    return EMS_E_INVALID;
+  const int ne = el->n_edges, a = el->axis;
+  const bool box = el->columns > 0;
+  if (!box && (!el->edge_r || !el->edge_l || !el->edge_c)) return EMS_E_INVALID;
+  auto has = [](double v) { return !std::isnan(v); };
+  auto ok = [](double v) { return std::isnan(v) || (std::isfinite(v) && v > 0.0); };
+  // which of R, L, C the element has (every edge the same set)
+  bool pr, pl, pc;
+  if (box) {
+    if (!ok(el->r) || !ok(el->l) || !ok(el->c) || el->series < 1) return EMS_E_INVALID;
+    pr = has(el->r), pl = has(el->l), pc = has(el->c);
+  } else {
+    pr = has(el->edge_r[0]), pl = has(el->edge_l[0]), pc = has(el->edge_c[0]);
+    for (int q = 0; q < ne; q++)
+      if (!ok(el->edge_r[q]) || !ok(el->edge_l[q]) || !ok(el->edge_c[q]) ||
+          has(el->edge_r[q]) != pr || has(el->edge_l[q]) != pl || has(el->edge_c[q]) != pc)
+        return EMS_E_INVALID;
+  }
+  const bool parallel = el->topology == 0;
+  const bool branch = parallel ? pl : (pl || pc);
+  const size_t N = (size_t)nodes[0] * nodes[1] * nodes[2];
+  for (int q = 0; q < ne; q++) {
+    const int32_t *p = el->nodes + 3 * q;
+    for (int t = 0; t < 3; t++)
+      if (p[t] < 0 || p[t] >= nodes[t]) return EMS_E_INVALID;
+    if (p[a] + 1 >= nodes[a]) return EMS_E_INVALID;
+    const size_t idx = ((size_t)p[0] * nodes[1] + p[1]) * nodes[2] + p[2];
+    if (idx >= N || cb[idx] == 0.0f) return EMS_E_INVALID;  // PEC edge
+  }
+  const double inf = INFINITY;
+  for (int q = 0; q < ne; q++) {
+    // the edge's share (lumped.rs:829-878): G, C in parallel, (R, L, C) of a branch
+    double ge, ce, br = 0.0, bl = 0.0, bc = inf;
+    bool eb = false;
+    if (box) {
+      const double m = (double)el->series, nn = (double)el->columns;
+      const double r = el->r, l = el->l, c = el->c;
+      if (parallel) {
+        ge = pr ? m / (nn * r) : 0.0, ce = pc ? m * c / nn : 0.0;
+        if (pl) eb = true, br = 0.0, bl = nn * l / m, bc = inf;
+      } else if (!branch) {
+        ge = pr ? m / (nn * r) : 0.0, ce = 0.0;
+      } else {
+        ge = 0.0, ce = 0.0, eb = true;
+        br = pr ? nn * r / m : 0.0, bl = pl ? nn * l / m : 0.0, bc = pc ? m * c / nn : inf;
+      }
+    } else {
+      const double r = el->edge_r[q], l = el->edge_l[q], c = el->edge_c[q];
+      if (parallel) {
+        ge = pr ? 1.0 / r : 0.0, ce = pc ? c : 0.0;
+        if (pl) eb = true, br = 0.0, bl = l, bc = inf;
+      } else if (!branch) {
+        ge = pr ? 1.0 / r : 0.0, ce = 0.0;
+      } else {
+        ge = 0.0, ce = 0.0, eb = true;
+        br = pr ? r : 0.0, bl = pl ? l : 0.0, bc = pc ? c : inf;
+      }
+    }
+    const int32_t *p = el->nodes + 3 * q;
+    const size_t idx = ((size_t)p[0] * nodes[1] + p[1]) * nodes[2] + p[2];
+    if (ge != 0.0 || ce != 0.0) {
+      float fa, fb;
+      ems_fold_parallel(ca[idx], cb[idx], dt, ge, ce, &fa, &fb);
+      ca[idx] = fa, cb[idx] = fb;
+    }
+    out->cb_e[q] = (double)cb[idx];
+    out->rec_g[q] = ge, out->rec_c[q] = ce;
+    out->g_src[q] = (branch && !parallel) ? 0.0 : ge;
+    if (branch) {
+      if (!eb) br = 0.0, bl = 0.0, bc = inf;
+      const double ic = std::isfinite(bc) ? 1.0 / bc : 0.0;
+      out->two_l_dt[q] = 2.0 * bl / dt;
+      out->inv_c[q] = ic;
+      out->z[q] = 2.0 * bl / dt + br + 0.5 * dt * ic;
+    }
+  }
+  out->branch = branch ? ((el->explicit_inductor && parallel) ? 2 : 1) : 0;
+  out->src_in_branch = branch && !parallel;
+  return EMS_OK;
 }
 
 }  // extern "C"

@@ -37,6 +37,7 @@ class Scheme(IntEnum):
 
 
 KIND_TRAPEZOIDAL, KIND_LEAPFROG = 0, 1
+POLE_KINDS = ("lorentz", "drude", "debye")  # ems_pole_entry.kind
 
 
 class Pole:
@@ -141,6 +142,12 @@ class Pole:
         return (l / dt - 0.5 * r) / d, 1.0 / d, inv_c, 0.5 * l
 
 
+def _entry(field: int, axis: int, node, pole: "Pole"):
+    """The ems_pole_entry of one pole on one edge."""
+    return nat.PoleEntryC(int(field), int(axis), (C.c_int32 * 3)(*node),
+                          POLE_KINDS.index(pole.kind), (C.c_double * 3)(*pole.p))
+
+
 class Dispersive:
     """Dispersive (dispersive.rs:1790-2700): poles on E edges (and H dual
     edges); install with ``engine.add_operator``."""
@@ -168,8 +175,9 @@ class Dispersive:
             self.entries.append((int(field), int(axis), tuple(int(v) for v in node), pole))
 
     def fill_cells(self, grid, cells, field: Field, poles: Sequence[Pole]) -> None:
-        """dispersive.rs:1896-1986: every edge touching the cell box gets the
-        poles scaled by the share of its averaging volume inside."""
+        """dispersive.rs:1896-1986 (ems_dispersive_fill_cells): every edge
+        touching the cell box gets the poles scaled by the share of its
+        averaging volume inside."""
         for p in poles:
             p.validate()
         nc = grid.cells()
@@ -177,50 +185,18 @@ class Dispersive:
         if any(cells[a][1] > nc[a] or cells[a][0] > cells[a][1] for a in range(3)):
             raise EmsError(nat.EMS_E_INVALID,
                            f"dispersive cell box {cells} does not fit the grid ({nc} cells)")
-        if any(r[0] == r[1] for r in cells):
+        if any(r[0] == r[1] for r in cells) or not poles:
             return
-        n = grid.nodes()
-        inside = lambda a, c: cells[a][0] <= c < cells[a][1]  # noqa: E731
-        for axis in Axis:
-            ab, ac = axis.others()
-            a, b, c = int(axis), int(ab), int(ac)
-
-            def rng(t, along):
-                return range(cells[t][0], cells[t][1]) if along else \
-                    range(cells[t][0], min(cells[t][1] + 1, n[t]))
-            if field == Field.Electric:
-                ra, rb, rc = rng(a, True), rng(b, False), rng(c, False)
-            else:
-                ra, rb, rc = rng(a, False), rng(b, True), rng(c, True)
-            for pa in ra:
-                for pb in rb:
-                    for pc in rc:
-                        node = [0, 0, 0]
-                        node[a], node[b], node[c] = pa, pb, pc
-                        w_in = w = 0.0
-                        if field == Field.Electric:
-                            for cb in (pb - 1, pb):
-                                if cb < 0 or cb >= n[b] - 1:
-                                    continue
-                                for cc in (pc - 1, pc):
-                                    if cc < 0 or cc >= n[c] - 1:
-                                        continue
-                                    area = grid.delta(b, cb) * grid.delta(c, cc)
-                                    w += area
-                                    if inside(b, cb) and inside(c, cc):
-                                        w_in += area
-                        else:
-                            for ca in (pa - 1, pa):
-                                if ca < 0 or ca >= n[a] - 1:
-                                    continue
-                                ln = grid.delta(a, ca)
-                                w += ln
-                                if inside(a, ca):
-                                    w_in += ln
-                        share = w_in / w if w > 0.0 else 0.0
-                        if share > 0.0:
-                            for p in poles:
-                                self.push(field, axis, node, p.scaled(share))
+        src = (nat.PoleEntryC * len(poles))(*[_entry(int(field), 0, (0, 0, 0), p) for p in poles])
+        box = (C.c_int32 * 6)(*[v for r in cells for v in r])
+        cnt = C.c_uint64()
+        args = (*grid._args(), box, int(field), src, len(poles))
+        nat.check(nat.ems_dispersive_fill_cells(*args, None, 0, C.byref(cnt)))
+        out = (nat.PoleEntryC * max(cnt.value, 1))()
+        nat.check(nat.ems_dispersive_fill_cells(*args, out, cnt.value, C.byref(cnt)))
+        for e in out[:cnt.value]:
+            self.entries.append((int(e.field), int(e.axis), tuple(int(v) for v in e.node),
+                                 Pole(POLE_KINDS[e.kind], e.p[0], e.p[1], e.p[2])))
 
     def fill_mask(self, grid, mask, field: Field, poles: Sequence[Pole]) -> None:
         """fill_cells for an arbitrary set of cells (mask: bool, cells-shaped):
@@ -280,7 +256,8 @@ class Dispersive:
 
     def _plan(self, grid, dt: float, lanes: np.ndarray) -> dict:
         """Dispersive::install (dispersive.rs:2063-2319) and the side layout
-        of slab_hooks (:2609-2770)."""
+        of slab_hooks (:2609-2770), by the host library
+        (ems_dispersive_plan)."""
         if self._installed is not None:
             raise EmsError(nat.EMS_E_INVALID, "dispersive medium already installed")
         n = grid.nodes()
@@ -296,58 +273,41 @@ class Dispersive:
                 raise EmsError(nat.EMS_E_INVALID,
                                f"dispersive {Field(field).name} edge {Axis(axis).name}:{node} "
                                f"is not in the grid ({n} nodes)")
-        key = lambda e: (e[2][0], e[0], e[1], e[2][1], e[2][2])  # noqa: E731
-        order = sorted(range(len(self.entries)), key=lambda q: key(self.entries[q]))
-        flat = [lanes[l].reshape(-1) for l in range(12)]
-        sides: Dict[int, dict] = {f: {"axis": [], "nodes": [], "sum_inv_z": [], "pole_end": [],
-                                      "cidx": []} for f in (0, 1)}
-        consts: List[tuple] = []
-        cmap: Dict[tuple, int] = {}
-        skipped = 0
-        last = None
-        for q in order:
-            field, axis, node, pole = self.entries[q]
-            k = key(self.entries[q])
-            ab, ac = (int(x) for x in Axis(axis).others())
-            idx = grid.index(node)
-            if field == Field.Electric:
-                coef = float(flat[3 + axis][idx])
-                geo = grid.dual_delta(ab, node[ab]) * grid.dual_delta(ac, node[ac]) / \
-                    grid.delta(axis, node[axis])
-                m = EPS0 * geo
-            else:
-                coef = float(flat[9 + axis][idx])
-                geo = grid.delta(ab, node[ab]) * grid.delta(ac, node[ac]) / \
-                    grid.dual_delta(axis, node[axis])
-                m = MU0 * geo
-            new_edge = last != k
-            last = k
-            if coef == 0.0:
-                if new_edge:
-                    skipped += 1
-                continue
-            s = sides[field]
-            if new_edge:
-                s["axis"].append(axis)
-                s["nodes"].append(node)
-                s["sum_inv_z"].append(0.0)
-                s["pole_end"].append(len(s["cidx"]))
-            leap = pole.leapfrog(m, dt) if self._scheme == Scheme.Leapfrog else None
-            if leap is not None:
-                ka, kb, inv_c, half_l = leap
-                cst = (ka, inv_c, kb, half_l, KIND_LEAPFROG)
-            else:
-                a_, inv_c, z = pole.branch(m, dt)
-                cst = (a_, inv_c, 1.0 / z, 0.25 * a_ * dt, KIND_TRAPEZOIDAL)
-                s["sum_inv_z"][-1] += cst[2]
-            ck = tuple(np.float64(v).tobytes() for v in cst[:4]) + (cst[4],)
-            if ck not in cmap:
-                cmap[ck] = len(consts)
-                consts.append(cst)
-            s["cidx"].append(cmap[ck])
-            s["pole_end"][-1] = len(s["cidx"])
-        self._skipped = skipped
-        return {"sides": sides, "consts": consts, "skipped": skipped}
+        ne = len(self.entries)
+        cap = max(ne, 1)
+        ents = (nat.PoleEntryC * cap)(*[_entry(f, a, nd, p) for f, a, nd, p in self.entries])
+        keep = {}
+
+        def buf(name, dtype, size):
+            keep[name] = np.zeros(size, dtype=dtype)
+            return keep[name]
+        plan = nat.PolePlanC()
+        for f in (0, 1):
+            plan.axis[f] = nat.ptr(buf(f"axis{f}", np.int32, cap), C.c_int32)
+            plan.nodes[f] = nat.ptr(buf(f"nodes{f}", np.int32, 3 * cap), C.c_int32)
+            plan.sum_inv_z[f] = nat.ptr(buf(f"siz{f}", np.float64, cap), C.c_double)
+            plan.pole_end[f] = nat.ptr(buf(f"pe{f}", np.uint32, cap), C.c_uint32)
+            plan.cidx[f] = nat.ptr(buf(f"cidx{f}", np.uint32, cap), C.c_uint32)
+        for name in ("a", "inv_c", "inv_z", "half_l"):
+            setattr(plan, name, nat.ptr(buf(name, np.float64, cap), C.c_double))
+        plan.kind = nat.ptr(buf("kind", np.int32, cap), C.c_int32)
+        dense = np.ascontiguousarray(lanes, dtype=np.float32)
+        nat.check(nat.ems_dispersive_plan(*grid._args(), float(dt), nat.ptr(dense, C.c_float),
+                                          int(self._scheme), ents, ne, C.byref(plan)))
+        sides: Dict[int, dict] = {}
+        for f in (0, 1):
+            m, q = plan.n_edges[f], plan.n_poles[f]
+            nd = keep[f"nodes{f}"][:3 * m].reshape(-1, 3)
+            sides[f] = {"axis": [int(v) for v in keep[f"axis{f}"][:m]],
+                        "nodes": [tuple(int(v) for v in r) for r in nd],
+                        "sum_inv_z": [float(v) for v in keep[f"siz{f}"][:m]],
+                        "pole_end": [int(v) for v in keep[f"pe{f}"][:m]],
+                        "cidx": [int(v) for v in keep[f"cidx{f}"][:q]]}
+        k = plan.n_consts
+        consts = [(float(keep["a"][i]), float(keep["inv_c"][i]), float(keep["inv_z"][i]),
+                   float(keep["half_l"][i]), int(keep["kind"][i])) for i in range(k)]
+        self._skipped = int(plan.skipped)
+        return {"sides": sides, "consts": consts, "skipped": self._skipped}
 
     # -- results
     def _bind(self, engine, oid: int, plan: dict) -> None:

@@ -286,85 +286,66 @@ class LumpedElement:
                 raise EmsError(nat.EMS_E_INVALID,
                                f"lumped edge {self._axis.name}:{n} is outside the grid")
 
-    def _edge_parts(self):
-        """Per-edge (g_e, C_e, branch (R_e, L_e, C_e) or None) (lumped.rs:829-878)."""
-        has_branch = self.has_branch()
-        inf = math.inf
-        if self._box is not None:
-            m, n = float(self._box.series_edges()), float(self._box.columns())
-            r, l, c = self.r_val, self.l_val, self.c_val
-            if self._topology == Topology.Parallel:
-                p = (0.0 if r is None else m / (n * r), 0.0 if c is None else m * c / n,
-                     None if l is None else (0.0, n * l / m, inf))
-            elif not has_branch:
-                p = (0.0 if r is None else m / (n * r), 0.0, None)
-            else:
-                p = (0.0, 0.0, (0.0 if r is None else n * r / m, 0.0 if l is None else n * l / m,
-                                inf if c is None else m * c / n))
-            return [p] * len(self._box.edges())
-        out = []
-        for e in self._edge_list:
-            if self._topology == Topology.Parallel:
-                out.append((0.0 if e.r is None else 1.0 / e.r, 0.0 if e.c is None else e.c,
-                            None if e.l is None else (0.0, e.l, inf)))
-            elif not has_branch:
-                out.append((0.0 if e.r is None else 1.0 / e.r, 0.0, None))
-            else:
-                out.append((0.0, 0.0, (0.0 if e.r is None else e.r, 0.0 if e.l is None else e.l,
-                                       inf if e.c is None else e.c)))
-        return out
-
     def _plan(self, grid, dt: float, lanes: np.ndarray) -> dict:
-        """LumpedElement::install (lumped.rs:880-1012) on the host: folds R
-        and C into copies of the axis's ca/cb lanes and returns them with
-        the device description."""
+        """LumpedElement::install (lumped.rs:880-1012) on the host, by the
+        host library (ems_lumped_plan): folds R and C into copies of the
+        axis's ca/cb lanes and returns them with the device description."""
         if self._installed is not None:
             raise EmsError(nat.EMS_E_INVALID, "lumped element already installed")
         self.validate()
         nodes = grid.nodes()
         self._check_fits(nodes)
         edges = self.edge_nodes()
-        parts = self._edge_parts()
         a = int(self._axis)
         shape = lanes[a].shape
-        ca = lanes[a].reshape(-1).copy()
-        cb = lanes[3 + a].reshape(-1).copy()
-        idx = [(p[0] * nodes[1] + p[1]) * nodes[2] + p[2] for p in edges]
-        for q, i in enumerate(idx):
-            if cb[i] == 0.0:
+        ca = np.ascontiguousarray(lanes[a].reshape(-1), dtype=np.float32).copy()
+        cb = np.ascontiguousarray(lanes[3 + a].reshape(-1), dtype=np.float32).copy()
+        for p in edges:
+            if cb[(p[0] * nodes[1] + p[1]) * nodes[2] + p[2]] == 0.0:
                 raise EmsError(nat.EMS_E_INVALID,
-                               f"lumped element edge {self._axis.name}:{edges[q]} is PEC "
+                               f"lumped element edge {self._axis.name}:{p} is PEC "
                                "(metal or a PEC outer face)")
-        cbs = []
-        for i, (g_e, c_e, _) in zip(idx, parts):
-            if g_e != 0.0 or c_e != 0.0:
-                ca[i], cb[i] = fold_parallel(ca[i], cb[i], dt, g_e, c_e)
-            cbs.append(float(cb[i]))
         ne = len(edges)
-        has_branch = self.has_branch()
-        branch = 0
-        two_l_dt = inv_c = z = None
-        if has_branch:
-            branch = 2 if (self._explicit and self._topology == Topology.Parallel) else 1
-            two_l_dt, inv_c, z = [], [], []
-            for _, _, br in parts:
-                r, l, c = br if br is not None else (0.0, 0.0, math.inf)
-                ic = 1.0 / c if math.isfinite(c) else 0.0
-                two_l_dt.append(2.0 * l / dt)
-                inv_c.append(ic)
-                z.append(2.0 * l / dt + r + 0.5 * dt * ic)
-        src_in_branch = has_branch and self._topology == Topology.Series
-        g_src = [0.0 if src_in_branch else p[0] for p in parts]
+        nd = np.ascontiguousarray(np.array(edges, dtype=np.int32).reshape(-1))
+        nan = math.nan
+        val = lambda v: nan if v is None else float(v)  # noqa: E731
+        el = nat.LumpedElementC()
+        el.axis, el.n_edges, el.nodes = a, ne, nat.ptr(nd, C.c_int32)
+        el.topology, el.explicit_inductor = int(self._topology), 1 if self._explicit else 0
+        keep = [nd]
+        if self._box is not None:
+            el.series, el.columns = self._box.series_edges(), self._box.columns()
+            el.r, el.l, el.c = val(self.r_val), val(self.l_val), val(self.c_val)
+        else:
+            el.series, el.columns = self._series, 0
+            for name, attr in (("edge_r", "r"), ("edge_l", "l"), ("edge_c", "c")):
+                arr = np.ascontiguousarray([val(getattr(e, attr)) for e in self._edge_list],
+                                           dtype=np.float64)
+                keep.append(arr)
+                setattr(el, name, nat.ptr(arr, C.c_double))
+        outs = {k: np.zeros(ne, dtype=np.float64)
+                for k in ("cb_e", "g_src", "rec_g", "rec_c", "two_l_dt", "inv_c", "z")}
+        inst = nat.LumpedInstallC()
+        for k, v in outs.items():
+            setattr(inst, k, nat.ptr(v, C.c_double))
+        nat.check(nat.ems_lumped_plan((C.c_int32 * 3)(*nodes), float(dt), C.byref(el),
+                                      nat.ptr(ca, C.c_float), nat.ptr(cb, C.c_float),
+                                      C.byref(inst)))
+        branch = int(inst.branch)
+        listed = lambda k: [float(v) for v in outs[k]]  # noqa: E731
         terms: List[Tuple[_Source, List[float]]] = []
         if self._source is not None:
             terms.append((self._source, [1.0 / self._series] * ne))
         terms += self._edge_sources
-        return {"ca": ca.reshape(shape), "cb": cb.reshape(shape), "axis": a, "edges": edges, "cb_e": cbs, "g_src": g_src,
-                "series": self._series, "src_in_branch": src_in_branch, "terms": terms,
-                "drive": list(self._drive_sources), "branch": branch, "two_l_dt": two_l_dt,
-                "inv_c": inv_c, "z": z,
-                "rec_g": [p[0] for p in parts] if self._record else None,
-                "rec_c": [p[1] for p in parts] if self._record else None,
+        return {"ca": ca.reshape(shape), "cb": cb.reshape(shape), "axis": a, "edges": edges,
+                "cb_e": listed("cb_e"), "g_src": listed("g_src"), "series": self._series,
+                "src_in_branch": bool(inst.src_in_branch), "terms": terms,
+                "drive": list(self._drive_sources), "branch": branch,
+                "two_l_dt": listed("two_l_dt") if branch else None,
+                "inv_c": listed("inv_c") if branch else None,
+                "z": listed("z") if branch else None,
+                "rec_g": listed("rec_g") if self._record else None,
+                "rec_c": listed("rec_c") if self._record else None,
                 "columns": 0}
 
     # -- results

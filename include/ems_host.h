@@ -221,6 +221,87 @@ int ems_run_until_batch(ems_gpu_batch *b, int n_models, ems_gpu *const *models,
 int ems_drive_script(const double *w, uint64_t nw, const double *signal, uint64_t ns,
                      const ems_run_limits *limits, ems_run_outcome *out);
 
+/* ---- setup producers (the reference's install steps on the host) ---- */
+
+/* One pole of a dispersive medium on one edge (Dispersive::push,
+ * src/engine/dispersive.rs:1870-1880): field 0 electric (E edge), 1
+ * magnetic (H dual edge); kind 0 Lorentz (omega_p, omega_0, gamma), 1 Drude
+ * (omega_p, gamma), 2 Debye (delta_eps, tau); every parameter finite and
+ * >= 0. A pole with p[0] = 0 is null and ignored. */
+typedef struct {
+    int32_t field, axis, node[3], kind;
+    double p[3];
+} ems_pole_entry;
+
+/* Dispersive::fill_cells (dispersive.rs:1896-1986): every edge of `field`
+ * touching the cell box cells = {x0, x1, y0, y1, z0, z1} (half-open cell
+ * ranges) gets each of the n_poles poles scaled by the share of its
+ * averaging area (E) or length (H) inside the box (Pole::scaled: omega_p
+ * by sqrt(share), a Debye delta_eps by the share), in the order axis x, y,
+ * z, then the edge's nodes along its axis and the two others, then the
+ * poles. Up to cap entries go to out (NULL to count); *count = all. */
+int ems_dispersive_fill_cells(const double *x, int nx, const double *y, int ny, const double *z,
+                              int nz, const int32_t cells[6], int field, const ems_pole_entry *poles,
+                              int n_poles, ems_pole_entry *out, uint64_t cap, uint64_t *count);
+
+/* The install plan of a dispersive medium (Dispersive::install,
+ * dispersive.rs:2063-2319), i.e. what ems_gpu_add_dispersive takes: per
+ * side (0 electric, 1 magnetic) the edges in plane order (x, field, axis,
+ * y, z) with the sum of their poles' 1/z and the end of their poles in
+ * cidx, and the distinct pole constants (a, 1/C, 1/z or k_b, L/4 dt or
+ * L/2, kind 0 trapezoidal, 1 leapfrog; leapfrog for Lorentz and Drude
+ * poles with scheme 1) at the edge's capacitance eps0 A/l (inductance
+ * mu0 A/l). Edges whose cb (db) is 0 are skipped and counted. The caller
+ * sizes every array for n entries (nodes 3 per edge). */
+typedef struct {
+    int32_t n_edges[2];
+    int32_t *axis[2], *nodes[2];
+    double *sum_inv_z[2];
+    uint32_t *pole_end[2];
+    uint64_t n_poles[2];
+    uint32_t *cidx[2];
+    int32_t n_consts;
+    double *a, *inv_c, *inv_z, *half_l;
+    int32_t *kind;
+    int32_t skipped;
+} ems_pole_plan;
+int ems_dispersive_plan(const double *x, int nx, const double *y, int ny, const double *z, int nz,
+                        double dt, const float *lanes, int scheme, const ems_pole_entry *entries,
+                        int n, ems_pole_plan *out);
+
+/* LumpedElement::install (src/engine/lumped.rs:829-1012): the edges'
+ * shares of an element's R, L, C (lumped.rs:829-878), the parallel R and C
+ * folded into their ca, cb (fold_parallel), and the RLC branch parameters.
+ * The element's n_edges edges (nodes, 3 per edge) run along axis; with
+ * columns > 0 the element's r, l, c (NaN: absent) are shared by `series`
+ * edges in series times `columns` in parallel (a box), otherwise every
+ * edge has its own edge_r, edge_l, edge_c (NaN: absent, the same set on
+ * every edge). topology 0 Parallel (R and C folded, L a branch), 1 Series
+ * (R, L, C in a branch when L or C is present); explicit_inductor: a
+ * parallel inductor integrated explicitly (branch 2). */
+typedef struct {
+    int32_t axis, n_edges;
+    const int32_t *nodes;
+    int32_t series, columns;
+    double r, l, c;
+    const double *edge_r, *edge_l, *edge_c;
+    int32_t topology, explicit_inductor;
+} ems_lumped_element;
+/* Out, n_edges each: cb_e (the edge's folded cb), g_src (the Thevenin
+ * conductance, 0 in a series branch), rec_g, rec_c (the edge's parallel
+ * G and C), and with a branch two_l_dt = 2L/dt, inv_c, z = 2L/dt + R +
+ * dt/(2C); branch 0 none, 1 the trapezoidal branch, 2 the explicit
+ * inductor; src_in_branch for a series branch. */
+typedef struct {
+    double *cb_e, *g_src, *rec_g, *rec_c, *two_l_dt, *inv_c, *z;
+    int32_t branch, src_in_branch;
+} ems_lumped_install;
+/* ca, cb: the dense lanes of the element's axis (nodes[0]*nodes[1]*nodes[2]
+ * floats each), folded in place. EMS_E_INVALID for an edge outside the
+ * grid, a PEC edge (cb = 0) or a value that is not finite and > 0. */
+int ems_lumped_plan(const int32_t nodes[3], double dt, const ems_lumped_element *el, float *ca,
+                    float *cb, ems_lumped_install *out);
+
 #ifdef __cplusplus
 }
 #endif
