@@ -569,11 +569,21 @@ __device__ __forceinline__ void stf(float *base, unsigned boff, float v) {
 // wave-uniform). ro[12] flags the lanes whose row holds one value: that
 // value is read once per wave through the scalar unit instead of a vector
 // load per lane (the bits are the same).
-__device__ __forceinline__ float ldc(const float *__restrict__ cpool, const uint32_t *ro, int l,
+// CT is the table's element type: uint32_t for pools under 4 GiB (one base
+// for every load, the row offset added to the lane's in a VGPR), uint64_t
+// for larger pools (the row's address a scalar sum, more scalar registers).
+template <typename CT>
+__device__ __forceinline__ float ldc(const float *__restrict__ cpool, const CT *ro, int l,
                                      unsigned k4) {
-  const uint32_t o = cload(ro, l);
-  if ((cload(ro, 12) >> l) & 1u) return cload((const float *)((const char *)cpool + o));
-  return ldf(cpool, o + k4);
+  if constexpr (sizeof(CT) == 4) {
+    const uint32_t o = cload(ro, l);
+    if ((cload(ro, 12) >> l) & 1u) return cload((const float *)((const char *)cpool + o));
+    return ldf(cpool, o + k4);
+  } else {
+    const float *row = (const float *)((const char *)cpool + cload(ro, l));
+    if ((cload(ro, 12) >> l) & 1u) return cload(row);
+    return ldf(row, k4);
+  }
 }
 
 // Coefficient of lane L at row r (i*ny + j, wave-uniform) at byte offset
@@ -596,8 +606,9 @@ struct PlaneIn {
 // Field buffer F holds u_x, u_y, u_z, i_x, i_y, i_z, N nodes each; the
 // coefficient pool holds every lane's rows, and croff[16*r + l] is the
 // offset of lane l's row for node row r (i*ny + j).
+template <typename CT = uint32_t>
 __device__ __forceinline__ void load_plane(PlaneIn &P, const float *F, size_t N,
-                                           const float *cpool, const uint32_t *croff, size_t ob,
+                                           const float *cpool, const CT *croff, size_t ob,
                                            size_t r, size_t plane, int nz, unsigned k4, bool iv,
                                            bool jv, bool kv, bool need_e = true) {
   // Branch-free, every address valid (neighbour offsets fall back to the
@@ -616,10 +627,10 @@ __device__ __forceinline__ void load_plane(PlaneIn &P, const float *F, size_t N,
   P.ix = ldf(F + 3 * N + ob, k4);
   P.iy = ldf(F + 4 * N + ob, k4);
   P.iz = ldf(F + 5 * N + ob, k4);
-  // One base for all lanes and a 32-bit byte offset per lane (the pool is
-  // kept under 4 GiB; croff holds byte offsets), so the coefficients need
-  // no 64-bit address math.
-  const uint32_t *ro = croff + 16 * r;
+  // The coefficients at the pool base (or, with 64-bit row offsets, the
+  // row's scalar address) plus a 32-bit byte offset per lane: no 64-bit
+  // vector address math.
+  const CT *ro = croff + 16 * r;
   // Rows that only update H (a tile's halo row, the plane before a chunk;
   // need_e is wave-uniform) skip the E coefficients.
 #pragma unroll
@@ -654,10 +665,10 @@ __device__ __forceinline__ void h_from(const PlaneIn &P, float uyc, float uzc, b
 
 // Tile t of the fused step (k tiles fastest, then j tiles, then x chunks):
 // the body shared by the single-model and the batched kernels.
-template <bool OE, int TJ, bool PF, bool CP = true>
+template <bool OE, int TJ, bool PF, bool CP = true, typename CT = uint32_t>
 __device__ __forceinline__ void fused_tile(const float *__restrict__ A, float *__restrict__ B,
                                            size_t N, const float *__restrict__ cpool,
-                                           const uint32_t *__restrict__ croff,
+                                           const CT *__restrict__ croff,
                                            const FSlab *__restrict__ slabs,
                                            const uint32_t *__restrict__ pmask, int nslab, int par,
                                            const DSlab *__restrict__ hout,
@@ -813,18 +824,18 @@ __host__ __device__ __forceinline__ int tile_of_block(int b, int ktiles, int nti
 
 // tiles: the tile of each block (one class of them), or null for all tiles
 // in tile_of_block's order.
-template <bool OE, int TJ, bool PF, bool CP>
+template <bool OE, int TJ, bool PF, bool CP, typename CT>
 __global__ void __launch_bounds__(64 * (TJ + 1))
     step_fused(const float *__restrict__ A, float *__restrict__ B, size_t N,
-               const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
+               const float *__restrict__ cpool, const CT *__restrict__ croff,
                const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
                int par, const DSlab *__restrict__ hout, const uint32_t *__restrict__ pmask_o,
                int nhout, int nx, int ny, int nz, int tk, int chunk, int ktiles, int jtiles,
                int ntiles, int xcd_group, const int *__restrict__ tiles) {
   const int t = tiles ? cload(tiles, blockIdx.x) : tile_of_block(blockIdx.x, ktiles, ntiles, xcd_group);
   if (t < 0) return;
-  fused_tile<OE, TJ, PF, CP>(A, B, N, cpool, croff, slabs, pmask, nslab, par, hout, pmask_o, nhout,
-                             nx, ny, nz, tk, chunk, ktiles, jtiles, t);
+  fused_tile<OE, TJ, PF, CP, CT>(A, B, N, cpool, croff, slabs, pmask, nslab, par, hout, pmask_o,
+                                 nhout, nx, ny, nz, tk, chunk, ktiles, jtiles, t);
 }
 
 // ---- fused step, bands of rows per wave ----
@@ -1376,21 +1387,21 @@ __global__ void pool_row_const(const float *pool, size_t prows, int nz, uint8_t 
 
 // croff[16*r + l]: byte offset in the pool of lane l's row for node row r;
 // croff[16*r + 12]: bit l set when that row is constant.
-__global__ void build_row_offsets(uint32_t *croff, const uint32_t *const *index,
-                                  const uint32_t *base, const uint8_t *flags, size_t rows,
-                                  int nz) {
+template <typename CT>
+__global__ void build_row_offsets(CT *croff, const uint32_t *const *index, const uint64_t *base,
+                                  const uint8_t *flags, size_t rows, int nz) {
   const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
-  uint32_t cm = 0;
+  CT cm = 0;
   for (int l = 0; l < 16; l++) {
-    uint32_t v = 0;
+    CT v = 0;
     if (l < 12) {
       const uint32_t *ix = index[l];
       // Lanes without rows point at the pool's zero row (base[l] == 0).
-      uint32_t prow = 0;
-      if (base[l] != 0) prow = base[l] / (uint32_t)nz + (ix ? ix[r] : (uint32_t)r);
-      v = 4u * prow * (uint32_t)nz;
-      cm |= (uint32_t)flags[prow] << l;
+      uint64_t prow = 0;
+      if (base[l] != 0) prow = base[l] / (uint64_t)nz + (ix ? (uint64_t)ix[r] : (uint64_t)r);
+      v = (CT)(4ull * prow * (uint64_t)nz);
+      cm |= (CT)flags[prow] << l;
     }
     croff[16 * r + l] = l == 12 ? cm : v;
   }
@@ -2694,7 +2705,8 @@ struct ems_gpu {
   float *fbuf[2] = {nullptr, nullptr};  // u_x..u_z, i_x..i_z, N nodes each
   // coefficient pool: a zero row, then every lane's rows; croff[16*r + l]
   float *cpool = nullptr;
-  uint32_t *croff = nullptr;
+  void *croff = nullptr;   // uint32_t entries, or uint64_t when croff_wide
+  bool croff_wide = false;  // the pool is 4 GiB or more (only the tiled kernel addresses it)
   bool lanes_dirty = true;
   int par = 0;     // buffer holding the current state
   Lane lanes[12]{};
@@ -3058,16 +3070,25 @@ int launch_cpml_out(ems_gpu *h, int par, bool after) {
   return 0;
 }
 
+template <bool OE, int TJ, bool PF, bool CP, typename CT>
+void launch_fused_ct(ems_gpu *h, int par, const FusedGeom &g, int nblocks, const int *tiles) {
+  const float *A = h->fbuf[par];
+  float *B = h->fbuf[1 - par];
+  hipLaunchKernelGGL((step_fused<OE, TJ, PF, CP, CT>), dim3(nblocks), dim3(64, TJ + 1), 0,
+                     h->stream, A, B, h->N, (const float *)h->cpool, (const CT *)h->croff,
+                     h->d_fslabs, (const uint32_t *)h->d_pmask, n_fslabs(h), par,
+                     (const DSlab *)h->d_hout, (const uint32_t *)h->d_pmask_o, h->n_hout, h->n[0],
+                     h->n[1], h->n[2], g.tk, g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group,
+                     tiles);
+}
+
 template <bool OE, int TJ, bool PF, bool CP>
 void launch_fused_one(ems_gpu *h, int par, const FusedGeom &g, int nblocks, const int *tiles) {
   if (nblocks <= 0) return;
-  const float *A = h->fbuf[par];
-  float *B = h->fbuf[1 - par];
-  hipLaunchKernelGGL((step_fused<OE, TJ, PF, CP>), dim3(nblocks), dim3(64, TJ + 1), 0, h->stream,
-                     A, B, h->N, (const float *)h->cpool, (const uint32_t *)h->croff, h->d_fslabs,
-                     (const uint32_t *)h->d_pmask, n_fslabs(h), par, (const DSlab *)h->d_hout,
-                     (const uint32_t *)h->d_pmask_o, h->n_hout, h->n[0], h->n[1], h->n[2], g.tk,
-                     g.chunk, g.ktiles, g.jtiles, g.ntiles, h->xcd_group, tiles);
+  if (h->croff_wide)
+    launch_fused_ct<OE, TJ, PF, CP, uint64_t>(h, par, g, nblocks, tiles);
+  else
+    launch_fused_ct<OE, TJ, PF, CP, uint32_t>(h, par, g, nblocks, tiles);
 }
 
 // The tiled step: one launch over every tile, or with tile classes the
@@ -3238,9 +3259,11 @@ int launch_band(ems_gpu *h, int par) {
 }
 
 int launch_fused(ems_gpu *h, int par) {
-  if (h->band) return launch_band(h, par);
-  if (h->packed && packed_ok(h)) return launch_packed(h, par);
-  if (h->rows && rows_ok(h)) return launch_rows(h, par);
+  // (the band, packed and full-row kernels take 32-bit row offsets: with a
+  // pool of 4 GiB or more the tiled kernel runs)
+  if (h->band && !h->croff_wide) return launch_band(h, par);
+  if (h->packed && packed_ok(h) && !h->croff_wide) return launch_packed(h, par);
+  if (h->rows && rows_ok(h) && !h->croff_wide) return launch_rows(h, par);
   const FusedGeom g = fused_geom(h);
   const int key = h->tj * 2 + (h->prefetch ? 1 : 0);
   int rc = launch_cpml_out(h, par, false);
@@ -3484,9 +3507,6 @@ int build_pool(ems_gpu *h) {
       base[l] = total;
       total += h->lane_rows[l] * (uint64_t)nz;
     }
-  // The fused kernel addresses the pool with 32-bit byte offsets.
-  if (total * sizeof(float) >= (1ull << 32))
-    return fail(h, EMS_E_UNSUPPORTED, "coefficient pool over 4 GiB");
   float *pool = nullptr;
   CK(h, hipMalloc(&pool, total * sizeof(float)));
   CK(h, hipMemset(pool, 0, (size_t)nz * sizeof(float)));
@@ -3494,24 +3514,37 @@ int build_pool(ems_gpu *h) {
     if (h->lanes[l].rows)
       CK(h, hipMemcpy(pool + base[l], h->lanes[l].rows, h->lane_rows[l] * nz * sizeof(float),
                       hipMemcpyDeviceToDevice));
-  if (!h->croff) CK(h, hipMalloc(&h->croff, rows * 16 * sizeof(uint32_t)));
+  // 32-bit row offsets while the pool stays under 4 GiB, else 64-bit ones
+  const bool wide = total * sizeof(float) >= (1ull << 32);
+  if (h->croff && wide != h->croff_wide) {
+    CK(h, hipFree(h->croff));
+    h->croff = nullptr;
+  }
+  h->croff_wide = wide;
+  if (!h->croff)
+    CK(h, hipMalloc(&h->croff, rows * 16 * (wide ? sizeof(uint64_t) : sizeof(uint32_t))));
   const uint32_t *ix[12];
-  uint32_t b32[12];
-  for (int l = 0; l < 12; l++) ix[l] = h->lanes[l].index, b32[l] = (uint32_t)base[l];
+  uint64_t b64[12];
+  for (int l = 0; l < 12; l++) ix[l] = h->lanes[l].index, b64[l] = base[l];
   void *d_ix = nullptr, *d_b = nullptr;
   CK(h, hipMalloc(&d_ix, sizeof ix));
-  CK(h, hipMalloc(&d_b, sizeof b32));
+  CK(h, hipMalloc(&d_b, sizeof b64));
   CK(h, hipMemcpy(d_ix, ix, sizeof ix, hipMemcpyHostToDevice));
-  CK(h, hipMemcpy(d_b, b32, sizeof b32, hipMemcpyHostToDevice));
+  CK(h, hipMemcpy(d_b, b64, sizeof b64, hipMemcpyHostToDevice));
   const size_t prows = (size_t)(total / (uint64_t)nz);
   uint8_t *d_flags = nullptr;
   CK(h, hipMalloc(&d_flags, prows));
   hipLaunchKernelGGL(pool_row_const, dim3((unsigned)((prows + 255) / 256)), dim3(256), 0,
                      h->stream, (const float *)pool, prows, nz, d_flags);
   CK(h, hipGetLastError());
-  hipLaunchKernelGGL(build_row_offsets, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0,
-                     h->stream, h->croff, (const uint32_t *const *)d_ix, (const uint32_t *)d_b,
-                     (const uint8_t *)d_flags, rows, nz);
+  if (wide)
+    hipLaunchKernelGGL(build_row_offsets<uint64_t>, dim3((unsigned)((rows + 255) / 256)), dim3(256),
+                       0, h->stream, (uint64_t *)h->croff, (const uint32_t *const *)d_ix,
+                       (const uint64_t *)d_b, (const uint8_t *)d_flags, rows, nz);
+  else
+    hipLaunchKernelGGL(build_row_offsets<uint32_t>, dim3((unsigned)((rows + 255) / 256)), dim3(256),
+                       0, h->stream, (uint32_t *)h->croff, (const uint32_t *const *)d_ix,
+                       (const uint64_t *)d_b, (const uint8_t *)d_flags, rows, nz);
   CK(h, hipGetLastError());
   CK(h, hipStreamSynchronize(h->stream));
   CK(h, hipFree(d_ix));
@@ -3538,7 +3571,8 @@ int build_pool(ems_gpu *h) {
 // Whether steps run the tiled fused kernel (the one that corrects the
 // z-normal CPML slabs outside itself).
 bool tiled_fused(const ems_gpu *h) {
-  return use_fused(h) && !h->band && !(h->packed && packed_ok(h)) && !(h->rows && rows_ok(h));
+  return use_fused(h) && ((!h->band && !(h->packed && packed_ok(h)) && !(h->rows && rows_ok(h))) ||
+                          h->croff_wide);
 }
 
 // The tile classes of the tiled step for the present geometry and slabs
@@ -5462,7 +5496,8 @@ int batch_args(ems_gpu_batch *b, const std::vector<int> &act) {
       const FusedGeom g = b->packed ? packed_geom(h, b->nchunk) : fused_geom(h);
       FusedArgs &a = fa[q];
       a.A = h->fbuf[p], a.B = h->fbuf[1 - p], a.N = h->N;
-      a.cpool = h->cpool, a.croff = h->croff, a.slabs = h->d_fslabs, a.pmask = h->d_pmask;
+      a.cpool = h->cpool, a.croff = (const uint32_t *)h->croff, a.slabs = h->d_fslabs,
+      a.pmask = h->d_pmask;
       a.nslab = n_fslabs(h), a.par = p, a.nx = h->n[0], a.ny = h->n[1], a.nz = h->n[2];
       a.tk = g.tk, a.chunk = g.chunk, a.ktiles = g.ktiles, a.jtiles = g.jtiles, a.ntiles = g.ntiles;
       if (p == 0) seg[q + 1] = seg[q] + (uint32_t)g.nblocks;
@@ -5637,6 +5672,9 @@ int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
     if (h->par != par) return bfail(b, EMS_E_INVALID, "models of a batch step in lockstep");
     const int rc = prepare(h, n, false);
     if (rc) return bfail(b, rc, h->err);
+    if (h->croff_wide)
+      return bfail(b, EMS_E_UNSUPPORTED,
+                   "model " + std::to_string(q) + " has a coefficient pool of 4 GiB or more");
     BCK(b, hipStreamSynchronize(h->stream));
   }
   int rc = batch_args(b, act);

@@ -132,3 +132,38 @@ def test_cpml_box_512_full_size(cpml_box_512, sched):
     same_fields(e, o, "cpml_box_512")
     e.close()
     o.close()
+
+
+def test_coefficient_pool_over_4_gib():
+    """A 601^3-node graded box: six dense coefficient lanes make a 5.2 GB
+    pool, addressed by the fused kernel through 64-bit row offsets; 3 steps
+    from seeded fields bit for bit against the oracle (every field and the
+    f64 energy)."""
+    from oracle.bridge import OracleEngine
+    n = 600
+    rng = np.random.default_rng(600)
+    lines = [np.concatenate([[0.0], np.cumsum(1e-3 * (0.8 + 0.4 * rng.random(n)))])
+             for _ in range(3)]
+    g = E.Grid(lines)
+    dt = 0.99 * E.courant_limit(g)
+    lanes = E.coefficients(g, None, [E.FaceCondition.Pec] * 6, dt)
+    dense = sum(int(np.any(lanes[l] != lanes[l].reshape(-1)[0])) for l in range(12))
+    assert dense >= 6
+    assert 4 * dense * g.num_nodes() > (1 << 32)
+    init = seeded_fields(g.nodes(), 601)
+    e = E.Engine(g, dt, lanes)
+    for field, a, v in init:
+        e.set_component(field, a, v)
+    e.setup.clear()
+    e.run(3)
+    we = e.energy()
+    o = OracleEngine(g, dt, lanes, threads=oracle_threads())
+    for field, a, v in init:
+        o.set_component(field, a, v)
+    del init
+    o.run(3)
+    wo = o.energy()
+    assert np.float64(we).tobytes() == np.float64(wo).tobytes(), (we, wo)
+    same_fields(e, o, "pool_over_4_gib")
+    e.close()
+    o.close()
