@@ -375,9 +375,9 @@ def main() -> int:
     if te == 0.0:
         name, dur, byt = "step_fused", th, hb
     elif th >= te:
-        name, dur, byt = "h_update", th, hb
+        name, dur, byt = "h_update_pf" if eng.unfused_prefetch & 1 else "h_update", th, hb
     else:
-        name, dur, byt = "e_update", te, eb
+        name, dur, byt = "e_update_pf" if eng.unfused_prefetch & 2 else "e_update", te, eb
     achieved = byt / (dur * 1e-3) / 1e9
     # HBM bytes per launch from the committed rocprof counter passes
     # (FETCH_SIZE, WRITE_SIZE, scaled by the copy-kernel calibration) of this

@@ -201,6 +201,7 @@ ems_gpu_set_schedule = _sig("ems_gpu_set_schedule", _i, _p, _i, _i, _i)
 ems_gpu_set_xcd_group = _sig("ems_gpu_set_xcd_group", _i, _p, _i)
 ems_gpu_set_cpml_split = _sig("ems_gpu_set_cpml_split", _i, _p, _i)
 ems_gpu_set_tile_classes = _sig("ems_gpu_set_tile_classes", _i, _p, _i)
+ems_gpu_set_unfused_prefetch = _sig("ems_gpu_set_unfused_prefetch", _i, _p, _i)
 ems_gpu_copy_bandwidth = _sig("ems_gpu_copy_bandwidth", _i, _p, _u64, _i, _dp)
 
 # include/ems_host.h

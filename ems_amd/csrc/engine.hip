@@ -222,6 +222,136 @@ __global__ void __launch_bounds__(256) e_update(Fields f, Lanes L, int nx, int n
   }
 }
 
+// ---- unfused H and E updates, software pipelined along x ----
+//
+// The same updates (same expressions, same values written) with a thread
+// marching a chunk of x-planes and the next plane's inputs loaded before
+// the present plane is computed, so the loads of one plane overlap the
+// arithmetic and stores of the one before. Every load address is valid
+// (neighbours past the grid's end fall back to the node itself; those
+// values are not used).
+
+// The block of a one-dimensional launch of gx * nch blocks that takes
+// block b's place, so that each XCD runs a contiguous share of the
+// (row block, x chunk) pairs: a launch hands its blocks to the 8 XCDs in
+// turn (block b to XCD b % 8), so block b + 1, whose rows neighbour block
+// b's, would otherwise run beside another L2. A bijection of [0, nb).
+__device__ __forceinline__ int xcd_contiguous(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, x = b % 8, q = b / 8;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + q;
+}
+
+// What the H update of one node reads (kernel.rs:443-511).
+struct HNode {
+  float uxc, uyn, uzn, ix, iy, iz, uz_j1, uy_k1, ux_k1, ux_j1;
+  float da[3], db[3];
+};
+
+__device__ __forceinline__ void h_node_load(HNode &q, const Fields &f, const Lanes &L, size_t o,
+                                            size_t r, int k, int nz, size_t plane, bool iv,
+                                            bool jv, bool kv) {
+  const GAS float *ux = (const GAS float *)f.u[0], *uy = (const GAS float *)f.u[1];
+  const GAS float *uz = (const GAS float *)f.u[2];
+  const size_t oi = iv ? o + plane : o, oj = jv ? o + nz : o, ok = kv ? o + 1 : o;
+  q.uxc = ux[o], q.uyn = uy[oi], q.uzn = uz[oi];
+  q.ix = ((const GAS float *)f.i[0])[o], q.iy = ((const GAS float *)f.i[1])[o];
+  q.iz = ((const GAS float *)f.i[2])[o];
+  q.uz_j1 = uz[oj], q.ux_j1 = ux[oj], q.uy_k1 = uy[ok], q.ux_k1 = ux[ok];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    q.da[a] = coefg(L.l[6 + a], r, k, nz, o);
+    q.db[a] = coefg(L.l[9 + a], r, k, nz, o);
+  }
+}
+
+template <bool OE>
+__global__ void __launch_bounds__(256) h_update_pf(Fields f, Lanes L, int nx, int ny, int nz,
+                                                   int chunk) {
+  const int plane = ny * nz, gx = (plane + 255) / 256;
+  const int b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int p = (b % gx) * blockDim.x + threadIdx.x;
+  if (p >= plane) return;
+  const int j = p / nz, k = p - j * nz;
+  const int i0 = (b / gx) * chunk;
+  const int i1 = min(nx, i0 + chunk);
+  const bool jv = j < ny - 1, kv = k < nz - 1;
+  GAS float *ix = (GAS float *)f.i[0], *iy = (GAS float *)f.i[1], *iz = (GAS float *)f.i[2];
+  size_t o = (size_t)i0 * plane + p;
+  float uyc = ((const GAS float *)f.u[1])[o], uzc = ((const GAS float *)f.u[2])[o];
+  HNode cur, nxt;
+  h_node_load(cur, f, L, o, (size_t)i0 * ny + j, k, nz, plane, i0 + 1 < nx, jv, kv);
+  for (int i = i0; i < i1; i++, o += plane) {
+    const bool iv = i + 1 < nx;
+    if (i + 1 < i1)
+      h_node_load(nxt, f, L, o + plane, (size_t)(i + 1) * ny + j, k, nz, plane, i + 2 < nx, jv,
+                  kv);
+    const float uyn = iv ? cur.uyn : 0.0f, uzn = iv ? cur.uzn : 0.0f;
+    if (jv && kv)  // curl_x = (uz[j+1] - uz[j]) - (uy[k+1] - uy[k])
+      ix[o] = h_expr<OE>(cur.da[0], cur.ix, cur.db[0], cur.uz_j1, uzc, cur.uy_k1, uyc);
+    if (iv && kv)  // curl_y = (ux[k+1] - ux[k]) - (uz[i+1] - uz[i])
+      iy[o] = h_expr<OE>(cur.da[1], cur.iy, cur.db[1], cur.ux_k1, cur.uxc, uzn, uzc);
+    if (iv && jv)  // curl_z = (uy[i+1] - uy[i]) - (ux[j+1] - ux[j])
+      iz[o] = h_expr<OE>(cur.da[2], cur.iz, cur.db[2], uyn, uyc, cur.ux_j1, cur.uxc);
+    uyc = uyn;
+    uzc = uzn;
+    cur = nxt;
+  }
+}
+
+// What the E update of one node reads (kernel.rs:620-697).
+struct ENode {
+  float ix, iy, iz, iz_jm, iy_km, ix_km, ix_jm, ux, uy, uz;
+  float ca[3], cb[3];
+};
+
+__device__ __forceinline__ void e_node_load(ENode &q, const Fields &f, const Lanes &L, size_t o,
+                                            size_t r, int j, int k, int nz) {
+  const GAS float *ix = (const GAS float *)f.i[0], *iy = (const GAS float *)f.i[1];
+  const GAS float *iz = (const GAS float *)f.i[2];
+  const size_t oj = j > 0 ? o - nz : o, ok = k > 0 ? o - 1 : o;
+  q.ix = ix[o], q.iy = iy[o], q.iz = iz[o];
+  q.iz_jm = iz[oj], q.ix_jm = ix[oj], q.iy_km = iy[ok], q.ix_km = ix[ok];
+  q.ux = ((const GAS float *)f.u[0])[o], q.uy = ((const GAS float *)f.u[1])[o];
+  q.uz = ((const GAS float *)f.u[2])[o];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    q.ca[a] = coefg(L.l[a], r, k, nz, o);
+    q.cb[a] = coefg(L.l[3 + a], r, k, nz, o);
+  }
+}
+
+template <bool OE>
+__global__ void __launch_bounds__(256) e_update_pf(Fields f, Lanes L, int nx, int ny, int nz,
+                                                   int chunk) {
+  const int plane = ny * nz, gx = (plane + 255) / 256;
+  const int b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int p = (b % gx) * blockDim.x + threadIdx.x;
+  if (p >= plane) return;
+  const int j = p / nz, k = p - j * nz;
+  const int i0 = (b / gx) * chunk;
+  const int i1 = min(nx, i0 + chunk);
+  GAS float *ux = (GAS float *)f.u[0], *uy = (GAS float *)f.u[1], *uz = (GAS float *)f.u[2];
+  size_t o = (size_t)i0 * plane + p;
+  float iyp = i0 > 0 ? ((const GAS float *)f.i[1])[o - plane] : 0.0f;
+  float izp = i0 > 0 ? ((const GAS float *)f.i[2])[o - plane] : 0.0f;
+  ENode cur, nxt;
+  e_node_load(cur, f, L, o, (size_t)i0 * ny + j, j, k, nz);
+  for (int i = i0; i < i1; i++, o += plane) {
+    if (i + 1 < i1) e_node_load(nxt, f, L, o + plane, (size_t)(i + 1) * ny + j, j, k, nz);
+    const float hz_jm = j > 0 ? cur.iz_jm : 0.0f, hx_jm = j > 0 ? cur.ix_jm : 0.0f;
+    const float hy_km = k > 0 ? cur.iy_km : 0.0f, hx_km = k > 0 ? cur.ix_km : 0.0f;
+    if (i + 1 < nx)  // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
+      ux[o] = e_expr<OE>(cur.ca[0], cur.ux, cur.cb[0], cur.iz, hz_jm, cur.iy, hy_km);
+    if (j + 1 < ny)  // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
+      uy[o] = e_expr<OE>(cur.ca[1], cur.uy, cur.cb[1], cur.ix, hx_km, cur.iz, izp);
+    if (k + 1 < nz)  // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
+      uz[o] = e_expr<OE>(cur.ca[2], cur.uz, cur.cb[2], cur.iy, iyp, cur.ix, hx_jm);
+    iyp = cur.iy;
+    izp = cur.iz;
+    cur = nxt;
+  }
+}
+
 // CPML slab correction (cpml.rs:469-543, 693-736).
 struct SlabArg {
   int e_side, comp, other, normal;
@@ -2731,6 +2861,7 @@ struct ems_gpu {
   uint64_t *d_step = nullptr;
   // fused schedule and its tiling
   bool fused = true, prefetch = true, rows = false, packed = false, band = false;
+  int unfused_pf = 1;  // ems_gpu_set_unfused_prefetch: bit 0 H, bit 1 E
   int xcd_group = 2;  // tile rows per XCD visit (2 measured best of 0, 1, 2, 4, 8)
   int tj = 7, nchunk = 0;
   // tile classes of the tiled step (ems_gpu_set_tile_classes): the tiles a
@@ -2766,13 +2897,15 @@ int fail(ems_gpu *h, int code, const std::string &m) {
     }                                                                                   \
   } while (0)
 
-int x_chunk(const ems_gpu *h, int *nchunks) {
-  // Two planes per thread by default: the kernels are latency-bound, and
-  // more threads beat the reuse of a carried neighbour plane (measured on
-  // 385 x 385 x 257 nodes: 11 chunks 1.73 ms per step, 2-plane chunks
-  // 1.56 ms, 1-plane 1.57 ms). The schedule's x chunks, where set, take
-  // precedence.
-  int nch = h->nchunk > 0 ? h->nchunk : (h->n[0] + 1) / 2;
+int x_chunk(const ems_gpu *h, int *nchunks, bool pf) {
+  // Two planes per thread by default for the plain kernels: they are
+  // latency-bound, and more threads beat the reuse of a carried neighbour
+  // plane (measured on 385 x 385 x 257 nodes: 11 chunks 1.73 ms per step,
+  // 2-plane chunks 1.56 ms, 1-plane 1.57 ms). The kernels that load the
+  // next plane ahead march 8 planes (H on the same grid: 0.548 ms with
+  // 8-plane chunks, 0.565 ms with 32, 0.569 ms with 2; 0.647 ms plain). The
+  // schedule's x chunks, where set, take precedence.
+  int nch = h->nchunk > 0 ? h->nchunk : (pf ? (h->n[0] + 7) / 8 : (h->n[0] + 1) / 2);
   if (nch < 1) nch = 1;
   if (nch > h->n[0]) nch = h->n[0];
   int chunk = (h->n[0] + nch - 1) / nch;
@@ -2871,9 +3004,17 @@ int launch_poles(ems_gpu *h, int stage, const Fields &f, const Fields &fp, bool 
 }
 
 int launch_h(ems_gpu *h, const Fields &f) {
-  int nch, chunk = x_chunk(h, &nch);
+  int nch, chunk = x_chunk(h, &nch, (h->unfused_pf & 1) != 0);
   dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
-  if (h->oe)
+  const dim3 line(grid.x * grid.y);  // the pf kernels' one-dimensional launch
+  if (h->unfused_pf & 1) {
+    if (h->oe)
+      hipLaunchKernelGGL(h_update_pf<true>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
+                         h->n[0], h->n[1], h->n[2], chunk);
+    else
+      hipLaunchKernelGGL(h_update_pf<false>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
+                         h->n[0], h->n[1], h->n[2], chunk);
+  } else if (h->oe)
     hipLaunchKernelGGL(h_update<true>, grid, dim3(256), 0, h->stream, f, lanes_arg(h), h->n[0],
                        h->n[1], h->n[2], chunk);
   else
@@ -2884,9 +3025,17 @@ int launch_h(ems_gpu *h, const Fields &f) {
 }
 
 int launch_e(ems_gpu *h, const Fields &f) {
-  int nch, chunk = x_chunk(h, &nch);
+  int nch, chunk = x_chunk(h, &nch, (h->unfused_pf & 2) != 0);
   dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
-  if (h->oe)
+  const dim3 line(grid.x * grid.y);
+  if (h->unfused_pf & 2) {
+    if (h->oe)
+      hipLaunchKernelGGL(e_update_pf<true>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
+                         h->n[0], h->n[1], h->n[2], chunk);
+    else
+      hipLaunchKernelGGL(e_update_pf<false>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
+                         h->n[0], h->n[1], h->n[2], chunk);
+  } else if (h->oe)
     hipLaunchKernelGGL(e_update<true>, grid, dim3(256), 0, h->stream, f, lanes_arg(h), h->n[0],
                        h->n[1], h->n[2], chunk);
   else
@@ -4949,6 +5098,14 @@ int ems_gpu_set_cpml_split(ems_gpu *h, int on) {
   return EMS_OK;
 }
 
+int ems_gpu_set_unfused_prefetch(ems_gpu *h, int which) {
+  ALIVE(h);
+  if (which < 0 || which > 3) return fail(h, EMS_E_INVALID, "unfused prefetch: 0 to 3");
+  h->unfused_pf = which;
+  drop_graphs(h);
+  return EMS_OK;
+}
+
 int ems_gpu_set_tile_classes(ems_gpu *h, int on) {
   ALIVE(h);
   h->tile_class = on != 0;
@@ -5337,8 +5494,10 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
       CK(h, hipEventElapsedTime(&c, ev[0], ev[4]));
     } else {
       const Fields &f = h->fb[h->par];
-      CK(h, hipEventRecord(ev[0], h->stream));
+      // The bracket starts after the side stream's launches are enqueued,
+      // so that it times the H kernel rather than the host's enqueueing.
       if ((rc = merged_fork(h, f))) return rc;
+      CK(h, hipEventRecord(ev[0], h->stream));
       if ((rc = launch_h(h, f))) return rc;
       CK(h, hipEventRecord(ev[1], h->stream));
       if ((rc = launch_upml(h, f, 0))) return rc;

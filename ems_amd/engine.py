@@ -746,6 +746,7 @@ class Engine:
         self._dense_lanes = lanes
         self._owns_lanes = False
         self._pool = pool
+        self.unfused_prefetch = 1
         self.setup: List[tuple] = []
         self.ops: list = []
         self._tables: list = []
@@ -1106,6 +1107,13 @@ class Engine:
         and the others in two launches (the latter without the slab code);
         the same values either way."""
         self._check(nat.ems_gpu_set_tile_classes(self._h, 1 if on else 0))
+
+    def set_unfused_prefetch(self, which: int = 1) -> None:
+        """Which unfused kernels (bit 0 H, bit 1 E; default 1) march longer x
+        chunks with the next plane's inputs loaded ahead; the same values
+        either way."""
+        self._check(nat.ems_gpu_set_unfused_prefetch(self._h, int(which)))
+        self.unfused_prefetch = int(which)
 
     def set_xcd_group(self, group: int) -> None:
         """Tile rows per XCD visit in the fused kernels' block order (0:

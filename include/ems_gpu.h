@@ -467,6 +467,14 @@ int ems_gpu_set_cpml_split(ems_gpu *h, int on);
  * others in a second one, with a kernel instantiation without the slab
  * code (fewer registers, more waves per SIMD). Same results either way. */
 int ems_gpu_set_tile_classes(ems_gpu *h, int on);
+/* The unfused H and E kernels (no reference counterpart): `which` bit 0
+ * (H) and bit 1 (E) pick the kernels whose threads march a chunk of
+ * x-planes (8 by default; the schedule's x chunks where set) with the next
+ * plane's inputs loaded before the present plane is computed; the others
+ * march two planes without that. Default 1 (H only: the loads ahead cost
+ * the E kernel occupancy for no gain). Same results either way; 0 to 3,
+ * else EMS_E_INVALID. */
+int ems_gpu_set_unfused_prefetch(ems_gpu *h, int which);
 
 /* Tile order of the fused kernels (no reference counterpart): groups of
  * `group` consecutive tile rows go round-robin to the 8 XCDs (their L2s),
