@@ -136,6 +136,16 @@ __device__ __forceinline__ float coefg(const Lane &L, size_t r, int k, int nz, s
   return rows[(size_t)((const GAS uint32_t *)L.index)[r] * nz + k];
 }
 
+// The block of a one-dimensional launch of gx * nch blocks that takes
+// block b's place, so that each XCD runs a contiguous share of the
+// (row block, x chunk) pairs: a launch hands its blocks to the 8 XCDs in
+// turn (block b to XCD b % 8), so block b + 1, whose rows neighbour block
+// b's, would otherwise run beside another L2. A bijection of [0, nb).
+__device__ __forceinline__ int xcd_contiguous(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, x = b % 8, q = b / 8;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + q;
+}
+
 // Faraday update (kernel.rs:443-511, 566-618): plane-wise in x, all three
 // components at node (i, j, k); padding entries are never written.
 template <bool OE>
@@ -230,16 +240,6 @@ __global__ void __launch_bounds__(256) e_update(Fields f, Lanes L, int nx, int n
 // arithmetic and stores of the one before. Every load address is valid
 // (neighbours past the grid's end fall back to the node itself; those
 // values are not used).
-
-// The block of a one-dimensional launch of gx * nch blocks that takes
-// block b's place, so that each XCD runs a contiguous share of the
-// (row block, x chunk) pairs: a launch hands its blocks to the 8 XCDs in
-// turn (block b to XCD b % 8), so block b + 1, whose rows neighbour block
-// b's, would otherwise run beside another L2. A bijection of [0, nb).
-__device__ __forceinline__ int xcd_contiguous(int b, int nb) {
-  const int per = nb / 8, rem = nb % 8, x = b % 8, q = b / 8;
-  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + q;
-}
 
 // What the H update of one node reads (kernel.rs:443-511).
 struct HNode {
@@ -2327,6 +2327,30 @@ __global__ void __launch_bounds__(256) mf_drive(Fields f, DevMerged m) {
   if (t < m.ng) m.cur[t] = m.next[t];
 }
 
+// MergedFaces::energy's terms (conformal.rs:2684-2698), the host's
+// expressions: per merge group q < ns, l/2 g_q g'_q; per block row r,
+// g_r/2 (M g')_r with (M g')_r folded from -0.0 in column order. `ml`
+// holds l, `mv` the M values, `mrs`/`mcol` the M rows, `row0` each row's
+// block start; g' (next) is the look-ahead update.
+__global__ void __launch_bounds__(256) mf_energy_terms(DevMerged m, const double *__restrict__ ml,
+                                                       const uint32_t *__restrict__ mrs,
+                                                       const uint32_t *__restrict__ mcol,
+                                                       const double *__restrict__ mv,
+                                                       const uint32_t *__restrict__ row0,
+                                                       int nrows, double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m.ns) {
+    out[t] = 0.5 * ml[t] * m.cur[t] * m.next[t];
+    return;
+  }
+  const int r = t - m.ns;
+  if (r >= nrows) return;
+  const uint32_t r0 = row0[r];
+  double mn = -0.0;
+  for (uint32_t k = mrs[r]; k < mrs[r + 1]; k++) mn += mv[k] * m.next[m.mem[r0 + mcol[k]]];
+  out[t] = 0.5 * m.cur[m.mem[r]] * mn;
+}
+
 // ---- dispersive media (dispersive.rs) ----
 
 struct DevPoleConst {
@@ -2343,6 +2367,18 @@ struct DevPoleSide {
   double *j, *q;
   float *prev;  // the field at the start of the update
 };
+
+// Side::add_energy's terms (dispersive.rs:1489-1517), the host's
+// expression: per pole half_l j^2 + inv_c q^2 / 2, `hc` the constants'
+// (half_l, inv_c) pairs.
+__global__ void __launch_bounds__(256) pole_energy_terms(DevPoleSide s, int np,
+                                                         const double *__restrict__ hc,
+                                                         double *__restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const uint32_t c = s.cidx[p];
+  out[p] = hc[2 * c] * s.j[p] * s.j[p] + 0.5 * hc[2 * c + 1] * s.q[p] * s.q[p];
+}
 
 // Side::correct / step_edge (dispersive.rs:1183-1343, 1392-1470) for
 // trapezoidal and leapfrog poles, a thread per edge: `fld` the field the
@@ -2789,6 +2825,13 @@ struct HostOp {
   uint32_t *mf_small = nullptr, *mf_big = nullptr;
   int mf_nsmall = 0, mf_nbig = 0, mf_lds_small = 0, mf_lds_big = 0;
   std::shared_ptr<struct MergedHost> mfh;
+  // the operator's energy terms: device array, pinned host copy, count;
+  // the device tables the terms need (pole constants; reduced currents' l,
+  // M rows, block starts)
+  double *d_et = nullptr, *h_et = nullptr;
+  size_t n_et = 0;
+  double *d_ec = nullptr;
+  uint32_t *d_erow0 = nullptr;
   // Mur boundary
   DevMur mur{};
   // openEMS's PML layers; upml_pos: device slot of each edge in the caller's order
@@ -2831,6 +2874,7 @@ struct ems_gpu {
   // first such operator; default priority)
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_energy = nullptr;  // the energy's copies have landed
   Fields fb[2]{};  // ping-pong field buffers (views into fbuf)
   float *fbuf[2] = {nullptr, nullptr};  // u_x..u_z, i_x..i_z, N nodes each
   // coefficient pool: a zero row, then every lane's rows; croff[16*r + l]
@@ -3006,7 +3050,9 @@ int launch_poles(ems_gpu *h, int stage, const Fields &f, const Fields &fp, bool 
 int launch_h(ems_gpu *h, const Fields &f) {
   int nch, chunk = x_chunk(h, &nch, (h->unfused_pf & 1) != 0);
   dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
-  const dim3 line(grid.x * grid.y);  // the pf kernels' one-dimensional launch
+  // the loading-ahead kernel runs one-dimensional (see xcd_contiguous; the
+  // two-plane kernel measured slower so: E 0.526 against 0.503 ms)
+  const dim3 line(grid.x * grid.y);
   if (h->unfused_pf & 1) {
     if (h->oe)
       hipLaunchKernelGGL(h_update_pf<true>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
@@ -3027,6 +3073,8 @@ int launch_h(ems_gpu *h, const Fields &f) {
 int launch_e(ems_gpu *h, const Fields &f) {
   int nch, chunk = x_chunk(h, &nch, (h->unfused_pf & 2) != 0);
   dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
+  // the loading-ahead kernel runs one-dimensional (see xcd_contiguous; the
+  // two-plane kernel measured slower so: E 0.526 against 0.503 ms)
   const dim3 line(grid.x * grid.y);
   if (h->unfused_pf & 2) {
     if (h->oe)
@@ -4104,6 +4152,7 @@ void ems_gpu_destroy(ems_gpu *h) {
     (void)hipFree(o.d_wd), (void)hipFree(o.d_amp), (void)hipFree(o.d_vals);
     (void)hipFree(o.d_count);
     for (void *p : o.mem) (void)hipFree(p);
+    if (o.h_et) (void)hipHostFree(o.h_et);
     for (double *p : o.tables) (void)hipFree(p);
     if (o.pw_src) (void)hipFree(o.pw_src);
     if (o.lp.rec) (void)hipFree(o.lp.rec), (void)hipFree(o.lp.pv), (void)hipFree(o.lp.pi);
@@ -4117,6 +4166,7 @@ void ems_gpu_destroy(ems_gpu *h) {
   if (h->stream) (void)hipStreamDestroy(h->stream);
   if (h->side) (void)hipStreamDestroy(h->side);
   if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_energy) (void)hipEventDestroy(h->ev_energy);
   if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   if (h->h_w) (void)hipHostFree(h->h_w);
   delete h;
@@ -5131,67 +5181,121 @@ int ems_gpu_run(ems_gpu *h, uint64_t n) {
 // has 0 but a lumped element with a branch (branch_energy,
 // lumped.rs:1075-1087, from the device state); the CPML slabs are one
 // operator.
-static int ops_energy(ems_gpu *h, double *out) {
+// The operators' energies in two parts. ops_energy_enqueue queues on the
+// engine's stream what each needs from the device and its copy into pinned
+// host memory: the poles' terms, the reduced currents' terms after their
+// look-ahead update (written to the update's own result buffer, which the
+// next step's update overwrites before anything reads it), the lumped
+// branches' state. ops_energy_finish, once that copy has landed (the
+// engine's ev_energy), adds them in the order of the sums it restates.
+static int ops_energy_enqueue(ems_gpu *h) {
+  for (auto &o : h->ops) {
+    size_t n = 0;
+    if (o.type == OP_POLES)
+      n = o.side_cidx[0].size() + o.side_cidx[1].size();
+    else if (o.type == OP_MERGED)
+      n = (size_t)o.mf.ns + o.mfh->mem.size();
+    else if (o.type == OP_LUMPED && o.lp.branch)
+      n = 2 * (size_t)o.lp.n;
+    if (!n) continue;
+    if (!o.h_et) {
+      CK(h, hipHostMalloc((void **)&o.h_et, n * sizeof(double), hipHostMallocDefault));
+      o.n_et = n;
+      if (o.type != OP_LUMPED) {
+        CK(h, hipMalloc(&o.d_et, n * sizeof(double)));
+        o.mem.push_back(o.d_et);
+      }
+      if (o.type == OP_POLES) {
+        std::vector<double> hc(2 * o.c_half_l.size());
+        for (size_t c = 0; c < o.c_half_l.size(); c++)
+          hc[2 * c] = o.c_half_l[c], hc[2 * c + 1] = o.c_inv_c[c];
+        int rc = upload(h, (void **)&o.d_ec, hc.data(), hc.size() * sizeof(double));
+        if (rc) return rc;
+        o.mem.push_back(o.d_ec);
+      }
+      if (o.type == OP_MERGED) {
+        const MergedHost &mh = *o.mfh;
+        const size_t nr = mh.mem.size(), nmv = mh.mval.size();
+        std::vector<uint32_t> row0(nr);
+        for (size_t b = 0; b + 1 < mh.bs.size(); b++)
+          for (uint32_t r = mh.bs[b]; r < mh.bs[b + 1]; r++) row0[r] = mh.bs[b];
+        // d_ec: l (ng), then the M values; d_erow0: block starts, M row
+        // starts (nr + 1), M columns
+        std::vector<double> dv(mh.l);
+        dv.insert(dv.end(), mh.mval.begin(), mh.mval.end());
+        std::vector<uint32_t> uv(row0);
+        uv.insert(uv.end(), mh.mrs.begin(), mh.mrs.end());
+        uv.insert(uv.end(), mh.mcol.begin(), mh.mcol.end());
+        int rc = upload(h, (void **)&o.d_ec, dv.data(), dv.size() * sizeof(double));
+        if (rc) return rc;
+        o.mem.push_back(o.d_ec);
+        if ((rc = upload(h, (void **)&o.d_erow0, uv.data(), uv.size() * sizeof(uint32_t))))
+          return rc;
+        o.mem.push_back(o.d_erow0);
+        (void)nmv;
+      }
+    }
+    if (o.type == OP_POLES) {
+      size_t off = 0;
+      for (int sd = 0; sd < 2; sd++) {
+        const int np = (int)o.side_cidx[sd].size();
+        if (np)
+          hipLaunchKernelGGL(pole_energy_terms, dim3((np + 255) / 256), dim3(256), 0, h->stream,
+                             o.side[sd], np, (const double *)o.d_ec, o.d_et + off);
+        off += (size_t)np;
+      }
+    } else if (o.type == OP_MERGED) {
+      const DevMerged &m = o.mf;
+      const int na = o.mf_nsmall + (m.ns + 63) / 64;
+      if (na)
+        hipLaunchKernelGGL(mf_advance, dim3(na), dim3(64), (size_t)o.mf_lds_small, h->stream, m,
+                           (const uint32_t *)o.mf_small, o.mf_nsmall, o.mf_lds_small);
+      if (o.mf_nbig)
+        hipLaunchKernelGGL(mf_advance, dim3(o.mf_nbig), dim3(64), (size_t)o.mf_lds_big, h->stream,
+                           m, (const uint32_t *)o.mf_big, o.mf_nbig, o.mf_lds_big);
+      const int nr = (int)o.mfh->mem.size(), ng = (int)o.mfh->l.size();
+      const uint32_t *row0 = o.d_erow0, *mrs = row0 + nr, *mcol = mrs + nr + 1;
+      hipLaunchKernelGGL(mf_energy_terms, dim3((unsigned)((o.n_et + 255) / 256)), dim3(256), 0,
+                         h->stream, m, (const double *)o.d_ec, mrs, mcol,
+                         (const double *)o.d_ec + ng, row0, nr, o.d_et);
+    }
+    CK(h, hipGetLastError());
+    if (o.type == OP_LUMPED) {
+      const size_t ne = (size_t)o.lp.n;
+      CK(h, hipMemcpyAsync(o.h_et, o.lp.j, ne * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+      CK(h, hipMemcpyAsync(o.h_et + ne, o.lp.q, ne * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+    } else {
+      CK(h, hipMemcpyAsync(o.h_et, o.d_et, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+  }
+  return 0;
+}
+
+static double ops_energy_finish(const ems_gpu *h) {
   double w = -0.0;
   if (!h->slabs.empty()) w += 0.0;
-  bool synced = false;
   for (auto &o : h->ops) {
     if (o.type == OP_POLES) {
       // Dispersive::energy (dispersive.rs:2789-2794): electric + magnetic,
       // each summed from 0.0 in pole order (Side::add_energy :1489-1517).
-      if (!synced) CK(h, hipStreamSynchronize(h->stream));
-      synced = true;
       double side_w[2] = {0.0, 0.0};
+      size_t off = 0;
       for (int sd = 0; sd < 2; sd++) {
-        const std::vector<uint32_t> &ci = o.side_cidx[sd];
-        const size_t np = ci.size();
-        if (!np) continue;
-        std::vector<double> j(np), q(np);
-        CK(h, hipMemcpy(j.data(), o.side[sd].j, np * sizeof(double), hipMemcpyDeviceToHost));
-        CK(h, hipMemcpy(q.data(), o.side[sd].q, np * sizeof(double), hipMemcpyDeviceToHost));
+        const size_t np = o.side_cidx[sd].size();
         double acc = 0.0;
-        for (size_t p = 0; p < np; p++)
-          acc += o.c_half_l[ci[p]] * j[p] * j[p] + 0.5 * o.c_inv_c[ci[p]] * q[p] * q[p];
+        for (size_t p = 0; p < np; p++) acc += o.h_et[off + p];
         side_w[sd] = acc;
+        off += np;
       }
       w += side_w[0] + side_w[1];
       continue;
     }
     if (o.type == OP_MERGED) {
-      // MergedFaces::energy (conformal.rs:2684-2698): the next update from
-      // the current g and the look-ahead circulations, evaluated here with
-      // the device's arithmetic, then 1/2 g^T M g'.
-      if (!synced) CK(h, hipStreamSynchronize(h->stream));
-      synced = true;
-      const MergedHost &mh = *o.mfh;
-      const int ng = o.mf.ng;
-      std::vector<double> cur(ng), circ(ng), next(ng, 0.0), x(mh.mem.size() + 1);
-      if (ng) {
-        CK(h, hipMemcpy(cur.data(), o.mf.cur, ng * sizeof(double), hipMemcpyDeviceToHost));
-        CK(h, hipMemcpy(circ.data(), o.mf.circ, ng * sizeof(double), hipMemcpyDeviceToHost));
-      }
-      DevMerged v = o.mf;
-      v.da = mh.da.data(), v.db = mh.db.data();
-      v.bs = mh.bs.data(), v.mem = mh.mem.data(), v.ef = mh.ef.data(), v.eo = mh.eo.data();
-      v.ev = mh.ev.data(), v.brs = mh.brs.data(), v.bc = mh.bc.data(), v.bv = mh.bv.data();
-      for (int q = 0; q < v.ns; q++) mf_scalar(v, q, cur.data(), circ.data(), next.data());
-      for (int b = 0; b < v.nb; b++) {
-        const uint32_t r0 = mh.bs[b], nr = mh.bs[b + 1] - r0;
-        for (uint32_t i = 0; i < nr; i++) x[i] = mf_rhs(v, r0, i, cur.data(), circ.data());
-        mf_solve(v, r0, nr, x.data());
-        for (uint32_t i = 0; i < nr; i++) next[mh.mem[r0 + i]] = (double)flush_hd((float)x[i]);
-      }
+      // MergedFaces::energy (conformal.rs:2684-2698): 1/2 g^T M g' from
+      // 0.0, the merge groups' terms first, then the block rows in order.
       double e = 0.0;
-      for (int q = 0; q < v.ns; q++) e += 0.5 * mh.l[q] * cur[q] * next[q];
-      for (int b = 0; b < v.nb; b++) {
-        const uint32_t r0 = mh.bs[b];
-        for (uint32_t r = r0; r < mh.bs[b + 1]; r++) {
-          double mn = -0.0;
-          for (uint32_t k = mh.mrs[r]; k < mh.mrs[r + 1]; k++)
-            mn += mh.mval[k] * next[mh.mem[r0 + mh.mcol[k]]];
-          e += 0.5 * cur[mh.mem[r]] * mn;
-        }
-      }
+      for (size_t t = 0; t < o.n_et; t++) e += o.h_et[t];
       w += e;
       continue;
     }
@@ -5199,12 +5303,8 @@ static int ops_energy(ems_gpu *h, double *out) {
       w += 0.0;
       continue;
     }
-    if (!synced) CK(h, hipStreamSynchronize(h->stream));
-    synced = true;
     const int ne = o.lp.n;
-    std::vector<double> j(ne), q(ne);
-    CK(h, hipMemcpy(j.data(), o.lp.j, ne * sizeof(double), hipMemcpyDeviceToHost));
-    CK(h, hipMemcpy(q.data(), o.lp.q, ne * sizeof(double), hipMemcpyDeviceToHost));
+    const double *j = o.h_et, *q = o.h_et + ne;
     double s = -0.0;
     for (int e = 0; e < ne; e++) {
       const double half_l = 0.25 * o.two_l_dt[e] * h->dt;
@@ -5212,31 +5312,42 @@ static int ops_energy(ems_gpu *h, double *out) {
     }
     w += s;
   }
-  *out = w;
+  return w;
+}
+
+// The field energy's and the operators' device parts and copies, queued,
+// and the event that marks their landing.
+static int energy_all_enqueue(ems_gpu *h) {
+  int rc = energy_enqueue(h);
+  if (rc) return rc;
+  if ((rc = ops_energy_enqueue(h))) return rc;
+  if (!h->ev_energy) CK(h, hipEventCreateWithFlags(&h->ev_energy, hipEventDisableTiming));
+  CK(h, hipEventRecord(h->ev_energy, h->stream));
   return 0;
 }
 
+// The step is queued before the wait, so the device runs it while the host
+// adds the operators' terms.
 int ems_gpu_step_with_energy(ems_gpu *h, double *w) {
   ALIVE(h);
   if (!w) return EMS_E_INVALID;
-  double w_ops = 0;
-  int rc = ops_energy(h, &w_ops);
+  int rc = energy_all_enqueue(h);
   if (rc) return rc;
-  double wf = 0;
-  if ((rc = energy_now(h, &wf))) return rc;
   if ((rc = run_steps(h, 1))) return rc;
-  *w = wf + w_ops;
+  CK(h, hipEventSynchronize(h->ev_energy));
+  const double w_ops = ops_energy_finish(h);
+  *w = *h->h_w + w_ops;
   return EMS_OK;
 }
 
 int ems_gpu_energy(ems_gpu *h, double *w) {
   ALIVE(h);
   if (!w) return EMS_E_INVALID;
-  double wf = 0, w_ops = 0;
-  int rc = energy_now(h, &wf);
+  int rc = energy_all_enqueue(h);
   if (rc) return rc;
-  if ((rc = ops_energy(h, &w_ops))) return rc;
-  *w = wf + w_ops;
+  CK(h, hipEventSynchronize(h->ev_energy));
+  const double w_ops = ops_energy_finish(h);
+  *w = *h->h_w + w_ops;
   return EMS_OK;
 }
 
@@ -5335,17 +5446,14 @@ int ems_gpu_energy_many(ems_gpu *const *models, int n, const uint8_t *active, do
     if (active && !active[q]) continue;
     ems_gpu *h = models[q];
     ALIVE(h);
-    int rc = energy_enqueue(h);
+    int rc = energy_all_enqueue(h);
     if (rc) return rc;
   }
   for (int q = 0; q < n; q++) {
     if (active && !active[q]) continue;
     ems_gpu *h = models[q];
-    CK(h, hipStreamSynchronize(h->stream));
-    double w_ops = 0;
-    int rc = ops_energy(h, &w_ops);
-    if (rc) return rc;
-    out[q] = *h->h_w + w_ops;
+    CK(h, hipEventSynchronize(h->ev_energy));
+    out[q] = *h->h_w + ops_energy_finish(h);
   }
   return EMS_OK;
 }
@@ -5406,7 +5514,7 @@ int ems_gpu_energy_noalloc(ems_gpu *h, double *w) {
   CK(h, hipMemcpyAsync(&wf, d_w, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   CK(h, hipStreamSynchronize(h->stream));
   // Engine::operator_energy of the remaining kinds (every one 0, folded
-  // from -0.0 as ops_energy does).
+  // from -0.0 as ops_energy_finish does).
   double w_ops = -0.0;
   if (!h->slabs.empty()) w_ops += 0.0;
   for (size_t q = 0; q < h->ops.size(); q++) w_ops += 0.0;
