@@ -232,126 +232,6 @@ __global__ void __launch_bounds__(256) e_update(Fields f, Lanes L, int nx, int n
   }
 }
 
-// ---- unfused H and E updates, software pipelined along x ----
-//
-// The same updates (same expressions, same values written) with a thread
-// marching a chunk of x-planes and the next plane's inputs loaded before
-// the present plane is computed, so the loads of one plane overlap the
-// arithmetic and stores of the one before. Every load address is valid
-// (neighbours past the grid's end fall back to the node itself; those
-// values are not used).
-
-// What the H update of one node reads (kernel.rs:443-511).
-struct HNode {
-  float uxc, uyn, uzn, ix, iy, iz, uz_j1, uy_k1, ux_k1, ux_j1;
-  float da[3], db[3];
-};
-
-__device__ __forceinline__ void h_node_load(HNode &q, const Fields &f, const Lanes &L, size_t o,
-                                            size_t r, int k, int nz, size_t plane, bool iv,
-                                            bool jv, bool kv) {
-  const GAS float *ux = (const GAS float *)f.u[0], *uy = (const GAS float *)f.u[1];
-  const GAS float *uz = (const GAS float *)f.u[2];
-  const size_t oi = iv ? o + plane : o, oj = jv ? o + nz : o, ok = kv ? o + 1 : o;
-  q.uxc = ux[o], q.uyn = uy[oi], q.uzn = uz[oi];
-  q.ix = ((const GAS float *)f.i[0])[o], q.iy = ((const GAS float *)f.i[1])[o];
-  q.iz = ((const GAS float *)f.i[2])[o];
-  q.uz_j1 = uz[oj], q.ux_j1 = ux[oj], q.uy_k1 = uy[ok], q.ux_k1 = ux[ok];
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    q.da[a] = coefg(L.l[6 + a], r, k, nz, o);
-    q.db[a] = coefg(L.l[9 + a], r, k, nz, o);
-  }
-}
-
-template <bool OE>
-__global__ void __launch_bounds__(256) h_update_pf(Fields f, Lanes L, int nx, int ny, int nz,
-                                                   int chunk) {
-  const int plane = ny * nz, gx = (plane + 255) / 256;
-  const int b = xcd_contiguous(blockIdx.x, gridDim.x);
-  const int p = (b % gx) * blockDim.x + threadIdx.x;
-  if (p >= plane) return;
-  const int j = p / nz, k = p - j * nz;
-  const int i0 = (b / gx) * chunk;
-  const int i1 = min(nx, i0 + chunk);
-  const bool jv = j < ny - 1, kv = k < nz - 1;
-  GAS float *ix = (GAS float *)f.i[0], *iy = (GAS float *)f.i[1], *iz = (GAS float *)f.i[2];
-  size_t o = (size_t)i0 * plane + p;
-  float uyc = ((const GAS float *)f.u[1])[o], uzc = ((const GAS float *)f.u[2])[o];
-  HNode cur, nxt;
-  h_node_load(cur, f, L, o, (size_t)i0 * ny + j, k, nz, plane, i0 + 1 < nx, jv, kv);
-  for (int i = i0; i < i1; i++, o += plane) {
-    const bool iv = i + 1 < nx;
-    if (i + 1 < i1)
-      h_node_load(nxt, f, L, o + plane, (size_t)(i + 1) * ny + j, k, nz, plane, i + 2 < nx, jv,
-                  kv);
-    const float uyn = iv ? cur.uyn : 0.0f, uzn = iv ? cur.uzn : 0.0f;
-    if (jv && kv)  // curl_x = (uz[j+1] - uz[j]) - (uy[k+1] - uy[k])
-      ix[o] = h_expr<OE>(cur.da[0], cur.ix, cur.db[0], cur.uz_j1, uzc, cur.uy_k1, uyc);
-    if (iv && kv)  // curl_y = (ux[k+1] - ux[k]) - (uz[i+1] - uz[i])
-      iy[o] = h_expr<OE>(cur.da[1], cur.iy, cur.db[1], cur.ux_k1, cur.uxc, uzn, uzc);
-    if (iv && jv)  // curl_z = (uy[i+1] - uy[i]) - (ux[j+1] - ux[j])
-      iz[o] = h_expr<OE>(cur.da[2], cur.iz, cur.db[2], uyn, uyc, cur.ux_j1, cur.uxc);
-    uyc = uyn;
-    uzc = uzn;
-    cur = nxt;
-  }
-}
-
-// What the E update of one node reads (kernel.rs:620-697).
-struct ENode {
-  float ix, iy, iz, iz_jm, iy_km, ix_km, ix_jm, ux, uy, uz;
-  float ca[3], cb[3];
-};
-
-__device__ __forceinline__ void e_node_load(ENode &q, const Fields &f, const Lanes &L, size_t o,
-                                            size_t r, int j, int k, int nz) {
-  const GAS float *ix = (const GAS float *)f.i[0], *iy = (const GAS float *)f.i[1];
-  const GAS float *iz = (const GAS float *)f.i[2];
-  const size_t oj = j > 0 ? o - nz : o, ok = k > 0 ? o - 1 : o;
-  q.ix = ix[o], q.iy = iy[o], q.iz = iz[o];
-  q.iz_jm = iz[oj], q.ix_jm = ix[oj], q.iy_km = iy[ok], q.ix_km = ix[ok];
-  q.ux = ((const GAS float *)f.u[0])[o], q.uy = ((const GAS float *)f.u[1])[o];
-  q.uz = ((const GAS float *)f.u[2])[o];
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    q.ca[a] = coefg(L.l[a], r, k, nz, o);
-    q.cb[a] = coefg(L.l[3 + a], r, k, nz, o);
-  }
-}
-
-template <bool OE>
-__global__ void __launch_bounds__(256) e_update_pf(Fields f, Lanes L, int nx, int ny, int nz,
-                                                   int chunk) {
-  const int plane = ny * nz, gx = (plane + 255) / 256;
-  const int b = xcd_contiguous(blockIdx.x, gridDim.x);
-  const int p = (b % gx) * blockDim.x + threadIdx.x;
-  if (p >= plane) return;
-  const int j = p / nz, k = p - j * nz;
-  const int i0 = (b / gx) * chunk;
-  const int i1 = min(nx, i0 + chunk);
-  GAS float *ux = (GAS float *)f.u[0], *uy = (GAS float *)f.u[1], *uz = (GAS float *)f.u[2];
-  size_t o = (size_t)i0 * plane + p;
-  float iyp = i0 > 0 ? ((const GAS float *)f.i[1])[o - plane] : 0.0f;
-  float izp = i0 > 0 ? ((const GAS float *)f.i[2])[o - plane] : 0.0f;
-  ENode cur, nxt;
-  e_node_load(cur, f, L, o, (size_t)i0 * ny + j, j, k, nz);
-  for (int i = i0; i < i1; i++, o += plane) {
-    if (i + 1 < i1) e_node_load(nxt, f, L, o + plane, (size_t)(i + 1) * ny + j, j, k, nz);
-    const float hz_jm = j > 0 ? cur.iz_jm : 0.0f, hx_jm = j > 0 ? cur.ix_jm : 0.0f;
-    const float hy_km = k > 0 ? cur.iy_km : 0.0f, hx_km = k > 0 ? cur.ix_km : 0.0f;
-    if (i + 1 < nx)  // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
-      ux[o] = e_expr<OE>(cur.ca[0], cur.ux, cur.cb[0], cur.iz, hz_jm, cur.iy, hy_km);
-    if (j + 1 < ny)  // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
-      uy[o] = e_expr<OE>(cur.ca[1], cur.uy, cur.cb[1], cur.ix, hx_km, cur.iz, izp);
-    if (k + 1 < nz)  // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
-      uz[o] = e_expr<OE>(cur.ca[2], cur.uz, cur.cb[2], cur.iy, iyp, cur.ix, hx_jm);
-    iyp = cur.iy;
-    izp = cur.iz;
-    cur = nxt;
-  }
-}
-
 // CPML slab correction (cpml.rs:469-543, 693-736).
 struct SlabArg {
   int e_side, comp, other, normal;
@@ -624,13 +504,19 @@ __device__ __forceinline__ size_t cpml_index(const FSlab &s, int i, int j, int k
 // memory round trip per slab. More slabs take the plain loop.
 constexpr int CPML_SLOTS = 6;
 
-template <bool ESIDE>
+//
+// INPLACE (the unfused kernels, which compute each node once): the psi of
+// parity `par` is read and written (the fused kernel's halo recompute reads
+// one buffer and writes the other). SLOTS false: the plain loop only (the
+// unfused kernels' columns mostly lie outside every slab, and the slot
+// loads would be wasted).
+template <bool ESIDE, bool SLOTS = true, bool INPLACE = false>
 __device__ __forceinline__ void cpml_node(const FSlab *__restrict__ slabs, uint32_t mask, int par,
                                           bool write_psi, int i, int j, int k, float d01,
                                           float d02, float d10, float d12, float d20, float d21,
                                           float cf0, float cf1, float cf2, float &t0, float &t1,
                                           float &t2) {
-  if (__builtin_popcount(mask) <= CPML_SLOTS) {
+  if (SLOTS && __builtin_popcount(mask) <= CPML_SLOTS) {
     float pv[CPML_SLOTS], bv[CPML_SLOTS], av[CPML_SLOTS], kv[CPML_SLOTS];
     uint32_t m = mask;
 #pragma unroll
@@ -660,7 +546,7 @@ __device__ __forceinline__ void cpml_node(const FSlab *__restrict__ slabs, uint3
           float ps;
           cpml_apply<ESIDE>(s, pv[q], bv[q], av[q], kv[q], d01, d02, d10, d12, d20, d21, cf0, cf1,
                             cf2, t0, t1, t2, ps);
-          GAS float *pout = (GAS float *)((ESIDE || par == 1) ? s.psi0 : s.psi1);
+          GAS float *pout = (GAS float *)((ESIDE || par == (INPLACE ? 0 : 1)) ? s.psi0 : s.psi1);
           if (write_psi) pout[cpml_index(s, i, j, k)] = ps;
         }
       }
@@ -676,12 +562,193 @@ __device__ __forceinline__ void cpml_node(const FSlab *__restrict__ slabs, uint3
     const int z = cpml_depth(s, i, j, k);
     const size_t qq = cpml_index(s, i, j, k);
     const GAS float *pin = (const GAS float *)((ESIDE || par == 0) ? s.psi0 : s.psi1);
-    GAS float *pout = (GAS float *)((ESIDE || par == 1) ? s.psi0 : s.psi1);
+    GAS float *pout = (GAS float *)((ESIDE || par == (INPLACE ? 0 : 1)) ? s.psi0 : s.psi1);
     float ps;
     cpml_apply<ESIDE>(s, pin[qq], ((const GAS float *)s.b)[z], ((const GAS float *)s.a)[z],
                       ((const GAS float *)s.k)[z], d01, d02, d10, d12, d20, d21, cf0, cf1, cf2, t0,
                       t1, t2, ps);
     if (write_psi) pout[qq] = ps;
+  }
+}
+
+// ---- unfused H and E updates, software pipelined along x ----
+//
+// The same updates (same expressions, same values written) with a thread
+// marching a chunk of x-planes and the next plane's inputs loaded before
+// the present plane is computed, so the loads of one plane overlap the
+// arithmetic and stores of the one before. Every load address is valid
+// (neighbours past the grid's end fall back to the node itself; those
+// values are not used).
+//
+// CP: the side's CPML slabs (the fused kernel's slab table, every slab of
+// the side in it) corrected here, after the update, in registration order
+// with the very expressions of cpml_slab, instead of in launches of their
+// own; a plane that a slab's x range holds stores all three components.
+
+// The slabs (bits of the first `nslab` table entries) whose (j, k) box
+// meets the block's nodes p0..p1 of a plane: rows p0/nz..p1/nz and, over
+// one row, its columns (else every column).
+__device__ __forceinline__ uint32_t block_slabs(const FSlab *__restrict__ slabs, int nslab,
+                                                int p0, int p1, int nz) {
+  const int j0 = p0 / nz, j1 = p1 / nz;
+  const int k0 = j0 == j1 ? p0 - j0 * nz : 0, k1 = j0 == j1 ? p1 - j1 * nz : nz - 1;
+  uint32_t m = 0;
+  for (int q = 0; q < nslab; q++) {
+    const FSlab s = cload(slabs, q);
+    if (s.lo1 <= j1 && j0 < s.hi1 && s.lo2 <= k1 && k0 < s.hi2) m |= 1u << q;
+  }
+  return m;
+}
+
+// The fold's tables: the slab table, per x-plane the slabs (H, E) whose x
+// range holds it, the table's length and the H psi parity.
+struct CpmlIn {
+  const FSlab *slabs;
+  const uint32_t *pmask;
+  int nslab, par;
+};
+
+// What the H update of one node reads (kernel.rs:443-511).
+struct HNode {
+  float uxc, uyn, uzn, ix, iy, iz, uz_j1, uy_k1, ux_k1, ux_j1;
+  float da[3], db[3];
+};
+
+__device__ __forceinline__ void h_node_load(HNode &q, const Fields &f, const Lanes &L, size_t o,
+                                            size_t r, int k, int nz, size_t plane, bool iv,
+                                            bool jv, bool kv) {
+  const GAS float *ux = (const GAS float *)f.u[0], *uy = (const GAS float *)f.u[1];
+  const GAS float *uz = (const GAS float *)f.u[2];
+  const size_t oi = iv ? o + plane : o, oj = jv ? o + nz : o, ok = kv ? o + 1 : o;
+  q.uxc = ux[o], q.uyn = uy[oi], q.uzn = uz[oi];
+  q.ix = ((const GAS float *)f.i[0])[o], q.iy = ((const GAS float *)f.i[1])[o];
+  q.iz = ((const GAS float *)f.i[2])[o];
+  q.uz_j1 = uz[oj], q.ux_j1 = ux[oj], q.uy_k1 = uy[ok], q.ux_k1 = ux[ok];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    q.da[a] = coefg(L.l[6 + a], r, k, nz, o);
+    q.db[a] = coefg(L.l[9 + a], r, k, nz, o);
+  }
+}
+
+template <bool OE, bool CP>
+__global__ void __launch_bounds__(256) h_update_pf(Fields f, Lanes L, int nx, int ny, int nz,
+                                                   int chunk, CpmlIn cp) {
+  const int plane = ny * nz, gx = (plane + 255) / 256;
+  const int b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int p = (b % gx) * blockDim.x + threadIdx.x;
+  const uint32_t mblk =
+      CP ? block_slabs(cp.slabs, cp.nslab, (b % gx) * 256, min((b % gx) * 256 + 255, plane - 1), nz)
+         : 0u;
+  if (p >= plane) return;
+  const int j = p / nz, k = p - j * nz;
+  const int i0 = (b / gx) * chunk;
+  const int i1 = min(nx, i0 + chunk);
+  const bool jv = j < ny - 1, kv = k < nz - 1;
+  GAS float *ix = (GAS float *)f.i[0], *iy = (GAS float *)f.i[1], *iz = (GAS float *)f.i[2];
+  size_t o = (size_t)i0 * plane + p;
+  float uyc = ((const GAS float *)f.u[1])[o], uzc = ((const GAS float *)f.u[2])[o];
+  HNode cur, nxt;
+  h_node_load(cur, f, L, o, (size_t)i0 * ny + j, k, nz, plane, i0 + 1 < nx, jv, kv);
+  for (int i = i0; i < i1; i++, o += plane) {
+    const bool iv = i + 1 < nx;
+    if (i + 1 < i1)
+      h_node_load(nxt, f, L, o + plane, (size_t)(i + 1) * ny + j, k, nz, plane, i + 2 < nx, jv,
+                  kv);
+    const float uyn = iv ? cur.uyn : 0.0f, uzn = iv ? cur.uzn : 0.0f;
+    const uint32_t m = CP ? mblk & cload(cp.pmask, 2 * (size_t)i) : 0u;
+    if (CP && m) {
+      float h0 = cur.ix, h1 = cur.iy, h2 = cur.iz;
+      if (jv && kv) h0 = h_expr<OE>(cur.da[0], h0, cur.db[0], cur.uz_j1, uzc, cur.uy_k1, uyc);
+      if (iv && kv) h1 = h_expr<OE>(cur.da[1], h1, cur.db[1], cur.ux_k1, cur.uxc, uzn, uzc);
+      if (iv && jv) h2 = h_expr<OE>(cur.da[2], h2, cur.db[2], uyn, uyc, cur.ux_j1, cur.uxc);
+      // T_p = u_other[node + e_p] - u_other[node] per (component, normal)
+      cpml_node<false, false, true>(cp.slabs, m, cp.par, true, i, j, k, cur.uz_j1 - uzc,
+                                    cur.uy_k1 - uyc, uzn - uzc, cur.ux_k1 - cur.uxc, uyn - uyc,
+                                    cur.ux_j1 - cur.uxc, cur.db[0], cur.db[1], cur.db[2], h0, h1,
+                                    h2);
+      ix[o] = h0, iy[o] = h1, iz[o] = h2;
+    } else {
+      if (jv && kv)  // curl_x = (uz[j+1] - uz[j]) - (uy[k+1] - uy[k])
+        ix[o] = h_expr<OE>(cur.da[0], cur.ix, cur.db[0], cur.uz_j1, uzc, cur.uy_k1, uyc);
+      if (iv && kv)  // curl_y = (ux[k+1] - ux[k]) - (uz[i+1] - uz[i])
+        iy[o] = h_expr<OE>(cur.da[1], cur.iy, cur.db[1], cur.ux_k1, cur.uxc, uzn, uzc);
+      if (iv && jv)  // curl_z = (uy[i+1] - uy[i]) - (ux[j+1] - ux[j])
+        iz[o] = h_expr<OE>(cur.da[2], cur.iz, cur.db[2], uyn, uyc, cur.ux_j1, cur.uxc);
+    }
+    uyc = uyn;
+    uzc = uzn;
+    cur = nxt;
+  }
+}
+
+// What the E update of one node reads (kernel.rs:620-697).
+struct ENode {
+  float ix, iy, iz, iz_jm, iy_km, ix_km, ix_jm, ux, uy, uz;
+  float ca[3], cb[3];
+};
+
+__device__ __forceinline__ void e_node_load(ENode &q, const Fields &f, const Lanes &L, size_t o,
+                                            size_t r, int j, int k, int nz) {
+  const GAS float *ix = (const GAS float *)f.i[0], *iy = (const GAS float *)f.i[1];
+  const GAS float *iz = (const GAS float *)f.i[2];
+  const size_t oj = j > 0 ? o - nz : o, ok = k > 0 ? o - 1 : o;
+  q.ix = ix[o], q.iy = iy[o], q.iz = iz[o];
+  q.iz_jm = iz[oj], q.ix_jm = ix[oj], q.iy_km = iy[ok], q.ix_km = ix[ok];
+  q.ux = ((const GAS float *)f.u[0])[o], q.uy = ((const GAS float *)f.u[1])[o];
+  q.uz = ((const GAS float *)f.u[2])[o];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    q.ca[a] = coefg(L.l[a], r, k, nz, o);
+    q.cb[a] = coefg(L.l[3 + a], r, k, nz, o);
+  }
+}
+
+template <bool OE, bool CP>
+__global__ void __launch_bounds__(256) e_update_pf(Fields f, Lanes L, int nx, int ny, int nz,
+                                                   int chunk, CpmlIn cp) {
+  const int plane = ny * nz, gx = (plane + 255) / 256;
+  const int b = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int p = (b % gx) * blockDim.x + threadIdx.x;
+  const uint32_t mblk =
+      CP ? block_slabs(cp.slabs, cp.nslab, (b % gx) * 256, min((b % gx) * 256 + 255, plane - 1), nz)
+         : 0u;
+  if (p >= plane) return;
+  const int j = p / nz, k = p - j * nz;
+  const int i0 = (b / gx) * chunk;
+  const int i1 = min(nx, i0 + chunk);
+  GAS float *ux = (GAS float *)f.u[0], *uy = (GAS float *)f.u[1], *uz = (GAS float *)f.u[2];
+  size_t o = (size_t)i0 * plane + p;
+  float iyp = i0 > 0 ? ((const GAS float *)f.i[1])[o - plane] : 0.0f;
+  float izp = i0 > 0 ? ((const GAS float *)f.i[2])[o - plane] : 0.0f;
+  ENode cur, nxt;
+  e_node_load(cur, f, L, o, (size_t)i0 * ny + j, j, k, nz);
+  for (int i = i0; i < i1; i++, o += plane) {
+    if (i + 1 < i1) e_node_load(nxt, f, L, o + plane, (size_t)(i + 1) * ny + j, j, k, nz);
+    const float hz_jm = j > 0 ? cur.iz_jm : 0.0f, hx_jm = j > 0 ? cur.ix_jm : 0.0f;
+    const float hy_km = k > 0 ? cur.iy_km : 0.0f, hx_km = k > 0 ? cur.ix_km : 0.0f;
+    const uint32_t m = CP ? mblk & cload(cp.pmask, 2 * (size_t)i + 1) : 0u;
+    if (CP && m) {
+      float e0 = cur.ux, e1 = cur.uy, e2 = cur.uz;
+      if (i + 1 < nx) e0 = e_expr<OE>(cur.ca[0], e0, cur.cb[0], cur.iz, hz_jm, cur.iy, hy_km);
+      if (j + 1 < ny) e1 = e_expr<OE>(cur.ca[1], e1, cur.cb[1], cur.ix, hx_km, cur.iz, izp);
+      if (k + 1 < nz) e2 = e_expr<OE>(cur.ca[2], e2, cur.cb[2], cur.iy, iyp, cur.ix, hx_jm);
+      // S_p = i_other[node] - i_other[node - e_p] (cpml.rs:495-521)
+      cpml_node<true, false, true>(cp.slabs, m, 0, true, i, j, k, cur.iz - hz_jm, cur.iy - hy_km,
+                                   cur.iz - izp, cur.ix - hx_km, cur.iy - iyp, cur.ix - hx_jm,
+                                   cur.cb[0], cur.cb[1], cur.cb[2], e0, e1, e2);
+      ux[o] = e0, uy[o] = e1, uz[o] = e2;
+    } else {
+      if (i + 1 < nx)  // curl_x = (iz[j] - iz[j-1]) - (iy[k] - iy[k-1])
+        ux[o] = e_expr<OE>(cur.ca[0], cur.ux, cur.cb[0], cur.iz, hz_jm, cur.iy, hy_km);
+      if (j + 1 < ny)  // curl_y = (ix[k] - ix[k-1]) - (iz[i] - iz[i-1])
+        uy[o] = e_expr<OE>(cur.ca[1], cur.uy, cur.cb[1], cur.ix, hx_km, cur.iz, izp);
+      if (k + 1 < nz)  // curl_z = (iy[i] - iy[i-1]) - (ix[j] - ix[j-1])
+        uz[o] = e_expr<OE>(cur.ca[2], cur.uz, cur.cb[2], cur.iy, iyp, cur.ix, hx_jm);
+    }
+    iyp = cur.iy;
+    izp = cur.iz;
+    cur = nxt;
   }
 }
 
@@ -2905,7 +2972,7 @@ struct ems_gpu {
   uint64_t *d_step = nullptr;
   // fused schedule and its tiling
   bool fused = true, prefetch = true, rows = false, packed = false, band = false;
-  int unfused_pf = 1;  // ems_gpu_set_unfused_prefetch: bit 0 H, bit 1 E
+  int unfused_pf = 11;  // ems_gpu_set_unfused_prefetch: bits 0, 1 load ahead; 2, 3 CPML in
   int xcd_group = 2;  // tile rows per XCD visit (2 measured best of 0, 1, 2, 4, 8)
   int tj = 7, nchunk = 0;
   // tile classes of the tiled step (ems_gpu_set_tile_classes): the tiles a
@@ -3047,6 +3114,43 @@ int launch_poles(ems_gpu *h, int stage, const Fields &f, const Fields &fp, bool 
   return 0;
 }
 
+// Whether side `side` (0 H, 1 E) corrects its CPML slabs inside its
+// unfused update: bit 2 + side of the unfused switch with the side's
+// loading-ahead kernel (bit side), and the slab table built for it (every slab but the z-normal
+// suffix, whose columns would run the slab code in a few lanes of most
+// waves; those keep their launches, after the kernel, so the corrections
+// of a node keep the registration order; at most 32 for the planes'
+// masks).
+bool cpml_folded(const ems_gpu *h, int side) {
+  if (use_fused(h) || !(h->unfused_pf & (4 << side)) || !(h->unfused_pf & (1 << side)) ||
+      !h->d_fslabs || !h->d_pmask || h->split_state != 2)
+    return false;
+  int n = 0;
+  for (auto &s : h->slabs) n += s.len != 0 && !s.out;
+  return n > 0 && n <= 32;
+}
+
+int n_fslabs(const ems_gpu *h);
+
+#define UNFUSED_LAUNCH(K, grid_, cp_on)                                                   \
+  do {                                                                                    \
+    if (h->oe) {                                                                          \
+      if (cp_on)                                                                          \
+        hipLaunchKernelGGL((K<true, true>), grid_, dim3(256), 0, h->stream, f, lanes_arg(h), \
+                           h->n[0], h->n[1], h->n[2], chunk, cp);                         \
+      else                                                                                \
+        hipLaunchKernelGGL((K<true, false>), grid_, dim3(256), 0, h->stream, f, lanes_arg(h), \
+                           h->n[0], h->n[1], h->n[2], chunk, cp);                         \
+    } else {                                                                              \
+      if (cp_on)                                                                          \
+        hipLaunchKernelGGL((K<false, true>), grid_, dim3(256), 0, h->stream, f, lanes_arg(h), \
+                           h->n[0], h->n[1], h->n[2], chunk, cp);                         \
+      else                                                                                \
+        hipLaunchKernelGGL((K<false, false>), grid_, dim3(256), 0, h->stream, f,           \
+                           lanes_arg(h), h->n[0], h->n[1], h->n[2], chunk, cp);           \
+    }                                                                                     \
+  } while (0)
+
 int launch_h(ems_gpu *h, const Fields &f) {
   int nch, chunk = x_chunk(h, &nch, (h->unfused_pf & 1) != 0);
   dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
@@ -3054,12 +3158,9 @@ int launch_h(ems_gpu *h, const Fields &f) {
   // two-plane kernel measured slower so: E 0.526 against 0.503 ms)
   const dim3 line(grid.x * grid.y);
   if (h->unfused_pf & 1) {
-    if (h->oe)
-      hipLaunchKernelGGL(h_update_pf<true>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
-                         h->n[0], h->n[1], h->n[2], chunk);
-    else
-      hipLaunchKernelGGL(h_update_pf<false>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
-                         h->n[0], h->n[1], h->n[2], chunk);
+    const bool cpf = cpml_folded(h, 0);
+    const CpmlIn cp{h->d_fslabs, h->d_pmask, cpf ? n_fslabs(h) : 0, h->par};
+    UNFUSED_LAUNCH(h_update_pf, line, cpf);
   } else if (h->oe)
     hipLaunchKernelGGL(h_update<true>, grid, dim3(256), 0, h->stream, f, lanes_arg(h), h->n[0],
                        h->n[1], h->n[2], chunk);
@@ -3073,16 +3174,11 @@ int launch_h(ems_gpu *h, const Fields &f) {
 int launch_e(ems_gpu *h, const Fields &f) {
   int nch, chunk = x_chunk(h, &nch, (h->unfused_pf & 2) != 0);
   dim3 grid((unsigned)((h->plane + 255) / 256), (unsigned)nch);
-  // the loading-ahead kernel runs one-dimensional (see xcd_contiguous; the
-  // two-plane kernel measured slower so: E 0.526 against 0.503 ms)
   const dim3 line(grid.x * grid.y);
   if (h->unfused_pf & 2) {
-    if (h->oe)
-      hipLaunchKernelGGL(e_update_pf<true>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
-                         h->n[0], h->n[1], h->n[2], chunk);
-    else
-      hipLaunchKernelGGL(e_update_pf<false>, line, dim3(256), 0, h->stream, f, lanes_arg(h),
-                         h->n[0], h->n[1], h->n[2], chunk);
+    const bool cpf = cpml_folded(h, 1);
+    const CpmlIn cp{h->d_fslabs, h->d_pmask, cpf ? n_fslabs(h) : 0, 0};
+    UNFUSED_LAUNCH(e_update_pf, line, cpf);
   } else if (h->oe)
     hipLaunchKernelGGL(e_update<true>, grid, dim3(256), 0, h->stream, f, lanes_arg(h), h->n[0],
                        h->n[1], h->n[2], chunk);
@@ -3092,10 +3188,12 @@ int launch_e(ems_gpu *h, const Fields &f) {
   CK(h, hipGetLastError());
   return 0;
 }
+#undef UNFUSED_LAUNCH
 
 int launch_cpml(ems_gpu *h, const Fields &f, int e_side, int par) {
+  const bool folded = cpml_folded(h, e_side);  // then only the suffix runs here
   for (auto &s : h->slabs) {
-    if (s.arg.e_side != e_side || s.len == 0) continue;
+    if (s.arg.e_side != e_side || s.len == 0 || (folded && !s.out)) continue;
     const Lane cf = h->lanes[e_side ? 3 + s.arg.comp : 9 + s.arg.comp];
     SlabArg a = s.arg;
     a.psi = e_side ? s.d_psi[0] : s.d_psi[par];
@@ -3210,12 +3308,24 @@ int upload_out_slabs(ems_gpu *h) {
   return 0;
 }
 
-int upload_fslabs(ems_gpu *h, bool split) {
-  if (!h->slabs_dirty && h->split_state == (split ? 1 : 0)) return 0;
-  mark_out_slabs(h, split);
-  int rc = upload_out_slabs(h);
+// state: 0 every slab in the table; 1 the fused kernel's split (the
+// z-normal suffix corrected by its passes outside it); 2 the same suffix
+// left to the unfused schedule's own slab launches (no outside tables).
+int upload_fslabs(ems_gpu *h, int state) {
+  if (!h->slabs_dirty && h->split_state == state) return 0;
+  mark_out_slabs(h, state != 0);
+  int rc = state == 2 ? 0 : upload_out_slabs(h);
   if (rc) return rc;
-  h->split_state = split ? 1 : 0;
+  if (state == 2) {
+    if (h->d_hout) CK(h, hipFree(h->d_hout));
+    if (h->d_eout) CK(h, hipFree(h->d_eout));
+    if (h->d_pmask_o) CK(h, hipFree(h->d_pmask_o));
+    h->d_hout = h->d_eout = nullptr;
+    h->d_pmask_o = nullptr;
+    h->n_hout = h->n_eout = 0;
+    h->hout_total = h->eout_total = 0;
+  }
+  h->split_state = state;
   h->slab_gen++;
   if (h->d_fslabs) CK(h, hipFree(h->d_fslabs));
   h->d_fslabs = nullptr;
@@ -3876,8 +3986,12 @@ int prepare(ems_gpu *h, uint64_t steps, bool allow_split = true) {
     }
   }
   if (grown) h->ops_dirty = true;
-  const bool split = allow_split && h->cpml_split && tiled_fused(h);
-  if (h->slabs_dirty || h->split_state != (split ? 1 : 0)) {
+  // the fused kernel's split, or the unfused kernels' CPML fold (bits 2, 3
+  // of the unfused switch), which takes the slabs but the z-normal suffix
+  const int split = (allow_split && h->cpml_split && tiled_fused(h))
+                        ? 1
+                        : ((!use_fused(h) && ((h->unfused_pf >> 2) & h->unfused_pf & 3)) ? 2 : 0);
+  if (h->slabs_dirty || h->split_state != split) {
     drop_graphs(h);
     int rc = upload_fslabs(h, split);
     if (rc) return rc;
@@ -5150,7 +5264,7 @@ int ems_gpu_set_cpml_split(ems_gpu *h, int on) {
 
 int ems_gpu_set_unfused_prefetch(ems_gpu *h, int which) {
   ALIVE(h);
-  if (which < 0 || which > 3) return fail(h, EMS_E_INVALID, "unfused prefetch: 0 to 3");
+  if (which < 0 || which > 15) return fail(h, EMS_E_INVALID, "unfused kernels: 0 to 15");
   h->unfused_pf = which;
   drop_graphs(h);
   return EMS_OK;
@@ -5657,9 +5771,16 @@ int ems_gpu_traffic_model(ems_gpu *h, double out[3]) {
     out[1] = (36.0 + dense_e) * (double)h->N;
   }
   // CPML slab launches of an unfused step: target r+w, psi r+w, two source
-  // values and the coefficient, 4 B each.
+  // values and the coefficient, 4 B each; slabs corrected inside a side's
+  // update add their psi (read + write) to it instead.
   double cp = 0;
-  for (auto &s : h->slabs) cp += 28.0 * (double)s.len;
+  const bool fold[2] = {cpml_folded(h, 0), cpml_folded(h, 1)};
+  for (auto &s : h->slabs) {
+    if (!use_fused(h) && fold[s.arg.e_side ? 1 : 0] && !s.out)
+      out[s.arg.e_side ? 1 : 0] += 8.0 * (double)s.len;
+    else
+      cp += 28.0 * (double)s.len;
+  }
   out[2] = use_fused(h) ? 0.0 : cp;
   return EMS_OK;
 }

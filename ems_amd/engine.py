@@ -746,7 +746,7 @@ class Engine:
         self._dense_lanes = lanes
         self._owns_lanes = False
         self._pool = pool
-        self.unfused_prefetch = 1
+        self.unfused_prefetch = 11
         self.setup: List[tuple] = []
         self.ops: list = []
         self._tables: list = []
@@ -1108,9 +1108,11 @@ class Engine:
         the same values either way."""
         self._check(nat.ems_gpu_set_tile_classes(self._h, 1 if on else 0))
 
-    def set_unfused_prefetch(self, which: int = 1) -> None:
-        """Which unfused kernels (bit 0 H, bit 1 E; default 1) march longer x
-        chunks with the next plane's inputs loaded ahead; the same values
+    def set_unfused_prefetch(self, which: int = 11) -> None:
+        """The unfused kernels: bits 0 (H) and 1 (E) march longer x chunks
+        with the next plane's inputs loaded ahead; bits 2 (H) and 3 (E),
+        with the side's loading-ahead kernel, correct the side's CPML slabs
+        (but a z-normal suffix) inside it. Default 11; the same values
         either way."""
         self._check(nat.ems_gpu_set_unfused_prefetch(self._h, int(which)))
         self.unfused_prefetch = int(which)

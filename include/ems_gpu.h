@@ -467,13 +467,15 @@ int ems_gpu_set_cpml_split(ems_gpu *h, int on);
  * others in a second one, with a kernel instantiation without the slab
  * code (fewer registers, more waves per SIMD). Same results either way. */
 int ems_gpu_set_tile_classes(ems_gpu *h, int on);
-/* The unfused H and E kernels (no reference counterpart): `which` bit 0
- * (H) and bit 1 (E) pick the kernels whose threads march a chunk of
- * x-planes (8 by default; the schedule's x chunks where set) with the next
- * plane's inputs loaded before the present plane is computed; the others
- * march two planes without that. Default 1 (H only: the loads ahead cost
- * the E kernel occupancy for no gain). Same results either way; 0 to 3,
- * else EMS_E_INVALID. */
+/* The unfused H and E kernels (no reference counterpart). `which` bit 0
+ * (H) and bit 1 (E): the side's threads march a chunk of x-planes (8 by
+ * default; the schedule's x chunks where set) with the next plane's inputs
+ * loaded before the present plane is computed, else two planes without
+ * that. Bit 2 (H) and bit 3 (E), with the side's bit 0 or 1: the side's
+ * CPML slabs, but a suffix of z-normal ones, are corrected inside the
+ * update instead of in launches of their own. Default 11 (H and E load
+ * ahead, E corrects its slabs inside: measured fastest). Same results
+ * either way; 0 to 15, else EMS_E_INVALID. */
 int ems_gpu_set_unfused_prefetch(ems_gpu *h, int which);
 
 /* Tile order of the fused kernels (no reference counterpart): groups of

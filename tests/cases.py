@@ -34,7 +34,7 @@ def graded(length: float, cells: int, alpha: float):
 # and whether the tiled step corrects its z-normal CPML slabs outside itself,
 # splits its tiles in classes, and which unfused kernels load ahead.
 SCHEDULE = {"fused": 1, "tile_rows": 0, "x_chunks": 0, "split": 0, "classes": 0,
-            "unfused_pf": 1}
+            "unfused_pf": 11}
 
 
 def make(kind: str, grid, mats=None, faces=None, ts=None, threads=1, curl_order=0, pool=True):
@@ -47,7 +47,7 @@ def make(kind: str, grid, mats=None, faces=None, ts=None, threads=1, curl_order=
         e.set_schedule(SCHEDULE["fused"], SCHEDULE["tile_rows"], SCHEDULE["x_chunks"])
         e.set_cpml_split(bool(SCHEDULE.get("split", 0)))
         e.set_tile_classes(bool(SCHEDULE.get("classes", 0)))
-        e.set_unfused_prefetch(SCHEDULE.get("unfused_pf", 1))
+        e.set_unfused_prefetch(SCHEDULE.get("unfused_pf", 11))
         return e
     from oracle.bridge import OracleEngine
     return OracleEngine(grid, dt, lanes, curl_order=curl_order, threads=threads, pool=pool)

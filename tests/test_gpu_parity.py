@@ -18,19 +18,27 @@ pytestmark = pytest.mark.gpu
 # full-row and the packed-row kernels (packed: grids with nz <= 64; the
 # others run the tiled kernel), the band kernel (bands of 4, 8, 16 rows) and
 # the tiled kernel with its z-normal CPML slabs in passes of their own
-# (flag 1) and with its tiles in two classes (flag 2). The unfused H kernel
-# loads the next plane ahead by default; flag 4 has both unfused kernels do
-# so, flag 8 neither.
-SCHEDULES = [(0, 0, 0), (0, 0, 3), (0, 0, 0, 4), (0, 0, 3, 4), (0, 0, 0, 8), (1, 3, 0), (1, 7, 0), (1, 15, 0), (1, 7, 5), (2, 7, 3), (2, 15, 0),
+# (flag 1) and with its tiles in two classes (flag 2). The unfused kernels
+# load the next plane ahead and the E one corrects its CPML slabs inside by
+# default; flag 4 has both load ahead with every slab in launches of its
+# own, flag 8 neither load ahead, flag 16 both correct their slabs inside,
+# flag 32 only H loads ahead and corrects inside, flag 64 only H loads ahead
+# and every slab is outside.
+SCHEDULES = [(0, 0, 0), (0, 0, 3), (0, 0, 0, 4), (0, 0, 3, 4), (0, 0, 0, 8), (0, 0, 0, 16),
+             (0, 0, 5, 16), (0, 0, 0, 32), (0, 0, 0, 64), (0, 0, 4, 64),
+             (1, 3, 0), (1, 7, 0), (1, 15, 0), (1, 7, 5), (2, 7, 3), (2, 15, 0),
              (3, 3, 0), (3, 7, 0), (3, 7, 4), (4, 7, 0), (4, 7, 3), (5, 7, 0),
              (6, 4, 0), (6, 8, 3), (6, 16, 0), (6, 16, 5), (1, 7, 0, 1), (2, 3, 5, 1),
              (1, 7, 0, 2), (2, 7, 3, 2), (2, 15, 4, 2), (2, 3, 5, 3)]
+UNFUSED = {4: ("pf_cpml_out", 3), 8: ("plain", 0), 16: ("pf_cpml_in", 15),
+           32: ("pf_h_cpml_in_h", 5), 64: ("pf_cpml_out_h_only_pf", 1)}
 
 
 def schedule_id(s):
     flags = s[3] if len(s) > 3 else 0
     return ("mode%d_tj%d_xc%d" % s[:3] + ("_split" if flags & 1 else "")
-            + ("_classes" if flags & 2 else "") + ("_pf" if flags & 4 else "") + ("_nopf" if flags & 8 else ""))
+            + ("_classes" if flags & 2 else "")
+            + ("_" + UNFUSED[flags & 124][0] if flags & 124 else ""))
 
 
 @pytest.fixture(autouse=True, params=SCHEDULES, ids=schedule_id)
@@ -39,7 +47,7 @@ def schedule(request):
     fl = flags[0] if flags else 0
     old = dict(cases.SCHEDULE)
     cases.SCHEDULE.update(fused=f, tile_rows=t, x_chunks=c, split=fl & 1, classes=(fl >> 1) & 1,
-                          unfused_pf=3 if fl & 4 else (0 if fl & 8 else 1))
+                          unfused_pf=UNFUSED[fl & 124][1] if fl & 124 else 11)
     yield
     cases.SCHEDULE.update(old)
 
