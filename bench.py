@@ -10,7 +10,10 @@ with all inputs resident in HBM before the timed region.
 Prints one JSON line with a roofline figure for the dominant kernel (HIP
 events on the engine's stream) and a CPU baseline: the C oracle (a port of
 the reference engine, x-slab threads) timed on a bounded sample of the same
-workload.
+workload. The default run also carries, under "dispersive_box", the line of
+the 384^3-cell dispersive / conformal / plane-wave box (unfused schedule,
+8000 steps, run in a child process after the CPML box's engine is closed;
+--no-dispersive skips it).
 """
 from __future__ import annotations
 
@@ -291,6 +294,8 @@ def main() -> int:
     ap.add_argument("--x-chunks", type=int, default=None,
                     help="x chunks per model (0 automatic; patch workloads default 16)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-dispersive", action="store_true",
+                    help="cpml_box: skip the dispersive box run the line carries beside it")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -400,6 +405,11 @@ def main() -> int:
     if not args.no_cpu_baseline:
         eng.close()
         cpu = cpu_baseline(wl, args.cpu_seconds, log)
+    dispersive = None
+    if args.workload == "cpml_box" and not args.no_dispersive:
+        if args.no_cpu_baseline:
+            eng.close()
+        dispersive = dispersive_line(log)
 
     n = wl.grid.cells()
     line = {
@@ -419,8 +429,36 @@ def main() -> int:
                    "autotune": tuned},
         "roofline": roof, "cpu_baseline": cpu,
     }
+    if dispersive is not None:
+        line["dispersive_box"] = dispersive
     print(json.dumps(line), flush=True)
     return 0
+
+
+def dispersive_line(log):
+    """The dispersive / conformal / plane-wave box (384x384x256 cells, the
+    unfused schedule, its full 8000 steps) run by this script in a child
+    process, its line's figures kept: the default run carries them beside
+    the CPML box's."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "dispersive_box",
+           "--no-cpu-baseline"]
+    log("dispersive box: " + " ".join(cmd[1:]))
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after 600 s"}
+    for l in r.stderr.splitlines():
+        if l.startswith("[bench]"):
+            log("  " + l)
+    if r.returncode != 0:
+        return {"error": f"exit status {r.returncode}", "stderr_tail": r.stderr[-400:]}
+    try:
+        d = json.loads(r.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": "no JSON line", "stdout_tail": r.stdout[-400:]}
+    return {k: d[k] for k in ("metric", "value", "unit", "steps", "warmup", "ms_per_step", "dtype",
+                              "data", "config", "roofline")}
 
 
 if __name__ == "__main__":

@@ -497,62 +497,20 @@ __device__ __forceinline__ size_t cpml_index(const FSlab &s, int i, int j, int k
 // derivative along normal n used by component c (cpml.rs:495-521); cf<c>
 // the cb (E) or db (H) of component c.
 //
-// Every slab has its own psi array, and the corrections of one component
-// add up in registration order, so the loads of all slabs can go first and
-// the psi stores last: up to CPML_SLOTS slabs (a node meets at most two
-// per face normal and side) every load is in flight at once instead of one
-// memory round trip per slab. More slabs take the plain loop.
-constexpr int CPML_SLOTS = 6;
-
+// One slab at a time: a form that issued every slab's loads first (up to 6
+// slabs' psi, b, a, k in registers) held 24 more VGPRs, six waves per SIMD
+// instead of seven for the whole tiled kernel, and measured slower on the
+// 512^3 CPML box (2.976 against 2.849 ms per launch; with 2 slots 2.949).
 //
 // INPLACE (the unfused kernels, which compute each node once): the psi of
 // parity `par` is read and written (the fused kernel's halo recompute reads
-// one buffer and writes the other). SLOTS false: the plain loop only (the
-// unfused kernels' columns mostly lie outside every slab, and the slot
-// loads would be wasted).
-template <bool ESIDE, bool SLOTS = true, bool INPLACE = false>
+// one buffer and writes the other).
+template <bool ESIDE, bool INPLACE = false>
 __device__ __forceinline__ void cpml_node(const FSlab *__restrict__ slabs, uint32_t mask, int par,
                                           bool write_psi, int i, int j, int k, float d01,
                                           float d02, float d10, float d12, float d20, float d21,
                                           float cf0, float cf1, float cf2, float &t0, float &t1,
                                           float &t2) {
-  if (SLOTS && __builtin_popcount(mask) <= CPML_SLOTS) {
-    float pv[CPML_SLOTS], bv[CPML_SLOTS], av[CPML_SLOTS], kv[CPML_SLOTS];
-    uint32_t m = mask;
-#pragma unroll
-    for (int q = 0; q < CPML_SLOTS; q++) {
-      pv[q] = bv[q] = av[q] = kv[q] = 0.0f;
-      if (m) {
-        const FSlab s = cload(slabs, __builtin_ctz(m));
-        m &= m - 1;
-        const bool in = cpml_inside(s, j, k);
-        // (outside the box: entry 0 of each array, a valid address)
-        const int z = in ? cpml_depth(s, i, j, k) : 0;
-        const size_t qq = in ? cpml_index(s, i, j, k) : 0;
-        const GAS float *pin = (const GAS float *)((ESIDE || par == 0) ? s.psi0 : s.psi1);
-        pv[q] = pin[qq];
-        bv[q] = ((const GAS float *)s.b)[z];
-        av[q] = ((const GAS float *)s.a)[z];
-        kv[q] = ((const GAS float *)s.k)[z];
-      }
-    }
-    m = mask;
-#pragma unroll
-    for (int q = 0; q < CPML_SLOTS; q++) {
-      if (m) {
-        const FSlab s = cload(slabs, __builtin_ctz(m));
-        m &= m - 1;
-        if (cpml_inside(s, j, k)) {
-          float ps;
-          cpml_apply<ESIDE>(s, pv[q], bv[q], av[q], kv[q], d01, d02, d10, d12, d20, d21, cf0, cf1,
-                            cf2, t0, t1, t2, ps);
-          GAS float *pout = (GAS float *)((ESIDE || par == (INPLACE ? 0 : 1)) ? s.psi0 : s.psi1);
-          if (write_psi) pout[cpml_index(s, i, j, k)] = ps;
-        }
-      }
-    }
-    return;
-  }
   while (mask) {
     const int q = __builtin_ctz(mask);
     mask &= mask - 1;
@@ -663,7 +621,7 @@ __global__ void __launch_bounds__(256) h_update_pf(Fields f, Lanes L, int nx, in
       if (iv && kv) h1 = h_expr<OE>(cur.da[1], h1, cur.db[1], cur.ux_k1, cur.uxc, uzn, uzc);
       if (iv && jv) h2 = h_expr<OE>(cur.da[2], h2, cur.db[2], uyn, uyc, cur.ux_j1, cur.uxc);
       // T_p = u_other[node + e_p] - u_other[node] per (component, normal)
-      cpml_node<false, false, true>(cp.slabs, m, cp.par, true, i, j, k, cur.uz_j1 - uzc,
+      cpml_node<false, true>(cp.slabs, m, cp.par, true, i, j, k, cur.uz_j1 - uzc,
                                     cur.uy_k1 - uyc, uzn - uzc, cur.ux_k1 - cur.uxc, uyn - uyc,
                                     cur.ux_j1 - cur.uxc, cur.db[0], cur.db[1], cur.db[2], h0, h1,
                                     h2);
@@ -734,7 +692,7 @@ __global__ void __launch_bounds__(256) e_update_pf(Fields f, Lanes L, int nx, in
       if (j + 1 < ny) e1 = e_expr<OE>(cur.ca[1], e1, cur.cb[1], cur.ix, hx_km, cur.iz, izp);
       if (k + 1 < nz) e2 = e_expr<OE>(cur.ca[2], e2, cur.cb[2], cur.iy, iyp, cur.ix, hx_jm);
       // S_p = i_other[node] - i_other[node - e_p] (cpml.rs:495-521)
-      cpml_node<true, false, true>(cp.slabs, m, 0, true, i, j, k, cur.iz - hz_jm, cur.iy - hy_km,
+      cpml_node<true, true>(cp.slabs, m, 0, true, i, j, k, cur.iz - hz_jm, cur.iy - hy_km,
                                    cur.iz - izp, cur.ix - hx_km, cur.iy - iyp, cur.ix - hx_jm,
                                    cur.cb[0], cur.cb[1], cur.cb[2], e0, e1, e2);
       ux[o] = e0, uy[o] = e1, uz[o] = e2;
