@@ -294,6 +294,8 @@ def main() -> int:
     ap.add_argument("--x-chunks", type=int, default=None,
                     help="x chunks per model (0 automatic; patch workloads default 16)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--xcd-group", type=int, default=None,
+                    help="with --tile-rows: tile rows per XCD visit (0 launch order)")
     ap.add_argument("--no-dispersive", action="store_true",
                     help="cpml_box: skip the dispersive box run the line carries beside it")
     args = ap.parse_args()
@@ -342,6 +344,8 @@ def main() -> int:
         eng.set_schedule(False)
     elif args.tile_rows or args.no_autotune:
         eng.set_schedule(args.mode, args.tile_rows, args.x_chunks)
+        if args.xcd_group is not None:
+            eng.set_xcd_group(args.xcd_group)
     else:
         # The autotuner's trial steps are part of the (untimed) warm-up.
         t0 = eng.timestep()

@@ -1130,7 +1130,8 @@ class Engine:
         when listed). Returns the choice with its mean kernel time in ms,
         and all timings."""
         if candidates is None:
-            candidates = [(m, t, c) for m in (1, 2) for t in (3, 7, 15) for c in (0, 16, 32, 64)]
+            candidates = [(m, t, c) for m in (1, 2) for t in (3, 7, 15)
+                          for c in (0, 16, 32, 64, 96, 128)]
             if self.grid.nodes()[2] <= 64:
                 candidates += [(m, 7, c) for m in (4, 5) for c in (0, 16, 32, 64)]
         results = []
