@@ -12,8 +12,9 @@ events on the engine's stream) and a CPU baseline: the C oracle (a port of
 the reference engine, x-slab threads) timed on a bounded sample of the same
 workload. The default run also carries, under "dispersive_box", the line of
 the 384^3-cell dispersive / conformal / plane-wave box (unfused schedule,
-8000 steps, run in a child process after the CPML box's engine is closed;
---no-dispersive skips it).
+8000 steps) and, under "patch_sweep", the batched 256-model patch-antenna
+sweep's S11 solves/hr, each run in a child process after the CPML box's
+engine is closed (--no-dispersive skips both).
 """
 from __future__ import annotations
 
@@ -297,7 +298,7 @@ def main() -> int:
     ap.add_argument("--xcd-group", type=int, default=None,
                     help="with --tile-rows: tile rows per XCD visit (0 launch order)")
     ap.add_argument("--no-dispersive", action="store_true",
-                    help="cpml_box: skip the dispersive box run the line carries beside it")
+                    help="cpml_box: skip the dispersive box and sweep runs the line carries")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -409,11 +410,18 @@ def main() -> int:
     if not args.no_cpu_baseline:
         eng.close()
         cpu = cpu_baseline(wl, args.cpu_seconds, log)
-    dispersive = None
+    dispersive = sweep_line = None
     if args.workload == "cpml_box" and not args.no_dispersive:
         if args.no_cpu_baseline:
             eng.close()
-        dispersive = dispersive_line(log)
+        dispersive = child_line(log, ["--workload", "dispersive_box", "--no-cpu-baseline"],
+                                ("metric", "value", "unit", "steps", "warmup", "ms_per_step",
+                                 "dtype", "data", "config", "roofline"))
+        # the batched leg of the 256-model patch sweep (each model of a batch
+        # is bit-identical to its own engine: the tests pin that)
+        sweep_line = child_line(log, ["--workload", "patch_sweep", "--skip-sequential"],
+                                ("metric", "value", "unit", "steps", "dtype", "data", "config",
+                                 "phases_s", "cell_updates", "batched_s"))
 
     n = wl.grid.cells()
     line = {
@@ -435,19 +443,19 @@ def main() -> int:
     }
     if dispersive is not None:
         line["dispersive_box"] = dispersive
+    if sweep_line is not None:
+        line["patch_sweep"] = sweep_line
     print(json.dumps(line), flush=True)
     return 0
 
 
-def dispersive_line(log):
-    """The dispersive / conformal / plane-wave box (384x384x256 cells, the
-    unfused schedule, its full 8000 steps) run by this script in a child
-    process, its line's figures kept: the default run carries them beside
-    the CPML box's."""
+def child_line(log, argv, keys):
+    """This script run in a child process on another workload (after the
+    CPML box's engine is closed), the listed figures of its line kept: the
+    default run carries them beside the CPML box's."""
     import subprocess
-    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "dispersive_box",
-           "--no-cpu-baseline"]
-    log("dispersive box: " + " ".join(cmd[1:]))
+    cmd = [sys.executable, os.path.abspath(__file__)] + list(argv)
+    log("child: " + " ".join(cmd[1:]))
     try:
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
     except subprocess.TimeoutExpired:
@@ -461,8 +469,7 @@ def dispersive_line(log):
         d = json.loads(r.stdout.strip().splitlines()[-1])
     except (ValueError, IndexError):
         return {"error": "no JSON line", "stdout_tail": r.stdout[-400:]}
-    return {k: d[k] for k in ("metric", "value", "unit", "steps", "warmup", "ms_per_step", "dtype",
-                              "data", "config", "roofline")}
+    return {k: d.get(k) for k in keys}
 
 
 if __name__ == "__main__":
