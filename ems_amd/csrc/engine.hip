@@ -1162,8 +1162,9 @@ struct FusedArgs {
 // Many models of uneven sizes in one launch: seg[m] is the first block of
 // model m (a multiple of 8, so the XCD order holds within each model),
 // seg[nm] the grid size.
+// (Held to eight waves per SIMD without the prefetch, as step_fused is.)
 template <bool OE, int TJ, bool PF>
-__global__ void __launch_bounds__(64 * (TJ + 1))
+__global__ void __launch_bounds__(64 * (TJ + 1), PF ? 1 : 8)
     step_fused_batch(const FusedArgs *__restrict__ ms, const uint32_t *__restrict__ seg, int nm,
                      int xcd_group) {
   const int b = blockIdx.x;
@@ -1351,8 +1352,12 @@ __device__ __forceinline__ void fused_packed(const float *__restrict__ A, float 
   }
 }
 
+// Without the prefetch held to eight waves per SIMD: 57 VGPRs allow them
+// and the scalar registers spill into vector lanes (the 256-model patch
+// batch, 16 x chunks: 1.201 ms per step against 1.331 at seven waves and
+// 1.336 with the prefetch at six).
 template <bool OE, int NW, bool PF>
-__global__ void __launch_bounds__(64 * NW)
+__global__ void __launch_bounds__(64 * NW, PF ? 1 : 8)
     step_packed(const float *__restrict__ A, float *__restrict__ B, size_t N,
                 const float *__restrict__ cpool, const uint32_t *__restrict__ croff,
                 const FSlab *__restrict__ slabs, const uint32_t *__restrict__ pmask, int nslab,
@@ -1366,7 +1371,7 @@ __global__ void __launch_bounds__(64 * NW)
 
 // The batched form: FusedArgs::tk holds R (rows per wave), ktiles is 1.
 template <bool OE, int NW, bool PF>
-__global__ void __launch_bounds__(64 * NW)
+__global__ void __launch_bounds__(64 * NW, PF ? 1 : 8)
     step_packed_batch(const FusedArgs *__restrict__ ms, const uint32_t *__restrict__ seg, int nm,
                       int xcd_group) {
   const int b = blockIdx.x;

@@ -32,8 +32,9 @@ def limits():
                        excitation_end=E.Waveform.gaussian(2e9, 1e9).duration())
 
 
+@pytest.mark.parametrize("prefetch", [False, True], ids=["noprefetch", "prefetch"])
 @pytest.mark.parametrize("tile_rows", [3, 7, "packed"])
-def test_batch_run_until_matches_single_runs(tile_rows):
+def test_batch_run_until_matches_single_runs(tile_rows, prefetch):
     ms = models()
     single = [engine(m) for m in ms]
     outs = []
@@ -43,9 +44,9 @@ def test_batch_run_until_matches_single_runs(tile_rows):
     batch = [engine(m) for m in ms]
     b = E.EngineBatch([e for e, _ in batch])
     if tile_rows == "packed":
-        b.set_schedule(7, 0, packed=True)
+        b.set_schedule(7, 0, prefetch=prefetch, packed=True)
     else:
-        b.set_schedule(tile_rows, 0)
+        b.set_schedule(tile_rows, 0, prefetch=prefetch)
     bouts = b.run_until(limits())
     for (e, p), (f, q), o, bo in zip(single, batch, outs, bouts):
         assert (o.reason, o.steps, o.timestep) == (bo.reason, bo.steps, bo.timestep)
