@@ -10,7 +10,7 @@ from .engine import (  # noqa: F401
     C0, EPS0, MU0, Z0, Axis, CellMaterials, Cpml, CpmlBoundary, EdgeSource, EndCriteria, Engine,
     EngineBatch, EngineBuilder, Face, FaceCondition, Grid, Material, Probe, ProbeQuantity, RunLimits,
     RunOutcome, Settle, SourceKind, StopReason, TimeStep, Waveform, coefficients,
-    courant_limit, cpml_coefficients, drive_script, edge_materials, pool_lane)
+    courant_limit, cpml_coefficients, drive_script, edge_materials, energy_many, pool_lane)
 from .lumped import (  # noqa: F401,E402
     EdgeValues, LumpedBox, LumpedElement, LumpedPort, Topology, fold_parallel)
 from .dispersive import Dispersive, Field, Pole, Scheme  # noqa: F401,E402

@@ -165,6 +165,8 @@ ems_gpu_planewave_aux = _sig("ems_gpu_planewave_aux", _i, _p, _i, _dp, _dp)
 ems_gpu_add_merged = _sig("ems_gpu_add_merged", _i, _p, _p, C.POINTER(_i))
 ems_gpu_merged_state = _sig("ems_gpu_merged_state", _i, _p, _i, _dp, _dp)
 ems_gpu_batch_create = _sig("ems_gpu_batch_create", _i, C.POINTER(_p), _i, C.POINTER(_p))
+ems_gpu_energy_many = _sig("ems_gpu_energy_many", _i, C.POINTER(_p), _i,
+                           C.POINTER(C.c_uint8), _dp)
 ems_gpu_batch_destroy = _sig("ems_gpu_batch_destroy", None, _p)
 ems_gpu_batch_last_error = _sig("ems_gpu_batch_last_error", C.c_char_p, _p)
 ems_gpu_batch_set_schedule = _sig("ems_gpu_batch_set_schedule", _i, _p, _i, _i, _i)

@@ -2534,13 +2534,13 @@ __global__ void __launch_bounds__(256) dft_run(const double *__restrict__ x, uin
 // (a0 + a1) + (a2 + a3). Rounds of 64 NT terms, NT per lane (NT sized to
 // the rows: 1 for rows of at most 64 nodes, up to 8).
 template <bool OE, int NT, bool MEAS>
-__global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, int ny, int nz,
-                                                     double *rowsum) {
+__device__ __forceinline__ void energy_h_rows_at(const Fields &f, const Lanes &L, int nx, int ny,
+                                                 int nz, double *rowsum, size_t blk,
+                                                 double (*buf)[64 * NT]) {
   constexpr int RND = 64 * NT;
-  __shared__ double buf[4][RND];
   const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
   const int rows_per_plane = 3 * ny - 2;
-  const size_t rid = (size_t)blockIdx.x * 4 + wv;
+  const size_t rid = blk * 4 + wv;
   const size_t total = (size_t)nx * rows_per_plane;
   if (rid >= total) return;  // uniform per wave; no block barrier below
   const int gi = (int)(rid / rows_per_plane);
@@ -2600,17 +2600,26 @@ __global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, 
   if (t == 0) rowsum[rid] = MEAS ? rw * ((acc + a1) + (a2 + a3)) : (acc + a1) + (a2 + a3);
 }
 
+template <bool OE, int NT, bool MEAS>
+__global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, int ny, int nz,
+                                                     double *rowsum) {
+  __shared__ double buf[4][64 * NT];
+  energy_h_rows_at<OE, NT, MEAS>(f, L, nx, ny, nz, rowsum, blockIdx.x, buf);
+}
+
 // E part: per (plane, component) four chains over the in-plane index
 // q = 0, 1, 2, 3 mod 4 in increasing q (e_energy_row, kernel.rs:218-244),
 // a wave per (plane, component), lanes 0..3 the chains. Rounds of 1024
 // terms, 16 per lane.
+constexpr int ECH_RND = 1024;  // terms per round of energy_e_chains (16 per lane)
+
 template <bool MEAS>
-__global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx, int ny, int nz,
-                                                       double *chains) {
+__device__ __forceinline__ void energy_e_chains_at(const Fields &f, const Lanes &L, int nx, int ny,
+                                                   int nz, double *chains, size_t blk,
+                                                   double (*buf)[ECH_RND]) {
   constexpr int NT = 16, RND = 64 * NT;
-  __shared__ double buf[4][RND];
   const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
-  const size_t w = (size_t)blockIdx.x * 4 + wv;
+  const size_t w = blk * 4 + wv;
   if (w >= (size_t)nx * 3) return;  // uniform per wave
   const int a = (int)(w % 3), gi = (int)(w / 3);
   const size_t plane = (size_t)ny * nz;
@@ -2652,14 +2661,21 @@ __global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx
   if (t < 4) chains[w * 4 + t] = acc;
 }
 
+template <bool MEAS>
+__global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx, int ny, int nz,
+                                                       double *chains) {
+  __shared__ double buf[4][ECH_RND];
+  energy_e_chains_at<MEAS>(f, L, nx, ny, nz, chains, blockIdx.x, buf);
+}
+
 // Plane values we + wh (h_slab, kernel.rs:404-420): a wave per plane, the
 // plane's row sums added in row order by lane 0 from LDS.
 template <bool MEAS>
-__global__ void __launch_bounds__(256) energy_planes(int nx, int ny, const double *rowsum,
-                                                     const double *chains, double *planes) {
-  __shared__ double buf[4][256];
+__device__ __forceinline__ void energy_planes_at(int nx, int ny, const double *rowsum,
+                                                 const double *chains, double *planes, int blk,
+                                                 double (*buf)[256]) {
   const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
-  const int gi = (int)blockIdx.x * 4 + wv;
+  const int gi = blk * 4 + wv;
   if (gi >= nx) return;  // uniform per wave
   const int rows_per_plane = 3 * ny - 2;
   const double *rs = rowsum + (size_t)gi * rows_per_plane;
@@ -2689,6 +2705,13 @@ __global__ void __launch_bounds__(256) energy_planes(int nx, int ny, const doubl
     }
     planes[gi] = MEAS ? EPS0_D * we + MU0_D * wh : we + wh;
   }
+}
+
+template <bool MEAS>
+__global__ void __launch_bounds__(256) energy_planes(int nx, int ny, const double *rowsum,
+                                                     const double *chains, double *planes) {
+  __shared__ double buf[4][256];
+  energy_planes_at<MEAS>(nx, ny, rowsum, chains, planes, (int)blockIdx.x, buf);
 }
 
 // Parts::energy_openems_slab (upml.rs:505-537): per H run of the layers
@@ -2790,9 +2813,8 @@ __global__ void __launch_bounds__(64) lookahead_edges(Fields f, Lanes L, int n, 
 // The plane values summed in plane order (pool.rs:675-692), folding from
 // -0.0, scaled by dt/4; one wave.
 template <bool MEAS>
-__global__ void __launch_bounds__(64) energy_combine(int nx, const double *planes, double dt,
-                                                     double *out) {
-  __shared__ double buf[256];
+__device__ __forceinline__ void energy_combine_at(int nx, const double *planes, double dt,
+                                                  double *out, double *buf) {
   const int t = threadIdx.x;
   double s = -0.0;
   for (int g0 = 0; g0 < nx; g0 += 256) {
@@ -2806,6 +2828,79 @@ __global__ void __launch_bounds__(64) energy_combine(int nx, const double *plane
     __syncthreads();
   }
   if (t == 0) *out = MEAS ? s : 0.25 * dt * s;  // scale_energy (pool.rs:684-689)
+}
+
+template <bool MEAS>
+__global__ void __launch_bounds__(64) energy_combine(int nx, const double *planes, double dt,
+                                                     double *out) {
+  __shared__ double buf[256];
+  energy_combine_at<MEAS>(nx, planes, dt, out, buf);
+}
+
+// ---- the invariant of many models at once (ems_gpu_energy_many) ----
+//
+// One launch per kernel above over the models of a table: each model's
+// blocks follow the previous model's (first block hb, eb, pb of the H-row,
+// E-chain and plane launches; the table's last entry holds the totals), a
+// block finds its model by binary search and runs that model's block
+// blk - first. The same terms in the same order as the single-model
+// launches, so the same bits.
+struct EnergyModel {
+  Fields f;
+  Lanes L;
+  int nx, ny, nz;
+  double dt;
+  double *rowsum, *chains, *planes;
+  uint32_t hb, eb, pb;
+};
+
+// The model of block b (the last whose first block is <= b), wave-uniform.
+// The kernels then read the model's entry in place (a copy of it would go
+// to scratch memory, its fields being passed by reference).
+template <int W>
+__device__ __forceinline__ int energy_model_of(const EnergyModel *__restrict__ ms, int nm,
+                                               uint32_t b) {
+  int lo = 0, hi = nm;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    const EnergyModel *m = ms + mid;
+    const uint32_t first = cload(W == 0 ? &m->hb : (W == 1 ? &m->eb : &m->pb));
+    if (first <= b) lo = mid; else hi = mid;
+  }
+  return __builtin_amdgcn_readfirstlane(lo);
+}
+
+template <bool OE, int NT>
+__global__ void __launch_bounds__(256) energy_h_rows_many(const EnergyModel *__restrict__ ms,
+                                                          int nm) {
+  __shared__ double buf[4][64 * NT];
+  const EnergyModel *m = ms + energy_model_of<0>(ms, nm, blockIdx.x);
+  energy_h_rows_at<OE, NT, false>(m->f, m->L, m->nx, m->ny, m->nz, m->rowsum,
+                                  blockIdx.x - cload(&m->hb), buf);
+}
+
+__global__ void __launch_bounds__(256) energy_e_chains_many(const EnergyModel *__restrict__ ms,
+                                                            int nm) {
+  __shared__ double buf[4][ECH_RND];
+  const EnergyModel *m = ms + energy_model_of<1>(ms, nm, blockIdx.x);
+  energy_e_chains_at<false>(m->f, m->L, m->nx, m->ny, m->nz, m->chains,
+                            blockIdx.x - cload(&m->eb), buf);
+}
+
+__global__ void __launch_bounds__(256) energy_planes_many(const EnergyModel *__restrict__ ms,
+                                                          int nm) {
+  __shared__ double buf[4][256];
+  const EnergyModel *m = ms + energy_model_of<2>(ms, nm, blockIdx.x);
+  energy_planes_at<false>(m->nx, m->ny, m->rowsum, m->chains, m->planes,
+                          (int)(blockIdx.x - cload(&m->pb)), buf);
+}
+
+// a block per model; W^n of model q into out[q]
+__global__ void __launch_bounds__(64) energy_combine_many(const EnergyModel *__restrict__ ms,
+                                                          double *out) {
+  __shared__ double buf[256];
+  const EnergyModel *m = ms + blockIdx.x;
+  energy_combine_at<false>(m->nx, m->planes, m->dt, out + blockIdx.x, buf);
 }
 
 // ------------------------------------------------------------------ host --
@@ -2908,6 +3003,11 @@ struct ems_gpu {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipEvent_t ev_energy = nullptr;  // the energy's copies have landed
+  // ems_gpu_energy_many's table and results when this engine is the first
+  // active model (device and pinned host copies, capacity in models)
+  EnergyModel *em_d = nullptr, *em_h = nullptr;
+  double *em_dout = nullptr, *em_hout = nullptr;
+  int em_cap = 0;
   Fields fb[2]{};  // ping-pong field buffers (views into fbuf)
   float *fbuf[2] = {nullptr, nullptr};  // u_x..u_z, i_x..i_z, N nodes each
   // coefficient pool: a zero row, then every lane's rows; croff[16*r + l]
@@ -4247,6 +4347,8 @@ void ems_gpu_destroy(ems_gpu *h) {
   if (h->side) (void)hipStreamDestroy(h->side);
   if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
   if (h->ev_energy) (void)hipEventDestroy(h->ev_energy);
+  if (h->em_d) (void)hipFree(h->em_d), (void)hipFree(h->em_dout);
+  if (h->em_h) (void)hipHostFree(h->em_h), (void)hipHostFree(h->em_hout);
   if (h->ev_join) (void)hipEventDestroy(h->ev_join);
   if (h->h_w) (void)hipHostFree(h->h_w);
   delete h;
@@ -5520,8 +5622,111 @@ int ems_gpu_energy_in(ems_gpu *h, int measure, double *w) {
   return EMS_OK;
 }
 
+// The batched form of ems_gpu_energy_many: two or more active models, one
+// curl order and one H-row round width (nz class), none with operator
+// energies read from the device. Every model's earlier work is waited for,
+// then one launch per energy kernel over all of them on the first model's
+// stream and one copy of the results; each value then gets its operators'
+// (zero) terms as ems_gpu_energy adds them.
+static bool energy_many_batchable(ems_gpu *const *models, int n, const uint8_t *active) {
+  int cnt = 0, oe = -1, nt = -1;
+  uint64_t hb = 0, eb = 0;
+  for (int q = 0; q < n; q++) {
+    if (active && !active[q]) continue;
+    const ems_gpu *h = models[q];
+    if (!h || h->dead) return false;
+    const int nz = h->n[2], t = nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8));
+    if ((oe >= 0 && oe != (h->oe ? 1 : 0)) || (nt >= 0 && nt != t)) return false;
+    oe = h->oe ? 1 : 0, nt = t;
+    for (auto &o : h->ops)
+      if (o.type == OP_POLES || o.type == OP_MERGED || (o.type == OP_LUMPED && o.lp.branch))
+        return false;
+    hb += ((uint64_t)h->n[0] * (3 * (uint64_t)h->n[1] - 2) + 3) / 4;
+    eb += ((uint64_t)h->n[0] * 3 + 3) / 4;
+    cnt++;
+  }
+  return cnt >= 2 && hb < (1ull << 31) && eb < (1ull << 31);
+}
+
+static int energy_many_batched(ems_gpu *const *models, int n, const uint8_t *active,
+                               double *out) {
+  std::vector<int> act;
+  for (int q = 0; q < n; q++)
+    if (!active || active[q]) act.push_back(q);
+  ems_gpu *h0 = models[act[0]];
+  const int nm = (int)act.size();
+  for (int q : act) {
+    ems_gpu *h = models[q];
+    const int nx = h->n[0], ny = h->n[1];
+    if (!h->d_rowsum) {
+      CK(h, hipMalloc(&h->d_rowsum, (size_t)nx * (3 * ny - 2) * sizeof(double)));
+      CK(h, hipMalloc(&h->d_chains, (size_t)nx * 12 * sizeof(double)));
+      CK(h, hipMalloc(&h->d_planes, (size_t)nx * sizeof(double)));
+      CK(h, hipMalloc(&h->d_w, sizeof(double)));
+      CK(h, hipHostMalloc((void **)&h->h_w, sizeof(double), hipHostMallocDefault));
+    }
+    CK(h, hipStreamSynchronize(h->stream));
+  }
+  if (h0->em_cap < nm) {
+    if (h0->em_d) {
+      CK(h0, hipFree(h0->em_d));
+      CK(h0, hipFree(h0->em_dout));
+    }
+    if (h0->em_h) {
+      CK(h0, hipHostFree(h0->em_h));
+      CK(h0, hipHostFree(h0->em_hout));
+    }
+    h0->em_d = h0->em_h = nullptr, h0->em_dout = h0->em_hout = nullptr, h0->em_cap = 0;
+    CK(h0, hipMalloc(&h0->em_d, nm * sizeof(EnergyModel)));
+    CK(h0, hipMalloc(&h0->em_dout, nm * sizeof(double)));
+    CK(h0, hipHostMalloc((void **)&h0->em_h, nm * sizeof(EnergyModel), hipHostMallocDefault));
+    CK(h0, hipHostMalloc((void **)&h0->em_hout, nm * sizeof(double), hipHostMallocDefault));
+    h0->em_cap = nm;
+  }
+  uint32_t hb = 0, eb = 0, pb = 0;
+  for (int i = 0; i < nm; i++) {
+    ems_gpu *h = models[act[i]];
+    EnergyModel &m = h0->em_h[i];
+    m.f = h->fb[h->par];
+    m.L = lanes_arg(h);
+    m.nx = h->n[0], m.ny = h->n[1], m.nz = h->n[2];
+    m.dt = h->dt;
+    m.rowsum = h->d_rowsum, m.chains = h->d_chains, m.planes = h->d_planes;
+    m.hb = hb, m.eb = eb, m.pb = pb;
+    hb += (uint32_t)(((size_t)m.nx * (3 * m.ny - 2) + 3) / 4);
+    eb += (uint32_t)(((size_t)m.nx * 3 + 3) / 4);
+    pb += (uint32_t)((m.nx + 3) / 4);
+  }
+  hipStream_t st = h0->stream;
+  CK(h0, hipMemcpyAsync(h0->em_d, h0->em_h, nm * sizeof(EnergyModel), hipMemcpyHostToDevice, st));
+  const int nz0 = h0->n[2], nt = nz0 <= 64 ? 1 : (nz0 <= 128 ? 2 : (nz0 <= 256 ? 4 : 8));
+  const EnergyModel *tab = h0->em_d;
+#define EHM(OE, NT) \
+  hipLaunchKernelGGL((energy_h_rows_many<OE, NT>), dim3(hb), dim3(256), 0, st, tab, nm)
+  if (h0->oe) {
+    if (nt == 1) EHM(true, 1); else if (nt == 2) EHM(true, 2); else if (nt == 4) EHM(true, 4);
+    else EHM(true, 8);
+  } else {
+    if (nt == 1) EHM(false, 1); else if (nt == 2) EHM(false, 2); else if (nt == 4) EHM(false, 4);
+    else EHM(false, 8);
+  }
+#undef EHM
+  CK(h0, hipGetLastError());
+  hipLaunchKernelGGL(energy_e_chains_many, dim3(eb), dim3(256), 0, st, tab, nm);
+  CK(h0, hipGetLastError());
+  hipLaunchKernelGGL(energy_planes_many, dim3(pb), dim3(256), 0, st, tab, nm);
+  CK(h0, hipGetLastError());
+  hipLaunchKernelGGL(energy_combine_many, dim3(nm), dim3(64), 0, st, tab, h0->em_dout);
+  CK(h0, hipGetLastError());
+  CK(h0, hipMemcpyAsync(h0->em_hout, h0->em_dout, nm * sizeof(double), hipMemcpyDeviceToHost, st));
+  CK(h0, hipStreamSynchronize(st));
+  for (int i = 0; i < nm; i++) out[act[i]] = h0->em_hout[i] + ops_energy_finish(models[act[i]]);
+  return EMS_OK;
+}
+
 int ems_gpu_energy_many(ems_gpu *const *models, int n, const uint8_t *active, double *out) {
   if (!models || n < 0 || !out) return EMS_E_INVALID;
+  if (energy_many_batchable(models, n, active)) return energy_many_batched(models, n, active, out);
   for (int q = 0; q < n; q++) {
     if (active && !active[q]) continue;
     ems_gpu *h = models[q];

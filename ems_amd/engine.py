@@ -1169,6 +1169,24 @@ class Engine:
         return tuple(out)
 
 
+def energy_many(engines: Sequence["Engine"], active: Optional[Sequence[bool]] = None) -> List[float]:
+    """ems_gpu_energy_many: the energy of every engine (of the active ones;
+    the others' entries are 0.0), each the value its own energy() gives.
+    Engines without operator energies (no poles, cut-face currents or
+    lumped branches) of one curl order and row width are evaluated
+    together, one launch per kernel."""
+    n = len(engines)
+    arr = (C.c_void_p * n)(*[e._h for e in engines])
+    act = None
+    if active is not None:
+        act = np.ascontiguousarray(np.array([1 if a else 0 for a in active], dtype=np.uint8))
+    out = np.zeros(n, dtype=np.float64)
+    rc = nat.ems_gpu_energy_many(arr, n, nat.ptr(act, C.c_uint8), out.ctypes.data_as(nat._dp))
+    if rc != 0:
+        raise EmsError(rc, "ems_gpu_energy_many failed")
+    return [float(v) for v in out]
+
+
 class EngineBatch:
     """Engines stepped together (include/ems_gpu.h batches): one fused launch
     over the tiles of every active model per step, each model bit-identical
