@@ -360,9 +360,14 @@ typedef struct {
 } ems_merged_desc;
 int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id);
 /* ems_gpu_energy of many engines at once (no reference counterpart; the
- * batch run loop's energy checks): every active engine's evaluation is
- * queued before any is waited for. out[q] for active[q] != 0 (all when
- * active is NULL). */
+ * batch run loop's energy checks), each out[q] the value ems_gpu_energy
+ * gives, for active[q] != 0 (all when active is NULL). Two or more active
+ * engines of one curl order and row-width class (nz up to 64, 128, 256,
+ * more) without operator energies (dispersive media, cut-face currents,
+ * lumped branches) are evaluated together: each engine's queued work is
+ * waited for, then one launch per energy kernel over all of them on the
+ * first one's stream. Otherwise every evaluation is queued on its engine's
+ * stream before any is waited for. */
 int ems_gpu_energy_many(ems_gpu *const *models, int n, const uint8_t *active, double *out);
 
 /* The reduced currents g (n_groups values) and the circulations of the
