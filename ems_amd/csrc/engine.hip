@@ -2600,6 +2600,77 @@ __device__ __forceinline__ void energy_h_rows_at(const Fields &f, const Lanes &L
   if (t == 0) rowsum[rid] = MEAS ? rw * ((acc + a1) + (a2 + a3)) : (acc + a1) + (a2 + a3);
 }
 
+// The same row sums two rows per wave for rows of at most 32 nodes (half
+// a wave per row, one round; lanes 0..3 of each half its accumulators):
+// the same terms added in the same order. Rows with no terms (the last
+// plane's y and z rows) load nothing.
+template <bool OE, bool MEAS>
+__device__ __forceinline__ void energy_h_rows2_at(const Fields &f, const Lanes &L, int nx, int ny,
+                                                  int nz, double *rowsum, size_t blk,
+                                                  double (*buf)[64]) {
+  const int wv = threadIdx.x >> 6, t = threadIdx.x & 63;
+  const int slot = t >> 5, tl = t & 31;
+  const int rows_per_plane = 3 * ny - 2;
+  const size_t total = (size_t)nx * rows_per_plane;
+  const size_t rid = (blk * 4 + wv) * 2 + slot;
+  if ((blk * 4 + wv) * 2 >= total) return;  // uniform per wave; no block barrier below
+  const bool live = rid < total;
+  const size_t rq = live ? rid : 0;  // (a slot past the end: row 0, nothing written)
+  const int gi = (int)(rq / rows_per_plane);
+  const int rr = (int)(rq % rows_per_plane);
+  int comp, j;
+  if (rr < ny - 1) {
+    comp = 0, j = rr;
+  } else if (rr < 2 * ny - 1) {
+    comp = 1, j = rr - (ny - 1);
+  } else {
+    comp = 2, j = rr - (2 * ny - 1);
+  }
+  const size_t plane = (size_t)ny * nz;
+  const size_t base = (size_t)gi * plane + (size_t)j * nz;
+  const size_t r = (size_t)gi * ny + j;
+  const int n = (comp == 0 || gi + 1 < nx) ? (comp == 2 ? nz : nz - 1) : 0;
+  const int ca_ = comp == 0 ? 2 : (comp == 1 ? 0 : 1), cc_ = comp == 0 ? 1 : (comp == 1 ? 2 : 0);
+  const float *A = pick3(f.u, ca_), *Cc = pick3(f.u, cc_), *H = pick3(f.i, comp);
+  const size_t sa = comp == 0 ? (size_t)nz : (comp == 1 ? 1 : plane);
+  const size_t sc = comp == 0 ? 1 : (comp == 1 ? plane : (size_t)nz);
+  const Lane lda = pick_lane(L, 6 + comp), ldb = pick_lane(L, 9 + comp);
+  double *b = buf[wv];
+  double term = 0.0;
+  if (tl < n) {  // (n <= nz <= 32)
+    const int k = tl;
+    const size_t o = base + k;
+    const float va = A[o + sa], vb = A[o], vc = Cc[o + sc], vd = Cc[o], old = H[o];
+    const float da = coef(lda, r, k, nz, o), db = coef(ldb, r, k, nz, o);
+    const float nw = h_expr<OE>(da, old, db, va, vb, vc, vd);
+    if (MEAS)
+      term = (comp == 2 && (k == 0 || k == n - 1) ? 4.0 : 1.0) * oe_square((double)nw);
+    else
+      term = weight(da, (double)old * (double)nw, db);
+  }
+  b[t] = term;
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  double acc = 0.0;
+  if (tl < 4) acc = chain_add(acc, b + 32 * slot, tl, n, 4);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int l0 = 32 * slot;
+  const double a1 = __shfl(acc, l0 + 1), a2 = __shfl(acc, l0 + 2), a3 = __shfl(acc, l0 + 3);
+  const double rw = MEAS && ((comp == 0 && (gi == 0 || gi + 1 == nx)) ||
+                             (comp == 1 && (j == 0 || j + 1 == ny)))
+                        ? 4.0 : 1.0;
+  if (tl == 0 && live)
+    rowsum[rid] = MEAS ? rw * ((acc + a1) + (a2 + a3)) : (acc + a1) + (a2 + a3);
+}
+
+template <bool OE, bool MEAS>
+__global__ void __launch_bounds__(256) energy_h_rows2(Fields f, Lanes L, int nx, int ny, int nz,
+                                                      double *rowsum) {
+  __shared__ double buf[4][64];
+  energy_h_rows2_at<OE, MEAS>(f, L, nx, ny, nz, rowsum, blockIdx.x, buf);
+}
+
 template <bool OE, int NT, bool MEAS>
 __global__ void __launch_bounds__(256) energy_h_rows(Fields f, Lanes L, int nx, int ny, int nz,
                                                      double *rowsum) {
@@ -2877,6 +2948,16 @@ __global__ void __launch_bounds__(256) energy_h_rows_many(const EnergyModel *__r
   const EnergyModel *m = ms + energy_model_of<0>(ms, nm, blockIdx.x);
   energy_h_rows_at<OE, NT, false>(m->f, m->L, m->nx, m->ny, m->nz, m->rowsum,
                                   blockIdx.x - cload(&m->hb), buf);
+}
+
+// (rows of at most 32 nodes: two per wave, eight per block)
+template <bool OE>
+__global__ void __launch_bounds__(256) energy_h_rows2_many(const EnergyModel *__restrict__ ms,
+                                                           int nm) {
+  __shared__ double buf[4][64];
+  const EnergyModel *m = ms + energy_model_of<0>(ms, nm, blockIdx.x);
+  energy_h_rows2_at<OE, false>(m->f, m->L, m->nx, m->ny, m->nz, m->rowsum,
+                               blockIdx.x - cload(&m->hb), buf);
 }
 
 __global__ void __launch_bounds__(256) energy_e_chains_many(const EnergyModel *__restrict__ ms,
@@ -4140,13 +4221,22 @@ int energy_kernels_t(ems_gpu *h, double *rowsum, double *chains, double *planes,
   const int nx = h->n[0], ny = h->n[1], nz = h->n[2];
   const size_t rows = (size_t)nx * (3 * ny - 2);
   const Fields &f = h->fb[h->par];
-  // a wave per H row, per (plane, component) E chains and per plane
+  // a wave per H row (two per wave when rows have at most 32 nodes), per
+  // (plane, component) E chains and per plane
   dim3 g1((unsigned)((rows + 3) / 4));
-  const int nt = nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8));
+  const int nt = nz <= 32 ? 0 : (nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8)));
 #define EH(OE, NT)                                                                        \
   hipLaunchKernelGGL((energy_h_rows<OE, NT, MEAS>), g1, dim3(256), 0, h->stream, f,       \
                      lanes_arg(h), nx, ny, nz, rowsum)
-  if (h->oe) {
+  if (nt == 0) {
+    const dim3 g8((unsigned)((rows + 7) / 8));
+    if (h->oe)
+      hipLaunchKernelGGL((energy_h_rows2<true, MEAS>), g8, dim3(256), 0, h->stream, f,
+                         lanes_arg(h), nx, ny, nz, rowsum);
+    else
+      hipLaunchKernelGGL((energy_h_rows2<false, MEAS>), g8, dim3(256), 0, h->stream, f,
+                         lanes_arg(h), nx, ny, nz, rowsum);
+  } else if (h->oe) {
     if (nt == 1) EH(true, 1); else if (nt == 2) EH(true, 2); else if (nt == 4) EH(true, 4);
     else EH(true, 8);
   } else {
@@ -5635,13 +5725,14 @@ static bool energy_many_batchable(ems_gpu *const *models, int n, const uint8_t *
     if (active && !active[q]) continue;
     const ems_gpu *h = models[q];
     if (!h || h->dead) return false;
-    const int nz = h->n[2], t = nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8));
+    const int nz = h->n[2];
+    const int t = nz <= 32 ? 0 : (nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8)));
     if ((oe >= 0 && oe != (h->oe ? 1 : 0)) || (nt >= 0 && nt != t)) return false;
     oe = h->oe ? 1 : 0, nt = t;
     for (auto &o : h->ops)
       if (o.type == OP_POLES || o.type == OP_MERGED || (o.type == OP_LUMPED && o.lp.branch))
         return false;
-    hb += ((uint64_t)h->n[0] * (3 * (uint64_t)h->n[1] - 2) + 3) / 4;
+    hb += ((uint64_t)h->n[0] * (3 * (uint64_t)h->n[1] - 2) + (t ? 3 : 7)) / (t ? 4 : 8);
     eb += ((uint64_t)h->n[0] * 3 + 3) / 4;
     cnt++;
   }
@@ -5683,6 +5774,9 @@ static int energy_many_batched(ems_gpu *const *models, int n, const uint8_t *act
     CK(h0, hipHostMalloc((void **)&h0->em_hout, nm * sizeof(double), hipHostMallocDefault));
     h0->em_cap = nm;
   }
+  const int nz0 = h0->n[2];
+  const int nt = nz0 <= 32 ? 0 : (nz0 <= 64 ? 1 : (nz0 <= 128 ? 2 : (nz0 <= 256 ? 4 : 8)));
+  const size_t rows_per_block = nt ? 4 : 8;  // H rows of one block
   uint32_t hb = 0, eb = 0, pb = 0;
   for (int i = 0; i < nm; i++) {
     ems_gpu *h = models[act[i]];
@@ -5693,17 +5787,21 @@ static int energy_many_batched(ems_gpu *const *models, int n, const uint8_t *act
     m.dt = h->dt;
     m.rowsum = h->d_rowsum, m.chains = h->d_chains, m.planes = h->d_planes;
     m.hb = hb, m.eb = eb, m.pb = pb;
-    hb += (uint32_t)(((size_t)m.nx * (3 * m.ny - 2) + 3) / 4);
+    hb += (uint32_t)(((size_t)m.nx * (3 * m.ny - 2) + rows_per_block - 1) / rows_per_block);
     eb += (uint32_t)(((size_t)m.nx * 3 + 3) / 4);
     pb += (uint32_t)((m.nx + 3) / 4);
   }
   hipStream_t st = h0->stream;
   CK(h0, hipMemcpyAsync(h0->em_d, h0->em_h, nm * sizeof(EnergyModel), hipMemcpyHostToDevice, st));
-  const int nz0 = h0->n[2], nt = nz0 <= 64 ? 1 : (nz0 <= 128 ? 2 : (nz0 <= 256 ? 4 : 8));
   const EnergyModel *tab = h0->em_d;
 #define EHM(OE, NT) \
   hipLaunchKernelGGL((energy_h_rows_many<OE, NT>), dim3(hb), dim3(256), 0, st, tab, nm)
-  if (h0->oe) {
+  if (nt == 0) {
+    if (h0->oe)
+      hipLaunchKernelGGL((energy_h_rows2_many<true>), dim3(hb), dim3(256), 0, st, tab, nm);
+    else
+      hipLaunchKernelGGL((energy_h_rows2_many<false>), dim3(hb), dim3(256), 0, st, tab, nm);
+  } else if (h0->oe) {
     if (nt == 1) EHM(true, 1); else if (nt == 2) EHM(true, 2); else if (nt == 4) EHM(true, 4);
     else EHM(true, 8);
   } else {

@@ -362,8 +362,8 @@ int ems_gpu_add_merged(ems_gpu *h, const ems_merged_desc *d, int *id);
 /* ems_gpu_energy of many engines at once (no reference counterpart; the
  * batch run loop's energy checks), each out[q] the value ems_gpu_energy
  * gives, for active[q] != 0 (all when active is NULL). Two or more active
- * engines of one curl order and row-width class (nz up to 64, 128, 256,
- * more) without operator energies (dispersive media, cut-face currents,
+ * engines of one curl order and row-width class (nz up to 32, 64, 128,
+ * 256, more) without operator energies (dispersive media, cut-face currents,
  * lumped branches) are evaluated together: each engine's queued work is
  * waited for, then one launch per energy kernel over all of them on the
  * first one's stream. Otherwise every evaluation is queued on its engine's
