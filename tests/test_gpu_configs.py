@@ -11,6 +11,14 @@
   from the first step; 24 steps on the device's default schedule and on the
   unfused one; all 48 probe traces, the f64 energy and all six fields bit
   for bit.
+* The 384x384x256-cell dispersive / conformal / plane-wave box at full size
+  (385x385x257 nodes: 10-cell CPML, 32 Lorentz and 16 Drude spheres, the
+  conformal PEC cylinder with its reduced currents, the TF/SF plane wave):
+  every field seeded the same way, so that every operator works from the
+  first step; 10 steps on the device's default unfused kernels (loading
+  ahead, the CPML-E slabs inside the E kernel) with the energy, operators'
+  terms included, after 6 and 10, the probe trace and all six fields bit
+  for bit.
 """
 import os
 
@@ -165,5 +173,31 @@ def test_coefficient_pool_over_4_gib():
     wo = o.energy()
     assert np.float64(we).tobytes() == np.float64(wo).tobytes(), (we, wo)
     same_fields(e, o, "pool_over_4_gib")
+    e.close()
+    o.close()
+
+
+def test_dispersive_box_full_size():
+    from oracle.bridge import OracleEngine
+    wl = W.dispersive_box()
+    assert wl.grid.nodes() == (385, 385, 257)
+    init = seeded_fields(wl.grid.nodes(), 384)
+    e = E.Engine(wl.grid, wl.dt, wl.lanes)
+    (pe,) = wl.build_ops(e)
+    for field, a, v in init:
+        e.set_component(field, a, v)
+    o = OracleEngine(wl.grid, wl.dt, wl.lanes, threads=oracle_threads())
+    (po,) = wl.build_ops(o)
+    for field, a, v in init:
+        o.set_component(field, a, v)
+    for n in (5, 3):
+        e.run(n)
+        o.run(n)
+        we, wo = e.step_with_energy(), o.step_with_energy()
+        assert np.float64(we).tobytes() == np.float64(wo).tobytes(), (we, wo)
+    a, b = pe.values(), o.probe_values(po)
+    assert a.size == b.size == 10 and np.abs(b).max() > 0.0
+    assert np.array_equal(bits(a), bits(b)), "probe differs"
+    same_fields(e, o, "dispersive_box_384")
     e.close()
     o.close()
