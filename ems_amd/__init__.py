@@ -19,6 +19,6 @@ from .post import PortResult, dft, s_param  # noqa: F401,E402
 from .learn import ConvNet, EarlyStop, TraceFormat  # noqa: F401,E402
 from .mur import MurBoundary  # noqa: F401,E402
 from .upml import OpenEmsPml, openems_pml_sigma  # noqa: F401,E402
-from . import openems_probes  # noqa: F401,E402
+from . import openems_probes, raster  # noqa: F401,E402
 from .conformal import (  # noqa: F401,E402
     ConformalEdge, ConformalFace, ConformalOptions, ConformalPec, ConformalReport)

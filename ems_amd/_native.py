@@ -88,6 +88,11 @@ class RunLimitsC(C.Structure):
                 ("energy_measure", C.c_int32)]
 
 
+class SolidC(C.Structure):
+    """ems_solid (include/ems_host.h)."""
+    _fields_ = [("kind", C.c_int32), ("axis", C.c_int32), ("p", C.c_double * 6)]
+
+
 class PoleEntryC(C.Structure):
     """ems_pole_entry (include/ems_host.h)."""
     _fields_ = [("field", C.c_int32), ("axis", C.c_int32), ("node", C.c_int32 * 3),
@@ -212,6 +217,8 @@ ems_coefficients_from_cells = _sig("ems_coefficients_from_cells", _i, _dp, _i, _
                                    _u16p, _dp, _i, _u8p, _ip, _d, _i, _i, _fp)
 ems_edge_materials_from_cells = _sig("ems_edge_materials_from_cells", _i, _dp, _i, _dp, _i, _dp,
                                      _i, _u16p, _dp, _i, _i, _dp, _dp)
+ems_raster_cells = _sig("ems_raster_cells", C.c_int64, _dp, _i, _dp, _i, _dp, _i,
+                        C.POINTER(SolidC), C.c_uint16, _u16p)
 ems_pool_lane = _sig("ems_pool_lane", _i, _fp, _i, _i, _i, _fp, _u32p, C.POINTER(_u64))
 ems_dispersive_fill_cells = _sig("ems_dispersive_fill_cells", _i, _dp, _i, _dp, _i, _dp, _i,
                                  C.POINTER(C.c_int32), _i, C.POINTER(PoleEntryC), _i,

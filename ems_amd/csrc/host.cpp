@@ -223,6 +223,67 @@ int ems_coefficients_from_cells(const double *x, int nx, const double *y, int ny
   return EMS_OK;
 }
 
+int64_t ems_raster_cells(const double *x, int nx, const double *y, int ny, const double *z,
+                         int nz, const ems_solid *solid, uint16_t id, uint16_t *cell_mat) {
+  if (!x || !y || !z || !solid || !cell_mat || nx < 2 || ny < 2 || nz < 2) return EMS_E_INVALID;
+  if (solid->kind < 0 || solid->kind > 2 || (solid->kind == 2 && (solid->axis < 0 ||
+                                                                   solid->axis > 2)))
+    return EMS_E_INVALID;
+  for (int q = 0; q < 6; q++)
+    if (std::isnan(solid->p[q])) return EMS_E_INVALID;
+  const double *l[3] = {x, y, z};
+  const int nc[3] = {nx - 1, ny - 1, nz - 1};
+  std::vector<double> c[3];
+  for (int a = 0; a < 3; a++) {
+    c[a].resize(nc[a]);
+    for (int i = 0; i < nc[a]; i++) c[a][i] = 0.5 * (l[a][i] + l[a][i + 1]);
+  }
+  // per axis the solid's extent [lo, hi] (closed for the round shapes'
+  // bounding box: its test decides) and the cells whose centres lie in it
+  const double *p = solid->p;
+  double lo[3], hi[3];
+  int rad[2] = {0, 0};
+  if (solid->kind == 0) {
+    for (int a = 0; a < 3; a++) lo[a] = p[a], hi[a] = p[3 + a];
+  } else if (solid->kind == 1) {
+    for (int a = 0; a < 3; a++) lo[a] = p[a] - p[3], hi[a] = p[a] + p[3];
+  } else {
+    const int ax = solid->axis;
+    rad[0] = ax == 0 ? 1 : 0, rad[1] = ax == 2 ? 1 : 2;
+    lo[rad[0]] = p[0] - p[2], hi[rad[0]] = p[0] + p[2];
+    lo[rad[1]] = p[1] - p[2], hi[rad[1]] = p[1] + p[2];
+    lo[ax] = p[3], hi[ax] = p[4];
+  }
+  int i0[3], i1[3];
+  for (int a = 0; a < 3; a++) {
+    i0[a] = (int)(std::lower_bound(c[a].begin(), c[a].end(), lo[a]) - c[a].begin());
+    i1[a] = (int)(std::upper_bound(c[a].begin(), c[a].end(), hi[a]) - c[a].begin());
+    if (i1[a] <= i0[a]) return 0;
+  }
+  int64_t count = 0;
+  for (int i = i0[0]; i < i1[0]; i++)
+    for (int j = i0[1]; j < i1[1]; j++) {
+      uint16_t *row = cell_mat + ((size_t)i * nc[1] + j) * nc[2];
+      for (int k = i0[2]; k < i1[2]; k++) {
+        const double cc[3] = {c[0][i], c[1][j], c[2][k]};
+        bool in;
+        if (solid->kind == 0) {
+          in = true;
+          for (int a = 0; a < 3; a++) in = in && lo[a] <= cc[a] && cc[a] < hi[a];
+        } else if (solid->kind == 1) {
+          const double dx = cc[0] - p[0], dy = cc[1] - p[1], dz = cc[2] - p[2];
+          in = dx * dx + dy * dy + dz * dz < p[3] * p[3];
+        } else {
+          const int ax = solid->axis;
+          const double d0 = cc[rad[0]] - p[0], d1 = cc[rad[1]] - p[1];
+          in = d0 * d0 + d1 * d1 < p[2] * p[2] && p[3] <= cc[ax] && cc[ax] < p[4];
+        }
+        if (in) row[k] = id, count++;
+      }
+    }
+  return count;
+}
+
 int ems_edge_materials_from_cells(const double *x, int nx, const double *y, int ny,
                                   const double *z, int nz, const uint16_t *cell_mat,
                                   const double *mat_table, int n_mat, int threads,

@@ -193,6 +193,13 @@ class CellMaterials:
     def set(self, cell, m: Material) -> None:
         self.ids[cell[0], cell[1], cell[2]] = self._id(m)
 
+    def fill_solid(self, solid, m: Material) -> int:
+        """The cells of a box, sphere or cylinder (ems_amd.raster.Solid) by
+        their centres, rasterised by the host library; returns how many."""
+        from .raster import raster_cells
+        q = self._id(m)
+        return raster_cells(self.grid, solid, self.ids, q)
+
     def _pec(self) -> np.ndarray:
         if self.pec is None:
             n = self.grid.nodes()

@@ -18,6 +18,7 @@ from typing import List
 import numpy as np
 
 from . import engine as E
+from .raster import Solid
 
 
 @dataclass
@@ -68,7 +69,10 @@ def cpml_box(n: int = 512, steps: int = 5000, seed: int = 2026, blocks: int = 64
         size = rng.integers(10, 61, size=3)
         size = np.minimum(size, hi_c - lo_c)
         start = [int(rng.integers(lo_c, hi_c - size[a] + 1)) for a in range(3)]
-        m.fill_box([(start[a], start[a] + int(size[a])) for a in range(3)], fr4)
+        # the block between node lines start and start + size: its cells by
+        # their centres (the host library's rasteriser)
+        m.fill_solid(Solid.box([g.lines[a][start[a]] for a in range(3)],
+                               [g.lines[a][start[a] + int(size[a])] for a in range(3)]), fr4)
     dt = E.TimeStep.courant(0.99).resolve(g)
     lanes = E.coefficients(g, m, [E.FaceCondition.Pec] * 6, dt, threads=threads)
     c = n // 2

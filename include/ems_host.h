@@ -52,6 +52,29 @@ int ems_edge_materials_from_cells(const double *x, int nx, const double *y, int 
                                   const double *mat_table, int n_mat, int threads,
                                   double *eps_r, double *mu_r);
 
+/* A solid for the cell rasteriser (no reference counterpart: the reference
+ * takes its solids from its external csx crate, whose rasteriser is not
+ * part of it). kind 0: the box [p0, p3) x [p1, p4) x [p2, p5); 1: the
+ * sphere of centre (p0, p1, p2) and radius p3; 2: the cylinder along
+ * `axis` of radius p2 around (p0, p1) in the other two axes (in increasing
+ * axis order), between p3 and p4 along it. Coordinates in metres. */
+typedef struct {
+    int32_t kind, axis;
+    double p[6];
+} ems_solid;
+
+/* Sets cell_mat[cell] = id (cell (i*ncy + j)*ncz + k of the nx-1 by ny-1
+ * by nz-1 cells) for every cell whose centre (the midpoints of its node
+ * lines, 0.5 (l[i] + l[i+1])) lies inside the solid: box lo <= c < hi per
+ * axis; sphere (cx - p0)^2 + (cy - p1)^2 + (cz - p2)^2 < r^2; cylinder the
+ * same over its two radial axes, and lo <= c < hi along its axis. Only the
+ * cells of the solid's bounding box are visited. Returns the number of
+ * cells set, or a negative EMS_E_* code. Fills the cell materials that
+ * ems_coefficients_from_cells and ems_edge_materials_from_cells average
+ * onto the edges (a quarter of each edge's dual face per touching cell). */
+int64_t ems_raster_cells(const double *x, int nx, const double *y, int ny, const double *z,
+                         int nz, const ems_solid *solid, uint16_t id, uint16_t *cell_mat);
+
 /* Lane::pool (src/engine/coeffs.rs:560-598): pools the nx*ny z-rows of a
  * dense lane if at most max(count/4, 1) are distinct. Returns 1 and fills
  * rows_out (<= count/4 rows, first-seen order) / index_out (nx*ny) when
