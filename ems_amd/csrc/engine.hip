@@ -5487,7 +5487,7 @@ static int ops_energy_enqueue(ems_gpu *h) {
       }
       if (o.type == OP_MERGED) {
         const MergedHost &mh = *o.mfh;
-        const size_t nr = mh.mem.size(), nmv = mh.mval.size();
+        const size_t nr = mh.mem.size();
         std::vector<uint32_t> row0(nr);
         for (size_t b = 0; b + 1 < mh.bs.size(); b++)
           for (uint32_t r = mh.bs[b]; r < mh.bs[b + 1]; r++) row0[r] = mh.bs[b];
@@ -5504,7 +5504,6 @@ static int ops_energy_enqueue(ems_gpu *h) {
         if ((rc = upload(h, (void **)&o.d_erow0, uv.data(), uv.size() * sizeof(uint32_t))))
           return rc;
         o.mem.push_back(o.d_erow0);
-        (void)nmv;
       }
     }
     if (o.type == OP_POLES) {
