@@ -150,6 +150,10 @@ def sweep(args, log) -> int:
     engines, ports = L._batch_engines(models)
     b = E.EngineBatch(engines)
     b.set_schedule(tj, args.x_chunks, packed=packed)
+    if args.no_graphs:
+        b.set_graphs(False)
+        log(f"device copy bandwidth: {engines[0].copy_bandwidth(1 << 30, 5):.0f} GB/s "
+            "(calibration for counter passes)")
     t1 = time.perf_counter()
     outs = b.run_until(lim)
     t2 = time.perf_counter()
@@ -168,8 +172,9 @@ def sweep(args, log) -> int:
     # common buffer parity (a batch steps in lockstep) stepped 64 more
     # times in one batch, the time per step from the synchronised runs'
     # wall clock (graph replays of the fused and gap kernels), algorithmic
-    # bytes the models' traffic models summed. No counter evidence: a kernel
-    # trace of the batch's graph launches crashed inside the runtime.
+    # bytes the models' traffic models summed, HBM traffic from the counter
+    # passes of a run without graphs (a kernel trace of the batch's graph
+    # launches crashed inside the runtime).
     par = [e.timestep() % 2 for e in engines]
     keep = 1 if 2 * sum(par) >= len(par) else 0
     act = [p_ == keep for p_ in par]
@@ -179,13 +184,21 @@ def sweep(args, log) -> int:
     per_step = (time.perf_counter() - tk) / 64
     byt = sum(e.traffic_model()[0] for e, a in zip(engines, act) if a)
     achieved = byt / per_step / 1e9
-    roof = {"bound": "hbm", "kernel": "step_packed_batch" if packed else "step_fused_batch",
+    kname = "step_packed_batch" if packed else "step_fused_batch"
+    traffic, traffic_cfg = None, None
+    tpath = os.path.join(ROOT, "profiles", f"{kname}_patch_sweep_{len(models)}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        traffic, traffic_cfg = tr["traffic_bytes"], tr["config"]
+    roof = {"bound": "hbm", "kernel": kname,
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_measured_at": traffic_cfg,
             "bytes_per_launch": byt, "launch_ms": round(per_step * 1e3, 4),
             "models": int(sum(act)),
             "timing": "wall clock of synchronised batch runs (64 steps, fused and gap kernels); "
-                      "no counter evidence"}
+                      "counters from a run without graphs"}
     log(f"batched kernel: {per_step*1e3:.3f} ms per step over {sum(act)} models, "
         f"{achieved:.0f} GB/s algorithmic")
     d, same_steps = None, None
@@ -307,6 +320,10 @@ def main() -> int:
     ap.add_argument("--model", default="", help="patch_earlystop: trained network (npz)")
     ap.add_argument("--skip-sequential", action="store_true",
                     help="patch_sweep: the batched leg only (for profiling)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="patch_sweep: the batch queues its launches one by one instead of "
+                         "replaying HIP graphs (same results; for launch-tracing profilers), "
+                         "and a copy-kernel calibration runs first")
     ap.add_argument("--kernel", default="packed", choices=["packed", "tiled"],
                     help="patch workloads: fused kernel (packed short rows where nz <= 64)")
     ap.add_argument("--n", type=int, default=None, help="cells per axis")

@@ -175,6 +175,7 @@ ems_gpu_energy_many = _sig("ems_gpu_energy_many", _i, C.POINTER(_p), _i,
 ems_gpu_batch_destroy = _sig("ems_gpu_batch_destroy", None, _p)
 ems_gpu_batch_last_error = _sig("ems_gpu_batch_last_error", C.c_char_p, _p)
 ems_gpu_batch_set_schedule = _sig("ems_gpu_batch_set_schedule", _i, _p, _i, _i, _i)
+ems_gpu_batch_set_graphs = _sig("ems_gpu_batch_set_graphs", _i, _p, _i)
 ems_gpu_batch_run = _sig("ems_gpu_batch_run", _i, _p, _u8p, _u64)
 
 # include/ems_learn.h

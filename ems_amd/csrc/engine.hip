@@ -6104,6 +6104,7 @@ struct ems_gpu_batch {
   int nblocks = 0;
   hipGraphExec_t g1[2] = {nullptr, nullptr}, gK[2] = {nullptr, nullptr};
   int K = 16;                     // even
+  bool graphs = true;             // ems_gpu_batch_set_graphs
   std::string err;
 };
 
@@ -6286,6 +6287,12 @@ const char *ems_gpu_batch_last_error(const ems_gpu_batch *b) {
   return b ? b->err.c_str() : "null batch";
 }
 
+int ems_gpu_batch_set_graphs(ems_gpu_batch *b, int on) {
+  if (!b) return EMS_E_INVALID;
+  b->graphs = on != 0;
+  return EMS_OK;
+}
+
 int ems_gpu_batch_set_schedule(ems_gpu_batch *b, int mode, int tile_rows, int x_chunks) {
   if (!b) return EMS_E_INVALID;
   if (mode != 1 && mode != 2 && mode != 4 && mode != 5)
@@ -6335,6 +6342,18 @@ int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n) {
   }
   int rc = batch_args(b, act);
   if (rc) return rc;
+  if (!b->graphs) {  // the same launches, queued one by one
+    int p = par;
+    for (uint64_t s = 0; s < n; s++, p ^= 1)
+      if ((rc = batch_enqueue_step(b, act, p))) return rc;
+    BCK(b, hipStreamSynchronize(b->stream));
+    for (int q : act) {
+      ems_gpu *h = b->m[q];
+      h->par = p;
+      advance_counts(h, n);
+    }
+    return EMS_OK;
+  }
   if (act != b->active) {
     batch_drop_graphs(b);
     for (int p = 0; p < 2; p++) {

@@ -1237,6 +1237,11 @@ class EngineBatch:
         mode = (4 if prefetch else 5) if packed else (1 if prefetch else 2)
         self._check(nat.ems_gpu_batch_set_schedule(self._h, mode, int(tile_rows), int(x_chunks)))
 
+    def set_graphs(self, on: bool) -> None:
+        """Whether runs replay captured HIP graphs (default) or queue the
+        same launches one by one; the same results either way."""
+        self._check(nat.ems_gpu_batch_set_graphs(self._h, 1 if on else 0))
+
     def run(self, n: int, active: Optional[Sequence[bool]] = None) -> None:
         act = None
         if active is not None:

@@ -398,6 +398,10 @@ int ems_gpu_batch_set_schedule(ems_gpu_batch *b, int mode, int tile_rows, int x_
 /* n steps of the models with active[q] != 0 (NULL: all); returns after
  * they are done. */
 int ems_gpu_batch_run(ems_gpu_batch *b, const uint8_t *active, uint64_t n);
+/* Whether ems_gpu_batch_run replays captured HIP graphs (on, the default)
+ * or queues the same launches one by one (no reference counterpart: for
+ * tools that trace launches but not graph replays). Same results. */
+int ems_gpu_batch_set_graphs(ems_gpu_batch *b, int on);
 
 /* Engine::run (src/engine/mod.rs:825-828). */
 int ems_gpu_run(ems_gpu *h, uint64_t n);

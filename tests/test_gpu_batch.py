@@ -60,6 +60,27 @@ def test_batch_run_until_matches_single_runs(tile_rows, prefetch):
 
 
 @pytest.mark.parametrize("packed", [False, True])
+def test_batch_without_graphs_matches(packed):
+    ms = models()
+    runs = []
+    for graphs in (True, False):
+        batch = [engine(m) for m in ms]
+        b = E.EngineBatch([e for e, _ in batch])
+        b.set_schedule(7, 2 if packed else 0, packed=packed)
+        b.set_graphs(graphs)
+        b.run(37)  # K = 16 graphs twice, then single steps
+        b.run(5, [True, False, True, False])
+        runs.append((batch, b))
+    for (e, p), (f, q) in zip(runs[0][0], runs[1][0]):
+        same_fields(e, f)
+        assert np.array_equal(bits(p.voltage()), bits(q.voltage()))
+    for batch, b in runs:
+        b.close()
+        for e, _ in batch:
+            e.close()
+
+
+@pytest.mark.parametrize("packed", [False, True])
 def test_batch_subset_runs(packed):
     ms = models()
     single = [engine(m) for m in ms]
