@@ -3803,14 +3803,53 @@ int launch_gap_op(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
 // Post-E gap on the result `f`; `fp` holds the state before a fused step
 // (prev = true), whose u the lumped pre_e hooks capture. Plane waves run
 // at their place among the gap's ops.
+// Whether a gap segment [q0, q1) between own-kernel operators has work, so
+// that empty ones are not launched (what run_gap, gap_mid and gap_post_e do
+// per operator kind): mid gap, the H sources and current probes in it, and
+// the first segment the lumped elements' capture of u^n; post-E gap, the E
+// sources, lumped elements and voltage probes in it, and the first segment
+// the current probes moved there and the capture when asked. The last
+// post-E segment always runs: it advances the step counter.
+bool lumped_captures(const ems_gpu *h) {
+  for (auto &o : h->ops)
+    if (o.type == OP_LUMPED && (o.lp.branch || o.lp.record)) return true;
+  return false;
+}
+
+bool mid_segment_work(const ems_gpu *h, int q0, int q1) {
+  for (int q = q0; q < q1; q++) {
+    const HostOp &o = h->ops[q];
+    if ((o.type == OP_SOURCE && (o.kind == 2 || o.kind == 3)) ||
+        (o.type == OP_PROBE && o.quantity == 1))
+      return true;
+  }
+  return q0 == 0 && lumped_captures(h);
+}
+
+bool post_segment_work(const ems_gpu *h, int q0, int q1, bool capture) {
+  for (int q = q0; q < q1; q++) {
+    const HostOp &o = h->ops[q];
+    if ((o.type == OP_SOURCE && (o.kind == 0 || o.kind == 1)) || o.type == OP_LUMPED ||
+        (o.type == OP_PROBE && o.quantity == 0))
+      return true;
+  }
+  if (q0 != 0) return false;
+  if (capture && lumped_captures(h)) return true;
+  for (auto &o : h->ops)
+    if (o.type == OP_PROBE && o.quantity == 1) return true;
+  return false;
+}
+
 int launch_gap_post_e(ems_gpu *h, const Fields &f, const Fields &fp, bool prev) {
   const int nops = (int)h->ops.size();
   int q0 = 0;
   for (int q = 0; q <= nops; q++) {
     if (q < nops && !own_gap_kernels(h->ops[q])) continue;
     const bool last = q == nops;
-    hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(256), 0, h->stream, f, fp,
-                       prev && q0 == 0 ? 1 : 0, h->d_ops, nops, q0, q, last ? 1 : 0, h->d_step);
+    if (last || post_segment_work(h, q0, q, prev && q0 == 0))
+      hipLaunchKernelGGL(gap_post_e, dim3(1), dim3(256), 0, h->stream, f, fp,
+                         prev && q0 == 0 ? 1 : 0, h->d_ops, nops, q0, q, last ? 1 : 0,
+                         h->d_step);
     CK(h, hipGetLastError());
     if (!last) {
       int rc = launch_gap_op(h, h->ops[q], f, 1);
@@ -3827,8 +3866,9 @@ int launch_gap_mid(ems_gpu *h, const Fields &f) {
   int q0 = 0;
   for (int q = 0; q <= nops; q++) {
     if (q < nops && !own_gap_kernels(h->ops[q])) continue;
-    hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops, nops, q0, q,
-                       (const uint64_t *)h->d_step);
+    if (mid_segment_work(h, q0, q))
+      hipLaunchKernelGGL(gap_mid, dim3(1), dim3(256), 0, h->stream, f, h->d_ops, nops, q0, q,
+                         (const uint64_t *)h->d_step);
     CK(h, hipGetLastError());
     if (q < nops) {
       int rc = launch_gap_op(h, h->ops[q], f, 0);
