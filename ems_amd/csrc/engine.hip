@@ -3034,6 +3034,7 @@ struct HostOp {
   uint32_t *mf_small = nullptr, *mf_big = nullptr;
   int mf_nsmall = 0, mf_nbig = 0, mf_lds_small = 0, mf_lds_big = 0;
   std::shared_ptr<struct MergedHost> mfh;
+  bool circ_stale = false;  // the post-E circulations not yet formed (merged_circ_now)
   // the operator's energy terms: device array, pinned host copy, count;
   // the device tables the terms need (pole constants; reduced currents' l,
   // M rows, block starts)
@@ -3795,14 +3796,30 @@ int launch_merged(ems_gpu *h, const HostOp &o, const Fields &f, int stage, hipSt
 // An own-kernel operator's hooks at its place in a gap: stage 0 the mid
 // gap, 1 the post-E gap. The reduced currents' update itself (stage 2 of
 // launch_merged) ran before, beside the H update (enqueue_step).
+// The reduced currents' post-E stage (the look-ahead circulations of u^n+1)
+// is read only by the energy's look-ahead and the state read-back: the next
+// step computes the same circulations again beside its H update before it
+// uses them. So the post-E gap leaves it, steps mark it stale, and
+// merged_circ_now runs it, as the gap would have, when it is read.
 int launch_gap_op(ems_gpu *h, const HostOp &o, const Fields &f, int stage) {
+  if (o.type == OP_MERGED && stage == 1) return 0;
   return o.type == OP_PW ? launch_pw(h, o, f, stage)
                          : launch_merged(h, o, f, stage, h->stream);
 }
 
-// Post-E gap on the result `f`; `fp` holds the state before a fused step
-// (prev = true), whose u the lumped pre_e hooks capture. Plane waves run
-// at their place among the gap's ops.
+void merged_mark_stale(ems_gpu *h) {
+  for (auto &o : h->ops)
+    if (o.type == OP_MERGED) o.circ_stale = true;
+}
+
+int merged_circ_now(ems_gpu *h, HostOp &o) {
+  if (!o.circ_stale) return 0;
+  int rc = launch_merged(h, o, h->fb[h->par], 1, h->stream);
+  if (rc) return rc;
+  o.circ_stale = false;
+  return 0;
+}
+
 // Whether a gap segment [q0, q1) between own-kernel operators has work, so
 // that empty ones are not launched (what run_gap, gap_mid and gap_post_e do
 // per operator kind): mid gap, the H sources and current probes in it, and
@@ -3840,6 +3857,9 @@ bool post_segment_work(const ems_gpu *h, int q0, int q1, bool capture) {
   return false;
 }
 
+// Post-E gap on the result `f`; `fp` holds the state before a fused step
+// (prev = true), whose u the lumped pre_e hooks capture. Plane waves run
+// at their place among the gap's ops.
 int launch_gap_post_e(ems_gpu *h, const Fields &f, const Fields &fp, bool prev) {
   const int nops = (int)h->ops.size();
   int q0 = 0;
@@ -4364,6 +4384,7 @@ int run_steps(ems_gpu *h, uint64_t n) {
     h->par ^= flip;
     left--;
   }
+  merged_mark_stale(h);
   advance_counts(h, n);
   return 0;
 }
@@ -5416,7 +5437,9 @@ int ems_gpu_merged_state(ems_gpu *h, int id, double *cur, double *circ) {
   ALIVE(h);
   if (id < 0 || id >= (int)h->ops.size() || h->ops[id].type != OP_MERGED)
     return fail(h, EMS_E_INVALID, "not a reduced-current operator id");
-  const HostOp &o = h->ops[id];
+  HostOp &o = h->ops[id];
+  int rc = merged_circ_now(h, o);
+  if (rc) return rc;
   CK(h, hipStreamSynchronize(h->stream));
   const size_t bytes = (size_t)o.mf.ng * sizeof(double);
   if (cur && bytes) CK(h, hipMemcpy(cur, o.mf.cur, bytes, hipMemcpyDeviceToHost));
@@ -5556,6 +5579,8 @@ static int ops_energy_enqueue(ems_gpu *h) {
         off += (size_t)np;
       }
     } else if (o.type == OP_MERGED) {
+      int rc = merged_circ_now(h, o);
+      if (rc) return rc;
       const DevMerged &m = o.mf;
       const int na = o.mf_nsmall + (m.ns + 63) / 64;
       if (na)
@@ -6053,6 +6078,7 @@ int ems_gpu_time_kernels(ems_gpu *h, uint64_t steps, double out[3]) {
     t0 += a, t1 += b, ts += c;
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
+  merged_mark_stale(h);
   advance_counts(h, steps);
   out[0] = t0 / steps, out[1] = t1 / steps, out[2] = ts / steps;
   return EMS_OK;
