@@ -446,7 +446,9 @@ def main() -> int:
             "bytes_per_launch": byt, "launch_ms": round(dur, 4),
             "step_ms_events": round(ts, 4), "device_copy_GBs": round(stream_gbs, 1),
             "frac_of_copy": round(achieved / stream_gbs, 4)}
-    log(f"kernels: {name} {th:.3f} ms (E {te:.3f} ms), step {ts:.3f} ms; {achieved:.0f} GB/s")
+    halves = f"H {th:.3f} ms, E {te:.3f} ms" if te else f"{th:.3f} ms"
+    log(f"kernels: {halves}, step {ts:.3f} ms; dominant {name} {dur:.3f} ms, "
+        f"{achieved:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu_baseline:
