@@ -2739,6 +2739,58 @@ __global__ void __launch_bounds__(256) energy_e_chains(Fields f, Lanes L, int nx
   energy_e_chains_at<MEAS>(f, L, nx, ny, nz, chains, blockIdx.x, buf);
 }
 
+// The same chains with a block per (plane, component): the four waves
+// compute a round of 4096 consecutive terms (16 per lane) into LDS, then
+// lanes 0..3 of the first wave add them to the four chains in increasing q,
+// so each chain gets the same terms in the same order. The in-plane index
+// is 32-bit (planes of fewer than 2^31 nodes; the host checks), which
+// turns the per-term 64-bit division into a 32-bit one by a fixed divisor.
+constexpr int ECB_RND = 4096;
+
+template <bool MEAS>
+__global__ void __launch_bounds__(256) energy_e_chains_blk(Fields f, Lanes L, int nx, int ny,
+                                                           int nz, double *chains) {
+  constexpr int NT = 16;
+  __shared__ double b[ECB_RND];
+  const int t = threadIdx.x;
+  const int w = blockIdx.x;  // 3 * plane + component
+  const int a = w % 3, gi = w / 3;
+  const uint32_t unz = (uint32_t)nz, uny = (uint32_t)ny, plane = uny * unz;
+  const float *u = pick3(f.u, a) + (size_t)gi * plane;
+  const Lane la = pick_lane(L, a), lb = pick_lane(L, 3 + a);
+  double acc = 0.0;
+  for (uint32_t q0 = 0; q0 < plane; q0 += ECB_RND) {
+    float x[NT], ca[NT], cb[NT];
+    bool in[NT];
+#pragma unroll
+    for (int s = 0; s < NT; s++) {
+      const uint32_t q = min(q0 + 256u * s + t, plane - 1);
+      const uint32_t j = q / unz, k = q - j * unz;
+      const size_t r = (size_t)gi * ny + j;
+      const size_t o = (size_t)gi * plane + q;
+      x[s] = u[q];
+      // openEMS's measure leaves out the E on the three upper faces
+      // (kernel.rs:410-418); the zero terms leave the chains unchanged.
+      in[s] = gi + 1 < nx && j + 1 < uny && k + 1 < unz;
+      if (!MEAS) ca[s] = coef(la, r, (int)k, nz, o), cb[s] = coef(lb, r, (int)k, nz, o);
+    }
+#pragma unroll
+    for (int s = 0; s < NT; s++) {
+      const double xd = (double)x[s];
+      double term;
+      if (MEAS)
+        term = in[s] ? oe_square(xd) : 0.0;
+      else
+        term = weight(ca[s], xd * xd, cb[s]);
+      b[256 * s + t] = q0 + 256u * s + t < plane ? term : 0.0;
+    }
+    __syncthreads();
+    if (t < 4) acc = chain_add(acc, b, t, (int)min((uint32_t)ECB_RND, plane - q0), 4);
+    __syncthreads();
+  }
+  if (t < 4) chains[(size_t)w * 4 + t] = acc;
+}
+
 // Plane values we + wh (h_slab, kernel.rs:404-420): a wave per plane, the
 // plane's row sums added in row order by lane 0 from LDS.
 template <bool MEAS>
@@ -4305,9 +4357,14 @@ int energy_kernels_t(ems_gpu *h, double *rowsum, double *chains, double *planes,
   }
 #undef EH
   CK(h, hipGetLastError());
-  dim3 g2((unsigned)(((size_t)nx * 3 + 3) / 4));
-  hipLaunchKernelGGL(energy_e_chains<MEAS>, g2, dim3(256), 0, h->stream, f, lanes_arg(h), nx, ny,
-                     nz, chains);
+  if ((size_t)ny * nz < ((size_t)1 << 31)) {
+    hipLaunchKernelGGL(energy_e_chains_blk<MEAS>, dim3((unsigned)nx * 3), dim3(256), 0, h->stream,
+                       f, lanes_arg(h), nx, ny, nz, chains);
+  } else {
+    dim3 g2((unsigned)(((size_t)nx * 3 + 3) / 4));
+    hipLaunchKernelGGL(energy_e_chains<MEAS>, g2, dim3(256), 0, h->stream, f, lanes_arg(h), nx,
+                       ny, nz, chains);
+  }
   CK(h, hipGetLastError());
   const dim3 gp((unsigned)((nx + 3) / 4));
   hipLaunchKernelGGL(energy_planes<MEAS>, gp, dim3(256), 0, h->stream, nx, ny, rowsum, chains,
