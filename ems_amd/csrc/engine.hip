@@ -2534,6 +2534,40 @@ __global__ void __launch_bounds__(256) dft_run(const double *__restrict__ x, uin
 // (a0 + a1) + (a2 + a3). Rounds of 64 NT terms, NT per lane (NT sized to
 // the rows: 1 for rows of at most 64 nodes, up to 8).
 template <bool OE, int NT, bool MEAS>
+__device__ __forceinline__ double energy_h_round(const float *A, const float *Cc, const float *H,
+                                                 size_t sa, size_t sc, const Lane &lda,
+                                                 const Lane &ldb, size_t base, size_t r, int nz,
+                                                 int comp, int n, int k0, int t, double *b,
+                                                 double acc) {
+  constexpr int RND = 64 * NT;
+  float va[NT], vb[NT], vc[NT], vd[NT], da[NT], db[NT], old[NT];
+#pragma unroll
+  for (int s = 0; s < NT; s++) {
+    const int k = min(k0 + 64 * s + t, n - 1);
+    const size_t o = base + k;
+    va[s] = A[o + sa], vb[s] = A[o], vc[s] = Cc[o + sc], vd[s] = Cc[o], old[s] = H[o];
+    da[s] = coef(lda, r, k, nz, o), db[s] = coef(ldb, r, k, nz, o);
+  }
+#pragma unroll
+  for (int s = 0; s < NT; s++) {
+    const float nw = h_expr<OE>(da[s], old[s], db[s], va[s], vb[s], vc[s], vd[s]);
+    const int k = k0 + 64 * s + t;
+    double term;
+    if (MEAS)  // openEMS's measure: the ends of a z row weigh 4 (kernel.rs:153-156, 497-507)
+      term = (comp == 2 && (k == 0 || k == n - 1) ? 4.0 : 1.0) * oe_square((double)nw);
+    else
+      term = weight(da[s], (double)old[s] * (double)nw, db[s]);
+    b[64 * s + t] = k < n ? term : 0.0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (t < 4) acc = chain_add(acc, b, t, min(RND, n - k0), 4);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  return acc;
+}
+
+template <bool OE, int NT, bool MEAS>
 __device__ __forceinline__ void energy_h_rows_at(const Fields &f, const Lanes &L, int nx, int ny,
                                                  int nz, double *rowsum, size_t blk,
                                                  double (*buf)[64 * NT]) {
@@ -2565,32 +2599,17 @@ __device__ __forceinline__ void energy_h_rows_at(const Fields &f, const Lanes &L
   const Lane lda = pick_lane(L, 6 + comp), ldb = pick_lane(L, 9 + comp);
   double acc = 0.0;  // lanes 0..3: the accumulator of k = lane mod 4
   double *b = buf[wv];
-  for (int k0 = 0; k0 < n; k0 += RND) {
-    float va[NT], vb[NT], vc[NT], vd[NT], da[NT], db[NT], old[NT];
-#pragma unroll
-    for (int s = 0; s < NT; s++) {
-      const int k = min(k0 + 64 * s + t, n - 1);
-      const size_t o = base + k;
-      va[s] = A[o + sa], vb[s] = A[o], vc[s] = Cc[o + sc], vd[s] = Cc[o], old[s] = H[o];
-      da[s] = coef(lda, r, k, nz, o), db[s] = coef(ldb, r, k, nz, o);
-    }
-#pragma unroll
-    for (int s = 0; s < NT; s++) {
-      const float nw = h_expr<OE>(da[s], old[s], db[s], va[s], vb[s], vc[s], vd[s]);
-      const int k = k0 + 64 * s + t;
-      double term;
-      if (MEAS)  // openEMS's measure: the ends of a z row weigh 4 (kernel.rs:153-156, 497-507)
-        term = (comp == 2 && (k == 0 || k == n - 1) ? 4.0 : 1.0) * oe_square((double)nw);
-      else
-        term = weight(da[s], (double)old[s] * (double)nw, db[s]);
-      b[64 * s + t] = k < n ? term : 0.0;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (t < 4) acc = chain_add(acc, b, t, min(RND, n - k0), 4);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-  }
+  // Rounds of RND terms while they fill more than NT - 1 of their 64-term
+  // parts, the rest in rounds of 64 (a 257-node z row: 256 + 64 slots, not
+  // 512). Every round starts at a multiple of 64, so each chain still gets
+  // its terms in increasing k.
+  int k0 = 0;
+  for (; n - k0 > RND - 64; k0 += RND)
+    acc = energy_h_round<OE, NT, MEAS>(A, Cc, H, sa, sc, lda, ldb, base, r, nz, comp, n, k0, t,
+                                       b, acc);
+  for (; k0 < n; k0 += 64)
+    acc = energy_h_round<OE, 1, MEAS>(A, Cc, H, sa, sc, lda, ldb, base, r, nz, comp, n, k0, t, b,
+                                      acc);
   const double a1 = __shfl(acc, 1), a2 = __shfl(acc, 2), a3 = __shfl(acc, 3);
   // openEMS's measure: x rows of the outer x planes and y rows j = 0, ny - 1
   // weigh 4 (h_plane, kernel.rs:457-493)
@@ -4336,8 +4355,11 @@ int energy_kernels_t(ems_gpu *h, double *rowsum, double *chains, double *planes,
   // a wave per H row (two per wave when rows have at most 32 nodes), per
   // (plane, component) E chains and per plane
   dim3 g1((unsigned)((rows + 3) / 4));
-  const int nt = nz <= 32 ? 0 : (nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 ? 4 : 8)));
-#define EH(OE, NT)                                                                        \
+  // (rows longer than 256 nodes: rounds of 512 only where they fill them,
+  // the rest of the row in rounds of 64)
+  const int nt =
+      nz <= 32 ? 0 : (nz <= 64 ? 1 : (nz <= 128 ? 2 : (nz <= 256 || nz - 1 < 512 ? 4 : 8)));
+#define EH(OE, NT)                                                                     \
   hipLaunchKernelGGL((energy_h_rows<OE, NT, MEAS>), g1, dim3(256), 0, h->stream, f,       \
                      lanes_arg(h), nx, ny, nz, rowsum)
   if (nt == 0) {
