@@ -3021,14 +3021,18 @@ __global__ void __launch_bounds__(256) energy_h_rows_many(const EnergyModel *__r
                                   blockIdx.x - cload(&m->hb), buf);
 }
 
-// (rows of at most 32 nodes: two per wave, eight per block)
+// (rows of at most 32 nodes: two per wave, eight per group, EH2_GROUPS
+// groups per block, so that one search of the model table serves 32 rows)
+constexpr int EH2_GROUPS = 4;
+
 template <bool OE>
 __global__ void __launch_bounds__(256) energy_h_rows2_many(const EnergyModel *__restrict__ ms,
                                                            int nm) {
   __shared__ double buf[4][64];
   const EnergyModel *m = ms + energy_model_of<0>(ms, nm, blockIdx.x);
-  energy_h_rows2_at<OE, false>(m->f, m->L, m->nx, m->ny, m->nz, m->rowsum,
-                               blockIdx.x - cload(&m->hb), buf);
+  const size_t g0 = (size_t)(blockIdx.x - cload(&m->hb)) * EH2_GROUPS;
+  for (int g = 0; g < EH2_GROUPS; g++)
+    energy_h_rows2_at<OE, false>(m->f, m->L, m->nx, m->ny, m->nz, m->rowsum, g0 + g, buf);
 }
 
 __global__ void __launch_bounds__(256) energy_e_chains_many(const EnergyModel *__restrict__ ms,
@@ -5875,7 +5879,8 @@ static bool energy_many_batchable(ems_gpu *const *models, int n, const uint8_t *
     for (auto &o : h->ops)
       if (o.type == OP_POLES || o.type == OP_MERGED || (o.type == OP_LUMPED && o.lp.branch))
         return false;
-    hb += ((uint64_t)h->n[0] * (3 * (uint64_t)h->n[1] - 2) + (t ? 3 : 7)) / (t ? 4 : 8);
+    const uint64_t rpb = t ? 4 : 8 * EH2_GROUPS;  // H rows per block
+    hb += ((uint64_t)h->n[0] * (3 * (uint64_t)h->n[1] - 2) + rpb - 1) / rpb;
     eb += ((uint64_t)h->n[0] * 3 + 3) / 4;
     cnt++;
   }
@@ -5919,7 +5924,7 @@ static int energy_many_batched(ems_gpu *const *models, int n, const uint8_t *act
   }
   const int nz0 = h0->n[2];
   const int nt = nz0 <= 32 ? 0 : (nz0 <= 64 ? 1 : (nz0 <= 128 ? 2 : (nz0 <= 256 ? 4 : 8)));
-  const size_t rows_per_block = nt ? 4 : 8;  // H rows of one block
+  const size_t rows_per_block = nt ? 4 : 8 * EH2_GROUPS;  // H rows of one block
   uint32_t hb = 0, eb = 0, pb = 0;
   for (int i = 0; i < nm; i++) {
     ems_gpu *h = models[act[i]];
